@@ -1,0 +1,31 @@
+// amg.hpp — host-side AMG hierarchy description (amg_setup.cpp) consumed by
+// the device V-cycle (amg.hip).
+#pragma once
+#include "internal.hpp"
+
+namespace fem {
+
+struct HCsr {
+    uint32_t n = 0, m = 0;
+    std::vector<uint32_t> rp{0}, ci;
+    std::vector<double> v;
+};
+
+HCsr transpose(const HCsr& A);
+HCsr spgemm(const HCsr& A, const HCsr& B);
+
+struct AmgHierarchy {
+    std::vector<HCsr> A;                  // level operators, A[0] = input
+    std::vector<HCsr> P;                  // prolongators, P[k]: n_k x n_{k+1}
+    std::vector<std::vector<double>> d;   // smoother diagonal (C/F: a_ii, SA: 1/a_ii)
+    std::vector<double> coarse;           // dense coarse solution operator (row-major)
+    std::vector<double> omega;            // smoother weight per level
+    // multicolour ordering per level (SGS smoother)
+    std::vector<std::vector<uint32_t>> cptr, crows;
+    std::vector<std::vector<double>> diag;
+};
+
+AmgHierarchy build_cf_hierarchy(const CsrMat& A, const fem_pc_params& p);
+AmgHierarchy build_sa_hierarchy(const CsrMat& A, const fem_pc_params& p);
+
+}  // namespace fem

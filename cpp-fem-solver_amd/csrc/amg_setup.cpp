@@ -1,0 +1,455 @@
+// amg_setup.cpp — AMG hierarchy construction.
+//   classical C/F (make_amg_data):      ode-solver/src/math/array.cpp:3182-3343
+//   smoothed aggregation (block AMG):   ode-solver/include/math/blockMatrix.hpp:1051-1179
+// The coarse products are exact sparse products accumulated in the
+// reference's traversal order (row p of A, then row q of B). The coarsest
+// solve is stored as an explicit dense operator C (C f = coarse solve of f,
+// column k = solve of e_k), computed here in float64.
+#include <algorithm>
+#include <cmath>
+#include <map>
+#include <queue>
+
+#include "amg.hpp"
+
+namespace fem {
+
+HCsr transpose(const HCsr& A) {
+    HCsr T;
+    T.n = A.m;
+    T.m = A.n;
+    T.rp.assign(A.m + 1, 0);
+    for (uint32_t c : A.ci) T.rp[c + 1]++;
+    for (uint32_t j = 0; j < A.m; ++j) T.rp[j + 1] += T.rp[j];
+    T.ci.resize(A.ci.size());
+    T.v.resize(A.v.size());
+    std::vector<uint32_t> off(T.rp.begin(), T.rp.end() - 1);
+    for (uint32_t i = 0; i < A.n; ++i)
+        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+            const uint32_t q = off[A.ci[p]]++;
+            T.ci[q] = i;
+            T.v[q] = A.v[p];
+        }
+    return T;
+}
+
+HCsr spgemm(const HCsr& A, const HCsr& B) {
+    HCsr C;
+    C.n = A.n;
+    C.m = B.m;
+    C.rp.assign(A.n + 1, 0);
+    std::vector<int64_t> mark(B.m, -1);
+    std::vector<double> acc(B.m, 0.0);
+    std::vector<uint32_t> cols;
+    C.ci.reserve(A.ci.size() * 2);
+    C.v.reserve(A.ci.size() * 2);
+    for (uint32_t i = 0; i < A.n; ++i) {
+        cols.clear();
+        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+            const uint32_t j = A.ci[p];
+            const double a = A.v[p];
+            for (uint32_t q = B.rp[j]; q < B.rp[j + 1]; ++q) {
+                const uint32_t k = B.ci[q];
+                if (mark[k] != (int64_t)i) { mark[k] = i; cols.push_back(k); acc[k] = 0.0; }
+                acc[k] += a * B.v[q];
+            }
+        }
+        std::sort(cols.begin(), cols.end());
+        for (uint32_t k : cols) { C.ci.push_back(k); C.v.push_back(acc[k]); }
+        C.rp[i + 1] = (uint32_t)C.ci.size();
+    }
+    return C;
+}
+
+static HCsr from_mat(const CsrMat& A) {
+    A.sync_host();
+    HCsr H;
+    H.n = A.n;
+    H.m = A.m;
+    H.rp = A.h_rp;
+    H.ci = A.h_ci;
+    H.v = A.h_v;
+    return H;
+}
+
+// greedy multicolouring of pattern(A + A^T) in row order (SGS smoother)
+static void colour(const HCsr& A, std::vector<uint32_t>& cptr, std::vector<uint32_t>& rows, std::vector<double>& dg) {
+    const uint32_t n = A.n;
+    HCsr T = transpose(A);
+    std::vector<int> c(n, -1);
+    std::vector<char> used(n + 2, 0);
+    std::vector<int> touched;
+    int nc = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        touched.clear();
+        auto mark = [&](uint32_t j) {
+            if (j == i || c[j] < 0 || used[c[j]]) return;
+            used[c[j]] = 1;
+            touched.push_back(c[j]);
+        };
+        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) mark(A.ci[p]);
+        for (uint32_t p = T.rp[i]; p < T.rp[i + 1]; ++p) mark(T.ci[p]);
+        int k = 0;
+        while (used[k]) ++k;
+        c[i] = k;
+        nc = std::max(nc, k + 1);
+        for (int t : touched) used[t] = 0;
+    }
+    cptr.assign(nc + 1, 0);
+    for (uint32_t i = 0; i < n; ++i) cptr[c[i] + 1]++;
+    for (int k = 0; k < nc; ++k) cptr[k + 1] += cptr[k];
+    rows.resize(n);
+    std::vector<uint32_t> off(cptr.begin(), cptr.end() - 1);
+    for (uint32_t i = 0; i < n; ++i) rows[off[c[i]]++] = i;
+    dg.assign(n, 0.0);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
+            if (A.ci[p] == i) { dg[i] = A.v[p]; break; }
+}
+
+static void check_coarse(uint32_t nc) {
+    if (nc > 4096) fail(FEM_ERR_RUNTIME, "AMG: coarsest level has " + std::to_string(nc) +
+                                             " rows; raise max_levels or coarse_size");
+}
+
+// dense partial-pivot LU with pivot floor 1e-14 (array.cpp:2252-2333);
+// returns the explicit solution operator
+static std::vector<double> cf_coarse_operator(const HCsr& A) {
+    const uint32_t n = A.n;
+    check_coarse(n);
+    std::vector<double> M((size_t)n * n, 0.0);
+    std::vector<uint32_t> piv(n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) M[(size_t)i * n + A.ci[p]] = A.v[p];
+    for (uint32_t i = 0; i < n; ++i) piv[i] = i;
+    for (uint32_t k = 0; k < n; ++k) {
+        uint32_t pm = k;
+        double vm = std::fabs(M[(size_t)k * n + k]);
+        for (uint32_t i = k + 1; i < n; ++i)
+            if (std::fabs(M[(size_t)i * n + k]) > vm) { vm = std::fabs(M[(size_t)i * n + k]); pm = i; }
+        if (pm != k) {
+            std::swap(piv[k], piv[pm]);
+            for (uint32_t j = 0; j < n; ++j) std::swap(M[(size_t)k * n + j], M[(size_t)pm * n + j]);
+        }
+        if (std::fabs(M[(size_t)k * n + k]) < 1e-14) M[(size_t)k * n + k] = 1e-14;
+        for (uint32_t i = k + 1; i < n; ++i) {
+            M[(size_t)i * n + k] /= M[(size_t)k * n + k];
+            for (uint32_t j = k + 1; j < n; ++j) M[(size_t)i * n + j] -= M[(size_t)i * n + k] * M[(size_t)k * n + j];
+        }
+    }
+    std::vector<double> C((size_t)n * n), x(n);
+    for (uint32_t col = 0; col < n; ++col) {
+        for (uint32_t i = 0; i < n; ++i) x[i] = (piv[i] == col) ? 1.0 : 0.0;
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t j = 0; j < i; ++j) x[i] -= M[(size_t)i * n + j] * x[j];
+        for (int64_t i = (int64_t)n - 1; i >= 0; --i) {
+            for (uint32_t j = i + 1; j < n; ++j) x[i] -= M[(size_t)i * n + j] * x[j];
+            x[i] /= M[(size_t)i * n + i];
+        }
+        for (uint32_t i = 0; i < n; ++i) C[(size_t)i * n + col] = x[i];
+    }
+    return C;
+}
+
+AmgHierarchy build_cf_hierarchy(const CsrMat& A0, const fem_pc_params& prm) {
+    AmgHierarchy H;
+    H.A.push_back(from_mat(A0));
+    const double theta = prm.theta;
+    const uint32_t cth = (uint32_t)std::max(1, prm.coarse_size);
+    const int maxl = std::max(1, prm.max_levels);
+    for (int k = 0;; ++k) {
+        const HCsr& A = H.A[k];
+        const uint32_t n = A.n;
+        std::vector<double> d(n, 0.0);
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
+                if (A.ci[p] == i) { d[i] = A.v[p]; break; }
+        H.omega.push_back(prm.omega);
+        if (n <= cth || k + 1 >= maxl) break;
+        // strength (array.cpp:3202-3235)
+        std::vector<uint32_t> Sp(n + 1, 0), Sc;
+        Sc.reserve(A.ci.size());
+        for (uint32_t i = 0; i < n; ++i) {
+            double mx = 0.0;
+            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
+                if (A.ci[p] != i) mx = std::max(mx, std::fabs(A.v[p]));
+            const double thr = theta * mx;
+            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
+                if (A.ci[p] != i && std::fabs(A.v[p]) >= thr) Sc.push_back(A.ci[p]);
+            Sp[i + 1] = (uint32_t)Sc.size();
+        }
+        // C/F splitting with a max-heap on (weight, index) (array.cpp:3237-3270)
+        std::vector<char> st(n, 0);
+        std::vector<uint32_t> w(n);
+        for (uint32_t i = 0; i < n; ++i) w[i] = Sp[i + 1] - Sp[i];
+        std::priority_queue<std::pair<uint32_t, uint32_t>> pq;
+        for (uint32_t i = 0; i < n; ++i) pq.push({w[i], i});
+        while (!pq.empty()) {
+            const uint32_t i = pq.top().second;
+            pq.pop();
+            if (st[i] != 0) continue;
+            st[i] = 1;
+            for (uint32_t p = Sp[i]; p < Sp[i + 1]; ++p) {
+                const uint32_t j = Sc[p];
+                if (st[j] != 0) continue;
+                st[j] = 2;
+                for (uint32_t q = Sp[j]; q < Sp[j + 1]; ++q) {
+                    const uint32_t kk = Sc[q];
+                    if (st[kk] == 0) { w[kk]++; pq.push({w[kk], kk}); }
+                }
+            }
+        }
+        for (uint32_t i = 0; i < n; ++i)
+            if (st[i] == 0) st[i] = 1;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (st[i] != 2) continue;
+            bool hasC = false;
+            for (uint32_t p = Sp[i]; p < Sp[i + 1]; ++p)
+                if (st[Sc[p]] == 1) { hasC = true; break; }
+            if (!hasC) st[i] = 1;
+        }
+        std::vector<uint32_t> mapC(n, 0);
+        uint32_t nc = 0;
+        for (uint32_t i = 0; i < n; ++i) { mapC[i] = nc; if (st[i] == 1) nc++; }
+        if (nc == 0 || nc == n) break;
+        // direct interpolation (array.cpp:3276-3319), without padding slots
+        HCsr P;
+        P.n = n;
+        P.m = nc;
+        P.rp.assign(n + 1, 0);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (st[i] == 1) { P.ci.push_back(mapC[i]); P.v.push_back(1.0); }
+            else {
+                double den = d[i];
+                for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+                    const uint32_t j = A.ci[p];
+                    if (j == i) continue;
+                    bool strong = false;
+                    for (uint32_t s = Sp[i]; s < Sp[i + 1]; ++s)
+                        if (Sc[s] == j) { strong = true; break; }
+                    if (!strong) den += A.v[p];
+                }
+                if (std::fabs(den) < 1e-300) den = d[i];
+                for (uint32_t s = Sp[i]; s < Sp[i + 1]; ++s) {
+                    const uint32_t j = Sc[s];
+                    if (st[j] != 1) continue;
+                    double aij = 0.0;
+                    for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
+                        if (A.ci[p] == j) { aij = A.v[p]; break; }
+                    P.ci.push_back(mapC[j]);
+                    P.v.push_back(-aij / den);
+                }
+            }
+            P.rp[i + 1] = (uint32_t)P.ci.size();
+        }
+        HCsr AP = spgemm(A, P);
+        HCsr R = transpose(P);
+        HCsr Ac = spgemm(R, AP);
+        H.d.push_back(std::move(d));
+        H.P.push_back(std::move(P));
+        H.A.push_back(std::move(Ac));
+    }
+    H.coarse = cf_coarse_operator(H.A.back());
+    if (prm.smoother == FEM_SMOOTH_SGS) {
+        for (size_t k = 0; k + 1 < H.A.size(); ++k) {
+            H.cptr.emplace_back();
+            H.crows.emplace_back();
+            H.diag.emplace_back();
+            colour(H.A[k], H.cptr.back(), H.crows.back(), H.diag.back());
+        }
+    }
+    return H;
+}
+
+// 1x1 condensation (blockMatrix.hpp:269-315 with BS = 1): columns sorted and
+// unique per row, duplicates summed, explicit zeros kept.
+static HCsr condense1(const HCsr& A) {
+    HCsr B;
+    B.n = A.n;
+    B.m = A.m;
+    B.rp.assign(A.n + 1, 0);
+    std::vector<std::pair<uint32_t, double>> row;
+    for (uint32_t i = 0; i < A.n; ++i) {
+        row.clear();
+        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) row.emplace_back(A.ci[p], A.v[p]);
+        std::stable_sort(row.begin(), row.end(), [](auto& a, auto& b) { return a.first < b.first; });
+        for (size_t k = 0; k < row.size();) {
+            const uint32_t j = row[k].first;
+            double s = 0.0 + row[k].second;
+            size_t e = k + 1;
+            while (e < row.size() && row[e].first == j) s += row[e++].second;
+            B.ci.push_back(j);
+            B.v.push_back(s);
+            k = e;
+        }
+        B.rp[i + 1] = (uint32_t)B.ci.size();
+    }
+    return B;
+}
+
+// coarsest solve of the block AMG (blockMatrix.hpp:958-999) as a dense
+// operator. exact = 0 reproduces the reference's sequence (the right-hand
+// side runs through the already-eliminated sub-diagonal entries without the
+// row swaps); exact = 1 is the pivoted LU solve. Pivots below 1e-14 in
+// magnitude are replaced by +-1e-14 in both.
+static std::vector<double> sa_coarse_operator(const HCsr& A, bool exact) {
+    const int sz = (int)A.n;
+    check_coarse(A.n);
+    std::vector<double> d((size_t)sz * sz, 0.0);
+    for (int i = 0; i < sz; ++i)
+        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) d[(size_t)i * sz + A.ci[p]] = A.v[p];
+    auto fl = [](double x) { return std::fabs(x) < 1e-14 ? (x >= 0 ? 1e-14 : -1e-14) : x; };
+    std::vector<int> pv(sz);
+    for (int i = 0; i < sz; ++i) pv[i] = i;
+    for (int k = 0; k < sz; ++k) {
+        int pk = k;
+        for (int i = k + 1; i < sz; ++i)
+            if (std::fabs(d[(size_t)i * sz + k]) > std::fabs(d[(size_t)pk * sz + k])) pk = i;
+        if (pk != k) {
+            std::swap(pv[k], pv[pk]);
+            for (int j = 0; j < sz; ++j) std::swap(d[(size_t)k * sz + j], d[(size_t)pk * sz + j]);
+        }
+        if (exact) {
+            d[(size_t)k * sz + k] = fl(d[(size_t)k * sz + k]);
+            for (int i = k + 1; i < sz; ++i) {
+                d[(size_t)i * sz + k] /= d[(size_t)k * sz + k];
+                for (int j = k + 1; j < sz; ++j) d[(size_t)i * sz + j] -= d[(size_t)i * sz + k] * d[(size_t)k * sz + j];
+            }
+        } else {
+            const double dk = fl(d[(size_t)k * sz + k]);
+            for (int i = k + 1; i < sz; ++i) {
+                const double f = d[(size_t)i * sz + k] / dk;
+                for (int j = k; j < sz; ++j) d[(size_t)i * sz + j] -= f * d[(size_t)k * sz + j];
+            }
+        }
+    }
+    std::vector<double> C((size_t)sz * sz), x(sz);
+    for (int col = 0; col < sz; ++col) {
+        if (exact) {
+            for (int i = 0; i < sz; ++i) x[i] = (pv[i] == col) ? 1.0 : 0.0;
+            for (int i = 0; i < sz; ++i)
+                for (int j = 0; j < i; ++j) x[i] -= d[(size_t)i * sz + j] * x[j];
+            for (int i = sz - 1; i >= 0; --i) {
+                for (int j = i + 1; j < sz; ++j) x[i] -= d[(size_t)i * sz + j] * x[j];
+                x[i] /= d[(size_t)i * sz + i];
+            }
+        } else {
+            for (int i = 0; i < sz; ++i) x[i] = (i == col) ? 1.0 : 0.0;
+            for (int k = 0; k < sz; ++k) {
+                const double dk = fl(d[(size_t)k * sz + k]);
+                for (int i = k + 1; i < sz; ++i) x[i] -= (d[(size_t)i * sz + k] / dk) * x[k];
+            }
+            for (int i = sz - 1; i >= 0; --i) {
+                for (int j = i + 1; j < sz; ++j) x[i] -= d[(size_t)i * sz + j] * x[j];
+                x[i] /= fl(d[(size_t)i * sz + i]);
+            }
+        }
+        for (int i = 0; i < sz; ++i) C[(size_t)i * sz + col] = x[i];
+    }
+    return C;
+}
+
+AmgHierarchy build_sa_hierarchy(const CsrMat& A0, const fem_pc_params& prm) {
+    AmgHierarchy H;
+    HCsr cur = condense1(from_mat(A0));
+    const double theta = prm.theta;
+    const int maxl = std::max(1, prm.max_levels);
+    const int max_agg = std::max(1, prm.max_agg);
+    const uint32_t cmax = (uint32_t)std::max(1, prm.coarse_size);
+    // p_omega <= 0 / omega <= 0 select the robust variant (no counterpart in
+    // the reference): per level w = (4/3) / rho with rho = max_i sum_j
+    // |a_ij / a_ii| (Gershgorin bound of D^{-1} A); coarsening also stops when
+    // a level keeps more than 85% of its rows.
+    const bool auto_p = prm.p_omega <= 0.0, auto_s = prm.omega <= 0.0;
+    for (int l = 0; l < maxl; ++l) {
+        const uint32_t n = cur.n;
+        std::vector<double> dinv(n, 0.0);
+        for (uint32_t i = 0; i < n; ++i) {
+            uint32_t dp = UINT32_MAX;
+            for (uint32_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p)
+                if (cur.ci[p] == i) { dp = p; break; }
+            if (dp == UINT32_MAX) fail(FEM_ERR_RUNTIME, "Block AMG: block row " + std::to_string(i) + " has no diagonal block");
+            // 1x1 Gauss-Jordan inverse (blockMatrix.hpp:70-100); a failed
+            // inversion leaves 0, as the reference ignores the failure
+            const double a = cur.v[dp];
+            if (std::fabs(a) >= 1e-300) dinv[i] = 1.0 / a;
+        }
+        double rho = 0.0;
+        if (auto_p || auto_s) {
+            for (uint32_t i = 0; i < n; ++i) {
+                double rs = 0.0;
+                for (uint32_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p) rs += std::fabs(0.0 + dinv[i] * cur.v[p]);
+                rho = std::max(rho, rs);
+            }
+        }
+        const double wauto = rho > 0.0 ? (4.0 / 3.0) / rho : 2.0 / 3.0;
+        H.omega.push_back(auto_s ? wauto : prm.omega);
+        const double pw = auto_p ? wauto : prm.p_omega;
+        H.A.push_back(cur);
+        if (n <= cmax) break;
+        // strength by (1x1) Frobenius norm (blockMatrix.hpp:1084-1099)
+        std::vector<uint32_t> Sp(n + 1, 0), Sc;
+        Sc.reserve(cur.ci.size());
+        for (uint32_t i = 0; i < n; ++i) {
+            double mx = 0.0;
+            for (uint32_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p)
+                if (cur.ci[p] != i) mx = std::max(mx, std::sqrt(cur.v[p] * cur.v[p]));
+            const double thr = theta * mx;
+            for (uint32_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p)
+                if (cur.ci[p] != i && std::sqrt(cur.v[p] * cur.v[p]) >= thr) Sc.push_back(cur.ci[p]);
+            Sp[i + 1] = (uint32_t)Sc.size();
+        }
+        // greedy aggregation (blockMatrix.hpp:1101-1111)
+        std::vector<int> agg(n, -1);
+        int na = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (agg[i] != -1) continue;
+            agg[i] = na;
+            int sz = 1;
+            for (uint32_t p = Sp[i]; p < Sp[i + 1]; ++p) {
+                const uint32_t j = Sc[p];
+                if (agg[j] == -1 && sz < max_agg) { agg[j] = na; ++sz; }
+            }
+            ++na;
+        }
+        // smoothed prolongator P = (I - w D^{-1} A) P_t (blockMatrix.hpp:1125-1164)
+        HCsr P;
+        P.n = n;
+        P.m = (uint32_t)na;
+        P.rp.assign(n + 1, 0);
+        std::vector<std::pair<uint32_t, double>> row;
+        for (uint32_t i = 0; i < n; ++i) {
+            row.clear();
+            row.emplace_back((uint32_t)agg[i], 1.0);
+            for (uint32_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p) {
+                const uint32_t g = (uint32_t)agg[cur.ci[p]];
+                const double sv = 0.0 + dinv[i] * cur.v[p];
+                auto it = std::find_if(row.begin(), row.end(), [g](auto& e) { return e.first == g; });
+                if (it == row.end()) { row.emplace_back(g, 0.0); it = row.end() - 1; }
+                it->second -= pw * sv;
+            }
+            std::sort(row.begin(), row.end(), [](auto& a, auto& b) { return a.first < b.first; });
+            for (auto& e : row) { P.ci.push_back(e.first); P.v.push_back(e.second); }
+            P.rp[i + 1] = (uint32_t)P.ci.size();
+        }
+        if (auto_p && (double)na > 0.85 * (double)n) break;  // coarsening stalled
+        HCsr R = transpose(P);
+        HCsr AP = spgemm(cur, P);
+        HCsr Ac = spgemm(R, AP);
+        H.d.push_back(std::move(dinv));
+        H.P.push_back(std::move(P));
+        cur = std::move(Ac);
+    }
+    H.coarse = sa_coarse_operator(H.A.back(), prm.coarse_exact != 0);
+    if (prm.smoother == FEM_SMOOTH_SGS) {
+        for (size_t k = 0; k + 1 < H.A.size(); ++k) {
+            H.cptr.emplace_back();
+            H.crows.emplace_back();
+            H.diag.emplace_back();
+            colour(H.A[k], H.cptr.back(), H.crows.back(), H.diag.back());
+        }
+    }
+    return H;
+}
+
+}  // namespace fem

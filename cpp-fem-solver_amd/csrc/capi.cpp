@@ -1,0 +1,456 @@
+// capi.cpp — the extern "C" boundary (include/femgpu.h).
+//
+// Every entry point converts fem::Error (and std::bad_alloc) into a status
+// code plus a thread-local message. Host-pointer calls stage through device
+// buffers owned by the handle and synchronise before returning, matching the
+// synchronous std::vector calls of the reference.
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+
+#include "handles.hpp"
+
+using namespace fem;
+
+namespace fem {
+hipStream_t new_stream() {
+    ensure_device();
+    hipStream_t s = nullptr;
+    FEM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    return s;
+}
+}  // namespace fem
+
+static thread_local std::string g_err;
+
+void fem_set_error(const std::string& m) { g_err = m; }
+
+template <class F>
+static int guard(F&& f) {
+    try {
+        f();
+        return FEM_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        g_err = "out of memory";
+        return FEM_ERR_DEVICE;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return FEM_ERR_RUNTIME;
+    }
+}
+
+static void check_csr(uint32_t rows, uint32_t cols, const uint32_t* rp, const uint32_t* ci) {
+    if (!rp) fail(FEM_ERR_INVALID_ARGUMENT, "row_ptr is null");
+    if (rp[0] != 0) fail(FEM_ERR_INVALID_ARGUMENT, "row_ptr[0] must be 0");
+    for (uint32_t i = 0; i < rows; ++i)
+        if (rp[i + 1] < rp[i]) fail(FEM_ERR_INVALID_ARGUMENT, "row_ptr must be non-decreasing");
+    const uint32_t nnz = rp[rows];
+    if (nnz && !ci) fail(FEM_ERR_INVALID_ARGUMENT, "col_idx is null");
+    for (uint32_t k = 0; k < nnz; ++k)
+        if (ci[k] >= cols) fail(FEM_ERR_OUT_OF_RANGE, "column index out of range");
+}
+
+// COO -> CSR with the reference constructor's semantics (array.cpp:2072-2182)
+static void coo_to_csr(uint32_t n, uint32_t m, uint64_t nin, const double* v, const uint32_t* r, const uint32_t* c,
+                       std::vector<uint32_t>& rp, std::vector<uint32_t>& ci, std::vector<double>& vv) {
+    std::vector<uint64_t> keep;
+    keep.reserve(nin);
+    for (uint64_t k = 0; k < nin; ++k) {
+        if (r[k] >= n || c[k] >= m) fail(FEM_ERR_OUT_OF_RANGE, "COO input: row or column index out of range");
+        if (v[k] != 0.0) keep.push_back(k);
+    }
+    // counting sort by row, stable; then sort each row by column (stable
+    // keeps input order among duplicates, whose sum order then matches a
+    // stable sort by (row, col))
+    std::vector<uint64_t> cnt(n + 1, 0);
+    for (uint64_t k : keep) cnt[r[k] + 1]++;
+    for (uint32_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<uint64_t> ord(keep.size());
+    {
+        std::vector<uint64_t> off(cnt.begin(), cnt.end() - 1);
+        for (uint64_t k : keep) ord[off[r[k]]++] = k;
+    }
+    rp.assign(n + 1, 0);
+    ci.clear();
+    vv.clear();
+    ci.reserve(keep.size());
+    vv.reserve(keep.size());
+    for (uint32_t i = 0; i < n; ++i) {
+        auto b = ord.begin() + cnt[i], e = ord.begin() + cnt[i + 1];
+        std::stable_sort(b, e, [&](uint64_t a, uint64_t bb) { return c[a] < c[bb]; });
+        for (auto it = b; it != e;) {
+            const uint32_t col = c[*it];
+            double s = v[*it];
+            ++it;
+            while (it != e && c[*it] == col) { s += v[*it]; ++it; }
+            if (s != 0.0) { ci.push_back(col); vv.push_back(s); }
+        }
+        rp[i + 1] = (uint32_t)ci.size();
+    }
+}
+
+void fem_solver_make(LinOp* op, hipStream_t stream, fem_pc M, const fem_solve_opts* opts, fem_solver_s* h) {
+    fem_solve_opts o;
+    if (opts) o = *opts;
+    else fem_solve_opts_default(FEM_SOLVER_PCG, &o);
+    h->op = op;
+    h->stream = stream;
+    h->S.reset(new KrylovSolver(op, M ? M->pc.get() : nullptr, o, stream));
+}
+
+void fem_solver_run(fem_solver_s* h, const double* b, double* x, fem_solve_stats* st) {
+    const uint32_t n = h->op->rows();
+    if (h->S->opts.device_ptrs) {
+        h->S->solve(b, x, st);
+        return;
+    }
+    if (h->b.n < n) { h->b.alloc(n); h->x.alloc(n); }
+    h->b.upload(b, n, h->stream);
+    h->S->solve(h->b.get(), h->x.get(), st);
+    h->x.download(x, n, h->stream);
+    FEM_HIP(hipStreamSynchronize(h->stream));
+}
+
+extern "C" {
+
+const char* fem_last_error(void) { return g_err.c_str(); }
+
+int fem_device_info(char* name, int name_len, int* cu_count) {
+    return guard([&] {
+        ensure_device();
+        int dev = 0;
+        FEM_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t p;
+        FEM_HIP(hipGetDeviceProperties(&p, dev));
+        if (name && name_len > 0) {
+            std::snprintf(name, (size_t)name_len, "%s (%s)", p.name, p.gcnArchName);
+        }
+        if (cu_count) *cu_count = p.multiProcessorCount;
+    });
+}
+
+void fem_pc_params_default(int kind, fem_pc_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->omega = 1.2;
+    p->theta = 0.25;
+    p->npre = 1;
+    p->npost = 1;
+    p->max_levels = 25;
+    p->coarse_size = 300;
+    p->max_agg = 4;
+    p->smoother = FEM_SMOOTH_JACOBI;
+    p->p_omega = 2.0 / 3.0;
+    p->coarse_exact = 0;
+    p->fp32 = 0;
+    if (kind == FEM_PC_AMG) p->omega = 2.0 / 3.0;
+    if (kind == FEM_PC_SAAMG || kind == FEM_PC_BSAAMG) {
+        p->omega = 2.0 / 3.0;
+        p->max_levels = 20;
+        p->coarse_size = 50;
+    }
+}
+
+void fem_solve_opts_default(int solver, fem_solve_opts* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->solver = solver;
+    o->tol = 1e-10;
+    o->max_iter = 2000;
+    o->restart = 30;
+    o->precision = FEM_PREC_FP64;
+    o->inner_tol = 1e-5;
+    o->max_refine = 30;
+    o->check_every = 0;
+    o->device_ptrs = 0;
+    o->use_graph = 1;
+}
+
+int fem_csr_create(uint32_t rows, uint32_t cols, const uint32_t* row_ptr, const uint32_t* col_idx,
+                   const double* values, fem_csr* out) {
+    return guard([&] {
+        if (!out) fail(FEM_ERR_INVALID_ARGUMENT, "out is null");
+        check_csr(rows, cols, row_ptr, col_idx);
+        auto h = std::make_unique<fem_csr_s>();
+        h->stream = new_stream();
+        const uint32_t nnz = row_ptr[rows];
+        h->A.set_host(rows, cols, std::vector<uint32_t>(row_ptr, row_ptr + rows + 1),
+                      std::vector<uint32_t>(col_idx, col_idx + nnz), std::vector<double>(values, values + nnz),
+                      h->stream);
+        h->red.init(h->stream);
+        *out = h.release();
+    });
+}
+
+int fem_csr_create_coo(uint32_t rows, uint32_t cols, uint64_t nnz, const double* values, const uint32_t* rowlist,
+                       const uint32_t* collist, fem_csr* out) {
+    return guard([&] {
+        if (!out) fail(FEM_ERR_INVALID_ARGUMENT, "out is null");
+        std::vector<uint32_t> rp, ci;
+        std::vector<double> v;
+        coo_to_csr(rows, cols, nnz, values, rowlist, collist, rp, ci, v);
+        auto h = std::make_unique<fem_csr_s>();
+        h->stream = new_stream();
+        h->A.set_host(rows, cols, std::move(rp), std::move(ci), std::move(v), h->stream);
+        h->red.init(h->stream);
+        *out = h.release();
+    });
+}
+
+int fem_csr_info(fem_csr A, uint32_t* rows, uint32_t* cols, uint64_t* nnz) {
+    return guard([&] {
+        if (!A) fail(FEM_ERR_INVALID_ARGUMENT, "null matrix");
+        if (rows) *rows = A->A.n;
+        if (cols) *cols = A->A.m;
+        if (nnz) *nnz = A->A.nnz;
+    });
+}
+
+int fem_csr_export(fem_csr A, uint32_t* row_ptr, uint32_t* col_idx, double* values) {
+    return guard([&] {
+        A->A.sync_host();
+        if (row_ptr) std::copy(A->A.h_rp.begin(), A->A.h_rp.end(), row_ptr);
+        if (col_idx) std::copy(A->A.h_ci.begin(), A->A.h_ci.end(), col_idx);
+        if (values) std::copy(A->A.h_v.begin(), A->A.h_v.end(), values);
+    });
+}
+
+int fem_csr_transpose(fem_csr A, fem_csr* out) {
+    return guard([&] {
+        std::vector<uint32_t> rp, ci;
+        std::vector<double> v;
+        csr_transpose_host(A->A, rp, ci, v);
+        auto h = std::make_unique<fem_csr_s>();
+        h->stream = new_stream();
+        h->A.set_host(A->A.m, A->A.n, std::move(rp), std::move(ci), std::move(v), h->stream);
+        h->red.init(h->stream);
+        *out = h.release();
+    });
+}
+
+int fem_csr_multiply(fem_csr A, fem_csr B, fem_csr* out) {
+    return guard([&] {
+        if (A->A.m != B->A.n) fail(FEM_ERR_INVALID_ARGUMENT, "matrix product: inner dimensions differ");
+        std::vector<uint32_t> rp, ci;
+        std::vector<double> v;
+        csr_spgemm_host(A->A, B->A, rp, ci, v);
+        auto h = std::make_unique<fem_csr_s>();
+        h->stream = new_stream();
+        h->A.set_host(A->A.n, B->A.m, std::move(rp), std::move(ci), std::move(v), h->stream);
+        h->red.init(h->stream);
+        *out = h.release();
+    });
+}
+
+int fem_csr_destroy(fem_csr A) {
+    return guard([&] { delete A; });
+}
+
+int fem_spmv(fem_csr A, const double* x, double* y) {
+    return guard([&] {
+        auto& M = A->A;
+        if (A->xs.n < M.m) A->xs.alloc(M.m);
+        if (A->ys.n < M.n) A->ys.alloc(M.n);
+        A->xs.upload(x, M.m, A->stream);
+        M.apply(A->xs.get(), A->ys.get(), A->stream);
+        A->ys.download(y, M.n, A->stream);
+        FEM_HIP(hipStreamSynchronize(A->stream));
+    });
+}
+
+int fem_spmv_device(fem_csr A, const void* x, void* y, int fp32) {
+    return guard([&] {
+        if (fp32) {
+            A->A.ensure_fp32(A->stream);
+            A->A.apply32((const float*)x, (float*)y, A->stream);
+        } else {
+            A->A.apply((const double*)x, (double*)y, A->stream);
+        }
+        FEM_HIP(hipStreamSynchronize(A->stream));
+    });
+}
+
+int fem_dot(uint64_t n, const double* a, const double* b, double* out) {
+    return guard([&] {
+        static std::mutex mu;
+        std::lock_guard<std::mutex> lk(mu);
+        static hipStream_t s = new_stream();
+        static RedScratch red;
+        red.init(s);
+        DevBuf<double> da(std::max<uint64_t>(n, 1)), db(std::max<uint64_t>(n, 1));
+        da.upload(a, n, s);
+        db.upload(b, n, s);
+        *out = n ? dot_sync(da.get(), db.get(), n, Reducer{red.partials.get(), red.counter.get()}, s) : 0.0;
+    });
+}
+
+int fem_precond_create(fem_csr A, int kind, const fem_pc_params* params, fem_pc* out) {
+    return guard([&] {
+        if (!A || !out) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
+        fem_pc_params p;
+        if (params) p = *params;
+        else fem_pc_params_default(kind, &p);
+        auto h = std::make_unique<fem_pc_s>();
+        h->stream = A->stream;
+        h->pc = make_precond(A->A, kind, p, A->stream);
+        *out = h.release();
+    });
+}
+
+int fem_precond_apply(fem_pc M, const double* r, double* z) {
+    return guard([&] {
+        const uint32_t n = M->pc->size();
+        if (M->r.n < n) { M->r.alloc(n); M->z.alloc(n); }
+        M->r.upload(r, n, M->stream);
+        M->pc->apply(M->r.get(), M->z.get(), M->stream);
+        M->z.download(z, n, M->stream);
+        FEM_HIP(hipStreamSynchronize(M->stream));
+    });
+}
+
+int fem_precond_destroy(fem_pc M) {
+    return guard([&] { delete M; });
+}
+
+int fem_precond_info(fem_pc M, int* levels, uint32_t* rows, uint64_t* nnz, int cap, double* setup_ms) {
+    return guard([&] {
+        if (levels) *levels = M->pc->levels;
+        for (int i = 0; i < cap && i < (int)M->pc->lvl_rows.size(); ++i) {
+            if (rows) rows[i] = M->pc->lvl_rows[i];
+            if (nnz) nnz[i] = M->pc->lvl_nnz[i];
+        }
+        if (setup_ms) *setup_ms = M->pc->setup_ms;
+    });
+}
+
+int fem_solve(fem_csr A, fem_pc M, const double* b, double* x, const fem_solve_opts* opts, fem_solve_stats* stats) {
+    return guard([&] {
+        fem_solver_s h;
+        fem_solver_make(&A->A, A->stream, M, opts, &h);
+        fem_solver_run(&h, b, x, stats);
+    });
+}
+
+static int simple_solve(fem_csr A, fem_pc M, int solver, const double* b, int restart, double tol, int max_iter,
+                        double* x, int* iterations) {
+    fem_solve_opts o;
+    fem_solve_opts_default(solver, &o);
+    o.tol = tol;
+    o.max_iter = max_iter;
+    o.restart = restart;
+    fem_solve_stats st{};
+    const int rc = fem_solve(A, M, b, x, &o, &st);
+    if (rc == FEM_OK && iterations) *iterations = st.iterations;
+    return rc;
+}
+
+int fem_pcg(fem_csr A, fem_pc M, const double* b, double tol, int max_iter, double* x, int* iterations) {
+    return simple_solve(A, M, FEM_SOLVER_PCG, b, 0, tol, max_iter, x, iterations);
+}
+int fem_bicgstab(fem_csr A, fem_pc M, const double* b, double tol, int max_iter, double* x, int* iterations) {
+    return simple_solve(A, M, FEM_SOLVER_BICGSTAB, b, 0, tol, max_iter, x, iterations);
+}
+int fem_gmres(fem_csr A, fem_pc M, const double* b, int restart, double tol, int max_iter, double* x,
+              int* iterations) {
+    return simple_solve(A, M, FEM_SOLVER_GMRES, b, restart, tol, max_iter, x, iterations);
+}
+
+// Host-callback preconditioner: every apply copies r to the host, calls the
+// user function and copies z back (the reference's std::function overload
+// runs serially as well, array.cpp:4402-4533).
+struct HostCbPc : Precond {
+    fem_host_precond_fn fn;
+    void* user;
+    uint32_t n;
+    std::vector<double> hr, hz;
+    HostCbPc(fem_host_precond_fn f, void* u, uint32_t n_) : fn(f), user(u), n(n_), hr(n_), hz(n_) {}
+    int kind() const override { return FEM_PC_NONE + 100; }
+    uint32_t size() const override { return n; }
+    bool graph_safe() const override { return false; }
+    void apply(const double* r, double* z, hipStream_t s, Gate g) override {
+        // the solver calls this only between device-side checks; read the
+        // gate state first so a finished solve skips the callback
+        int done = 0, need = 1;
+        if (g.done) FEM_HIP(hipMemcpyAsync(&done, g.done, sizeof(int), hipMemcpyDeviceToHost, s));
+        if (g.need) FEM_HIP(hipMemcpyAsync(&need, g.need, sizeof(int), hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipMemcpyAsync(hr.data(), r, (size_t)n * 8, hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipStreamSynchronize(s));
+        if (done || !need) return;
+        fn(hr.data(), hz.data(), n, user);
+        FEM_HIP(hipMemcpyAsync(z, hz.data(), (size_t)n * 8, hipMemcpyHostToDevice, s));
+        FEM_HIP(hipStreamSynchronize(s));
+    }
+    void apply32(const float*, float*, hipStream_t, Gate) override {
+        fail(FEM_ERR_INVALID_ARGUMENT, "a host-callback preconditioner has no float32 form");
+    }
+};
+
+int fem_gmres_callback(fem_csr A, fem_host_precond_fn fn, void* user, const double* b, int restart, double tol,
+                       int max_iter, double* x, int* iterations) {
+    return guard([&] {
+        if (!fn) fail(FEM_ERR_INVALID_ARGUMENT, "null preconditioner callback");
+        fem_pc_s pc;
+        pc.pc.reset(new HostCbPc(fn, user, A->A.n));
+        pc.stream = A->stream;
+        fem_solve_opts o;
+        fem_solve_opts_default(FEM_SOLVER_GMRES, &o);
+        o.tol = tol;
+        o.max_iter = max_iter;
+        o.restart = restart;
+        o.use_graph = 0;
+        fem_solver_s h;
+        fem_solver_make(&A->A, A->stream, &pc, &o, &h);
+        fem_solve_stats st{};
+        fem_solver_run(&h, b, x, &st);
+        if (iterations) *iterations = st.iterations;
+    });
+}
+
+int fem_solver_create(fem_csr A, fem_pc M, const fem_solve_opts* opts, fem_solver* out) {
+    return guard([&] {
+        auto h = std::make_unique<fem_solver_s>();
+        h->A = A;
+        fem_solver_make(&A->A, A->stream, M, opts, h.get());
+        *out = h.release();
+    });
+}
+
+int fem_solver_solve(fem_solver S, const double* b, double* x, fem_solve_stats* stats) {
+    return guard([&] { fem_solver_run(S, b, x, stats); });
+}
+
+int fem_solver_destroy(fem_solver S) {
+    return guard([&] { delete S; });
+}
+
+int fem_bench_spmv(fem_csr A, int fp32, int reps, double* avg_ms) {
+    return guard([&] {
+        auto& M = A->A;
+        hipStream_t s = A->stream;
+        if (fp32) M.ensure_fp32(s);
+        const size_t es = fp32 ? 4 : 8;
+        DevBuf<char> x((size_t)M.m * es), y((size_t)M.n * es);
+        FEM_HIP(hipMemsetAsync(x.get(), 0, (size_t)M.m * es, s));
+        hipEvent_t e0, e1;
+        FEM_HIP(hipEventCreate(&e0));
+        FEM_HIP(hipEventCreate(&e1));
+        auto one = [&] {
+            if (fp32) M.apply32((const float*)x.get(), (float*)y.get(), s);
+            else M.apply((const double*)x.get(), (double*)y.get(), s);
+        };
+        one();
+        FEM_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; ++i) one();
+        FEM_HIP(hipEventRecord(e1, s));
+        FEM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = reps > 0 ? ms / reps : 0.0;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
+}  // extern "C"

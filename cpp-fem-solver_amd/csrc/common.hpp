@@ -1,0 +1,163 @@
+// common.hpp — shared internals of libfemgpu (gfx950 / MI355X).
+//
+// Error model: internal code throws fem::Error carrying one of the C-ABI
+// status codes; the C-ABI layer (capi.cpp) catches it, stores the message in
+// a thread-local buffer and returns the code. The codes map 1:1 onto the
+// reference's exception types (std::invalid_argument, std::out_of_range,
+// std::runtime_error; ode-solver/src/math/array.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <functional>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "femgpu.h"
+
+namespace fem {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+[[noreturn]] inline void fail(int code, const std::string& msg) { throw Error(code, msg); }
+
+#define FEM_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            ::fem::fail(FEM_ERR_DEVICE, std::string("HIP error: ") + hipGetErrorString(e_) + \
+                                            " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+    } while (0)
+
+// Owning device buffer.
+template <typename T>
+struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    explicit DevBuf(size_t count) { alloc(count); }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void alloc(size_t count) {
+        release();
+        n = count;
+        if (count) FEM_HIP(hipMalloc(&p, count * sizeof(T)));
+    }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void upload(const T* h, size_t count, hipStream_t s) {
+        if (count > n) alloc(count);
+        if (count) FEM_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+    void download(T* h, size_t count, hipStream_t s) const {
+        if (count) FEM_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+    T* get() const { return p; }
+};
+
+// Pinned host staging buffer (hipHostMalloc; never hipHostRegister).
+template <typename T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    void ensure(size_t count) {
+        if (count <= n) return;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        FEM_HIP(hipHostMalloc(&p, count * sizeof(T), hipHostMallocDefault));
+        n = count;
+    }
+};
+
+// Process-wide device selection: the library runs on the device that is
+// current when the first handle is created.
+void ensure_device();
+
+constexpr int kBlock = 256;           // threads per block for streaming kernels
+constexpr int kMaxRedBlocks = 2048;   // cap on blocks of a reduction kernel
+
+inline unsigned red_blocks(size_t n, int per_thread = 4) {
+    size_t b = (n + (size_t)kBlock * per_thread - 1) / ((size_t)kBlock * per_thread);
+    if (b < 1) b = 1;
+    if (b > (size_t)kMaxRedBlocks) b = kMaxRedBlocks;
+    return (unsigned)b;
+}
+
+// Scratch for two-stage deterministic reductions: per-block partials plus an
+// arrival counter that the last block resets to zero after use (so the same
+// scratch can be replayed inside a hipGraph without a memset node).
+struct RedScratch {
+    DevBuf<double> partials;  // kMaxRedBlocks * kMaxRedVals
+    DevBuf<unsigned> counter; // one per concurrent use slot
+    static constexpr int kMaxRedVals = 4;
+    static constexpr int kSlots = 16;
+    void init(hipStream_t s);
+};
+
+// Device-resident Krylov state shared by the kernels of one solve. Only the
+// last block of a reduction kernel (or a single-thread kernel) writes it.
+struct KState {
+    double normb;     // ||b||
+    double tol;       // relative tolerance
+    double rz;        // PCG rho = r^T z; BiCGSTAB rho_old
+    double pap;       // p^T A p
+    double alpha, beta, omega;
+    double normr;     // ||r|| of the recursive residual
+    double rho;       // BiCGSTAB rho_{k+1}
+    double tt, ts, rv;
+    double hnorm;     // GMRES h_{j+1,j} before rotation
+    double beta_g;    // GMRES cycle residual norm
+    double true_normr;
+    int it;           // loop index (GMRES: total Arnoldi steps)
+    int it_end;       // reference iteration semantics
+    int max_iter;
+    int done;         // 0 running, 1 converged/stopped, 2 error, 3 iteration cap
+    int err;          // SolverErr code (negative: soft stop inside refinement)
+    int soft;         // 1: breakdowns end the solve instead of failing it
+    int check;        // BiCGSTAB: true-residual check requested
+    int replace;      // BiCGSTAB: residual replacement requested
+    int live;         // GMRES: inner Arnoldi loop continues
+    int conv;         // GMRES: converged inside the current cycle
+    int j_used;       // GMRES: Arnoldi steps in the current cycle
+    int need_true;    // GMRES: cycle starts from the true residual
+    int iters;        // iterations run (operator applications in the loop)
+    int minicg;       // 1: mini_cg semantics (no breakdown tests, raw loop index)
+};
+
+enum SolverErr : int {
+    SE_NONE = 0,
+    SE_PCG_RZ = 1,        // r^T M^{-1} r <= 0
+    SE_PCG_PAP_SMALL = 2, // |p^T A p| < 1e-30
+    SE_PCG_PAP_NEG = 3,   // p^T A p <= 0
+    SE_BI_RHO = 4,
+    SE_BI_RHO_OLD = 5,
+    SE_BI_OMEGA = 6,
+    SE_BI_RV = 7,
+    SE_BI_TT = 8,
+    SE_GM_DEN = 9,
+    SE_GM_TRI = 10,
+};
+
+const char* solver_err_msg(int code);
+
+}  // namespace fem

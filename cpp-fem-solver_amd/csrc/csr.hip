@@ -1,0 +1,164 @@
+// csr.hip — device CSR matrix and SpMV (sparse::SMatrix::spmv,
+// ode-solver/src/math/array.cpp:3418-3442).
+//
+// SpMV kernel ("CSR-stream"): the host cuts the rows into consecutive ranges
+// whose nonzeros fit a block's LDS staging area (at most kStageNnz nonzeros
+// and kStageRows rows). A 256-thread block streams its contiguous slice of
+// values/col_idx with fully coalesced loads, forms each product v*x[col] and
+// stages it in LDS; then one thread per row adds its row's products in
+// stored column order starting from 0.0. The products are rounded before the
+// sum exactly as the reference's `sum += v * x` (no fused multiply-add: the
+// library is built with -ffp-contract=off), so FP64 results are bit-identical
+// to the reference's sequential row sum. Rows longer than the staging area
+// get a block each and are reduced with a strided sum + block tree.
+#include <algorithm>
+
+#include "internal.hpp"
+#include "reduce.hpp"
+
+namespace fem {
+
+constexpr int kStageNnz = 2048;
+constexpr int kStageRows = 256;
+
+void SpmvPlan::build(const uint32_t* h_rp, uint32_t n, hipStream_t s) {
+    std::vector<uint32_t> b;
+    b.reserve(n / 128 + 4);
+    b.push_back(0);
+    uint32_t start = 0;
+    while (start < n) {
+        if (h_rp[start + 1] - h_rp[start] > (uint32_t)kStageNnz) {
+            ++start;
+            b.push_back(start);
+            continue;
+        }
+        uint32_t end = start;
+        while (end < n && end - start < (uint32_t)kStageRows && h_rp[end + 1] - h_rp[start] <= (uint32_t)kStageNnz) ++end;
+        b.push_back(end);
+        start = end;
+    }
+    nblk = (uint32_t)b.size() - 1;
+    blk_row.upload(b, s);
+}
+
+template <typename T, int EPI>
+__device__ __forceinline__ T spmv_epilogue(uint32_t row, T s, const SpmvArgs<T>& a) {
+    if constexpr (EPI == EPI_STORE) return s;
+    else if constexpr (EPI == EPI_RESID) return a.f[row] - s;
+    else if constexpr (EPI == EPI_ADD) return a.u[row] + s;
+    else if constexpr (EPI == EPI_JACOBI) return a.u[row] + ((T)a.w * (a.f[row] - s)) / a.d[row];
+    else return a.u[row] + (T)a.w * (a.d[row] * (a.f[row] - s));
+}
+
+template <typename T, int EPI>
+__global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                                 const T* __restrict__ v, const T* __restrict__ x, T* __restrict__ y,
+                                                 const uint32_t* __restrict__ blk_row, SpmvArgs<T> a, Gate g) {
+    if (g.closed()) return;
+    __shared__ T prod[kStageNnz];
+    const uint32_t r0 = blk_row[blockIdx.x], r1 = blk_row[blockIdx.x + 1];
+    const uint32_t k0 = rp[r0], k1 = rp[r1];
+    const uint32_t cnt = k1 - k0;
+    const uint32_t t = threadIdx.x;
+    if (cnt <= (uint32_t)kStageNnz) {
+#pragma unroll
+        for (int u = 0; u < kStageNnz / kBlock; ++u) {
+            const uint32_t k = (uint32_t)u * kBlock + t;
+            if (k < cnt) prod[k] = v[k0 + k] * x[ci[k0 + k]];
+        }
+        __syncthreads();
+        for (uint32_t row = r0 + t; row < r1; row += kBlock) {
+            const uint32_t b = rp[row] - k0, e = rp[row + 1] - k0;
+            T s = (T)0;
+            for (uint32_t k = b; k < e; ++k) s += prod[k];
+            y[row] = spmv_epilogue<T, EPI>(row, s, a);
+        }
+    } else {
+        double s = 0.0;
+        for (uint32_t k = k0 + t; k < k1; k += kBlock) s += (double)(v[k] * x[ci[k]]);
+        double tot[1] = {s};
+        block_sum<1>(tot);
+        if (t == 0) y[r0] = spmv_epilogue<T, EPI>(r0, (T)tot[0], a);
+    }
+}
+
+template <typename T>
+void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, hipStream_t s, Gate g) {
+    if (A.n == 0) return;
+    const T* vals;
+    if constexpr (std::is_same<T, double>::value) vals = A.v.get();
+    else vals = A.v32.get();
+    if (!vals) fail(FEM_ERR_RUNTIME, "float32 copy of the matrix was not built");
+    dim3 grid(A.plan.nblk), block(kBlock);
+    switch (epi) {
+        case EPI_STORE:
+            hipLaunchKernelGGL((k_spmv<T, EPI_STORE>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
+                               A.plan.blk_row.get(), a, g);
+            break;
+        case EPI_RESID:
+            hipLaunchKernelGGL((k_spmv<T, EPI_RESID>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
+                               A.plan.blk_row.get(), a, g);
+            break;
+        case EPI_ADD:
+            hipLaunchKernelGGL((k_spmv<T, EPI_ADD>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
+                               A.plan.blk_row.get(), a, g);
+            break;
+        case EPI_JACOBI:
+            hipLaunchKernelGGL((k_spmv<T, EPI_JACOBI>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
+                               A.plan.blk_row.get(), a, g);
+            break;
+        default:
+            hipLaunchKernelGGL((k_spmv<T, EPI_JACOBI_INV>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
+                               A.plan.blk_row.get(), a, g);
+            break;
+    }
+    FEM_HIP(hipGetLastError());
+}
+
+template void csr_spmv<double>(const CsrMat&, const double*, double*, int, const SpmvArgs<double>&, hipStream_t, Gate);
+template void csr_spmv<float>(const CsrMat&, const float*, float*, int, const SpmvArgs<float>&, hipStream_t, Gate);
+
+void CsrMat::apply(const double* x, double* y, hipStream_t s, Gate g) const {
+    csr_spmv<double>(*this, x, y, EPI_STORE, {}, s, g);
+}
+void CsrMat::apply32(const float* x, float* y, hipStream_t s, Gate g) const {
+    csr_spmv<float>(*this, x, y, EPI_STORE, {}, s, g);
+}
+
+void CsrMat::ensure_fp32(hipStream_t s) {
+    if (v32.get() || nnz == 0) return;
+    v32.alloc(nnz);
+    vec_convert<float, double>(v32.get(), v.get(), 1.0, nnz, s);
+}
+
+void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, std::vector<uint32_t> ci_,
+                      std::vector<double> v_, hipStream_t s) {
+    n = rows;
+    m = cols;
+    nnz = v_.size();
+    stream = s;
+    rp.upload(rp_, s);
+    ci.upload(ci_, s);
+    v.upload(v_, s);
+    plan.build(rp_.data(), n, s);
+    h_rp = std::move(rp_);
+    h_ci = std::move(ci_);
+    h_v = std::move(v_);
+    host_valid = true;
+    v32.release();
+    FEM_HIP(hipStreamSynchronize(s));
+}
+
+void CsrMat::sync_host() const {
+    if (host_valid) return;
+    h_rp.resize(n + 1);
+    h_ci.resize(nnz);
+    h_v.resize(nnz);
+    rp.download(h_rp.data(), n + 1, stream);
+    ci.download(h_ci.data(), nnz, stream);
+    v.download(h_v.data(), nnz, stream);
+    FEM_HIP(hipStreamSynchronize(stream));
+    host_valid = true;
+}
+
+}  // namespace fem

@@ -1,0 +1,144 @@
+// internal.hpp — device data structures of libfemgpu.
+#pragma once
+#include <chrono>
+#include <memory>
+#include <vector>
+
+#include "common.hpp"
+
+namespace fem {
+
+// Kernel gate: a kernel runs only while the solve is live (*done == 0) and,
+// when `need` is set, only if *need != 0. This is how data-dependent control
+// flow (convergence, BiCGSTAB's true-residual check, GMRES early exit) is
+// expressed inside captured hipGraphs without host round trips.
+struct Gate {
+    const int* done = nullptr;
+    const int* need = nullptr;
+    __device__ __forceinline__ bool closed() const {
+        return (done && *(volatile const int*)done) || (need && !*(volatile const int*)need);
+    }
+};
+
+struct Timer {
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double ms() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Linear operators on device vectors
+// ---------------------------------------------------------------------------
+struct LinOp {
+    virtual ~LinOp() {}
+    virtual uint32_t rows() const = 0;
+    virtual uint32_t cols() const = 0;
+    virtual void apply(const double* x, double* y, hipStream_t s, Gate g = {}) const = 0;
+    virtual void apply32(const float* x, float* y, hipStream_t s, Gate g = {}) const = 0;
+    virtual void ensure_fp32(hipStream_t s) = 0;
+    virtual uint64_t bytes_per_apply(bool fp32) const = 0;
+};
+
+// SpMV launch schedule: consecutive row ranges whose nonzeros fit one block's
+// LDS staging area, plus single long rows that get a whole block each.
+struct SpmvPlan {
+    DevBuf<uint32_t> blk_row;  // [nblk+1]
+    uint32_t nblk = 0;
+    void build(const uint32_t* h_rp, uint32_t n, hipStream_t s);
+};
+
+struct CsrMat : LinOp {
+    uint32_t n = 0, m = 0;
+    uint64_t nnz = 0;
+    DevBuf<uint32_t> rp, ci;
+    DevBuf<double> v;
+    DevBuf<float> v32;
+    SpmvPlan plan;
+    // host mirror (filled on demand for host-side setup)
+    mutable std::vector<uint32_t> h_rp, h_ci;
+    mutable std::vector<double> h_v;
+    mutable bool host_valid = false;
+    hipStream_t stream = nullptr;  // owning stream
+
+    uint32_t rows() const override { return n; }
+    uint32_t cols() const override { return m; }
+    void apply(const double* x, double* y, hipStream_t s, Gate g = {}) const override;
+    void apply32(const float* x, float* y, hipStream_t s, Gate g = {}) const override;
+    void ensure_fp32(hipStream_t s) override;
+    uint64_t bytes_per_apply(bool fp32) const override {
+        const uint64_t sv = fp32 ? 4 : 8;
+        return nnz * (sv + 4) + 4ull * (n + 1) + (uint64_t)m * sv + (uint64_t)n * sv;
+    }
+    // upload host CSR (takes ownership of the host vectors as mirror)
+    void set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, std::vector<uint32_t> ci_,
+                  std::vector<double> v_, hipStream_t s);
+    void sync_host() const;  // download into the host mirror if needed
+};
+
+// Epilogue kinds of the CSR SpMV kernel.
+enum SpmvEpi : int {
+    EPI_STORE = 0,     // y = A x
+    EPI_RESID = 1,     // y = f - A x
+    EPI_ADD = 2,       // y = u + A x
+    EPI_JACOBI = 3,    // y = u + w (f - A x) / d
+    EPI_JACOBI_INV = 4 // y = u + w (dinv (f - A x))
+};
+
+template <typename T>
+struct SpmvArgs {
+    const T* f = nullptr;
+    const T* u = nullptr;
+    const T* d = nullptr;
+    double w = 0.0;
+};
+
+// Launch CSR SpMV with an epilogue (T = double or float values/vectors).
+template <typename T>
+void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, hipStream_t s, Gate g);
+
+// ---------------------------------------------------------------------------
+// Preconditioners
+// ---------------------------------------------------------------------------
+struct Precond {
+    virtual ~Precond() {}
+    virtual int kind() const = 0;
+    virtual uint32_t size() const = 0;
+    // z = M^{-1} r on device vectors (z != r)
+    virtual void apply(const double* r, double* z, hipStream_t s, Gate g = {}) = 0;
+    virtual void apply32(const float* r, float* z, hipStream_t s, Gate g = {}) = 0;
+    virtual bool has_fp32() const { return false; }
+    virtual void ensure_fp32(hipStream_t) {}
+    virtual bool graph_safe() const { return true; }
+    // Non-null for FEM_PC_DIAG: the diagonal d (z = r / d) for fused kernels.
+    virtual const double* diag64() const { return nullptr; }
+    virtual const float* diag32() const { return nullptr; }
+    double setup_ms = 0.0;
+    int levels = 0;
+    std::vector<uint32_t> lvl_rows;
+    std::vector<uint64_t> lvl_nnz;
+};
+
+std::unique_ptr<Precond> make_precond(CsrMat& A, int kind, const fem_pc_params& p, hipStream_t s);
+
+// Host-side sparse kernels used during setup (csr_ops.cpp).
+void csr_transpose_host(const CsrMat& A, std::vector<uint32_t>& rp, std::vector<uint32_t>& ci, std::vector<double>& v);
+void csr_spgemm_host(const CsrMat& A, const CsrMat& B, std::vector<uint32_t>& rp, std::vector<uint32_t>& ci,
+                     std::vector<double>& v);
+
+// ---------------------------------------------------------------------------
+// Vector kernels (vec.hip)
+// ---------------------------------------------------------------------------
+struct Reducer {
+    double* partials = nullptr;   // kMaxRedBlocks * kMaxRedVals doubles
+    unsigned* counters = nullptr; // RedScratch::kSlots counters
+};
+
+template <typename T>
+void vec_fill(T* y, T val, size_t n, hipStream_t s, Gate g = {});
+template <typename TD, typename TS>
+void vec_convert(TD* y, const TS* x, double scale, size_t n, hipStream_t s, Gate g = {});
+// out = a . b, synchronous helper (host result)
+double dot_sync(const double* a, const double* b, size_t n, const Reducer& R, hipStream_t s);
+
+}  // namespace fem

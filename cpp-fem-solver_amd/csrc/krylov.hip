@@ -1,0 +1,896 @@
+// krylov.hip — PCG, BiCGSTAB and restarted GMRES on the device.
+//
+// Algorithms and stopping rules follow the reference exactly:
+//   pcg       ode-solver/src/math/array.cpp:3445-3601
+//   bicgstab  ode-solver/src/math/array.cpp:3602-3842 (true-residual replacement)
+//   gmres     ode-solver/src/math/array.cpp:4190-4401 (right preconditioning,
+//             modified Gram-Schmidt, Givens rotations)
+// The scalars live in a device KState; every reduction ends in the last
+// block of its kernel, which also evaluates the scalar recurrences, the
+// breakdown tests and the stopping test. All kernels are gated on the state,
+// so a block of iterations can be captured once as a hipGraph and replayed
+// until the device reports completion; the host reads back 4 bytes per
+// replay. Reductions are deterministic (fixed grid and order).
+//
+// FP32_IR: the chosen Krylov method runs in float32 on the scaled residual
+// inside float64 iterative refinement x += ||r|| d, r = b - A x (float64).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+
+#include "reduce.hpp"
+#include "solver.hpp"
+
+namespace fem {
+
+enum PcMode : int { PCM_GEN = 0, PCM_NONE = 1, PCM_DIAG = 2 };
+
+__device__ __forceinline__ void kfail(KState* st, int code) {
+    if (st->soft) { st->done = 1; st->err = -code; }
+    else { st->done = 2; st->err = code; }
+}
+
+#define GRID_LOOP(i, n) for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
+#define DONE_RET(st) if (*(volatile int*)&(st)->done) return
+
+// ---------------------------------------------------------------------------
+// PCG kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_pcg_init(const T* b, T* x, T* r, size_t n, KState* st, double* part, unsigned* cnt) {
+    DONE_RET(st);
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        const T bi = b[i];
+        x[i] = (T)0;
+        r[i] = bi;
+        s += (double)bi * (double)bi;
+    }
+    double v[1] = {s};
+    reduce_finalize<1>(v, part, cnt, [&](double (&t)[1]) {
+        st->normb = sqrt(t[0]);
+        st->normr = st->normb;
+        st->it = 0;
+        st->it_end = 0;
+        st->iters = 0;
+        if (st->normb < 1e-300) { st->done = 1; return; }
+        if (!st->minicg && st->normr / st->normb < st->tol) { st->done = 1; return; }
+    });
+}
+
+template <typename T, int PCM>
+__device__ __forceinline__ T zval(const T* r, const T* z, const T* d, size_t i) {
+    if constexpr (PCM == PCM_NONE) return r[i];
+    else if constexpr (PCM == PCM_DIAG) return r[i] / d[i];
+    else return z[i];
+}
+
+// z (given or implied) -> p = z, rho = r^T z
+template <typename T, int PCM>
+__global__ void k_pcg_init_rz(const T* r, const T* z, const T* d, T* p, size_t n, KState* st, double* part,
+                              unsigned* cnt) {
+    DONE_RET(st);
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        const T zi = zval<T, PCM>(r, z, d, i);
+        p[i] = zi;
+        s += (double)r[i] * (double)zi;
+    }
+    double v[1] = {s};
+    reduce_finalize<1>(v, part, cnt, [&](double (&t)[1]) {
+        if (t[0] <= 0.0 && !st->minicg) { kfail(st, SE_PCG_RZ); return; }
+        st->rz = t[0];
+    });
+}
+
+template <typename T>
+__global__ void k_pcg_pq(const T* p, const T* q, size_t n, KState* st, double* part, unsigned* cnt) {
+    DONE_RET(st);
+    double s = 0.0;
+    GRID_LOOP(i, n) s += (double)p[i] * (double)q[i];
+    double v[1] = {s};
+    reduce_finalize<1>(v, part, cnt, [&](double (&t)[1]) {
+        const double pap = t[0];
+        st->pap = pap;
+        st->iters += 1;
+        if (!st->minicg) {
+            if (fabs(pap) < 1e-30) { kfail(st, SE_PCG_PAP_SMALL); return; }
+            if (pap <= 0.0) { kfail(st, SE_PCG_PAP_NEG); return; }
+        }
+        st->alpha = st->rz / pap;
+    });
+}
+
+__device__ __forceinline__ void pcg_finish(KState* st, double rz_new) {
+    if (rz_new <= 0.0 && !st->minicg) { kfail(st, SE_PCG_RZ); return; }
+    st->beta = rz_new / st->rz;
+    st->rz = rz_new;
+    st->it_end = st->it;
+    st->it += 1;
+    if (st->it >= st->max_iter) st->done = 3;
+}
+
+// x += alpha p ; r -= alpha q ; ||r|| stopping test (and rho for none/diag)
+template <typename T, int PCM>
+__global__ void k_pcg_xr(T* x, T* r, const T* p, const T* q, const T* d, size_t n, KState* st, double* part,
+                         unsigned* cnt) {
+    DONE_RET(st);
+    const T a = (T)st->alpha;
+    double rr = 0.0, rz = 0.0;
+    GRID_LOOP(i, n) {
+        x[i] += a * p[i];
+        const T ri = r[i] - a * q[i];
+        r[i] = ri;
+        rr += (double)ri * (double)ri;
+        if constexpr (PCM == PCM_DIAG) rz += (double)ri * (double)(ri / d[i]);
+    }
+    if constexpr (PCM == PCM_NONE) rz = rr;
+    double v[2] = {rr, rz};
+    reduce_finalize<2>(v, part, cnt, [&](double (&t)[2]) {
+        st->normr = sqrt(t[0]);
+        if (st->normr / st->normb < st->tol) { st->done = 1; return; }
+        if constexpr (PCM != PCM_GEN) pcg_finish(st, t[1]);
+    });
+}
+
+template <typename T>
+__global__ void k_pcg_rz(const T* r, const T* z, size_t n, KState* st, double* part, unsigned* cnt) {
+    DONE_RET(st);
+    double s = 0.0;
+    GRID_LOOP(i, n) s += (double)r[i] * (double)z[i];
+    double v[1] = {s};
+    reduce_finalize<1>(v, part, cnt, [&](double (&t)[1]) { pcg_finish(st, t[0]); });
+}
+
+template <typename T, int PCM>
+__global__ void k_pcg_p(const T* r, const T* z, const T* d, T* p, size_t n, KState* st) {
+    DONE_RET(st);
+    const T b = (T)st->beta;
+    GRID_LOOP(i, n) p[i] = zval<T, PCM>(r, z, d, i) + b * p[i];
+}
+
+// ---------------------------------------------------------------------------
+// BiCGSTAB kernels
+// ---------------------------------------------------------------------------
+constexpr double kBT = 1e-30;
+
+template <typename T>
+__global__ void k_bi_init(const T* b, T* x, T* r, T* rt, T* p, T* v, size_t n, KState* st, double* part,
+                          unsigned* cnt) {
+    DONE_RET(st);
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        const T bi = b[i];
+        x[i] = (T)0;
+        r[i] = bi;
+        rt[i] = bi;
+        p[i] = (T)0;
+        v[i] = (T)0;
+        s += (double)bi * (double)bi;
+    }
+    double w[1] = {s};
+    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+        st->normb = sqrt(t[0]);
+        st->normr = st->normb;
+        st->it = 0;
+        st->it_end = 0;
+        st->iters = 0;
+        st->rz = 1.0;  // rho_old
+        st->alpha = 1.0;
+        st->omega = 1.0;
+        st->check = 0;
+        st->replace = 0;
+        if (st->normb < 1e-300) { st->done = 1; return; }
+        if (st->normr / st->normb < st->tol) { st->done = 1; return; }
+    });
+}
+
+template <typename T>
+__global__ void k_bi_rho(const T* rt, const T* r, size_t n, KState* st, double* part, unsigned* cnt) {
+    DONE_RET(st);
+    double s = 0.0;
+    GRID_LOOP(i, n) s += (double)rt[i] * (double)r[i];
+    double w[1] = {s};
+    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+        const double rho = t[0];
+        st->rho = rho;
+        if (fabs(rho) < kBT) { kfail(st, SE_BI_RHO); return; }
+        if (fabs(st->rz) < kBT) { kfail(st, SE_BI_RHO_OLD); return; }
+        if (fabs(st->omega) < kBT) { kfail(st, SE_BI_OMEGA); return; }
+        st->beta = (rho / st->rz) * (st->alpha / st->omega);
+    });
+}
+
+template <typename T>
+__global__ void k_bi_p(const T* r, const T* v, T* p, size_t n, KState* st) {
+    DONE_RET(st);
+    const T b = (T)st->beta, om = (T)st->omega;
+    GRID_LOOP(i, n) p[i] = r[i] + b * (p[i] - om * v[i]);
+}
+
+template <typename T>
+__global__ void k_bi_rv(const T* rt, const T* v, size_t n, KState* st, double* part, unsigned* cnt) {
+    DONE_RET(st);
+    double s = 0.0;
+    GRID_LOOP(i, n) s += (double)rt[i] * (double)v[i];
+    double w[1] = {s};
+    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+        st->rv = t[0];
+        st->iters += 1;
+        if (fabs(t[0]) < kBT) { kfail(st, SE_BI_RV); return; }
+        st->alpha = st->rho / t[0];
+    });
+}
+
+template <typename T>
+__global__ void k_bi_s(const T* r, const T* v, T* sv, size_t n, KState* st) {
+    DONE_RET(st);
+    const T a = (T)st->alpha;
+    GRID_LOOP(i, n) sv[i] = r[i] - a * v[i];
+}
+
+template <typename T>
+__global__ void k_bi_tt(const T* t_, const T* sv, size_t n, KState* st, double* part, unsigned* cnt) {
+    DONE_RET(st);
+    double a = 0.0, b = 0.0;
+    GRID_LOOP(i, n) {
+        const double ti = (double)t_[i];
+        a += ti * ti;
+        b += ti * (double)sv[i];
+    }
+    double w[2] = {a, b};
+    reduce_finalize<2>(w, part, cnt, [&](double (&t)[2]) {
+        st->tt = t[0];
+        st->ts = t[1];
+        if (t[0] < kBT) { kfail(st, SE_BI_TT); return; }
+        st->omega = t[1] / t[0];
+    });
+}
+
+template <typename T>
+__global__ void k_bi_xr(T* x, T* r, const T* y, const T* z, const T* sv, const T* t_, size_t n, KState* st,
+                        double* part, unsigned* cnt) {
+    DONE_RET(st);
+    const T a = (T)st->alpha, om = (T)st->omega;
+    double rr = 0.0;
+    GRID_LOOP(i, n) {
+        x[i] += a * y[i] + om * z[i];
+        const T ri = sv[i] - om * t_[i];
+        r[i] = ri;
+        rr += (double)ri * (double)ri;
+    }
+    double w[1] = {rr};
+    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+        st->normr = sqrt(t[0]);
+        st->it_end = st->it;
+        if (st->normr / st->normb < st->tol) st->check = 1;
+        else { st->check = 0; st->rz = st->rho; }
+    });
+}
+
+// true residual b - A x  (ax = A x computed before, gated on check)
+template <typename T>
+__global__ void k_bi_true(const T* b, const T* ax, size_t n, KState* st, double* part, unsigned* cnt) {
+    DONE_RET(st);
+    if (!st->check) return;
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        const double d = (double)b[i] - (double)ax[i];
+        s += d * d;
+    }
+    double w[1] = {s};
+    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+        st->true_normr = sqrt(t[0]);
+        if (st->true_normr / st->normb < st->tol) { st->done = 1; st->replace = 0; }
+        else { st->replace = 1; st->normr = st->true_normr; }
+    });
+}
+
+template <typename T>
+__global__ void k_bi_replace(const T* b, const T* ax, const T* rt, T* r, size_t n, KState* st, double* part,
+                             unsigned* cnt) {
+    DONE_RET(st);
+    if (!st->replace) return;
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        const T ri = b[i] - ax[i];
+        r[i] = ri;
+        s += (double)rt[i] * (double)ri;
+    }
+    double w[1] = {s};
+    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+        if (fabs(t[0]) < kBT) { st->done = 1; return; }
+        st->rz = t[0];
+    });
+}
+
+__global__ void k_bi_tail(KState* st) {
+    DONE_RET(st);
+    st->check = 0;
+    st->replace = 0;
+    st->it += 1;
+    if (st->it >= st->max_iter) st->done = 3;
+}
+
+// ---------------------------------------------------------------------------
+// GMRES kernels; gm = [H (m+1) x m | cs m | sn m | g m+1 | y m]
+// ---------------------------------------------------------------------------
+struct GmLayout {
+    int m;
+    __host__ __device__ double* H(double* gm) const { return gm; }
+    __host__ __device__ double* cs(double* gm) const { return gm + (size_t)(m + 1) * m; }
+    __host__ __device__ double* sn(double* gm) const { return cs(gm) + m; }
+    __host__ __device__ double* g(double* gm) const { return sn(gm) + m; }
+    __host__ __device__ double* y(double* gm) const { return g(gm) + m + 1; }
+    static size_t size(int m) { return (size_t)(m + 1) * m + 4 * (size_t)m + 1; }
+};
+
+template <typename T>
+__global__ void k_gm_init(const T* b, T* x, size_t n, KState* st, double* part, unsigned* cnt) {
+    DONE_RET(st);
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        x[i] = (T)0;
+        s += (double)b[i] * (double)b[i];
+    }
+    double w[1] = {s};
+    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+        st->normb = sqrt(t[0]);
+        st->it = 0;
+        st->it_end = 0;
+        st->iters = 0;
+        if (st->normb < 1e-300) st->done = 1;
+    });
+}
+
+__global__ void k_gm_cycle_begin(KState* st) {
+    DONE_RET(st);
+    st->live = st->it < st->max_iter ? 1 : 0;
+    st->need_true = st->it > 0 ? 1 : 0;
+    st->conv = 0;
+    st->j_used = 0;
+}
+
+template <typename T>
+__global__ void k_gm_v0(const T* b, const T* ax, T* v0, size_t n, KState* st, double* gm, GmLayout L, double* part,
+                        unsigned* cnt) {
+    DONE_RET(st);
+    const int nt = st->need_true;
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        const T ri = nt ? (T)(b[i] - ax[i]) : b[i];
+        v0[i] = ri;
+        s += (double)ri * (double)ri;
+    }
+    double w[1] = {s};
+    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+        const double beta = nt ? sqrt(t[0]) : st->normb;
+        st->beta_g = beta;
+        if (nt && beta / st->normb < st->tol) { st->done = 1; return; }
+        double* g = L.g(gm);
+        for (int i = 0; i <= L.m; ++i) g[i] = 0.0;
+        g[0] = beta;
+        double* H = L.H(gm);
+        for (int i = 0; i < (L.m + 1) * L.m; ++i) H[i] = 0.0;
+    });
+}
+
+template <typename T>
+__global__ void k_gm_scale(T* v0, size_t n, KState* st) {
+    DONE_RET(st);
+    const T beta = (T)st->beta_g;
+    GRID_LOOP(i, n) v0[i] = v0[i] / beta;
+}
+
+__device__ __forceinline__ bool gm_closed(const KState* st) {
+    return *(volatile const int*)&st->done || !*(volatile const int*)&st->live;
+}
+
+// h_{0j} = w . V_0
+template <typename T>
+__global__ void k_gm_dot0(const T* w, const T* v0, size_t n, int j, KState* st, double* gm, GmLayout L,
+                          double* part, unsigned* cnt) {
+    if (gm_closed(st)) return;
+    double s = 0.0;
+    GRID_LOOP(i, n) s += (double)w[i] * (double)v0[i];
+    double a[1] = {s};
+    reduce_finalize<1>(a, part, cnt, [&](double (&t)[1]) { L.H(gm)[0 * L.m + j] = t[0]; });
+}
+
+// w -= h_{i-1,j} V_{i-1}; h_{ij} = w . V_i
+template <typename T>
+__global__ void k_gm_axpy_dot(T* w, const T* vprev, const T* vi, size_t n, int i_, int j, KState* st, double* gm,
+                              GmLayout L, double* part, unsigned* cnt) {
+    if (gm_closed(st)) return;
+    const T h = (T)L.H(gm)[(i_ - 1) * L.m + j];
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        const T wi = w[i] - h * vprev[i];
+        w[i] = wi;
+        s += (double)wi * (double)vi[i];
+    }
+    double a[1] = {s};
+    reduce_finalize<1>(a, part, cnt, [&](double (&t)[1]) { L.H(gm)[i_ * L.m + j] = t[0]; });
+}
+
+// w -= h_{jj} V_j; h_{j+1,j} = ||w||; Givens rotations and stopping test
+template <typename T>
+__global__ void k_gm_axpy_norm(T* w, const T* vj, size_t n, int j, KState* st, double* gm, GmLayout L,
+                               double* part, unsigned* cnt) {
+    if (gm_closed(st)) return;
+    const T h = (T)L.H(gm)[j * L.m + j];
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        const T wi = w[i] - h * vj[i];
+        w[i] = wi;
+        s += (double)wi * (double)wi;
+    }
+    double a[1] = {s};
+    reduce_finalize<1>(a, part, cnt, [&](double (&t)[1]) {
+        const int m = L.m;
+        double* H = L.H(gm);
+        double* cs = L.cs(gm);
+        double* sn = L.sn(gm);
+        double* g = L.g(gm);
+        const double hn = sqrt(t[0]);
+        st->hnorm = hn;
+        H[(j + 1) * m + j] = hn;
+        for (int i = 0; i < j; ++i) {
+            const double t1 = cs[i] * H[i * m + j] + sn[i] * H[(i + 1) * m + j];
+            const double t2 = -sn[i] * H[i * m + j] + cs[i] * H[(i + 1) * m + j];
+            H[i * m + j] = t1;
+            H[(i + 1) * m + j] = t2;
+        }
+        const double den = hypot(H[j * m + j], H[(j + 1) * m + j]);
+        if (den < kBT) { kfail(st, SE_GM_DEN); return; }
+        cs[j] = H[j * m + j] / den;
+        sn[j] = H[(j + 1) * m + j] / den;
+        H[j * m + j] = cs[j] * H[j * m + j] + sn[j] * H[(j + 1) * m + j];
+        H[(j + 1) * m + j] = 0.0;
+        const double g1 = cs[j] * g[j] + sn[j] * g[j + 1];
+        const double g2 = -sn[j] * g[j] + cs[j] * g[j + 1];
+        g[j] = g1;
+        g[j + 1] = g2;
+        st->j_used = j + 1;
+        st->it_end = st->it;
+        st->it += 1;
+        st->iters += 1;
+        if (fabs(g[j + 1]) / st->normb < st->tol) { st->conv = 1; st->live = 0; }
+        else if (st->it >= st->max_iter) st->live = 0;
+    });
+}
+
+// V_{j+1} = w * (1 / h_{j+1,j}) when h_{j+1,j} > 1e-30. Skipped once the
+// stopping test fired: V_{j+1} is then never read.
+template <typename T>
+__global__ void k_gm_normalize(const T* w, T* vnext, size_t n, KState* st) {
+    if (gm_closed(st)) return;
+    const double hn = st->hnorm;
+    if (!(hn > kBT)) return;
+    const T inv = (T)(1.0 / hn);
+    GRID_LOOP(i, n) vnext[i] = w[i] * inv;
+}
+
+__global__ void k_gm_backsolve(KState* st, double* gm, GmLayout L) {
+    DONE_RET(st);
+    const int m = L.m, ju = st->j_used;
+    double* H = L.H(gm);
+    double* g = L.g(gm);
+    double* y = L.y(gm);
+    for (int i = ju - 1; i >= 0; --i) {
+        double s = g[i];
+        for (int k = i + 1; k < ju; ++k) s -= H[i * m + k] * y[k];
+        if (fabs(H[i * m + i]) < kBT) { kfail(st, SE_GM_TRI); return; }
+        y[i] = s / H[i * m + i];
+    }
+}
+
+template <typename T>
+__global__ void k_gm_combine(const T* V, size_t n, T* z, KState* st, const double* gm, GmLayout L) {
+    DONE_RET(st);
+    const int ju = st->j_used;
+    const double* y = L.y(const_cast<double*>(gm));
+    GRID_LOOP(t, n) {
+        T acc = (T)0;
+        for (int i = 0; i < ju; ++i) acc += (T)y[i] * V[(size_t)i * n + t];
+        z[t] = acc;
+    }
+}
+
+template <typename T>
+__global__ void k_add(T* x, const T* dz, size_t n, KState* st) {
+    DONE_RET(st);
+    GRID_LOOP(i, n) x[i] += dz[i];
+}
+
+__global__ void k_gm_cycle_end(KState* st) {
+    DONE_RET(st);
+    if (st->conv) st->done = 1;
+    else if (st->it >= st->max_iter) st->done = 3;
+}
+
+// ---------------------------------------------------------------------------
+// Refinement kernels
+// ---------------------------------------------------------------------------
+// res = b - ax, partial ||res||^2 -> out[0]
+__global__ void k_resid_norm(const double* b, const double* ax, double* res, size_t n, double* out, double* part,
+                             unsigned* cnt) {
+    double s = 0.0;
+    GRID_LOOP(i, n) {
+        const double r = b[i] - ax[i];
+        res[i] = r;
+        s += r * r;
+    }
+    double w[1] = {s};
+    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) { out[0] = t[0]; });
+}
+
+__global__ void k_axpy_f32(double* x, const float* d, double a, size_t n) {
+    GRID_LOOP(i, n) x[i] += a * (double)d[i];
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+template <typename T>
+void KWork<T>::ensure(int solver, uint32_t n_, int m_) {
+    if (n == n_ && m == m_ && x.get()) return;
+    n = n_;
+    m = m_;
+    x.alloc(n); r.alloc(n); z.alloc(n); p.alloc(n); q.alloc(n); b.alloc(n);
+    if (solver == FEM_SOLVER_BICGSTAB) { rt.alloc(n); y.alloc(n); s.alloc(n); t.alloc(n); ax.alloc(n); }
+    if (solver == FEM_SOLVER_GMRES) {
+        V.alloc((size_t)(m + 1) * n); w.alloc(n); wt.alloc(n); dz.alloc(n); ax.alloc(n);
+        gm.alloc(GmLayout::size(m));
+    }
+}
+template struct KWork<double>;
+template struct KWork<float>;
+
+void GraphCache::reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    tag = -1;
+}
+
+KrylovSolver::KrylovSolver(LinOp* A, Precond* M, const fem_solve_opts& o, hipStream_t s)
+    : opts(o), A_(A), M_(M), s_(s) {
+    if (A->rows() != A->cols()) fail(FEM_ERR_INVALID_ARGUMENT, "the solver needs a square matrix");
+    if (M && M->size() != A->rows()) fail(FEM_ERR_INVALID_ARGUMENT, "preconditioner size differs from the matrix");
+    if (o.solver == FEM_SOLVER_GMRES && o.restart < 1) fail(FEM_ERR_INVALID_ARGUMENT, "GMRES restart must be >= 1");
+    if (o.solver < 0 || o.solver > FEM_SOLVER_MINICG) fail(FEM_ERR_INVALID_ARGUMENT, "unknown solver kind");
+    red_.init(s);
+    st_.alloc(1);
+    hst_.ensure(1);
+    auto mode = [&](bool fp32) {
+        if (!M || M->kind() == FEM_PC_NONE) return (int)PCM_NONE;
+        if (fp32 ? M->diag32() != nullptr : M->diag64() != nullptr) return (int)PCM_DIAG;
+        return (int)PCM_GEN;
+    };
+    pc_mode64_ = mode(false);
+    if (o.precision == FEM_PREC_FP32_IR) {
+        A_->ensure_fp32(s);
+        if (M) M_->ensure_fp32(s);
+        pc_mode32_ = mode(true);
+    }
+}
+
+KrylovSolver::~KrylovSolver() {}
+
+template <typename T>
+void KrylovSolver::precond(const T* r, T* z, Gate g) {
+    if constexpr (std::is_same<T, double>::value) M_->apply(r, z, s_, g);
+    else M_->apply32(r, z, s_, g);
+}
+template <typename T>
+void KrylovSolver::matvec(const T* x, T* y, Gate g) {
+    if constexpr (std::is_same<T, double>::value) A_->apply(x, y, s_, g);
+    else A_->apply32(x, y, s_, g);
+}
+
+#define LAUNCH(kern, grid, ...)                                                  \
+    do {                                                                         \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s_, __VA_ARGS__);  \
+        FEM_HIP(hipGetLastError());                                              \
+    } while (0)
+
+template <typename T>
+void KrylovSolver::pcg_init(KWork<T>& W) {
+    const size_t n = W.n;
+    const unsigned G = red_blocks(n);
+    KState* st = st_.get();
+    double* part = red_.partials.get();
+    unsigned* cnt = red_.counter.get();
+    const int pm = std::is_same<T, double>::value ? pc_mode64_ : pc_mode32_;
+    const T* d = nullptr;
+    if (pm == PCM_DIAG) {
+        if constexpr (std::is_same<T, double>::value) d = M_->diag64();
+        else d = M_->diag32();
+    }
+    LAUNCH((k_pcg_init<T>), G, W.b.get(), W.x.get(), W.r.get(), n, st, part, cnt);
+    if (pm == PCM_GEN) {
+        precond<T>(W.r.get(), W.z.get(), Gate{&st->done});
+        LAUNCH((k_pcg_init_rz<T, PCM_GEN>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st, part, cnt);
+    } else if (pm == PCM_DIAG) {
+        LAUNCH((k_pcg_init_rz<T, PCM_DIAG>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st, part, cnt);
+    } else {
+        LAUNCH((k_pcg_init_rz<T, PCM_NONE>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st, part, cnt);
+    }
+}
+
+template <typename T>
+void KrylovSolver::pcg_iter(KWork<T>& W) {
+    const size_t n = W.n;
+    const unsigned G = red_blocks(n);
+    KState* st = st_.get();
+    double* part = red_.partials.get();
+    unsigned* cnt = red_.counter.get();
+    const int pm = std::is_same<T, double>::value ? pc_mode64_ : pc_mode32_;
+    const T* d = nullptr;
+    if (pm == PCM_DIAG) {
+        if constexpr (std::is_same<T, double>::value) d = M_->diag64();
+        else d = M_->diag32();
+    }
+    Gate g{&st->done};
+    matvec<T>(W.p.get(), W.q.get(), g);
+    LAUNCH((k_pcg_pq<T>), G, W.p.get(), W.q.get(), n, st, part, cnt);
+    if (pm == PCM_GEN) {
+        LAUNCH((k_pcg_xr<T, PCM_GEN>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part, cnt);
+        precond<T>(W.r.get(), W.z.get(), g);
+        LAUNCH((k_pcg_rz<T>), G, W.r.get(), W.z.get(), n, st, part, cnt);
+        LAUNCH((k_pcg_p<T, PCM_GEN>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
+    } else if (pm == PCM_DIAG) {
+        LAUNCH((k_pcg_xr<T, PCM_DIAG>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part, cnt);
+        LAUNCH((k_pcg_p<T, PCM_DIAG>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
+    } else {
+        LAUNCH((k_pcg_xr<T, PCM_NONE>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part, cnt);
+        LAUNCH((k_pcg_p<T, PCM_NONE>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
+    }
+}
+
+template <typename T>
+void KrylovSolver::bi_init(KWork<T>& W) {
+    const size_t n = W.n;
+    LAUNCH((k_bi_init<T>), red_blocks(n), W.b.get(), W.x.get(), W.r.get(), W.rt.get(), W.p.get(), W.q.get(), n,
+           st_.get(), red_.partials.get(), red_.counter.get());
+}
+
+template <typename T>
+void KrylovSolver::bi_iter(KWork<T>& W) {
+    const size_t n = W.n;
+    const unsigned G = red_blocks(n);
+    KState* st = st_.get();
+    double* part = red_.partials.get();
+    unsigned* cnt = red_.counter.get();
+    Gate g{&st->done};
+    T* v = W.q.get();
+    // with the identity preconditioner y = p and z = s (aliases, no copies)
+    const bool has_pc = M_ && M_->kind() != FEM_PC_NONE;
+    T* y = has_pc ? W.y.get() : W.p.get();
+    T* z = has_pc ? W.z.get() : W.s.get();
+    LAUNCH((k_bi_rho<T>), G, W.rt.get(), W.r.get(), n, st, part, cnt);
+    LAUNCH((k_bi_p<T>), G, W.r.get(), v, W.p.get(), n, st);
+    if (has_pc) precond<T>(W.p.get(), y, g);
+    matvec<T>(y, v, g);
+    LAUNCH((k_bi_rv<T>), G, W.rt.get(), v, n, st, part, cnt);
+    LAUNCH((k_bi_s<T>), G, W.r.get(), v, W.s.get(), n, st);
+    if (has_pc) precond<T>(W.s.get(), z, g);
+    matvec<T>(z, W.t.get(), g);
+    LAUNCH((k_bi_tt<T>), G, W.t.get(), W.s.get(), n, st, part, cnt);
+    LAUNCH((k_bi_xr<T>), G, W.x.get(), W.r.get(), y, z, W.s.get(), W.t.get(), n, st, part, cnt);
+    matvec<T>(W.x.get(), W.ax.get(), Gate{&st->done, &st->check});
+    LAUNCH((k_bi_true<T>), G, W.b.get(), W.ax.get(), n, st, part, cnt);
+    LAUNCH((k_bi_replace<T>), G, W.b.get(), W.ax.get(), W.rt.get(), W.r.get(), n, st, part, cnt);
+    LAUNCH(k_bi_tail, 1, st);
+}
+
+template <typename T>
+void KrylovSolver::gm_cycle(KWork<T>& W) {
+    const size_t n = W.n;
+    const unsigned G = red_blocks(n);
+    const int m = W.m;
+    KState* st = st_.get();
+    double* part = red_.partials.get();
+    unsigned* cnt = red_.counter.get();
+    GmLayout L{m};
+    double* gm = W.gm.get();
+    T* V = W.V.get();
+    auto Vi = [&](int i) { return V + (size_t)i * n; };
+    LAUNCH(k_gm_cycle_begin, 1, st);
+    matvec<T>(W.x.get(), W.ax.get(), Gate{&st->done, &st->need_true});
+    LAUNCH((k_gm_v0<T>), G, W.b.get(), W.ax.get(), Vi(0), n, st, gm, L, part, cnt);
+    LAUNCH((k_gm_scale<T>), G, Vi(0), n, st);
+    const bool has_pc = M_ && M_->kind() != FEM_PC_NONE;
+    for (int j = 0; j < m; ++j) {
+        Gate g{&st->done, &st->live};
+        const T* src = Vi(j);
+        if (has_pc) {
+            precond<T>(Vi(j), W.wt.get(), g);
+            src = W.wt.get();
+        }
+        matvec<T>(src, W.w.get(), g);
+        LAUNCH((k_gm_dot0<T>), G, W.w.get(), Vi(0), n, j, st, gm, L, part, cnt);
+        for (int i = 1; i <= j; ++i)
+            LAUNCH((k_gm_axpy_dot<T>), G, W.w.get(), Vi(i - 1), Vi(i), n, i, j, st, gm, L, part, cnt);
+        LAUNCH((k_gm_axpy_norm<T>), G, W.w.get(), Vi(j), n, j, st, gm, L, part, cnt);
+        LAUNCH((k_gm_normalize<T>), G, W.w.get(), Vi(j + 1), n, st);
+    }
+    LAUNCH(k_gm_backsolve, 1, st, gm, L);
+    LAUNCH((k_gm_combine<T>), G, V, n, W.z.get(), st, gm, L);
+    if (has_pc) {
+        precond<T>(W.z.get(), W.dz.get(), Gate{&st->done});
+        LAUNCH((k_add<T>), G, W.x.get(), W.dz.get(), n, st);
+    } else {
+        LAUNCH((k_add<T>), G, W.x.get(), W.z.get(), n, st);
+    }
+    LAUNCH(k_gm_cycle_end, 1, st);
+}
+
+void KrylovSolver::replay(GraphCache& gc, int tag, const std::function<void()>& body) {
+    if (!opts.use_graph || (M_ && !M_->graph_safe())) {
+        body();
+        return;
+    }
+    if (gc.tag != tag) {
+        gc.reset();
+        FEM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeThreadLocal));
+        try {
+            body();
+        } catch (...) {
+            hipGraph_t tmp = nullptr;
+            (void)hipStreamEndCapture(s_, &tmp);
+            if (tmp) (void)hipGraphDestroy(tmp);
+            throw;
+        }
+        FEM_HIP(hipStreamEndCapture(s_, &gc.graph));
+        FEM_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
+        gc.tag = tag;
+    }
+    FEM_HIP(hipGraphLaunch(gc.exec, s_));
+}
+
+void KrylovSolver::read_state() {
+    FEM_HIP(hipMemcpyAsync(hst_.p, st_.get(), sizeof(KState), hipMemcpyDeviceToHost, s_));
+    FEM_HIP(hipStreamSynchronize(s_));
+}
+
+template <typename T>
+int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long long* spmv) {
+    KState h{};
+    h.tol = tol;
+    h.max_iter = max_iter;
+    h.soft = soft ? 1 : 0;
+    h.minicg = opts.solver == FEM_SOLVER_MINICG ? 1 : 0;
+    *hst_.p = h;
+    FEM_HIP(hipMemcpyAsync(st_.get(), hst_.p, sizeof(KState), hipMemcpyHostToDevice, s_));
+    const int solver = opts.solver == FEM_SOLVER_MINICG ? FEM_SOLVER_PCG : opts.solver;
+    if (solver == FEM_SOLVER_PCG) pcg_init<T>(W);
+    else if (solver == FEM_SOLVER_BICGSTAB) bi_init<T>(W);
+    else LAUNCH((k_gm_init<T>), red_blocks(W.n), W.b.get(), W.x.get(), (size_t)W.n, st_.get(), red_.partials.get(),
+                red_.counter.get());
+    read_state();
+    if (max_iter <= 0 && solver != FEM_SOLVER_GMRES) {
+        // the reference's loop body never runs: x stays 0
+        return 0;
+    }
+    int chunk = opts.check_every > 0 ? opts.check_every : (W.n < 200000 ? 32 : 16);
+    GraphCache& gc = std::is_same<T, double>::value ? g64_ : g32_;
+    const int tag = solver * 100000 + (solver == FEM_SOLVER_GMRES ? W.m : chunk);
+    auto body = [&] {
+        if (solver == FEM_SOLVER_PCG) for (int k = 0; k < chunk; ++k) pcg_iter<T>(W);
+        else if (solver == FEM_SOLVER_BICGSTAB) for (int k = 0; k < chunk; ++k) bi_iter<T>(W);
+        else gm_cycle<T>(W);
+    };
+    // every replay runs `chunk` iterations (GMRES: one cycle of >= 1 step)
+    const long long guard_max = (long long)max_iter / (solver == FEM_SOLVER_GMRES ? 1 : chunk) + 4;
+    long long rounds = 0;
+    while (!hst_.p->done) {
+        replay(gc, tag, body);
+        FEM_HIP(hipMemcpyAsync(&hst_.p->done, &st_.get()->done, sizeof(int), hipMemcpyDeviceToHost, s_));
+        FEM_HIP(hipStreamSynchronize(s_));
+        if (++rounds > guard_max) fail(FEM_ERR_RUNTIME, "solver made no progress");
+    }
+    read_state();
+    if (hst_.p->done == 2) fail(FEM_ERR_RUNTIME, solver_err_msg(hst_.p->err));
+    if (spmv) *spmv += hst_.p->iters;
+    return hst_.p->iters;
+}
+
+double KrylovSolver::true_residual(const double* b, const double* x, double* work) {
+    const size_t n = A_->rows();
+    A_->apply(x, work, s_);
+    DevBuf<double> out(1);
+    LAUNCH(k_resid_norm, red_blocks(n), b, work, work, n, out.get(), red_.partials.get(), red_.counter.get());
+    double h = 0.0;
+    out.download(&h, 1, s_);
+    FEM_HIP(hipStreamSynchronize(s_));
+    return std::sqrt(h);
+}
+
+void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
+    const uint32_t n = A_->rows();
+    const int m = opts.solver == FEM_SOLVER_GMRES ? opts.restart : 0;
+    fem_solve_stats S{};
+    Timer tm;
+    if (opts.precision == FEM_PREC_FP64) {
+        w64_.ensure(opts.solver, n, m);
+        FEM_HIP(hipMemcpyAsync(w64_.b.get(), b, (size_t)n * 8, hipMemcpyDeviceToDevice, s_));
+        S.inner_iterations = run<double>(w64_, opts.tol, opts.max_iter, false, &S.spmv_count);
+        // mini_cg returns the raw loop index (test_BlockMatrix_mpi.cpp:157-182)
+        S.iterations = opts.solver == FEM_SOLVER_MINICG ? hst_.p->it : hst_.p->it_end;
+        S.converged = hst_.p->done == 1 ? 1 : 0;
+        FEM_HIP(hipMemcpyAsync(x, w64_.x.get(), (size_t)n * 8, hipMemcpyDeviceToDevice, s_));
+        FEM_HIP(hipStreamSynchronize(s_));
+        S.solve_ms = tm.ms();
+    } else {
+        // float32 Krylov inside float64 iterative refinement
+        w32_.ensure(opts.solver, n, m);
+        if (x64_.n != n) { x64_.alloc(n); ax64_.alloc(n); res64_.alloc(n); }
+        DevBuf<double> nrm(1);
+        vec_fill<double>(x64_.get(), 0.0, n, s_);
+        FEM_HIP(hipMemcpyAsync(res64_.get(), b, (size_t)n * 8, hipMemcpyDeviceToDevice, s_));
+        // ||b||
+        vec_fill<double>(ax64_.get(), 0.0, n, s_);
+        LAUNCH(k_resid_norm, red_blocks(n), b, ax64_.get(), res64_.get(), (size_t)n, nrm.get(), red_.partials.get(),
+               red_.counter.get());
+        double h = 0.0;
+        nrm.download(&h, 1, s_);
+        FEM_HIP(hipStreamSynchronize(s_));
+        const double normb = std::sqrt(h);
+        double normr = normb;
+        int total = 0;
+        int refine = 0;
+        const int max_ref = opts.max_refine > 0 ? opts.max_refine : 50;
+        bool conv = false;
+        if (normb < 1e-300) conv = true;
+        while (!conv && refine < max_ref && total < opts.max_iter) {
+            if (normr / normb < opts.tol) { conv = true; break; }
+            vec_convert<float, double>(w32_.b.get(), res64_.get(), 1.0 / normr, n, s_);
+            const double want = std::max(opts.inner_tol, 0.5 * opts.tol * normb / normr);
+            total += run<float>(w32_, want, opts.max_iter - total, true, &S.spmv_count);
+            LAUNCH(k_axpy_f32, red_blocks(n), x64_.get(), w32_.x.get(), normr, (size_t)n);
+            A_->apply(x64_.get(), ax64_.get(), s_);
+            S.spmv_count += 1;
+            LAUNCH(k_resid_norm, red_blocks(n), b, ax64_.get(), res64_.get(), (size_t)n, nrm.get(),
+                   red_.partials.get(), red_.counter.get());
+            nrm.download(&h, 1, s_);
+            FEM_HIP(hipStreamSynchronize(s_));
+            const double nr = std::sqrt(h);
+            ++refine;
+            if (std::getenv("FEMGPU_DEBUG"))
+                std::fprintf(stderr, "[femgpu] refine %d: inner tol %.2e iters %d -> ||r||/||b|| %.3e (inner done %d err %d)\n",
+                             refine, want, total, nr / normb, hst_.p->done, hst_.p->err);
+            if (!(nr < normr * 0.999)) { normr = nr; break; }  // stagnation in float32
+            normr = nr;
+        }
+        if (normr / normb < opts.tol) conv = true;
+        S.inner_iterations = total;
+        S.iterations = total;
+        S.refinements = refine;
+        S.converged = conv ? 1 : 0;
+        FEM_HIP(hipMemcpyAsync(x, x64_.get(), (size_t)n * 8, hipMemcpyDeviceToDevice, s_));
+        FEM_HIP(hipStreamSynchronize(s_));
+        S.solve_ms = tm.ms();
+    }
+    if (stats) {
+        DevBuf<double> work(n);
+        const double normb = [&] {
+            DevBuf<double> zero(n);
+            vec_fill<double>(zero.get(), 0.0, n, s_);
+            DevBuf<double> out(1), tmp(n);
+            LAUNCH(k_resid_norm, red_blocks(n), b, zero.get(), tmp.get(), (size_t)n, out.get(), red_.partials.get(),
+                   red_.counter.get());
+            double hh = 0.0;
+            out.download(&hh, 1, s_);
+            FEM_HIP(hipStreamSynchronize(s_));
+            return std::sqrt(hh);
+        }();
+        const double tr = true_residual(b, x, work.get());
+        S.true_relres = normb > 0 ? tr / normb : 0.0;
+        *stats = S;
+    }
+}
+
+}  // namespace fem

@@ -1,0 +1,69 @@
+// solver.hpp — Krylov solver object (krylov.hip).
+#pragma once
+#include "internal.hpp"
+
+namespace fem {
+
+template <typename T>
+struct KWork {
+    DevBuf<T> x, r, z, p, q;          // PCG (+ BiCGSTAB: q = v)
+    DevBuf<T> rt, y, s, t, ax;        // BiCGSTAB
+    DevBuf<T> V, w, wt, dz;           // GMRES basis (m+1)*n, work vectors
+    DevBuf<double> gm;                // GMRES small dense state
+    DevBuf<T> b;                      // right-hand side (internal copy)
+    uint32_t n = 0;
+    int m = 0;
+    void ensure(int solver, uint32_t n_, int m_);
+};
+
+struct GraphCache {
+    hipGraphExec_t exec = nullptr;
+    hipGraph_t graph = nullptr;
+    int tag = -1;
+    void reset();
+    ~GraphCache() { reset(); }
+};
+
+class KrylovSolver {
+   public:
+    KrylovSolver(LinOp* A, Precond* M, const fem_solve_opts& o, hipStream_t s);
+    ~KrylovSolver();
+    // b, x: device pointers (x is overwritten; every solve starts from x0 = 0)
+    void solve(const double* b, double* x, fem_solve_stats* st);
+    fem_solve_opts opts;
+
+   private:
+    template <typename T>
+    int run(KWork<T>& W, double tol, int max_iter, bool soft, long long* spmv);
+    template <typename T>
+    void pcg_init(KWork<T>& W);
+    template <typename T>
+    void pcg_iter(KWork<T>& W);
+    template <typename T>
+    void bi_init(KWork<T>& W);
+    template <typename T>
+    void bi_iter(KWork<T>& W);
+    template <typename T>
+    void gm_cycle(KWork<T>& W);
+    template <typename T>
+    void precond(const T* r, T* z, Gate g);
+    template <typename T>
+    void matvec(const T* x, T* y, Gate g);
+    void replay(GraphCache& gc, int tag, const std::function<void()>& body);
+    void read_state();
+    double true_residual(const double* b, const double* x, double* work);
+
+    LinOp* A_;
+    Precond* M_;
+    hipStream_t s_;
+    RedScratch red_;
+    DevBuf<KState> st_;
+    PinnedBuf<KState> hst_;
+    KWork<double> w64_;
+    KWork<float> w32_;
+    DevBuf<double> res64_, x64_, ax64_;
+    GraphCache g64_, g32_;
+    int pc_mode64_ = 0, pc_mode32_ = 0;  // 0 general, 1 none, 2 diag
+};
+
+}  // namespace fem

@@ -1,0 +1,108 @@
+// vec.hip — elementwise vector kernels, deterministic dot product, and the
+// library-wide device/bookkeeping helpers.
+#include <mutex>
+
+#include "internal.hpp"
+#include "reduce.hpp"
+
+namespace fem {
+
+static std::once_flag g_dev_once;
+static int g_dev = -1;
+
+void ensure_device() {
+    std::call_once(g_dev_once, [] {
+        int cnt = 0;
+        if (hipGetDeviceCount(&cnt) != hipSuccess || cnt <= 0) return;
+        (void)hipGetDevice(&g_dev);
+    });
+    if (g_dev < 0) fail(FEM_ERR_DEVICE, "no HIP device is available (libfemgpu has no CPU fallback)");
+    FEM_HIP(hipSetDevice(g_dev));
+}
+
+const char* solver_err_msg(int code) {
+    switch (code) {
+        case SE_PCG_RZ: return "PCG failed: r^T M^{-1} r <= 0, the preconditioner may not be SPD";
+        case SE_PCG_PAP_SMALL: return "PCG breakdown: p^T A p too small";
+        case SE_PCG_PAP_NEG: return "PCG failed: p^T A p <= 0, the matrix A may not be SPD";
+        case SE_BI_RHO: return "BiCGSTAB breakdown: rho_{k+1} = r0~^T r_k too small";
+        case SE_BI_RHO_OLD: return "BiCGSTAB breakdown: rho_k too small";
+        case SE_BI_OMEGA: return "BiCGSTAB breakdown: omega_k = 0";
+        case SE_BI_RV: return "BiCGSTAB breakdown: r0~^T v_{k+1} too small";
+        case SE_BI_TT: return "BiCGSTAB breakdown: t^T t too small";
+        case SE_GM_DEN: return "GMRES breakdown: Hessenberg pivot too small";
+        case SE_GM_TRI: return "GMRES breakdown: upper triangular pivot too small";
+        default: return "solver error";
+    }
+}
+
+void RedScratch::init(hipStream_t s) {
+    if (partials.get()) return;
+    partials.alloc((size_t)kMaxRedBlocks * kMaxRedVals);
+    counter.alloc(kSlots);
+    // one-time clear outside any graph; afterwards the last block of every
+    // reduction resets its counter
+    std::vector<unsigned> z(kSlots, 0u);
+    counter.upload(z, s);
+    FEM_HIP(hipStreamSynchronize(s));
+}
+
+template <typename T>
+__global__ void k_fill(T* y, T val, size_t n, Gate g) {
+    if (g.closed()) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) y[i] = val;
+}
+
+template <typename TD, typename TS>
+__global__ void k_convert(TD* y, const TS* x, double scale, size_t n, Gate g) {
+    if (g.closed()) return;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        y[i] = (TD)(scale == 1.0 ? (double)x[i] : (double)x[i] * scale);
+}
+
+static inline unsigned ew_blocks(size_t n) {
+    size_t b = (n + kBlock * 4 - 1) / (kBlock * 4);
+    if (b < 1) b = 1;
+    if (b > 8192) b = 8192;
+    return (unsigned)b;
+}
+
+template <typename T>
+void vec_fill(T* y, T val, size_t n, hipStream_t s, Gate g) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_fill<T>), dim3(ew_blocks(n)), dim3(kBlock), 0, s, y, val, n, g);
+    FEM_HIP(hipGetLastError());
+}
+template <typename TD, typename TS>
+void vec_convert(TD* y, const TS* x, double scale, size_t n, hipStream_t s, Gate g) {
+    if (!n) return;
+    hipLaunchKernelGGL((k_convert<TD, TS>), dim3(ew_blocks(n)), dim3(kBlock), 0, s, y, x, scale, n, g);
+    FEM_HIP(hipGetLastError());
+}
+template void vec_fill<double>(double*, double, size_t, hipStream_t, Gate);
+template void vec_fill<float>(float*, float, size_t, hipStream_t, Gate);
+template void vec_fill<uint32_t>(uint32_t*, uint32_t, size_t, hipStream_t, Gate);
+template void vec_convert<float, double>(float*, const double*, double, size_t, hipStream_t, Gate);
+template void vec_convert<double, float>(double*, const float*, double, size_t, hipStream_t, Gate);
+template void vec_convert<double, double>(double*, const double*, double, size_t, hipStream_t, Gate);
+template void vec_convert<float, float>(float*, const float*, double, size_t, hipStream_t, Gate);
+
+__global__ void k_dot(const double* a, const double* b, size_t n, double* partials, unsigned* counter, double* out) {
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        s += a[i] * b[i];
+    double v[1] = {s};
+    reduce_finalize<1>(v, partials, counter, [&](double (&t)[1]) { *out = t[0]; });
+}
+
+double dot_sync(const double* a, const double* b, size_t n, const Reducer& R, hipStream_t s) {
+    DevBuf<double> out(1);
+    hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(kBlock), 0, s, a, b, n, R.partials, R.counters, out.get());
+    FEM_HIP(hipGetLastError());
+    double h = 0.0;
+    out.download(&h, 1, s);
+    FEM_HIP(hipStreamSynchronize(s));
+    return h;
+}
+
+}  // namespace fem

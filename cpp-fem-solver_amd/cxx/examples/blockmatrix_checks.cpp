@@ -1,0 +1,149 @@
+// blockmatrix_checks.cpp — the reference's BlockMatrix checks
+// (test_BlockMatrix_mpi.cpp T1-T4, T6-T7; test_BlockMatrix_schur3d.cpp) run
+// through the drop-in block::BlockMatrix facade on the GPU. Exit status 0 only
+// if every threshold holds.
+#include <cmath>
+#include <cstdio>
+
+#include "math/array.hpp"
+#include "math/blockMatrix.hpp"
+
+using namespace sparse;
+using namespace block;
+
+constexpr int BS = 3;
+using Blk = BlockMatrix<double, BS>;
+static int failures = 0;
+
+static double drand(size_t i) {
+    double s = std::sin((double)i * 0.371 + 1.7) * 43758.5453;
+    return s - std::floor(s);
+}
+
+// node-major BS-component test matrix on an N x N grid (:29-59)
+static SMatrix make_test_scalar(uint32_t N) {
+    std::vector<double> v;
+    std::vector<uint32_t> r, c;
+    for (uint32_t i = 0; i < N; ++i)
+        for (uint32_t j = 0; j < N; ++j) {
+            const uint32_t idx = i * N + j;
+            for (int a = 0; a < BS; ++a) {
+                const uint32_t row = idx * BS + a;
+                for (int b = 0; b < BS; ++b) { r.push_back(row); c.push_back(idx * BS + b); v.push_back(a == b ? 4.0 + BS : 0.1); }
+                if (j > 0) { r.push_back(row); c.push_back((idx - 1) * BS + a); v.push_back(-1.0); }
+                if (j < N - 1) { r.push_back(row); c.push_back((idx + 1) * BS + a); v.push_back(-1.0); }
+                if (i > 0) { r.push_back(row); c.push_back((idx - N) * BS + a); v.push_back(-1.0); }
+                if (i < N - 1) { r.push_back(row); c.push_back((idx + N) * BS + a); v.push_back(-1.0); }
+            }
+        }
+    const uint32_t n = N * N * BS;
+    return SMatrix(n, n, v, r, c);
+}
+
+template <typename Mat, typename Pre>
+static std::pair<std::vector<double>, int> mini_cg(const Mat& M, const std::vector<double>& b, const Pre& solveM,
+                                                   double tol, int maxit) {
+    const size_t n = b.size();
+    std::vector<double> x(n, 0.0), r = b, z, p, Ap;
+    solveM(r, z);
+    p = z;
+    double rz = 0.0, nb = 0.0;
+    for (size_t i = 0; i < n; ++i) { rz += r[i] * z[i]; nb += b[i] * b[i]; }
+    nb = std::sqrt(nb);
+    int it = 0;
+    for (; it < maxit; ++it) {
+        M.matvec(p, Ap);
+        double pAp = 0.0;
+        for (size_t i = 0; i < n; ++i) pAp += p[i] * Ap[i];
+        const double alpha = rz / pAp;
+        double nr = 0.0;
+        for (size_t i = 0; i < n; ++i) { x[i] += alpha * p[i]; r[i] -= alpha * Ap[i]; nr += r[i] * r[i]; }
+        if (std::sqrt(nr) / nb < tol) break;
+        solveM(r, z);
+        double rz2 = 0.0;
+        for (size_t i = 0; i < n; ++i) rz2 += r[i] * z[i];
+        const double beta = rz2 / rz;
+        for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        rz = rz2;
+    }
+    return {x, it};
+}
+
+static void report(const char* t, uint32_t N, double v, double thr, bool extra = true) {
+    const bool ok = v < thr && extra;
+    std::printf("[%s] N=%4u value=%.3e threshold=%.0e %s\n", t, N, v, thr, ok ? "pass" : "FAIL");
+    if (!ok) ++failures;
+}
+
+int main() {
+    for (uint32_t N : {32u, 128u}) {
+        SMatrix S = make_test_scalar(N);
+        Blk B = Blk::condense(S, N * N, N * N);
+        std::vector<double> x(S.rows()), yb;
+        for (size_t i = 0; i < x.size(); ++i) x[i] = drand(i);
+        auto ys = SMatrix::spmv(S, x);
+        B.matvec(x, yb);
+        double e = 0;
+        for (size_t i = 0; i < ys.size(); ++i) e = std::max(e, std::fabs(ys[i] - yb[i]));
+        report("T1", N, e, 1e-12);
+        Blk Bt = B.transpose();
+        std::vector<double> u(S.rows()), v(S.rows()), Av, Atu;
+        for (size_t i = 0; i < u.size(); ++i) { u[i] = drand(i + 11); v[i] = drand(i + 29); }
+        B.matvec(v, Av);
+        Bt.matvec(u, Atu);
+        double l = 0, r = 0;
+        for (size_t i = 0; i < u.size(); ++i) { l += u[i] * Av[i]; r += Atu[i] * v[i]; }
+        report("T2", N, std::fabs(l - r) / std::fabs(l), 1e-12);
+        // block-Jacobi through BlockMatrix::gmres's "blockjacobi" and through CG
+        std::vector<double> bb(S.rows());
+        for (size_t i = 0; i < bb.size(); ++i) bb[i] = 1.0 + 0.5 * drand(i + 7);
+        auto ilu = make_block_ilu0_preconditioner(B);
+        auto ident = [](const std::vector<double>& a, std::vector<double>& z) { z = a; };
+        auto [xi, iti] = mini_cg(B, bb, ilu, 1e-8, 3000);
+        auto [xn, itn] = mini_cg(B, bb, ident, 1e-8, 3000);
+        std::vector<double> Ax;
+        B.matvec(xi, Ax);
+        double rr = 0, b2 = 0;
+        for (size_t i = 0; i < bb.size(); ++i) { rr += (bb[i] - Ax[i]) * (bb[i] - Ax[i]); b2 += bb[i] * bb[i]; }
+        report("T7", N, std::sqrt(rr / b2), 1e-6, iti <= itn);
+        std::printf("     CG+blockILU=%d steps, CG=%d steps\n", iti + 1, itn + 1);
+        for (const char* pc : {"blockjacobi", "blockilu0", "none"}) {
+            auto [xg, itg] = Blk::gmres(B, bb, pc, 1e-10, 2000, 30);
+            std::vector<double> Ag;
+            B.matvec(xg, Ag);
+            double r2 = 0;
+            for (size_t i = 0; i < bb.size(); ++i) r2 += (bb[i] - Ag[i]) * (bb[i] - Ag[i]);
+            std::printf("     gmres/%s: %d steps\n", pc, itg + 1);
+            report("GMRES", N, std::sqrt(r2 / b2), 1e-9);
+        }
+    }
+    // BS = 1 block SA-AMG on a scalar Laplacian (test_BlockMatrix_schur3d.cpp:143-146)
+    {
+        const uint32_t N = 24, n = N * N * N;
+        std::vector<double> v;
+        std::vector<uint32_t> r, c;
+        for (uint32_t k = 0; k < N; ++k)
+            for (uint32_t j = 0; j < N; ++j)
+                for (uint32_t i = 0; i < N; ++i) {
+                    const uint32_t row = i + j * N + k * N * N;
+                    r.push_back(row); c.push_back(row); v.push_back(6.0 + 1e-8);
+                    const uint32_t nb[6] = {((i + 1) % N) + j * N + k * N * N, ((i + N - 1) % N) + j * N + k * N * N,
+                                            i + ((j + 1) % N) * N + k * N * N, i + ((j + N - 1) % N) * N + k * N * N,
+                                            i + j * N + ((k + 1) % N) * N * N, i + j * N + ((k + N - 1) % N) * N * N};
+                    for (uint32_t q : nb) { r.push_back(row); c.push_back(q); v.push_back(-1.0); }
+                }
+        SMatrix L(n, n, v, r, c);
+        BlockMatrix<double, 1> Lb = BlockMatrix<double, 1>::condense(L, n, n);
+        auto amg = make_block_amg_preconditioner(Lb);
+        std::vector<double> xe(n), rhs;
+        for (uint32_t i = 0; i < n; ++i) xe[i] = drand(i);
+        rhs = SMatrix::spmv(L, xe);
+        auto [xs, its] = SMatrix::gmres(L, rhs, amg, 100);
+        double err = 0;
+        for (uint32_t i = 0; i < n; ++i) err += (xs[i] - xe[i]) * (xs[i] - xe[i]);
+        std::printf("     periodic Laplacian + block AMG (BS=1) GMRES: %d steps\n", its + 1);
+        report("AMG", N, std::sqrt(err), 1e-6);
+    }
+    std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
+    return failures ? 1 : 0;
+}

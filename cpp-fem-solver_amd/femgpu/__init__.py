@@ -1,0 +1,381 @@
+"""Python binding (ctypes) of libfemgpu, the MI355X sparse solve path.
+
+The binding calls the C ABI declared in include/femgpu.h. It never computes
+anything itself: when the shared library is missing, or no gfx950 device is
+present, every call raises (there is no CPU fallback).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfemgpu.so")
+
+# enums (femgpu.h)
+PC_NONE, PC_DIAG, PC_IC0, PC_ILU0, PC_SSOR, PC_AMG, PC_SAAMG, PC_LEARNED, PC_BJACOBI, PC_BILU0, PC_BSAAMG = range(11)
+PCG, BICGSTAB, GMRES, MINICG = 0, 1, 2, 3
+FP64, FP32_IR = 0, 1
+SMOOTH_JACOBI, SMOOTH_SGS = 0, 1
+
+PC_KINDS = {"none": PC_NONE, "diag": PC_DIAG, "ic0": PC_IC0, "ilu0": PC_ILU0, "ssor": PC_SSOR, "amg": PC_AMG,
+            "saamg": PC_SAAMG, "learned": PC_LEARNED, "bjacobi": PC_BJACOBI, "bilu0": PC_BILU0, "bsaamg": PC_BSAAMG}
+SOLVERS = {"pcg": PCG, "bicgstab": BICGSTAB, "gmres": GMRES, "minicg": MINICG}
+
+
+class FemError(Exception):
+    code = -1
+
+
+class InvalidArgument(FemError, ValueError):
+    code = 1
+
+
+class OutOfRange(FemError, IndexError):
+    code = 2
+
+
+class SolverFailure(FemError, RuntimeError):
+    code = 3
+
+
+class DeviceError(FemError, RuntimeError):
+    code = 4
+
+
+_ERRS = {1: InvalidArgument, 2: OutOfRange, 3: SolverFailure, 4: DeviceError}
+
+
+class PcParams(C.Structure):
+    _fields_ = [("omega", C.c_double), ("theta", C.c_double), ("npre", C.c_int), ("npost", C.c_int),
+                ("max_levels", C.c_int), ("coarse_size", C.c_int), ("max_agg", C.c_int), ("smoother", C.c_int),
+                ("p_omega", C.c_double), ("coarse_exact", C.c_int), ("fp32", C.c_int)]
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("solver", C.c_int), ("tol", C.c_double), ("max_iter", C.c_int), ("restart", C.c_int),
+                ("precision", C.c_int), ("inner_tol", C.c_double), ("max_refine", C.c_int),
+                ("check_every", C.c_int), ("device_ptrs", C.c_int), ("use_graph", C.c_int)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("inner_iterations", C.c_int), ("refinements", C.c_int),
+                ("converged", C.c_int), ("true_relres", C.c_double), ("setup_ms", C.c_double),
+                ("solve_ms", C.c_double), ("spmv_count", C.c_longlong)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+HOST_CB = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64, C.c_void_p)
+
+_lib = None
+
+P_u32 = C.POINTER(C.c_uint32)
+P_f64 = C.POINTER(C.c_double)
+P_vp = C.c_void_p
+
+_SIGS = {
+    "fem_last_error": (C.c_char_p, []),
+    "fem_device_info": (C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_int)]),
+    "fem_pc_params_default": (None, [C.c_int, C.POINTER(PcParams)]),
+    "fem_solve_opts_default": (None, [C.c_int, C.POINTER(SolveOpts)]),
+    "fem_csr_create": (C.c_int, [C.c_uint32, C.c_uint32, P_u32, P_u32, P_f64, C.POINTER(P_vp)]),
+    "fem_csr_create_coo": (C.c_int, [C.c_uint32, C.c_uint32, C.c_uint64, P_f64, P_u32, P_u32, C.POINTER(P_vp)]),
+    "fem_csr_info": (C.c_int, [P_vp, P_u32, P_u32, C.POINTER(C.c_uint64)]),
+    "fem_csr_export": (C.c_int, [P_vp, P_u32, P_u32, P_f64]),
+    "fem_csr_transpose": (C.c_int, [P_vp, C.POINTER(P_vp)]),
+    "fem_csr_multiply": (C.c_int, [P_vp, P_vp, C.POINTER(P_vp)]),
+    "fem_csr_destroy": (C.c_int, [P_vp]),
+    "fem_spmv": (C.c_int, [P_vp, P_f64, P_f64]),
+    "fem_spmv_device": (C.c_int, [P_vp, P_vp, P_vp, C.c_int]),
+    "fem_dot": (C.c_int, [C.c_uint64, P_f64, P_f64, P_f64]),
+    "fem_precond_create": (C.c_int, [P_vp, C.c_int, C.POINTER(PcParams), C.POINTER(P_vp)]),
+    "fem_precond_apply": (C.c_int, [P_vp, P_f64, P_f64]),
+    "fem_precond_destroy": (C.c_int, [P_vp]),
+    "fem_precond_info": (C.c_int, [P_vp, C.POINTER(C.c_int), P_u32, C.POINTER(C.c_uint64), C.c_int, P_f64]),
+    "fem_solve": (C.c_int, [P_vp, P_vp, P_f64, P_f64, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "fem_pcg": (C.c_int, [P_vp, P_vp, P_f64, C.c_double, C.c_int, P_f64, C.POINTER(C.c_int)]),
+    "fem_bicgstab": (C.c_int, [P_vp, P_vp, P_f64, C.c_double, C.c_int, P_f64, C.POINTER(C.c_int)]),
+    "fem_gmres": (C.c_int, [P_vp, P_vp, P_f64, C.c_int, C.c_double, C.c_int, P_f64, C.POINTER(C.c_int)]),
+    "fem_gmres_callback": (C.c_int, [P_vp, HOST_CB, P_vp, P_f64, C.c_int, C.c_double, C.c_int, P_f64,
+                                     C.POINTER(C.c_int)]),
+    "fem_solver_create": (C.c_int, [P_vp, P_vp, C.POINTER(SolveOpts), C.POINTER(P_vp)]),
+    "fem_solver_solve": (C.c_int, [P_vp, P_f64, P_f64, C.POINTER(SolveStats)]),
+    "fem_solver_destroy": (C.c_int, [P_vp]),
+    "fem_bsr_create": (C.c_int, [C.c_int, C.c_uint32, C.c_uint32, P_u32, P_u32, P_f64, C.POINTER(P_vp)]),
+    "fem_bsr_condense": (C.c_int, [P_vp, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(P_vp)]),
+    "fem_bsr_info": (C.c_int, [P_vp, C.POINTER(C.c_int), P_u32, P_u32, C.POINTER(C.c_uint64)]),
+    "fem_bsr_export": (C.c_int, [P_vp, P_u32, P_u32, P_f64]),
+    "fem_bsr_transpose": (C.c_int, [P_vp, C.POINTER(P_vp)]),
+    "fem_bsr_matvec": (C.c_int, [P_vp, P_f64, P_f64]),
+    "fem_bsr_matvec_device": (C.c_int, [P_vp, P_vp, P_vp, C.c_int]),
+    "fem_bsr_precond_create": (C.c_int, [P_vp, C.c_int, C.POINTER(PcParams), C.POINTER(P_vp)]),
+    "fem_bsr_solve": (C.c_int, [P_vp, P_vp, P_f64, P_f64, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    "fem_bsr_destroy": (C.c_int, [P_vp]),
+    "fem_bench_spmv": (C.c_int, [P_vp, C.c_int, C.c_int, P_f64]),
+}
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/femgpu.h."""
+    import re
+    hdr = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "femgpu.h")
+    txt = open(hdr).read()
+    return sorted(set(re.findall(r"\b(fem_[a-z0-9_]+)\s*\(", txt)))
+
+
+def lib():
+    """Load libfemgpu.so (built by __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DeviceError(f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    L = C.CDLL(LIB_PATH)
+    for name, (res, args) in _SIGS.items():
+        f = getattr(L, name, None)
+        if f is None:
+            continue
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        msg = lib().fem_last_error().decode("utf-8", "replace")
+        raise _ERRS.get(rc, FemError)(msg)
+
+
+def _u32(a):
+    return np.ascontiguousarray(a, dtype=np.uint32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def device_info() -> tuple[str, int]:
+    buf = C.create_string_buffer(256)
+    cu = C.c_int(0)
+    _check(lib().fem_device_info(buf, 256, C.byref(cu)))
+    return buf.value.decode(), cu.value
+
+
+class CsrMatrix:
+    """Device CSR matrix handle (sparse::SMatrix)."""
+
+    def __init__(self, rows, cols, row_ptr, col_idx, values, _handle=None):
+        self._h = C.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            rp, ci, v = _u32(row_ptr), _u32(col_idx), _f64(values)
+            _check(lib().fem_csr_create(rows, cols, _p(rp, C.c_uint32), _p(ci, C.c_uint32), _p(v, C.c_double),
+                                        C.byref(self._h)))
+        r, c, z = C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(lib().fem_csr_info(self._h, C.byref(r), C.byref(c), C.byref(z)))
+        self.rows, self.cols, self.nnz = r.value, c.value, z.value
+
+    @classmethod
+    def from_gen(cls, A):
+        return cls(A.n, A.m, A.row_ptr, A.col_idx, A.values)
+
+    @classmethod
+    def from_coo(cls, rows, cols, values, rowlist, collist):
+        v, r, c = _f64(values), _u32(rowlist), _u32(collist)
+        h = C.c_void_p()
+        _check(lib().fem_csr_create_coo(rows, cols, v.size, _p(v, C.c_double), _p(r, C.c_uint32),
+                                        _p(c, C.c_uint32), C.byref(h)))
+        return cls(rows, cols, None, None, None, _handle=h)
+
+    def export(self):
+        rp = np.empty(self.rows + 1, np.uint32)
+        ci = np.empty(self.nnz, np.uint32)
+        v = np.empty(self.nnz, np.float64)
+        _check(lib().fem_csr_export(self._h, _p(rp, C.c_uint32), _p(ci, C.c_uint32), _p(v, C.c_double)))
+        return rp, ci, v
+
+    def transpose(self):
+        h = C.c_void_p()
+        _check(lib().fem_csr_transpose(self._h, C.byref(h)))
+        return CsrMatrix(0, 0, None, None, None, _handle=h)
+
+    def __matmul__(self, other):
+        if isinstance(other, CsrMatrix):
+            h = C.c_void_p()
+            _check(lib().fem_csr_multiply(self._h, other._h, C.byref(h)))
+            return CsrMatrix(0, 0, None, None, None, _handle=h)
+        return self.spmv(other)
+
+    def spmv(self, x):
+        x = _f64(x)
+        if x.size != self.cols:
+            raise InvalidArgument("spmv: vector length differs from the matrix columns")
+        y = np.empty(self.rows, np.float64)
+        _check(lib().fem_spmv(self._h, _p(x, C.c_double), _p(y, C.c_double)))
+        return y
+
+    def bench_spmv(self, fp32=False, reps=100) -> float:
+        ms = C.c_double()
+        _check(lib().fem_bench_spmv(self._h, int(fp32), reps, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self._h:
+            lib().fem_csr_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pc_params(kind: int, **kw) -> PcParams:
+    p = PcParams()
+    lib().fem_pc_params_default(kind, C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise InvalidArgument(f"unknown preconditioner parameter {k}")
+        setattr(p, k, v)
+    return p
+
+
+class Preconditioner:
+    def __init__(self, A, kind, **params):
+        self.kind = PC_KINDS[kind] if isinstance(kind, str) else kind
+        self._h = C.c_void_p()
+        self.n = A.rows if isinstance(A, CsrMatrix) else A.nrows
+        p = pc_params(self.kind, **params)
+        if isinstance(A, CsrMatrix):
+            _check(lib().fem_precond_create(A._h, self.kind, C.byref(p), C.byref(self._h)))
+        else:
+            _check(lib().fem_bsr_precond_create(A._h, self.kind, C.byref(p), C.byref(self._h)))
+
+    def apply(self, r):
+        r = _f64(r)
+        z = np.empty(self.n, np.float64)
+        _check(lib().fem_precond_apply(self._h, _p(r, C.c_double), _p(z, C.c_double)))
+        return z
+
+    def info(self, cap=64):
+        lv = C.c_int()
+        rows = np.zeros(cap, np.uint32)
+        nnz = np.zeros(cap, np.uint64)
+        ms = C.c_double()
+        _check(lib().fem_precond_info(self._h, C.byref(lv), _p(rows, C.c_uint32),
+                                      nnz.ctypes.data_as(C.POINTER(C.c_uint64)), cap, C.byref(ms)))
+        return {"levels": lv.value, "rows": rows, "nnz": nnz, "setup_ms": ms.value}
+
+    def close(self):
+        if self._h:
+            lib().fem_precond_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_opts(solver="pcg", tol=1e-10, max_iter=2000, restart=30, precision=FP64, inner_tol=None,
+               max_refine=None, check_every=0, use_graph=True, device_ptrs=False) -> SolveOpts:
+    o = SolveOpts()
+    sk = SOLVERS[solver] if isinstance(solver, str) else solver
+    lib().fem_solve_opts_default(sk, C.byref(o))
+    o.tol, o.max_iter, o.restart, o.precision = tol, max_iter, restart, precision
+    if inner_tol is not None:
+        o.inner_tol = inner_tol
+    if max_refine is not None:
+        o.max_refine = max_refine
+    o.check_every = check_every
+    o.use_graph = int(use_graph)
+    o.device_ptrs = int(device_ptrs)
+    return o
+
+
+def solve(A, M, b, **kw):
+    """Solve A x = b; returns (x, stats dict)."""
+    b = _f64(b)
+    x = np.empty_like(b)
+    o = solve_opts(**kw)
+    st = SolveStats()
+    mh = M._h if M is not None else None
+    if isinstance(A, CsrMatrix):
+        _check(lib().fem_solve(A._h, mh, _p(b, C.c_double), _p(x, C.c_double), C.byref(o), C.byref(st)))
+    else:
+        _check(lib().fem_bsr_solve(A._h, mh, _p(b, C.c_double), _p(x, C.c_double), C.byref(o), C.byref(st)))
+    return x, st.as_dict()
+
+
+class Solver:
+    """Persistent solver (workspace and captured graphs reused)."""
+
+    def __init__(self, A, M, **kw):
+        self.A, self.M = A, M
+        self.n = A.rows
+        self._h = C.c_void_p()
+        o = solve_opts(**kw)
+        _check(lib().fem_solver_create(A._h, M._h if M is not None else None, C.byref(o), C.byref(self._h)))
+
+    def solve(self, b):
+        b = _f64(b)
+        x = np.empty_like(b)
+        st = SolveStats()
+        _check(lib().fem_solver_solve(self._h, _p(b, C.c_double), _p(x, C.c_double), C.byref(st)))
+        return x, st.as_dict()
+
+    def solve_device(self, b_ptr: int, x_ptr: int):
+        st = SolveStats()
+        _check(lib().fem_solver_solve(self._h, C.cast(b_ptr, P_f64), C.cast(x_ptr, P_f64), C.byref(st)))
+        return st.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().fem_solver_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def gmres_callback(A, fn, b, restart=30, tol=1e-10, max_iter=2000):
+    """GMRES with a host-side preconditioner fn(r) -> z (SMatrix::gmres with
+    a std::function preconditioner)."""
+    n = A.rows
+
+    def cb(r, z, nn, user):
+        rr = np.ctypeslib.as_array(r, shape=(nn,))
+        zz = np.ctypeslib.as_array(z, shape=(nn,))
+        zz[:] = fn(rr.copy())
+
+    cfn = HOST_CB(cb)
+    b = _f64(b)
+    x = np.empty(n)
+    it = C.c_int()
+    _check(lib().fem_gmres_callback(A._h, cfn, None, _p(b, C.c_double), restart, tol, max_iter,
+                                    _p(x, C.c_double), C.byref(it)))
+    return x, it.value
+
+
+def dot(a, b) -> float:
+    a, b = _f64(a), _f64(b)
+    if a.size != b.size:
+        raise InvalidArgument("dot: vector lengths differ")
+    out = C.c_double()
+    _check(lib().fem_dot(a.size, _p(a, C.c_double), _p(b, C.c_double), C.byref(out)))
+    return out.value

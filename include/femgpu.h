@@ -1,0 +1,230 @@
+/*
+ * femgpu.h — C ABI of the MI355X (gfx950) sparse solve path.
+ *
+ * This is the drop-in boundary for the sparse solve path of the C++17
+ * library cpp-FEM-solver. The reference exposes it as the C++ classes
+ * sparse::SMatrix (ode-solver/include/math/array.hpp:264-373) and
+ * block::BlockMatrix (ode-solver/include/math/blockMatrix.hpp:121-451); the
+ * C++ facade shipped with this library (cxx/include/math/array.hpp and
+ * cxx/include/math/blockMatrix.hpp) keeps those declarations and calls the
+ * functions below. Every entry point says which reference interface it
+ * replaces.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only. Unless a name ends in _device, vector
+ *    arguments are host pointers and the call is synchronous, like the
+ *    reference's std::vector calls.
+ *  - Indices are uint32 and values float64, as in SMatrix (array.hpp:269-273).
+ *  - Every call returns a status code; fem_last_error() gives the message of
+ *    the last failure on the calling thread. The codes map onto the
+ *    reference's exception types (array.cpp): FEM_ERR_INVALID_ARGUMENT ->
+ *    std::invalid_argument, FEM_ERR_OUT_OF_RANGE -> std::out_of_range,
+ *    FEM_ERR_RUNTIME -> std::runtime_error. FEM_ERR_DEVICE means no usable
+ *    gfx950 device or a HIP failure; there is no CPU fallback.
+ *  - Non-convergence is not an error (array.cpp:3597, 3841, 4400): solvers
+ *    return FEM_OK with the reference's iteration semantics.
+ *  - A handle owns one HIP stream; calls on one handle are not reentrant
+ *    (the reference's preconditioner closures are not either).
+ */
+#ifndef FEMGPU_H
+#define FEMGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fem_status {
+    FEM_OK = 0,
+    FEM_ERR_INVALID_ARGUMENT = 1,
+    FEM_ERR_OUT_OF_RANGE = 2,
+    FEM_ERR_RUNTIME = 3,
+    FEM_ERR_DEVICE = 4
+};
+
+typedef struct fem_csr_s* fem_csr;
+typedef struct fem_bsr_s* fem_bsr;
+typedef struct fem_pc_s* fem_pc;
+typedef struct fem_solver_s* fem_solver;
+
+/* Preconditioner kinds. The reference's string names map onto these through
+ * per-solver alias tables (array.cpp:3484-3511, 3641-3669, 4215-4238). */
+enum fem_pc_kind {
+    FEM_PC_NONE = 0,    /* "none"/"identity"/"noprecond"(/"cg")            */
+    FEM_PC_DIAG = 1,    /* "diag"/"diagonal"/"jacobi"   array.cpp:2450     */
+    FEM_PC_IC0 = 2,     /* "ic"/"ic0"/"ichol"/...        array.cpp:2509     */
+    FEM_PC_ILU0 = 3,    /* "ilu"/"ilu0"                   array.cpp:2733     */
+    FEM_PC_SSOR = 4,    /* "sor"/"ssor"                   array.cpp:3068     */
+    FEM_PC_AMG = 5,     /* "amg" (classical C/F)          array.cpp:3390     */
+    FEM_PC_SAAMG = 6,   /* smoothed aggregation on a BS=1 condensation,
+                           blockMatrix.hpp:1051                              */
+    FEM_PC_LEARNED = 7, /* learned sparse approximate inverse (no reference
+                           counterpart; see fem_spai_*)                      */
+    FEM_PC_BJACOBI = 8, /* block-Jacobi, blockMatrix.hpp:714-735 and
+                           test_BlockMatrix_mpi.cpp:132-155                   */
+    FEM_PC_BILU0 = 9,   /* block ILU(0), blockMatrix.hpp:454-623             */
+    FEM_PC_BSAAMG = 10  /* block SA-AMG on a BSR matrix, blockMatrix.hpp:1051 */
+};
+
+enum fem_solver_kind {
+    FEM_SOLVER_PCG = 0,      /* SMatrix::pcg       array.cpp:3445-3601 */
+    FEM_SOLVER_BICGSTAB = 1, /* SMatrix::bicgstab  array.cpp:3602-3842 */
+    FEM_SOLVER_GMRES = 2,    /* SMatrix::gmres     array.cpp:4190-4401 */
+    FEM_SOLVER_MINICG = 3    /* mini_cg of test_BlockMatrix_mpi.cpp:157-182 */
+};
+
+enum fem_precision {
+    FEM_PREC_FP64 = 0,    /* every operation in float64 (the reference)       */
+    FEM_PREC_FP32_IR = 1  /* float32 Krylov + preconditioner inside float64
+                             iterative refinement (true residual, update and
+                             AMG coarsest solve in float64)                   */
+};
+
+enum fem_smoother {
+    FEM_SMOOTH_JACOBI = 0, /* damped (block-)Jacobi, array.cpp:3355-3369 */
+    FEM_SMOOTH_SGS = 1     /* symmetric multicolour Gauss-Seidel/SSOR      */
+};
+
+typedef struct fem_pc_params {
+    double omega;       /* SSOR relaxation (1.2, array.hpp:312); AMG smoother damping */
+    double theta;       /* AMG strength threshold (0.25) */
+    int npre, npost;    /* AMG pre/post smoothing sweeps */
+    int max_levels;     /* AMG level cap (25 C/F, 20 SA) */
+    int coarse_size;    /* AMG coarsest size (300 rows C/F, 50 block rows SA) */
+    int max_agg;        /* SA aggregate size cap (4) */
+    int smoother;       /* enum fem_smoother */
+    double p_omega;     /* SA prolongator smoothing weight (2/3) */
+    int coarse_exact;   /* SA: 1 = pivoted LU coarse solve; 0 = the reference's
+                           elimination sequence (blockMatrix.hpp:958-999) */
+    int fp32;           /* also build a float32 copy for FEM_PREC_FP32_IR */
+} fem_pc_params;
+
+typedef struct fem_solve_opts {
+    int solver;        /* enum fem_solver_kind */
+    double tol;        /* relative residual tolerance (reference default 1e-10) */
+    int max_iter;      /* iteration cap (reference default 2000) */
+    int restart;       /* GMRES restart length */
+    int precision;     /* enum fem_precision */
+    double inner_tol;  /* FP32_IR: relative reduction asked of each inner solve */
+    int max_refine;    /* FP32_IR: refinement step cap */
+    int check_every;   /* iterations captured per hipGraph replay (0 = auto) */
+    int device_ptrs;   /* 1: b and x are device pointers on this device */
+    int use_graph;     /* 1: replay captured hipGraphs (default), 0: plain launches */
+} fem_solve_opts;
+
+typedef struct fem_solve_stats {
+    int iterations;        /* reference iteration semantics (array.cpp:3597 etc.) */
+    int inner_iterations;  /* Krylov iterations actually run */
+    int refinements;       /* FP32_IR outer steps */
+    int converged;         /* 1 if the stopping test passed */
+    double true_relres;    /* ||b - A x||_2 / ||b||_2 in float64 at exit */
+    double setup_ms;       /* preconditioner setup time of this call (0 if prebuilt) */
+    double solve_ms;       /* wall time of the iterations */
+    long long spmv_count;  /* operator applications */
+} fem_solve_stats;
+
+/* ---- library ---------------------------------------------------------- */
+const char* fem_last_error(void);
+/* Name and compute-unit count of the current device; FEM_ERR_DEVICE if none. */
+int fem_device_info(char* name, int name_len, int* cu_count);
+void fem_pc_params_default(int kind, fem_pc_params* p);
+void fem_solve_opts_default(int solver, fem_solve_opts* o);
+
+/* ---- CSR matrix (sparse::SMatrix) -------------------------------------- */
+/* CSR constructor without checks: SMatrix(rows, cols, row_ptr, col_idx,
+ * values), array.cpp:2183-2189. Arrays are copied to the device. */
+int fem_csr_create(uint32_t rows, uint32_t cols, const uint32_t* row_ptr, const uint32_t* col_idx,
+                   const double* values, fem_csr* out);
+/* COO constructor: SMatrix(rows, cols, values, rowlist, collist),
+ * array.cpp:2072-2182 (validate, drop zeros, sort, merge duplicates by sum,
+ * drop merged zeros), done on the device. */
+int fem_csr_create_coo(uint32_t rows, uint32_t cols, uint64_t nnz, const double* values,
+                       const uint32_t* rowlist, const uint32_t* collist, fem_csr* out);
+/* rows()/cols()/nnz(), array.hpp:296-298 */
+int fem_csr_info(fem_csr A, uint32_t* rows, uint32_t* cols, uint64_t* nnz);
+/* row_ptr()/col_idx()/values(), array.hpp:300-302 (copies to host) */
+int fem_csr_export(fem_csr A, uint32_t* row_ptr, uint32_t* col_idx, double* values);
+/* SMatrix::transpose, array.cpp:2202-2235 */
+int fem_csr_transpose(fem_csr A, fem_csr* out);
+/* SMatrix::operator*, array.cpp:2336-2447 (mathematically exact product) */
+int fem_csr_multiply(fem_csr A, fem_csr B, fem_csr* out);
+int fem_csr_destroy(fem_csr A);
+
+/* ---- primitives --------------------------------------------------------- */
+/* SMatrix::spmv, array.cpp:3418-3442 */
+int fem_spmv(fem_csr A, const double* x, double* y);
+/* Same on device buffers of this device, enqueued on A's stream; fp32 != 0
+ * uses the float32 copy of A with float32 x and y. */
+int fem_spmv_device(fem_csr A, const void* x, void* y, int fp32);
+/* SMatrix::dot, array.cpp:3401-3416 */
+int fem_dot(uint64_t n, const double* a, const double* b, double* out);
+
+/* ---- preconditioners (SMatrix::make_*_preconditioner, array.hpp:309-313) -- */
+int fem_precond_create(fem_csr A, int kind, const fem_pc_params* params, fem_pc* out);
+/* z = M^{-1} r: the std::function the reference factories return */
+int fem_precond_apply(fem_pc M, const double* r, double* z);
+int fem_precond_destroy(fem_pc M);
+/* Setup time and summary of a preconditioner: levels = triangular-solve levels
+ * (IC0/ILU0/block ILU0) or AMG levels; fills up to cap entries of rows/nnz
+ * per AMG level. */
+int fem_precond_info(fem_pc M, int* levels, uint32_t* rows, uint64_t* nnz, int cap, double* setup_ms);
+
+/* ---- Krylov solvers ----------------------------------------------------- */
+/* General entry: solver, precision, graph capture and pointer space from opts.
+ * M may be NULL (no preconditioner). */
+int fem_solve(fem_csr A, fem_pc M, const double* b, double* x, const fem_solve_opts* opts,
+              fem_solve_stats* stats);
+/* SMatrix::pcg(A, b, precond, tol, max_iter), array.cpp:3445 */
+int fem_pcg(fem_csr A, fem_pc M, const double* b, double tol, int max_iter, double* x, int* iterations);
+/* SMatrix::bicgstab, array.cpp:3602 */
+int fem_bicgstab(fem_csr A, fem_pc M, const double* b, double tol, int max_iter, double* x, int* iterations);
+/* SMatrix::gmres(A, b, precond, restart, tol, max_iter), array.cpp:4190 */
+int fem_gmres(fem_csr A, fem_pc M, const double* b, int restart, double tol, int max_iter, double* x,
+              int* iterations);
+/* SMatrix::gmres(A, b, std::function solveM, ...), array.cpp:4402: the
+ * preconditioner is a host callback, called with host vectors of length n. */
+typedef void (*fem_host_precond_fn)(const double* r, double* z, uint64_t n, void* user);
+int fem_gmres_callback(fem_csr A, fem_host_precond_fn fn, void* user, const double* b, int restart, double tol,
+                       int max_iter, double* x, int* iterations);
+
+/* Persistent solver: keeps workspace and captured graphs across solves. */
+int fem_solver_create(fem_csr A, fem_pc M, const fem_solve_opts* opts, fem_solver* out);
+int fem_solver_solve(fem_solver S, const double* b, double* x, fem_solve_stats* stats);
+int fem_solver_destroy(fem_solver S);
+
+/* ---- block CSR (block::BlockMatrix<double,BS>) -------------------------- */
+/* BlockMatrix(nbr, nbc, row_ptr, col_idx, values), blockMatrix.hpp:170-181;
+ * bs in {1,2,3,4}; values row-major bs*bs per block. */
+int fem_bsr_create(int bs, uint32_t nbr, uint32_t nbc, const uint32_t* row_ptr, const uint32_t* col_idx,
+                   const double* values, fem_bsr* out);
+/* BlockMatrix::condense(SMatrix, nbr, nbc), blockMatrix.hpp:269-315 */
+int fem_bsr_condense(fem_csr A, int bs, uint32_t nbr, uint32_t nbc, fem_bsr* out);
+int fem_bsr_info(fem_bsr A, int* bs, uint32_t* nbr, uint32_t* nbc, uint64_t* nnzb);
+int fem_bsr_export(fem_bsr A, uint32_t* row_ptr, uint32_t* col_idx, double* values);
+/* BlockMatrix::transpose, blockMatrix.hpp:237-267 */
+int fem_bsr_transpose(fem_bsr A, fem_bsr* out);
+/* BlockMatrix::matvec, blockMatrix.hpp:208-234 */
+int fem_bsr_matvec(fem_bsr A, const double* x, double* y);
+int fem_bsr_matvec_device(fem_bsr A, const void* x, void* y, int fp32);
+/* FEM_PC_BJACOBI / FEM_PC_BILU0 (make_block_ilu0_preconditioner,
+ * blockMatrix.hpp:455) / FEM_PC_BSAAMG (make_block_amg_preconditioner,
+ * blockMatrix.hpp:1051-1053) */
+int fem_bsr_precond_create(fem_bsr A, int kind, const fem_pc_params* params, fem_pc* out);
+/* PCG / mini_cg / GMRES on a block matrix; BlockMatrix::gmres(A, b, precond,
+ * tol, max_iter, restart), blockMatrix.hpp:444-451 when opts->solver is GMRES */
+int fem_bsr_solve(fem_bsr A, fem_pc M, const double* b, double* x, const fem_solve_opts* opts,
+                  fem_solve_stats* stats);
+int fem_bsr_destroy(fem_bsr A);
+
+/* ---- measurement --------------------------------------------------------- */
+/* Average device time of `reps` back-to-back SpMV launches on A's stream,
+ * timed with HIP events (inputs resident in HBM). */
+int fem_bench_spmv(fem_csr A, int fp32, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEMGPU_H */
