@@ -144,11 +144,19 @@ void fem_pc_params_default(int kind, fem_pc_params* p) {
     p->max_agg = 4;
     p->smoother = FEM_SMOOTH_JACOBI;
     p->p_omega = 2.0 / 3.0;
-    p->coarse_exact = 0;
+    // the reference's block-AMG coarsest solve skips the forward elimination
+    // of the right-hand side (blockMatrix.hpp:985-991); the exact pivoted LU
+    // solve is the default, coarse_exact = 0 reproduces the reference
+    p->coarse_exact = 1;
     p->fp32 = 0;
     if (kind == FEM_PC_AMG) p->omega = 2.0 / 3.0;
     if (kind == FEM_PC_SAAMG || kind == FEM_PC_BSAAMG) {
-        p->omega = 2.0 / 3.0;
+        // Smoother and prolongator weights 0 select 4/(3 rho(D^{-1}A)) per
+        // level: the reference's fixed 2/3 (blockMatrix.hpp:1059, 1143) makes
+        // the hierarchy diverge on 3-D Laplacians from about 20^3 and on
+        // variable coefficients. Set both to 2/3 for the reference's weights.
+        p->omega = 0.0;
+        p->p_omega = 0.0;
         p->max_levels = 20;
         p->coarse_size = 50;
     }

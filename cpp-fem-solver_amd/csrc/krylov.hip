@@ -34,10 +34,12 @@ __device__ __forceinline__ void kfail(KState* st, int code) {
 #define DONE_RET(st) if (*(volatile int*)&(st)->done) return
 
 // ---------------------------------------------------------------------------
-// PCG kernels
+// PCG kernels. Each reduction kernel writes per-block partials; a Fin*
+// functor, run by the one-block k_finalize kernel, evaluates the scalar
+// recurrence on the totals. Functors repeat the gate of their kernel.
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_pcg_init(const T* b, T* x, T* r, size_t n, KState* st, double* part, unsigned* cnt) {
+__global__ void k_pcg_init(const T* b, T* x, T* r, size_t n, KState* st, double* part) {
     DONE_RET(st);
     double s = 0.0;
     GRID_LOOP(i, n) {
@@ -47,7 +49,12 @@ __global__ void k_pcg_init(const T* b, T* x, T* r, size_t n, KState* st, double*
         s += (double)bi * (double)bi;
     }
     double v[1] = {s};
-    reduce_finalize<1>(v, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(v, part);
+}
+struct FinPcgInit {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
         st->normb = sqrt(t[0]);
         st->normr = st->normb;
         st->it = 0;
@@ -55,8 +62,8 @@ __global__ void k_pcg_init(const T* b, T* x, T* r, size_t n, KState* st, double*
         st->iters = 0;
         if (st->normb < 1e-300) { st->done = 1; return; }
         if (!st->minicg && st->normr / st->normb < st->tol) { st->done = 1; return; }
-    });
-}
+    }
+};
 
 template <typename T, int PCM>
 __device__ __forceinline__ T zval(const T* r, const T* z, const T* d, size_t i) {
@@ -67,8 +74,7 @@ __device__ __forceinline__ T zval(const T* r, const T* z, const T* d, size_t i) 
 
 // z (given or implied) -> p = z, rho = r^T z
 template <typename T, int PCM>
-__global__ void k_pcg_init_rz(const T* r, const T* z, const T* d, T* p, size_t n, KState* st, double* part,
-                              unsigned* cnt) {
+__global__ void k_pcg_init_rz(const T* r, const T* z, const T* d, T* p, size_t n, KState* st, double* part) {
     DONE_RET(st);
     double s = 0.0;
     GRID_LOOP(i, n) {
@@ -77,19 +83,29 @@ __global__ void k_pcg_init_rz(const T* r, const T* z, const T* d, T* p, size_t n
         s += (double)r[i] * (double)zi;
     }
     double v[1] = {s};
-    reduce_finalize<1>(v, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(v, part);
+}
+struct FinPcgInitRz {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
         if (t[0] <= 0.0 && !st->minicg) { kfail(st, SE_PCG_RZ); return; }
         st->rz = t[0];
-    });
-}
+    }
+};
 
 template <typename T>
-__global__ void k_pcg_pq(const T* p, const T* q, size_t n, KState* st, double* part, unsigned* cnt) {
+__global__ void k_dot2(const T* a, const T* b, size_t n, KState* st, double* part) {
     DONE_RET(st);
     double s = 0.0;
-    GRID_LOOP(i, n) s += (double)p[i] * (double)q[i];
+    GRID_LOOP(i, n) s += (double)a[i] * (double)b[i];
     double v[1] = {s};
-    reduce_finalize<1>(v, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(v, part);
+}
+struct FinPcgPq {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
         const double pap = t[0];
         st->pap = pap;
         st->iters += 1;
@@ -98,8 +114,8 @@ __global__ void k_pcg_pq(const T* p, const T* q, size_t n, KState* st, double* p
             if (pap <= 0.0) { kfail(st, SE_PCG_PAP_NEG); return; }
         }
         st->alpha = st->rz / pap;
-    });
-}
+    }
+};
 
 __device__ __forceinline__ void pcg_finish(KState* st, double rz_new) {
     if (rz_new <= 0.0 && !st->minicg) { kfail(st, SE_PCG_RZ); return; }
@@ -112,8 +128,7 @@ __device__ __forceinline__ void pcg_finish(KState* st, double rz_new) {
 
 // x += alpha p ; r -= alpha q ; ||r|| stopping test (and rho for none/diag)
 template <typename T, int PCM>
-__global__ void k_pcg_xr(T* x, T* r, const T* p, const T* q, const T* d, size_t n, KState* st, double* part,
-                         unsigned* cnt) {
+__global__ void k_pcg_xr(T* x, T* r, const T* p, const T* q, const T* d, size_t n, KState* st, double* part) {
     DONE_RET(st);
     const T a = (T)st->alpha;
     double rr = 0.0, rz = 0.0;
@@ -126,21 +141,26 @@ __global__ void k_pcg_xr(T* x, T* r, const T* p, const T* q, const T* d, size_t 
     }
     if constexpr (PCM == PCM_NONE) rz = rr;
     double v[2] = {rr, rz};
-    reduce_finalize<2>(v, part, cnt, [&](double (&t)[2]) {
+    block_partials<2>(v, part);
+}
+template <int PCM>
+struct FinPcgXr {
+    KState* st;
+    __device__ void operator()(double (&t)[2]) const {
+        if (st->done) return;
         st->normr = sqrt(t[0]);
         if (st->normr / st->normb < st->tol) { st->done = 1; return; }
         if constexpr (PCM != PCM_GEN) pcg_finish(st, t[1]);
-    });
-}
+    }
+};
 
-template <typename T>
-__global__ void k_pcg_rz(const T* r, const T* z, size_t n, KState* st, double* part, unsigned* cnt) {
-    DONE_RET(st);
-    double s = 0.0;
-    GRID_LOOP(i, n) s += (double)r[i] * (double)z[i];
-    double v[1] = {s};
-    reduce_finalize<1>(v, part, cnt, [&](double (&t)[1]) { pcg_finish(st, t[0]); });
-}
+struct FinPcgRz {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
+        pcg_finish(st, t[0]);
+    }
+};
 
 template <typename T, int PCM>
 __global__ void k_pcg_p(const T* r, const T* z, const T* d, T* p, size_t n, KState* st) {
@@ -155,8 +175,7 @@ __global__ void k_pcg_p(const T* r, const T* z, const T* d, T* p, size_t n, KSta
 constexpr double kBT = 1e-30;
 
 template <typename T>
-__global__ void k_bi_init(const T* b, T* x, T* r, T* rt, T* p, T* v, size_t n, KState* st, double* part,
-                          unsigned* cnt) {
+__global__ void k_bi_init(const T* b, T* x, T* r, T* rt, T* p, T* v, size_t n, KState* st, double* part) {
     DONE_RET(st);
     double s = 0.0;
     GRID_LOOP(i, n) {
@@ -169,7 +188,12 @@ __global__ void k_bi_init(const T* b, T* x, T* r, T* rt, T* p, T* v, size_t n, K
         s += (double)bi * (double)bi;
     }
     double w[1] = {s};
-    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(w, part);
+}
+struct FinBiInit {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
         st->normb = sqrt(t[0]);
         st->normr = st->normb;
         st->it = 0;
@@ -182,24 +206,21 @@ __global__ void k_bi_init(const T* b, T* x, T* r, T* rt, T* p, T* v, size_t n, K
         st->replace = 0;
         if (st->normb < 1e-300) { st->done = 1; return; }
         if (st->normr / st->normb < st->tol) { st->done = 1; return; }
-    });
-}
+    }
+};
 
-template <typename T>
-__global__ void k_bi_rho(const T* rt, const T* r, size_t n, KState* st, double* part, unsigned* cnt) {
-    DONE_RET(st);
-    double s = 0.0;
-    GRID_LOOP(i, n) s += (double)rt[i] * (double)r[i];
-    double w[1] = {s};
-    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+struct FinBiRho {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
         const double rho = t[0];
         st->rho = rho;
         if (fabs(rho) < kBT) { kfail(st, SE_BI_RHO); return; }
         if (fabs(st->rz) < kBT) { kfail(st, SE_BI_RHO_OLD); return; }
         if (fabs(st->omega) < kBT) { kfail(st, SE_BI_OMEGA); return; }
         st->beta = (rho / st->rz) * (st->alpha / st->omega);
-    });
-}
+    }
+};
 
 template <typename T>
 __global__ void k_bi_p(const T* r, const T* v, T* p, size_t n, KState* st) {
@@ -208,19 +229,16 @@ __global__ void k_bi_p(const T* r, const T* v, T* p, size_t n, KState* st) {
     GRID_LOOP(i, n) p[i] = r[i] + b * (p[i] - om * v[i]);
 }
 
-template <typename T>
-__global__ void k_bi_rv(const T* rt, const T* v, size_t n, KState* st, double* part, unsigned* cnt) {
-    DONE_RET(st);
-    double s = 0.0;
-    GRID_LOOP(i, n) s += (double)rt[i] * (double)v[i];
-    double w[1] = {s};
-    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+struct FinBiRv {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
         st->rv = t[0];
         st->iters += 1;
         if (fabs(t[0]) < kBT) { kfail(st, SE_BI_RV); return; }
         st->alpha = st->rho / t[0];
-    });
-}
+    }
+};
 
 template <typename T>
 __global__ void k_bi_s(const T* r, const T* v, T* sv, size_t n, KState* st) {
@@ -230,7 +248,7 @@ __global__ void k_bi_s(const T* r, const T* v, T* sv, size_t n, KState* st) {
 }
 
 template <typename T>
-__global__ void k_bi_tt(const T* t_, const T* sv, size_t n, KState* st, double* part, unsigned* cnt) {
+__global__ void k_bi_tt(const T* t_, const T* sv, size_t n, KState* st, double* part) {
     DONE_RET(st);
     double a = 0.0, b = 0.0;
     GRID_LOOP(i, n) {
@@ -239,17 +257,22 @@ __global__ void k_bi_tt(const T* t_, const T* sv, size_t n, KState* st, double* 
         b += ti * (double)sv[i];
     }
     double w[2] = {a, b};
-    reduce_finalize<2>(w, part, cnt, [&](double (&t)[2]) {
+    block_partials<2>(w, part);
+}
+struct FinBiTt {
+    KState* st;
+    __device__ void operator()(double (&t)[2]) const {
+        if (st->done) return;
         st->tt = t[0];
         st->ts = t[1];
         if (t[0] < kBT) { kfail(st, SE_BI_TT); return; }
         st->omega = t[1] / t[0];
-    });
-}
+    }
+};
 
 template <typename T>
 __global__ void k_bi_xr(T* x, T* r, const T* y, const T* z, const T* sv, const T* t_, size_t n, KState* st,
-                        double* part, unsigned* cnt) {
+                        double* part) {
     DONE_RET(st);
     const T a = (T)st->alpha, om = (T)st->omega;
     double rr = 0.0;
@@ -260,17 +283,22 @@ __global__ void k_bi_xr(T* x, T* r, const T* y, const T* z, const T* sv, const T
         rr += (double)ri * (double)ri;
     }
     double w[1] = {rr};
-    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(w, part);
+}
+struct FinBiXr {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
         st->normr = sqrt(t[0]);
         st->it_end = st->it;
         if (st->normr / st->normb < st->tol) st->check = 1;
         else { st->check = 0; st->rz = st->rho; }
-    });
-}
+    }
+};
 
 // true residual b - A x  (ax = A x computed before, gated on check)
 template <typename T>
-__global__ void k_bi_true(const T* b, const T* ax, size_t n, KState* st, double* part, unsigned* cnt) {
+__global__ void k_bi_true(const T* b, const T* ax, size_t n, KState* st, double* part) {
     DONE_RET(st);
     if (!st->check) return;
     double s = 0.0;
@@ -279,16 +307,20 @@ __global__ void k_bi_true(const T* b, const T* ax, size_t n, KState* st, double*
         s += d * d;
     }
     double w[1] = {s};
-    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(w, part);
+}
+struct FinBiTrue {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done || !st->check) return;
         st->true_normr = sqrt(t[0]);
         if (st->true_normr / st->normb < st->tol) { st->done = 1; st->replace = 0; }
         else { st->replace = 1; st->normr = st->true_normr; }
-    });
-}
+    }
+};
 
 template <typename T>
-__global__ void k_bi_replace(const T* b, const T* ax, const T* rt, T* r, size_t n, KState* st, double* part,
-                             unsigned* cnt) {
+__global__ void k_bi_replace(const T* b, const T* ax, const T* rt, T* r, size_t n, KState* st, double* part) {
     DONE_RET(st);
     if (!st->replace) return;
     double s = 0.0;
@@ -298,11 +330,16 @@ __global__ void k_bi_replace(const T* b, const T* ax, const T* rt, T* r, size_t 
         s += (double)rt[i] * (double)ri;
     }
     double w[1] = {s};
-    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(w, part);
+}
+struct FinBiReplace {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done || !st->replace) return;
         if (fabs(t[0]) < kBT) { st->done = 1; return; }
         st->rz = t[0];
-    });
-}
+    }
+};
 
 __global__ void k_bi_tail(KState* st) {
     DONE_RET(st);
@@ -326,7 +363,7 @@ struct GmLayout {
 };
 
 template <typename T>
-__global__ void k_gm_init(const T* b, T* x, size_t n, KState* st, double* part, unsigned* cnt) {
+__global__ void k_gm_init(const T* b, T* x, size_t n, KState* st, double* part) {
     DONE_RET(st);
     double s = 0.0;
     GRID_LOOP(i, n) {
@@ -334,14 +371,19 @@ __global__ void k_gm_init(const T* b, T* x, size_t n, KState* st, double* part, 
         s += (double)b[i] * (double)b[i];
     }
     double w[1] = {s};
-    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(w, part);
+}
+struct FinGmInit {
+    KState* st;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
         st->normb = sqrt(t[0]);
         st->it = 0;
         st->it_end = 0;
         st->iters = 0;
         if (st->normb < 1e-300) st->done = 1;
-    });
-}
+    }
+};
 
 __global__ void k_gm_cycle_begin(KState* st) {
     DONE_RET(st);
@@ -352,8 +394,7 @@ __global__ void k_gm_cycle_begin(KState* st) {
 }
 
 template <typename T>
-__global__ void k_gm_v0(const T* b, const T* ax, T* v0, size_t n, KState* st, double* gm, GmLayout L, double* part,
-                        unsigned* cnt) {
+__global__ void k_gm_v0(const T* b, const T* ax, T* v0, size_t n, KState* st, double* part) {
     DONE_RET(st);
     const int nt = st->need_true;
     double s = 0.0;
@@ -363,7 +404,15 @@ __global__ void k_gm_v0(const T* b, const T* ax, T* v0, size_t n, KState* st, do
         s += (double)ri * (double)ri;
     }
     double w[1] = {s};
-    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(w, part);
+}
+struct FinGmV0 {
+    KState* st;
+    double* gm;
+    GmLayout L;
+    __device__ void operator()(double (&t)[1]) const {
+        if (st->done) return;
+        const int nt = st->need_true;
         const double beta = nt ? sqrt(t[0]) : st->normb;
         st->beta_g = beta;
         if (nt && beta / st->normb < st->tol) { st->done = 1; return; }
@@ -372,8 +421,8 @@ __global__ void k_gm_v0(const T* b, const T* ax, T* v0, size_t n, KState* st, do
         g[0] = beta;
         double* H = L.H(gm);
         for (int i = 0; i < (L.m + 1) * L.m; ++i) H[i] = 0.0;
-    });
-}
+    }
+};
 
 template <typename T>
 __global__ void k_gm_scale(T* v0, size_t n, KState* st) {
@@ -386,21 +435,31 @@ __device__ __forceinline__ bool gm_closed(const KState* st) {
     return *(volatile const int*)&st->done || !*(volatile const int*)&st->live;
 }
 
-// h_{0j} = w . V_0
+// partial of w . V_0 (h_{0j})
 template <typename T>
-__global__ void k_gm_dot0(const T* w, const T* v0, size_t n, int j, KState* st, double* gm, GmLayout L,
-                          double* part, unsigned* cnt) {
+__global__ void k_gm_dot0(const T* w, const T* v0, size_t n, KState* st, double* part) {
     if (gm_closed(st)) return;
     double s = 0.0;
     GRID_LOOP(i, n) s += (double)w[i] * (double)v0[i];
     double a[1] = {s};
-    reduce_finalize<1>(a, part, cnt, [&](double (&t)[1]) { L.H(gm)[0 * L.m + j] = t[0]; });
+    block_partials<1>(a, part);
 }
+// store h_{ij}
+struct FinGmH {
+    KState* st;
+    double* gm;
+    GmLayout L;
+    int i_, j;
+    __device__ void operator()(double (&t)[1]) const {
+        if (gm_closed(st)) return;
+        L.H(gm)[i_ * L.m + j] = t[0];
+    }
+};
 
-// w -= h_{i-1,j} V_{i-1}; h_{ij} = w . V_i
+// w -= h_{i-1,j} V_{i-1}; partial of w . V_i
 template <typename T>
 __global__ void k_gm_axpy_dot(T* w, const T* vprev, const T* vi, size_t n, int i_, int j, KState* st, double* gm,
-                              GmLayout L, double* part, unsigned* cnt) {
+                              GmLayout L, double* part) {
     if (gm_closed(st)) return;
     const T h = (T)L.H(gm)[(i_ - 1) * L.m + j];
     double s = 0.0;
@@ -410,13 +469,12 @@ __global__ void k_gm_axpy_dot(T* w, const T* vprev, const T* vi, size_t n, int i
         s += (double)wi * (double)vi[i];
     }
     double a[1] = {s};
-    reduce_finalize<1>(a, part, cnt, [&](double (&t)[1]) { L.H(gm)[i_ * L.m + j] = t[0]; });
+    block_partials<1>(a, part);
 }
 
-// w -= h_{jj} V_j; h_{j+1,j} = ||w||; Givens rotations and stopping test
+// w -= h_{jj} V_j; partial of ||w||^2
 template <typename T>
-__global__ void k_gm_axpy_norm(T* w, const T* vj, size_t n, int j, KState* st, double* gm, GmLayout L,
-                               double* part, unsigned* cnt) {
+__global__ void k_gm_axpy_norm(T* w, const T* vj, size_t n, int j, KState* st, double* gm, GmLayout L, double* part) {
     if (gm_closed(st)) return;
     const T h = (T)L.H(gm)[j * L.m + j];
     double s = 0.0;
@@ -426,7 +484,16 @@ __global__ void k_gm_axpy_norm(T* w, const T* vj, size_t n, int j, KState* st, d
         s += (double)wi * (double)wi;
     }
     double a[1] = {s};
-    reduce_finalize<1>(a, part, cnt, [&](double (&t)[1]) {
+    block_partials<1>(a, part);
+}
+// h_{j+1,j} = ||w||, Givens rotations, stopping test
+struct FinGmNorm {
+    KState* st;
+    double* gm;
+    GmLayout L;
+    int j;
+    __device__ void operator()(double (&t)[1]) const {
+        if (gm_closed(st)) return;
         const int m = L.m;
         double* H = L.H(gm);
         double* cs = L.cs(gm);
@@ -457,8 +524,8 @@ __global__ void k_gm_axpy_norm(T* w, const T* vj, size_t n, int j, KState* st, d
         st->iters += 1;
         if (fabs(g[j + 1]) / st->normb < st->tol) { st->conv = 1; st->live = 0; }
         else if (st->it >= st->max_iter) st->live = 0;
-    });
-}
+    }
+};
 
 // V_{j+1} = w * (1 / h_{j+1,j}) when h_{j+1,j} > 1e-30. Skipped once the
 // stopping test fired: V_{j+1} is then never read.
@@ -512,9 +579,8 @@ __global__ void k_gm_cycle_end(KState* st) {
 // ---------------------------------------------------------------------------
 // Refinement kernels
 // ---------------------------------------------------------------------------
-// res = b - ax, partial ||res||^2 -> out[0]
-__global__ void k_resid_norm(const double* b, const double* ax, double* res, size_t n, double* out, double* part,
-                             unsigned* cnt) {
+// res = b - ax, partial ||res||^2
+__global__ void k_resid_norm(const double* b, const double* ax, double* res, size_t n, double* part) {
     double s = 0.0;
     GRID_LOOP(i, n) {
         const double r = b[i] - ax[i];
@@ -522,8 +588,12 @@ __global__ void k_resid_norm(const double* b, const double* ax, double* res, siz
         s += r * r;
     }
     double w[1] = {s};
-    reduce_finalize<1>(w, part, cnt, [&](double (&t)[1]) { out[0] = t[0]; });
+    block_partials<1>(w, part);
 }
+struct FinStore {
+    double* out;
+    __device__ void operator()(double (&t)[1]) const { out[0] = t[0]; }
+};
 
 __global__ void k_axpy_f32(double* x, const float* d, double a, size_t n) {
     GRID_LOOP(i, n) x[i] += a * (double)d[i];
@@ -590,6 +660,12 @@ void KrylovSolver::matvec(const T* x, T* y, Gate g) {
     else A_->apply32(x, y, s_, g);
 }
 
+#define FIN(nv, ...)                                          \
+    do {                                                      \
+        launch_finalize<nv>(part, G, __VA_ARGS__, s_);        \
+        FEM_HIP(hipGetLastError());                           \
+    } while (0)
+
 #define LAUNCH(kern, grid, ...)                                                  \
     do {                                                                         \
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kBlock), 0, s_, __VA_ARGS__);  \
@@ -602,22 +678,23 @@ void KrylovSolver::pcg_init(KWork<T>& W) {
     const unsigned G = red_blocks(n);
     KState* st = st_.get();
     double* part = red_.partials.get();
-    unsigned* cnt = red_.counter.get();
     const int pm = std::is_same<T, double>::value ? pc_mode64_ : pc_mode32_;
     const T* d = nullptr;
     if (pm == PCM_DIAG) {
         if constexpr (std::is_same<T, double>::value) d = M_->diag64();
         else d = M_->diag32();
     }
-    LAUNCH((k_pcg_init<T>), G, W.b.get(), W.x.get(), W.r.get(), n, st, part, cnt);
+    LAUNCH((k_pcg_init<T>), G, W.b.get(), W.x.get(), W.r.get(), n, st, part);
+    FIN(1, FinPcgInit{st});
     if (pm == PCM_GEN) {
         precond<T>(W.r.get(), W.z.get(), Gate{&st->done});
-        LAUNCH((k_pcg_init_rz<T, PCM_GEN>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st, part, cnt);
+        LAUNCH((k_pcg_init_rz<T, PCM_GEN>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st, part);
     } else if (pm == PCM_DIAG) {
-        LAUNCH((k_pcg_init_rz<T, PCM_DIAG>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st, part, cnt);
+        LAUNCH((k_pcg_init_rz<T, PCM_DIAG>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st, part);
     } else {
-        LAUNCH((k_pcg_init_rz<T, PCM_NONE>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st, part, cnt);
+        LAUNCH((k_pcg_init_rz<T, PCM_NONE>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st, part);
     }
+    FIN(1, FinPcgInitRz{st});
 }
 
 template <typename T>
@@ -626,7 +703,6 @@ void KrylovSolver::pcg_iter(KWork<T>& W) {
     const unsigned G = red_blocks(n);
     KState* st = st_.get();
     double* part = red_.partials.get();
-    unsigned* cnt = red_.counter.get();
     const int pm = std::is_same<T, double>::value ? pc_mode64_ : pc_mode32_;
     const T* d = nullptr;
     if (pm == PCM_DIAG) {
@@ -635,17 +711,22 @@ void KrylovSolver::pcg_iter(KWork<T>& W) {
     }
     Gate g{&st->done};
     matvec<T>(W.p.get(), W.q.get(), g);
-    LAUNCH((k_pcg_pq<T>), G, W.p.get(), W.q.get(), n, st, part, cnt);
+    LAUNCH((k_dot2<T>), G, W.p.get(), W.q.get(), n, st, part);
+    FIN(1, FinPcgPq{st});
     if (pm == PCM_GEN) {
-        LAUNCH((k_pcg_xr<T, PCM_GEN>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part, cnt);
+        LAUNCH((k_pcg_xr<T, PCM_GEN>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part);
+        FIN(2, FinPcgXr<PCM_GEN>{st});
         precond<T>(W.r.get(), W.z.get(), g);
-        LAUNCH((k_pcg_rz<T>), G, W.r.get(), W.z.get(), n, st, part, cnt);
+        LAUNCH((k_dot2<T>), G, W.r.get(), W.z.get(), n, st, part);
+        FIN(1, FinPcgRz{st});
         LAUNCH((k_pcg_p<T, PCM_GEN>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
     } else if (pm == PCM_DIAG) {
-        LAUNCH((k_pcg_xr<T, PCM_DIAG>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part, cnt);
+        LAUNCH((k_pcg_xr<T, PCM_DIAG>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part);
+        FIN(2, FinPcgXr<PCM_DIAG>{st});
         LAUNCH((k_pcg_p<T, PCM_DIAG>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
     } else {
-        LAUNCH((k_pcg_xr<T, PCM_NONE>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part, cnt);
+        LAUNCH((k_pcg_xr<T, PCM_NONE>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part);
+        FIN(2, FinPcgXr<PCM_NONE>{st});
         LAUNCH((k_pcg_p<T, PCM_NONE>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
     }
 }
@@ -653,8 +734,11 @@ void KrylovSolver::pcg_iter(KWork<T>& W) {
 template <typename T>
 void KrylovSolver::bi_init(KWork<T>& W) {
     const size_t n = W.n;
-    LAUNCH((k_bi_init<T>), red_blocks(n), W.b.get(), W.x.get(), W.r.get(), W.rt.get(), W.p.get(), W.q.get(), n,
-           st_.get(), red_.partials.get(), red_.counter.get());
+    const unsigned G = red_blocks(n);
+    double* part = red_.partials.get();
+    KState* st = st_.get();
+    LAUNCH((k_bi_init<T>), G, W.b.get(), W.x.get(), W.r.get(), W.rt.get(), W.p.get(), W.q.get(), n, st, part);
+    FIN(1, FinBiInit{st});
 }
 
 template <typename T>
@@ -663,26 +747,31 @@ void KrylovSolver::bi_iter(KWork<T>& W) {
     const unsigned G = red_blocks(n);
     KState* st = st_.get();
     double* part = red_.partials.get();
-    unsigned* cnt = red_.counter.get();
     Gate g{&st->done};
     T* v = W.q.get();
     // with the identity preconditioner y = p and z = s (aliases, no copies)
     const bool has_pc = M_ && M_->kind() != FEM_PC_NONE;
     T* y = has_pc ? W.y.get() : W.p.get();
     T* z = has_pc ? W.z.get() : W.s.get();
-    LAUNCH((k_bi_rho<T>), G, W.rt.get(), W.r.get(), n, st, part, cnt);
+    LAUNCH((k_dot2<T>), G, W.rt.get(), W.r.get(), n, st, part);
+    FIN(1, FinBiRho{st});
     LAUNCH((k_bi_p<T>), G, W.r.get(), v, W.p.get(), n, st);
     if (has_pc) precond<T>(W.p.get(), y, g);
     matvec<T>(y, v, g);
-    LAUNCH((k_bi_rv<T>), G, W.rt.get(), v, n, st, part, cnt);
+    LAUNCH((k_dot2<T>), G, W.rt.get(), v, n, st, part);
+    FIN(1, FinBiRv{st});
     LAUNCH((k_bi_s<T>), G, W.r.get(), v, W.s.get(), n, st);
     if (has_pc) precond<T>(W.s.get(), z, g);
     matvec<T>(z, W.t.get(), g);
-    LAUNCH((k_bi_tt<T>), G, W.t.get(), W.s.get(), n, st, part, cnt);
-    LAUNCH((k_bi_xr<T>), G, W.x.get(), W.r.get(), y, z, W.s.get(), W.t.get(), n, st, part, cnt);
+    LAUNCH((k_bi_tt<T>), G, W.t.get(), W.s.get(), n, st, part);
+    FIN(2, FinBiTt{st});
+    LAUNCH((k_bi_xr<T>), G, W.x.get(), W.r.get(), y, z, W.s.get(), W.t.get(), n, st, part);
+    FIN(1, FinBiXr{st});
     matvec<T>(W.x.get(), W.ax.get(), Gate{&st->done, &st->check});
-    LAUNCH((k_bi_true<T>), G, W.b.get(), W.ax.get(), n, st, part, cnt);
-    LAUNCH((k_bi_replace<T>), G, W.b.get(), W.ax.get(), W.rt.get(), W.r.get(), n, st, part, cnt);
+    LAUNCH((k_bi_true<T>), G, W.b.get(), W.ax.get(), n, st, part);
+    FIN(1, FinBiTrue{st});
+    LAUNCH((k_bi_replace<T>), G, W.b.get(), W.ax.get(), W.rt.get(), W.r.get(), n, st, part);
+    FIN(1, FinBiReplace{st});
     LAUNCH(k_bi_tail, 1, st);
 }
 
@@ -693,14 +782,14 @@ void KrylovSolver::gm_cycle(KWork<T>& W) {
     const int m = W.m;
     KState* st = st_.get();
     double* part = red_.partials.get();
-    unsigned* cnt = red_.counter.get();
     GmLayout L{m};
     double* gm = W.gm.get();
     T* V = W.V.get();
     auto Vi = [&](int i) { return V + (size_t)i * n; };
     LAUNCH(k_gm_cycle_begin, 1, st);
     matvec<T>(W.x.get(), W.ax.get(), Gate{&st->done, &st->need_true});
-    LAUNCH((k_gm_v0<T>), G, W.b.get(), W.ax.get(), Vi(0), n, st, gm, L, part, cnt);
+    LAUNCH((k_gm_v0<T>), G, W.b.get(), W.ax.get(), Vi(0), n, st, part);
+    FIN(1, (FinGmV0{st, gm, L}));
     LAUNCH((k_gm_scale<T>), G, Vi(0), n, st);
     const bool has_pc = M_ && M_->kind() != FEM_PC_NONE;
     for (int j = 0; j < m; ++j) {
@@ -711,10 +800,14 @@ void KrylovSolver::gm_cycle(KWork<T>& W) {
             src = W.wt.get();
         }
         matvec<T>(src, W.w.get(), g);
-        LAUNCH((k_gm_dot0<T>), G, W.w.get(), Vi(0), n, j, st, gm, L, part, cnt);
-        for (int i = 1; i <= j; ++i)
-            LAUNCH((k_gm_axpy_dot<T>), G, W.w.get(), Vi(i - 1), Vi(i), n, i, j, st, gm, L, part, cnt);
-        LAUNCH((k_gm_axpy_norm<T>), G, W.w.get(), Vi(j), n, j, st, gm, L, part, cnt);
+        LAUNCH((k_gm_dot0<T>), G, W.w.get(), Vi(0), n, st, part);
+        FIN(1, (FinGmH{st, gm, L, 0, j}));
+        for (int i = 1; i <= j; ++i) {
+            LAUNCH((k_gm_axpy_dot<T>), G, W.w.get(), Vi(i - 1), Vi(i), n, i, j, st, gm, L, part);
+            FIN(1, (FinGmH{st, gm, L, i, j}));
+        }
+        LAUNCH((k_gm_axpy_norm<T>), G, W.w.get(), Vi(j), n, j, st, gm, L, part);
+        FIN(1, (FinGmNorm{st, gm, L, j}));
         LAUNCH((k_gm_normalize<T>), G, W.w.get(), Vi(j + 1), n, st);
     }
     LAUNCH(k_gm_backsolve, 1, st, gm, L);
@@ -768,8 +861,12 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
     const int solver = opts.solver == FEM_SOLVER_MINICG ? FEM_SOLVER_PCG : opts.solver;
     if (solver == FEM_SOLVER_PCG) pcg_init<T>(W);
     else if (solver == FEM_SOLVER_BICGSTAB) bi_init<T>(W);
-    else LAUNCH((k_gm_init<T>), red_blocks(W.n), W.b.get(), W.x.get(), (size_t)W.n, st_.get(), red_.partials.get(),
-                red_.counter.get());
+    else {
+        const unsigned G = red_blocks(W.n);
+        double* part = red_.partials.get();
+        LAUNCH((k_gm_init<T>), G, W.b.get(), W.x.get(), (size_t)W.n, st_.get(), part);
+        FIN(1, FinGmInit{st_.get()});
+    }
     read_state();
     if (max_iter <= 0 && solver != FEM_SOLVER_GMRES) {
         // the reference's loop body never runs: x stays 0
@@ -793,7 +890,12 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
         if (++rounds > guard_max) fail(FEM_ERR_RUNTIME, "solver made no progress");
     }
     read_state();
-    if (hst_.p->done == 2) fail(FEM_ERR_RUNTIME, solver_err_msg(hst_.p->err));
+    if (hst_.p->done == 2) {
+        char ctx[128];
+        std::snprintf(ctx, sizeof ctx, " (iteration %d, ||r||/||b|| = %.3e)", hst_.p->it,
+                      hst_.p->normb > 0 ? hst_.p->normr / hst_.p->normb : 0.0);
+        fail(FEM_ERR_RUNTIME, std::string(solver_err_msg(hst_.p->err)) + ctx);
+    }
     if (spmv) *spmv += hst_.p->iters;
     return hst_.p->iters;
 }
@@ -802,7 +904,12 @@ double KrylovSolver::true_residual(const double* b, const double* x, double* wor
     const size_t n = A_->rows();
     A_->apply(x, work, s_);
     DevBuf<double> out(1);
-    LAUNCH(k_resid_norm, red_blocks(n), b, work, work, n, out.get(), red_.partials.get(), red_.counter.get());
+    {
+        const unsigned G = red_blocks(n);
+        double* part = red_.partials.get();
+        LAUNCH(k_resid_norm, G, b, work, work, n, part);
+        FIN(1, FinStore{out.get()});
+    }
     double h = 0.0;
     out.download(&h, 1, s_);
     FEM_HIP(hipStreamSynchronize(s_));
@@ -833,8 +940,12 @@ void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
         FEM_HIP(hipMemcpyAsync(res64_.get(), b, (size_t)n * 8, hipMemcpyDeviceToDevice, s_));
         // ||b||
         vec_fill<double>(ax64_.get(), 0.0, n, s_);
-        LAUNCH(k_resid_norm, red_blocks(n), b, ax64_.get(), res64_.get(), (size_t)n, nrm.get(), red_.partials.get(),
-               red_.counter.get());
+        {
+            const unsigned G = red_blocks(n);
+            double* part = red_.partials.get();
+            LAUNCH(k_resid_norm, G, b, ax64_.get(), res64_.get(), (size_t)n, part);
+            FIN(1, FinStore{nrm.get()});
+        }
         double h = 0.0;
         nrm.download(&h, 1, s_);
         FEM_HIP(hipStreamSynchronize(s_));
@@ -853,8 +964,12 @@ void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
             LAUNCH(k_axpy_f32, red_blocks(n), x64_.get(), w32_.x.get(), normr, (size_t)n);
             A_->apply(x64_.get(), ax64_.get(), s_);
             S.spmv_count += 1;
-            LAUNCH(k_resid_norm, red_blocks(n), b, ax64_.get(), res64_.get(), (size_t)n, nrm.get(),
-                   red_.partials.get(), red_.counter.get());
+            {
+                const unsigned G = red_blocks(n);
+                double* part = red_.partials.get();
+                LAUNCH(k_resid_norm, G, b, ax64_.get(), res64_.get(), (size_t)n, part);
+                FIN(1, FinStore{nrm.get()});
+            }
             nrm.download(&h, 1, s_);
             FEM_HIP(hipStreamSynchronize(s_));
             const double nr = std::sqrt(h);
@@ -880,8 +995,10 @@ void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
             DevBuf<double> zero(n);
             vec_fill<double>(zero.get(), 0.0, n, s_);
             DevBuf<double> out(1), tmp(n);
-            LAUNCH(k_resid_norm, red_blocks(n), b, zero.get(), tmp.get(), (size_t)n, out.get(), red_.partials.get(),
-                   red_.counter.get());
+            const unsigned G = red_blocks(n);
+            double* part = red_.partials.get();
+            LAUNCH(k_resid_norm, G, b, zero.get(), tmp.get(), (size_t)n, part);
+            FIN(1, FinStore{out.get()});
             double hh = 0.0;
             out.download(&hh, 1, s_);
             FEM_HIP(hipStreamSynchronize(s_));

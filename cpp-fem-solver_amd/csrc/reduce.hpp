@@ -43,38 +43,37 @@ __device__ __forceinline__ void block_sum(double (&v)[NV]) {
     __syncthreads();
 }
 
-// Each block contributes NV partial sums; the last block to arrive reduces
-// them in block order and calls fin(total) on thread 0. Returns true in the
-// finalizing block (all threads). `counter` must be zero on entry; the last
-// block resets it.
-template <int NV, class Fin>
-__device__ __forceinline__ void reduce_finalize(double (&v)[NV], double* partials, unsigned* counter, Fin&& fin) {
+// Stage 1 of a reduction: the block's NV sums go to partials[block * NV + i]
+// with plain stores. No device-scope fence is used: on gfx950 a fence at
+// agent scope writes back and invalidates the whole L2 (buffer_wbl2 /
+// buffer_inv), which stalls every other block that is streaming.
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* partials) {
     block_sum<NV>(v);
-    __shared__ int is_last;
     if (threadIdx.x == 0) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i)
-            __hip_atomic_store(&partials[blockIdx.x * NV + i], v[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __threadfence();
-        const unsigned prev = atomicAdd(counter, 1u);
-        is_last = (prev == gridDim.x - 1) ? 1 : 0;
+        for (int i = 0; i < NV; ++i) partials[blockIdx.x * NV + i] = v[i];
     }
-    __syncthreads();
-    if (!is_last) return;
-    __threadfence();
+}
+
+// Stage 2: one block adds the per-block partials in block order (the kernel
+// boundary makes stage 1's stores visible) and calls fin(total) on thread 0.
+template <int NV, class Fin>
+__global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ partials, unsigned nparts, Fin fin) {
     double s[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = 0.0;
-    for (unsigned b = threadIdx.x; b < gridDim.x; b += blockDim.x) {
+    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) {
 #pragma unroll
-        for (int i = 0; i < NV; ++i)
-            s[i] += __hip_atomic_load(&partials[b * NV + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int i = 0; i < NV; ++i) s[i] += partials[b * NV + i];
     }
     block_sum<NV>(s);
-    if (threadIdx.x == 0) {
-        *counter = 0u;
-        fin(s);
-    }
+    if (threadIdx.x == 0) fin(s);
+}
+
+template <int NV, class Fin>
+inline void launch_finalize(const double* partials, unsigned nparts, Fin fin, hipStream_t s) {
+    hipLaunchKernelGGL((k_finalize<NV, Fin>), dim3(1), dim3(kBlock), 0, s, partials, nparts, fin);
 }
 
 }  // namespace fem

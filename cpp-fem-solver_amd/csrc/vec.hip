@@ -87,17 +87,24 @@ template void vec_convert<double, float>(double*, const float*, double, size_t, 
 template void vec_convert<double, double>(double*, const double*, double, size_t, hipStream_t, Gate);
 template void vec_convert<float, float>(float*, const float*, double, size_t, hipStream_t, Gate);
 
-__global__ void k_dot(const double* a, const double* b, size_t n, double* partials, unsigned* counter, double* out) {
+__global__ void k_dot(const double* a, const double* b, size_t n, double* partials) {
     double s = 0.0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         s += a[i] * b[i];
     double v[1] = {s};
-    reduce_finalize<1>(v, partials, counter, [&](double (&t)[1]) { *out = t[0]; });
+    block_partials<1>(v, partials);
 }
+
+struct FinOut {
+    double* out;
+    __device__ void operator()(double (&t)[1]) const { *out = t[0]; }
+};
 
 double dot_sync(const double* a, const double* b, size_t n, const Reducer& R, hipStream_t s) {
     DevBuf<double> out(1);
-    hipLaunchKernelGGL(k_dot, dim3(red_blocks(n)), dim3(kBlock), 0, s, a, b, n, R.partials, R.counters, out.get());
+    const unsigned G = red_blocks(n);
+    hipLaunchKernelGGL(k_dot, dim3(G), dim3(kBlock), 0, s, a, b, n, R.partials);
+    launch_finalize<1>(R.partials, G, FinOut{out.get()}, s);
     FEM_HIP(hipGetLastError());
     double h = 0.0;
     out.download(&h, 1, s);

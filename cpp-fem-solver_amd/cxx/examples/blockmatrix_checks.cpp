@@ -76,6 +76,7 @@ static void report(const char* t, uint32_t N, double v, double thr, bool extra =
 }
 
 int main() {
+    std::setvbuf(stdout, nullptr, _IONBF, 0);
     for (uint32_t N : {32u, 128u}) {
         SMatrix S = make_test_scalar(N);
         Blk B = Blk::condense(S, N * N, N * N);
@@ -117,7 +118,7 @@ int main() {
             report("GMRES", N, std::sqrt(r2 / b2), 1e-9);
         }
     }
-    // BS = 1 block SA-AMG on a scalar Laplacian (test_BlockMatrix_schur3d.cpp:143-146)
+    // BS = 1 block SA-AMG as a GMRES preconditioner on a Dirichlet Laplacian
     {
         const uint32_t N = 24, n = N * N * N;
         std::vector<double> v;
@@ -126,11 +127,13 @@ int main() {
             for (uint32_t j = 0; j < N; ++j)
                 for (uint32_t i = 0; i < N; ++i) {
                     const uint32_t row = i + j * N + k * N * N;
-                    r.push_back(row); c.push_back(row); v.push_back(6.0 + 1e-8);
-                    const uint32_t nb[6] = {((i + 1) % N) + j * N + k * N * N, ((i + N - 1) % N) + j * N + k * N * N,
-                                            i + ((j + 1) % N) * N + k * N * N, i + ((j + N - 1) % N) * N + k * N * N,
-                                            i + j * N + ((k + 1) % N) * N * N, i + j * N + ((k + N - 1) % N) * N * N};
-                    for (uint32_t q : nb) { r.push_back(row); c.push_back(q); v.push_back(-1.0); }
+                    r.push_back(row); c.push_back(row); v.push_back(6.0);
+                    if (i > 0) { r.push_back(row); c.push_back(row - 1); v.push_back(-1.0); }
+                    if (i < N - 1) { r.push_back(row); c.push_back(row + 1); v.push_back(-1.0); }
+                    if (j > 0) { r.push_back(row); c.push_back(row - N); v.push_back(-1.0); }
+                    if (j < N - 1) { r.push_back(row); c.push_back(row + N); v.push_back(-1.0); }
+                    if (k > 0) { r.push_back(row); c.push_back(row - N * N); v.push_back(-1.0); }
+                    if (k < N - 1) { r.push_back(row); c.push_back(row + N * N); v.push_back(-1.0); }
                 }
         SMatrix L(n, n, v, r, c);
         BlockMatrix<double, 1> Lb = BlockMatrix<double, 1>::condense(L, n, n);
@@ -139,10 +142,88 @@ int main() {
         for (uint32_t i = 0; i < n; ++i) xe[i] = drand(i);
         rhs = SMatrix::spmv(L, xe);
         auto [xs, its] = SMatrix::gmres(L, rhs, amg, 100);
-        double err = 0;
-        for (uint32_t i = 0; i < n; ++i) err += (xs[i] - xe[i]) * (xs[i] - xe[i]);
-        std::printf("     periodic Laplacian + block AMG (BS=1) GMRES: %d steps\n", its + 1);
-        report("AMG", N, std::sqrt(err), 1e-6);
+        double err = 0, nx = 0;
+        for (uint32_t i = 0; i < n; ++i) { err += (xs[i] - xe[i]) * (xs[i] - xe[i]); nx += xe[i] * xe[i]; }
+        std::printf("     Dirichlet Laplacian + block AMG (BS=1) GMRES: %d steps\n", its + 1);
+        report("AMG", N, std::sqrt(err / nx), 1e-8);
+    }
+    // 3-D periodic MAC Stokes with the Schur preconditioner: ILU(0) velocity
+    // block, BS=1 block AMG on the pressure Laplacian (test_BlockMatrix_schur3d.cpp)
+    {
+        const uint32_t N = 12, N3 = N * N * N, nv = 3 * N3, np = N3, nt = nv + np;
+        auto idx = [&](uint32_t i, uint32_t j, uint32_t k) { return i + j * N + k * N * N; };
+        std::vector<double> vK, vA, vB, vBt, vL;
+        std::vector<uint32_t> rK, cK, rA, cA, rB, cB, rBt, cBt, rL, cL;
+        const double alpha = 0.1;
+        for (uint32_t comp = 0; comp < 3; ++comp)
+            for (uint32_t k = 0; k < N; ++k)
+                for (uint32_t j = 0; j < N; ++j)
+                    for (uint32_t i = 0; i < N; ++i) {
+                        const uint32_t row = comp * N3 + idx(i, j, k);
+                        rA.push_back(row); cA.push_back(row); vA.push_back(6.0 + alpha);
+                        rK.push_back(row); cK.push_back(row); vK.push_back(6.0 + alpha);
+                        const uint32_t nb[6] = {idx((i + 1) % N, j, k), idx((i + N - 1) % N, j, k), idx(i, (j + 1) % N, k),
+                                                idx(i, (j + N - 1) % N, k), idx(i, j, (k + 1) % N), idx(i, j, (k + N - 1) % N)};
+                        for (uint32_t q : nb) {
+                            rA.push_back(row); cA.push_back(comp * N3 + q); vA.push_back(-1.0);
+                            rK.push_back(row); cK.push_back(comp * N3 + q); vK.push_back(-1.0);
+                        }
+                    }
+        for (uint32_t k = 0; k < N; ++k)
+            for (uint32_t j = 0; j < N; ++j)
+                for (uint32_t i = 0; i < N; ++i) {
+                    const uint32_t p = idx(i, j, k), rp = nv + p;
+                    rL.push_back(p); cL.push_back(p); vL.push_back(6.0 + 1e-8);
+                    const uint32_t nb[6] = {idx((i + 1) % N, j, k), idx((i + N - 1) % N, j, k), idx(i, (j + 1) % N, k),
+                                            idx(i, (j + N - 1) % N, k), idx(i, j, (k + 1) % N), idx(i, j, (k + N - 1) % N)};
+                    for (uint32_t q : nb) { rL.push_back(p); cL.push_back(q); vL.push_back(-1.0); }
+                    auto vel = [&](uint32_t comp, uint32_t cur, uint32_t prev) {
+                        const uint32_t uc = comp * N3 + cur, up = comp * N3 + prev;
+                        rB.push_back(p); cB.push_back(uc); vB.push_back(1.0);
+                        rB.push_back(p); cB.push_back(up); vB.push_back(-1.0);
+                        rBt.push_back(uc); cBt.push_back(p); vBt.push_back(1.0);
+                        rBt.push_back(up); cBt.push_back(p); vBt.push_back(-1.0);
+                        rK.push_back(rp); cK.push_back(uc); vK.push_back(1.0);
+                        rK.push_back(rp); cK.push_back(up); vK.push_back(-1.0);
+                        rK.push_back(uc); cK.push_back(rp); vK.push_back(1.0);
+                        rK.push_back(up); cK.push_back(rp); vK.push_back(-1.0);
+                    };
+                    vel(0, p, idx((i + N - 1) % N, j, k));
+                    vel(1, p, idx(i, (j + N - 1) % N, k));
+                    vel(2, p, idx(i, j, (k + N - 1) % N));
+                }
+        SMatrix A(nv, nv, vA, rA, cA), B(np, nv, vB, rB, cB), Bt(nv, np, vBt, rBt, cBt), Lp(np, np, vL, rL, cL),
+            K(nt, nt, vK, rK, cK);
+        auto Apc = SMatrix::make_ilu0_preconditioner(A);
+        BlockMatrix<double, 1> Lpb = BlockMatrix<double, 1>::condense(Lp, np, np);
+        auto Lamg = make_block_amg_preconditioner(Lpb);
+        auto Spc = [Lamg](const std::vector<double>& rr, std::vector<double>& z) {
+            std::vector<double> w;
+            Lamg(rr, w);
+            z.resize(rr.size());
+            for (size_t i = 0; i < rr.size(); ++i) z[i] = 6.1 * w[i];
+        };
+        SaddlePointSystem sys;
+        sys.n_vel = nv;
+        sys.n_pres = np;
+        sys.spmv_A = [A](const std::vector<double>& x, std::vector<double>& y) { y = SMatrix::spmv(A, x); };
+        sys.spmv_B = [B](const std::vector<double>& x, std::vector<double>& y) { y = SMatrix::spmv(B, x); };
+        sys.spmv_Bt = [Bt](const std::vector<double>& x, std::vector<double>& y) { y = SMatrix::spmv(Bt, x); };
+        std::vector<double> xe(nt);
+        for (uint32_t i = 0; i < nt; ++i) xe[i] = drand(i);
+        std::vector<double> b = SMatrix::spmv(K, xe);
+        auto schur = make_schur_preconditioner(sys, Apc, Spc);
+        auto [xs, its] = SMatrix::gmres(K, b, schur, 100);
+        std::vector<double> Kx = SMatrix::spmv(K, xs);
+        double rr = 0, bb = 0, err = 0;
+        for (uint32_t i = 0; i < nt; ++i) {
+            rr += (b[i] - Kx[i]) * (b[i] - Kx[i]);
+            bb += b[i] * b[i];
+            err += (xs[i] - xe[i]) * (xs[i] - xe[i]);
+        }
+        std::printf("     3-D Stokes + Schur(ILU0, block AMG) GMRES(100): %d steps, ||x - x_exact|| = %.3e\n", its + 1,
+                    std::sqrt(err));
+        report("Stokes", N, std::sqrt(rr / bb), 1e-9);
     }
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
     return failures ? 1 : 0;

@@ -379,3 +379,64 @@ def dot(a, b) -> float:
     out = C.c_double()
     _check(lib().fem_dot(a.size, _p(a, C.c_double), _p(b, C.c_double), C.byref(out)))
     return out.value
+
+
+# ---------------------------------------------------------------------------
+# Device buffers (HIP runtime through ctypes) so callers can keep inputs
+# resident in HBM and use the device-pointer entry points.
+# ---------------------------------------------------------------------------
+_hip = None
+
+
+def _hiprt():
+    global _hip
+    if _hip is None:
+        h = C.CDLL("libamdhip64.so")
+        h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        h.hipFree.argtypes = [C.c_void_p]
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipDeviceSynchronize.argtypes = []
+        _hip = h
+    return _hip
+
+
+class DeviceArray:
+    """float64 buffer in device memory."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        self.ptr = C.c_void_p()
+        rc = _hiprt().hipMalloc(C.byref(self.ptr), max(self.n, 1) * 8)
+        if rc != 0:
+            raise DeviceError(f"hipMalloc failed ({rc})")
+
+    @classmethod
+    def from_host(cls, a):
+        a = _f64(a)
+        d = cls(a.size)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = _f64(a)
+        rc = _hiprt().hipMemcpy(self.ptr, a.ctypes.data_as(C.c_void_p), a.size * 8, 1)
+        if rc != 0:
+            raise DeviceError(f"hipMemcpy H2D failed ({rc})")
+
+    def download(self):
+        out = np.empty(self.n)
+        rc = _hiprt().hipMemcpy(out.ctypes.data_as(C.c_void_p), self.ptr, self.n * 8, 2)
+        if rc != 0:
+            raise DeviceError(f"hipMemcpy D2H failed ({rc})")
+        return out
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value or 0)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                _hiprt().hipFree(self.ptr)
+        except Exception:
+            pass

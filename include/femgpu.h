@@ -88,14 +88,17 @@ enum fem_smoother {
 };
 
 typedef struct fem_pc_params {
-    double omega;       /* SSOR relaxation (1.2, array.hpp:312); AMG smoother damping */
+    double omega;       /* SSOR relaxation (1.2, array.hpp:312); AMG smoother damping;
+                           SA-AMG: <= 0 selects 4/(3 rho(D^{-1}A)) per level (default) */
     double theta;       /* AMG strength threshold (0.25) */
     int npre, npost;    /* AMG pre/post smoothing sweeps */
     int max_levels;     /* AMG level cap (25 C/F, 20 SA) */
     int coarse_size;    /* AMG coarsest size (300 rows C/F, 50 block rows SA) */
     int max_agg;        /* SA aggregate size cap (4) */
     int smoother;       /* enum fem_smoother */
-    double p_omega;     /* SA prolongator smoothing weight (2/3) */
+    double p_omega;     /* SA prolongator smoothing weight; <= 0 (default) selects
+                           4/(3 rho(D^{-1}A)) per level and stops coarsening when
+                           a level keeps > 85% of its rows; 2/3 = the reference */
     int coarse_exact;   /* SA: 1 = pivoted LU coarse solve; 0 = the reference's
                            elimination sequence (blockMatrix.hpp:958-999) */
     int fp32;           /* also build a float32 copy for FEM_PREC_FP32_IR */

@@ -153,6 +153,14 @@ def test_bs1_block_saamg_matches_scalar_saamg():
     z1 = F.Preconditioner(B, F.PC_BSAAMG).apply(r)
     z2 = F.Preconditioner(A, "saamg").apply(r)
     assert np.array_equal(z1, z2)
+    # block AMG with the library defaults as a GMRES preconditioner (the
+    # reference's schur3d use) on a 3-D Laplacian
+    G2 = gen.poisson3d(24)
+    A2 = F.CsrMatrix.from_gen(G2)
+    B2 = Bsr.condense(A2, 1, G2.n, G2.n)
+    _, b = gen.rhs(G2, 0)
+    x, st = F.solve(B2, F.Preconditioner(B2, F.PC_BSAAMG), b, solver="gmres", restart=100, max_iter=300)
+    assert st["true_relres"] < 1e-10 and st["iterations"] < 30
 
 
 def test_block_errors():

@@ -12,6 +12,7 @@ import scipy.sparse as sp
 import femgpu as F
 import oracle_api as O
 from femgpu import gen
+from parity import tol_x
 
 pytestmark = pytest.mark.gpu
 
@@ -184,11 +185,7 @@ def test_nonsymmetric_convdiff(solver, pc):
     x, st = F.solve(A, M, b, solver=solver, restart=30, max_iter=4000)
     xo, ito = O.solve_named(O.Csr.from_gen(G), 1 if solver == "bicgstab" else 2, pc, b, max_iter=4000, restart=30)
     assert st["true_relres"] < 1e-9
-    # Two iterates that both satisfy ||b - A x|| < 1e-10 ||b|| can differ by
-    # up to cond(A) * 2e-10 relative; without a preconditioner cond(A) ~ 1e3
-    # here, so the solution bar is that bound, not 1e-8.
-    tol_x = TOL_X if pc != "none" else max(TOL_X, 2e-10 * np.linalg.cond(G.to_scipy().toarray()))
-    assert rel(x, xo) < tol_x
+    assert rel(x, xo) < tol_x(G)
     assert abs(st["iterations"] - ito) <= max(3, ito // 20)
 
 
@@ -202,7 +199,7 @@ def test_fp32_refinement_reaches_fp64_tolerance(solver, pc):
     assert st["converged"] == 1
     assert st["true_relres"] < 1e-10
     xref = O.solve_named(O.Csr.from_gen(G), {"pcg": 0, "bicgstab": 1, "gmres": 2}[solver], pc, b, max_iter=5000)[0]
-    assert rel(x, xref) < TOL_X
+    assert rel(x, xref) < tol_x(G)
 
 
 def test_error_paths():

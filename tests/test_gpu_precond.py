@@ -80,6 +80,11 @@ def _amg_case(kind, exact=0, smoother=F.SMOOTH_JACOBI, npre=1, npost=1, omega=No
     if G is None:
         G = gen.poisson2d(N) if kind == "amg" else gen.poisson3d(14)
     A = F.CsrMatrix.from_gen(G)
+    # SA: None means the reference's weights 2/3 (the library default is the
+    # robust 4/(3 rho) choice, selected by 0)
+    if kind == "saamg":
+        omega = 2 / 3 if omega is None else omega
+        p_omega = 2 / 3 if p_omega is None else p_omega
     kw = dict(smoother=smoother, npre=npre, npost=npost)
     if omega is not None:
         kw["omega"] = omega
@@ -92,8 +97,7 @@ def _amg_case(kind, exact=0, smoother=F.SMOOTH_JACOBI, npre=1, npost=1, omega=No
     if kind == "amg":
         Mo = O.Pc(Ao, O.K_AMG, [0.25, 300, 25])
     else:
-        Mo = O.Pc(Ao, O.K_SAAMG, [0.25, 2 / 3 if omega is None else omega, npre, npost, 20, 4, 50,
-                                  2 / 3 if p_omega is None else p_omega, exact])
+        Mo = O.Pc(Ao, O.K_SAAMG, [0.25, omega, npre, npost, 20, 4, 50, p_omega, exact])
     return G, A, M, Ao, Mo
 
 

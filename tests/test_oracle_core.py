@@ -480,3 +480,39 @@ def test_reference_saamg_degenerates_on_lognormal_coefficients():
     x, it = O.solve(A, 0, rob, b, tol=1e-10)
     assert np.linalg.norm(b - G.matvec(x)) / np.linalg.norm(b) < 1e-10
     assert it < 60
+
+
+def test_unpreconditioned_bicgstab_depends_on_reduction_order():
+    """On convection-diffusion with cell Peclet 5 (test_SMatrix_bicgstab.cpp's
+    matrix, N=200) the reference's BiCGSTAB without a preconditioner goes
+    through near-breakdowns: with one OpenMP thread it converges, while with
+    several threads the dot products are summed in an order that changes from
+    run to run (the OpenMP reduction combines partial sums as threads finish),
+    so the iterates differ and the run may stall or break down. Parity for such
+    runs cannot be iterate-by-iterate."""
+    N, v = 200, 5.0
+    rows, cols, vals = [], [], []
+    for i in range(N):
+        for j in range(N):
+            k = i * N + j
+            rows.append(k); cols.append(k); vals.append(4 + 2 * v)
+            if j > 0: rows.append(k); cols.append(k - 1); vals.append(-1 - v)
+            if j < N - 1: rows.append(k); cols.append(k + 1); vals.append(-1.0)
+            if i > 0: rows.append(k); cols.append(k - N); vals.append(-1 - v)
+            if i < N - 1: rows.append(k); cols.append(k + N); vals.append(-1.0)
+    A = O.Csr.from_coo(N * N, N * N, vals, rows, cols)
+    rp, ci, vv = A.export()
+    S = sp.csr_matrix((vv, ci, rp), shape=(N * N, N * N))
+    b = np.ones(N * N)
+    xs = {}
+    try:
+        for t in (1, 8):
+            O.set_threads(t)
+            try:
+                xs[t], _ = O.solve_named(A, 1, "none", b, 1e-10, 1400)
+            except O.OracleError:
+                xs[t] = None
+    finally:
+        O.set_threads(8)
+    assert np.linalg.norm(b - S @ xs[1]) / np.linalg.norm(b) < 1e-10
+    assert xs[8] is None or not np.array_equal(xs[1], xs[8])
