@@ -14,13 +14,17 @@
 //   C/F Jacobi   u <- u + w (f - A u) / d          (array.cpp:3355-3369)
 //   SA  Jacobi   u <- u + w (d^{-1} (f - A u))      (blockMatrix.hpp:1010-1024)
 //   SGS          multicolour Gauss-Seidel, colours ascending before the coarse
-//                correction and descending after it (symmetric V-cycle)
+//                correction and descending after it (symmetric V-cycle), on
+//                levels that colour into at most kMaxGsColours colours; other
+//                levels are smoothed by damped Jacobi with weight
+//                (4/3) / rho(D^{-1} A), rho the Gershgorin bound
 // Every level starts from u = 0, as the reference's V-cycle does.
 #include <algorithm>
 #include <cmath>
 
 #include "amg.hpp"
 #include "precond.hpp"
+#include "reduce.hpp"
 
 namespace fem {
 
@@ -39,39 +43,92 @@ __global__ void k_jacobi0(const T* __restrict__ f, const T* __restrict__ d, T* _
     else u[i] = (T)0 + (w * r) / d[i];
 }
 
-// one multicolour Gauss-Seidel colour: u_i += w (f_i - (A u)_i) / a_ii
-template <typename T>
-__global__ void k_gs_colour(uint32_t lo, uint32_t hi, const uint32_t* __restrict__ rows,
-                            const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
-                            const T* __restrict__ v, const T* __restrict__ d, const T* __restrict__ f, T* u, T w,
-                            Gate g) {
-    if (g.closed()) return;
-    const uint32_t t = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= hi) return;
-    const uint32_t i = rows[t];
+// Multicolour Gauss-Seidel on a colour-grouped copy of the level matrix:
+// row t of the grouped CSR (gp, gci, gv) is row rows[t] of A, the rows of
+// each colour are contiguous, gd[t] = a_ii. In the one-workgroup kernel for
+// small levels LANES lanes share one row (the row's products summed by a
+// butterfly over those lanes).
+//   u_i += w (f_i - (A u)_i) / a_ii   for the rows t in [lo, hi)
+template <typename T, int LANES>
+__device__ __forceinline__ void gs_rows(uint32_t t, uint32_t hi, int lane, const uint32_t* __restrict__ rows,
+                                        const uint32_t* __restrict__ gp, const uint32_t* __restrict__ gci,
+                                        const T* __restrict__ gv, const T* __restrict__ gd, const T* __restrict__ f,
+                                        T* u, T w) {
     T s = (T)0;
-    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * u[ci[p]];
-    u[i] = u[i] + (w * (f[i] - s)) / d[i];
+    if (t < hi)
+        for (uint32_t p = gp[t] + lane; p < gp[t + 1]; p += LANES) s += gv[p] * u[gci[p]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
+    if (t < hi && lane == 0) {
+        const uint32_t i = rows[t];
+        u[i] = u[i] + (w * (f[i] - s)) / gd[t];
+    }
 }
 
-// all colours of `sweeps` Gauss-Seidel sweeps in one workgroup (small levels)
-template <typename T>
+// all colours of `sweeps` sweeps in one workgroup (levels of <= 8192 rows)
+template <typename T, int LANES>
 __global__ __launch_bounds__(1024) void k_gs_small(int ncol, const uint32_t* __restrict__ cptr,
-                                                   const uint32_t* __restrict__ rows, const uint32_t* __restrict__ rp,
-                                                   const uint32_t* __restrict__ ci, const T* __restrict__ v,
-                                                   const T* __restrict__ d, const T* __restrict__ f, T* u, T w,
+                                                   const uint32_t* __restrict__ rows, const uint32_t* __restrict__ gp,
+                                                   const uint32_t* __restrict__ gci, const T* __restrict__ gv,
+                                                   const T* __restrict__ gd, const T* __restrict__ f, T* u, T w,
                                                    int sweeps, int reverse, Gate g) {
     if (g.closed()) return;
+    constexpr uint32_t kRowsPerPass = 1024 / LANES;
+    const int lane = threadIdx.x % LANES;
     for (int sw = 0; sw < sweeps; ++sw) {
         for (int cc = 0; cc < ncol; ++cc) {
             const int c = reverse ? ncol - 1 - cc : cc;
-            for (uint32_t t = cptr[c] + threadIdx.x; t < cptr[c + 1]; t += blockDim.x) {
-                const uint32_t i = rows[t];
-                T s = (T)0;
-                for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * u[ci[p]];
-                u[i] = u[i] + (w * (f[i] - s)) / d[i];
-            }
+            const uint32_t lo = cptr[c], hi = cptr[c + 1];
+            for (uint32_t base = lo; base < hi; base += kRowsPerPass)
+                gs_rows<T, LANES>(base + threadIdx.x / LANES, hi, lane, rows, gp, gci, gv, gd, f, u, w);
             __syncthreads();
+        }
+    }
+}
+
+// One colour of a large level, blocked like the CSR-stream SpMV over the
+// colour-grouped matrix: a block stages the products of its rows' entries
+// in LDS (coalesced loads of gci/gv), then one thread per row sums them and
+// applies the update to u[rows[t]]. blk[] holds row boundaries of the
+// grouped matrix cut at colour boundaries; the launch covers one colour.
+// Rows of one colour share no entries, so blocks never read a u value that
+// another block of the same launch writes (the diagonal entry is read in
+// the staging phase, before this block's own update).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict__ blk,
+                                                      const uint32_t* __restrict__ rows,
+                                                      const uint32_t* __restrict__ gp,
+                                                      const uint32_t* __restrict__ gci, const T* __restrict__ gv,
+                                                      const T* __restrict__ gd, const T* __restrict__ f, T* u, T w,
+                                                      Gate g) {
+    if (g.closed()) return;
+    __shared__ T prod[kStageNnz];
+    const uint32_t r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+    const uint32_t k0 = gp[r0], k1 = gp[r1];
+    const uint32_t cnt = k1 - k0;
+    const uint32_t t = threadIdx.x;
+    if (cnt <= (uint32_t)kStageNnz) {
+#pragma unroll
+        for (int q = 0; q < kStageNnz / kBlock; ++q) {
+            const uint32_t k = (uint32_t)q * kBlock + t;
+            if (k < cnt) prod[k] = gv[k0 + k] * u[gci[k0 + k]];
+        }
+        __syncthreads();
+        for (uint32_t row = r0 + t; row < r1; row += kBlock) {
+            const uint32_t b = gp[row] - k0, e = gp[row + 1] - k0;
+            T s = (T)0;
+            for (uint32_t k = b; k < e; ++k) s += prod[k];
+            const uint32_t i = rows[row];
+            u[i] = u[i] + (w * (f[i] - s)) / gd[row];
+        }
+    } else {
+        double s = 0.0;
+        for (uint32_t k = k0 + t; k < k1; k += kBlock) s += (double)(gv[k] * u[gci[k]]);
+        double tot[1] = {s};
+        block_sum<1>(tot);
+        if (t == 0) {
+            const uint32_t i = rows[r0];
+            u[i] = u[i] + (w * (f[i] - (T)tot[0])) / gd[r0];
         }
     }
 }
@@ -114,10 +171,20 @@ struct AmgLevelDev {
     DevBuf<float> f32, u32, u232, res32;
     // multicolour ordering for the SGS smoother
     int ncol = 0;
+    bool use_gs = false;  // SGS smoother: colour-by-colour GS on this level
+    double jw = 0.0;      // SGS smoother: Jacobi weight when !use_gs
     std::vector<uint32_t> h_cptr;
     DevBuf<uint32_t> cptr, crows;
-    DevBuf<double> diag;  // a_ii for SGS
+    DevBuf<double> diag;  // a_ii for SGS (natural order)
     DevBuf<float> diag32;
+    // colour-grouped copy of A for the GS sweeps; gblk: CSR-stream blocks of
+    // the grouped rows, colour c owning blocks [h_cblk[c], h_cblk[c+1])
+    int lanes = 1;
+    std::vector<uint32_t> h_cblk;
+    DevBuf<uint32_t> gblk;
+    DevBuf<uint32_t> gp, gci;
+    DevBuf<double> gv, gd;
+    DevBuf<float> gv32, gd32;
 };
 
 struct PcAmg : Precond {
@@ -142,6 +209,11 @@ struct PcAmg : Precond {
             D->omega = k < H.omega.size() ? H.omega[k] : p.omega;
             lvl_rows.push_back(A.n);
             lvl_nnz.push_back(A.v.size());
+            // the colour-grouped copy reads A before its arrays move to the level
+            if (smoother == FEM_SMOOTH_SGS && k + 1 < L && (int)H.cptr[k].size() - 1 <= kMaxGsColours) {
+                D->h_cptr = H.cptr[k];
+                group_rows(*D, A, H.crows[k], H.diag[k], s);
+            }
             D->A.set_host(A.n, A.m, std::move(A.rp), std::move(A.ci), std::move(A.v), s);
             D->A.stream = s;
             if (k + 1 < L) {
@@ -156,9 +228,12 @@ struct PcAmg : Precond {
                 D->res.alloc(D->n);
                 if (smoother == FEM_SMOOTH_SGS) {
                     D->ncol = (int)H.cptr[k].size() - 1;
-                    D->h_cptr = H.cptr[k];
-                    D->cptr.upload(H.cptr[k], s);
-                    D->crows.upload(H.crows[k], s);
+                    D->use_gs = D->ncol <= kMaxGsColours;
+                    D->jw = H.jw[k];
+                    if (D->use_gs) {
+                        D->cptr.upload(H.cptr[k], s);
+                        D->crows.upload(H.crows[k], s);
+                    }
                     D->diag.upload(H.diag[k], s);
                 }
             } else {
@@ -170,6 +245,39 @@ struct PcAmg : Precond {
         nco = lv.back()->n;
         coarse.upload(H.coarse, s);
         FEM_HIP(hipStreamSynchronize(s));
+    }
+    // rows of A in colour order (crows) as a CSR of their own
+    static void group_rows(AmgLevelDev& D, const HCsr& A, const std::vector<uint32_t>& crows,
+                           const std::vector<double>& diag, hipStream_t s) {
+        const uint32_t n = A.n;
+        std::vector<uint32_t> gp(n + 1, 0), gci;
+        std::vector<double> gv, gd(n);
+        gci.reserve(A.ci.size());
+        gv.reserve(A.v.size());
+        for (uint32_t t = 0; t < n; ++t) {
+            const uint32_t i = crows[t];
+            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+                gci.push_back(A.ci[p]);
+                gv.push_back(A.v[p]);
+            }
+            gp[t + 1] = (uint32_t)gci.size();
+            gd[t] = diag[i];
+        }
+        std::vector<uint32_t> blk;
+        const int ncol = (int)D.h_cptr.size() - 1;
+        D.h_cblk.assign(1, 0);
+        for (int c = 0; c < ncol; ++c) {
+            plan_rows(gp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk);
+            D.h_cblk.push_back((uint32_t)blk.size() - 1);
+        }
+        D.gblk.upload(blk, s);
+        const double avg = n ? (double)A.ci.size() / n : 0.0;
+        D.lanes = avg <= 1.5 ? 1 : avg <= 3 ? 4 : avg <= 12 ? 8 : avg <= 24 ? 16 : avg <= 48 ? 32 : 64;
+        D.gp.upload(gp, s);
+        D.gci.upload(gci, s);
+        D.gv.upload(gv, s);
+        D.gd.upload(gd, s);
+        FEM_HIP(hipStreamSynchronize(s));  // the host arrays end with this call
     }
     int kind() const override { return kind_; }
     uint32_t size() const override { return lv[0]->n; }
@@ -187,6 +295,13 @@ struct PcAmg : Precond {
                 if (D.diag.get()) {
                     D.diag32.alloc(D.n);
                     vec_convert<float, double>(D.diag32.get(), D.diag.get(), 1.0, D.n, s);
+                }
+                if (D.gv.get()) {
+                    const size_t nz = D.gv.n;
+                    D.gv32.alloc(nz);
+                    vec_convert<float, double>(D.gv32.get(), D.gv.get(), 1.0, nz, s);
+                    D.gd32.alloc(D.n);
+                    vec_convert<float, double>(D.gd32.get(), D.gd.get(), 1.0, D.n, s);
                 }
                 D.u232.alloc(D.n);
                 D.res32.alloc(D.n);
@@ -213,48 +328,76 @@ struct PcAmg : Precond {
 
     // Jacobi sweeps; `cur` holds the iterate (nullptr = zero); returns the
     // buffer holding the result. `a`/`b` are the two buffers to alternate.
+    // `inv`: d holds 1/a_ii (else a_ii); `wt`: the damping weight
     template <typename T>
-    T* jacobi(AmgLevelDev& D, const T* f, const T* d, T* cur, T* a, T* b, int sweeps, hipStream_t s, Gate g) {
-        const T w = (T)D.omega;
+    T* jacobi(AmgLevelDev& D, const T* f, const T* d, bool inv, double wt, T* cur, T* a, T* b, int sweeps,
+              hipStream_t s, Gate g) {
+        const T w = (T)wt;
         for (int k = 0; k < sweeps; ++k) {
             T* out = (cur == a) ? b : a;
             if (!cur) {
-                if (inv_diag) hipLaunchKernelGGL((k_jacobi0<T, true>), g1(D.n), dim3(256), 0, s, f, d, out, D.n, w, g);
+                if (inv) hipLaunchKernelGGL((k_jacobi0<T, true>), g1(D.n), dim3(256), 0, s, f, d, out, D.n, w, g);
                 else hipLaunchKernelGGL((k_jacobi0<T, false>), g1(D.n), dim3(256), 0, s, f, d, out, D.n, w, g);
             } else {
                 SpmvArgs<T> ar;
                 ar.f = f;
                 ar.u = cur;
                 ar.d = d;
-                ar.w = D.omega;
-                csr_spmv<T>(D.A, cur, out, inv_diag ? EPI_JACOBI_INV : EPI_JACOBI, ar, s, g);
+                ar.w = wt;
+                csr_spmv<T>(D.A, cur, out, inv ? EPI_JACOBI_INV : EPI_JACOBI, ar, s, g);
             }
             cur = out;
         }
         return cur;
     }
 
-    template <typename T>
-    void gs(AmgLevelDev& D, const T* f, const T* diag, T* u, int sweeps, bool reverse, hipStream_t s, Gate g) {
-        if (sweeps <= 0) return;
+    template <typename T, int LANES>
+    void gs_launch(AmgLevelDev& D, const T* gv, const T* gd, const T* f, T* u, int sweeps, bool reverse,
+                   hipStream_t s, Gate g) {
         const T w = (T)D.omega;
-        const T* vals;
-        if constexpr (std::is_same<T, double>::value) vals = D.A.v.get();
-        else vals = D.A.v32.get();
         if (D.n <= 8192) {
-            hipLaunchKernelGGL((k_gs_small<T>), dim3(1), dim3(1024), 0, s, D.ncol, D.cptr.get(), D.crows.get(),
-                               D.A.rp.get(), D.A.ci.get(), vals, diag, f, u, w, sweeps, reverse ? 1 : 0, g);
+            hipLaunchKernelGGL((k_gs_small<T, LANES>), dim3(1), dim3(1024), 0, s, D.ncol, D.cptr.get(),
+                               D.crows.get(), D.gp.get(), D.gci.get(), gv, gd, f, u, w, sweeps, reverse ? 1 : 0, g);
             return;
         }
         for (int sw = 0; sw < sweeps; ++sw)
             for (int cc = 0; cc < D.ncol; ++cc) {
                 const int c = reverse ? D.ncol - 1 - cc : cc;
-                const uint32_t lo = D.h_cptr[c], hi = D.h_cptr[c + 1];
-                if (lo >= hi) continue;
-                hipLaunchKernelGGL((k_gs_colour<T>), g1(hi - lo), dim3(256), 0, s, lo, hi, D.crows.get(),
-                                   D.A.rp.get(), D.A.ci.get(), vals, diag, f, u, w, g);
+                const uint32_t b0 = D.h_cblk[c], b1 = D.h_cblk[c + 1];
+                if (b0 >= b1) continue;
+                hipLaunchKernelGGL((k_gs_stream<T>), dim3(b1 - b0), dim3(kBlock), 0, s, D.gblk.get() + b0,
+                                   D.crows.get(), D.gp.get(), D.gci.get(), gv, gd, f, u, w, g);
             }
     }
+
+    template <typename T>
+    void gs(AmgLevelDev& D, const T* f, T* u, int sweeps, bool reverse, hipStream_t s, Gate g) {
+        if (sweeps <= 0) return;
+        const T *gv, *gd;
+        if constexpr (std::is_same<T, double>::value) {
+            gv = D.gv.get();
+            gd = D.gd.get();
+        } else {
+            gv = D.gv32.get();
+            gd = D.gd32.get();
+        }
+        switch (D.lanes) {
+            case 1: gs_launch<T, 1>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
+            case 4: gs_launch<T, 4>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
+            case 8: gs_launch<T, 8>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
+            case 16: gs_launch<T, 16>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
+            case 32: gs_launch<T, 32>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
+            default: gs_launch<T, 64>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
+        }
+    }
+
+    // Jacobi data of a level: the SGS smoother's fallback divides by a_ii
+    // with the level's Gershgorin weight; the Jacobi smoother uses d and the
+    // configured (or robust) weight
+    template <typename T>
+    const T* jd(AmgLevelDev&, const Bufs<T>& B) const { return smoother == FEM_SMOOTH_SGS ? B.diag : B.d; }
+    bool jinv() const { return smoother == FEM_SMOOTH_SGS ? false : inv_diag; }
+    double jwt(const AmgLevelDev& D) const { return smoother == FEM_SMOOTH_SGS ? D.jw : D.omega; }
 
     template <typename T>
     void vcycle(const T* r, T* z, hipStream_t s, Gate g) {
@@ -267,10 +410,10 @@ struct PcAmg : Precond {
             Bufs<T> B = bufs<T>(D);
             const T* f = fk[k];
             T* u;
-            if (smoother == FEM_SMOOTH_SGS) {
+            if (smoother == FEM_SMOOTH_SGS && D.use_gs) {
                 u = (k == 0) ? z : B.u;
                 hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, u, D.n, g);
-                gs<T>(D, f, B.diag, u, npre, false, s, g);
+                gs<T>(D, f, u, npre, false, s, g);
             } else {
                 // Jacobi alternates between two buffers; on level 0 pick the
                 // first one so that the last post-smoothing sweep writes z
@@ -284,7 +427,7 @@ struct PcAmg : Precond {
                 xa[k] = X;
                 xb[k] = Y;
                 if (npre > 0) {
-                    u = jacobi<T>(D, f, B.d, nullptr, X, Y, npre, s, g);
+                    u = jacobi<T>(D, f, jd(D, B), jinv(), jwt(D), nullptr, X, Y, npre, s, g);
                 } else {
                     hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, X, D.n, g);
                     u = X;
@@ -316,8 +459,8 @@ struct PcAmg : Precond {
             SpmvArgs<T> ar;
             ar.u = u;
             csr_spmv<T>(D.P, ucur[k + 1], u, EPI_ADD, ar, s, g);
-            if (smoother == FEM_SMOOTH_SGS) gs<T>(D, fk[k], B.diag, u, npost, true, s, g);
-            else u = jacobi<T>(D, fk[k], B.d, u, xa[k], xb[k], npost, s, g);
+            if (smoother == FEM_SMOOTH_SGS && D.use_gs) gs<T>(D, fk[k], u, npost, true, s, g);
+            else u = jacobi<T>(D, fk[k], jd(D, B), jinv(), jwt(D), u, xa[k], xb[k], npost, s, g);
             ucur[k] = u;
         }
         FEM_HIP(hipGetLastError());

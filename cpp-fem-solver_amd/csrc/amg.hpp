@@ -14,6 +14,11 @@ struct HCsr {
 HCsr transpose(const HCsr& A);
 HCsr spgemm(const HCsr& A, const HCsr& B);
 
+// Multicolour Gauss-Seidel runs one launch per colour; a level needing more
+// colours than this is smoothed by damped Jacobi instead (coarse SA operators
+// with 27+ entries per row colour greedily into 15-40 colours).
+constexpr int kMaxGsColours = 8;
+
 struct AmgHierarchy {
     std::vector<HCsr> A;                  // level operators, A[0] = input
     std::vector<HCsr> P;                  // prolongators, P[k]: n_k x n_{k+1}
@@ -23,9 +28,13 @@ struct AmgHierarchy {
     // multicolour ordering per level (SGS smoother)
     std::vector<std::vector<uint32_t>> cptr, crows;
     std::vector<std::vector<double>> diag;
+    // SGS smoother: damped-Jacobi weight (4/3) / rho(D^{-1} A) (Gershgorin)
+    // for the levels whose colouring has more than kMaxGsColours colours
+    std::vector<double> jw;
 };
 
 AmgHierarchy build_cf_hierarchy(const CsrMat& A, const fem_pc_params& p);
 AmgHierarchy build_sa_hierarchy(const CsrMat& A, const fem_pc_params& p);
+int mis2_aggregate(const HCsr& A, double theta, std::vector<int>& agg);
 
 }  // namespace fem

@@ -18,25 +18,28 @@
 
 namespace fem {
 
-constexpr int kStageNnz = 2048;
-constexpr int kStageRows = 256;
-
-void SpmvPlan::build(const uint32_t* h_rp, uint32_t n, hipStream_t s) {
-    std::vector<uint32_t> b;
-    b.reserve(n / 128 + 4);
-    b.push_back(0);
-    uint32_t start = 0;
-    while (start < n) {
+void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint32_t>& b) {
+    if (b.empty()) b.push_back(lo);
+    uint32_t start = lo;
+    while (start < hi) {
         if (h_rp[start + 1] - h_rp[start] > (uint32_t)kStageNnz) {
             ++start;
             b.push_back(start);
             continue;
         }
         uint32_t end = start;
-        while (end < n && end - start < (uint32_t)kStageRows && h_rp[end + 1] - h_rp[start] <= (uint32_t)kStageNnz) ++end;
+        while (end < hi && end - start < (uint32_t)kStageRows && h_rp[end + 1] - h_rp[start] <= (uint32_t)kStageNnz)
+            ++end;
         b.push_back(end);
         start = end;
     }
+}
+
+void SpmvPlan::build(const uint32_t* h_rp, uint32_t n, hipStream_t s) {
+    std::vector<uint32_t> b;
+    b.reserve(n / 128 + 4);
+    b.push_back(0);
+    plan_rows(h_rp, 0, n, b);
     nblk = (uint32_t)b.size() - 1;
     blk_row.upload(b, s);
 }

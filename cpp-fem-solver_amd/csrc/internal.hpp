@@ -42,6 +42,13 @@ struct LinOp {
 
 // SpMV launch schedule: consecutive row ranges whose nonzeros fit one block's
 // LDS staging area, plus single long rows that get a whole block each.
+// CSR-stream SpMV blocking: a block stages at most kStageNnz products of at
+// most kStageRows consecutive rows in LDS (a longer row gets a block alone)
+constexpr int kStageNnz = 2048;
+constexpr int kStageRows = 256;
+// append the block boundaries of rows [lo, hi) to b (b starts with lo)
+void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint32_t>& b);
+
 struct SpmvPlan {
     DevBuf<uint32_t> blk_row;  // [nblk+1]
     uint32_t nblk = 0;

@@ -872,22 +872,42 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
         // the reference's loop body never runs: x stays 0
         return 0;
     }
-    int chunk = opts.check_every > 0 ? opts.check_every : (W.n < 200000 ? 32 : 16);
-    GraphCache& gc = std::is_same<T, double>::value ? g64_ : g32_;
-    const int tag = solver * 100000 + (solver == FEM_SOLVER_GMRES ? W.m : chunk);
+    // PCG / BiCGSTAB replay a captured chunk of iterations between host
+    // checks. Iterations after convergence still launch (their kernels exit
+    // on the done flag), so the chunk length follows the observed
+    // convergence rate: power-of-two chunks, at most `cmax`, sized to the
+    // iterations the residual history predicts are left. GMRES replays one
+    // restart cycle at a time.
+    const int cmax = opts.check_every > 0 ? opts.check_every : (W.n < 200000 ? 32 : 16);
+    GraphSet& gs = std::is_same<T, double>::value ? g64_ : g32_;
+    const double rel0 = hst_.p->normb > 0 ? hst_.p->normr / hst_.p->normb : 1.0;
+    int chunk = opts.check_every > 0 ? cmax : std::min(cmax, 8);
     auto body = [&] {
         if (solver == FEM_SOLVER_PCG) for (int k = 0; k < chunk; ++k) pcg_iter<T>(W);
         else if (solver == FEM_SOLVER_BICGSTAB) for (int k = 0; k < chunk; ++k) bi_iter<T>(W);
         else gm_cycle<T>(W);
     };
-    // every replay runs `chunk` iterations (GMRES: one cycle of >= 1 step)
-    const long long guard_max = (long long)max_iter / (solver == FEM_SOLVER_GMRES ? 1 : chunk) + 4;
+    // every replay runs >= 1 iteration (GMRES: one cycle of >= 1 step)
+    const long long guard_max = (long long)max_iter + 4;
     long long rounds = 0;
     while (!hst_.p->done) {
-        replay(gc, tag, body);
-        FEM_HIP(hipMemcpyAsync(&hst_.p->done, &st_.get()->done, sizeof(int), hipMemcpyDeviceToHost, s_));
-        FEM_HIP(hipStreamSynchronize(s_));
+        const int tag = solver * 100000 + (solver == FEM_SOLVER_GMRES ? W.m : chunk);
+        replay(gs.at(tag), tag, body);
+        read_state();
         if (++rounds > guard_max) fail(FEM_ERR_RUNTIME, "solver made no progress");
+        if (solver == FEM_SOLVER_GMRES || opts.check_every > 0 || hst_.p->done) continue;
+        const double rel = hst_.p->normb > 0 ? hst_.p->normr / hst_.p->normb : 0.0;
+        const int it = hst_.p->it;
+        int next = cmax;
+        if (it > 0 && rel > 0 && rel < rel0) {
+            const double rate = std::pow(rel / rel0, 1.0 / it);
+            const double left = std::log(tol / rel) / std::log(rate);
+            if (left > 0 && left < cmax) {
+                next = 1;
+                while (next < cmax && next < 1.1 * left + 1) next *= 2;
+            }
+        }
+        chunk = next;
     }
     read_state();
     if (hst_.p->done == 2) {

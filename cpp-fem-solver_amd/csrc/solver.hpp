@@ -1,5 +1,8 @@
 // solver.hpp — Krylov solver object (krylov.hip).
 #pragma once
+#include <map>
+#include <memory>
+
 #include "internal.hpp"
 
 namespace fem {
@@ -22,6 +25,16 @@ struct GraphCache {
     int tag = -1;
     void reset();
     ~GraphCache() { reset(); }
+};
+
+// one captured graph per (solver, chunk length) in use
+struct GraphSet {
+    std::map<int, std::unique_ptr<GraphCache>> g;
+    GraphCache& at(int tag) {
+        auto& p = g[tag];
+        if (!p) p.reset(new GraphCache);
+        return *p;
+    }
 };
 
 class KrylovSolver {
@@ -62,7 +75,7 @@ class KrylovSolver {
     KWork<double> w64_;
     KWork<float> w32_;
     DevBuf<double> res64_, x64_, ax64_;
-    GraphCache g64_, g32_;
+    GraphSet g64_, g32_;
     int pc_mode64_ = 0, pc_mode32_ = 0;  // 0 general, 1 none, 2 diag
 };
 

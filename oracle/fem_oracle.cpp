@@ -1012,6 +1012,94 @@ static Bsr<BS> blk_matmul(const Bsr<BS>& A, const Bsr<BS>& B) {
     return C;
 }
 
+// Robust-mode aggregation (no counterpart in the reference): aggregates
+// grown around a distance-2 maximal independent set of the strength graph,
+// following Bell, Dalton & Olson, SIAM J. Sci. Comput. 34(4), 2012.
+//   strong(i,j), j != i:  |a_ij| >= theta * sqrt(m_i * m_j), a_ij != 0,
+//                         m_i = max_{k != i} |a_ik|   (block: Frobenius norm)
+//   priority of node i:   h(i) = lowbias32(i) (C. Wellons' integer hash),
+//                         key = state<<62 | (h & 0x3fffffff)<<32 | i with
+//                         state 2 root, 1 undecided, 0 out
+//   each round: two max-propagations of the keys over the strong neighbours;
+//   an undecided node whose two-hop max is its own key becomes a root, one
+//   whose two-hop max has state 2 goes out; repeat until nothing undecided.
+//   Roots are numbered in index order; a non-root joins the strong root
+//   neighbour with the largest key; a node still free joins the neighbour
+//   (assigned in that first pass) with the largest key; leftovers become
+//   singletons in index order. Written out with per-node lists, one node at
+//   a time, as a restatement independent of the product's array sweeps.
+static uint32_t lowbias32(uint32_t x) {
+    x ^= x >> 16;
+    x *= 0x7feb352dU;
+    x ^= x >> 15;
+    x *= 0x846ca68bU;
+    x ^= x >> 16;
+    return x;
+}
+
+template <class Norm>
+static int mis2_agg(uint32_t n, const std::vector<uint32_t>& rp, const std::vector<uint32_t>& ci, Norm nrm,
+                    double theta, std::vector<int>& agg, std::vector<int>* roots = nullptr) {
+    std::vector<double> m(n, 0.0);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
+            if (ci[p] != i) m[i] = std::max(m[i], nrm(p));
+    std::vector<std::vector<uint32_t>> S(n);
+    for (uint32_t i = 0; i < n; ++i)
+        for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
+            const uint32_t j = ci[p];
+            const double a = nrm(p);
+            if (j != i && a != 0.0 && a >= theta * std::sqrt(m[i] * m[j])) S[i].push_back(j);
+        }
+    auto key = [](int state, uint32_t i) {
+        return ((uint64_t)state << 62) | ((uint64_t)(lowbias32(i) & 0x3fffffffU) << 32) | (uint64_t)i;
+    };
+    std::vector<int> state(n, 1);
+    while (true) {
+        std::vector<uint64_t> one(n), two(n);
+        for (uint32_t i = 0; i < n; ++i) {
+            one[i] = key(state[i], i);
+            for (uint32_t j : S[i]) one[i] = std::max(one[i], key(state[j], j));
+        }
+        for (uint32_t i = 0; i < n; ++i) {
+            two[i] = one[i];
+            for (uint32_t j : S[i]) two[i] = std::max(two[i], one[j]);
+        }
+        int undecided = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (state[i] != 1) continue;
+            if ((two[i] & 0xffffffffULL) == i) state[i] = 2;
+            else if ((two[i] >> 62) == 2) state[i] = 0;
+            else ++undecided;
+        }
+        if (!undecided) break;
+    }
+    agg.assign(n, -1);
+    int na = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (state[i] == 2) agg[i] = na++;
+    if (roots) roots->assign(state.begin(), state.end());
+    std::vector<int> first = agg;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (state[i] == 2) continue;
+        bool found = false;
+        uint64_t best = 0;
+        for (uint32_t j : S[i])
+            if (state[j] == 2 && (!found || key(0, j) > best)) { found = true; best = key(0, j); first[i] = agg[j]; }
+    }
+    agg = first;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (first[i] >= 0) continue;
+        bool found = false;
+        uint64_t best = 0;
+        for (uint32_t j : S[i])
+            if (first[j] >= 0 && (!found || key(0, j) > best)) { found = true; best = key(0, j); agg[i] = first[j]; }
+    }
+    for (uint32_t i = 0; i < n; ++i)
+        if (agg[i] < 0) agg[i] = na++;
+    return na;
+}
+
 // Block SA-AMG (block.hpp:1051-1179) with the smoother made configurable:
 // reference defaults theta=0.25, omega=2/3, one block-Jacobi sweep pre and
 // post, aggregates of at most 4 block rows, coarsest level <= 50 block rows
@@ -1067,24 +1155,28 @@ struct PcBlockAMG : Pc {
             L.om = om > 0.0 ? om : (rho > 0.0 ? (4.0 / 3.0) / rho : 2.0 / 3.0);
             const double pw = auto_p ? (rho > 0.0 ? (4.0 / 3.0) / rho : 2.0 / 3.0) : p_omega;
             if (n <= coarse_max) { lv.push_back(std::move(L)); break; }
-            std::vector<std::vector<uint32_t>> S(n);
-            for (uint32_t i = 0; i < n; ++i) {
-                double mx = 0.0;
-                for (uint32_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p)
-                    if (cur.ci[p] != i) mx = std::max(mx, fro(cur.blk(p)));
-                const double thr = theta * mx;
-                for (uint32_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p)
-                    if (cur.ci[p] != i && fro(cur.blk(p)) >= thr) S[i].push_back(cur.ci[p]);
-            }
             std::vector<int> agg(n, -1);
             int na = 0;
-            for (uint32_t i = 0; i < n; ++i) {
-                if (agg[i] != -1) continue;
-                agg[i] = na;
-                int sz = 1;
-                for (uint32_t j : S[i])
-                    if (agg[j] == -1 && sz < max_agg) { agg[j] = na; ++sz; }
-                ++na;
+            if (auto_p) {
+                na = mis2_agg(n, cur.rp, cur.ci, [&](uint32_t p) { return fro(cur.blk(p)); }, theta, agg);
+            } else {
+                std::vector<std::vector<uint32_t>> S(n);
+                for (uint32_t i = 0; i < n; ++i) {
+                    double mx = 0.0;
+                    for (uint32_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p)
+                        if (cur.ci[p] != i) mx = std::max(mx, fro(cur.blk(p)));
+                    const double thr = theta * mx;
+                    for (uint32_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p)
+                        if (cur.ci[p] != i && fro(cur.blk(p)) >= thr) S[i].push_back(cur.ci[p]);
+                }
+                for (uint32_t i = 0; i < n; ++i) {
+                    if (agg[i] != -1) continue;
+                    agg[i] = na;
+                    int sz = 1;
+                    for (uint32_t j : S[i])
+                        if (agg[j] == -1 && sz < max_agg) { agg[j] = na; ++sz; }
+                    ++na;
+                }
             }
             Bsr<BS> P;
             P.nbr = n;
@@ -1649,6 +1741,16 @@ int or_amg_levels(OrPc* h, uint32_t* rows, uint64_t* nnz, int cap) {
     auto* p = static_cast<PcBlockAMG<1>*>(h->pc.get());
     for (int i = 0; i < (int)p->lv.size() && i < cap; ++i) { rows[i] = p->lv[i].n; nnz[i] = p->lv[i].A.nnzb(); }
     return (int)p->lv.size();
+}
+// Robust-mode aggregation of a scalar CSR matrix (mis2_agg): agg[n], the
+// final MIS-2 state per node (2 root, 0 other) and the number of aggregates.
+int or_mis2_aggregate(const Csr* A, double theta, int32_t* agg, int32_t* state, int* na) {
+    return guard([&] {
+        std::vector<int> g, st;
+        *na = mis2_agg(A->n, A->rp, A->ci, [&](uint32_t p) { return std::sqrt(A->v[p] * A->v[p]); }, theta, g, &st);
+        std::copy(g.begin(), g.end(), agg);
+        std::copy(st.begin(), st.end(), state);
+    });
 }
 // Export one AMG level: A (and P when not coarsest) as CSR.
 int or_amg_level_export(OrPc* h, int l, int which, uint32_t* rp, uint32_t* ci, double* v, uint32_t* dims) {

@@ -47,6 +47,7 @@ def lib():
         L.or_alias.argtypes = [C.c_char_p, C.c_int]
         L.or_amg_levels.argtypes = [P, P, P, C.c_int]
         L.or_amg_level_export.argtypes = [P, C.c_int, C.c_int, P, P, P, P]
+        L.or_mis2_aggregate.argtypes = [P, C.c_double, P, P, C.POINTER(C.c_int)]
         _L = L
     return _L
 
@@ -229,6 +230,15 @@ def solve_named(A: Csr, solver: int, name: str, b, tol=1e-10, max_iter=2000, res
     it = C.c_int()
     _chk(lib().or_solve_named(A.h, solver, name.encode(), _ptr(b), tol, max_iter, restart, _ptr(x), C.byref(it)))
     return x, it.value
+
+
+def mis2_aggregate(A: Csr, theta: float):
+    """Robust-mode SA aggregation: (agg[n], root flags[n], number of aggregates)."""
+    agg = np.empty(A.n, np.int32)
+    st = np.empty(A.n, np.int32)
+    na = C.c_int()
+    _chk(lib().or_mis2_aggregate(A.h, theta, _ptr(agg), _ptr(st), C.byref(na)))
+    return agg, st == 2, na.value
 
 
 def alias(name: str, solver: int) -> int:

@@ -139,7 +139,8 @@ def test_robust_saamg_lognormal_parity():
     G_, A, M, Ao, Mo = _amg_case("saamg", 1, omega=0.0, p_omega=0.0, G=G)
     info = M.info()
     rows, nnz = Mo.amg_levels()
-    assert np.array_equal(info["rows"][: len(rows)], rows)
+    assert info["levels"] == len(rows)
+    assert np.array_equal(info["rows"][: len(rows)], rows) and np.array_equal(info["nnz"][: len(nnz)], nnz)
     r = np.random.default_rng(3).standard_normal(G.n)
     zo = Mo.apply(r)
     np.testing.assert_allclose(M.apply(r), zo, rtol=1e-11, atol=1e-11 * np.abs(zo).max())

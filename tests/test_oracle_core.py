@@ -516,3 +516,41 @@ def test_unpreconditioned_bicgstab_depends_on_reduction_order():
         O.set_threads(8)
     assert np.linalg.norm(b - S @ xs[1]) / np.linalg.norm(b) < 1e-10
     assert xs[8] is None or not np.array_equal(xs[1], xs[8])
+
+def _strength(G, theta):
+    """Strong couplings |a_ij| >= theta sqrt(m_i m_j), m_i = max_{k!=i} |a_ik|."""
+    S = sp.csr_matrix((np.abs(G.values), G.col_idx.astype(np.int64), G.row_ptr.astype(np.int64)), shape=(G.n, G.n))
+    S.setdiag(0)
+    S.eliminate_zeros()
+    m = np.asarray(S.max(axis=1).todense()).ravel()
+    C = S.tocoo()
+    keep = C.data >= theta * np.sqrt(m[C.row] * m[C.col])
+    return sp.csr_matrix((np.ones(keep.sum()), (C.row[keep], C.col[keep])), shape=(G.n, G.n))
+
+
+@pytest.mark.parametrize("case", ["poisson3d", "vardiff3d", "poisson2d"])
+def test_mis2_aggregation_properties(case):
+    """Robust SA aggregation: roots form a distance-2 maximal independent set
+    of the strength graph, each aggregate holds exactly one root, root
+    aggregates are numbered in index order, and every member lies within two
+    strong hops of its root."""
+    G = {"poisson3d": lambda: gen.poisson3d(9), "vardiff3d": lambda: gen.vardiff3d(9, seed=5)[0],
+         "poisson2d": lambda: gen.poisson2d(30)}[case]()
+    theta = 0.25
+    agg, root, na = O.mis2_aggregate(O.Csr.from_gen(G), theta)
+    S = _strength(G, theta)
+    assert (S != S.T).nnz == 0                      # symmetric strength on SPD input
+    I = sp.identity(G.n, format="csr")
+    R1 = (I + S).astype(bool).astype(np.int64)
+    R2 = (R1 @ R1).astype(bool)                     # within two strong hops
+    ridx = np.flatnonzero(root)
+    assert R2[ridx][:, ridx].nnz == len(ridx)       # independence: only the diagonal
+    assert np.all(np.asarray(R2[:, ridx].sum(axis=1)).ravel() >= 1)   # maximality
+    assert np.array_equal(agg[ridx], np.arange(len(ridx)))
+    assert agg.min() == 0 and agg.max() == na - 1 and len(np.unique(agg)) == na
+    assert na == len(ridx)                          # no leftover singletons
+    root_of = np.empty(na, np.int64)
+    root_of[agg[ridx]] = ridx
+    own = root_of[agg]
+    assert np.all(np.asarray(R2[np.arange(G.n), own]).ravel())
+    assert G.n / na > (6 if case != "poisson2d" else 3)   # coarsening ratio
