@@ -542,30 +542,11 @@ struct fem_bsr_s {
     }
 };
 
-void fem_set_error(const std::string& m);
-
-template <class F>
-static int bguard(F&& f) {
-    try {
-        f();
-        return FEM_OK;
-    } catch (const Error& e) {
-        fem_set_error(e.what());
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        fem_set_error("out of memory");
-        return FEM_ERR_DEVICE;
-    } catch (const std::exception& e) {
-        fem_set_error(e.what());
-        return FEM_ERR_RUNTIME;
-    }
-}
-
 extern "C" {
 
 int fem_bsr_create(int bs, uint32_t nbr, uint32_t nbc, const uint32_t* row_ptr, const uint32_t* col_idx,
                    const double* values, fem_bsr* out) {
-    return bguard([&] {
+    return api_guard([&] {
         if (bs < 1 || bs > 4) fail(FEM_ERR_INVALID_ARGUMENT, "block size must be 1..4");
         if (!row_ptr || row_ptr[0] != 0) fail(FEM_ERR_INVALID_ARGUMENT, "row_ptr must start at 0");
         for (uint32_t i = 0; i < nbr; ++i)
@@ -583,7 +564,7 @@ int fem_bsr_create(int bs, uint32_t nbr, uint32_t nbc, const uint32_t* row_ptr, 
 }
 
 int fem_bsr_condense(fem_csr A, int bs, uint32_t nbr, uint32_t nbc, fem_bsr* out) {
-    return bguard([&] {
+    return api_guard([&] {
         if (bs < 1 || bs > 4) fail(FEM_ERR_INVALID_ARGUMENT, "block size must be 1..4");
         std::vector<uint32_t> rp, ci;
         std::vector<double> v;
@@ -596,7 +577,7 @@ int fem_bsr_condense(fem_csr A, int bs, uint32_t nbr, uint32_t nbc, fem_bsr* out
 }
 
 int fem_bsr_info(fem_bsr A, int* bs, uint32_t* nbr, uint32_t* nbc, uint64_t* nnzb) {
-    return bguard([&] {
+    return api_guard([&] {
         if (bs) *bs = A->A.bs;
         if (nbr) *nbr = A->A.nbr;
         if (nbc) *nbc = A->A.nbc;
@@ -605,7 +586,7 @@ int fem_bsr_info(fem_bsr A, int* bs, uint32_t* nbr, uint32_t* nbc, uint64_t* nnz
 }
 
 int fem_bsr_export(fem_bsr A, uint32_t* row_ptr, uint32_t* col_idx, double* values) {
-    return bguard([&] {
+    return api_guard([&] {
         if (row_ptr) std::copy(A->A.h_rp.begin(), A->A.h_rp.end(), row_ptr);
         if (col_idx) std::copy(A->A.h_ci.begin(), A->A.h_ci.end(), col_idx);
         if (values) std::copy(A->A.h_v.begin(), A->A.h_v.end(), values);
@@ -615,7 +596,7 @@ int fem_bsr_export(fem_bsr A, uint32_t* row_ptr, uint32_t* col_idx, double* valu
 // transpose (blockMatrix.hpp:237-267): block pattern transposed by counting
 // sort, each block transposed
 int fem_bsr_transpose(fem_bsr A, fem_bsr* out) {
-    return bguard([&] {
+    return api_guard([&] {
         const BsrMat& M = A->A;
         const int bs = M.bs, BV = bs * bs;
         std::vector<uint32_t> rp(M.nbc + 1, 0), ci(M.nnzb);
@@ -638,7 +619,7 @@ int fem_bsr_transpose(fem_bsr A, fem_bsr* out) {
 }
 
 int fem_bsr_matvec(fem_bsr A, const double* x, double* y) {
-    return bguard([&] {
+    return api_guard([&] {
         const uint32_t n = A->A.rows(), m = A->A.cols();
         if (A->xs.n < m) A->xs.alloc(std::max<uint32_t>(m, 1));
         if (A->ys.n < n) A->ys.alloc(std::max<uint32_t>(n, 1));
@@ -650,7 +631,7 @@ int fem_bsr_matvec(fem_bsr A, const double* x, double* y) {
 }
 
 int fem_bsr_matvec_device(fem_bsr A, const void* x, void* y, int fp32) {
-    return bguard([&] {
+    return api_guard([&] {
         if (fp32) {
             A->A.ensure_fp32(A->stream);
             A->A.apply32((const float*)x, (float*)y, A->stream);
@@ -662,7 +643,7 @@ int fem_bsr_matvec_device(fem_bsr A, const void* x, void* y, int fp32) {
 }
 
 int fem_bsr_precond_create(fem_bsr A, int kind, const fem_pc_params* params, fem_pc* out) {
-    return bguard([&] {
+    return api_guard([&] {
         fem_pc_params p;
         if (params) p = *params;
         else fem_pc_params_default(kind, &p);
@@ -684,7 +665,7 @@ int fem_bsr_precond_create(fem_bsr A, int kind, const fem_pc_params* params, fem
 }
 
 int fem_bsr_solve(fem_bsr A, fem_pc M, const double* b, double* x, const fem_solve_opts* opts, fem_solve_stats* stats) {
-    return bguard([&] {
+    return api_guard([&] {
         fem_solver_s h;
         fem_solver_make(&A->A, A->stream, M, opts, &h);
         fem_solver_run(&h, b, x, stats);
@@ -692,7 +673,7 @@ int fem_bsr_solve(fem_bsr A, fem_pc M, const double* b, double* x, const fem_sol
 }
 
 int fem_bsr_destroy(fem_bsr A) {
-    return bguard([&] { delete A; });
+    return api_guard([&] { delete A; });
 }
 
 }  // extern "C"

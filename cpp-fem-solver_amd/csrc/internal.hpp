@@ -120,6 +120,11 @@ struct Precond {
     // Non-null for FEM_PC_DIAG: the diagonal d (z = r / d) for fused kernels.
     virtual const double* diag64() const { return nullptr; }
     virtual const float* diag32() const { return nullptr; }
+    // Learned preconditioners: the iteration budget of a solve before the
+    // fallback preconditioner takes over (0: no fallback), and the fallback
+    // (built on first use).
+    virtual int budget() const { return 0; }
+    virtual Precond* fallback(hipStream_t) { return nullptr; }
     double setup_ms = 0.0;
     int levels = 0;
     std::vector<uint32_t> lvl_rows;

@@ -923,20 +923,54 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
 double KrylovSolver::true_residual(const double* b, const double* x, double* work) {
     const size_t n = A_->rows();
     A_->apply(x, work, s_);
-    DevBuf<double> out(1);
+    if (!tr_out_.get()) tr_out_.alloc(1);
     {
         const unsigned G = red_blocks(n);
         double* part = red_.partials.get();
         LAUNCH(k_resid_norm, G, b, work, work, n, part);
-        FIN(1, FinStore{out.get()});
+        FIN(1, FinStore{tr_out_.get()});
     }
     double h = 0.0;
-    out.download(&h, 1, s_);
+    tr_out_.download(&h, 1, s_);
     FEM_HIP(hipStreamSynchronize(s_));
     return std::sqrt(h);
 }
 
+// A preconditioner with an iteration budget (the learned SPAI) gets one
+// attempt of at most that many iterations; if it does not converge (or breaks
+// down) the system is solved again from x0 = 0 with its fallback
+// preconditioner, and stats->fallback is set.
 void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
+    const int budget = M_ ? M_->budget() : 0;
+    if (budget <= 0) {
+        solve_once(b, x, stats);
+        return;
+    }
+    Timer tm;
+    const int saved = opts.max_iter;
+    opts.max_iter = std::min(saved, budget);
+    fem_solve_stats S{};
+    bool ok = false;
+    try {
+        solve_once(b, x, &S);
+        ok = S.converged == 1;
+    } catch (const Error& e) {
+        opts.max_iter = saved;
+        if (e.code != FEM_ERR_RUNTIME) throw;
+    }
+    opts.max_iter = saved;
+    if (!ok) {
+        const long long first = S.spmv_count;
+        if (!fb_) fb_.reset(new KrylovSolver(A_, M_->fallback(s_), opts, s_));
+        fb_->solve(b, x, &S);
+        S.fallback = 1;
+        S.spmv_count += first;
+    }
+    S.solve_ms = tm.ms();
+    if (stats) *stats = S;
+}
+
+void KrylovSolver::solve_once(const double* b, double* x, fem_solve_stats* stats) {
     const uint32_t n = A_->rows();
     const int m = opts.solver == FEM_SOLVER_GMRES ? opts.restart : 0;
     fem_solve_stats S{};
@@ -955,7 +989,8 @@ void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
         // float32 Krylov inside float64 iterative refinement
         w32_.ensure(opts.solver, n, m);
         if (x64_.n != n) { x64_.alloc(n); ax64_.alloc(n); res64_.alloc(n); }
-        DevBuf<double> nrm(1);
+        if (!nrm_out_.get()) nrm_out_.alloc(1);
+        DevBuf<double>& nrm = nrm_out_;
         vec_fill<double>(x64_.get(), 0.0, n, s_);
         FEM_HIP(hipMemcpyAsync(res64_.get(), b, (size_t)n * 8, hipMemcpyDeviceToDevice, s_));
         // ||b||
@@ -1010,21 +1045,24 @@ void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
         S.solve_ms = tm.ms();
     }
     if (stats) {
-        DevBuf<double> work(n);
+        if (st_work_.n != n) {
+            st_work_.alloc(n);
+            st_zero_.alloc(n);
+            st_out_.alloc(1);
+            vec_fill<double>(st_zero_.get(), 0.0, n, s_);
+        }
+        double* work = st_work_.get();
         const double normb = [&] {
-            DevBuf<double> zero(n);
-            vec_fill<double>(zero.get(), 0.0, n, s_);
-            DevBuf<double> out(1), tmp(n);
             const unsigned G = red_blocks(n);
             double* part = red_.partials.get();
-            LAUNCH(k_resid_norm, G, b, zero.get(), tmp.get(), (size_t)n, part);
-            FIN(1, FinStore{out.get()});
+            LAUNCH(k_resid_norm, G, b, st_zero_.get(), work, (size_t)n, part);
+            FIN(1, FinStore{st_out_.get()});
             double hh = 0.0;
-            out.download(&hh, 1, s_);
+            st_out_.download(&hh, 1, s_);
             FEM_HIP(hipStreamSynchronize(s_));
             return std::sqrt(hh);
         }();
-        const double tr = true_residual(b, x, work.get());
+        const double tr = true_residual(b, x, work);
         S.true_relres = normb > 0 ? tr / normb : 0.0;
         *stats = S;
     }

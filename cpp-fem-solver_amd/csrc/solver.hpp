@@ -46,6 +46,7 @@ class KrylovSolver {
     fem_solve_opts opts;
 
    private:
+    void solve_once(const double* b, double* x, fem_solve_stats* st);
     template <typename T>
     int run(KWork<T>& W, double tol, int max_iter, bool soft, long long* spmv);
     template <typename T>
@@ -76,6 +77,9 @@ class KrylovSolver {
     KWork<float> w32_;
     DevBuf<double> res64_, x64_, ax64_;
     GraphSet g64_, g32_;
+    DevBuf<double> st_work_, st_zero_, st_out_;   // end-of-solve statistics
+    DevBuf<double> tr_out_, nrm_out_;             // single scalars read back by the host
+    std::unique_ptr<KrylovSolver> fb_;            // solver over the fallback preconditioner
     int pc_mode64_ = 0, pc_mode32_ = 0;  // 0 general, 1 none, 2 diag
 };
 

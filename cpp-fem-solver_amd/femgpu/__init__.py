@@ -63,10 +63,14 @@ class SolveOpts(C.Structure):
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("inner_iterations", C.c_int), ("refinements", C.c_int),
                 ("converged", C.c_int), ("true_relres", C.c_double), ("setup_ms", C.c_double),
-                ("solve_ms", C.c_double), ("spmv_count", C.c_longlong)]
+                ("solve_ms", C.c_double), ("spmv_count", C.c_longlong), ("fallback", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AdamOpts(C.Structure):
+    _fields_ = [("lr", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double)]
 
 
 HOST_CB = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_uint64, C.c_void_p)
@@ -75,6 +79,7 @@ _lib = None
 
 P_u32 = C.POINTER(C.c_uint32)
 P_f64 = C.POINTER(C.c_double)
+P_f32 = C.POINTER(C.c_float)
 P_vp = C.c_void_p
 
 _SIGS = {
@@ -116,6 +121,15 @@ _SIGS = {
     "fem_bsr_solve": (C.c_int, [P_vp, P_vp, P_f64, P_f64, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "fem_bsr_destroy": (C.c_int, [P_vp]),
     "fem_bench_spmv": (C.c_int, [P_vp, C.c_int, C.c_int, P_f64]),
+    "fem_mlp_create": (C.c_int, [P_f32, C.c_uint64, C.POINTER(P_vp)]),
+    "fem_mlp_destroy": (C.c_int, [P_vp]),
+    "fem_mlp_get_params": (C.c_int, [P_vp, P_f32]),
+    "fem_mlp_set_params": (C.c_int, [P_vp, P_f32]),
+    "fem_mlp_forward": (C.c_int, [P_vp, P_f32, C.c_uint32, P_f32]),
+    "fem_spai_features": (C.c_int, [P_vp, P_u32, P_f32]),
+    "fem_spai_loss_grad": (C.c_int, [P_vp, P_vp, P_u32, P_f64, C.c_int, C.c_int, P_f64, P_f32]),
+    "fem_spai_train_step": (C.c_int, [P_vp, P_vp, P_u32, P_f64, C.c_int, C.c_int, C.POINTER(AdamOpts), P_f64]),
+    "fem_precond_create_learned": (C.c_int, [P_vp, P_u32, P_vp, C.c_int, C.c_int, C.POINTER(P_vp)]),
 }
 
 
@@ -289,6 +303,20 @@ class Preconditioner:
             pass
 
 
+class LearnedPreconditioner(Preconditioner):
+    """Learned SPAI M (fem_precond_create_learned): one SpMV per apply; a
+    solve stops after `budget` iterations (0: no limit) and is redone with
+    the fallback preconditioner if it has not converged."""
+
+    def __init__(self, A, grid, net, fallback="ic0", budget=0):
+        self.kind = PC_LEARNED
+        self._h = C.c_void_p()
+        self.n = A.rows
+        fb = PC_KINDS[fallback] if isinstance(fallback, str) else fallback
+        g = _u32(grid)
+        _check(lib().fem_precond_create_learned(A._h, _p(g, C.c_uint32), net._h, fb, int(budget), C.byref(self._h)))
+
+
 def solve_opts(solver="pcg", tol=1e-10, max_iter=2000, restart=30, precision=FP64, inner_tol=None,
                max_refine=None, check_every=0, use_graph=True, device_ptrs=False) -> SolveOpts:
     o = SolveOpts()
@@ -440,3 +468,86 @@ class DeviceArray:
                 _hiprt().hipFree(self.ptr)
         except Exception:
             pass
+
+
+# ---- learned sparse approximate inverse --------------------------------------
+MLP_NPARAM = 5063
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+class Mlp:
+    """The learned SPAI network (6 -> 64 -> 64 -> 7) with its Adam state on
+    the device (fem_mlp_*)."""
+
+    def __init__(self, params=None, seed=0):
+        self._h = C.c_void_p()
+        if params is None:
+            _check(lib().fem_mlp_create(None, seed, C.byref(self._h)))
+        else:
+            p = _f32(params)
+            if p.size != MLP_NPARAM:
+                raise InvalidArgument(f"expected {MLP_NPARAM} parameters")
+            _check(lib().fem_mlp_create(_p(p, C.c_float), seed, C.byref(self._h)))
+
+    @property
+    def params(self):
+        p = np.empty(MLP_NPARAM, np.float32)
+        _check(lib().fem_mlp_get_params(self._h, _p(p, C.c_float)))
+        return p
+
+    @params.setter
+    def params(self, value):
+        p = _f32(value)
+        _check(lib().fem_mlp_set_params(self._h, _p(p, C.c_float)))
+
+    def forward(self, feat):
+        f = _f32(feat)
+        n = f.shape[0]
+        out = np.empty((n, 7), np.float32)
+        _check(lib().fem_mlp_forward(self._h, _p(f, C.c_float), n, _p(out, C.c_float)))
+        return out
+
+    def loss_grad(self, A, grid, V):
+        V = _f64(np.atleast_2d(V))
+        g = np.empty(MLP_NPARAM, np.float32)
+        L = C.c_double()
+        gr = _u32(grid)
+        _check(lib().fem_spai_loss_grad(self._h, A._h, _p(gr, C.c_uint32), _p(V, C.c_double), V.shape[0], 0,
+                                        C.byref(L), _p(g, C.c_float)))
+        return L.value, g
+
+    def train_step(self, A, grid, V, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, device_k=None):
+        """One Adam step. V: (k, n) host array, or a device pointer with
+        device_k = k."""
+        o = AdamOpts(lr, beta1, beta2, eps)
+        L = C.c_double()
+        gr = _u32(grid)
+        if device_k is None:
+            V = _f64(np.atleast_2d(V))
+            _check(lib().fem_spai_train_step(self._h, A._h, _p(gr, C.c_uint32), _p(V, C.c_double), V.shape[0], 0,
+                                             C.byref(o), C.byref(L)))
+        else:
+            _check(lib().fem_spai_train_step(self._h, A._h, _p(gr, C.c_uint32), C.cast(V, P_f64), int(device_k), 1,
+                                             C.byref(o), C.byref(L)))
+        return L.value
+
+    def close(self):
+        if self._h:
+            lib().fem_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def spai_features(A, grid):
+    f = np.empty((A.rows, 6), np.float32)
+    g = _u32(grid)
+    _check(lib().fem_spai_features(A._h, _p(g, C.c_uint32), _p(f, C.c_float)))
+    return f

@@ -126,6 +126,8 @@ typedef struct fem_solve_stats {
     double setup_ms;       /* preconditioner setup time of this call (0 if prebuilt) */
     double solve_ms;       /* wall time of the iterations */
     long long spmv_count;  /* operator applications */
+    int fallback;          /* 1 if a learned preconditioner's budget ran out and
+                              the system was solved with its fallback */
 } fem_solve_stats;
 
 /* ---- library ---------------------------------------------------------- */
@@ -220,6 +222,49 @@ int fem_bsr_precond_create(fem_bsr A, int kind, const fem_pc_params* params, fem
 int fem_bsr_solve(fem_bsr A, fem_pc M, const double* b, double* x, const fem_solve_opts* opts,
                   fem_solve_stats* stats);
 int fem_bsr_destroy(fem_bsr A);
+
+/* ---- learned sparse approximate inverse ------------------------------------
+ * No counterpart in the reference: an MLP (6 -> 64 -> 64 -> 7, ReLU) reads
+ * each row's scaled stencil couplings f_i[d] = -6 a_ij / sqrt(a_ii a_jj) of a
+ * 7-point grid matrix (rows (z*ny + y)*nx + x, directions d = 0..5 for the
+ * column offsets -nx*ny, -nx, -1, +1, +nx, +nx*ny) and emits the row's entries
+ * of an SPD approximate inverse M on A's pattern:
+ *   s_ij = (o_i[1+d] + o_j[1+(5-d)]) / 2,  m_ij = s_ij / sqrt(a_ii a_jj),
+ *   m_ii = (softplus(o_i[0]) + sum_d |s_i,d|) / a_ii.
+ * Parameters are float32, flat, FEM_MLP_NPARAM of them: W1[6][64], b1[64],
+ * W2[64][64], b2[64], W3[64][7], b3[7] (W[in][out]). The device kernels run
+ * the layers on the matrix cores with bfloat16 operands and float32
+ * accumulation. Training minimises L = 1/(n k) sum_c ||M A v_c - v_c||^2 over
+ * k probe vectors with Adam. grid[3] = {nx, ny, nz}. */
+#define FEM_MLP_NPARAM 5063
+typedef struct fem_mlp_s* fem_mlp;
+typedef struct fem_adam_opts {
+    double lr, beta1, beta2, eps;
+} fem_adam_opts;
+/* params: FEM_MLP_NPARAM floats, or NULL for He-initialised weights from seed */
+int fem_mlp_create(const float* params, uint64_t seed, fem_mlp* out);
+int fem_mlp_destroy(fem_mlp net);
+int fem_mlp_get_params(fem_mlp net, float* params);
+int fem_mlp_set_params(fem_mlp net, const float* params);
+/* network outputs (n x 7, host) for host features (n x 6) */
+int fem_mlp_forward(fem_mlp net, const float* feat, uint32_t n, float* out);
+/* the features of A (n x 6, host); FEM_ERR_INVALID_ARGUMENT if A is not a
+ * symmetric-pattern 7-point matrix on the grid with a positive diagonal */
+int fem_spai_features(fem_csr A, const uint32_t* grid, float* feat);
+/* loss and gradient (FEM_MLP_NPARAM floats, may be NULL) for k probe vectors
+ * V (k x n, probe c at V + c*n; device pointers if device_ptrs) */
+int fem_spai_loss_grad(fem_mlp net, fem_csr A, const uint32_t* grid, const double* V, int k, int device_ptrs,
+                       double* loss, float* grad);
+/* one Adam step on that loss; *loss is the loss before the step */
+int fem_spai_train_step(fem_mlp net, fem_csr A, const uint32_t* grid, const double* V, int k, int device_ptrs,
+                        const fem_adam_opts* opt, double* loss);
+/* M from the network (one SpMV per apply). A solve with this preconditioner
+ * runs at most `budget` iterations (0: no limit); if it has not converged by
+ * then, or breaks down, the system is solved again with the fallback
+ * preconditioner (FEM_PC_IC0, FEM_PC_ILU0, FEM_PC_AMG, FEM_PC_SAAMG or
+ * FEM_PC_DIAG, built on first use) and fem_solve_stats.fallback is set. */
+int fem_precond_create_learned(fem_csr A, const uint32_t* grid, fem_mlp net, int fallback_kind, int budget,
+                               fem_pc* out);
 
 /* ---- measurement --------------------------------------------------------- */
 /* Average device time of `reps` back-to-back SpMV launches on A's stream,

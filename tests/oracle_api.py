@@ -48,6 +48,10 @@ def lib():
         L.or_amg_levels.argtypes = [P, P, P, C.c_int]
         L.or_amg_level_export.argtypes = [P, C.c_int, C.c_int, P, P, P, P]
         L.or_mis2_aggregate.argtypes = [P, C.c_double, P, P, C.POINTER(C.c_int)]
+        L.or_spai_features.argtypes = [C.c_uint32, P, P, P, P, P, P]
+        L.or_mlp_forward.argtypes = [P, C.c_uint32, P, C.c_int, P]
+        L.or_spai_assemble.argtypes = [C.c_uint32, P, P, P, P, P, C.c_int, P]
+        L.or_spai_loss_grad.argtypes = [C.c_uint32, P, P, P, P, P, P, C.c_int, C.c_int, P, P]
         _L = L
     return _L
 
@@ -239,6 +243,53 @@ def mis2_aggregate(A: Csr, theta: float):
     na = C.c_int()
     _chk(lib().or_mis2_aggregate(A.h, theta, _ptr(agg), _ptr(st), C.byref(na)))
     return agg, st == 2, na.value
+
+
+# ---- learned sparse approximate inverse (spai_oracle.cpp) -------------------
+def spai_nparam() -> int:
+    return lib().or_spai_nparam()
+
+
+def _grid_csr(G, grid):
+    rp, ci, v = _a(G.row_ptr, np.uint32), _a(G.col_idx, np.uint32), _a(G.values, np.float64)
+    return rp, ci, v, _a(grid, np.uint32)
+
+
+def spai_features(G, grid):
+    rp, ci, v, g = _grid_csr(G, grid)
+    feat = np.empty((G.n, 6))
+    dis = np.empty(G.n)
+    _chk(lib().or_spai_features(G.n, _ptr(rp), _ptr(ci), _ptr(v), _ptr(g), _ptr(feat), _ptr(dis)))
+    return feat, dis
+
+
+def mlp_forward(params, feat, bf16=False):
+    params = _a(params, np.float32)
+    feat = _a(feat, np.float64)
+    out = np.empty((feat.shape[0], 7))
+    _chk(lib().or_mlp_forward(_ptr(params), feat.shape[0], _ptr(feat), int(bf16), _ptr(out)))
+    return out
+
+
+def spai_assemble(G, grid, params, bf16=False):
+    rp, ci, v, g = _grid_csr(G, grid)
+    params = _a(params, np.float32)
+    m = np.empty(len(v))
+    _chk(lib().or_spai_assemble(G.n, _ptr(rp), _ptr(ci), _ptr(v), _ptr(g), _ptr(params), int(bf16), _ptr(m)))
+    return m
+
+
+def spai_loss_grad(G, grid, params, V, bf16=False):
+    """V: (k, n) probe vectors. Returns (loss, grad[nparam])."""
+    rp, ci, v, g = _grid_csr(G, grid)
+    params = _a(params, np.float32)
+    V = _a(V, np.float64)
+    k = V.shape[0]
+    loss = np.zeros(1)
+    grad = np.empty(spai_nparam())
+    _chk(lib().or_spai_loss_grad(G.n, _ptr(rp), _ptr(ci), _ptr(v), _ptr(g), _ptr(params), _ptr(V), k, int(bf16),
+                                 _ptr(loss), _ptr(grad)))
+    return float(loss[0]), grad
 
 
 def alias(name: str, solver: int) -> int:
