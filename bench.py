@@ -55,6 +55,120 @@ def measured_traffic():
     return None
 
 
+def c4_leg(F, gen, args, log):
+    """Learned SPAI on the variable-coefficient 96^3 diffusion system
+    (kappa = exp(g), g a Gaussian random field, sigma 1, correlation length 8
+    cells). The network is trained on the GPU on `c4_train` seeded fields of
+    32^3 cells (same field statistics; the network only sees 7-point
+    stencils) and tested on `c4_test` held-out 96^3 seeds: learned SPAI PCG
+    in float32-inner/float64-refinement form against plain float64 IC(0)-PCG
+    (every 4th seed), float64 Jacobi-PCG and SA-AMG PCG. Solve times are
+    single timed solves after one warm-up, b and x in HBM."""
+    from femgpu import learned as Lr
+    t0 = time.time()
+    fields = Lr.make_fields(32, range(args.c4_train))
+    gen_s = time.time() - t0
+    net = F.Mlp(seed=0)
+    t0 = time.time()
+    for f in fields:
+        f.errors = Lr.error_vectors(f)
+    err_s = time.time() - t0
+    steps = 3 * len(fields)
+    hist, train_s = Lr.train(net, fields, steps=steps)
+    del fields
+    log(f"[bench] c4 training: {args.c4_train} fields, {steps} Adam steps in {train_s:.2f} s "
+        f"(field generation {gen_s:.1f} s, error vectors {err_s:.1f} s), loss {hist[:10].mean():.3f} -> {hist[-50:].mean():.3f}")
+    N = 96
+    sa = dict(smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0, coarse_exact=1, fp32=1)
+    rows = {k: [] for k in ("learned_mp", "learned_fp64", "jacobi_fp64", "ic0_fp64", "saamg_mp")}
+    worst = {"relres": 0.0, "rel_err": 0.0}
+    fallbacks = 0
+    for t, seed in enumerate(range(256, 256 + args.c4_test)):
+        G = gen.vardiff3d(N, seed)[0]
+        xt, b = gen.rhs(G, seed)
+        A = F.CsrMatrix.from_gen(G)
+        bd = F.DeviceArray.from_host(b)
+        xd = F.DeviceArray(G.n)
+
+        def run(name, M, prec):
+            S = F.Solver(A, M, solver="pcg", precision=prec, max_iter=20000, device_ptrs=True, tol=1e-10)
+            S.solve_device(bd.address, xd.address)
+            t1 = time.perf_counter()
+            st = S.solve_device(bd.address, xd.address)
+            ms = (time.perf_counter() - t1) * 1e3
+            x = xd.download()
+            err = float(np.linalg.norm(x - xt) / np.linalg.norm(xt))
+            worst["relres"] = max(worst["relres"], st["true_relres"])
+            worst["rel_err"] = max(worst["rel_err"], err)
+            rows[name].append((seed, ms, st["iterations"]))
+            S.close()
+            return st
+
+        Ml = F.LearnedPreconditioner(A, (N, N, N), net, fallback="ic0", budget=2000)
+        fallbacks += run("learned_mp", Ml, F.FP32_IR)["fallback"]
+        run("learned_fp64", Ml, F.FP64)
+        run("jacobi_fp64", F.Preconditioner(A, "diag"), F.FP64)
+        if t % 4 == 0:
+            run("ic0_fp64", F.Preconditioner(A, "ic0"), F.FP64)
+            run("saamg_mp", F.Preconditioner(A, "saamg", **sa), F.FP32_IR)
+        Ml.close()
+        A.close()
+    med = {k: (float(np.median([r[1] for r in v])), float(np.median([r[2] for r in v]))) for k, v in rows.items() if v}
+    ic0_seeds = [r[0] for r in rows["ic0_fp64"]]
+    lm = {r[0]: r[1] for r in rows["learned_mp"]}
+    speed = float(np.median([r[1] / lm[r[0]] for r in rows["ic0_fp64"]])) if ic0_seeds else None
+    for k, (ms, it) in med.items():
+        log(f"[bench] c4 {k}: median {ms:.2f} ms, {it:.0f} iterations")
+    log(f"[bench] c4 learned MP vs FP64 IC0-PCG: {speed:.1f}x (median over seeds {ic0_seeds})")
+    return {"workload": "3-D log-normal diffusion 96^3 (n=884736), sigma 1, correlation length 8 cells, tol 1e-10",
+            "train": {"fields": args.c4_train, "field_cells": "32^3", "steps": steps, "seconds": round(train_s, 2),
+                      "loss_first": round(float(hist[:10].mean()), 4), "loss_last": round(float(hist[-50:].mean()), 4)},
+            "test_seeds": f"256..{255 + args.c4_test}",
+            "median_ms": {k: round(v[0], 3) for k, v in med.items()},
+            "median_iterations": {k: v[1] for k, v in med.items()},
+            "speedup_learned_mp_vs_ic0_fp64": round(speed, 2) if speed else None,
+            "fallbacks": int(fallbacks), "worst_true_relres": worst["relres"], "worst_rel_err": worst["rel_err"]}
+
+
+def c5_leg(F, gen, args, log):
+    """Batch of independent SPD systems (half 2-D Poisson, side U{32..256};
+    half 3x3-block 7-point systems, side U{16..48}; 4 right-hand sides each;
+    seed 0) solved by block-Jacobi PCG in float32 inside float64 refinement
+    in one persistent launch, schedule from the autotuner; device time of the
+    launch, every right-hand side checked against tol 1e-10."""
+    t0 = time.time()
+    sysl = gen.batch_c5(0, count=args.c5_count)
+    gen_s = time.time() - t0
+    B = F.Batch([s["A"] for s in sysl], nrhs=4)
+    B.set_rhs([s["B"] for s in sysl])
+    t0 = time.time()
+    best, trials = B.tune(divisor=8)
+    tune_s = time.time() - t0
+    B.set_schedule(**best)
+    B.solve()
+    times = []
+    for _ in range(3):
+        st, ms = B.solve()
+        times.append(ms)
+    X = B.solution()
+    worst = max(s["max_relres"] for s in st)
+    conv = sum(s["converged"] for s in st)
+    err = max(float(np.linalg.norm(x - s["X"]) / np.linalg.norm(s["X"])) for x, s in zip(X, sysl))
+    its = [s["iterations"] for s in st]
+    unknowns = sum(int(x.size) for x in X)
+    log(f"[bench] c5: {len(sysl)} systems ({unknowns} unknowns x rhs), schedule {best}, "
+        f"solve {np.median(times):.1f} ms, {conv}/{4 * len(sysl)} converged, worst relres {worst:.2e}")
+    B.close()
+    return {"workload": f"{len(sysl)} SPD systems (half 2-D Poisson side 32..256, half 3x3-block 7-point side 16..48), "
+                        "4 rhs each, block-Jacobi PCG FP32 inner / FP64 refinement, tol 1e-10, one persistent launch",
+            "solve_ms_median": round(float(np.median(times)), 2), "unknowns_times_rhs": unknowns,
+            "schedule": best, "tune_seconds": round(tune_s, 2), "trial_ms": [round(float(t), 2) for t in trials],
+            "converged": int(conv), "rhs_total": 4 * len(sysl), "worst_true_relres": worst,
+            "worst_rel_err": err, "inner_iterations": {"min": int(min(its)), "median": float(np.median(its)),
+                                                       "max": int(max(its))},
+            "generation_seconds": round(gen_s, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +177,11 @@ def main():
     ap.add_argument("--n", type=int, default=128, help="grid side of the 3-D Poisson workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--methods", default="all", help="comma list: saamg_mp,saamg_fp64,ic0_fp64,jacobi_fp64")
+    ap.add_argument("--no-c4", action="store_true", help="skip the learned-preconditioner leg")
+    ap.add_argument("--c4-train", type=int, default=256, help="training fields (32^3 cells each)")
+    ap.add_argument("--c4-test", type=int, default=32, help="held-out 96^3 fields")
+    ap.add_argument("--no-c5", action="store_true", help="skip the batched-solve leg")
+    ap.add_argument("--c5-count", type=int, default=512, help="systems in the batch")
     ap.add_argument("--spmv-probe", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration) and the float64 SpMV; "
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes")
@@ -150,6 +269,12 @@ def main():
         S.close()
         M.close()
 
+    # ---- learned preconditioner leg: 96^3 log-normal diffusion -----------
+    c4 = None if args.no_c4 else c4_leg(F, gen, args, log)
+
+    # ---- batched leg: 512 independent systems in one persistent launch ----
+    c5 = None if args.no_c5 else c5_leg(F, gen, args, log)
+
     # ---- CPU baseline: the oracle's float64 IC(0)-PCG on this host -------
     cpu = None
     if not args.no_cpu:
@@ -194,6 +319,10 @@ def main():
                      "bytes_per_launch": B64, "avg_launch_us": round(spmv_ms * 1e3, 2)},
         "cpu_baseline": cpu,
     }
+    if c4 is not None:
+        out["config"]["c4"] = c4
+    if c5 is not None:
+        out["config"]["c5"] = c5
     if results.get("ic0_fp64") and results.get("saamg_mp"):
         out["config"]["speedup_mp_vs_fp64_ic0"] = round(results["ic0_fp64"]["solve_ms"] / results["saamg_mp"]["solve_ms"], 2)
     print(json.dumps(out))

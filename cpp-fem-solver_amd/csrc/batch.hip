@@ -1,0 +1,695 @@
+// batch.hip — many independent SPD systems solved in one persistent launch
+// (no counterpart in the reference; each system is solved as the reference's
+// block-Jacobi preconditioned CG, blockMatrix.hpp:714-735 with the cached
+// inverses of test_BlockMatrix_mpi.cpp:132-155).
+//
+// A workgroup takes the next system from an atomic work queue and solves it
+// completely — all right-hand sides together, float64 iterative refinement
+// around a float32 block-Jacobi PCG — with workgroup barriers and LDS
+// reductions only; then it takes the next one. The queue counter only grows,
+// so every workgroup leaves the loop once it passes the number of systems.
+//
+// Layout (all systems concatenated): BSR arrays (row pointers local to each
+// system), float64 and float32 block values, float32 inverse diagonal
+// blocks; vectors interleaved by right-hand side, entry (row i, rhs c) at
+// [i * NR + c], NR = 4 slots (unused slots stay zero).
+//
+// Per outer step: r = b - A x in float64 (converted to float32), inner PCG on
+// A d = r (float32, stop at ||r_k|| < max(inner_tol, tol ||b|| / (2 ||r||))
+// ||r||), x += d in float64; converged when ||b - A x|| < tol ||b|| for every
+// right-hand side. Per inner iteration three passes over the rows:
+//   A: q = A p, partial p.q;  B: d += a p, r -= a q, partial r.r, r.z;
+//   C: p = z + beta p (z = D^{-1} r recomputed).
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+#include "handles.hpp"
+
+namespace fem {
+namespace {
+
+constexpr int NR = 4;  // right-hand-side slots per system
+
+struct BSys {
+    int bs;
+    int nrhs;
+    uint32_t nb;      // block rows
+    uint64_t rp;      // offset into rp (nb + 1 entries, local values)
+    uint64_t ci;      // offset into ci (blocks)
+    uint64_t val;     // offset into values (scalars)
+    uint64_t dinv;    // offset into dinv (scalars)
+    uint64_t vec;     // offset into vectors (scalars * NR)
+    int cap;          // > 0: at most this many inner iterations (autotuner trials)
+    int pad;
+};
+
+struct BOut {
+    int iterations;   // inner iterations (all right-hand sides advance together)
+    int refinements;
+    int converged;    // number of converged right-hand sides
+    int pad;
+    double relres[NR];
+};
+
+struct BArgs {
+    const BSys* sys;
+    const int* order;
+    int nsys;
+    int* queue;
+    const uint32_t* rp;
+    const uint32_t* ci;
+    const double* v64;
+    const float* v32;
+    const float* dinv;
+    const double* b;
+    double* x;
+    float *r, *d, *p, *q;
+    BOut* out;
+    double tol, inner_tol;
+    int max_refine, max_inner;
+};
+
+// deterministic workgroup sum of K doubles per thread: wave butterflies, one
+// LDS row per wave, every thread adds the rows in wave order. `buf` alternates
+// between calls so a buffer is not rewritten before all threads read it.
+template <int BLOCK, int K>
+__device__ __forceinline__ void wg_sum(double (&v)[K], double* red) {
+    constexpr int NW = BLOCK / 64;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+    const int w = threadIdx.x / 64;
+    if ((threadIdx.x & 63) == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int ww = 0; ww < NW; ++ww) s += red[ww * K + k];
+        v[k] = s;
+    }
+}
+
+template <int BS>
+__device__ __forceinline__ void dinv_apply(const float* Di, const float (&r)[BS][NR], float (&z)[BS][NR]) {
+#pragma unroll
+    for (int a = 0; a < BS; ++a)
+#pragma unroll
+        for (int c = 0; c < NR; ++c) {
+            float s = 0.f;
+#pragma unroll
+            for (int b = 0; b < BS; ++b) s += Di[a * BS + b] * r[b][c];
+            z[a][c] = s;
+        }
+}
+
+template <int BS>
+__device__ __forceinline__ void load_blk(const float* v, uint32_t I, float (&o)[BS][NR]) {
+    const float4* p = reinterpret_cast<const float4*>(v + (size_t)I * BS * NR);
+#pragma unroll
+    for (int a = 0; a < BS; ++a) {
+        const float4 t = p[a];
+        o[a][0] = t.x; o[a][1] = t.y; o[a][2] = t.z; o[a][3] = t.w;
+    }
+}
+
+template <int BS>
+__device__ __forceinline__ void store_blk(float* v, uint32_t I, const float (&o)[BS][NR]) {
+    float4* p = reinterpret_cast<float4*>(v + (size_t)I * BS * NR);
+#pragma unroll
+    for (int a = 0; a < BS; ++a) p[a] = make_float4(o[a][0], o[a][1], o[a][2], o[a][3]);
+}
+
+template <int BLOCK, int BS>
+__device__ void solve_system(const BArgs& A, const BSys& S, double* red, BOut* o) {
+    constexpr int BV = BS * BS;
+    constexpr int U = BS == 1 ? 4 : 2;  // rows per thread per step
+    const uint32_t nb = S.nb;
+    const uint32_t* rp = A.rp + S.rp;
+    const uint32_t* ci = A.ci + S.ci;
+    const double* v64 = A.v64 + S.val;
+    const float* v32 = A.v32 + S.val;
+    const float* dv = A.dinv + S.dinv;
+    const double* b = A.b + S.vec;
+    double* x = A.x + S.vec;
+    float* __restrict__ r = A.r + S.vec;
+    float* __restrict__ d = A.d + S.vec;
+    float* __restrict__ p = A.p + S.vec;
+    float* __restrict__ q = A.q + S.vec;
+    const uint32_t tid = threadIdx.x;
+    int rb = 0;  // alternating reduction buffer
+    auto RED = [&]() { double* p_ = red + (rb & 1) * (BLOCK / 64) * 8; ++rb; return p_; };
+
+    // ||b||, x = 0
+    double nb2[NR];
+    for (int c = 0; c < NR; ++c) nb2[c] = 0.0;
+    for (uint32_t i = tid; i < nb * BS; i += BLOCK)
+        for (int c = 0; c < NR; ++c) {
+            const double t = b[(size_t)i * NR + c];
+            nb2[c] += t * t;
+            x[(size_t)i * NR + c] = 0.0;
+        }
+    wg_sum<BLOCK, NR>(nb2, RED());
+    double normb[NR];
+    for (int c = 0; c < NR; ++c) normb[c] = sqrt(nb2[c]);
+    int iters = 0, refs = 0;
+    double relres[NR] = {0, 0, 0, 0};
+    bool done = false;
+    for (int ref = 0; !done; ++ref) {
+        // r = b - A x (float64), stored as float32; ||r||^2
+        __syncthreads();
+        double rn2[NR] = {0, 0, 0, 0};
+        for (uint32_t I = tid; I < nb; I += BLOCK) {
+            double acc[BS][NR];
+            for (int a = 0; a < BS; ++a)
+                for (int c = 0; c < NR; ++c) acc[a][c] = b[((size_t)I * BS + a) * NR + c];
+            for (uint32_t k = rp[I]; k < rp[I + 1]; ++k) {
+                const uint32_t J = ci[k];
+                const double* blk = v64 + (size_t)k * BV;
+                for (int bb = 0; bb < BS; ++bb) {
+                    double xv[NR];
+                    for (int c = 0; c < NR; ++c) xv[c] = x[((size_t)J * BS + bb) * NR + c];
+                    for (int a = 0; a < BS; ++a)
+                        for (int c = 0; c < NR; ++c) acc[a][c] -= blk[a * BS + bb] * xv[c];
+                }
+            }
+            float rf[BS][NR];
+            for (int a = 0; a < BS; ++a)
+                for (int c = 0; c < NR; ++c) {
+                    rn2[c] += acc[a][c] * acc[a][c];
+                    rf[a][c] = (float)acc[a][c];
+                }
+            store_blk<BS>(r, I, rf);
+        }
+        wg_sum<BLOCK, NR>(rn2, RED());
+        bool all = true;
+        float want[NR];
+        bool act[NR];
+        for (int c = 0; c < NR; ++c) {
+            const double rn = sqrt(rn2[c]);
+            relres[c] = normb[c] > 0 ? rn / normb[c] : 0.0;
+            act[c] = c < S.nrhs && normb[c] > 0 && rn >= A.tol * normb[c];
+            if (act[c]) all = false;
+            want[c] = act[c] ? (float)fmax(A.inner_tol, 0.5 * A.tol * normb[c] / rn) : 0.f;
+        }
+        if (all || ref >= A.max_refine) {
+            done = true;
+            break;
+        }
+        ++refs;
+        // inner PCG: d = 0, z = D^-1 r, p = z, rz = r.z, rr0 = ||r||
+        double t2[2 * NR];
+        for (int k = 0; k < 2 * NR; ++k) t2[k] = 0.0;
+        for (uint32_t I = tid; I < nb; I += BLOCK) {
+            float rr[BS][NR], zz[BS][NR], zero[BS][NR];
+            load_blk<BS>(r, I, rr);
+            dinv_apply<BS>(dv + (size_t)I * BV, rr, zz);
+            for (int a = 0; a < BS; ++a)
+                for (int c = 0; c < NR; ++c) {
+                    zero[a][c] = 0.f;
+                    t2[c] += (double)rr[a][c] * zz[a][c];
+                    t2[NR + c] += (double)rr[a][c] * rr[a][c];
+                }
+            store_blk<BS>(p, I, zz);
+            store_blk<BS>(d, I, zero);
+        }
+        wg_sum<BLOCK, 2 * NR>(t2, RED());
+        double rz[NR], rr0[NR];
+        for (int c = 0; c < NR; ++c) {
+            rz[c] = t2[c];
+            rr0[c] = sqrt(t2[NR + c]);
+            if (!(rz[c] > 0.0)) act[c] = false;
+        }
+        const int max_inner = S.cap > 0 ? min(A.max_inner, S.cap) : A.max_inner;
+        for (int it = 0; it < max_inner; ++it) {
+            bool any = false;
+            for (int c = 0; c < NR; ++c) any |= act[c];
+            if (!any) break;
+            ++iters;
+            __syncthreads();  // p complete
+            // pass A: q = A p, p.q. U rows per thread at a time: their row
+            // pointers, entries and gathers are all issued before any use,
+            // so a thread keeps U dependent load chains in flight.
+            double pq[NR] = {0, 0, 0, 0};
+            for (uint32_t base = tid; base < nb; base += U * BLOCK) {
+                uint32_t k0[U], k1[U], len = 0;
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t I = base + u * BLOCK;
+                    k0[u] = I < nb ? rp[I] : 0u;
+                    k1[u] = I < nb ? rp[I + 1] : 0u;
+                    len = max(len, k1[u] - k0[u]);
+                }
+                float acc[U][BS][NR];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    for (int a = 0; a < BS; ++a)
+                        for (int c = 0; c < NR; ++c) acc[u][a][c] = 0.f;
+                for (uint32_t t = 0; t < len; ++t) {
+#pragma unroll
+                    for (int u = 0; u < U; ++u) {
+                        if (k0[u] + t >= k1[u]) continue;
+                        const uint32_t k = k0[u] + t;
+                        const uint32_t J = ci[k];
+                        const float* blk = v32 + (size_t)k * BV;
+                        float pj[BS][NR];
+                        load_blk<BS>(p, J, pj);
+                        for (int a = 0; a < BS; ++a)
+                            for (int bb = 0; bb < BS; ++bb)
+                                for (int c = 0; c < NR; ++c) acc[u][a][c] += blk[a * BS + bb] * pj[bb][c];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t I = base + u * BLOCK;
+                    if (I >= nb) continue;
+                    float pi[BS][NR];
+                    load_blk<BS>(p, I, pi);
+                    for (int a = 0; a < BS; ++a)
+                        for (int c = 0; c < NR; ++c) pq[c] += (double)pi[a][c] * acc[u][a][c];
+                    store_blk<BS>(q, I, acc[u]);
+                }
+            }
+            wg_sum<BLOCK, NR>(pq, RED());
+            float alpha[NR];
+            for (int c = 0; c < NR; ++c) {
+                if (act[c] && !(pq[c] > 0.0)) act[c] = false;  // breakdown: keep the iterate
+                alpha[c] = act[c] ? (float)(rz[c] / pq[c]) : 0.f;
+            }
+            // pass B: d += a p, r -= a q, r.r, r.z (U rows per step, loads first)
+            double t[2 * NR];
+            for (int k = 0; k < 2 * NR; ++k) t[k] = 0.0;
+            for (uint32_t base = tid; base < nb; base += U * BLOCK) {
+                float pi[U][BS][NR], qi[U][BS][NR], ri[U][BS][NR], di[U][BS][NR];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t I = base + u * BLOCK;
+                    if (I >= nb) continue;
+                    load_blk<BS>(p, I, pi[u]);
+                    load_blk<BS>(q, I, qi[u]);
+                    load_blk<BS>(r, I, ri[u]);
+                    load_blk<BS>(d, I, di[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t I = base + u * BLOCK;
+                    if (I >= nb) continue;
+                    float zz[BS][NR];
+                    for (int a = 0; a < BS; ++a)
+                        for (int c = 0; c < NR; ++c) {
+                            di[u][a][c] += alpha[c] * pi[u][a][c];
+                            ri[u][a][c] -= alpha[c] * qi[u][a][c];
+                        }
+                    dinv_apply<BS>(dv + (size_t)I * BV, ri[u], zz);
+                    for (int a = 0; a < BS; ++a)
+                        for (int c = 0; c < NR; ++c) {
+                            t[c] += (double)ri[u][a][c] * ri[u][a][c];
+                            t[NR + c] += (double)ri[u][a][c] * zz[a][c];
+                        }
+                    store_blk<BS>(d, I, di[u]);
+                    store_blk<BS>(r, I, ri[u]);
+                }
+            }
+            wg_sum<BLOCK, 2 * NR>(t, RED());
+            float beta[NR];
+            for (int c = 0; c < NR; ++c) {
+                if (act[c] && sqrt(t[c]) < want[c] * rr0[c]) act[c] = false;
+                beta[c] = (act[c] && rz[c] != 0.0) ? (float)(t[NR + c] / rz[c]) : 0.f;
+                if (act[c]) rz[c] = t[NR + c];
+                if (act[c] && !(rz[c] > 0.0)) act[c] = false;
+            }
+            // pass C: p = z + beta p (rows of finished right-hand sides keep p)
+            for (uint32_t base = tid; base < nb; base += U * BLOCK) {
+                float ri[U][BS][NR], pi[U][BS][NR];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t I = base + u * BLOCK;
+                    if (I >= nb) continue;
+                    load_blk<BS>(r, I, ri[u]);
+                    load_blk<BS>(p, I, pi[u]);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const uint32_t I = base + u * BLOCK;
+                    if (I >= nb) continue;
+                    float zz[BS][NR];
+                    dinv_apply<BS>(dv + (size_t)I * BV, ri[u], zz);
+                    for (int a = 0; a < BS; ++a)
+                        for (int c = 0; c < NR; ++c) pi[u][a][c] = act[c] ? zz[a][c] + beta[c] * pi[u][a][c] : pi[u][a][c];
+                    store_blk<BS>(p, I, pi[u]);
+                }
+            }
+        }
+        __syncthreads();
+        // x += d (float64)
+        for (uint32_t i = tid; i < nb * BS; i += BLOCK)
+            for (int c = 0; c < NR; ++c) x[(size_t)i * NR + c] += (double)d[(size_t)i * NR + c];
+    }
+    if (tid == 0) {
+        o->iterations = iters;
+        o->refinements = refs;
+        int conv = 0;
+        for (int c = 0; c < S.nrhs; ++c) conv += (relres[c] < A.tol) ? 1 : 0;
+        o->converged = conv;
+        for (int c = 0; c < NR; ++c) o->relres[c] = relres[c];
+    }
+    __syncthreads();
+}
+
+template <int BLOCK>
+__global__ __launch_bounds__(BLOCK) void k_batch_pcg(BArgs a) {
+    __shared__ double red[2 * (BLOCK / 64) * 8];
+    __shared__ int slot;
+    while (true) {
+        __syncthreads();
+        if (threadIdx.x == 0) slot = atomicAdd(a.queue, 1);
+        __syncthreads();
+        const int k = slot;
+        if (k >= a.nsys) break;
+        const int s = a.order[k];
+        const BSys S = a.sys[s];
+        if (S.bs == 3) solve_system<BLOCK, 3>(a, S, red, a.out + s);
+        else solve_system<BLOCK, 1>(a, S, red, a.out + s);
+    }
+}
+
+// host-side Gauss-Jordan inverse of a BS x BS block (blockMatrix.hpp:70-100)
+template <int BS>
+bool blk_inv(const double* M, double* Mi) {
+    double a[BS][2 * BS];
+    for (int i = 0; i < BS; ++i)
+        for (int j = 0; j < 2 * BS; ++j) a[i][j] = j < BS ? M[i * BS + j] : (j - BS == i ? 1.0 : 0.0);
+    for (int c = 0; c < BS; ++c) {
+        int piv = c;
+        for (int i = c + 1; i < BS; ++i)
+            if (std::fabs(a[i][c]) > std::fabs(a[piv][c])) piv = i;
+        if (std::fabs(a[piv][c]) < 1e-300) return false;
+        if (piv != c)
+            for (int j = 0; j < 2 * BS; ++j) std::swap(a[c][j], a[piv][j]);
+        const double inv = 1.0 / a[c][c];
+        for (int j = 0; j < 2 * BS; ++j) a[c][j] *= inv;
+        for (int i = 0; i < BS; ++i) {
+            if (i == c) continue;
+            const double f = a[i][c];
+            for (int j = 0; j < 2 * BS; ++j) a[i][j] -= f * a[c][j];
+        }
+    }
+    for (int i = 0; i < BS; ++i)
+        for (int j = 0; j < BS; ++j) Mi[i * BS + j] = a[i][BS + j];
+    return true;
+}
+
+}  // namespace
+}  // namespace fem
+
+using namespace fem;
+
+struct fem_batch_s {
+    hipStream_t stream = nullptr;
+    int nsys = 0;
+    std::vector<BSys> h_sys;
+    std::vector<double> work;           // work estimate per system (for ordering)
+    DevBuf<BSys> sys;
+    DevBuf<int> order, queue;
+    DevBuf<uint32_t> rp, ci;
+    DevBuf<double> v64, b, x;
+    DevBuf<float> v32, dinv, r, d, p, q;
+    DevBuf<BOut> out;
+    uint64_t nvec = 0;                  // scalars * NR
+    // schedule: block size, workgroups per CU, queue order (0 largest work
+    // first, 1 input order)
+    int block = 256, wg_per_cu = 0, order_kind = 0;
+    ~fem_batch_s() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+namespace {
+
+void set_order(fem_batch_s* h, int kind) {
+    std::vector<int> ord(h->nsys);
+    std::iota(ord.begin(), ord.end(), 0);
+    if (kind == 0)
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return h->work[a] > h->work[b]; });
+    h->order.upload(ord, h->stream);
+    h->order_kind = kind;
+}
+
+template <int BLOCK>
+int max_wg_per_cu() {
+    int nb = 0;
+    FEM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void*)k_batch_pcg<BLOCK>, BLOCK, 0));
+    return std::max(1, nb);
+}
+
+int cu_count() {
+    int dev = 0, cu = 0;
+    FEM_HIP(hipGetDevice(&dev));
+    FEM_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+    return cu;
+}
+
+// one persistent launch over systems sys[0..nsys) (queue reset first); returns ms
+double launch(fem_batch_s* h, const BSys* sys, const int* order, int nsys, BOut* out, double tol, double inner_tol,
+              int max_refine, int max_inner) {
+    hipStream_t s = h->stream;
+    vec_fill<uint32_t>((uint32_t*)h->queue.get(), 0u, 1, s);
+    BArgs a{sys, order, nsys, h->queue.get(), h->rp.get(), h->ci.get(), h->v64.get(), h->v32.get(), h->dinv.get(),
+            h->b.get(), h->x.get(), h->r.get(), h->d.get(), h->p.get(), h->q.get(), out, tol, inner_tol, max_refine,
+            max_inner};
+    int per = h->wg_per_cu;
+    int cap;
+    switch (h->block) {
+        case 512: cap = max_wg_per_cu<512>(); break;
+        case 1024: cap = max_wg_per_cu<1024>(); break;
+        default: cap = max_wg_per_cu<256>(); break;
+    }
+    if (per <= 0 || per > cap) per = cap;
+    const unsigned grid = (unsigned)std::max(1, std::min(nsys, per * cu_count()));
+    hipEvent_t e0, e1;
+    FEM_HIP(hipEventCreate(&e0));
+    FEM_HIP(hipEventCreate(&e1));
+    FEM_HIP(hipEventRecord(e0, s));
+    switch (h->block) {
+        case 512: hipLaunchKernelGGL(k_batch_pcg<512>, dim3(grid), dim3(512), 0, s, a); break;
+        case 1024: hipLaunchKernelGGL(k_batch_pcg<1024>, dim3(grid), dim3(1024), 0, s, a); break;
+        default: hipLaunchKernelGGL(k_batch_pcg<256>, dim3(grid), dim3(256), 0, s, a); break;
+    }
+    FEM_HIP(hipGetLastError());
+    FEM_HIP(hipEventRecord(e1, s));
+    FEM_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fem_batch_create(int nsys, const fem_batch_system* systems, int nrhs, fem_batch* out) {
+    return api_guard([&] {
+        if (!systems || !out || nsys < 1) fail(FEM_ERR_INVALID_ARGUMENT, "need at least one system");
+        if (nrhs < 1 || nrhs > NR) fail(FEM_ERR_INVALID_ARGUMENT, "nrhs must be 1..4");
+        ensure_device();
+        auto h = std::make_unique<fem_batch_s>();
+        h->stream = new_stream();
+        h->nsys = nsys;
+        std::vector<uint32_t> rp, ci;
+        std::vector<double> v64, dv;
+        uint64_t vec = 0;
+        for (int k = 0; k < nsys; ++k) {
+            const fem_batch_system& in = systems[k];
+            const int bs = in.bs;
+            if (bs != 1 && bs != 3) fail(FEM_ERR_INVALID_ARGUMENT, "batch systems need block size 1 or 3");
+            if (!in.row_ptr || !in.col_idx || !in.values || in.nb == 0)
+                fail(FEM_ERR_INVALID_ARGUMENT, "system " + std::to_string(k) + ": empty or null arrays");
+            if (in.row_ptr[0] != 0) fail(FEM_ERR_INVALID_ARGUMENT, "row_ptr must start at 0");
+            const uint32_t nb = in.nb;
+            const uint64_t nnzb = in.row_ptr[nb];
+            BSys S{bs, nrhs, nb, rp.size(), ci.size(), v64.size(), dv.size(), vec, 0, 0};
+            const int BV = bs * bs;
+            for (uint32_t I = 0; I <= nb; ++I) rp.push_back(in.row_ptr[I]);
+            std::vector<double> inv((size_t)nb * BV, 0.0);
+            for (uint32_t I = 0; I < nb; ++I) {
+                if (in.row_ptr[I + 1] < in.row_ptr[I]) fail(FEM_ERR_INVALID_ARGUMENT, "row_ptr must be non-decreasing");
+                int64_t dp = -1;
+                for (uint32_t t = in.row_ptr[I]; t < in.row_ptr[I + 1]; ++t) {
+                    if (in.col_idx[t] >= nb)
+                        fail(FEM_ERR_OUT_OF_RANGE, "system " + std::to_string(k) + ": column index out of range");
+                    if (in.col_idx[t] == I) dp = t;
+                }
+                if (dp < 0) fail(FEM_ERR_RUNTIME, "system " + std::to_string(k) + ": block row " + std::to_string(I) + " has no diagonal block");
+                const bool ok = bs == 3 ? blk_inv<3>(in.values + (size_t)dp * BV, &inv[(size_t)I * BV])
+                                        : blk_inv<1>(in.values + (size_t)dp * BV, &inv[(size_t)I * BV]);
+                if (!ok) fail(FEM_ERR_RUNTIME, "system " + std::to_string(k) + ": singular diagonal block " + std::to_string(I));
+            }
+            ci.insert(ci.end(), in.col_idx, in.col_idx + nnzb);
+            v64.insert(v64.end(), in.values, in.values + nnzb * BV);
+            dv.insert(dv.end(), inv.begin(), inv.end());
+            vec += (uint64_t)nb * bs * NR;
+            h->h_sys.push_back(S);
+            // work estimate: matrix entries x sqrt(rows) (CG iterations grow
+            // with the square root of the condition number)
+            h->work.push_back((double)nnzb * BV * std::sqrt((double)nb * bs));
+        }
+        hipStream_t s = h->stream;
+        h->nvec = vec;
+        h->sys.upload(h->h_sys, s);
+        h->rp.upload(rp, s);
+        h->ci.upload(ci, s);
+        h->v64.upload(v64, s);
+        std::vector<float> f32(v64.begin(), v64.end()), d32(dv.begin(), dv.end());
+        h->v32.upload(f32, s);
+        h->dinv.upload(d32, s);
+        h->b.alloc(vec);
+        h->x.alloc(vec);
+        h->r.alloc(vec);
+        h->d.alloc(vec);
+        h->p.alloc(vec);
+        h->q.alloc(vec);
+        vec_fill<double>(h->b.get(), 0.0, vec, s);
+        vec_fill<float>(h->r.get(), 0.f, vec, s);
+        vec_fill<float>(h->d.get(), 0.f, vec, s);
+        vec_fill<float>(h->p.get(), 0.f, vec, s);
+        vec_fill<float>(h->q.get(), 0.f, vec, s);
+        h->out.alloc(nsys);
+        h->queue.alloc(1);
+        set_order(h.get(), 0);
+        FEM_HIP(hipStreamSynchronize(s));
+        *out = h.release();
+    });
+}
+
+int fem_batch_destroy(fem_batch h) {
+    return api_guard([&] { delete h; });
+}
+
+int fem_batch_set_rhs(fem_batch h, const double* B) {
+    return api_guard([&] {
+        if (!h || !B) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
+        std::vector<double> v(h->nvec, 0.0);
+        const double* src = B;
+        for (const BSys& S : h->h_sys) {
+            const uint64_t n = (uint64_t)S.nb * S.bs;
+            for (int c = 0; c < S.nrhs; ++c)
+                for (uint64_t i = 0; i < n; ++i) v[S.vec + i * NR + c] = src[(uint64_t)c * n + i];
+            src += (uint64_t)S.nrhs * n;
+        }
+        h->b.upload(v, h->stream);
+        FEM_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+int fem_batch_get_solution(fem_batch h, double* X) {
+    return api_guard([&] {
+        if (!h || !X) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
+        std::vector<double> v(h->nvec);
+        h->x.download(v.data(), h->nvec, h->stream);
+        FEM_HIP(hipStreamSynchronize(h->stream));
+        double* dst = X;
+        for (const BSys& S : h->h_sys) {
+            const uint64_t n = (uint64_t)S.nb * S.bs;
+            for (int c = 0; c < S.nrhs; ++c)
+                for (uint64_t i = 0; i < n; ++i) dst[(uint64_t)c * n + i] = v[S.vec + i * NR + c];
+            dst += (uint64_t)S.nrhs * n;
+        }
+    });
+}
+
+int fem_batch_solve(fem_batch h, const fem_batch_opts* opts, fem_batch_stats* per_system, double* ms) {
+    return api_guard([&] {
+        if (!h || !opts) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
+        if (!(opts->tol > 0) || opts->max_refine < 1 || opts->max_inner < 1)
+            fail(FEM_ERR_INVALID_ARGUMENT, "tol > 0, max_refine >= 1 and max_inner >= 1 are required");
+        const double t = launch(h, h->sys.get(), h->order.get(), h->nsys, h->out.get(), opts->tol, opts->inner_tol,
+                                opts->max_refine, opts->max_inner);
+        if (ms) *ms = t;
+        if (per_system) {
+            std::vector<BOut> o(h->nsys);
+            h->out.download(o.data(), h->nsys, h->stream);
+            FEM_HIP(hipStreamSynchronize(h->stream));
+            for (int k = 0; k < h->nsys; ++k) {
+                per_system[k].iterations = o[k].iterations;
+                per_system[k].refinements = o[k].refinements;
+                per_system[k].converged = o[k].converged;
+                double w = 0.0;
+                for (int c = 0; c < h->h_sys[k].nrhs; ++c) w = std::max(w, o[k].relres[c]);
+                per_system[k].max_relres = w;
+            }
+        }
+    });
+}
+
+// Autotuner (profile-guided): one solve with the current schedule records
+// every system's inner iterations; then each schedule (block size
+// 256/512/1024 x workgroups per CU 1/2/occupancy limit x queue order) runs
+// the whole batch with one refinement step of iterations_k / divisor inner
+// iterations per system, which keeps the real mix of long and short systems
+// (and so the makespan) at 1/divisor of the cost. Returns the fastest.
+int fem_batch_tune(fem_batch h, const fem_batch_opts* opts, int divisor, fem_batch_schedule* best, double* trial_ms,
+                   int cap) {
+    return api_guard([&] {
+        if (!h || !opts) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
+        if (divisor < 1) divisor = 8;
+        launch(h, h->sys.get(), h->order.get(), h->nsys, h->out.get(), opts->tol, opts->inner_tol, opts->max_refine,
+               opts->max_inner);
+        std::vector<BOut> o(h->nsys);
+        h->out.download(o.data(), h->nsys, h->stream);
+        FEM_HIP(hipStreamSynchronize(h->stream));
+        std::vector<BSys> tsys = h->h_sys;
+        for (int k = 0; k < h->nsys; ++k) tsys[k].cap = std::max(1, o[k].iterations / divisor);
+        DevBuf<BSys> dsys;
+        dsys.upload(tsys, h->stream);
+        std::vector<int> ord0(h->nsys), ord1(h->nsys);
+        std::iota(ord1.begin(), ord1.end(), 0);
+        ord0 = ord1;
+        std::stable_sort(ord0.begin(), ord0.end(), [&](int a, int b) { return h->work[a] > h->work[b]; });
+        DevBuf<int> dord0, dord1;
+        dord0.upload(ord0, h->stream);
+        dord1.upload(ord1, h->stream);
+        DevBuf<BOut> dout(h->nsys);
+        const int blocks[3] = {256, 512, 1024};
+        const int pers[3] = {1, 2, 0};
+        double best_ms = 1e300;
+        fem_batch_schedule bs{256, 0, 0};
+        int t = 0;
+        const int keep_block = h->block, keep_per = h->wg_per_cu;
+        for (int bi = 0; bi < 3; ++bi)
+            for (int pi = 0; pi < 3; ++pi)
+                for (int ok = 0; ok < 2; ++ok) {
+                    h->block = blocks[bi];
+                    h->wg_per_cu = pers[pi];
+                    const double ms = launch(h, dsys.get(), ok == 0 ? dord0.get() : dord1.get(), h->nsys, dout.get(),
+                                             opts->tol, opts->inner_tol, 1, opts->max_inner);
+                    if (trial_ms && t < cap) trial_ms[t] = ms;
+                    ++t;
+                    if (ms < best_ms) {
+                        best_ms = ms;
+                        bs = fem_batch_schedule{blocks[bi], pers[pi], ok};
+                    }
+                }
+        h->block = keep_block;
+        h->wg_per_cu = keep_per;
+        if (best) *best = bs;
+    });
+}
+
+int fem_batch_set_schedule(fem_batch h, const fem_batch_schedule* sch) {
+    return api_guard([&] {
+        if (!h || !sch) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
+        if (sch->block != 256 && sch->block != 512 && sch->block != 1024)
+            fail(FEM_ERR_INVALID_ARGUMENT, "block must be 256, 512 or 1024");
+        if (sch->order != 0 && sch->order != 1) fail(FEM_ERR_INVALID_ARGUMENT, "order must be 0 or 1");
+        h->block = sch->block;
+        h->wg_per_cu = std::max(0, sch->wg_per_cu);
+        set_order(h, sch->order);
+        FEM_HIP(hipStreamSynchronize(h->stream));
+    });
+}
+
+}  // extern "C"

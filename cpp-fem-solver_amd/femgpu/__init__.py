@@ -69,6 +69,24 @@ class SolveStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BatchSystem(C.Structure):
+    _fields_ = [("bs", C.c_int), ("nb", C.c_uint32), ("row_ptr", C.POINTER(C.c_uint32)),
+                ("col_idx", C.POINTER(C.c_uint32)), ("values", C.POINTER(C.c_double))]
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("inner_tol", C.c_double), ("max_refine", C.c_int), ("max_inner", C.c_int)]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("converged", C.c_int),
+                ("max_relres", C.c_double)]
+
+
+class BatchSchedule(C.Structure):
+    _fields_ = [("block", C.c_int), ("wg_per_cu", C.c_int), ("order", C.c_int)]
+
+
 class AdamOpts(C.Structure):
     _fields_ = [("lr", C.c_double), ("beta1", C.c_double), ("beta2", C.c_double), ("eps", C.c_double)]
 
@@ -130,6 +148,13 @@ _SIGS = {
     "fem_spai_loss_grad": (C.c_int, [P_vp, P_vp, P_u32, P_f64, C.c_int, C.c_int, P_f64, P_f32]),
     "fem_spai_train_step": (C.c_int, [P_vp, P_vp, P_u32, P_f64, C.c_int, C.c_int, C.POINTER(AdamOpts), P_f64]),
     "fem_precond_create_learned": (C.c_int, [P_vp, P_u32, P_vp, C.c_int, C.c_int, C.POINTER(P_vp)]),
+    "fem_batch_create": (C.c_int, [C.c_int, C.POINTER(BatchSystem), C.c_int, C.POINTER(P_vp)]),
+    "fem_batch_set_rhs": (C.c_int, [P_vp, P_f64]),
+    "fem_batch_solve": (C.c_int, [P_vp, C.POINTER(BatchOpts), C.POINTER(BatchStats), P_f64]),
+    "fem_batch_get_solution": (C.c_int, [P_vp, P_f64]),
+    "fem_batch_tune": (C.c_int, [P_vp, C.POINTER(BatchOpts), C.c_int, C.POINTER(BatchSchedule), P_f64, C.c_int]),
+    "fem_batch_set_schedule": (C.c_int, [P_vp, C.POINTER(BatchSchedule)]),
+    "fem_batch_destroy": (C.c_int, [P_vp]),
 }
 
 
@@ -551,3 +576,72 @@ def spai_features(A, grid):
     g = _u32(grid)
     _check(lib().fem_spai_features(A._h, _p(g, C.c_uint32), _p(f, C.c_float)))
     return f
+
+
+# ---- batched solves ------------------------------------------------------------
+class Batch:
+    """Independent SPD systems solved in one persistent launch (fem_batch_*).
+    systems: gen.CSR (block size 1) or gen.BSR (block size 3) matrices."""
+
+    def __init__(self, systems, nrhs=4):
+        keep = []
+        arr = (BatchSystem * len(systems))()
+        self.sizes = []
+        for k, A in enumerate(systems):
+            if hasattr(A, "bs"):
+                bs, nb, rp, ci, v = A.bs, A.nbr, A.row_ptr, A.col_idx, A.values
+            else:
+                bs, nb, rp, ci, v = 1, A.n, A.row_ptr, A.col_idx, A.values
+            rp, ci, v = _u32(rp), _u32(ci), _f64(v)
+            keep += [rp, ci, v]
+            arr[k] = BatchSystem(bs, nb, _p(rp, C.c_uint32), _p(ci, C.c_uint32), _p(v, C.c_double))
+            self.sizes.append(nb * bs)
+        self.nrhs = nrhs
+        self._h = C.c_void_p()
+        _check(lib().fem_batch_create(len(systems), arr, nrhs, C.byref(self._h)))
+
+    def set_rhs(self, Bs):
+        """Bs: one (nrhs, n) array per system."""
+        flat = _f64(np.concatenate([np.asarray(B, np.float64).reshape(self.nrhs, -1).ravel() for B in Bs]))
+        _check(lib().fem_batch_set_rhs(self._h, _p(flat, C.c_double)))
+
+    def solve(self, tol=1e-10, inner_tol=1e-4, max_refine=40, max_inner=20000):
+        st = (BatchStats * len(self.sizes))()
+        ms = C.c_double()
+        o = BatchOpts(tol, inner_tol, max_refine, max_inner)
+        _check(lib().fem_batch_solve(self._h, C.byref(o), st, C.byref(ms)))
+        stats = [{k: getattr(x, k) for k, _ in BatchStats._fields_} for x in st]
+        return stats, ms.value
+
+    def solution(self):
+        X = np.empty(sum(self.sizes) * self.nrhs)
+        _check(lib().fem_batch_get_solution(self._h, _p(X, C.c_double)))
+        out, o = [], 0
+        for n in self.sizes:
+            out.append(X[o:o + n * self.nrhs].reshape(self.nrhs, n))
+            o += n * self.nrhs
+        return out
+
+    def tune(self, divisor=8, tol=1e-10, inner_tol=1e-4, max_refine=40, max_inner=20000):
+        """Profile-guided schedule search (fem_batch_tune); returns the best
+        schedule and the 18 trial times (ms)."""
+        best = BatchSchedule()
+        t = np.zeros(32)
+        o = BatchOpts(tol, inner_tol, max_refine, max_inner)
+        _check(lib().fem_batch_tune(self._h, C.byref(o), divisor, C.byref(best), _p(t, C.c_double), 32))
+        return {"block": best.block, "wg_per_cu": best.wg_per_cu, "order": best.order}, t[:18]
+
+    def set_schedule(self, block=256, wg_per_cu=0, order=0):
+        s = BatchSchedule(block, wg_per_cu, order)
+        _check(lib().fem_batch_set_schedule(self._h, C.byref(s)))
+
+    def close(self):
+        if self._h:
+            lib().fem_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -45,11 +45,9 @@ class CSR:
         return int(self.values.size)
 
     def matvec(self, x: np.ndarray) -> np.ndarray:
-        """y = A x with a per-row sequential sum in stored order."""
-        import scipy.sparse as sp
-        A = sp.csr_matrix((self.values, self.col_idx.astype(np.int64), self.row_ptr.astype(np.int64)),
-                          shape=(self.n, self.m))
-        return A @ x
+        """y = A x with a per-row sequential sum in stored order (x may hold
+        several vectors as columns)."""
+        return self.to_scipy() @ x
 
     def to_scipy(self):
         import scipy.sparse as sp
@@ -224,6 +222,12 @@ class BSR:
     def nnzb(self) -> int:
         return int(self.col_idx.size)
 
+    def to_scipy(self):
+        import scipy.sparse as sp
+        bs = self.bs
+        return sp.bsr_matrix((self.values.reshape(-1, bs, bs), self.col_idx.astype(np.int64),
+                              self.row_ptr.astype(np.int64)), shape=(self.nbr * bs, self.nbc * bs))
+
     def to_csr(self) -> CSR:
         """Flatten to scalar CSR in node-major numbering (block row b, row r
         -> scalar row b*bs + r)."""
@@ -287,7 +291,7 @@ def rhs(A: CSR, seed: int, nrhs: int | None = None) -> tuple[np.ndarray, np.ndar
         x = rng.uniform(-1.0, 1.0, A.n)
         return x, A.matvec(x)
     X = rng.uniform(-1.0, 1.0, (nrhs, A.n))
-    return X, np.stack([A.matvec(x) for x in X])
+    return X, np.ascontiguousarray((A.to_scipy() @ X.T).T)
 
 
 def batch_c5(seed: int = 0, count: int = 512, poisson_sides=(32, 256), bsr_sides=(16, 48), nrhs: int = 4):
@@ -306,7 +310,7 @@ def batch_c5(seed: int = 0, count: int = 512, poisson_sides=(32, 256), bsr_sides
         out.append(dict(kind="poisson2d", side=int(s), A=A, X=X, B=B))
     for k, t in enumerate(ts):
         Ab = bsr7(int(t), seed * 100003 + half + k)
-        As = Ab.to_csr()
-        X, B = rhs(As, seed * 100003 + half + k + 7, nrhs)
+        X = np.random.default_rng(seed * 100003 + half + k + 7).uniform(-1.0, 1.0, (nrhs, Ab.nbr * Ab.bs))
+        B = np.ascontiguousarray((Ab.to_scipy() @ X.T).T)
         out.append(dict(kind="bsr3", side=int(t), A=Ab, X=X, B=B))
     return out

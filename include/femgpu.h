@@ -266,6 +266,58 @@ int fem_spai_train_step(fem_mlp net, fem_csr A, const uint32_t* grid, const doub
 int fem_precond_create_learned(fem_csr A, const uint32_t* grid, fem_mlp net, int fallback_kind, int budget,
                                fem_pc* out);
 
+/* ---- batched solves ----------------------------------------------------------
+ * No counterpart in the reference: many independent SPD systems (block size
+ * 1 or 3, up to 4 right-hand sides each) solved in one persistent launch by
+ * block-Jacobi PCG (the reference's block-Jacobi, blockMatrix.hpp:714-735,
+ * with cached inverse blocks as in test_BlockMatrix_mpi.cpp:132-155) in
+ * float32 inside float64 iterative refinement. A workgroup solves one
+ * system at a time, all its right-hand sides together, and takes the next
+ * system from a work queue. Systems are BSR with node-major vectors. */
+typedef struct fem_batch_s* fem_batch;
+typedef struct fem_batch_system {
+    int bs;                   /* 1 or 3 */
+    uint32_t nb;              /* block rows (= block columns) */
+    const uint32_t* row_ptr;  /* nb + 1 */
+    const uint32_t* col_idx;  /* row_ptr[nb] block columns */
+    const double* values;     /* row_ptr[nb] row-major bs x bs blocks */
+} fem_batch_system;
+typedef struct fem_batch_opts {
+    double tol;               /* ||b - A x|| < tol ||b|| per right-hand side */
+    double inner_tol;         /* float32 inner solves: relative reduction */
+    int max_refine;           /* float64 refinement steps */
+    int max_inner;            /* inner iterations per refinement step */
+} fem_batch_opts;
+typedef struct fem_batch_stats {
+    int iterations;           /* inner iterations (all right-hand sides together) */
+    int refinements;
+    int converged;            /* right-hand sides that reached tol */
+    double max_relres;        /* largest final ||b - A x|| / ||b|| (float64) */
+} fem_batch_stats;
+typedef struct fem_batch_schedule {
+    int block;                /* workgroup size: 256, 512 or 1024 */
+    int wg_per_cu;            /* resident workgroups per CU (0: occupancy limit) */
+    int order;                /* queue: 0 largest estimated work first, 1 input order */
+} fem_batch_schedule;
+/* copies the systems to the device and inverts the diagonal blocks */
+int fem_batch_create(int nsys, const fem_batch_system* systems, int nrhs, fem_batch* out);
+/* right-hand sides: for each system in turn, nrhs vectors of its nb*bs rows */
+int fem_batch_set_rhs(fem_batch b, const double* B);
+/* solves every system from x0 = 0; per_system (nsys entries) may be NULL;
+ * *ms = device time of the persistent launch */
+int fem_batch_solve(fem_batch b, const fem_batch_opts* opts, fem_batch_stats* per_system, double* ms);
+/* solutions, same layout as the right-hand sides */
+int fem_batch_get_solution(fem_batch b, double* X);
+/* schedule autotuner (profile-guided): solves once with the current schedule
+ * to record every system's iterations, then times each of the 18 schedules
+ * on the whole batch with every system capped at 1/divisor of its recorded
+ * iterations (trial_ms receives up to cap timings) and returns the fastest;
+ * apply it with fem_batch_set_schedule */
+int fem_batch_tune(fem_batch b, const fem_batch_opts* opts, int divisor, fem_batch_schedule* best, double* trial_ms,
+                   int cap);
+int fem_batch_set_schedule(fem_batch b, const fem_batch_schedule* schedule);
+int fem_batch_destroy(fem_batch b);
+
 /* ---- measurement --------------------------------------------------------- */
 /* Average device time of `reps` back-to-back SpMV launches on A's stream,
  * timed with HIP events (inputs resident in HBM). */
