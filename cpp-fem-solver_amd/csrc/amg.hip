@@ -189,6 +189,7 @@ struct AmgLevelDev {
 
 struct PcAmg : Precond {
     int kind_;
+    bool device_setup = false;  // hierarchy built by build_sa_hierarchy_device
     bool inv_diag;  // SA smoother multiplies by the inverse diagonal
     int smoother, npre, npost;
     double omega;
@@ -480,8 +481,24 @@ std::unique_ptr<Precond> make_amg_cf(CsrMat& A, const fem_pc_params& p, hipStrea
 std::unique_ptr<Precond> make_saamg(CsrMat& A, const fem_pc_params& p, hipStream_t s) {
     if (A.n != A.m) fail(FEM_ERR_INVALID_ARGUMENT, "Block AMG needs a square matrix");
     A.sync_host();
-    AmgHierarchy H = build_sa_hierarchy(A, p);
-    return std::unique_ptr<Precond>(new PcAmg(FEM_PC_SAAMG, std::move(H), p, s));
+    // the robust variant is built on the device when asked for and the rows
+    // are sorted and duplicate free (the host path condenses them first)
+    bool dev = p.p_omega <= 0.0 && p.setup_device != 0;
+    for (uint32_t i = 0; dev && i < A.n; ++i)
+        for (uint32_t q = A.h_rp[i] + 1; q < A.h_rp[i + 1]; ++q)
+            if (A.h_ci[q] <= A.h_ci[q - 1]) { dev = false; break; }
+    AmgHierarchy H;
+    if (dev) {
+        try {
+            H = build_sa_hierarchy_device(A, p, s);
+        } catch (const DeviceSetupTooLarge&) {
+            dev = false;
+        }
+    }
+    if (!dev) H = build_sa_hierarchy(A, p);
+    auto pc = std::unique_ptr<PcAmg>(new PcAmg(FEM_PC_SAAMG, std::move(H), p, s));
+    pc->device_setup = dev;
+    return pc;
 }
 
 }  // namespace fem

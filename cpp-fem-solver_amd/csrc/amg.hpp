@@ -36,5 +36,13 @@ struct AmgHierarchy {
 AmgHierarchy build_cf_hierarchy(const CsrMat& A, const fem_pc_params& p);
 AmgHierarchy build_sa_hierarchy(const CsrMat& A, const fem_pc_params& p);
 int mis2_aggregate(const HCsr& A, double theta, std::vector<int>& agg);
+// the robust SA hierarchy (p_omega <= 0) built on the device (amg_device.hip);
+// identical to build_sa_hierarchy's. Throws DeviceSetupTooLarge when a level
+// has more columns than the product's LDS bitmap holds (524288).
+struct DeviceSetupTooLarge {};
+AmgHierarchy build_sa_hierarchy_device(const CsrMat& A, const fem_pc_params& p, hipStream_t s);
+// host parts shared by both setups
+std::vector<double> sa_coarse_operator_host(const HCsr& A, bool exact);
+void sgs_levels_host(AmgHierarchy& H);
 
 }  // namespace fem

@@ -109,7 +109,7 @@ static void colour(const HCsr& A, std::vector<uint32_t>& cptr, std::vector<uint3
 
 // SGS smoother data for every level but the coarsest: the multicolour
 // ordering, the diagonal and the damped-Jacobi fallback weight.
-static void sgs_levels(AmgHierarchy& H) {
+void sgs_levels_host(AmgHierarchy& H) {
     for (size_t k = 0; k + 1 < H.A.size(); ++k) {
         H.cptr.emplace_back();
         H.crows.emplace_back();
@@ -270,7 +270,7 @@ AmgHierarchy build_cf_hierarchy(const CsrMat& A0, const fem_pc_params& prm) {
         H.A.push_back(std::move(Ac));
     }
     H.coarse = cf_coarse_operator(H.A.back());
-    if (prm.smoother == FEM_SMOOTH_SGS) sgs_levels(H);
+    if (prm.smoother == FEM_SMOOTH_SGS) sgs_levels_host(H);
     return H;
 }
 
@@ -305,7 +305,7 @@ static HCsr condense1(const HCsr& A) {
 // side runs through the already-eliminated sub-diagonal entries without the
 // row swaps); exact = 1 is the pivoted LU solve. Pivots below 1e-14 in
 // magnitude are replaced by +-1e-14 in both.
-static std::vector<double> sa_coarse_operator(const HCsr& A, bool exact) {
+std::vector<double> sa_coarse_operator_host(const HCsr& A, bool exact) {
     const int sz = (int)A.n;
     check_coarse(A.n);
     std::vector<double> d((size_t)sz * sz, 0.0);
@@ -558,8 +558,8 @@ AmgHierarchy build_sa_hierarchy(const CsrMat& A0, const fem_pc_params& prm) {
         H.P.push_back(std::move(P));
         cur = std::move(Ac);
     }
-    H.coarse = sa_coarse_operator(H.A.back(), prm.coarse_exact != 0);
-    if (prm.smoother == FEM_SMOOTH_SGS) sgs_levels(H);
+    H.coarse = sa_coarse_operator_host(H.A.back(), prm.coarse_exact != 0);
+    if (prm.smoother == FEM_SMOOTH_SGS) sgs_levels_host(H);
     return H;
 }
 

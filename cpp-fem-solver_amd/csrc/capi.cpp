@@ -149,6 +149,7 @@ void fem_pc_params_default(int kind, fem_pc_params* p) {
     // solve is the default, coarse_exact = 0 reproduces the reference
     p->coarse_exact = 1;
     p->fp32 = 0;
+    p->setup_device = 1;
     if (kind == FEM_PC_AMG) p->omega = 2.0 / 3.0;
     if (kind == FEM_PC_SAAMG || kind == FEM_PC_BSAAMG) {
         // Smoother and prolongator weights 0 select 4/(3 rho(D^{-1}A)) per

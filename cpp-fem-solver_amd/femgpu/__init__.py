@@ -51,7 +51,7 @@ _ERRS = {1: InvalidArgument, 2: OutOfRange, 3: SolverFailure, 4: DeviceError}
 class PcParams(C.Structure):
     _fields_ = [("omega", C.c_double), ("theta", C.c_double), ("npre", C.c_int), ("npost", C.c_int),
                 ("max_levels", C.c_int), ("coarse_size", C.c_int), ("max_agg", C.c_int), ("smoother", C.c_int),
-                ("p_omega", C.c_double), ("coarse_exact", C.c_int), ("fp32", C.c_int)]
+                ("p_omega", C.c_double), ("coarse_exact", C.c_int), ("fp32", C.c_int), ("setup_device", C.c_int)]
 
 
 class SolveOpts(C.Structure):

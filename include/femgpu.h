@@ -97,11 +97,17 @@ typedef struct fem_pc_params {
     int max_agg;        /* SA aggregate size cap (4) */
     int smoother;       /* enum fem_smoother */
     double p_omega;     /* SA prolongator smoothing weight; <= 0 (default) selects
-                           4/(3 rho(D^{-1}A)) per level and stops coarsening when
-                           a level keeps > 85% of its rows; 2/3 = the reference */
+                           the robust variant: 4/(3 rho(D^{-1}A)) per level,
+                           aggregation by a distance-2 maximal independent set
+                           of the strength graph, and a stop when a level keeps
+                           > 85% of its rows; > 0 (2/3 = the reference) keeps
+                           the reference's greedy aggregation */
     int coarse_exact;   /* SA: 1 = pivoted LU coarse solve; 0 = the reference's
                            elimination sequence (blockMatrix.hpp:958-999) */
     int fp32;           /* also build a float32 copy for FEM_PREC_FP32_IR */
+    int setup_device;   /* SA robust variant: 1 (default) builds the hierarchy on
+                           the device (same result as the host build), 0 on the
+                           host */
 } fem_pc_params;
 
 typedef struct fem_solve_opts {

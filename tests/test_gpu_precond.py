@@ -175,3 +175,22 @@ def test_amg_fp32_refinement():
     x, st = F.solve(A, M, b, solver="pcg", precision=F.FP32_IR)
     assert st["converged"] == 1 and st["true_relres"] < 1e-10
     assert rel(x, xt) < 1e-7
+
+
+@pytest.mark.parametrize("case,smoother", [("vardiff3d", F.SMOOTH_JACOBI), ("poisson3d", F.SMOOTH_SGS),
+                                           ("poisson3d48", F.SMOOTH_JACOBI)])
+def test_saamg_device_setup_equals_host_setup(case, smoother):
+    """The robust SA hierarchy built on the device is the host build's, bit
+    for bit: same level sizes and the same V-cycle output."""
+    G = {"vardiff3d": lambda: gen.vardiff3d(20, seed=6)[0], "poisson3d": lambda: gen.poisson3d(24),
+         "poisson3d48": lambda: gen.poisson3d(48)}[case]()
+    A = F.CsrMatrix.from_gen(G)
+    kw = dict(smoother=smoother, npre=2, npost=2, coarse_exact=1, omega=0.0 if smoother == F.SMOOTH_JACOBI else 1.0,
+              p_omega=0.0)
+    Md = F.Preconditioner(A, "saamg", setup_device=1, **kw)
+    Mh = F.Preconditioner(A, "saamg", setup_device=0, **kw)
+    a, b = Md.info(), Mh.info()
+    assert a["levels"] == b["levels"] > 2
+    assert np.array_equal(a["rows"], b["rows"]) and np.array_equal(a["nnz"], b["nnz"])
+    r = np.random.default_rng(4).standard_normal(G.n)
+    assert np.array_equal(Md.apply(r), Mh.apply(r))
