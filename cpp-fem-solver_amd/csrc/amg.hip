@@ -94,19 +94,27 @@ __global__ __launch_bounds__(1024) void k_gs_small(int ncol, const uint32_t* __r
 // Rows of one colour share no entries, so blocks never read a u value that
 // another block of the same launch writes (the diagonal entry is read in
 // the staging phase, before this block's own update).
-template <typename T>
+// MODE 0 updates u; MODE 1 writes the residual res[i] = f_i - (A u)_i of the
+// rows instead, and blocks from `zero_from` on write res = 0 without reading
+// anything (the last colour right after a forward sweep with weight 1, whose
+// residual is zero in exact arithmetic).
+template <typename T, int MODE>
 __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict__ blk,
                                                       const uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ gp,
                                                       const uint32_t* __restrict__ gci, const T* __restrict__ gv,
                                                       const T* __restrict__ gd, const T* __restrict__ f, T* u, T w,
-                                                      Gate g) {
+                                                      T* res, uint32_t zero_from, Gate g) {
     if (g.closed()) return;
     __shared__ T prod[kStageNnz];
     const uint32_t r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+    const uint32_t t = threadIdx.x;
+    if (MODE == 1 && blockIdx.x >= zero_from) {
+        for (uint32_t row = r0 + t; row < r1; row += kBlock) res[rows[row]] = (T)0;
+        return;
+    }
     const uint32_t k0 = gp[r0], k1 = gp[r1];
     const uint32_t cnt = k1 - k0;
-    const uint32_t t = threadIdx.x;
     if (cnt <= (uint32_t)kStageNnz) {
 #pragma unroll
         for (int q = 0; q < kStageNnz / kBlock; ++q) {
@@ -119,7 +127,8 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
             T s = (T)0;
             for (uint32_t k = b; k < e; ++k) s += prod[k];
             const uint32_t i = rows[row];
-            u[i] = u[i] + (w * (f[i] - s)) / gd[row];
+            if (MODE == 0) u[i] = u[i] + (w * (f[i] - s)) / gd[row];
+            else res[i] = f[i] - s;
         }
     } else {
         double s = 0.0;
@@ -128,9 +137,23 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
         block_sum<1>(tot);
         if (t == 0) {
             const uint32_t i = rows[r0];
-            u[i] = u[i] + (w * (f[i] - (T)tot[0])) / gd[r0];
+            if (MODE == 0) u[i] = u[i] + (w * (f[i] - (T)tot[0])) / gd[r0];
+            else res[i] = f[i] - (T)tot[0];
         }
     }
+}
+
+// The first colour of a sweep from u = 0 and the zero start for the other
+// colours in one pass: grouped rows t < c0_end get u = w f / a_ii (the
+// Gauss-Seidel update with every neighbour still zero), the rest u = 0.
+template <typename T>
+__global__ void k_gs_first(const uint32_t* __restrict__ rows, uint32_t n, uint32_t c0_end, const T* __restrict__ gd,
+                           const T* __restrict__ f, T* __restrict__ u, T w, Gate g) {
+    if (g.closed()) return;
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = rows[t];
+    u[i] = t < c0_end ? (w * f[i]) / gd[t] : (T)0;
 }
 
 // u = C f with C dense (row-major, float64); f, u of type T
@@ -217,12 +240,17 @@ struct PcAmg : Precond {
             }
             D->A.set_host(A.n, A.m, std::move(A.rp), std::move(A.ci), std::move(A.v), s);
             D->A.stream = s;
+            if (k > 0) D->A.use_vector_rows();  // coarse Galerkin operators
             if (k + 1 < L) {
                 HCsr& P = H.P[k];
                 D->nc = P.m;
                 HCsr R = transpose(P);
                 D->P.set_host(P.n, P.m, std::move(P.rp), std::move(P.ci), std::move(P.v), s);
                 D->R.set_host(R.n, R.m, std::move(R.rp), std::move(R.ci), std::move(R.v), s);
+                // transfer operators: summation order is free, long rows get
+                // lane groups
+                D->P.use_vector_rows();
+                D->R.use_vector_rows();
                 D->d.upload(H.d[k], s);
                 D->u.alloc(D->n);
                 D->u2.alloc(D->n);
@@ -352,11 +380,13 @@ struct PcAmg : Precond {
         return cur;
     }
 
+    // `from_zero`: u has not been initialised; the sweep starts from u = 0
     template <typename T, int LANES>
     void gs_launch(AmgLevelDev& D, const T* gv, const T* gd, const T* f, T* u, int sweeps, bool reverse,
-                   hipStream_t s, Gate g) {
+                   bool from_zero, hipStream_t s, Gate g) {
         const T w = (T)D.omega;
         if (D.n <= 8192) {
+            if (from_zero) hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, u, D.n, g);
             hipLaunchKernelGGL((k_gs_small<T, LANES>), dim3(1), dim3(1024), 0, s, D.ncol, D.cptr.get(),
                                D.crows.get(), D.gp.get(), D.gci.get(), gv, gd, f, u, w, sweeps, reverse ? 1 : 0, g);
             return;
@@ -364,16 +394,25 @@ struct PcAmg : Precond {
         for (int sw = 0; sw < sweeps; ++sw)
             for (int cc = 0; cc < D.ncol; ++cc) {
                 const int c = reverse ? D.ncol - 1 - cc : cc;
+                if (from_zero && sw == 0 && cc == 0) {
+                    const uint32_t lo = D.h_cptr[c], hi = D.h_cptr[c + 1];
+                    if (lo != 0) fail(FEM_ERR_RUNTIME, "AMG: a sweep from zero must start with the first colour");
+                    hipLaunchKernelGGL((k_gs_first<T>), g1(D.n), dim3(256), 0, s, D.crows.get(), D.n, hi, gd, f, u, w,
+                                       g);
+                    continue;
+                }
                 const uint32_t b0 = D.h_cblk[c], b1 = D.h_cblk[c + 1];
                 if (b0 >= b1) continue;
-                hipLaunchKernelGGL((k_gs_stream<T>), dim3(b1 - b0), dim3(kBlock), 0, s, D.gblk.get() + b0,
-                                   D.crows.get(), D.gp.get(), D.gci.get(), gv, gd, f, u, w, g);
+                hipLaunchKernelGGL((k_gs_stream<T, 0>), dim3(b1 - b0), dim3(kBlock), 0, s, D.gblk.get() + b0,
+                                   D.crows.get(), D.gp.get(), D.gci.get(), gv, gd, f, u, w, (T*)nullptr, 0u, g);
             }
+        if (from_zero && sweeps == 0) hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, u, D.n, g);
     }
 
     template <typename T>
-    void gs(AmgLevelDev& D, const T* f, T* u, int sweeps, bool reverse, hipStream_t s, Gate g) {
-        if (sweeps <= 0) return;
+    void gs(AmgLevelDev& D, const T* f, T* u, int sweeps, bool reverse, hipStream_t s, Gate g,
+            bool from_zero = false) {
+        if (sweeps <= 0 && !from_zero) return;
         const T *gv, *gd;
         if constexpr (std::is_same<T, double>::value) {
             gv = D.gv.get();
@@ -383,13 +422,28 @@ struct PcAmg : Precond {
             gd = D.gd32.get();
         }
         switch (D.lanes) {
-            case 1: gs_launch<T, 1>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
-            case 4: gs_launch<T, 4>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
-            case 8: gs_launch<T, 8>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
-            case 16: gs_launch<T, 16>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
-            case 32: gs_launch<T, 32>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
-            default: gs_launch<T, 64>(D, gv, gd, f, u, sweeps, reverse, s, g); break;
+            case 1: gs_launch<T, 1>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
+            case 4: gs_launch<T, 4>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
+            case 8: gs_launch<T, 8>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
+            case 16: gs_launch<T, 16>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
+            case 32: gs_launch<T, 32>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
+            default: gs_launch<T, 64>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
         }
+    }
+
+    // residual f - A u after a forward Gauss-Seidel sweep with weight 1 on a
+    // large level: the last colour's rows are zero, the others are computed
+    // on the colour-grouped matrix
+    template <typename T>
+    bool gs_residual(AmgLevelDev& D, const T* f, T* u, T* res, hipStream_t s, Gate g) {
+        if (!D.use_gs || D.n <= 8192 || D.omega != 1.0 || npre < 1 || D.ncol < 2) return false;
+        const T* gv;
+        if constexpr (std::is_same<T, double>::value) gv = D.gv.get();
+        else gv = D.gv32.get();
+        const uint32_t nb = D.h_cblk[D.ncol], zero_from = D.h_cblk[D.ncol - 1];
+        hipLaunchKernelGGL((k_gs_stream<T, 1>), dim3(nb), dim3(kBlock), 0, s, D.gblk.get(), D.crows.get(), D.gp.get(),
+                           D.gci.get(), gv, (const T*)nullptr, f, u, (T)1, res, zero_from, g);
+        return true;
     }
 
     // Jacobi data of a level: the SGS smoother's fallback divides by a_ii
@@ -413,8 +467,7 @@ struct PcAmg : Precond {
             T* u;
             if (smoother == FEM_SMOOTH_SGS && D.use_gs) {
                 u = (k == 0) ? z : B.u;
-                hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, u, D.n, g);
-                gs<T>(D, f, u, npre, false, s, g);
+                gs<T>(D, f, u, npre, false, s, g, true);
             } else {
                 // Jacobi alternates between two buffers; on level 0 pick the
                 // first one so that the last post-smoothing sweep writes z
@@ -437,7 +490,8 @@ struct PcAmg : Precond {
             ucur[k] = u;
             SpmvArgs<T> ar;
             ar.f = f;
-            csr_spmv<T>(D.A, u, B.res, EPI_RESID, ar, s, g);
+            if (!(smoother == FEM_SMOOTH_SGS && gs_residual<T>(D, f, u, B.res, s, g)))
+                csr_spmv<T>(D.A, u, B.res, EPI_RESID, ar, s, g);
             AmgLevelDev& C = *lv[k + 1];
             Bufs<T> BC = bufs<T>(C);
             csr_spmv<T>(D.R, B.res, BC.f, EPI_STORE, {}, s, g);

@@ -462,4 +462,37 @@ int fem_bench_spmv(fem_csr A, int fp32, int reps, double* avg_ms) {
     });
 }
 
+// Same SpMV, each launch timed alone after a 1 GiB write has evicted the L2
+// and the 256 MB Infinity Cache (MALL), so the matrix streams from HBM.
+int fem_bench_spmv_cold(fem_csr A, int fp32, int reps, double* avg_ms) {
+    return guard([&] {
+        auto& M = A->A;
+        hipStream_t s = A->stream;
+        if (fp32) M.ensure_fp32(s);
+        const size_t es = fp32 ? 4 : 8;
+        DevBuf<char> x((size_t)M.m * es), y((size_t)M.n * es);
+        const size_t flush_n = (size_t)1 << 27;  // doubles: 1 GiB
+        DevBuf<double> flush(flush_n);
+        FEM_HIP(hipMemsetAsync(x.get(), 0, (size_t)M.m * es, s));
+        hipEvent_t e0, e1;
+        FEM_HIP(hipEventCreate(&e0));
+        FEM_HIP(hipEventCreate(&e1));
+        double tot = 0.0;
+        for (int i = 0; i < reps + 1; ++i) {
+            vec_fill<double>(flush.get(), (double)i, flush_n, s);
+            FEM_HIP(hipEventRecord(e0, s));
+            if (fp32) M.apply32((const float*)x.get(), (float*)y.get(), s);
+            else M.apply((const double*)x.get(), (double*)y.get(), s);
+            FEM_HIP(hipEventRecord(e1, s));
+            FEM_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            if (i > 0) tot += ms;  // the first launch warms instruction caches
+        }
+        *avg_ms = reps > 0 ? tot / reps : 0.0;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
 }  // extern "C"

@@ -85,6 +85,43 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
     }
 }
 
+// Vector-per-row SpMV: LANES consecutive lanes own one row, stride over its
+// entries and add their partial sums by a butterfly; lane 0 of the group
+// applies the epilogue. For long rows (restriction, coarse Galerkin
+// operators) where one thread per row would serialise.
+template <typename T, int EPI, int LANES>
+__global__ __launch_bounds__(kBlock) void k_spmv_vec(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                                     const T* __restrict__ v, const T* __restrict__ x,
+                                                     T* __restrict__ y, uint32_t n, SpmvArgs<T> a, Gate g) {
+    if (g.closed()) return;
+    const uint32_t row = (blockIdx.x * (uint32_t)kBlock + threadIdx.x) / LANES;
+    const int lane = threadIdx.x % LANES;
+    T s = (T)0;
+    if (row < n)
+        for (uint32_t k = rp[row] + lane; k < rp[row + 1]; k += LANES) s += v[k] * x[ci[k]];
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
+    if (row < n && lane == 0) y[row] = spmv_epilogue<T, EPI>(row, s, a);
+}
+
+template <typename T, int LANES>
+void spmv_vec_launch(const CsrMat& A, const T* vals, const T* x, T* y, int epi, const SpmvArgs<T>& a, hipStream_t s,
+                     Gate g) {
+    const dim3 grid((unsigned)(((uint64_t)A.n * LANES + kBlock - 1) / kBlock)), block(kBlock);
+    switch (epi) {
+        case EPI_STORE: hipLaunchKernelGGL((k_spmv_vec<T, EPI_STORE, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+        case EPI_RESID: hipLaunchKernelGGL((k_spmv_vec<T, EPI_RESID, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+        case EPI_ADD: hipLaunchKernelGGL((k_spmv_vec<T, EPI_ADD, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+        case EPI_JACOBI: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+        default: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI_INV, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+    }
+}
+
+void CsrMat::use_vector_rows() {
+    const double avg = n ? (double)nnz / n : 0.0;
+    lanes = avg < 40 ? 0 : avg < 96 ? 32 : 64;
+}
+
 template <typename T>
 void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, hipStream_t s, Gate g) {
     if (A.n == 0) return;
@@ -92,6 +129,16 @@ void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, 
     if constexpr (std::is_same<T, double>::value) vals = A.v.get();
     else vals = A.v32.get();
     if (!vals) fail(FEM_ERR_RUNTIME, "float32 copy of the matrix was not built");
+    if (A.lanes > 0) {
+        switch (A.lanes) {
+            case 8: spmv_vec_launch<T, 8>(A, vals, x, y, epi, a, s, g); break;
+            case 16: spmv_vec_launch<T, 16>(A, vals, x, y, epi, a, s, g); break;
+            case 32: spmv_vec_launch<T, 32>(A, vals, x, y, epi, a, s, g); break;
+            default: spmv_vec_launch<T, 64>(A, vals, x, y, epi, a, s, g); break;
+        }
+        FEM_HIP(hipGetLastError());
+        return;
+    }
     dim3 grid(A.plan.nblk), block(kBlock);
     switch (epi) {
         case EPI_STORE:

@@ -139,6 +139,7 @@ _SIGS = {
     "fem_bsr_solve": (C.c_int, [P_vp, P_vp, P_f64, P_f64, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "fem_bsr_destroy": (C.c_int, [P_vp]),
     "fem_bench_spmv": (C.c_int, [P_vp, C.c_int, C.c_int, P_f64]),
+    "fem_bench_spmv_cold": (C.c_int, [P_vp, C.c_int, C.c_int, P_f64]),
     "fem_mlp_create": (C.c_int, [P_f32, C.c_uint64, C.POINTER(P_vp)]),
     "fem_mlp_destroy": (C.c_int, [P_vp]),
     "fem_mlp_get_params": (C.c_int, [P_vp, P_f32]),
@@ -263,9 +264,12 @@ class CsrMatrix:
         _check(lib().fem_spmv(self._h, _p(x, C.c_double), _p(y, C.c_double)))
         return y
 
-    def bench_spmv(self, fp32=False, reps=100) -> float:
+    def bench_spmv(self, fp32=False, reps=100, cold=False) -> float:
+        """Average SpMV time (ms): back-to-back launches, or each launch
+        after evicting the caches (cold)."""
         ms = C.c_double()
-        _check(lib().fem_bench_spmv(self._h, int(fp32), reps, C.byref(ms)))
+        f = lib().fem_bench_spmv_cold if cold else lib().fem_bench_spmv
+        _check(f(self._h, int(fp32), reps, C.byref(ms)))
         return ms.value
 
     def close(self):

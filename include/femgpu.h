@@ -328,6 +328,9 @@ int fem_batch_destroy(fem_batch b);
 /* Average device time of `reps` back-to-back SpMV launches on A's stream,
  * timed with HIP events (inputs resident in HBM). */
 int fem_bench_spmv(fem_csr A, int fp32, int reps, double* avg_ms);
+/* The same, each launch timed alone after a 1 GiB write has evicted the L2
+ * and the 256 MB Infinity Cache, so the matrix streams from HBM. */
+int fem_bench_spmv_cold(fem_csr A, int fp32, int reps, double* avg_ms);
 
 #ifdef __cplusplus
 }
