@@ -29,6 +29,8 @@ namespace fem {
 
 constexpr uint32_t kChainRows = 1024;  // levels this small run inside one workgroup
 constexpr int kChainThreads = 1024;
+constexpr uint32_t kChainLevels = 4096;  // levels per chain launch (LDS level table)
+constexpr uint32_t kRing = 4096;         // LDS ring of the most recent sweep results
 
 // ---------------------------------------------------------------------------
 // Level schedule + SELL-64 sweep layout
@@ -36,17 +38,26 @@ constexpr int kChainThreads = 1024;
 struct Segment {
     uint32_t s0, s1;       // slot range
     uint32_t lvl0, lvl1;   // level range (chain when lvl1 - lvl0 > 1)
+    uint32_t kc = 0;       // chain: entries per slot in the dense layout
+    uint64_t cbase = 0;    // chain: offset of slot s0 in the dense arrays
 };
 
 struct Sweep {
     uint32_t n = 0, nlev = 0;
     std::vector<uint32_t> h_lvl_ptr;    // [nlev+1]
     std::vector<Segment> segs;
-    DevBuf<uint32_t> lvl_ptr, slot_row, gptr, col, vmap, dmap;
+    DevBuf<uint32_t> lvl_ptr, slot_row, gptr, col, dslot, vmap, dmap;
     DevBuf<uint16_t> len;
     DevBuf<double> val, diag;
     DevBuf<float> val32, diag32;
     uint64_t nent = 0;
+    // chains: dense per-slot copies (entry e of slot s at cbase + (s - s0) kc + e)
+    // of the columns, dependency slots and values, the SELL index each value
+    // comes from, and right-hand-side staging in slot order
+    DevBuf<uint32_t> ccol, cds, cmap;
+    DevBuf<double> cval, rslot;
+    DevBuf<float> cval32, rslot32;
+    uint64_t cent = 0;
 
     // rows: dependency lists per row (entries in summation order), as
     // (col, position-in-factor-values). lev: level per row.
@@ -75,12 +86,15 @@ struct Sweep {
             g[gi + 1] = g[gi] + w * 64;
         }
         nent = g[ngrp];
-        std::vector<uint32_t> hc(nent, 0), hm(nent, UINT32_MAX);
+        std::vector<uint32_t> slot_of(n);
+        for (uint32_t s = 0; s < n; ++s) slot_of[order[s]] = s;
+        std::vector<uint32_t> hc(nent, 0), hm(nent, UINT32_MAX), hd(nent, 0);
         for (uint32_t s = 0; s < n; ++s) {
             const uint32_t i = order[s];
             const uint32_t base = g[s >> 6] + (s & 63);
             for (uint32_t e = 0; e < ln[s]; ++e) {
                 hc[base + e * 64] = cols[rp[i] + e];
+                hd[base + e * 64] = slot_of[cols[rp[i] + e]];
                 hm[base + e * 64] = pos[rp[i] + e];
             }
         }
@@ -95,14 +109,45 @@ struct Sweep {
                 continue;
             }
             uint32_t e = l;
-            while (e < nlev && h_lvl_ptr[e + 1] - h_lvl_ptr[e] <= kChainRows) ++e;
+            while (e < nlev && e - l < kChainLevels && h_lvl_ptr[e + 1] - h_lvl_ptr[e] <= kChainRows) ++e;
             segs.push_back({h_lvl_ptr[l], h_lvl_ptr[e], l, e});
             l = e;
+        }
+        // dense layout of the chains
+        std::vector<uint32_t> dc, dd, dm;
+        for (Segment& sg : segs) {
+            if (sg.lvl1 - sg.lvl0 <= 1) continue;
+            uint32_t kc = 1;
+            for (uint32_t q = sg.s0; q < sg.s1; ++q) kc = std::max<uint32_t>(kc, ln[q]);
+            // the chain kernel's stride is its template width: pad to 2, 4 or 8
+            // (wider chains are swept one level per launch, no dense copy)
+            if (kc > 8) continue;
+            kc = kc <= 2 ? 2u : kc <= 4 ? 4u : 8u;
+            sg.kc = kc;
+            sg.cbase = dc.size();
+            for (uint32_t q = sg.s0; q < sg.s1; ++q) {
+                const uint32_t base = g[q >> 6] + (q & 63);
+                for (uint32_t e = 0; e < kc; ++e) {
+                    const bool real = e < ln[q];
+                    dc.push_back(real ? hc[base + e * 64] : 0u);
+                    dd.push_back(real ? hd[base + e * 64] : 0u);
+                    dm.push_back(real ? (uint32_t)(base + e * 64) : UINT32_MAX);
+                }
+            }
+        }
+        cent = dc.size();
+        if (cent) {
+            ccol.upload(dc, s);
+            cds.upload(dd, s);
+            cmap.upload(dm, s);
+            cval.alloc(cent);
+            rslot.alloc(n);
         }
         lvl_ptr.upload(h_lvl_ptr, s);
         slot_row.upload(order, s);
         gptr.upload(g, s);
         col.upload(hc, s);
+        dslot.upload(hd, s);
         vmap.upload(hm, s);
         len.upload(ln, s);
         val.alloc(std::max<uint64_t>(nent, 1));
@@ -122,6 +167,38 @@ __global__ void k_gather_vals(const double* src, const uint32_t* map, double* ds
         const uint32_t m = map[i];
         dst[i] = m == UINT32_MAX ? 0.0 : src[m];
     }
+}
+
+template <typename T>
+__global__ void k_gather_dense(const T* src, const uint32_t* map, T* dst, uint64_t n) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i < n) {
+        const uint32_t m = map[i];
+        dst[i] = m == UINT32_MAX ? (T)0 : src[m];
+    }
+}
+
+// rhs in slot order for the slots [s0, s1) of a chain
+template <typename T>
+__global__ void k_gather_rhs(const uint32_t* __restrict__ slot_row, uint32_t s0, uint32_t s1, const T* __restrict__ rhs,
+                             T* __restrict__ out, Gate g) {
+    if (g.closed()) return;
+    const uint32_t s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < s1) out[s] = rhs[slot_row[s]];
+}
+
+// copy the sweep's values (and float32 copy when present) into the chain layout
+static void densify(Sweep& S, hipStream_t s) {
+    if (!S.cent) return;
+    const unsigned G = (unsigned)((S.cent + 255) / 256);
+    hipLaunchKernelGGL((k_gather_dense<double>), dim3(G), dim3(256), 0, s, S.val.get(), S.cmap.get(), S.cval.get(), S.cent);
+    if (S.val32.get()) {
+        S.cval32.alloc(S.cent);
+        S.rslot32.alloc(S.n);
+        hipLaunchKernelGGL((k_gather_dense<float>), dim3(G), dim3(256), 0, s, S.val32.get(), S.cmap.get(),
+                           S.cval32.get(), S.cent);
+    }
+    FEM_HIP(hipGetLastError());
 }
 
 static void gather(const double* src, const uint32_t* map, double* dst, uint64_t n, hipStream_t s) {
@@ -149,42 +226,139 @@ __global__ __launch_bounds__(256) void k_sweep_level(uint32_t s0, uint32_t s1, c
     y[i] = acc;
 }
 
-// A run of consecutive small levels in one workgroup.
-template <typename T, bool DIV>
+// A run of consecutive small levels (<= kChainRows rows each) in one
+// workgroup, one row per thread per level. Every result also goes into an
+// LDS ring indexed by sweep slot; a dependency written earlier in this launch
+// and still inside the ring window is read from LDS instead of through L2.
+// For level [a, b) the window is slots >= max(s0, b - kRing): this level's
+// writes (slots a..b-1) land on ring positions of slots below b - kRing, so
+// they never clobber an entry still being read. The chain's entries sit in a
+// dense per-slot layout and its right-hand side in slot order, so every load
+// of a row is addressed by the slot alone; each thread loads the row it will
+// handle several levels ahead before the barrier. Per-row arithmetic order is
+// unchanged (rhs first, then the entries in ascending order).
+template <typename T, bool DIV, int KC>
 __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint32_t l1, const uint32_t* __restrict__ lvl_ptr,
                                                                const uint32_t* __restrict__ slot_row,
                                                                const uint16_t* __restrict__ len,
-                                                               const uint32_t* __restrict__ gptr,
-                                                               const uint32_t* __restrict__ col, const T* __restrict__ val,
-                                                               const T* __restrict__ diag, const T* __restrict__ rhs, T* y,
-                                                               Gate g) {
+                                                               const uint32_t* __restrict__ ccol,
+                                                               const uint32_t* __restrict__ cds,
+                                                               const T* __restrict__ cval, const T* __restrict__ diag,
+                                                               const T* __restrict__ rslot, T* y, Gate g) {
     if (g.closed()) return;
-    for (uint32_t l = l0; l < l1; ++l) {
-        const uint32_t a = lvl_ptr[l], b = lvl_ptr[l + 1];
-        for (uint32_t s = a + threadIdx.x; s < b; s += blockDim.x) {
-            const uint32_t i = slot_row[s];
-            const uint32_t base = gptr[s >> 6] + (s & 63);
-            const uint32_t L = len[s];
-            T acc = rhs[i];
-            for (uint32_t e = 0; e < L; ++e) acc -= val[base + e * 64] * y[col[base + e * 64]];
-            if constexpr (DIV) acc = acc / diag[s];
-            y[i] = acc;
+    __shared__ T ring[kRing];
+    __shared__ uint32_t lp[kChainLevels + 1];
+    for (uint32_t t = threadIdx.x; t <= l1 - l0; t += blockDim.x) lp[t] = lvl_ptr[l0 + t];
+    __syncthreads();
+    const uint32_t s0 = lp[0];
+    struct Row {
+        bool on;
+        uint32_t i, L;
+        T r, d;
+        uint32_t c[KC], ds[KC];
+        T v[KC];
+    };
+    auto fetch = [&](uint32_t l, Row& R) {
+        R.on = false;
+        if (l >= l1) return;
+        const uint32_t s = lp[l - l0] + threadIdx.x;
+        if (s >= lp[l - l0 + 1]) return;
+        R.on = true;
+        R.i = slot_row[s];
+        R.L = len[s];
+        R.r = rslot[s];
+        R.d = DIV ? diag[s] : (T)1;
+        const uint64_t o = (uint64_t)(s - s0) * KC;
+#pragma unroll
+        for (int e = 0; e < KC; ++e) {
+            R.c[e] = ccol[o + e];
+            R.ds[e] = cds[o + e];
+            R.v[e] = cval[o + e];
         }
-        __syncthreads();
+    };
+    // D rows in flight per thread: buf[k] holds level l0 + q D + k; after a
+    // level is done its slot is refilled with the level D further on. The
+    // rotation is unrolled so buf stays in registers.
+    constexpr int D = KC <= 2 ? 6 : KC <= 4 ? 4 : 2;
+    Row buf[D];
+#pragma unroll
+    for (int k = 0; k < D; ++k) fetch(l0 + k, buf[k]);
+    for (uint32_t base = l0; base < l1; base += D) {
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            const uint32_t l = base + k;
+            if (l >= l1) break;  // uniform over the workgroup
+            const uint32_t a = lp[l - l0], b = lp[l - l0 + 1];
+            const uint32_t lo = max(s0, b > kRing ? b - kRing : 0u);
+            Row& cur = buf[k];
+            if (cur.on) {
+                T acc = cur.r;
+#pragma unroll
+                for (int e = 0; e < KC; ++e)
+                    if ((uint32_t)e < cur.L) {
+                        const T yv = cur.ds[e] >= lo ? ring[cur.ds[e] % kRing] : y[cur.c[e]];
+                        acc -= cur.v[e] * yv;
+                    }
+                if constexpr (DIV) acc = acc / cur.d;
+                const uint32_t s = a + threadIdx.x;
+                ring[s % kRing] = acc;
+                y[cur.i] = acc;
+            }
+            fetch(l + D, cur);
+            // Between levels only the LDS ring must be ordered: wait for LDS
+            // operations and barrier, leaving the loads fetched ahead in
+            // flight (__syncthreads would also wait for them). A dependency
+            // read from global memory lies more than kRing slots back (levels
+            // hold <= 1024 slots), so it was written before some level that
+            // crossed a multiple of kRing / 2 slots; those levels end with a
+            // full barrier that waits for every store.
+            if (a / (kRing / 2) != b / (kRing / 2)) {
+                __syncthreads();
+            } else {
+                __asm__ volatile("" ::: "memory");
+                __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS done
+                __builtin_amdgcn_s_barrier();
+                __asm__ volatile("" ::: "memory");
+            }
+        }
     }
+}
+
+template <typename T, bool DIV, int KC>
+static void chain_launch(const Sweep& S, const Segment& sg, const T* cval, const T* diag, T* rslot, T* y, hipStream_t s,
+                         Gate g) {
+    hipLaunchKernelGGL((k_sweep_chain<T, DIV, KC>), dim3(1), dim3(kChainThreads), 0, s, sg.lvl0, sg.lvl1,
+                       S.lvl_ptr.get(), S.slot_row.get(), S.len.get(), S.ccol.get() + sg.cbase, S.cds.get() + sg.cbase,
+                       cval + sg.cbase, diag, rslot, y, g);
 }
 
 template <typename T, bool DIV>
 static void run_sweep(const Sweep& S, const T* val, const T* diag, const T* rhs, T* y, hipStream_t s, Gate g) {
+    const T* cval;
+    T* rslot;
+    if constexpr (std::is_same<T, double>::value) {
+        cval = S.cval.get();
+        rslot = S.rslot.get();
+    } else {
+        cval = S.cval32.get();
+        rslot = S.rslot32.get();
+    }
     for (const Segment& sg : S.segs) {
-        if (sg.lvl1 - sg.lvl0 > 1) {
-            hipLaunchKernelGGL((k_sweep_chain<T, DIV>), dim3(1), dim3(kChainThreads), 0, s, sg.lvl0, sg.lvl1,
-                               S.lvl_ptr.get(), S.slot_row.get(), S.len.get(), S.gptr.get(), S.col.get(), val, diag,
-                               rhs, y, g);
-        } else {
+        // a chain with a dense copy: its stride sg.kc equals the kernel width
+        if (sg.lvl1 - sg.lvl0 > 1 && (sg.kc == 2 || sg.kc == 4 || sg.kc == 8)) {
             const uint32_t cnt = sg.s1 - sg.s0;
-            hipLaunchKernelGGL((k_sweep_level<T, DIV>), dim3((cnt + 255) / 256), dim3(256), 0, s, sg.s0, sg.s1,
-                               S.slot_row.get(), S.len.get(), S.gptr.get(), S.col.get(), val, diag, rhs, y, g);
+            hipLaunchKernelGGL((k_gather_rhs<T>), dim3((cnt + 255) / 256), dim3(256), 0, s, S.slot_row.get(), sg.s0,
+                               sg.s1, rhs, rslot, g);
+            if (sg.kc == 2) chain_launch<T, DIV, 2>(S, sg, cval, diag, rslot, y, s, g);
+            else if (sg.kc == 4) chain_launch<T, DIV, 4>(S, sg, cval, diag, rslot, y, s, g);
+            else chain_launch<T, DIV, 8>(S, sg, cval, diag, rslot, y, s, g);
+        } else {
+            // one launch per level (also for chains with rows longer than 8)
+            for (uint32_t l = sg.lvl0; l < sg.lvl1; ++l) {
+                const uint32_t a = S.h_lvl_ptr[l], b = S.h_lvl_ptr[l + 1];
+                hipLaunchKernelGGL((k_sweep_level<T, DIV>), dim3((b - a + 255) / 256), dim3(256), 0, s, a, b,
+                                   S.slot_row.get(), S.len.get(), S.gptr.get(), S.col.get(), val, diag, rhs, y, g);
+            }
         }
     }
     FEM_HIP(hipGetLastError());
@@ -342,6 +516,8 @@ struct PcIC0 : Precond {
         gather(Lv.get(), bw.vmap.get(), bw.val.get(), bw.nent, s);
         gather(Lv.get(), fw.dmap.get(), fw.diag.get(), n, s);
         gather(Lv.get(), bw.dmap.get(), bw.diag.get(), n, s);
+        densify(fw, s);
+        densify(bw, s);
         y.alloc(n);
         FEM_HIP(hipStreamSynchronize(s));
     }
@@ -362,6 +538,8 @@ struct PcIC0 : Precond {
         to_f32(bw.val, bw.val32, bw.nent, s);
         to_f32(fw.diag, fw.diag32, n, s);
         to_f32(bw.diag, bw.diag32, n, s);
+        densify(fw, s);
+        densify(bw, s);
         y32.alloc(n);
     }
 };
@@ -458,6 +636,8 @@ struct PcILU0 : Precond {
         gather(Lv.get(), fw.vmap.get(), fw.val.get(), fw.nent, s);
         gather(Uv.get(), bw.vmap.get(), bw.val.get(), bw.nent, s);
         gather(Uv.get(), bw.dmap.get(), bw.diag.get(), n, s);
+        densify(fw, s);
+        densify(bw, s);
         y.alloc(n);
         FEM_HIP(hipStreamSynchronize(s));
     }
@@ -477,6 +657,8 @@ struct PcILU0 : Precond {
         to_f32(fw.val, fw.val32, fw.nent, s);
         to_f32(bw.val, bw.val32, bw.nent, s);
         to_f32(bw.diag, bw.diag32, n, s);
+        densify(fw, s);
+        densify(bw, s);
         y32.alloc(n);
     }
 };
