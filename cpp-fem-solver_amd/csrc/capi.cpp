@@ -462,8 +462,11 @@ int fem_bench_spmv(fem_csr A, int fp32, int reps, double* avg_ms) {
     });
 }
 
-// Same SpMV, each launch timed alone after a 1 GiB write has evicted the L2
-// and the 256 MB Infinity Cache (MALL), so the matrix streams from HBM.
+// Same SpMV, each launch timed alone after reading 1 GiB of other data has
+// evicted the L2 and the 256 MB Infinity Cache (MALL) with clean lines, so
+// the matrix streams from HBM and no write-back is charged to the SpMV. All
+// flush/SpMV pairs are queued back to back (no host round trip between
+// them), each SpMV between its own pair of events.
 int fem_bench_spmv_cold(fem_csr A, int fp32, int reps, double* avg_ms) {
     return guard([&] {
         auto& M = A->A;
@@ -472,26 +475,28 @@ int fem_bench_spmv_cold(fem_csr A, int fp32, int reps, double* avg_ms) {
         const size_t es = fp32 ? 4 : 8;
         DevBuf<char> x((size_t)M.m * es), y((size_t)M.n * es);
         const size_t flush_n = (size_t)1 << 27;  // doubles: 1 GiB
-        DevBuf<double> flush(flush_n);
+        DevBuf<double> flush(flush_n), sink(2048);
+        vec_fill<double>(flush.get(), 1.0, flush_n, s);
         FEM_HIP(hipMemsetAsync(x.get(), 0, (size_t)M.m * es, s));
-        hipEvent_t e0, e1;
-        FEM_HIP(hipEventCreate(&e0));
-        FEM_HIP(hipEventCreate(&e1));
-        double tot = 0.0;
-        for (int i = 0; i < reps + 1; ++i) {
-            vec_fill<double>(flush.get(), (double)i, flush_n, s);
-            FEM_HIP(hipEventRecord(e0, s));
+        const int total = reps + 1;  // the first pair warms instruction caches
+        std::vector<hipEvent_t> ev(2 * (size_t)total);
+        for (auto& e : ev) FEM_HIP(hipEventCreate(&e));
+        for (int i = 0; i < total; ++i) {
+            cache_flush_read(flush.get(), flush_n, sink.get(), 2048, s);
+            FEM_HIP(hipEventRecord(ev[2 * i], s));
             if (fp32) M.apply32((const float*)x.get(), (float*)y.get(), s);
             else M.apply((const double*)x.get(), (double*)y.get(), s);
-            FEM_HIP(hipEventRecord(e1, s));
-            FEM_HIP(hipEventSynchronize(e1));
-            float ms = 0.f;
-            FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
-            if (i > 0) tot += ms;  // the first launch warms instruction caches
+            FEM_HIP(hipEventRecord(ev[2 * i + 1], s));
         }
+        FEM_HIP(hipEventSynchronize(ev.back()));
+        double tot = 0.0;
+        for (int i = 1; i < total; ++i) {
+            float ms = 0.f;
+            FEM_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+            tot += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
         *avg_ms = reps > 0 ? tot / reps : 0.0;
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
     });
 }
 

@@ -177,7 +177,8 @@ void CsrMat::apply32(const float* x, float* y, hipStream_t s, Gate g) const {
 
 void CsrMat::ensure_fp32(hipStream_t s) {
     if (v32.get() || nnz == 0) return;
-    v32.alloc(nnz);
+    v32.alloc(nnz + 4);
+    FEM_HIP(hipMemsetAsync(v32.get() + nnz, 0, 4 * sizeof(float), s));
     vec_convert<float, double>(v32.get(), v.get(), 1.0, nnz, s);
 }
 
@@ -188,6 +189,11 @@ void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, s
     nnz = v_.size();
     stream = s;
     rp.upload(rp_, s);
+    // 4 zero entries past the end (room for vector-width reads)
+    ci.alloc(nnz + 4);
+    v.alloc(nnz + 4);
+    FEM_HIP(hipMemsetAsync(ci.get() + nnz, 0, 4 * sizeof(uint32_t), s));
+    FEM_HIP(hipMemsetAsync(v.get() + nnz, 0, 4 * sizeof(double), s));
     ci.upload(ci_, s);
     v.upload(v_, s);
     plan.build(rp_.data(), n, s);

@@ -157,6 +157,8 @@ void vec_fill(T* y, T val, size_t n, hipStream_t s, Gate g = {});
 template <typename TD, typename TS>
 void vec_convert(TD* y, const TS* x, double scale, size_t n, hipStream_t s, Gate g = {});
 // out = a . b, synchronous helper (host result)
+// read n doubles (cache eviction without dirty lines); sink: `blocks` doubles
+void cache_flush_read(const double* buf, size_t n, double* sink, unsigned blocks, hipStream_t s);
 double dot_sync(const double* a, const double* b, size_t n, const Reducer& R, hipStream_t s);
 
 }  // namespace fem

@@ -87,6 +87,21 @@ template void vec_convert<double, float>(double*, const float*, double, size_t, 
 template void vec_convert<double, double>(double*, const double*, double, size_t, hipStream_t, Gate);
 template void vec_convert<float, float>(float*, const float*, double, size_t, hipStream_t, Gate);
 
+// reads n doubles and leaves one sum per block in sink (evicts the caches
+// with clean lines: the next kernel pays no write-back)
+__global__ void k_read_sweep(const double* a, size_t n, double* sink) {
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) s += a[i];
+    double v[1] = {s};
+    block_sum<1>(v);
+    if (threadIdx.x == 0) sink[blockIdx.x] = v[0];
+}
+
+void cache_flush_read(const double* buf, size_t n, double* sink, unsigned blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_read_sweep, dim3(blocks), dim3(kBlock), 0, s, buf, n, sink);
+    FEM_HIP(hipGetLastError());
+}
+
 __global__ void k_dot(const double* a, const double* b, size_t n, double* partials) {
     double s = 0.0;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
