@@ -488,40 +488,6 @@ struct PcIdentity : Precond {
 
 // condense (blockMatrix.hpp:269-315): block column j/BS, duplicates summed in
 // scalar CSR order, block columns sorted and unique, explicit zeros kept
-static void condense_host(const CsrMat& A, int bs, uint32_t nbr, uint32_t nbc, std::vector<uint32_t>& rp,
-                          std::vector<uint32_t>& ci, std::vector<double>& v) {
-    A.sync_host();
-    if ((uint64_t)nbr * bs > A.n) fail(FEM_ERR_OUT_OF_RANGE, "condense: nbr * BS exceeds the matrix rows");
-    const int BV = bs * bs;
-    rp.assign(nbr + 1, 0);
-    ci.clear();
-    std::vector<uint32_t> cols;
-    for (uint32_t b = 0; b < nbr; ++b) {
-        cols.clear();
-        for (uint32_t i = b * bs; i < (b + 1) * (uint32_t)bs; ++i)
-            for (uint32_t p = A.h_rp[i]; p < A.h_rp[i + 1]; ++p) {
-                const uint32_t bj = A.h_ci[p] / bs;
-                if (bj >= nbc) fail(FEM_ERR_OUT_OF_RANGE, "condense: block column out of range");
-                cols.push_back(bj);
-            }
-        std::sort(cols.begin(), cols.end());
-        cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
-        ci.insert(ci.end(), cols.begin(), cols.end());
-        rp[b + 1] = (uint32_t)ci.size();
-    }
-    v.assign((size_t)rp[nbr] * BV, 0.0);
-    for (uint32_t b = 0; b < nbr; ++b)
-        for (uint32_t i = b * bs; i < (b + 1) * (uint32_t)bs; ++i) {
-            const int r = i % bs;
-            for (uint32_t p = A.h_rp[i]; p < A.h_rp[i + 1]; ++p) {
-                const uint32_t bj = A.h_ci[p] / bs, c = A.h_ci[p] % bs;
-                const auto it = std::lower_bound(ci.begin() + rp[b], ci.begin() + rp[b + 1], bj);
-                const size_t off = (size_t)(it - ci.begin());
-                v[off * BV + (size_t)r * bs + c] += A.h_v[p];
-            }
-        }
-}
-
 std::unique_ptr<Precond> make_block_saamg(const BsrMat& A, const fem_pc_params& p, hipStream_t s) {
     if (A.bs != 1) fail(FEM_ERR_INVALID_ARGUMENT, "block SA-AMG on the device supports BS = 1 (scalar condensation)");
     CsrMat C;
@@ -568,7 +534,7 @@ int fem_bsr_condense(fem_csr A, int bs, uint32_t nbr, uint32_t nbc, fem_bsr* out
         if (bs < 1 || bs > 4) fail(FEM_ERR_INVALID_ARGUMENT, "block size must be 1..4");
         std::vector<uint32_t> rp, ci;
         std::vector<double> v;
-        condense_host(A->A, bs, nbr, nbc, rp, ci, v);
+        condense_device(A->A, bs, nbr, nbc, rp, ci, v, A->stream);
         auto h = std::make_unique<fem_bsr_s>();
         h->stream = new_stream();
         h->A.set_host(bs, nbr, nbc, std::move(rp), std::move(ci), std::move(v), h->stream);

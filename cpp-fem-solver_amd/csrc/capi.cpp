@@ -54,45 +54,6 @@ static void check_csr(uint32_t rows, uint32_t cols, const uint32_t* rp, const ui
         if (ci[k] >= cols) fail(FEM_ERR_OUT_OF_RANGE, "column index out of range");
 }
 
-// COO -> CSR with the reference constructor's semantics (array.cpp:2072-2182)
-static void coo_to_csr(uint32_t n, uint32_t m, uint64_t nin, const double* v, const uint32_t* r, const uint32_t* c,
-                       std::vector<uint32_t>& rp, std::vector<uint32_t>& ci, std::vector<double>& vv) {
-    std::vector<uint64_t> keep;
-    keep.reserve(nin);
-    for (uint64_t k = 0; k < nin; ++k) {
-        if (r[k] >= n || c[k] >= m) fail(FEM_ERR_OUT_OF_RANGE, "COO input: row or column index out of range");
-        if (v[k] != 0.0) keep.push_back(k);
-    }
-    // counting sort by row, stable; then sort each row by column (stable
-    // keeps input order among duplicates, whose sum order then matches a
-    // stable sort by (row, col))
-    std::vector<uint64_t> cnt(n + 1, 0);
-    for (uint64_t k : keep) cnt[r[k] + 1]++;
-    for (uint32_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
-    std::vector<uint64_t> ord(keep.size());
-    {
-        std::vector<uint64_t> off(cnt.begin(), cnt.end() - 1);
-        for (uint64_t k : keep) ord[off[r[k]]++] = k;
-    }
-    rp.assign(n + 1, 0);
-    ci.clear();
-    vv.clear();
-    ci.reserve(keep.size());
-    vv.reserve(keep.size());
-    for (uint32_t i = 0; i < n; ++i) {
-        auto b = ord.begin() + cnt[i], e = ord.begin() + cnt[i + 1];
-        std::stable_sort(b, e, [&](uint64_t a, uint64_t bb) { return c[a] < c[bb]; });
-        for (auto it = b; it != e;) {
-            const uint32_t col = c[*it];
-            double s = v[*it];
-            ++it;
-            while (it != e && c[*it] == col) { s += v[*it]; ++it; }
-            if (s != 0.0) { ci.push_back(col); vv.push_back(s); }
-        }
-        rp[i + 1] = (uint32_t)ci.size();
-    }
-}
-
 void fem_solver_make(LinOp* op, hipStream_t stream, fem_pc M, const fem_solve_opts* opts, fem_solver_s* h) {
     fem_solve_opts o;
     if (opts) o = *opts;
@@ -197,12 +158,10 @@ int fem_csr_create_coo(uint32_t rows, uint32_t cols, uint64_t nnz, const double*
                        const uint32_t* collist, fem_csr* out) {
     return guard([&] {
         if (!out) fail(FEM_ERR_INVALID_ARGUMENT, "out is null");
-        std::vector<uint32_t> rp, ci;
-        std::vector<double> v;
-        coo_to_csr(rows, cols, nnz, values, rowlist, collist, rp, ci, v);
+        if (nnz && (!values || !rowlist || !collist)) fail(FEM_ERR_INVALID_ARGUMENT, "null COO arrays");
         auto h = std::make_unique<fem_csr_s>();
         h->stream = new_stream();
-        h->A.set_host(rows, cols, std::move(rp), std::move(ci), std::move(v), h->stream);
+        coo_to_csr_device(rows, cols, nnz, values, rowlist, collist, h->A, h->stream);
         h->red.init(h->stream);
         *out = h.release();
     });

@@ -205,6 +205,28 @@ void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, s
     FEM_HIP(hipStreamSynchronize(s));
 }
 
+void CsrMat::adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&& rp_, DevBuf<uint32_t>&& ci_,
+                   DevBuf<double>&& v_, hipStream_t s) {
+    n = rows;
+    m = cols;
+    nnz = nz;
+    stream = s;
+    rp = std::move(rp_);
+    ci = std::move(ci_);
+    v = std::move(v_);
+    std::vector<uint32_t> hrp(n + 1);
+    rp.download(hrp.data(), n + 1, s);
+    FEM_HIP(hipStreamSynchronize(s));
+    plan.build(hrp.data(), n, s);
+    h_rp.clear();
+    h_ci.clear();
+    h_v.clear();
+    host_valid = false;
+    v32.release();
+    lanes = 0;
+    FEM_HIP(hipStreamSynchronize(s));
+}
+
 void CsrMat::sync_host() const {
     if (host_valid) return;
     h_rp.resize(n + 1);

@@ -87,7 +87,17 @@ struct CsrMat : LinOp {
     void set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, std::vector<uint32_t> ci_,
                   std::vector<double> v_, hipStream_t s);
     void sync_host() const;  // download into the host mirror if needed
+    // take ownership of device arrays (ci, v with 4 padding entries); the
+    // host mirror is filled on demand
+    void adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&& rp_, DevBuf<uint32_t>&& ci_,
+               DevBuf<double>&& v_, hipStream_t s);
 };
+
+// the input step on the device (coo_device.hip)
+void coo_to_csr_device(uint32_t n, uint32_t m, uint64_t nin, const double* v, const uint32_t* r, const uint32_t* c,
+                       CsrMat& out, hipStream_t s);
+void condense_device(const CsrMat& A, int bs, uint32_t nbr, uint32_t nbc, std::vector<uint32_t>& rp,
+                     std::vector<uint32_t>& ci, std::vector<double>& v, hipStream_t s);
 
 // Epilogue kinds of the CSR SpMV kernel.
 enum SpmvEpi : int {

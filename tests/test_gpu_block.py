@@ -123,6 +123,29 @@ def test_T6_T7_block_ilu0():
     assert si["iterations"] <= sj["iterations"]                      # T7
 
 
+@pytest.mark.parametrize("bs", [1, 3])
+def test_condense_random_pattern(bs):
+    """BlockMatrix::condense on the device against the oracle (block sizes of
+    the reference's instantiations) on a random scalar CSR with unsorted and
+    duplicate columns, which the CSR ctor keeps."""
+    rng = np.random.default_rng(bs)
+    nb = 300
+    n = nb * bs
+    rows = []
+    for i in range(n):
+        cols = rng.integers(0, n, rng.integers(0, 12))
+        rows.append(cols)
+    rp = np.zeros(n + 1, np.uint32)
+    rp[1:] = np.cumsum([len(cc) for cc in rows])
+    ci = np.concatenate(rows).astype(np.uint32)
+    v = rng.uniform(-1, 1, ci.size)
+    A = F.CsrMatrix(n, n, rp, ci, v)
+    B = Bsr.condense(A, bs, nb, nb)
+    Bo = O.Bsr.condense(O.Csr(n, n, rp, ci, v), bs, nb, nb)
+    for a, b in zip(B.export(), Bo.export()):
+        assert np.array_equal(a, b)
+
+
 def test_block_ilu0_random_bsr_bitexact():
     Bg = gen.bsr7(6, seed=3)
     B = Bsr.create(3, Bg.nbr, Bg.nbc, Bg.row_ptr, Bg.col_idx, Bg.values)

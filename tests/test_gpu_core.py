@@ -81,6 +81,31 @@ def test_coo_constructor_matches_oracle():
         assert np.array_equal(a, b)
 
 
+@pytest.mark.parametrize("n,m,nin", [(20000, 15000, 400000), (7, 5, 0), (1, 1, 3)])
+def test_coo_device_conversion_large(n, m, nin):
+    """COO -> CSR on the device: duplicates (several per key, summed in input
+    order), explicit zeros, duplicates that cancel to zero, shuffled input;
+    bit-identical to the oracle's restatement of the reference ctor."""
+    rng = np.random.default_rng(11)
+    r = rng.integers(0, n, nin)
+    c = rng.integers(0, m, nin)
+    v = rng.uniform(-1, 1, nin)
+    if nin:
+        v[::13] = 0.0
+        k = nin // 4
+        r[k:2 * k] = r[:k]
+        c[k:2 * k] = c[:k]
+        v[k:k + 50] = -v[:50]          # pairs that cancel
+        perm = rng.permutation(nin)
+        r, c, v = r[perm], c[perm], v[perm]
+    A = F.CsrMatrix.from_coo(n, m, v, r, c)
+    Ao = O.Csr.from_coo(n, m, v, r, c)
+    for a, b in zip(A.export(), Ao.export()):
+        assert np.array_equal(a, b)
+    x = rng.standard_normal(m)
+    assert np.array_equal(A.spmv(x), Ao.spmv(x))
+
+
 def test_transpose_and_product():
     rng = np.random.default_rng(8)
     M = sp.random(80, 60, density=0.1, random_state=rng, format="csr")
