@@ -81,6 +81,7 @@ def c4_leg(F, gen, args, log):
     N = 96
     sa = dict(smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0, coarse_exact=1, fp32=1)
     rows = {k: [] for k in ("learned_mp", "learned_fp64", "jacobi_fp64", "ic0_fp64", "saamg_mp")}
+    want = list(rows) if args.c4_methods == "all" else args.c4_methods.split(",")
     worst = {"relres": 0.0, "rel_err": 0.0}
     fallbacks = 0
     for t, seed in enumerate(range(256, 256 + args.c4_test)):
@@ -105,21 +106,27 @@ def c4_leg(F, gen, args, log):
             return st
 
         Ml = F.LearnedPreconditioner(A, (N, N, N), net, fallback="ic0", budget=2000)
-        fallbacks += run("learned_mp", Ml, F.FP32_IR)["fallback"]
-        run("learned_fp64", Ml, F.FP64)
-        run("jacobi_fp64", F.Preconditioner(A, "diag"), F.FP64)
+        if "learned_mp" in want:
+            fallbacks += run("learned_mp", Ml, F.FP32_IR)["fallback"]
+        if "learned_fp64" in want:
+            run("learned_fp64", Ml, F.FP64)
+        if "jacobi_fp64" in want:
+            run("jacobi_fp64", F.Preconditioner(A, "diag"), F.FP64)
         if t % 4 == 0:
-            run("ic0_fp64", F.Preconditioner(A, "ic0"), F.FP64)
-            run("saamg_mp", F.Preconditioner(A, "saamg", **sa), F.FP32_IR)
+            if "ic0_fp64" in want:
+                run("ic0_fp64", F.Preconditioner(A, "ic0"), F.FP64)
+            if "saamg_mp" in want:
+                run("saamg_mp", F.Preconditioner(A, "saamg", **sa), F.FP32_IR)
         Ml.close()
         A.close()
     med = {k: (float(np.median([r[1] for r in v])), float(np.median([r[2] for r in v]))) for k, v in rows.items() if v}
     ic0_seeds = [r[0] for r in rows["ic0_fp64"]]
     lm = {r[0]: r[1] for r in rows["learned_mp"]}
-    speed = float(np.median([r[1] / lm[r[0]] for r in rows["ic0_fp64"]])) if ic0_seeds else None
+    speed = float(np.median([r[1] / lm[r[0]] for r in rows["ic0_fp64"]])) if ic0_seeds and lm else None
     for k, (ms, it) in med.items():
         log(f"[bench] c4 {k}: median {ms:.2f} ms, {it:.0f} iterations")
-    log(f"[bench] c4 learned MP vs FP64 IC0-PCG: {speed:.1f}x (median over seeds {ic0_seeds})")
+    if speed:
+        log(f"[bench] c4 learned MP vs FP64 IC0-PCG: {speed:.1f}x (median over seeds {ic0_seeds})")
     return {"workload": "3-D log-normal diffusion 96^3 (n=884736), sigma 1, correlation length 8 cells, tol 1e-10",
             "train": {"fields": args.c4_train, "field_cells": "32^3", "steps": steps, "seconds": round(train_s, 2),
                       "loss_first": round(float(hist[:10].mean()), 4), "loss_last": round(float(hist[-50:].mean()), 4)},
@@ -176,10 +183,13 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=128, help="grid side of the 3-D Poisson workload")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--methods", default="all", help="comma list: saamg_mp,saamg_fp64,ic0_fp64,jacobi_fp64")
+    ap.add_argument("--methods", default="all",
+                    help="comma list: saamg_mp,saamg_fp64,ic0_fp64,jacobi_fp64 ('none': no 128^3 solves)")
     ap.add_argument("--no-c4", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--c4-train", type=int, default=256, help="training fields (32^3 cells each)")
     ap.add_argument("--c4-test", type=int, default=32, help="held-out 96^3 fields")
+    ap.add_argument("--c4-methods", default="all",
+                    help="comma list of 96^3 comparisons: learned_mp,learned_fp64,jacobi_fp64,ic0_fp64,saamg_mp")
     ap.add_argument("--no-c5", action="store_true", help="skip the batched-solve leg")
     ap.add_argument("--c5-count", type=int, default=512, help="systems in the batch")
     ap.add_argument("--spmv-probe", action="store_true",
@@ -239,7 +249,7 @@ def main():
         "ic0_fp64": ("ic0", {}, F.FP64),
         "jacobi_fp64": ("diag", {}, F.FP64),
     }
-    want = list(methods) if args.methods == "all" else args.methods.split(",")
+    want = list(methods) if args.methods == "all" else [] if args.methods == "none" else args.methods.split(",")
     results = {}
     headline = None
     for name in want:
@@ -298,17 +308,20 @@ def main():
                          f"SpMV/dot on {threads} OpenMP threads, triangular sweeps serial as in the reference"}
         log(f"[bench] cpu oracle ic0-pcg {cpu_ms:.0f} ms it {ito}")
 
-    if headline is None:
-        headline = (results[want[0]]["solve_ms"], [results[want[0]]["solve_ms"]])
+    if headline is None and results:
+        first = next(iter(results.values()))["solve_ms"]
+        headline = (first, [first])
+    elif headline is None:
+        headline = (None, [])
     value = headline[0]
     out = {
         "metric": "solve time to 1e-10 rel-resid (learned/MP vs plain FP64), SpMV HBM GB/s",
-        "value": round(value, 3),
+        "value": round(value, 3) if value is not None else None,
         "unit": "ms per solve",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(value, 3),
+        "ms_per_step": round(value, 3) if value is not None else None,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,

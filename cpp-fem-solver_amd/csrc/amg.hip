@@ -184,32 +184,6 @@ static inline dim3 g1(uint32_t n) { return dim3((n + 255) / 256); }
 // ---------------------------------------------------------------------------
 // preconditioner
 // ---------------------------------------------------------------------------
-struct AmgLevelDev {
-    CsrMat A, P, R;
-    uint32_t n = 0, nc = 0;
-    double omega = 0.0;  // smoother weight
-    DevBuf<double> d;      // C/F: diagonal; SA: inverse diagonal
-    DevBuf<float> d32;
-    DevBuf<double> f, u, u2, res;
-    DevBuf<float> f32, u32, u232, res32;
-    // multicolour ordering for the SGS smoother
-    int ncol = 0;
-    bool use_gs = false;  // SGS smoother: colour-by-colour GS on this level
-    double jw = 0.0;      // SGS smoother: Jacobi weight when !use_gs
-    std::vector<uint32_t> h_cptr;
-    DevBuf<uint32_t> cptr, crows;
-    DevBuf<double> diag;  // a_ii for SGS (natural order)
-    DevBuf<float> diag32;
-    // colour-grouped copy of A for the GS sweeps; gblk: CSR-stream blocks of
-    // the grouped rows, colour c owning blocks [h_cblk[c], h_cblk[c+1])
-    int lanes = 1;
-    std::vector<uint32_t> h_cblk;
-    DevBuf<uint32_t> gblk;
-    DevBuf<uint32_t> gp, gci;
-    DevBuf<double> gv, gd;
-    DevBuf<float> gv32, gd32;
-};
-
 struct PcAmg : Precond {
     int kind_;
     bool device_setup = false;  // hierarchy built by build_sa_hierarchy_device
@@ -221,92 +195,40 @@ struct PcAmg : Precond {
     uint32_t nco = 0;
     bool fp32 = false;
 
-    PcAmg(int kind, AmgHierarchy&& H, const fem_pc_params& p, hipStream_t s)
+    PcAmg(int kind, AmgDevHierarchy&& H, const fem_pc_params& p, hipStream_t s)
         : kind_(kind), inv_diag(kind == FEM_PC_SAAMG || kind == FEM_PC_BSAAMG), smoother(p.smoother), npre(p.npre),
           npost(p.npost), omega(p.omega) {
-        const size_t L = H.A.size();
+        const size_t L = H.lv.size();
         levels = (int)L;
+        SetupTrace tr("AMG levels", s);
+        lv = std::move(H.lv);
         for (size_t k = 0; k < L; ++k) {
-            auto D = std::make_unique<AmgLevelDev>();
-            HCsr& A = H.A[k];
-            D->n = A.n;
-            D->omega = k < H.omega.size() ? H.omega[k] : p.omega;
-            lvl_rows.push_back(A.n);
-            lvl_nnz.push_back(A.v.size());
-            // the colour-grouped copy reads A before its arrays move to the level
-            if (smoother == FEM_SMOOTH_SGS && k + 1 < L && (int)H.cptr[k].size() - 1 <= kMaxGsColours) {
-                D->h_cptr = H.cptr[k];
-                group_rows(*D, A, H.crows[k], H.diag[k], s);
-            }
-            D->A.set_host(A.n, A.m, std::move(A.rp), std::move(A.ci), std::move(A.v), s);
-            D->A.stream = s;
-            if (k > 0) D->A.use_vector_rows();  // coarse Galerkin operators
+            AmgLevelDev& D = *lv[k];
+            lvl_rows.push_back(D.n);
+            lvl_nnz.push_back(D.A.nnz);
+            D.A.stream = s;
+            if (k > 0) D.A.use_vector_rows();  // coarse Galerkin operators
             if (k + 1 < L) {
-                HCsr& P = H.P[k];
-                D->nc = P.m;
-                HCsr R = transpose(P);
-                D->P.set_host(P.n, P.m, std::move(P.rp), std::move(P.ci), std::move(P.v), s);
-                D->R.set_host(R.n, R.m, std::move(R.rp), std::move(R.ci), std::move(R.v), s);
                 // transfer operators: summation order is free, long rows get
                 // lane groups
-                D->P.use_vector_rows();
-                D->R.use_vector_rows();
-                D->d.upload(H.d[k], s);
-                D->u.alloc(D->n);
-                D->u2.alloc(D->n);
-                D->res.alloc(D->n);
+                D.P.use_vector_rows();
+                D.R.use_vector_rows();
+                D.u.alloc(D.n);
+                D.u2.alloc(D.n);
+                D.res.alloc(D.n);
                 if (smoother == FEM_SMOOTH_SGS) {
-                    D->ncol = (int)H.cptr[k].size() - 1;
-                    D->use_gs = D->ncol <= kMaxGsColours;
-                    D->jw = H.jw[k];
-                    if (D->use_gs) {
-                        D->cptr.upload(H.cptr[k], s);
-                        D->crows.upload(H.crows[k], s);
-                    }
-                    D->diag.upload(H.diag[k], s);
+                    sgs_setup_device(D, s);
+                    tr.lap("smoother colouring and grouped copy");
                 }
             } else {
-                D->u.alloc(D->n);
+                D.u.alloc(D.n);
             }
-            if (k > 0) D->f.alloc(D->n);
-            lv.push_back(std::move(D));
+            if (k > 0) D.f.alloc(D.n);
         }
         nco = lv.back()->n;
         coarse.upload(H.coarse, s);
         FEM_HIP(hipStreamSynchronize(s));
-    }
-    // rows of A in colour order (crows) as a CSR of their own
-    static void group_rows(AmgLevelDev& D, const HCsr& A, const std::vector<uint32_t>& crows,
-                           const std::vector<double>& diag, hipStream_t s) {
-        const uint32_t n = A.n;
-        std::vector<uint32_t> gp(n + 1, 0), gci;
-        std::vector<double> gv, gd(n);
-        gci.reserve(A.ci.size());
-        gv.reserve(A.v.size());
-        for (uint32_t t = 0; t < n; ++t) {
-            const uint32_t i = crows[t];
-            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
-                gci.push_back(A.ci[p]);
-                gv.push_back(A.v[p]);
-            }
-            gp[t + 1] = (uint32_t)gci.size();
-            gd[t] = diag[i];
-        }
-        std::vector<uint32_t> blk;
-        const int ncol = (int)D.h_cptr.size() - 1;
-        D.h_cblk.assign(1, 0);
-        for (int c = 0; c < ncol; ++c) {
-            plan_rows(gp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk);
-            D.h_cblk.push_back((uint32_t)blk.size() - 1);
-        }
-        D.gblk.upload(blk, s);
-        const double avg = n ? (double)A.ci.size() / n : 0.0;
-        D.lanes = avg <= 1.5 ? 1 : avg <= 3 ? 4 : avg <= 12 ? 8 : avg <= 24 ? 16 : avg <= 48 ? 32 : 64;
-        D.gp.upload(gp, s);
-        D.gci.upload(gci, s);
-        D.gv.upload(gv, s);
-        D.gd.upload(gd, s);
-        FEM_HIP(hipStreamSynchronize(s));  // the host arrays end with this call
+        tr.lap("vectors, coarsest operator");
     }
     int kind() const override { return kind_; }
     uint32_t size() const override { return lv[0]->n; }
@@ -528,20 +450,17 @@ struct PcAmg : Precond {
 std::unique_ptr<Precond> make_amg_cf(CsrMat& A, const fem_pc_params& p, hipStream_t s) {
     if (A.n != A.m) fail(FEM_ERR_INVALID_ARGUMENT, "AMG needs a square matrix");
     A.sync_host();
-    AmgHierarchy H = build_cf_hierarchy(A, p);
+    AmgDevHierarchy H = upload_hierarchy(build_cf_hierarchy(A, p), s);
     return std::unique_ptr<Precond>(new PcAmg(FEM_PC_AMG, std::move(H), p, s));
 }
 
 std::unique_ptr<Precond> make_saamg(CsrMat& A, const fem_pc_params& p, hipStream_t s) {
     if (A.n != A.m) fail(FEM_ERR_INVALID_ARGUMENT, "Block AMG needs a square matrix");
-    A.sync_host();
+    SetupTrace tr("saamg setup", s);
     // the robust variant is built on the device when asked for and the rows
     // are sorted and duplicate free (the host path condenses them first)
-    bool dev = p.p_omega <= 0.0 && p.setup_device != 0;
-    for (uint32_t i = 0; dev && i < A.n; ++i)
-        for (uint32_t q = A.h_rp[i] + 1; q < A.h_rp[i + 1]; ++q)
-            if (A.h_ci[q] <= A.h_ci[q - 1]) { dev = false; break; }
-    AmgHierarchy H;
+    bool dev = p.p_omega <= 0.0 && p.setup_device != 0 && rows_sorted_device(A, s);
+    AmgDevHierarchy H;
     if (dev) {
         try {
             H = build_sa_hierarchy_device(A, p, s);
@@ -549,8 +468,13 @@ std::unique_ptr<Precond> make_saamg(CsrMat& A, const fem_pc_params& p, hipStream
             dev = false;
         }
     }
-    if (!dev) H = build_sa_hierarchy(A, p);
+    if (!dev) {
+        A.sync_host();
+        H = upload_hierarchy(build_sa_hierarchy(A, p), s);
+    }
+    tr.lap(dev ? "hierarchy (device)" : "hierarchy (host)");
     auto pc = std::unique_ptr<PcAmg>(new PcAmg(FEM_PC_SAAMG, std::move(H), p, s));
+    tr.lap("levels ready");
     pc->device_setup = dev;
     return pc;
 }

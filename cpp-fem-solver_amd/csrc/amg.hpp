@@ -1,6 +1,8 @@
 // amg.hpp — host-side AMG hierarchy description (amg_setup.cpp) consumed by
 // the device V-cycle (amg.hip).
 #pragma once
+#include <memory>
+
 #include "internal.hpp"
 
 namespace fem {
@@ -19,18 +21,48 @@ HCsr spgemm(const HCsr& A, const HCsr& B);
 // with 27+ entries per row colour greedily into 15-40 colours).
 constexpr int kMaxGsColours = 8;
 
+// A hierarchy as the host setups build it.
 struct AmgHierarchy {
     std::vector<HCsr> A;                  // level operators, A[0] = input
     std::vector<HCsr> P;                  // prolongators, P[k]: n_k x n_{k+1}
     std::vector<std::vector<double>> d;   // smoother diagonal (C/F: a_ii, SA: 1/a_ii)
     std::vector<double> coarse;           // dense coarse solution operator (row-major)
     std::vector<double> omega;            // smoother weight per level
-    // multicolour ordering per level (SGS smoother)
-    std::vector<std::vector<uint32_t>> cptr, crows;
-    std::vector<std::vector<double>> diag;
-    // SGS smoother: damped-Jacobi weight (4/3) / rho(D^{-1} A) (Gershgorin)
-    // for the levels whose colouring has more than kMaxGsColours colours
-    std::vector<double> jw;
+};
+
+// One level on the device: the operators, the smoother data and the V-cycle's
+// work vectors.
+struct AmgLevelDev {
+    CsrMat A, P, R;  // P, R: none on the coarsest level
+    uint32_t n = 0, nc = 0;
+    double omega = 0.0;  // smoother weight
+    DevBuf<double> d;      // C/F: diagonal; SA: inverse diagonal
+    DevBuf<float> d32;
+    DevBuf<double> f, u, u2, res;
+    DevBuf<float> f32, u32, u232, res32;
+    // SGS smoother: greedy multicolour ordering (colour of row i = the least
+    // colour that no lower-numbered neighbour in pattern(A + A^T) holds)
+    int ncol = 0;         // colours; kMaxGsColours + 1 stands for "more"
+    bool use_gs = false;  // colour-by-colour GS on this level
+    double jw = 0.0;      // Jacobi weight (4/3) / max_i sum_j |a_ij / a_ii| when !use_gs
+    std::vector<uint32_t> h_cptr;
+    DevBuf<uint32_t> cptr, crows;
+    DevBuf<double> diag;  // a_ii (natural order)
+    DevBuf<float> diag32;
+    // colour-grouped copy of A for the GS sweeps; gblk: CSR-stream blocks of
+    // the grouped rows, colour c owning blocks [h_cblk[c], h_cblk[c+1])
+    int lanes = 1;
+    std::vector<uint32_t> h_cblk;
+    DevBuf<uint32_t> gblk;
+    DevBuf<uint32_t> gp, gci;
+    DevBuf<double> gv, gd;
+    DevBuf<float> gv32, gd32;
+};
+
+// A hierarchy on the device, ready for the V-cycle (smoother data aside).
+struct AmgDevHierarchy {
+    std::vector<std::unique_ptr<AmgLevelDev>> lv;
+    std::vector<double> coarse;  // dense coarsest solution operator (row-major)
 };
 
 AmgHierarchy build_cf_hierarchy(const CsrMat& A, const fem_pc_params& p);
@@ -40,9 +72,16 @@ int mis2_aggregate(const HCsr& A, double theta, std::vector<int>& agg);
 // identical to build_sa_hierarchy's. Throws DeviceSetupTooLarge when a level
 // has more columns than the product's LDS bitmap holds (524288).
 struct DeviceSetupTooLarge {};
-AmgHierarchy build_sa_hierarchy_device(const CsrMat& A, const fem_pc_params& p, hipStream_t s);
-// host parts shared by both setups
+AmgDevHierarchy build_sa_hierarchy_device(const CsrMat& A, const fem_pc_params& p, hipStream_t s);
+// a host-built hierarchy moved to the device (R_k = P_k^T formed there)
+AmgDevHierarchy upload_hierarchy(AmgHierarchy&& H, hipStream_t s);
 std::vector<double> sa_coarse_operator_host(const HCsr& A, bool exact);
-void sgs_levels_host(AmgHierarchy& H);
+// R = P^T on the device; each row of R lists its columns in ascending order
+void transpose_device(const CsrMat& P, CsrMat& R, hipStream_t s);
+// SGS smoother data of level D (colouring, diagonal, Jacobi weight and, with
+// at most kMaxGsColours colours, the colour-grouped copy of D.A)
+void sgs_setup_device(AmgLevelDev& D, hipStream_t s);
+// true if every row of A lists strictly increasing columns
+bool rows_sorted_device(const CsrMat& A, hipStream_t s);
 
 }  // namespace fem

@@ -244,34 +244,27 @@ __global__ void k_colcount(const uint32_t* ci, uint64_t nnz, uint32_t* cnt) {
     if (p < nnz) atomicAdd(&cnt[ci[p]], 1u);
 }
 
-__global__ void k_tfill(const uint32_t* rp, const uint32_t* ci, const double* v, uint32_t n, uint32_t* cursor,
-                        uint32_t* tci, double* tv) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
-        const uint32_t q = atomicAdd(&cursor[ci[p]], 1u);
-        tci[q] = i;
-        tv[q] = v[p];
-    }
+__global__ void k_iota(uint32_t* x, uint64_t n) {
+    const uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k < n) x[k] = (uint32_t)k;
 }
 
-// sort each row of the transpose by column (insertion sort, rows are short)
-__global__ void k_rowsort(const uint32_t* rp, uint32_t n, uint32_t* ci, double* v) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= n) return;
-    const uint32_t b = rp[r], e = rp[r + 1];
-    for (uint32_t p = b + 1; p < e; ++p) {
-        const uint32_t c = ci[p];
-        const double x = v[p];
-        uint32_t q = p;
-        while (q > b && ci[q - 1] > c) {
-            ci[q] = ci[q - 1];
-            v[q] = v[q - 1];
-            --q;
-        }
-        ci[q] = c;
-        v[q] = x;
-    }
+// row index of every stored entry
+__global__ void k_entry_rows(const uint32_t* rp, uint32_t n, uint32_t* row) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) row[p] = i;
+}
+
+// entries of the transpose in (column, row) order: `idx` lists the entries
+// of P stably sorted by column, so each column keeps ascending rows
+__global__ void k_tgather(const uint32_t* idx, const uint32_t* row, const double* v, uint64_t nnz, uint32_t* tci,
+                          double* tv) {
+    const uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (k >= nnz) return;
+    const uint32_t e = idx[k];
+    tci[k] = row[e];
+    tv[k] = v[e];
 }
 
 // ---- sparse product C = A B --------------------------------------------------------
@@ -385,6 +378,16 @@ struct DCsr {
     uint64_t nnz = 0;
     DevBuf<uint32_t> rp, ci;
     DevBuf<double> v;
+    // entry arrays with the 4 zero entries past the end that CsrMat keeps
+    void alloc_entries(hipStream_t s) {
+        ci.alloc(nnz + 4);
+        v.alloc(nnz + 4);
+        FEM_HIP(hipMemsetAsync(ci.get() + nnz, 0, 4 * sizeof(uint32_t), s));
+        FEM_HIP(hipMemsetAsync(v.get() + nnz, 0, 4 * sizeof(double), s));
+    }
+    void to_mat(CsrMat& M, hipStream_t s) {
+        M.adopt(n, m, nnz, std::move(rp), std::move(ci), std::move(v), s);
+    }
 };
 
 constexpr uint32_t kMaxBitmapWords = 16384;  // 64 KB of LDS: B with up to 524288 columns
@@ -405,8 +408,7 @@ DCsr spgemm_dev(const DCsr& A, const DCsr& B, hipStream_t s) {
     FEM_HIP(hipMemsetAsync(cnt.get() + A.n, 0, 4, s));
     scan_u32(cnt.get(), C.rp.get(), (size_t)A.n + 1, s);
     C.nnz = read_u32(C.rp.get() + A.n, s);
-    C.ci.alloc(std::max<uint64_t>(C.nnz, 1));
-    C.v.alloc(std::max<uint64_t>(C.nnz, 1));
+    C.alloc_entries(s);
     hipLaunchKernelGGL((k_spgemm_sym<1>), dim3(grid), dim3(64), lds, s, A.rp.get(), A.ci.get(), B.rp.get(),
                        B.ci.get(), A.n, words, nullptr, C.rp.get(), C.ci.get());
     hipLaunchKernelGGL(k_spgemm_num, dim3(grid), dim3(64), 0, s, A.rp.get(), A.ci.get(), A.v.get(), B.rp.get(),
@@ -415,48 +417,346 @@ DCsr spgemm_dev(const DCsr& A, const DCsr& B, hipStream_t s) {
     return C;
 }
 
-DCsr transpose_dev(const DCsr& P, hipStream_t s) {
+// R = P^T for P given by its arrays (n x m, nnz entries)
+DCsr transpose_arrays(const uint32_t* prp, const uint32_t* pci, const double* pv, uint32_t n, uint32_t m,
+                      uint64_t nnz, hipStream_t s) {
     DCsr R;
-    R.n = P.m;
-    R.m = P.n;
-    R.nnz = P.nnz;
-    DevBuf<uint32_t> cnt(P.m + 1), cur(P.m + 1);
-    FEM_HIP(hipMemsetAsync(cnt.get(), 0, (size_t)(P.m + 1) * 4, s));
-    hipLaunchKernelGGL(k_colcount, dim3(g1(P.nnz)), dim3(256), 0, s, P.ci.get(), P.nnz, cnt.get());
-    R.rp.alloc(P.m + 1);
-    scan_u32(cnt.get(), R.rp.get(), (size_t)P.m + 1, s);
-    FEM_HIP(hipMemcpyAsync(cur.get(), R.rp.get(), (size_t)(P.m + 1) * 4, hipMemcpyDeviceToDevice, s));
-    R.ci.alloc(std::max<uint64_t>(R.nnz, 1));
-    R.v.alloc(std::max<uint64_t>(R.nnz, 1));
-    hipLaunchKernelGGL(k_tfill, dim3(g1(P.n)), dim3(256), 0, s, P.rp.get(), P.ci.get(), P.v.get(), P.n, cur.get(),
-                       R.ci.get(), R.v.get());
-    hipLaunchKernelGGL(k_rowsort, dim3(g1(R.n)), dim3(256), 0, s, R.rp.get(), R.n, R.ci.get(), R.v.get());
+    R.n = m;
+    R.m = n;
+    R.nnz = nnz;
+    DevBuf<uint32_t> cnt(m + 1);
+    FEM_HIP(hipMemsetAsync(cnt.get(), 0, (size_t)(m + 1) * 4, s));
+    if (nnz) hipLaunchKernelGGL(k_colcount, dim3(g1(nnz)), dim3(256), 0, s, pci, nnz, cnt.get());
+    R.rp.alloc(m + 1);
+    scan_u32(cnt.get(), R.rp.get(), (size_t)m + 1, s);
+    R.alloc_entries(s);
+    if (nnz == 0) {
+        FEM_HIP(hipStreamSynchronize(s));
+        return R;
+    }
+    // stable LSD radix sort of the column indices with the entry position as
+    // payload: within a column the entries stay in row order
+    DevBuf<uint32_t> keys(nnz), pos(nnz), idx(nnz), row(nnz);
+    hipLaunchKernelGGL(k_iota, dim3(g1(nnz)), dim3(256), 0, s, pos.get(), nnz);
+    hipLaunchKernelGGL(k_entry_rows, dim3(g1(n)), dim3(256), 0, s, prp, n, row.get());
+    int bits = 1;
+    while (bits < 32 && (1ull << bits) < (uint64_t)m) ++bits;
+    size_t tmp = 0;
+    FEM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, pci, keys.get(), pos.get(), idx.get(), (int64_t)nnz, 0,
+                                               bits, s));
+    DevBuf<char> t(std::max<size_t>(tmp, 1));
+    FEM_HIP(hipcub::DeviceRadixSort::SortPairs(t.get(), tmp, pci, keys.get(), pos.get(), idx.get(), (int64_t)nnz, 0,
+                                               bits, s));
+    hipLaunchKernelGGL(k_tgather, dim3(g1(nnz)), dim3(256), 0, s, idx.get(), row.get(), pv, nnz, R.ci.get(),
+                       R.v.get());
     FEM_HIP(hipGetLastError());
+    FEM_HIP(hipStreamSynchronize(s));  // the temporaries end with this call
     return R;
 }
 
-HCsr to_host(const DCsr& D, hipStream_t s) {
+DCsr transpose_dev(const DCsr& P, hipStream_t s) {
+    return transpose_arrays(P.rp.get(), P.ci.get(), P.v.get(), P.n, P.m, P.nnz, s);
+}
+
+HCsr to_host(const CsrMat& M) {
+    M.sync_host();
     HCsr H;
-    H.n = D.n;
-    H.m = D.m;
-    H.rp.resize(D.n + 1);
-    H.ci.resize(D.nnz);
-    H.v.resize(D.nnz);
-    D.rp.download(H.rp.data(), D.n + 1, s);
-    D.ci.download(H.ci.data(), D.nnz, s);
-    D.v.download(H.v.data(), D.nnz, s);
-    FEM_HIP(hipStreamSynchronize(s));
+    H.n = M.n;
+    H.m = M.m;
+    H.rp = M.h_rp;
+    H.ci = M.h_ci;
+    H.v = M.h_v;
     return H;
+}
+
+// ---- SGS smoother data ----------------------------------------------------------
+// The lower neighbours of row r in pattern(A + A^T): every off-diagonal entry
+// (i, j) lists min(i, j) under max(i, j) (a symmetric pattern lists each
+// neighbour twice, which the colouring does not mind).
+__global__ void k_low_count(const uint32_t* rp, const uint32_t* ci, uint32_t n, uint32_t* cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
+        const uint32_t j = ci[p];
+        if (j != i) atomicAdd(&cnt[j > i ? j : i], 1u);
+    }
+}
+
+__global__ void k_low_fill(const uint32_t* rp, const uint32_t* ci, uint32_t n, uint32_t* cur, uint32_t* lci) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
+        const uint32_t j = ci[p];
+        if (j == i) continue;
+        const uint32_t r = j > i ? j : i;
+        lci[atomicAdd(&cur[r], 1u)] = j > i ? i : j;
+    }
+}
+
+// One round of the greedy colouring: a row whose lower neighbours all hold a
+// colour takes the least colour none of them holds (colours from
+// kMaxGsColours on are stored as kMaxGsColours). A colour, once written, is
+// final, so reading a neighbour coloured earlier in the same round is as
+// good as reading it in the next one.
+__global__ void k_colour_round(const uint32_t* lrp, const uint32_t* lci, uint32_t n, int8_t* col, int* over) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    volatile int8_t* vc = col;
+    if (vc[i] >= 0) return;
+    uint32_t mask = 0;
+    for (uint32_t p = lrp[i]; p < lrp[i + 1]; ++p) {
+        const int c = vc[lci[p]];
+        if (c < 0) return;
+        mask |= 1u << c;
+    }
+    const uint32_t avail = ~mask & ((1u << kMaxGsColours) - 1u);
+    const int c = avail ? __builtin_ctz(avail) : kMaxGsColours;
+    vc[i] = (int8_t)c;
+    if (c == kMaxGsColours) *over = 1;
+}
+
+__global__ void k_any_uncoloured(const int8_t* col, uint32_t n, int* left) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && col[i] < 0) *left = 1;
+}
+
+// rows per colour: a histogram per block, added to the totals
+__global__ __launch_bounds__(256) void k_colour_hist(const int8_t* col, uint32_t n, uint32_t* cnt) {
+    __shared__ uint32_t h[kMaxGsColours + 1];
+    if (threadIdx.x <= kMaxGsColours) h[threadIdx.x] = 0;
+    __syncthreads();
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        atomicAdd(&h[col[i]], 1u);
+    __syncthreads();
+    if (threadIdx.x <= kMaxGsColours && h[threadIdx.x]) atomicAdd(&cnt[threadIdx.x], h[threadIdx.x]);
+}
+
+// a_ii (the first stored diagonal entry; 0 when there is none)
+__global__ void k_diag_val(const uint32_t* rp, const uint32_t* ci, const double* v, uint32_t n, double* d) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double a = 0.0;
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
+        if (ci[p] == i) { a = v[p]; break; }
+    d[i] = a;
+}
+
+// per-block max of sum_j |a_ij / a_ii| (row sums in stored order)
+__global__ __launch_bounds__(kBlock) void k_gersh_div(const uint32_t* rp, const double* v, const double* d, uint32_t n,
+                                                      double* partials) {
+    double mx = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        double rs = 0.0;
+        for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) rs += fabs(v[p] / d[i]);
+        mx = fmax(mx, rs);
+    }
+    __shared__ double sh[kBlock];
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = kBlock / 2; o > 0; o >>= 1) {
+        if (threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+__global__ void k_group_len(const uint32_t* rp, const uint32_t* rows, uint32_t n, uint32_t* len) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) len[t] = rp[rows[t] + 1] - rp[rows[t]];
+    if (t == n) len[t] = 0;
+}
+
+// row t of the grouped copy is row rows[t] of A, entries in stored order
+__global__ void k_group_fill(const uint32_t* rp, const uint32_t* ci, const double* v, const double* diag,
+                             const uint32_t* rows, uint32_t n, const uint32_t* gp, uint32_t* gci, double* gv,
+                             double* gd) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = rows[t];
+    uint32_t o = gp[t];
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p, ++o) {
+        gci[o] = ci[p];
+        gv[o] = v[p];
+    }
+    gd[t] = diag[i];
+}
+
+__global__ void k_rows_sorted(const uint32_t* rp, const uint32_t* ci, uint32_t n, int* bad) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    for (uint32_t p = rp[i] + 1; p < rp[i + 1]; ++p)
+        if (ci[p] <= ci[p - 1]) { *bad = 1; return; }
+}
+
+int read_i32(const int* p, hipStream_t s) {
+    int h = 0;
+    FEM_HIP(hipMemcpyAsync(&h, p, 4, hipMemcpyDeviceToHost, s));
+    FEM_HIP(hipStreamSynchronize(s));
+    return h;
 }
 
 }  // namespace
 
-// The robust SA hierarchy built on the device; the host copies of the level
-// operators and prolongators are filled for the consumer (PcAmg uploads and
-// colours from them) — a copy per level, made once per setup.
-AmgHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params& prm, hipStream_t s) {
+bool rows_sorted_device(const CsrMat& A, hipStream_t s) {
+    DevBuf<int> bad(1);
+    FEM_HIP(hipMemsetAsync(bad.get(), 0, 4, s));
+    hipLaunchKernelGGL(k_rows_sorted, dim3(g1(A.n)), dim3(256), 0, s, A.rp.get(), A.ci.get(), A.n, bad.get());
+    FEM_HIP(hipGetLastError());
+    return read_i32(bad.get(), s) == 0;
+}
+
+void transpose_device(const CsrMat& P, CsrMat& R, hipStream_t s) {
+    DCsr T = transpose_arrays(P.rp.get(), P.ci.get(), P.v.get(), P.n, P.m, P.nnz, s);
+    T.to_mat(R, s);
+}
+
+// Greedy multicolouring in row order, computed in rounds (a row is coloured
+// once all its lower neighbours are; the lowest uncoloured row always is, so
+// every round makes progress), stopped early once a row needs a colour
+// beyond kMaxGsColours. Then the diagonal, the Jacobi weight and, when the
+// colours fit, the rows grouped by colour (ascending row order within a
+// colour) with a CSR copy of A in that order and its CSR-stream blocking.
+void sgs_setup_device(AmgLevelDev& D, hipStream_t s) {
+    const CsrMat& A = D.A;
+    const uint32_t n = A.n;
+    D.diag.alloc(n);
+    hipLaunchKernelGGL(k_diag_val, dim3(g1(n)), dim3(256), 0, s, A.rp.get(), A.ci.get(), A.v.get(), n, D.diag.get());
+    {
+        const unsigned G = red_blocks(n);
+        DevBuf<double> part(G);
+        hipLaunchKernelGGL(k_gersh_div, dim3(G), dim3(kBlock), 0, s, A.rp.get(), A.v.get(), D.diag.get(), n,
+                           part.get());
+        std::vector<double> hp(G);
+        part.download(hp.data(), G, s);
+        FEM_HIP(hipStreamSynchronize(s));
+        double rho = 0.0;
+        for (double x : hp) rho = std::max(rho, x);
+        D.jw = rho > 0.0 ? (4.0 / 3.0) / rho : 2.0 / 3.0;
+    }
+    // lower-neighbour lists
+    DevBuf<uint32_t> cnt(n + 1), lrp(n + 1);
+    FEM_HIP(hipMemsetAsync(cnt.get(), 0, (size_t)(n + 1) * 4, s));
+    hipLaunchKernelGGL(k_low_count, dim3(g1(n)), dim3(256), 0, s, A.rp.get(), A.ci.get(), n, cnt.get());
+    scan_u32(cnt.get(), lrp.get(), (size_t)n + 1, s);
+    const uint32_t nlow = read_u32(lrp.get() + n, s);
+    DevBuf<uint32_t> lci(std::max<uint32_t>(nlow, 1));
+    FEM_HIP(hipMemcpyAsync(cnt.get(), lrp.get(), (size_t)(n + 1) * 4, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_low_fill, dim3(g1(n)), dim3(256), 0, s, A.rp.get(), A.ci.get(), n, cnt.get(), lci.get());
+    // colouring rounds, checked every kBatch rounds
+    DevBuf<int8_t> col(n);
+    DevBuf<int> flags(2);  // [0] a colour beyond the cap, [1] rows left
+    FEM_HIP(hipMemsetAsync(col.get(), 0xff, n, s));
+    FEM_HIP(hipMemsetAsync(flags.get(), 0, 8, s));
+    constexpr int kBatch = 32;
+    int rounds = 0;
+    bool over = false;
+    for (;;) {
+        for (int r = 0; r < kBatch; ++r)
+            hipLaunchKernelGGL(k_colour_round, dim3(g1(n)), dim3(256), 0, s, lrp.get(), lci.get(), n, col.get(),
+                               flags.get());
+        rounds += kBatch;
+        FEM_HIP(hipMemsetAsync(flags.get() + 1, 0, 4, s));
+        hipLaunchKernelGGL(k_any_uncoloured, dim3(g1(n)), dim3(256), 0, s, col.get(), n, flags.get() + 1);
+        int h[2];
+        FEM_HIP(hipMemcpyAsync(h, flags.get(), 8, hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipStreamSynchronize(s));
+        if (h[0]) { over = true; break; }
+        if (!h[1]) break;
+        if ((uint64_t)rounds > (uint64_t)n + kBatch) fail(FEM_ERR_RUNTIME, "AMG setup: colouring did not finish");
+    }
+    FEM_HIP(hipGetLastError());
+    if (over) {
+        D.ncol = kMaxGsColours + 1;
+        D.use_gs = false;
+        return;
+    }
+    DevBuf<uint32_t> hc(kMaxGsColours + 1);
+    FEM_HIP(hipMemsetAsync(hc.get(), 0, (kMaxGsColours + 1) * 4, s));
+    hipLaunchKernelGGL(k_colour_hist, dim3(std::min<unsigned>(g1(n), 1024)), dim3(256), 0, s, col.get(), n, hc.get());
+    std::vector<uint32_t> counts(kMaxGsColours + 1);
+    hc.download(counts.data(), kMaxGsColours + 1, s);
+    FEM_HIP(hipStreamSynchronize(s));
+    int ncol = 0;
+    for (int c = 0; c <= kMaxGsColours; ++c)
+        if (counts[c]) ncol = c + 1;
+    D.ncol = ncol;
+    D.use_gs = ncol <= kMaxGsColours;
+    D.h_cptr.assign(ncol + 1, 0);
+    for (int c = 0; c < ncol; ++c) D.h_cptr[c + 1] = D.h_cptr[c] + counts[c];
+    D.cptr.upload(D.h_cptr, s);
+    // rows stably sorted by colour
+    D.crows.alloc(std::max<uint32_t>(n, 1));
+    {
+        DevBuf<uint8_t> ksorted(std::max<uint32_t>(n, 1));
+        DevBuf<uint32_t> ids(std::max<uint32_t>(n, 1));
+        hipLaunchKernelGGL(k_iota, dim3(g1(n)), dim3(256), 0, s, ids.get(), (uint64_t)n);
+        const uint8_t* keys = reinterpret_cast<const uint8_t*>(col.get());
+        size_t tmp = 0;
+        FEM_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, keys, ksorted.get(), ids.get(), D.crows.get(),
+                                                   (int64_t)n, 0, 4, s));
+        DevBuf<char> t(std::max<size_t>(tmp, 1));
+        FEM_HIP(hipcub::DeviceRadixSort::SortPairs(t.get(), tmp, keys, ksorted.get(), ids.get(), D.crows.get(),
+                                                   (int64_t)n, 0, 4, s));
+        FEM_HIP(hipStreamSynchronize(s));
+    }
+    // the grouped copy
+    DevBuf<uint32_t> len(n + 1);
+    hipLaunchKernelGGL(k_group_len, dim3(g1(n + 1)), dim3(256), 0, s, A.rp.get(), D.crows.get(), n, len.get());
+    D.gp.alloc(n + 1);
+    scan_u32(len.get(), D.gp.get(), (size_t)n + 1, s);
+    D.gci.alloc(A.nnz + 4);
+    D.gv.alloc(A.nnz + 4);
+    D.gd.alloc(std::max<uint32_t>(n, 1));
+    FEM_HIP(hipMemsetAsync(D.gci.get() + A.nnz, 0, 4 * sizeof(uint32_t), s));
+    FEM_HIP(hipMemsetAsync(D.gv.get() + A.nnz, 0, 4 * sizeof(double), s));
+    hipLaunchKernelGGL(k_group_fill, dim3(g1(n)), dim3(256), 0, s, A.rp.get(), A.ci.get(), A.v.get(), D.diag.get(),
+                       D.crows.get(), n, D.gp.get(), D.gci.get(), D.gv.get(), D.gd.get());
+    FEM_HIP(hipGetLastError());
+    std::vector<uint32_t> gp(n + 1);
+    D.gp.download(gp.data(), n + 1, s);
+    FEM_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> blk;
+    D.h_cblk.assign(1, 0);
+    for (int c = 0; c < ncol; ++c) {
+        plan_rows(gp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk);
+        D.h_cblk.push_back((uint32_t)blk.size() - 1);
+    }
+    D.gblk.upload(blk, s);
+    const double avg = n ? (double)A.nnz / n : 0.0;
+    D.lanes = avg <= 1.5 ? 1 : avg <= 3 ? 4 : avg <= 12 ? 8 : avg <= 24 ? 16 : avg <= 48 ? 32 : 64;
+    FEM_HIP(hipStreamSynchronize(s));  // the host arrays end with this call
+}
+
+AmgDevHierarchy upload_hierarchy(AmgHierarchy&& H, hipStream_t s) {
+    AmgDevHierarchy D;
+    const size_t L = H.A.size();
+    for (size_t k = 0; k < L; ++k) {
+        auto lv = std::make_unique<AmgLevelDev>();
+        HCsr& A = H.A[k];
+        lv->n = A.n;
+        lv->omega = H.omega[k];
+        lv->A.set_host(A.n, A.m, std::move(A.rp), std::move(A.ci), std::move(A.v), s);
+        if (k + 1 < L) {
+            HCsr& P = H.P[k];
+            lv->nc = P.m;
+            lv->P.set_host(P.n, P.m, std::move(P.rp), std::move(P.ci), std::move(P.v), s);
+            transpose_device(lv->P, lv->R, s);
+            lv->d.upload(H.d[k], s);
+        }
+        D.lv.push_back(std::move(lv));
+    }
+    D.coarse = std::move(H.coarse);
+    FEM_HIP(hipStreamSynchronize(s));
+    return D;
+}
+
+// The robust SA hierarchy built on the device; every level operator,
+// prolongator and restriction stays there (only the coarsest operator is
+// copied to the host, for its dense solve).
+AmgDevHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params& prm, hipStream_t s) {
     if (!(prm.p_omega <= 0.0)) fail(FEM_ERR_RUNTIME, "device SA setup is the robust variant only");
-    AmgHierarchy H;
+    AmgDevHierarchy H;
+    SetupTrace tr("device SA", s);
     const double theta = prm.theta;
     const int maxl = std::max(1, prm.max_levels);
     const uint32_t cmax = (uint32_t)std::max(1, prm.coarse_size);
@@ -468,8 +768,7 @@ AmgHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params& pr
     cur.m = A0.m;
     cur.nnz = A0.nnz;
     cur.rp.alloc(A0.n + 1);
-    cur.ci.alloc(std::max<uint64_t>(A0.nnz, 1));
-    cur.v.alloc(std::max<uint64_t>(A0.nnz, 1));
+    cur.alloc_entries(s);
     FEM_HIP(hipMemcpyAsync(cur.rp.get(), A0.rp.get(), (size_t)(A0.n + 1) * 4, hipMemcpyDeviceToDevice, s));
     FEM_HIP(hipMemcpyAsync(cur.ci.get(), A0.ci.get(), A0.nnz * 4, hipMemcpyDeviceToDevice, s));
     FEM_HIP(hipMemcpyAsync(cur.v.get(), A0.v.get(), A0.nnz * 8, hipMemcpyDeviceToDevice, s));
@@ -477,6 +776,8 @@ AmgHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params& pr
     DevBuf<double> part(kMaxRedBlocks);
     for (int l = 0; l < maxl; ++l) {
         const uint32_t n = cur.n;
+        auto lv = std::make_unique<AmgLevelDev>();
+        lv->n = n;
         DevBuf<double> dinv(n);
         const int big = 0x7fffffff;
         FEM_HIP(hipMemcpyAsync(flag.get(), &big, 4, hipMemcpyHostToDevice, s));
@@ -493,10 +794,14 @@ AmgHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params& pr
         double rho = 0.0;
         for (double x : hp) rho = std::max(rho, x);
         const double wauto = rho > 0.0 ? (4.0 / 3.0) / rho : 2.0 / 3.0;
-        H.omega.push_back(auto_s ? wauto : prm.omega);
+        lv->omega = auto_s ? wauto : prm.omega;
         const double pw = wauto;
-        H.A.push_back(to_host(cur, s));
-        if (n <= cmax) break;
+        tr.lap("diagonal, Gershgorin");
+        if (n <= cmax) {
+            cur.to_mat(lv->A, s);
+            H.lv.push_back(std::move(lv));
+            break;
+        }
         // strength and MIS-2 aggregation
         DevBuf<double> mrow(n);
         DevBuf<uint8_t> sm(std::max<uint64_t>(cur.nnz, 1)), st(n);
@@ -528,7 +833,12 @@ AmgHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params& pr
         const uint32_t nleft = read_u32(pos.get() + n, s);
         hipLaunchKernelGGL(k_phase3, dim3(g1(n)), dim3(256), 0, s, pos.get(), nroots, n, agg.get());
         const uint32_t na = nroots + nleft;
-        if ((double)na > 0.85 * (double)n) break;  // coarsening stalled
+        tr.lap("strength, MIS-2 aggregation");
+        if ((double)na > 0.85 * (double)n) {  // coarsening stalled
+            cur.to_mat(lv->A, s);
+            H.lv.push_back(std::move(lv));
+            break;
+        }
         // smoothed prolongator
         DCsr P;
         P.n = n;
@@ -539,25 +849,36 @@ AmgHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params& pr
                            flag.get());
         scan_u32(f.get(), P.rp.get(), (size_t)n + 1, s);
         P.nnz = read_u32(P.rp.get() + n, s);
-        P.ci.alloc(P.nnz);
-        P.v.alloc(P.nnz);
+        P.alloc_entries(s);
         hipLaunchKernelGGL(k_pfill, dim3(g1(n)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), cur.v.get(), agg.get(),
                            dinv.get(), pw, n, P.rp.get(), P.ci.get(), P.v.get(), flag.get());
         int ovf = 0;
         FEM_HIP(hipMemcpyAsync(&ovf, flag.get(), 4, hipMemcpyDeviceToHost, s));
         FEM_HIP(hipStreamSynchronize(s));
         if (ovf) fail(FEM_ERR_RUNTIME, "AMG setup: a prolongator row has more than 64 aggregates");
+        tr.lap("prolongator");
         DCsr R = transpose_dev(P, s);
+        tr.lap("transpose");
         DCsr AP = spgemm_dev(cur, P, s);
         DCsr Ac = spgemm_dev(R, AP, s);
-        std::vector<double> hd(n);
-        dinv.download(hd.data(), n, s);
-        H.P.push_back(to_host(P, s));
-        H.d.push_back(std::move(hd));
+        tr.lap("Galerkin products");
+        lv->nc = na;
+        lv->d = std::move(dinv);
+        cur.to_mat(lv->A, s);
+        P.to_mat(lv->P, s);
+        R.to_mat(lv->R, s);
+        H.lv.push_back(std::move(lv));
         cur = std::move(Ac);
+        tr.lap("level operators kept");
     }
-    H.coarse = sa_coarse_operator_host(H.A.back(), prm.coarse_exact != 0);
-    if (prm.smoother == FEM_SMOOTH_SGS) sgs_levels_host(H);
+    // a hierarchy cut by max_levels ends at the last level built, whose
+    // transfer operators go unused
+    AmgLevelDev& last = *H.lv.back();
+    last.P = CsrMat();
+    last.R = CsrMat();
+    last.nc = 0;
+    H.coarse = sa_coarse_operator_host(to_host(last.A), prm.coarse_exact != 0);
+    tr.lap("coarsest operator");
     return H;
 }
 

@@ -72,61 +72,6 @@ static HCsr from_mat(const CsrMat& A) {
     return H;
 }
 
-// greedy multicolouring of pattern(A + A^T) in row order (SGS smoother)
-static void colour(const HCsr& A, std::vector<uint32_t>& cptr, std::vector<uint32_t>& rows, std::vector<double>& dg) {
-    const uint32_t n = A.n;
-    HCsr T = transpose(A);
-    std::vector<int> c(n, -1);
-    std::vector<char> used(n + 2, 0);
-    std::vector<int> touched;
-    int nc = 0;
-    for (uint32_t i = 0; i < n; ++i) {
-        touched.clear();
-        auto mark = [&](uint32_t j) {
-            if (j == i || c[j] < 0 || used[c[j]]) return;
-            used[c[j]] = 1;
-            touched.push_back(c[j]);
-        };
-        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) mark(A.ci[p]);
-        for (uint32_t p = T.rp[i]; p < T.rp[i + 1]; ++p) mark(T.ci[p]);
-        int k = 0;
-        while (used[k]) ++k;
-        c[i] = k;
-        nc = std::max(nc, k + 1);
-        for (int t : touched) used[t] = 0;
-    }
-    cptr.assign(nc + 1, 0);
-    for (uint32_t i = 0; i < n; ++i) cptr[c[i] + 1]++;
-    for (int k = 0; k < nc; ++k) cptr[k + 1] += cptr[k];
-    rows.resize(n);
-    std::vector<uint32_t> off(cptr.begin(), cptr.end() - 1);
-    for (uint32_t i = 0; i < n; ++i) rows[off[c[i]]++] = i;
-    dg.assign(n, 0.0);
-    for (uint32_t i = 0; i < n; ++i)
-        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
-            if (A.ci[p] == i) { dg[i] = A.v[p]; break; }
-}
-
-// SGS smoother data for every level but the coarsest: the multicolour
-// ordering, the diagonal and the damped-Jacobi fallback weight.
-void sgs_levels_host(AmgHierarchy& H) {
-    for (size_t k = 0; k + 1 < H.A.size(); ++k) {
-        H.cptr.emplace_back();
-        H.crows.emplace_back();
-        H.diag.emplace_back();
-        colour(H.A[k], H.cptr.back(), H.crows.back(), H.diag.back());
-        const HCsr& A = H.A[k];
-        const std::vector<double>& d = H.diag.back();
-        double rho = 0.0;
-        for (uint32_t i = 0; i < A.n; ++i) {
-            double rs = 0.0;
-            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) rs += std::fabs(A.v[p] / d[i]);
-            rho = std::max(rho, rs);
-        }
-        H.jw.push_back(rho > 0.0 ? (4.0 / 3.0) / rho : 2.0 / 3.0);
-    }
-}
-
 static void check_coarse(uint32_t nc) {
     if (nc > 4096) fail(FEM_ERR_RUNTIME, "AMG: coarsest level has " + std::to_string(nc) +
                                              " rows; raise max_levels or coarse_size");
@@ -270,7 +215,6 @@ AmgHierarchy build_cf_hierarchy(const CsrMat& A0, const fem_pc_params& prm) {
         H.A.push_back(std::move(Ac));
     }
     H.coarse = cf_coarse_operator(H.A.back());
-    if (prm.smoother == FEM_SMOOTH_SGS) sgs_levels_host(H);
     return H;
 }
 
@@ -559,7 +503,6 @@ AmgHierarchy build_sa_hierarchy(const CsrMat& A0, const fem_pc_params& prm) {
         cur = std::move(Ac);
     }
     H.coarse = sa_coarse_operator_host(H.A.back(), prm.coarse_exact != 0);
-    if (prm.smoother == FEM_SMOOTH_SGS) sgs_levels_host(H);
     return H;
 }
 

@@ -8,6 +8,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <functional>
@@ -159,5 +160,30 @@ enum SolverErr : int {
 };
 
 const char* solver_err_msg(int code);
+
+// Setup-phase timing: with FEM_TRACE_SETUP=1 in the environment, lap() waits
+// for the stream and prints "[femgpu] <scope>: <phase> <ms>" to stderr, the
+// host wall time since the previous lap. Without it, lap() does nothing.
+bool trace_setup();
+struct SetupTrace {
+    const char* scope;
+    hipStream_t s;
+    bool on;
+    std::chrono::steady_clock::time_point t0;
+    SetupTrace(const char* sc, hipStream_t st) : scope(sc), s(st), on(trace_setup()) {
+        if (on) {
+            (void)hipStreamSynchronize(s);
+            t0 = std::chrono::steady_clock::now();
+        }
+    }
+    void lap(const char* phase) {
+        if (!on) return;
+        (void)hipStreamSynchronize(s);
+        const auto t1 = std::chrono::steady_clock::now();
+        std::fprintf(stderr, "[femgpu] %s: %s %.3f ms\n", scope, phase,
+                     std::chrono::duration<double, std::milli>(t1 - t0).count());
+        t0 = t1;
+    }
+};
 
 }  // namespace fem

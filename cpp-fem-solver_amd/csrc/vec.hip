@@ -1,5 +1,6 @@
 // vec.hip — elementwise vector kernels, deterministic dot product, and the
 // library-wide device/bookkeeping helpers.
+#include <cstdlib>
 #include <mutex>
 
 #include "internal.hpp"
@@ -18,6 +19,14 @@ void ensure_device() {
     });
     if (g_dev < 0) fail(FEM_ERR_DEVICE, "no HIP device is available (libfemgpu has no CPU fallback)");
     FEM_HIP(hipSetDevice(g_dev));
+}
+
+bool trace_setup() {
+    static const bool on = [] {
+        const char* e = std::getenv("FEM_TRACE_SETUP");
+        return e && *e && std::string(e) != "0";
+    }();
+    return on;
 }
 
 const char* solver_err_msg(int code) {

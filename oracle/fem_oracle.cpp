@@ -1113,13 +1113,75 @@ struct PcBlockAMG : Pc {
         uint32_t n;
         double om;  // smoother weight of this level
     };
+    // SGS smoother (BS = 1 only; no counterpart in the reference): per level
+    // the greedy colouring of pattern(A + A^T) in row order (row i takes the
+    // least colour none of its lower-numbered neighbours holds). With at most
+    // kSgsColours colours the level is smoothed by Gauss-Seidel colour by
+    // colour (ascending before the coarse correction, descending after it),
+    // u_i <- u_i + (w (f_i - sum_j a_ij u_j)) / a_ii with w the level's
+    // smoother weight; otherwise by damped Jacobi with a_ii and the weight
+    // (4/3) / max_i sum_j |a_ij / a_ii|.
+    static constexpr int kSgsColours = 8;
+    struct Sgs {
+        bool gs = false;
+        std::vector<std::vector<uint32_t>> rows;  // rows of each colour, ascending
+        vec diag;
+        double jw = 0.0;
+    };
     std::vector<Level> lv;
+    std::vector<Sgs> sg;
     double omega;
     int npre, npost;
     bool coarse_exact;
+    int smoother;  // 0 Jacobi (the reference's), 1 SGS
     PcBlockAMG(const Bsr<BS>& A0, double theta, double om, int pre, int post, int max_levels, int max_agg,
-               uint32_t coarse_max, double p_omega, bool exact = false)
-        : omega(om), npre(pre), npost(post), coarse_exact(exact) {
+               uint32_t coarse_max, double p_omega, bool exact = false, int smoother_kind = 0)
+        : omega(om), npre(pre), npost(post), coarse_exact(exact), smoother(smoother_kind) {
+        if (smoother != 0 && BS != 1) throw std::invalid_argument("the SGS smoother is for 1x1 blocks only");
+        build(A0, theta, om, max_levels, max_agg, coarse_max, p_omega);
+        if (smoother != 0)
+            for (size_t l = 0; l + 1 < lv.size(); ++l) sg.push_back(sgs_setup(lv[l].A));
+    }
+    static Sgs sgs_setup(const Bsr<BS>& A) {
+        Sgs S;
+        const uint32_t n = A.nbr;
+        S.diag.assign(n, 0.0);
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
+                if (A.ci[p] == i) { S.diag[i] = A.blk(p)[0]; break; }
+        double rho = 0.0;
+        for (uint32_t i = 0; i < n; ++i) {
+            double rs = 0.0;
+            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) rs += std::fabs(A.blk(p)[0] / S.diag[i]);
+            rho = std::max(rho, rs);
+        }
+        S.jw = rho > 0.0 ? (4.0 / 3.0) / rho : 2.0 / 3.0;
+        std::vector<std::vector<uint32_t>> low(n);
+        for (uint32_t i = 0; i < n; ++i)
+            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+                const uint32_t j = A.ci[p];
+                if (j != i) low[std::max(i, j)].push_back(std::min(i, j));
+            }
+        std::vector<int> col(n, -1);
+        int ncol = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            std::vector<char> used(low[i].size() + 2, 0);
+            for (uint32_t j : low[i])
+                if ((size_t)col[j] < used.size()) used[col[j]] = 1;
+            int c = 0;
+            while (used[c]) ++c;
+            col[i] = c;
+            ncol = std::max(ncol, c + 1);
+        }
+        S.gs = ncol <= kSgsColours;
+        if (S.gs) {
+            S.rows.assign(ncol, {});
+            for (uint32_t i = 0; i < n; ++i) S.rows[col[i]].push_back(i);
+        }
+        return S;
+    }
+    void build(const Bsr<BS>& A0, double theta, double om, int max_levels, int max_agg, uint32_t coarse_max,
+               double p_omega) {
         // Weights <= 0 select the robust variant (no counterpart in the
         // reference): per level w = (4/3) / rho with rho the Gershgorin bound
         // max_row sum |D^{-1} A|, for the smoother (om <= 0) and for the
@@ -1218,6 +1280,27 @@ struct PcBlockAMG : Pc {
         for (int i = 0; i < BV; ++i) s += b[i] * b[i];
         return std::sqrt(s);
     }
+    void smooth_sgs(size_t l, const vec& rhs, vec& sol, int ns, bool reverse) {
+        const Level& L = lv[l];
+        const Sgs& S = sg[l];
+        const uint32_t n = L.n;
+        if (!S.gs) {
+            vec tmp;
+            for (int s = 0; s < ns; ++s) {
+                L.A.matvec(sol, tmp);
+                for (uint32_t i = 0; i < n; ++i) sol[i] = sol[i] + (S.jw * (rhs[i] - tmp[i])) / S.diag[i];
+            }
+            return;
+        }
+        const int nc = (int)S.rows.size();
+        for (int s = 0; s < ns; ++s)
+            for (int cc = 0; cc < nc; ++cc)
+                for (uint32_t i : S.rows[reverse ? nc - 1 - cc : cc]) {
+                    double acc = 0.0;
+                    for (uint32_t p = L.A.rp[i]; p < L.A.rp[i + 1]; ++p) acc += L.A.blk(p)[0] * sol[L.A.ci[p]];
+                    sol[i] = sol[i] + (L.om * (rhs[i] - acc)) / S.diag[i];
+                }
+    }
     void smooth(const Level& L, const vec& rhs, vec& sol, int ns) {
         const uint32_t n = L.n, nd = n * BS;
         vec tmp, r(nd);
@@ -1307,7 +1390,8 @@ struct PcBlockAMG : Pc {
         const uint32_t nd = L.n * BS;
         sol.resize(nd);
         if (l + 1 == lv.size()) { coarse_solve(L, rhs, sol); return; }
-        smooth(L, rhs, sol, npre);
+        if (smoother) smooth_sgs(l, rhs, sol, npre, false);
+        else smooth(L, rhs, sol, npre);
         vec tmp, r(nd), rc, ec, pec;
         L.A.matvec(sol, tmp);
         for (uint32_t i = 0; i < nd; ++i) r[i] = rhs[i] - tmp[i];
@@ -1315,7 +1399,8 @@ struct PcBlockAMG : Pc {
         vcycle(l + 1, rc, ec);
         L.P.matvec(ec, pec);
         for (uint32_t i = 0; i < nd; ++i) sol[i] += pec[i];
-        smooth(L, rhs, sol, npost);
+        if (smoother) smooth_sgs(l, rhs, sol, npost, true);
+        else smooth(L, rhs, sol, npost);
     }
     void apply(const vec& r, vec& z) override {
         z.assign(r.size(), 0.0);
@@ -1664,6 +1749,7 @@ int or_dot(uint64_t n, const double* a, const double* b, double* out) {
 // params: [0] omega (SSOR) | AMG: [0] theta [1] coarse_thresh [2] max_levels
 // SAAMG: [0] theta [1] smoother omega [2] npre [3] npost [4] max_levels
 //        [5] max_agg [6] coarse_max [7] prolongator omega [8] exact coarse solve
+//        [9] smoother (0 Jacobi, 1 SGS)
 int or_pc_create(const Csr* A, int kind, const double* prm, OrPc** out) {
     return guard([&] {
         auto h = std::make_unique<OrPc>();
@@ -1680,10 +1766,10 @@ int or_pc_create(const Csr* A, int kind, const double* prm, OrPc** out) {
             case K_SAAMG: {
                 if (A->n != A->m) throw std::invalid_argument("Block AMG needs a square matrix");
                 Bsr<1> B = condense<1>(*A, A->n, A->m);
-                const double d[9] = {0.25, 2.0 / 3.0, 1, 1, 20, 4, 50, 2.0 / 3.0, 0};
+                const double d[10] = {0.25, 2.0 / 3.0, 1, 1, 20, 4, 50, 2.0 / 3.0, 0, 0};
                 const double* q = prm ? prm : d;
                 h->pc.reset(new PcBlockAMG<1>(B, q[0], q[1], (int)q[2], (int)q[3], (int)q[4], (int)q[5], (uint32_t)q[6], q[7],
-                                              q[8] != 0.0));
+                                              q[8] != 0.0, (int)q[9]));
                 break;
             }
             default: throw std::invalid_argument("unknown preconditioner kind");

@@ -151,6 +151,9 @@ class Pc:
         if params is None:
             pp = None
         else:
+            params = list(params)
+            if kind == K_SAAMG and len(params) == 9:
+                params.append(0)  # Jacobi smoother
             self._pp = np.ascontiguousarray(params, dtype=np.float64)
             pp = _ptr(self._pp)
         if A is not None:

@@ -150,6 +150,29 @@ def test_robust_saamg_lognormal_parity():
     assert st["true_relres"] < 1.5e-10 and rel(x, xo) < 1e-8 and abs(st["iterations"] - ito) <= 2
 
 
+@pytest.mark.parametrize("case,omega", [("poisson3d", 1.0), ("vardiff3d", 0.0), ("poisson2d", 1.0)])
+def test_saamg_sgs_vcycle_matches_oracle(case, omega):
+    """The SGS V-cycle (greedy colouring computed on the device, colour-grouped
+    sweeps, Jacobi on levels with more than 8 colours) against the oracle's
+    restatement, float64: levels of more than 8192 rows take the CSR-stream
+    sweeps and the residual shortcut after a weight-1 forward sweep, smaller
+    levels the single-workgroup sweeps."""
+    G = {"poisson3d": lambda: gen.poisson3d(24), "vardiff3d": lambda: gen.vardiff3d(22, seed=4)[0],
+         "poisson2d": lambda: gen.poisson2d(150)}[case]()
+    A = F.CsrMatrix.from_gen(G)
+    Ao = O.Csr.from_gen(G)
+    M = F.Preconditioner(A, "saamg", smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=omega, p_omega=0.0,
+                         coarse_exact=1)
+    Mo = O.Pc(Ao, O.K_SAAMG, [0.25, omega, 2, 2, 20, 4, 50, 0.0, 1, 1])
+    rows, nnz = Mo.amg_levels()
+    info = M.info()
+    assert info["levels"] == len(rows) > 2
+    assert np.array_equal(info["rows"][: len(rows)], rows) and np.array_equal(info["nnz"][: len(nnz)], nnz)
+    r = np.random.default_rng(7).standard_normal(G.n)
+    zo = Mo.apply(r)
+    np.testing.assert_allclose(M.apply(r), zo, rtol=1e-11, atol=1e-11 * np.abs(zo).max())
+
+
 def test_saamg_sgs_is_symmetric_and_converges():
     G = gen.poisson3d(20)
     A = F.CsrMatrix.from_gen(G)
