@@ -71,7 +71,7 @@ def c4_leg(F, gen, args, log):
     net = F.Mlp(seed=0)
     t0 = time.time()
     for f in fields:
-        f.errors = Lr.error_vectors(f)
+        f.errors = Lr.error_vectors(f) if args.c4_errors == "ic0" else np.empty((0, f.G.n))
     err_s = time.time() - t0
     steps = 3 * len(fields)
     hist, train_s = Lr.train(net, fields, steps=steps)
@@ -137,6 +137,38 @@ def c4_leg(F, gen, args, log):
             "fallbacks": int(fallbacks), "worst_true_relres": worst["relres"], "worst_rel_err": worst["rel_err"]}
 
 
+def c3_leg(F, gen, args, log):
+    """Nonsymmetric 2-D upwind convection-diffusion, 1024^2 cells, cell Peclet
+    10, velocity (cos 2 pi y, sin 2 pi x): BiCGSTAB and GMRES(30) with the
+    level-scheduled ILU(0) in float64, tol 1e-10, b and x in HBM; one timed
+    solve each after the preconditioner is built."""
+    G = gen.convdiff2d(args.c3_n)
+    xt, b = gen.rhs(G, 0)
+    A = F.CsrMatrix.from_gen(G)
+    bd = F.DeviceArray.from_host(b)
+    xd = F.DeviceArray(G.n)
+    t = time.time()
+    M = F.Preconditioner(A, "ilu0")
+    setup_ms = (time.time() - t) * 1e3
+    out = {"workload": f"2-D upwind convection-diffusion {args.c3_n}^2 (n={G.n}, nnz={G.nnz}), cell Peclet 10, "
+                       "ILU(0), float64, tol 1e-10",
+           "ilu0_setup_ms": round(setup_ms, 1), "ilu0_levels": M.info()["levels"]}
+    for name, solver in (("bicgstab_ilu0", "bicgstab"), ("gmres30_ilu0", "gmres")):
+        S = F.Solver(A, M, solver=solver, precision=F.FP64, max_iter=args.c3_max_iter, restart=30, device_ptrs=True,
+                     tol=1e-10)
+        t = time.perf_counter()
+        st = S.solve_device(bd.address, xd.address)
+        ms = (time.perf_counter() - t) * 1e3
+        x = xd.download()
+        err = float(np.linalg.norm(x - xt) / np.linalg.norm(xt))
+        out[name] = {"solve_ms": round(ms, 1), "iterations": st["iterations"], "converged": st["converged"],
+                     "relres": st["true_relres"], "rel_err": err}
+        log(f"[bench] c3 {name}: {ms:.0f} ms, {st['iterations']} iterations, converged {st['converged']}, "
+            f"relres {st['true_relres']:.2e}")
+        S.close()
+    return out
+
+
 def c5_leg(F, gen, args, log):
     """Batch of independent SPD systems (half 2-D Poisson, side U{32..256};
     half 3x3-block 7-point systems, side U{16..48}; 4 right-hand sides each;
@@ -188,9 +220,14 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--c4-train", type=int, default=256, help="training fields (32^3 cells each)")
     ap.add_argument("--c4-test", type=int, default=32, help="held-out 96^3 fields")
+    ap.add_argument("--c4-errors", choices=("ic0", "none"), default="ic0",
+                    help="training probes besides random signs: IC(0)-PCG error vectors, or none")
     ap.add_argument("--c4-methods", default="all",
                     help="comma list of 96^3 comparisons: learned_mp,learned_fp64,jacobi_fp64,ic0_fp64,saamg_mp")
     ap.add_argument("--no-c5", action="store_true", help="skip the batched-solve leg")
+    ap.add_argument("--c3", action="store_true", help="add the convection-diffusion ILU(0) leg (about 30 s)")
+    ap.add_argument("--c3-n", type=int, default=1024, help="grid side of the convection-diffusion leg")
+    ap.add_argument("--c3-max-iter", type=int, default=3000)
     ap.add_argument("--c5-count", type=int, default=512, help="systems in the batch")
     ap.add_argument("--spmv-probe", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration) and the float64 SpMV; "
@@ -290,6 +327,7 @@ def main():
 
     # ---- batched leg: 512 independent systems in one persistent launch ----
     c5 = None if args.no_c5 else c5_leg(F, gen, args, log)
+    c3 = c3_leg(F, gen, args, log) if args.c3 else None
 
     # ---- CPU baseline: the oracle's float64 IC(0)-PCG on this host -------
     cpu = None
@@ -345,6 +383,8 @@ def main():
         out["config"]["c4"] = c4
     if c5 is not None:
         out["config"]["c5"] = c5
+    if c3 is not None:
+        out["config"]["c3"] = c3
     if results.get("ic0_fp64") and results.get("saamg_mp"):
         out["config"]["speedup_mp_vs_fp64_ic0"] = round(results["ic0_fp64"]["solve_ms"] / results["saamg_mp"]["solve_ms"], 2)
     print(json.dumps(out))

@@ -216,6 +216,14 @@ int fem_csr_destroy(fem_csr A) {
     return guard([&] { delete A; });
 }
 
+int fem_csr_set_row_lanes(fem_csr A, int lanes) {
+    return guard([&] {
+        if (lanes != 0 && lanes != 4 && lanes != 8 && lanes != 16 && lanes != 32 && lanes != 64)
+            fail(FEM_ERR_INVALID_ARGUMENT, "row lanes must be 0, 4, 8, 16, 32 or 64");
+        A->A.lanes = lanes;
+    });
+}
+
 int fem_spmv(fem_csr A, const double* x, double* y) {
     return guard([&] {
         auto& M = A->A;

@@ -131,6 +131,7 @@ void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, 
     if (!vals) fail(FEM_ERR_RUNTIME, "float32 copy of the matrix was not built");
     if (A.lanes > 0) {
         switch (A.lanes) {
+            case 4: spmv_vec_launch<T, 4>(A, vals, x, y, epi, a, s, g); break;
             case 8: spmv_vec_launch<T, 8>(A, vals, x, y, epi, a, s, g); break;
             case 16: spmv_vec_launch<T, 16>(A, vals, x, y, epi, a, s, g); break;
             case 32: spmv_vec_launch<T, 32>(A, vals, x, y, epi, a, s, g); break;

@@ -67,12 +67,11 @@ struct CsrMat : LinOp {
     mutable std::vector<double> h_v;
     mutable bool host_valid = false;
     hipStream_t stream = nullptr;  // owning stream
-    // > 0: SpMV with `lanes` lanes per row and a butterfly sum (operators
-    // with long rows whose summation order is free: the AMG transfer and
-    // coarse operators); 0: CSR-stream with sequential row sums, bit-exact
-    // with the reference's spmv
+    // 0: CSR-stream with sequential row sums, bit-exact with the reference's
+    // spmv; > 0 (operators whose summation order is free: the AMG transfer
+    // and coarse operators): that many lanes per row and a butterfly sum
     int lanes = 0;
-    void use_vector_rows();  // lanes from the mean row length (0 below 40)
+    void use_vector_rows();  // the free-order variant for this matrix's rows
 
     uint32_t rows() const override { return n; }
     uint32_t cols() const override { return m; }

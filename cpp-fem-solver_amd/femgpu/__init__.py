@@ -112,6 +112,7 @@ _SIGS = {
     "fem_csr_transpose": (C.c_int, [P_vp, C.POINTER(P_vp)]),
     "fem_csr_multiply": (C.c_int, [P_vp, P_vp, C.POINTER(P_vp)]),
     "fem_csr_destroy": (C.c_int, [P_vp]),
+    "fem_csr_set_row_lanes": (C.c_int, [P_vp, C.c_int]),
     "fem_spmv": (C.c_int, [P_vp, P_f64, P_f64]),
     "fem_spmv_device": (C.c_int, [P_vp, P_vp, P_vp, C.c_int]),
     "fem_dot": (C.c_int, [C.c_uint64, P_f64, P_f64, P_f64]),
@@ -263,6 +264,11 @@ class CsrMatrix:
         y = np.empty(self.rows, np.float64)
         _check(lib().fem_spmv(self._h, _p(x, C.c_double), _p(y, C.c_double)))
         return y
+
+    def set_row_lanes(self, lanes: int):
+        """SpMV variant: 0 = CSR-stream (bit-exact with SMatrix::spmv),
+        4/8/16/32/64 = that many lanes per row."""
+        _check(lib().fem_csr_set_row_lanes(self._h, int(lanes)))
 
     def bench_spmv(self, fp32=False, reps=100, cold=False) -> float:
         """Average SpMV time (ms): back-to-back launches, or each launch

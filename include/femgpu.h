@@ -162,6 +162,11 @@ int fem_csr_transpose(fem_csr A, fem_csr* out);
 /* SMatrix::operator*, array.cpp:2336-2447 (mathematically exact product) */
 int fem_csr_multiply(fem_csr A, fem_csr B, fem_csr* out);
 int fem_csr_destroy(fem_csr A);
+/* SpMV variant of A (no counterpart in the reference, which has one loop):
+ * lanes = 0 selects the CSR-stream kernel, whose row sums run sequentially in
+ * stored order and match SMatrix::spmv bit for bit; 4, 8, 16, 32 or 64 give
+ * each row that many lanes and a butterfly sum (last bits differ). Default 0. */
+int fem_csr_set_row_lanes(fem_csr A, int lanes);
 
 /* ---- primitives --------------------------------------------------------- */
 /* SMatrix::spmv, array.cpp:3418-3442 */
