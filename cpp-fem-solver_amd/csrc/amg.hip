@@ -107,9 +107,10 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
                                                       T* res, uint32_t zero_from, Gate g) {
     if (g.closed()) return;
     __shared__ T prod[kStageNnz];
-    const uint32_t r0 = blk[blockIdx.x], r1 = blk[blockIdx.x + 1];
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t r0 = blk[bid], r1 = blk[bid + 1];
     const uint32_t t = threadIdx.x;
-    if (MODE == 1 && blockIdx.x >= zero_from) {
+    if (MODE == 1 && bid >= zero_from) {
         for (uint32_t row = r0 + t; row < r1; row += kBlock) res[rows[row]] = (T)0;
         return;
     }

@@ -59,7 +59,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
                                                  const uint32_t* __restrict__ blk_row, SpmvArgs<T> a, Gate g) {
     if (g.closed()) return;
     __shared__ T prod[kStageNnz];
-    const uint32_t r0 = blk_row[blockIdx.x], r1 = blk_row[blockIdx.x + 1];
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    const uint32_t r0 = blk_row[bid], r1 = blk_row[bid + 1];
     const uint32_t k0 = rp[r0], k1 = rp[r1];
     const uint32_t cnt = k1 - k0;
     const uint32_t t = threadIdx.x;

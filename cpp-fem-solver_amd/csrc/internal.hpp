@@ -40,6 +40,16 @@ struct LinOp {
     virtual uint64_t bytes_per_apply(bool fp32) const = 0;
 };
 
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup b runs on
+// XCD b % 8, each XCD with its own L2). This permutation hands each XCD a
+// contiguous range of the logical block ids, so blocks that run on one XCD
+// at about the same time work on neighbouring rows and share the L2 lines of
+// the vector they gather from.
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
+    const uint32_t q = nb >> 3, r = nb & 7u, x = b & 7u;
+    return x * q + (x < r ? x : r) + (b >> 3);
+}
+
 // SpMV launch schedule: consecutive row ranges whose nonzeros fit one block's
 // LDS staging area, plus single long rows that get a whole block each.
 // CSR-stream SpMV blocking: a block stages at most kStageNnz products of at
