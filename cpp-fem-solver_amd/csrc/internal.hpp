@@ -12,11 +12,18 @@ namespace fem {
 // when `need` is set, only if *need != 0. This is how data-dependent control
 // flow (convergence, BiCGSTAB's true-residual check, GMRES early exit) is
 // expressed inside captured hipGraphs without host round trips.
+//
+// The flags are written only by earlier kernels (a kernel that writes one
+// reads it first), so a kernel reads them once, through the scalar cache,
+// which is invalidated at every kernel launch.
+__device__ __forceinline__ int flag_load(const int* p) {
+    return *(const __attribute__((address_space(4))) int*)p;
+}
 struct Gate {
     const int* done = nullptr;
     const int* need = nullptr;
     __device__ __forceinline__ bool closed() const {
-        return (done && *(volatile const int*)done) || (need && !*(volatile const int*)need);
+        return (done && flag_load(done)) || (need && !flag_load(need));
     }
 };
 

@@ -31,7 +31,7 @@ __device__ __forceinline__ void kfail(KState* st, int code) {
 }
 
 #define GRID_LOOP(i, n) for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < (n); i += (size_t)gridDim.x * blockDim.x)
-#define DONE_RET(st) if (*(volatile int*)&(st)->done) return
+#define DONE_RET(st) if (flag_load(&(st)->done)) return
 
 // ---------------------------------------------------------------------------
 // PCG kernels. Each reduction kernel writes per-block partials; a Fin*
