@@ -71,8 +71,10 @@ struct BArgs {
 };
 
 // deterministic workgroup sum of K doubles per thread: wave butterflies, one
-// LDS row per wave, every thread adds the rows in wave order. `buf` alternates
-// between calls so a buffer is not rewritten before all threads read it.
+// LDS row per wave, then threads 0..K-1 add the rows in wave order and every
+// thread reads the K totals. `red` holds (BLOCK/64 + 1) * K doubles and
+// alternates between calls, so a buffer is not rewritten before all threads
+// have read it.
 template <int BLOCK, int K>
 __device__ __forceinline__ void wg_sum(double (&v)[K], double* red) {
     constexpr int NW = BLOCK / 64;
@@ -84,12 +86,15 @@ __device__ __forceinline__ void wg_sum(double (&v)[K], double* red) {
 #pragma unroll
         for (int k = 0; k < K; ++k) red[w * K + k] = v[k];
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
+    if (threadIdx.x < K) {
         double s = 0.0;
-        for (int ww = 0; ww < NW; ++ww) s += red[ww * K + k];
-        v[k] = s;
+#pragma unroll 1
+        for (int ww = 0; ww < NW; ++ww) s += red[ww * K + threadIdx.x];
+        red[NW * K + threadIdx.x] = s;
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = red[NW * K + k];
 }
 
 template <int BS>
@@ -122,10 +127,18 @@ __device__ __forceinline__ void store_blk(float* v, uint32_t I, const float (&o)
     for (int a = 0; a < BS; ++a) p[a] = make_float4(o[a][0], o[a][1], o[a][2], o[a][3]);
 }
 
+// per-system scalars that stay fixed through an inner solve, kept in LDS
+// (written by thread 0 between barriers) rather than in every thread's
+// registers
+struct SysShared {
+    double normb[NR], relres[NR], rr0[NR];
+    float want[NR];
+};
+
 template <int BLOCK, int BS>
-__device__ void solve_system(const BArgs& A, const BSys& S, double* red, BOut* o) {
+__device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShared& sh, BOut* o) {
     constexpr int BV = BS * BS;
-    constexpr int U = BS == 1 ? 4 : 2;  // rows per thread per step
+    constexpr int U = BS == 1 ? 4 : 1;  // rows per thread per step
     const uint32_t nb = S.nb;
     const uint32_t* rp = A.rp + S.rp;
     const uint32_t* ci = A.ci + S.ci;
@@ -140,7 +153,7 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, BOut* o
     float* __restrict__ q = A.q + S.vec;
     const uint32_t tid = threadIdx.x;
     int rb = 0;  // alternating reduction buffer
-    auto RED = [&]() { double* p_ = red + (rb & 1) * (BLOCK / 64) * 8; ++rb; return p_; };
+    auto RED = [&]() { double* p_ = red + (rb & 1) * (BLOCK / 64 + 1) * 8; ++rb; return p_; };
 
     // ||b||, x = 0
     double nb2[NR];
@@ -152,10 +165,12 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, BOut* o
             x[(size_t)i * NR + c] = 0.0;
         }
     wg_sum<BLOCK, NR>(nb2, RED());
-    double normb[NR];
-    for (int c = 0; c < NR; ++c) normb[c] = sqrt(nb2[c]);
+    if (tid == 0)
+        for (int c = 0; c < NR; ++c) {
+            sh.normb[c] = sqrt(nb2[c]);
+            sh.relres[c] = 0.0;
+        }
     int iters = 0, refs = 0;
-    double relres[NR] = {0, 0, 0, 0};
     bool done = false;
     for (int ref = 0; !done; ++ref) {
         // r = b - A x (float64), stored as float32; ||r||^2
@@ -183,17 +198,23 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, BOut* o
                 }
             store_blk<BS>(r, I, rf);
         }
-        wg_sum<BLOCK, NR>(rn2, RED());
+        wg_sum<BLOCK, NR>(rn2, RED());  // ends with a barrier after sh was last written
         bool all = true;
-        float want[NR];
         bool act[NR];
         for (int c = 0; c < NR; ++c) {
             const double rn = sqrt(rn2[c]);
-            relres[c] = normb[c] > 0 ? rn / normb[c] : 0.0;
-            act[c] = c < S.nrhs && normb[c] > 0 && rn >= A.tol * normb[c];
+            const double nbc = sh.normb[c];
+            act[c] = c < S.nrhs && nbc > 0 && rn >= A.tol * nbc;
             if (act[c]) all = false;
-            want[c] = act[c] ? (float)fmax(A.inner_tol, 0.5 * A.tol * normb[c] / rn) : 0.f;
         }
+        __syncthreads();  // every thread has read sh before thread 0 rewrites it
+        if (tid == 0)
+            for (int c = 0; c < NR; ++c) {
+                const double rn = sqrt(rn2[c]);
+                const double nbc = sh.normb[c];
+                sh.relres[c] = nbc > 0 ? rn / nbc : 0.0;
+                sh.want[c] = act[c] ? (float)fmax(A.inner_tol, 0.5 * A.tol * nbc / rn) : 0.f;
+            }
         if (all || ref >= A.max_refine) {
             done = true;
             break;
@@ -216,12 +237,14 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, BOut* o
             store_blk<BS>(d, I, zero);
         }
         wg_sum<BLOCK, 2 * NR>(t2, RED());
-        double rz[NR], rr0[NR];
+        double rz[NR];
         for (int c = 0; c < NR; ++c) {
             rz[c] = t2[c];
-            rr0[c] = sqrt(t2[NR + c]);
             if (!(rz[c] > 0.0)) act[c] = false;
         }
+        if (tid == 0)
+            for (int c = 0; c < NR; ++c) sh.rr0[c] = sqrt(t2[NR + c]);
+        __syncthreads();
         const int max_inner = S.cap > 0 ? min(A.max_inner, S.cap) : A.max_inner;
         for (int it = 0; it < max_inner; ++it) {
             bool any = false;
@@ -315,7 +338,7 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, BOut* o
             wg_sum<BLOCK, 2 * NR>(t, RED());
             float beta[NR];
             for (int c = 0; c < NR; ++c) {
-                if (act[c] && sqrt(t[c]) < want[c] * rr0[c]) act[c] = false;
+                if (act[c] && sqrt(t[c]) < sh.want[c] * sh.rr0[c]) act[c] = false;
                 beta[c] = (act[c] && rz[c] != 0.0) ? (float)(t[NR + c] / rz[c]) : 0.f;
                 if (act[c]) rz[c] = t[NR + c];
                 if (act[c] && !(rz[c] > 0.0)) act[c] = false;
@@ -351,16 +374,17 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, BOut* o
         o->iterations = iters;
         o->refinements = refs;
         int conv = 0;
-        for (int c = 0; c < S.nrhs; ++c) conv += (relres[c] < A.tol) ? 1 : 0;
+        for (int c = 0; c < S.nrhs; ++c) conv += (sh.relres[c] < A.tol) ? 1 : 0;
         o->converged = conv;
-        for (int c = 0; c < NR; ++c) o->relres[c] = relres[c];
+        for (int c = 0; c < NR; ++c) o->relres[c] = sh.relres[c];
     }
     __syncthreads();
 }
 
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_batch_pcg(BArgs a) {
-    __shared__ double red[2 * (BLOCK / 64) * 8];
+    __shared__ double red[2 * (BLOCK / 64 + 1) * 8];
+    __shared__ SysShared sh;
     __shared__ int slot;
     while (true) {
         __syncthreads();
@@ -370,8 +394,8 @@ __global__ __launch_bounds__(BLOCK) void k_batch_pcg(BArgs a) {
         if (k >= a.nsys) break;
         const int s = a.order[k];
         const BSys S = a.sys[s];
-        if (S.bs == 3) solve_system<BLOCK, 3>(a, S, red, a.out + s);
-        else solve_system<BLOCK, 1>(a, S, red, a.out + s);
+        if (S.bs == 3) solve_system<BLOCK, 3>(a, S, red, sh, a.out + s);
+        else solve_system<BLOCK, 1>(a, S, red, sh, a.out + s);
     }
 }
 
@@ -655,10 +679,9 @@ int fem_batch_tune(fem_batch h, const fem_batch_opts* opts, int divisor, fem_bat
         DevBuf<BOut> dout(h->nsys);
         const int blocks[3] = {256, 512, 1024};
         const int pers[3] = {1, 2, 0};
-        double best_ms = 1e300;
-        fem_batch_schedule bs{256, 0, 0};
+        std::vector<std::pair<double, fem_batch_schedule>> cand;
         int t = 0;
-        const int keep_block = h->block, keep_per = h->wg_per_cu;
+        const int keep_block = h->block, keep_per = h->wg_per_cu, keep_order = h->order_kind;
         for (int bi = 0; bi < 3; ++bi)
             for (int pi = 0; pi < 3; ++pi)
                 for (int ok = 0; ok < 2; ++ok) {
@@ -668,13 +691,36 @@ int fem_batch_tune(fem_batch h, const fem_batch_opts* opts, int divisor, fem_bat
                                              opts->tol, opts->inner_tol, 1, opts->max_inner);
                     if (trial_ms && t < cap) trial_ms[t] = ms;
                     ++t;
-                    if (ms < best_ms) {
-                        best_ms = ms;
-                        bs = fem_batch_schedule{blocks[bi], pers[pi], ok};
-                    }
+                    cand.push_back({ms, fem_batch_schedule{blocks[bi], pers[pi], ok}});
                 }
+        // the capped runs rank the schedules by throughput; the makespan of a
+        // full run also depends on the longest systems' chains, so the best
+        // capped schedule of each workgroup size is timed again on the whole
+        // batch
+        std::vector<fem_batch_schedule> finalists;
+        for (int bi = 0; bi < 3; ++bi) {
+            int kbest = -1;
+            for (int k = 0; k < (int)cand.size(); ++k)
+                if (cand[k].second.block == blocks[bi] && (kbest < 0 || cand[k].first < cand[kbest].first)) kbest = k;
+            finalists.push_back(cand[kbest].second);
+        }
+        double best_ms = 1e300;
+        fem_batch_schedule bs = finalists[0];
+        for (const fem_batch_schedule& c : finalists) {
+            h->block = c.block;
+            h->wg_per_cu = c.wg_per_cu;
+            const double ms = launch(h, h->sys.get(), c.order == 0 ? dord0.get() : dord1.get(), h->nsys, dout.get(),
+                                     opts->tol, opts->inner_tol, opts->max_refine, opts->max_inner);
+            if (trial_ms && t < cap) trial_ms[t] = ms;
+            ++t;
+            if (ms < best_ms) {
+                best_ms = ms;
+                bs = c;
+            }
+        }
         h->block = keep_block;
         h->wg_per_cu = keep_per;
+        set_order(h, keep_order);
         if (best) *best = bs;
     });
 }

@@ -634,12 +634,13 @@ class Batch:
 
     def tune(self, divisor=8, tol=1e-10, inner_tol=1e-4, max_refine=40, max_inner=20000):
         """Profile-guided schedule search (fem_batch_tune); returns the best
-        schedule and the 18 trial times (ms)."""
+        schedule and the trial times (ms): 18 capped runs, then full runs of
+        the fastest of each workgroup size."""
         best = BatchSchedule()
         t = np.zeros(32)
         o = BatchOpts(tol, inner_tol, max_refine, max_inner)
         _check(lib().fem_batch_tune(self._h, C.byref(o), divisor, C.byref(best), _p(t, C.c_double), 32))
-        return {"block": best.block, "wg_per_cu": best.wg_per_cu, "order": best.order}, t[:18]
+        return {"block": best.block, "wg_per_cu": best.wg_per_cu, "order": best.order}, t[:21]
 
     def set_schedule(self, block=256, wg_per_cu=0, order=0):
         s = BatchSchedule(block, wg_per_cu, order)

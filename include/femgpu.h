@@ -320,10 +320,12 @@ int fem_batch_solve(fem_batch b, const fem_batch_opts* opts, fem_batch_stats* pe
 /* solutions, same layout as the right-hand sides */
 int fem_batch_get_solution(fem_batch b, double* X);
 /* schedule autotuner (profile-guided): solves once with the current schedule
- * to record every system's iterations, then times each of the 18 schedules
- * on the whole batch with every system capped at 1/divisor of its recorded
- * iterations (trial_ms receives up to cap timings) and returns the fastest;
- * apply it with fem_batch_set_schedule */
+ * to record every system's iterations, times each of the 18 schedules on the
+ * whole batch with every system capped at 1/divisor of its recorded
+ * iterations, then times the fastest of each workgroup size on full solves
+ * and returns the fastest of those (trial_ms receives up to cap timings: the
+ * 18 capped runs, then the 3 full runs); apply it with fem_batch_set_schedule. The solutions
+ * are left as the last trial computed them. */
 int fem_batch_tune(fem_batch b, const fem_batch_opts* opts, int divisor, fem_batch_schedule* best, double* trial_ms,
                    int cap);
 int fem_batch_set_schedule(fem_batch b, const fem_batch_schedule* schedule);
