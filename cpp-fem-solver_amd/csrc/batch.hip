@@ -41,7 +41,7 @@ struct BSys {
     uint64_t dinv;    // offset into dinv (scalars)
     uint64_t vec;     // offset into vectors (scalars * NR)
     int cap;          // > 0: at most this many inner iterations (autotuner trials)
-    int pad;
+    int nr;           // vector layout: 4 interleaved right-hand-side slots, or 1
 };
 
 struct BOut {
@@ -97,12 +97,12 @@ __device__ __forceinline__ void wg_sum(double (&v)[K], double* red) {
     for (int k = 0; k < K; ++k) v[k] = red[NW * K + k];
 }
 
-template <int BS>
-__device__ __forceinline__ void dinv_apply(const float* Di, const float (&r)[BS][NR], float (&z)[BS][NR]) {
+template <int BS, int NRT>
+__device__ __forceinline__ void dinv_apply(const float* Di, const float (&r)[BS][NRT], float (&z)[BS][NRT]) {
 #pragma unroll
     for (int a = 0; a < BS; ++a)
 #pragma unroll
-        for (int c = 0; c < NR; ++c) {
+        for (int c = 0; c < NRT; ++c) {
             float s = 0.f;
 #pragma unroll
             for (int b = 0; b < BS; ++b) s += Di[a * BS + b] * r[b][c];
@@ -110,21 +110,32 @@ __device__ __forceinline__ void dinv_apply(const float* Di, const float (&r)[BS]
         }
 }
 
-template <int BS>
-__device__ __forceinline__ void load_blk(const float* v, uint32_t I, float (&o)[BS][NR]) {
-    const float4* p = reinterpret_cast<const float4*>(v + (size_t)I * BS * NR);
+// block row I of a vector: BS rows of NRT interleaved right-hand sides
+template <int BS, int NRT>
+__device__ __forceinline__ void load_blk(const float* v, uint32_t I, float (&o)[BS][NRT]) {
+    if constexpr (NRT == 4) {
+        const float4* p = reinterpret_cast<const float4*>(v + (size_t)I * BS * 4);
 #pragma unroll
-    for (int a = 0; a < BS; ++a) {
-        const float4 t = p[a];
-        o[a][0] = t.x; o[a][1] = t.y; o[a][2] = t.z; o[a][3] = t.w;
+        for (int a = 0; a < BS; ++a) {
+            const float4 t = p[a];
+            o[a][0] = t.x; o[a][1] = t.y; o[a][2] = t.z; o[a][3] = t.w;
+        }
+    } else {
+#pragma unroll
+        for (int a = 0; a < BS; ++a) o[a][0] = v[(size_t)I * BS + a];
     }
 }
 
-template <int BS>
-__device__ __forceinline__ void store_blk(float* v, uint32_t I, const float (&o)[BS][NR]) {
-    float4* p = reinterpret_cast<float4*>(v + (size_t)I * BS * NR);
+template <int BS, int NRT>
+__device__ __forceinline__ void store_blk(float* v, uint32_t I, const float (&o)[BS][NRT]) {
+    if constexpr (NRT == 4) {
+        float4* p = reinterpret_cast<float4*>(v + (size_t)I * BS * 4);
 #pragma unroll
-    for (int a = 0; a < BS; ++a) p[a] = make_float4(o[a][0], o[a][1], o[a][2], o[a][3]);
+        for (int a = 0; a < BS; ++a) p[a] = make_float4(o[a][0], o[a][1], o[a][2], o[a][3]);
+    } else {
+#pragma unroll
+        for (int a = 0; a < BS; ++a) v[(size_t)I * BS + a] = o[a][0];
+    }
 }
 
 // per-system scalars that stay fixed through an inner solve, kept in LDS
@@ -135,7 +146,7 @@ struct SysShared {
     float want[NR];
 };
 
-template <int BLOCK, int BS>
+template <int BLOCK, int BS, int NRT>
 __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShared& sh, BOut* o) {
     constexpr int BV = BS * BS;
     constexpr int U = BS == 1 ? 4 : 1;  // rows per thread per step
@@ -156,17 +167,17 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShar
     auto RED = [&]() { double* p_ = red + (rb & 1) * (BLOCK / 64 + 1) * 8; ++rb; return p_; };
 
     // ||b||, x = 0
-    double nb2[NR];
-    for (int c = 0; c < NR; ++c) nb2[c] = 0.0;
+    double nb2[NRT];
+    for (int c = 0; c < NRT; ++c) nb2[c] = 0.0;
     for (uint32_t i = tid; i < nb * BS; i += BLOCK)
-        for (int c = 0; c < NR; ++c) {
-            const double t = b[(size_t)i * NR + c];
+        for (int c = 0; c < NRT; ++c) {
+            const double t = b[(size_t)i * NRT + c];
             nb2[c] += t * t;
-            x[(size_t)i * NR + c] = 0.0;
+            x[(size_t)i * NRT + c] = 0.0;
         }
-    wg_sum<BLOCK, NR>(nb2, RED());
+    wg_sum<BLOCK, NRT>(nb2, RED());
     if (tid == 0)
-        for (int c = 0; c < NR; ++c) {
+        for (int c = 0; c < NRT; ++c) {
             sh.normb[c] = sqrt(nb2[c]);
             sh.relres[c] = 0.0;
         }
@@ -175,33 +186,33 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShar
     for (int ref = 0; !done; ++ref) {
         // r = b - A x (float64), stored as float32; ||r||^2
         __syncthreads();
-        double rn2[NR] = {0, 0, 0, 0};
+        double rn2[NRT] = {};
         for (uint32_t I = tid; I < nb; I += BLOCK) {
-            double acc[BS][NR];
+            double acc[BS][NRT];
             for (int a = 0; a < BS; ++a)
-                for (int c = 0; c < NR; ++c) acc[a][c] = b[((size_t)I * BS + a) * NR + c];
+                for (int c = 0; c < NRT; ++c) acc[a][c] = b[((size_t)I * BS + a) * NRT + c];
             for (uint32_t k = rp[I]; k < rp[I + 1]; ++k) {
                 const uint32_t J = ci[k];
                 const double* blk = v64 + (size_t)k * BV;
                 for (int bb = 0; bb < BS; ++bb) {
-                    double xv[NR];
-                    for (int c = 0; c < NR; ++c) xv[c] = x[((size_t)J * BS + bb) * NR + c];
+                    double xv[NRT];
+                    for (int c = 0; c < NRT; ++c) xv[c] = x[((size_t)J * BS + bb) * NRT + c];
                     for (int a = 0; a < BS; ++a)
-                        for (int c = 0; c < NR; ++c) acc[a][c] -= blk[a * BS + bb] * xv[c];
+                        for (int c = 0; c < NRT; ++c) acc[a][c] -= blk[a * BS + bb] * xv[c];
                 }
             }
-            float rf[BS][NR];
+            float rf[BS][NRT];
             for (int a = 0; a < BS; ++a)
-                for (int c = 0; c < NR; ++c) {
+                for (int c = 0; c < NRT; ++c) {
                     rn2[c] += acc[a][c] * acc[a][c];
                     rf[a][c] = (float)acc[a][c];
                 }
-            store_blk<BS>(r, I, rf);
+            store_blk<BS, NRT>(r, I, rf);
         }
-        wg_sum<BLOCK, NR>(rn2, RED());  // ends with a barrier after sh was last written
+        wg_sum<BLOCK, NRT>(rn2, RED());  // ends with a barrier after sh was last written
         bool all = true;
-        bool act[NR];
-        for (int c = 0; c < NR; ++c) {
+        bool act[NRT];
+        for (int c = 0; c < NRT; ++c) {
             const double rn = sqrt(rn2[c]);
             const double nbc = sh.normb[c];
             act[c] = c < S.nrhs && nbc > 0 && rn >= A.tol * nbc;
@@ -209,7 +220,7 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShar
         }
         __syncthreads();  // every thread has read sh before thread 0 rewrites it
         if (tid == 0)
-            for (int c = 0; c < NR; ++c) {
+            for (int c = 0; c < NRT; ++c) {
                 const double rn = sqrt(rn2[c]);
                 const double nbc = sh.normb[c];
                 sh.relres[c] = nbc > 0 ? rn / nbc : 0.0;
@@ -221,41 +232,41 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShar
         }
         ++refs;
         // inner PCG: d = 0, z = D^-1 r, p = z, rz = r.z, rr0 = ||r||
-        double t2[2 * NR];
-        for (int k = 0; k < 2 * NR; ++k) t2[k] = 0.0;
+        double t2[2 * NRT];
+        for (int k = 0; k < 2 * NRT; ++k) t2[k] = 0.0;
         for (uint32_t I = tid; I < nb; I += BLOCK) {
-            float rr[BS][NR], zz[BS][NR], zero[BS][NR];
-            load_blk<BS>(r, I, rr);
-            dinv_apply<BS>(dv + (size_t)I * BV, rr, zz);
+            float rr[BS][NRT], zz[BS][NRT], zero[BS][NRT];
+            load_blk<BS, NRT>(r, I, rr);
+            dinv_apply<BS, NRT>(dv + (size_t)I * BV, rr, zz);
             for (int a = 0; a < BS; ++a)
-                for (int c = 0; c < NR; ++c) {
+                for (int c = 0; c < NRT; ++c) {
                     zero[a][c] = 0.f;
                     t2[c] += (double)rr[a][c] * zz[a][c];
-                    t2[NR + c] += (double)rr[a][c] * rr[a][c];
+                    t2[NRT + c] += (double)rr[a][c] * rr[a][c];
                 }
-            store_blk<BS>(p, I, zz);
-            store_blk<BS>(d, I, zero);
+            store_blk<BS, NRT>(p, I, zz);
+            store_blk<BS, NRT>(d, I, zero);
         }
-        wg_sum<BLOCK, 2 * NR>(t2, RED());
-        double rz[NR];
-        for (int c = 0; c < NR; ++c) {
+        wg_sum<BLOCK, 2 * NRT>(t2, RED());
+        double rz[NRT];
+        for (int c = 0; c < NRT; ++c) {
             rz[c] = t2[c];
             if (!(rz[c] > 0.0)) act[c] = false;
         }
         if (tid == 0)
-            for (int c = 0; c < NR; ++c) sh.rr0[c] = sqrt(t2[NR + c]);
+            for (int c = 0; c < NRT; ++c) sh.rr0[c] = sqrt(t2[NRT + c]);
         __syncthreads();
         const int max_inner = S.cap > 0 ? min(A.max_inner, S.cap) : A.max_inner;
         for (int it = 0; it < max_inner; ++it) {
             bool any = false;
-            for (int c = 0; c < NR; ++c) any |= act[c];
+            for (int c = 0; c < NRT; ++c) any |= act[c];
             if (!any) break;
             ++iters;
             __syncthreads();  // p complete
             // pass A: q = A p, p.q. U rows per thread at a time: their row
             // pointers, entries and gathers are all issued before any use,
             // so a thread keeps U dependent load chains in flight.
-            double pq[NR] = {0, 0, 0, 0};
+            double pq[NRT] = {};
             for (uint32_t base = tid; base < nb; base += U * BLOCK) {
                 uint32_t k0[U], k1[U], len = 0;
 #pragma unroll
@@ -265,11 +276,11 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShar
                     k1[u] = I < nb ? rp[I + 1] : 0u;
                     len = max(len, k1[u] - k0[u]);
                 }
-                float acc[U][BS][NR];
+                float acc[U][BS][NRT];
 #pragma unroll
                 for (int u = 0; u < U; ++u)
                     for (int a = 0; a < BS; ++a)
-                        for (int c = 0; c < NR; ++c) acc[u][a][c] = 0.f;
+                        for (int c = 0; c < NRT; ++c) acc[u][a][c] = 0.f;
                 for (uint32_t t = 0; t < len; ++t) {
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
@@ -277,98 +288,98 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShar
                         const uint32_t k = k0[u] + t;
                         const uint32_t J = ci[k];
                         const float* blk = v32 + (size_t)k * BV;
-                        float pj[BS][NR];
-                        load_blk<BS>(p, J, pj);
+                        float pj[BS][NRT];
+                        load_blk<BS, NRT>(p, J, pj);
                         for (int a = 0; a < BS; ++a)
                             for (int bb = 0; bb < BS; ++bb)
-                                for (int c = 0; c < NR; ++c) acc[u][a][c] += blk[a * BS + bb] * pj[bb][c];
+                                for (int c = 0; c < NRT; ++c) acc[u][a][c] += blk[a * BS + bb] * pj[bb][c];
                     }
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t I = base + u * BLOCK;
                     if (I >= nb) continue;
-                    float pi[BS][NR];
-                    load_blk<BS>(p, I, pi);
+                    float pi[BS][NRT];
+                    load_blk<BS, NRT>(p, I, pi);
                     for (int a = 0; a < BS; ++a)
-                        for (int c = 0; c < NR; ++c) pq[c] += (double)pi[a][c] * acc[u][a][c];
-                    store_blk<BS>(q, I, acc[u]);
+                        for (int c = 0; c < NRT; ++c) pq[c] += (double)pi[a][c] * acc[u][a][c];
+                    store_blk<BS, NRT>(q, I, acc[u]);
                 }
             }
-            wg_sum<BLOCK, NR>(pq, RED());
-            float alpha[NR];
-            for (int c = 0; c < NR; ++c) {
+            wg_sum<BLOCK, NRT>(pq, RED());
+            float alpha[NRT];
+            for (int c = 0; c < NRT; ++c) {
                 if (act[c] && !(pq[c] > 0.0)) act[c] = false;  // breakdown: keep the iterate
                 alpha[c] = act[c] ? (float)(rz[c] / pq[c]) : 0.f;
             }
             // pass B: d += a p, r -= a q, r.r, r.z (U rows per step, loads first)
-            double t[2 * NR];
-            for (int k = 0; k < 2 * NR; ++k) t[k] = 0.0;
+            double t[2 * NRT];
+            for (int k = 0; k < 2 * NRT; ++k) t[k] = 0.0;
             for (uint32_t base = tid; base < nb; base += U * BLOCK) {
-                float pi[U][BS][NR], qi[U][BS][NR], ri[U][BS][NR], di[U][BS][NR];
+                float pi[U][BS][NRT], qi[U][BS][NRT], ri[U][BS][NRT], di[U][BS][NRT];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t I = base + u * BLOCK;
                     if (I >= nb) continue;
-                    load_blk<BS>(p, I, pi[u]);
-                    load_blk<BS>(q, I, qi[u]);
-                    load_blk<BS>(r, I, ri[u]);
-                    load_blk<BS>(d, I, di[u]);
+                    load_blk<BS, NRT>(p, I, pi[u]);
+                    load_blk<BS, NRT>(q, I, qi[u]);
+                    load_blk<BS, NRT>(r, I, ri[u]);
+                    load_blk<BS, NRT>(d, I, di[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t I = base + u * BLOCK;
                     if (I >= nb) continue;
-                    float zz[BS][NR];
+                    float zz[BS][NRT];
                     for (int a = 0; a < BS; ++a)
-                        for (int c = 0; c < NR; ++c) {
+                        for (int c = 0; c < NRT; ++c) {
                             di[u][a][c] += alpha[c] * pi[u][a][c];
                             ri[u][a][c] -= alpha[c] * qi[u][a][c];
                         }
-                    dinv_apply<BS>(dv + (size_t)I * BV, ri[u], zz);
+                    dinv_apply<BS, NRT>(dv + (size_t)I * BV, ri[u], zz);
                     for (int a = 0; a < BS; ++a)
-                        for (int c = 0; c < NR; ++c) {
+                        for (int c = 0; c < NRT; ++c) {
                             t[c] += (double)ri[u][a][c] * ri[u][a][c];
-                            t[NR + c] += (double)ri[u][a][c] * zz[a][c];
+                            t[NRT + c] += (double)ri[u][a][c] * zz[a][c];
                         }
-                    store_blk<BS>(d, I, di[u]);
-                    store_blk<BS>(r, I, ri[u]);
+                    store_blk<BS, NRT>(d, I, di[u]);
+                    store_blk<BS, NRT>(r, I, ri[u]);
                 }
             }
-            wg_sum<BLOCK, 2 * NR>(t, RED());
-            float beta[NR];
-            for (int c = 0; c < NR; ++c) {
+            wg_sum<BLOCK, 2 * NRT>(t, RED());
+            float beta[NRT];
+            for (int c = 0; c < NRT; ++c) {
                 if (act[c] && sqrt(t[c]) < sh.want[c] * sh.rr0[c]) act[c] = false;
-                beta[c] = (act[c] && rz[c] != 0.0) ? (float)(t[NR + c] / rz[c]) : 0.f;
-                if (act[c]) rz[c] = t[NR + c];
+                beta[c] = (act[c] && rz[c] != 0.0) ? (float)(t[NRT + c] / rz[c]) : 0.f;
+                if (act[c]) rz[c] = t[NRT + c];
                 if (act[c] && !(rz[c] > 0.0)) act[c] = false;
             }
             // pass C: p = z + beta p (rows of finished right-hand sides keep p)
             for (uint32_t base = tid; base < nb; base += U * BLOCK) {
-                float ri[U][BS][NR], pi[U][BS][NR];
+                float ri[U][BS][NRT], pi[U][BS][NRT];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t I = base + u * BLOCK;
                     if (I >= nb) continue;
-                    load_blk<BS>(r, I, ri[u]);
-                    load_blk<BS>(p, I, pi[u]);
+                    load_blk<BS, NRT>(r, I, ri[u]);
+                    load_blk<BS, NRT>(p, I, pi[u]);
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
                     const uint32_t I = base + u * BLOCK;
                     if (I >= nb) continue;
-                    float zz[BS][NR];
-                    dinv_apply<BS>(dv + (size_t)I * BV, ri[u], zz);
+                    float zz[BS][NRT];
+                    dinv_apply<BS, NRT>(dv + (size_t)I * BV, ri[u], zz);
                     for (int a = 0; a < BS; ++a)
-                        for (int c = 0; c < NR; ++c) pi[u][a][c] = act[c] ? zz[a][c] + beta[c] * pi[u][a][c] : pi[u][a][c];
-                    store_blk<BS>(p, I, pi[u]);
+                        for (int c = 0; c < NRT; ++c) pi[u][a][c] = act[c] ? zz[a][c] + beta[c] * pi[u][a][c] : pi[u][a][c];
+                    store_blk<BS, NRT>(p, I, pi[u]);
                 }
             }
         }
         __syncthreads();
         // x += d (float64)
         for (uint32_t i = tid; i < nb * BS; i += BLOCK)
-            for (int c = 0; c < NR; ++c) x[(size_t)i * NR + c] += (double)d[(size_t)i * NR + c];
+            for (int c = 0; c < NRT; ++c) x[(size_t)i * NRT + c] += (double)d[(size_t)i * NRT + c];
     }
     if (tid == 0) {
         o->iterations = iters;
@@ -376,7 +387,7 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShar
         int conv = 0;
         for (int c = 0; c < S.nrhs; ++c) conv += (sh.relres[c] < A.tol) ? 1 : 0;
         o->converged = conv;
-        for (int c = 0; c < NR; ++c) o->relres[c] = sh.relres[c];
+        for (int c = 0; c < NR; ++c) o->relres[c] = c < NRT ? sh.relres[c] : 0.0;
     }
     __syncthreads();
 }
@@ -394,8 +405,13 @@ __global__ __launch_bounds__(BLOCK) void k_batch_pcg(BArgs a) {
         if (k >= a.nsys) break;
         const int s = a.order[k];
         const BSys S = a.sys[s];
-        if (S.bs == 3) solve_system<BLOCK, 3>(a, S, red, sh, a.out + s);
-        else solve_system<BLOCK, 1>(a, S, red, sh, a.out + s);
+        if (S.bs == 3) {
+            if (S.nr == 1) solve_system<BLOCK, 3, 1>(a, S, red, sh, a.out + s);
+            else solve_system<BLOCK, 3, NR>(a, S, red, sh, a.out + s);
+        } else {
+            if (S.nr == 1) solve_system<BLOCK, 1, 1>(a, S, red, sh, a.out + s);
+            else solve_system<BLOCK, 1, NR>(a, S, red, sh, a.out + s);
+        }
     }
 }
 
@@ -433,9 +449,18 @@ using namespace fem;
 struct fem_batch_s {
     hipStream_t stream = nullptr;
     int nsys = 0;
-    std::vector<BSys> h_sys;
-    std::vector<double> work;           // work estimate per system (for ordering)
-    DevBuf<BSys> sys;
+    std::vector<BSys> h_sys;            // the systems (all right-hand sides interleaved)
+    std::vector<double> work;           // work estimate per system (static, then from the profile)
+    std::vector<uint64_t> nnzb;         // stored blocks per system
+    std::vector<char> split;            // system solved as one work item per right-hand side
+    // work items: a whole system, or one right-hand side of a split system
+    // (vectors of that right-hand side stored contiguously in the system's
+    // vector range)
+    std::vector<BSys> items;
+    std::vector<int> item_sys, item_rhs;
+    std::vector<double> item_work;
+    std::vector<double> hb;             // right-hand sides as given (for re-layout)
+    DevBuf<BSys> sys;                   // items on the device
     DevBuf<int> order, queue;
     DevBuf<uint32_t> rp, ci;
     DevBuf<double> v64, b, x;
@@ -443,8 +468,8 @@ struct fem_batch_s {
     DevBuf<BOut> out;
     uint64_t nvec = 0;                  // scalars * NR
     // schedule: block size, workgroups per CU, queue order (0 largest work
-    // first, 1 input order)
-    int block = 256, wg_per_cu = 0, order_kind = 0;
+    // first, 1 input order), number of largest systems split per right-hand side
+    int block = 256, wg_per_cu = 0, order_kind = 0, nsplit = 0;
     ~fem_batch_s() {
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -453,12 +478,78 @@ struct fem_batch_s {
 namespace {
 
 void set_order(fem_batch_s* h, int kind) {
-    std::vector<int> ord(h->nsys);
+    std::vector<int> ord(h->items.size());
     std::iota(ord.begin(), ord.end(), 0);
     if (kind == 0)
-        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return h->work[a] > h->work[b]; });
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return h->item_work[a] > h->item_work[b]; });
     h->order.upload(ord, h->stream);
     h->order_kind = kind;
+}
+
+// the right-hand sides in the current layout
+void upload_b(fem_batch_s* h) {
+    std::vector<double> v(h->nvec, 0.0);
+    if (!h->hb.empty()) {
+        const double* src = h->hb.data();
+        for (int k = 0; k < h->nsys; ++k) {
+            const BSys& S = h->h_sys[k];
+            const uint64_t n = (uint64_t)S.nb * S.bs;
+            for (int c = 0; c < S.nrhs; ++c)
+                for (uint64_t i = 0; i < n; ++i) {
+                    const uint64_t at = h->split[k] ? S.vec + (uint64_t)c * n + i : S.vec + i * NR + c;
+                    v[at] = src[(uint64_t)c * n + i];
+                }
+            src += (uint64_t)S.nrhs * n;
+        }
+    }
+    h->b.upload(v, h->stream);
+    FEM_HIP(hipStreamSynchronize(h->stream));
+}
+
+// split the `count` systems of largest work (ties: lower index), rebuild the
+// work items, re-lay the right-hand sides and the queue order
+void apply_split(fem_batch_s* h, int count) {
+    count = std::max(0, std::min(count, h->nsys));
+    std::vector<int> rank(h->nsys);
+    std::iota(rank.begin(), rank.end(), 0);
+    std::stable_sort(rank.begin(), rank.end(), [&](int a, int b) { return h->work[a] > h->work[b]; });
+    h->split.assign(h->nsys, 0);
+    for (int t = 0; t < count; ++t)
+        if (h->h_sys[rank[t]].nrhs > 1) h->split[rank[t]] = 1;
+    h->nsplit = count;
+    h->items.clear();
+    h->item_sys.clear();
+    h->item_rhs.clear();
+    h->item_work.clear();
+    for (int k = 0; k < h->nsys; ++k) {
+        const BSys& S = h->h_sys[k];
+        if (!h->split[k]) {
+            BSys it = S;
+            it.nr = NR;
+            h->items.push_back(it);
+            h->item_sys.push_back(k);
+            h->item_rhs.push_back(-1);
+            h->item_work.push_back(h->work[k]);
+            continue;
+        }
+        const uint64_t n = (uint64_t)S.nb * S.bs;
+        // one right-hand side reads the matrix as often as all four do
+        const double bv = (double)S.bs * S.bs;
+        const double frac = (bv + 8.0) / (bv + 8.0 * NR);
+        for (int c = 0; c < S.nrhs; ++c) {
+            BSys it = S;
+            it.nrhs = 1;
+            it.nr = 1;
+            it.vec = S.vec + (uint64_t)c * n;
+            h->items.push_back(it);
+            h->item_sys.push_back(k);
+            h->item_rhs.push_back(c);
+            h->item_work.push_back(h->work[k] * frac);
+        }
+    }
+    h->sys.upload(h->items, h->stream);
+    set_order(h, h->order_kind);
+    upload_b(h);
 }
 
 template <int BLOCK>
@@ -475,12 +566,13 @@ int cu_count() {
     return cu;
 }
 
-// one persistent launch over systems sys[0..nsys) (queue reset first); returns ms
-double launch(fem_batch_s* h, const BSys* sys, const int* order, int nsys, BOut* out, double tol, double inner_tol,
+// one persistent launch over work items sys[0..nitems) (queue reset first);
+// returns ms
+double launch(fem_batch_s* h, const BSys* sys, const int* order, int nitems, BOut* out, double tol, double inner_tol,
               int max_refine, int max_inner) {
     hipStream_t s = h->stream;
     vec_fill<uint32_t>((uint32_t*)h->queue.get(), 0u, 1, s);
-    BArgs a{sys, order, nsys, h->queue.get(), h->rp.get(), h->ci.get(), h->v64.get(), h->v32.get(), h->dinv.get(),
+    BArgs a{sys, order, nitems, h->queue.get(), h->rp.get(), h->ci.get(), h->v64.get(), h->v32.get(), h->dinv.get(),
             h->b.get(), h->x.get(), h->r.get(), h->d.get(), h->p.get(), h->q.get(), out, tol, inner_tol, max_refine,
             max_inner};
     int per = h->wg_per_cu;
@@ -491,7 +583,7 @@ double launch(fem_batch_s* h, const BSys* sys, const int* order, int nsys, BOut*
         default: cap = max_wg_per_cu<256>(); break;
     }
     if (per <= 0 || per > cap) per = cap;
-    const unsigned grid = (unsigned)std::max(1, std::min(nsys, per * cu_count()));
+    const unsigned grid = (unsigned)std::max(1, std::min(nitems, per * cu_count()));
     hipEvent_t e0, e1;
     FEM_HIP(hipEventCreate(&e0));
     FEM_HIP(hipEventCreate(&e1));
@@ -509,6 +601,34 @@ double launch(fem_batch_s* h, const BSys* sys, const int* order, int nsys, BOut*
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return ms;
+}
+
+double run_items(fem_batch_s* h, const fem_batch_opts* o, int max_refine) {
+    return launch(h, h->sys.get(), h->order.get(), (int)h->items.size(), h->out.get(), o->tol, o->inner_tol,
+                  max_refine, o->max_inner);
+}
+
+// per-system results from the items' (a split system: the largest counts of
+// its right-hand sides, each right-hand side's own residual)
+std::vector<fem_batch_stats> merge_out(fem_batch_s* h, const std::vector<BOut>& o) {
+    std::vector<fem_batch_stats> st(h->nsys);
+    for (int k = 0; k < h->nsys; ++k) st[k] = fem_batch_stats{0, 0, 0, 0.0};
+    for (size_t t = 0; t < h->items.size(); ++t) {
+        fem_batch_stats& S = st[h->item_sys[t]];
+        S.iterations = std::max(S.iterations, o[t].iterations);
+        S.refinements = std::max(S.refinements, o[t].refinements);
+        S.converged += o[t].converged;
+        const int nr = h->items[t].nrhs;
+        for (int c = 0; c < nr; ++c) S.max_relres = std::max(S.max_relres, o[t].relres[c]);
+    }
+    return st;
+}
+
+std::vector<BOut> download_out(fem_batch_s* h) {
+    std::vector<BOut> o(h->items.size());
+    h->out.download(o.data(), o.size(), h->stream);
+    FEM_HIP(hipStreamSynchronize(h->stream));
+    return o;
 }
 
 }  // namespace
@@ -535,7 +655,7 @@ int fem_batch_create(int nsys, const fem_batch_system* systems, int nrhs, fem_ba
             if (in.row_ptr[0] != 0) fail(FEM_ERR_INVALID_ARGUMENT, "row_ptr must start at 0");
             const uint32_t nb = in.nb;
             const uint64_t nnzb = in.row_ptr[nb];
-            BSys S{bs, nrhs, nb, rp.size(), ci.size(), v64.size(), dv.size(), vec, 0, 0};
+            BSys S{bs, nrhs, nb, rp.size(), ci.size(), v64.size(), dv.size(), vec, 0, NR};
             const int BV = bs * bs;
             for (uint32_t I = 0; I <= nb; ++I) rp.push_back(in.row_ptr[I]);
             std::vector<double> inv((size_t)nb * BV, 0.0);
@@ -557,13 +677,13 @@ int fem_batch_create(int nsys, const fem_batch_system* systems, int nrhs, fem_ba
             dv.insert(dv.end(), inv.begin(), inv.end());
             vec += (uint64_t)nb * bs * NR;
             h->h_sys.push_back(S);
+            h->nnzb.push_back(nnzb);
             // work estimate: matrix entries x sqrt(rows) (CG iterations grow
             // with the square root of the condition number)
             h->work.push_back((double)nnzb * BV * std::sqrt((double)nb * bs));
         }
         hipStream_t s = h->stream;
         h->nvec = vec;
-        h->sys.upload(h->h_sys, s);
         h->rp.upload(rp, s);
         h->ci.upload(ci, s);
         h->v64.upload(v64, s);
@@ -576,14 +696,14 @@ int fem_batch_create(int nsys, const fem_batch_system* systems, int nrhs, fem_ba
         h->d.alloc(vec);
         h->p.alloc(vec);
         h->q.alloc(vec);
-        vec_fill<double>(h->b.get(), 0.0, vec, s);
+        vec_fill<double>(h->x.get(), 0.0, vec, s);
         vec_fill<float>(h->r.get(), 0.f, vec, s);
         vec_fill<float>(h->d.get(), 0.f, vec, s);
         vec_fill<float>(h->p.get(), 0.f, vec, s);
         vec_fill<float>(h->q.get(), 0.f, vec, s);
-        h->out.alloc(nsys);
+        h->out.alloc((size_t)nsys * NR);
         h->queue.alloc(1);
-        set_order(h.get(), 0);
+        apply_split(h.get(), 0);
         FEM_HIP(hipStreamSynchronize(s));
         *out = h.release();
     });
@@ -596,16 +716,10 @@ int fem_batch_destroy(fem_batch h) {
 int fem_batch_set_rhs(fem_batch h, const double* B) {
     return api_guard([&] {
         if (!h || !B) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
-        std::vector<double> v(h->nvec, 0.0);
-        const double* src = B;
-        for (const BSys& S : h->h_sys) {
-            const uint64_t n = (uint64_t)S.nb * S.bs;
-            for (int c = 0; c < S.nrhs; ++c)
-                for (uint64_t i = 0; i < n; ++i) v[S.vec + i * NR + c] = src[(uint64_t)c * n + i];
-            src += (uint64_t)S.nrhs * n;
-        }
-        h->b.upload(v, h->stream);
-        FEM_HIP(hipStreamSynchronize(h->stream));
+        uint64_t total = 0;
+        for (const BSys& S : h->h_sys) total += (uint64_t)S.nrhs * S.nb * S.bs;
+        h->hb.assign(B, B + total);
+        upload_b(h);
     });
 }
 
@@ -616,10 +730,12 @@ int fem_batch_get_solution(fem_batch h, double* X) {
         h->x.download(v.data(), h->nvec, h->stream);
         FEM_HIP(hipStreamSynchronize(h->stream));
         double* dst = X;
-        for (const BSys& S : h->h_sys) {
+        for (int k = 0; k < h->nsys; ++k) {
+            const BSys& S = h->h_sys[k];
             const uint64_t n = (uint64_t)S.nb * S.bs;
             for (int c = 0; c < S.nrhs; ++c)
-                for (uint64_t i = 0; i < n; ++i) dst[(uint64_t)c * n + i] = v[S.vec + i * NR + c];
+                for (uint64_t i = 0; i < n; ++i)
+                    dst[(uint64_t)c * n + i] = h->split[k] ? v[S.vec + (uint64_t)c * n + i] : v[S.vec + i * NR + c];
             dst += (uint64_t)S.nrhs * n;
         }
     });
@@ -630,42 +746,43 @@ int fem_batch_solve(fem_batch h, const fem_batch_opts* opts, fem_batch_stats* pe
         if (!h || !opts) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
         if (!(opts->tol > 0) || opts->max_refine < 1 || opts->max_inner < 1)
             fail(FEM_ERR_INVALID_ARGUMENT, "tol > 0, max_refine >= 1 and max_inner >= 1 are required");
-        const double t = launch(h, h->sys.get(), h->order.get(), h->nsys, h->out.get(), opts->tol, opts->inner_tol,
-                                opts->max_refine, opts->max_inner);
+        const double t = run_items(h, opts, opts->max_refine);
         if (ms) *ms = t;
         if (per_system) {
-            std::vector<BOut> o(h->nsys);
-            h->out.download(o.data(), h->nsys, h->stream);
-            FEM_HIP(hipStreamSynchronize(h->stream));
-            for (int k = 0; k < h->nsys; ++k) {
-                per_system[k].iterations = o[k].iterations;
-                per_system[k].refinements = o[k].refinements;
-                per_system[k].converged = o[k].converged;
-                double w = 0.0;
-                for (int c = 0; c < h->h_sys[k].nrhs; ++c) w = std::max(w, o[k].relres[c]);
-                per_system[k].max_relres = w;
-            }
+            const std::vector<fem_batch_stats> st = merge_out(h, download_out(h));
+            std::copy(st.begin(), st.end(), per_system);
         }
     });
 }
 
 // Autotuner (profile-guided): one solve with the current schedule records
-// every system's inner iterations; then each schedule (block size
-// 256/512/1024 x workgroups per CU 1/2/occupancy limit x queue order) runs
-// the whole batch with one refinement step of iterations_k / divisor inner
+// every system's inner iterations, which become the work estimates (inner
+// iterations x matrix entries). Each schedule (block size 256/512/1024 x
+// workgroups per CU 1/2/occupancy limit x queue order) then runs the whole
+// batch, unsplit, with one refinement step of iterations_k / divisor inner
 // iterations per system, which keeps the real mix of long and short systems
-// (and so the makespan) at 1/divisor of the cost. Returns the fastest.
+// at 1/divisor of the cost. The best capped schedule of each block size is
+// timed on full solves, unsplit and with the systems whose work exceeds an
+// even share per resident workgroup split into one item per right-hand side
+// (their chains then run side by side); the fastest is returned.
 int fem_batch_tune(fem_batch h, const fem_batch_opts* opts, int divisor, fem_batch_schedule* best, double* trial_ms,
                    int cap) {
     return api_guard([&] {
         if (!h || !opts) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
         if (divisor < 1) divisor = 8;
-        launch(h, h->sys.get(), h->order.get(), h->nsys, h->out.get(), opts->tol, opts->inner_tol, opts->max_refine,
-               opts->max_inner);
-        std::vector<BOut> o(h->nsys);
-        h->out.download(o.data(), h->nsys, h->stream);
-        FEM_HIP(hipStreamSynchronize(h->stream));
-        std::vector<BSys> tsys = h->h_sys;
+        const int keep_block = h->block, keep_per = h->wg_per_cu, keep_order = h->order_kind;
+        apply_split(h, 0);
+        run_items(h, opts, opts->max_refine);
+        const std::vector<BOut> o = download_out(h);
+        // profiled work: inner iterations x bytes of one iteration (matrix in
+        // float32 with its indices, about nine vector passes)
+        for (int k = 0; k < h->nsys; ++k) {
+            const BSys& S = h->h_sys[k];
+            const double bv = (double)S.bs * S.bs;
+            h->work[k] = (double)std::max(1, o[k].iterations) *
+                         ((double)h->nnzb[k] * (4.0 * bv + 4.0) + (double)S.nb * S.bs * NR * 4.0 * 9.0);
+        }
+        std::vector<BSys> tsys = h->items;  // unsplit: item k is system k
         for (int k = 0; k < h->nsys; ++k) tsys[k].cap = std::max(1, o[k].iterations / divisor);
         DevBuf<BSys> dsys;
         dsys.upload(tsys, h->stream);
@@ -681,7 +798,6 @@ int fem_batch_tune(fem_batch h, const fem_batch_opts* opts, int divisor, fem_bat
         const int pers[3] = {1, 2, 0};
         std::vector<std::pair<double, fem_batch_schedule>> cand;
         int t = 0;
-        const int keep_block = h->block, keep_per = h->wg_per_cu, keep_order = h->order_kind;
         for (int bi = 0; bi < 3; ++bi)
             for (int pi = 0; pi < 3; ++pi)
                 for (int ok = 0; ok < 2; ++ok) {
@@ -691,12 +807,18 @@ int fem_batch_tune(fem_batch h, const fem_batch_opts* opts, int divisor, fem_bat
                                              opts->tol, opts->inner_tol, 1, opts->max_inner);
                     if (trial_ms && t < cap) trial_ms[t] = ms;
                     ++t;
-                    cand.push_back({ms, fem_batch_schedule{blocks[bi], pers[pi], ok}});
+                    cand.push_back({ms, fem_batch_schedule{blocks[bi], pers[pi], ok, 0}});
                 }
         // the capped runs rank the schedules by throughput; the makespan of a
         // full run also depends on the longest systems' chains, so the best
         // capped schedule of each workgroup size is timed again on the whole
-        // batch
+        // batch, with and without splitting the largest systems
+        double total = 0.0;
+        for (double w : h->work) total += w;
+        const double share = total / std::max(1, 2 * cu_count());
+        int nbig = 0;
+        for (int k = 0; k < h->nsys; ++k)
+            if (h->work[k] > share && h->h_sys[k].nrhs > 1) ++nbig;
         std::vector<fem_batch_schedule> finalists;
         for (int bi = 0; bi < 3; ++bi) {
             int kbest = -1;
@@ -706,21 +828,29 @@ int fem_batch_tune(fem_batch h, const fem_batch_opts* opts, int divisor, fem_bat
         }
         double best_ms = 1e300;
         fem_batch_schedule bs = finalists[0];
-        for (const fem_batch_schedule& c : finalists) {
-            h->block = c.block;
-            h->wg_per_cu = c.wg_per_cu;
-            const double ms = launch(h, h->sys.get(), c.order == 0 ? dord0.get() : dord1.get(), h->nsys, dout.get(),
-                                     opts->tol, opts->inner_tol, opts->max_refine, opts->max_inner);
-            if (trial_ms && t < cap) trial_ms[t] = ms;
-            ++t;
-            if (ms < best_ms) {
-                best_ms = ms;
-                bs = c;
+        std::vector<int> splits{0};
+        if (nbig > 0) splits.push_back(nbig);
+        for (int sp : splits) {
+            for (const fem_batch_schedule& c0 : finalists) {
+                fem_batch_schedule c = c0;
+                c.split = sp;
+                h->block = c.block;
+                h->wg_per_cu = c.wg_per_cu;
+                h->order_kind = c.order;
+                apply_split(h, sp);
+                const double ms = run_items(h, opts, opts->max_refine);
+                if (trial_ms && t < cap) trial_ms[t] = ms;
+                ++t;
+                if (ms < best_ms) {
+                    best_ms = ms;
+                    bs = c;
+                }
             }
         }
         h->block = keep_block;
         h->wg_per_cu = keep_per;
-        set_order(h, keep_order);
+        h->order_kind = keep_order;
+        apply_split(h, 0);
         if (best) *best = bs;
     });
 }
@@ -731,10 +861,11 @@ int fem_batch_set_schedule(fem_batch h, const fem_batch_schedule* sch) {
         if (sch->block != 256 && sch->block != 512 && sch->block != 1024)
             fail(FEM_ERR_INVALID_ARGUMENT, "block must be 256, 512 or 1024");
         if (sch->order != 0 && sch->order != 1) fail(FEM_ERR_INVALID_ARGUMENT, "order must be 0 or 1");
+        if (sch->split < 0) fail(FEM_ERR_INVALID_ARGUMENT, "split must be >= 0");
         h->block = sch->block;
         h->wg_per_cu = std::max(0, sch->wg_per_cu);
-        set_order(h, sch->order);
-        FEM_HIP(hipStreamSynchronize(h->stream));
+        h->order_kind = sch->order;
+        apply_split(h, sch->split);
     });
 }
 

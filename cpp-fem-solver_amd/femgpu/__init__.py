@@ -84,7 +84,7 @@ class BatchStats(C.Structure):
 
 
 class BatchSchedule(C.Structure):
-    _fields_ = [("block", C.c_int), ("wg_per_cu", C.c_int), ("order", C.c_int)]
+    _fields_ = [("block", C.c_int), ("wg_per_cu", C.c_int), ("order", C.c_int), ("split", C.c_int)]
 
 
 class AdamOpts(C.Structure):
@@ -635,15 +635,18 @@ class Batch:
     def tune(self, divisor=8, tol=1e-10, inner_tol=1e-4, max_refine=40, max_inner=20000):
         """Profile-guided schedule search (fem_batch_tune); returns the best
         schedule and the trial times (ms): 18 capped runs, then full runs of
-        the fastest of each workgroup size."""
+        the fastest of each workgroup size, unsplit and split."""
         best = BatchSchedule()
-        t = np.zeros(32)
+        t = np.full(32, np.nan)
         o = BatchOpts(tol, inner_tol, max_refine, max_inner)
         _check(lib().fem_batch_tune(self._h, C.byref(o), divisor, C.byref(best), _p(t, C.c_double), 32))
-        return {"block": best.block, "wg_per_cu": best.wg_per_cu, "order": best.order}, t[:21]
+        return ({"block": best.block, "wg_per_cu": best.wg_per_cu, "order": best.order, "split": best.split},
+                t[~np.isnan(t)])
 
-    def set_schedule(self, block=256, wg_per_cu=0, order=0):
-        s = BatchSchedule(block, wg_per_cu, order)
+    def set_schedule(self, block=256, wg_per_cu=0, order=0, split=0):
+        """split: the `split` systems of largest estimated work are queued as
+        one work item per right-hand side."""
+        s = BatchSchedule(block, wg_per_cu, order, split)
         _check(lib().fem_batch_set_schedule(self._h, C.byref(s)))
 
     def close(self):

@@ -300,7 +300,8 @@ typedef struct fem_batch_opts {
     int max_inner;            /* inner iterations per refinement step */
 } fem_batch_opts;
 typedef struct fem_batch_stats {
-    int iterations;           /* inner iterations (all right-hand sides together) */
+    int iterations;           /* inner iterations (all right-hand sides together; for a
+                                 split system the largest of its right-hand sides') */
     int refinements;
     int converged;            /* right-hand sides that reached tol */
     double max_relres;        /* largest final ||b - A x|| / ||b|| (float64) */
@@ -309,6 +310,8 @@ typedef struct fem_batch_schedule {
     int block;                /* workgroup size: 256, 512 or 1024 */
     int wg_per_cu;            /* resident workgroups per CU (0: occupancy limit) */
     int order;                /* queue: 0 largest estimated work first, 1 input order */
+    int split;                /* the `split` systems of largest estimated work are
+                                 queued as one work item per right-hand side */
 } fem_batch_schedule;
 /* copies the systems to the device and inverts the diagonal blocks */
 int fem_batch_create(int nsys, const fem_batch_system* systems, int nrhs, fem_batch* out);
@@ -319,13 +322,16 @@ int fem_batch_set_rhs(fem_batch b, const double* B);
 int fem_batch_solve(fem_batch b, const fem_batch_opts* opts, fem_batch_stats* per_system, double* ms);
 /* solutions, same layout as the right-hand sides */
 int fem_batch_get_solution(fem_batch b, double* X);
-/* schedule autotuner (profile-guided): solves once with the current schedule
- * to record every system's iterations, times each of the 18 schedules on the
- * whole batch with every system capped at 1/divisor of its recorded
- * iterations, then times the fastest of each workgroup size on full solves
- * and returns the fastest of those (trial_ms receives up to cap timings: the
- * 18 capped runs, then the 3 full runs); apply it with fem_batch_set_schedule. The solutions
- * are left as the last trial computed them. */
+/* schedule autotuner (profile-guided): solves once unsplit to record every
+ * system's iterations (they become the work estimates), times each of the 18
+ * unsplit schedules on the whole batch with every system capped at
+ * 1/divisor of its recorded iterations, then times the fastest of each
+ * workgroup size on full solves, unsplit and with the systems whose work
+ * exceeds an even share per resident workgroup split, and returns the
+ * fastest (trial_ms receives up to cap timings: the 18 capped runs, then the
+ * full runs). The batch is left unsplit with its previous schedule; apply
+ * the result with fem_batch_set_schedule. Solutions are left as the last
+ * trial computed them. */
 int fem_batch_tune(fem_batch b, const fem_batch_opts* opts, int divisor, fem_batch_schedule* best, double* trial_ms,
                    int cap);
 int fem_batch_set_schedule(fem_batch b, const fem_batch_schedule* schedule);

@@ -68,7 +68,7 @@ def test_batch_deterministic_and_schedules():
     assert all(np.array_equal(a, b) for a, b in zip(x1, x2))
     assert s1 == s2
     best, trials = B.tune(divisor=4)
-    assert best["block"] in (256, 512, 1024) and best["order"] in (0, 1)
+    assert best["block"] in (256, 512, 1024) and best["order"] in (0, 1) and best["split"] >= 0
     assert np.all(trials > 0)
     for blk in (512, 1024):
         B.set_schedule(block=blk, order=1)
@@ -77,6 +77,32 @@ def test_batch_deterministic_and_schedules():
         for s, t, x in zip(sysl, st, X):
             assert t["converged"] == 4
             assert rel(x[0], s["X"][0]) < tol_x(_scalar(s["A"]))
+
+
+@pytest.mark.parametrize("block", [256, 512])
+def test_split_right_hand_sides_give_identical_results(block):
+    """Queuing systems as one work item per right-hand side changes the
+    vector layout and the schedule, not the arithmetic: solutions, residuals
+    and convergence are bit for bit those of the unsplit solve."""
+    sysl = small_batch(seed=3)
+    B = F.Batch([s["A"] for s in sysl], nrhs=4)
+    B.set_rhs([s["B"] for s in sysl])
+    B.set_schedule(block=block, order=0, split=0)
+    s0, _ = B.solve()
+    x0 = B.solution()
+    for split in (2, len(sysl)):
+        B.set_schedule(block=block, order=0, split=split)
+        s1, _ = B.solve()
+        x1 = B.solution()
+        assert all(np.array_equal(a, b) for a, b in zip(x0, x1))
+        for a, b in zip(s0, s1):
+            assert a["converged"] == b["converged"] and a["refinements"] == b["refinements"]
+            assert a["max_relres"] == b["max_relres"]
+            # unsplit, the right-hand sides of a system step together: its
+            # count is the sum over refinements of the per-step maximum
+            assert b["iterations"] <= a["iterations"]
+    with pytest.raises(F.InvalidArgument):
+        B.set_schedule(block=block, split=-1)
 
 
 def test_batch_errors():
