@@ -54,7 +54,9 @@ struct Sweep {
     // chains: dense per-slot copies (entry e of slot s at cbase + (s - s0) kc + e)
     // of the columns, dependency slots and values, the SELL index each value
     // comes from, and right-hand-side staging in slot order
-    DevBuf<uint32_t> ccol, cds, cmap;
+    DevBuf<uint32_t> cds, cmap;
+    DevBuf<double> sink;     // chain kernels: target of the stores of threads without a row
+    DevBuf<float> sink32;
     DevBuf<double> cval, rslot;
     DevBuf<float> cval32, rslot32;
     uint64_t cent = 0;
@@ -108,13 +110,28 @@ struct Sweep {
                 ++l;
                 continue;
             }
+            // a chain holds only levels whose rows depend on slots at most
+            // kRing behind the level's end (the LDS ring window; the chain
+            // kernel preloads the kRing slots before the chain into it), so
+            // every dependency of a chain is read from LDS
+            auto in_window = [&](uint32_t lev) {
+                const uint32_t end = h_lvl_ptr[lev + 1];
+                for (uint32_t q = h_lvl_ptr[lev]; q < end; ++q) {
+                    const uint32_t i = order[q];
+                    for (uint32_t t = rp[i]; t < rp[i + 1]; ++t)
+                        if (end - slot_of[cols[t]] > kRing) return false;
+                }
+                return true;
+            };
             uint32_t e = l;
-            while (e < nlev && e - l < kChainLevels && h_lvl_ptr[e + 1] - h_lvl_ptr[e] <= kChainRows) ++e;
+            while (e < nlev && e - l < kChainLevels && h_lvl_ptr[e + 1] - h_lvl_ptr[e] <= kChainRows && in_window(e))
+                ++e;
+            if (e == l) e = l + 1;  // a level alone (one launch)
             segs.push_back({h_lvl_ptr[l], h_lvl_ptr[e], l, e});
             l = e;
         }
         // dense layout of the chains
-        std::vector<uint32_t> dc, dd, dm;
+        std::vector<uint32_t> dd, dm;
         for (Segment& sg : segs) {
             if (sg.lvl1 - sg.lvl0 <= 1) continue;
             uint32_t kc = 1;
@@ -124,20 +141,20 @@ struct Sweep {
             if (kc > 8) continue;
             kc = kc <= 2 ? 2u : kc <= 4 ? 4u : 8u;
             sg.kc = kc;
-            sg.cbase = dc.size();
+            sg.cbase = dd.size();
             for (uint32_t q = sg.s0; q < sg.s1; ++q) {
                 const uint32_t base = g[q >> 6] + (q & 63);
                 for (uint32_t e = 0; e < kc; ++e) {
                     const bool real = e < ln[q];
-                    dc.push_back(real ? hc[base + e * 64] : 0u);
                     dd.push_back(real ? hd[base + e * 64] : 0u);
                     dm.push_back(real ? (uint32_t)(base + e * 64) : UINT32_MAX);
                 }
             }
         }
-        cent = dc.size();
+        cent = dd.size();
         if (cent) {
-            ccol.upload(dc, s);
+            sink.alloc(1);
+            sink32.alloc(1);
             cds.upload(dd, s);
             cmap.upload(dm, s);
             cval.alloc(cent);
@@ -227,43 +244,49 @@ __global__ __launch_bounds__(256) void k_sweep_level(uint32_t s0, uint32_t s1, c
 }
 
 // A run of consecutive small levels (<= kChainRows rows each) in one
-// workgroup, one row per thread per level. Every result also goes into an
-// LDS ring indexed by sweep slot; a dependency written earlier in this launch
-// and still inside the ring window is read from LDS instead of through L2.
-// For level [a, b) the window is slots >= max(s0, b - kRing): this level's
-// writes (slots a..b-1) land on ring positions of slots below b - kRing, so
-// they never clobber an entry still being read. The chain's entries sit in a
-// dense per-slot layout and its right-hand side in slot order, so every load
-// of a row is addressed by the slot alone; each thread loads the row it will
-// handle several levels ahead before the barrier. Per-row arithmetic order is
-// unchanged (rhs first, then the entries in ascending order).
+// workgroup, one row per thread per level. Results go to y and to an LDS ring
+// indexed by sweep slot (position slot % kRing); the kernel first loads the
+// kRing slots before the chain into it, and the host builds chains whose
+// rows depend only on slots at most kRing behind their level's end, so every
+// dependency is read from the ring. For level [a, b) the ring then holds
+// slots b - kRing .. b - 1 at distinct positions: this level's writes (slots
+// a..b-1) never clobber an entry still being read. The chain's entries sit
+// in a dense per-slot layout and its right-hand side in slot order, so every
+// load of a row is addressed by the slot alone; each thread loads the row it
+// will handle D levels ahead, unconditionally (a slot past the level reloads
+// slot s0), so no wait on those loads precedes their use. Per-row arithmetic
+// order is unchanged (rhs first, then the entries in ascending order).
 template <typename T, bool DIV, int KC>
 __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint32_t l1, const uint32_t* __restrict__ lvl_ptr,
                                                                const uint32_t* __restrict__ slot_row,
                                                                const uint16_t* __restrict__ len,
-                                                               const uint32_t* __restrict__ ccol,
                                                                const uint32_t* __restrict__ cds,
                                                                const T* __restrict__ cval, const T* __restrict__ diag,
-                                                               const T* __restrict__ rslot, T* y, Gate g) {
+                                                               const T* __restrict__ rslot, T* y, T* sink,
+                                                               Gate g) {
     if (g.closed()) return;
-    __shared__ T ring[kRing];
+    __shared__ T ring[kRing + 1];  // + a sink for threads without a row
     __shared__ uint32_t lp[kChainLevels + 1];
     for (uint32_t t = threadIdx.x; t <= l1 - l0; t += blockDim.x) lp[t] = lvl_ptr[l0 + t];
     __syncthreads();
     const uint32_t s0 = lp[0];
+    // the kRing slots before the chain (written by earlier launches)
+    for (uint32_t t = threadIdx.x; t < kRing; t += blockDim.x) {
+        const uint32_t back = kRing - t;  // slot s0 - back
+        ring[(s0 - back) % kRing] = back <= s0 ? y[slot_row[s0 - back]] : (T)0;
+    }
+    __syncthreads();
     struct Row {
-        bool on;
-        uint32_t i, L;
+        uint32_t on, i, L;
         T r, d;
-        uint32_t c[KC], ds[KC];
+        uint32_t ds[KC];
         T v[KC];
     };
     auto fetch = [&](uint32_t l, Row& R) {
-        R.on = false;
-        if (l >= l1) return;
-        const uint32_t s = lp[l - l0] + threadIdx.x;
-        if (s >= lp[l - l0 + 1]) return;
-        R.on = true;
+        const uint32_t lc = l < l1 ? l : l1 - 1;
+        const uint32_t sr = lp[lc - l0] + threadIdx.x;
+        R.on = (l < l1 && sr < lp[lc - l0 + 1]) ? 1u : 0u;
+        const uint32_t s = R.on ? sr : s0;
         R.i = slot_row[s];
         R.L = len[s];
         R.r = rslot[s];
@@ -271,7 +294,6 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint
         const uint64_t o = (uint64_t)(s - s0) * KC;
 #pragma unroll
         for (int e = 0; e < KC; ++e) {
-            R.c[e] = ccol[o + e];
             R.ds[e] = cds[o + e];
             R.v[e] = cval[o + e];
         }
@@ -288,38 +310,27 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint
         for (int k = 0; k < D; ++k) {
             const uint32_t l = base + k;
             if (l >= l1) break;  // uniform over the workgroup
-            const uint32_t a = lp[l - l0], b = lp[l - l0 + 1];
-            const uint32_t lo = max(s0, b > kRing ? b - kRing : 0u);
             Row& cur = buf[k];
-            if (cur.on) {
-                T acc = cur.r;
+            T acc = cur.r;
 #pragma unroll
-                for (int e = 0; e < KC; ++e)
-                    if ((uint32_t)e < cur.L) {
-                        const T yv = cur.ds[e] >= lo ? ring[cur.ds[e] % kRing] : y[cur.c[e]];
-                        acc -= cur.v[e] * yv;
-                    }
-                if constexpr (DIV) acc = acc / cur.d;
-                const uint32_t s = a + threadIdx.x;
-                ring[s % kRing] = acc;
-                y[cur.i] = acc;
+            for (int e = 0; e < KC; ++e) {
+                const T t = acc - cur.v[e] * ring[cur.ds[e] % kRing];
+                acc = (uint32_t)e < cur.L ? t : acc;
             }
+            if constexpr (DIV) acc = acc / cur.d;
+            // stores without a branch: a thread without a row writes to the
+            // sinks (ring[kRing], and sink[0] in global memory)
+            const uint32_t s = lp[l - l0] + threadIdx.x;
+            ring[cur.on ? s % kRing : kRing] = acc;
+            *(cur.on ? y + cur.i : sink) = acc;
             fetch(l + D, cur);
             // Between levels only the LDS ring must be ordered: wait for LDS
             // operations and barrier, leaving the loads fetched ahead in
-            // flight (__syncthreads would also wait for them). A dependency
-            // read from global memory lies more than kRing slots back (levels
-            // hold <= 1024 slots), so it was written before some level that
-            // crossed a multiple of kRing / 2 slots; those levels end with a
-            // full barrier that waits for every store.
-            if (a / (kRing / 2) != b / (kRing / 2)) {
-                __syncthreads();
-            } else {
-                __asm__ volatile("" ::: "memory");
-                __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS done
-                __builtin_amdgcn_s_barrier();
-                __asm__ volatile("" ::: "memory");
-            }
+            // flight (__syncthreads would also wait for them).
+            __asm__ volatile("" ::: "memory");
+            __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS done
+            __builtin_amdgcn_s_barrier();
+            __asm__ volatile("" ::: "memory");
         }
     }
 }
@@ -327,9 +338,12 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint
 template <typename T, bool DIV, int KC>
 static void chain_launch(const Sweep& S, const Segment& sg, const T* cval, const T* diag, T* rslot, T* y, hipStream_t s,
                          Gate g) {
+    T* sink;
+    if constexpr (std::is_same<T, double>::value) sink = S.sink.get();
+    else sink = S.sink32.get();
     hipLaunchKernelGGL((k_sweep_chain<T, DIV, KC>), dim3(1), dim3(kChainThreads), 0, s, sg.lvl0, sg.lvl1,
-                       S.lvl_ptr.get(), S.slot_row.get(), S.len.get(), S.ccol.get() + sg.cbase, S.cds.get() + sg.cbase,
-                       cval + sg.cbase, diag, rslot, y, g);
+                       S.lvl_ptr.get(), S.slot_row.get(), S.len.get(), S.cds.get() + sg.cbase, cval + sg.cbase, diag,
+                       rslot, y, sink, g);
 }
 
 template <typename T, bool DIV>
