@@ -311,18 +311,25 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint
             const uint32_t l = base + k;
             if (l >= l1) break;  // uniform over the workgroup
             Row& cur = buf[k];
-            T acc = cur.r;
+            // a wavefront with no row in a level skips its arithmetic (the
+            // fetch stays unconditional, which keeps the compiler's waits on
+            // the prefetched rows exact): levels are narrower than the
+            // workgroup most of the time
+            const uint32_t wave0 = threadIdx.x & ~63u;
+            if (lp[l - l0] + wave0 < lp[l - l0 + 1]) {
+                T acc = cur.r;
 #pragma unroll
-            for (int e = 0; e < KC; ++e) {
-                const T t = acc - cur.v[e] * ring[cur.ds[e] % kRing];
-                acc = (uint32_t)e < cur.L ? t : acc;
+                for (int e = 0; e < KC; ++e) {
+                    const T t = acc - cur.v[e] * ring[cur.ds[e] % kRing];
+                    acc = (uint32_t)e < cur.L ? t : acc;
+                }
+                if constexpr (DIV) acc = acc / cur.d;
+                // stores without a branch: a thread without a row writes to
+                // the sinks (ring[kRing], and sink[0] in global memory)
+                const uint32_t s = lp[l - l0] + threadIdx.x;
+                ring[cur.on ? s % kRing : kRing] = acc;
+                *(cur.on ? y + cur.i : sink) = acc;
             }
-            if constexpr (DIV) acc = acc / cur.d;
-            // stores without a branch: a thread without a row writes to the
-            // sinks (ring[kRing], and sink[0] in global memory)
-            const uint32_t s = lp[l - l0] + threadIdx.x;
-            ring[cur.on ? s % kRing : kRing] = acc;
-            *(cur.on ? y + cur.i : sink) = acc;
             fetch(l + D, cur);
             // Between levels only the LDS ring must be ordered: wait for LDS
             // operations and barrier, leaving the loads fetched ahead in
