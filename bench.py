@@ -153,6 +153,13 @@ def c3_leg(F, gen, args, log):
     out = {"workload": f"2-D upwind convection-diffusion {args.c3_n}^2 (n={G.n}, nnz={G.nnz}), cell Peclet 10, "
                        "ILU(0), float64, tol 1e-10",
            "ilu0_setup_ms": round(setup_ms, 1), "ilu0_levels": M.info()["levels"]}
+    if args.c3_applies > 0:
+        # only preconditioner applications (for kernel counters), no solves
+        r = np.random.default_rng(0).standard_normal(G.n)
+        for _ in range(args.c3_applies):
+            M.apply(r)
+        out["applies"] = args.c3_applies
+        return out
     for name, solver in (("bicgstab_ilu0", "bicgstab"), ("gmres30_ilu0", "gmres")):
         S = F.Solver(A, M, solver=solver, precision=F.FP64, max_iter=args.c3_max_iter, restart=30, device_ptrs=True,
                      tol=1e-10)
@@ -228,6 +235,8 @@ def main():
     ap.add_argument("--c3", action="store_true", help="add the convection-diffusion ILU(0) leg (about 30 s)")
     ap.add_argument("--c3-n", type=int, default=1024, help="grid side of the convection-diffusion leg")
     ap.add_argument("--c3-max-iter", type=int, default=3000)
+    ap.add_argument("--c3-applies", type=int, default=0,
+                    help="only apply the ILU(0) preconditioner this many times (no solves)")
     ap.add_argument("--c5-count", type=int, default=512, help="systems in the batch")
     ap.add_argument("--spmv-probe", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration) and the float64 SpMV; "

@@ -60,6 +60,24 @@ def test_spmv_bitexact(case):
         assert np.array_equal(y, yo)
 
 
+@pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
+def test_spmv_row_lanes(lanes):
+    """The lane-per-row SpMV variants sum in another order: within 1e-13 of
+    the oracle; lanes 0 is the bit-exact CSR-stream kernel again."""
+    rng = np.random.default_rng(5)
+    M = sp.random(3000, 2500, density=0.01, random_state=rng, format="csr")
+    M.sort_indices()
+    A = F.CsrMatrix(3000, 2500, M.indptr, M.indices, M.data)
+    x = rng.uniform(-1, 1, 2500)
+    yo = O.Csr(3000, 2500, M.indptr, M.indices, M.data).spmv(x)
+    A.set_row_lanes(lanes)
+    np.testing.assert_allclose(A.spmv(x), yo, rtol=1e-13, atol=1e-13 * np.abs(yo).max())
+    A.set_row_lanes(0)
+    assert np.array_equal(A.spmv(x), yo)
+    with pytest.raises(F.InvalidArgument):
+        A.set_row_lanes(3)
+
+
 def test_dot():
     rng = np.random.default_rng(3)
     a, b = rng.standard_normal(1_000_003), rng.standard_normal(1_000_003)
