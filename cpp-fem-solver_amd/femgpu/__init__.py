@@ -463,6 +463,13 @@ def _hiprt():
     return _hip
 
 
+def device_synchronize():
+    """Wait for all work on the device (hipDeviceSynchronize)."""
+    rc = _hiprt().hipDeviceSynchronize()
+    if rc != 0:
+        raise DeviceError(f"hipDeviceSynchronize failed ({rc})")
+
+
 class DeviceArray:
     """float64 buffer in device memory."""
 
