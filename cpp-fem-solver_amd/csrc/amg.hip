@@ -106,7 +106,8 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
                                                       const T* __restrict__ gd, const T* __restrict__ f, T* u, T w,
                                                       T* res, uint32_t zero_from, Gate g) {
     if (g.closed()) return;
-    __shared__ T prod[kStageNnz];
+    constexpr int STAGE = stage_nnz<T>();
+    __shared__ T prod[STAGE];
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t r0 = blk[bid], r1 = blk[bid + 1];
     const uint32_t t = threadIdx.x;
@@ -116,9 +117,9 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
     }
     const uint32_t k0 = gp[r0], k1 = gp[r1];
     const uint32_t cnt = k1 - k0;
-    if (cnt <= (uint32_t)kStageNnz) {
+    if (cnt <= (uint32_t)STAGE) {
 #pragma unroll
-        for (int q = 0; q < kStageNnz / kBlock; ++q) {
+        for (int q = 0; q < STAGE / kBlock; ++q) {
             const uint32_t k = (uint32_t)q * kBlock + t;
             if (k < cnt) prod[k] = gv[k0 + k] * u[gci[k0 + k]];
         }
@@ -254,6 +255,7 @@ struct PcAmg : Precond {
                     vec_convert<float, double>(D.gv32.get(), D.gv.get(), 1.0, nz, s);
                     D.gd32.alloc(D.n);
                     vec_convert<float, double>(D.gd32.get(), D.gd.get(), 1.0, D.n, s);
+                    sgs_blocks32(D, s);
                 }
                 D.u232.alloc(D.n);
                 D.res32.alloc(D.n);
@@ -303,6 +305,16 @@ struct PcAmg : Precond {
         return cur;
     }
 
+    // the CSR-stream blocks of the grouped matrix for sweeps in type T
+    template <typename T>
+    static const std::vector<uint32_t>& cblk(const AmgLevelDev& D) {
+        return std::is_same<T, double>::value ? D.h_cblk : D.h_cblk32;
+    }
+    template <typename T>
+    static const uint32_t* gblk(const AmgLevelDev& D) {
+        return std::is_same<T, double>::value ? D.gblk.get() : D.gblk32.get();
+    }
+
     // `from_zero`: u has not been initialised; the sweep starts from u = 0
     template <typename T, int LANES>
     void gs_launch(AmgLevelDev& D, const T* gv, const T* gd, const T* f, T* u, int sweeps, bool reverse,
@@ -324,9 +336,9 @@ struct PcAmg : Precond {
                                        g);
                     continue;
                 }
-                const uint32_t b0 = D.h_cblk[c], b1 = D.h_cblk[c + 1];
+                const uint32_t b0 = cblk<T>(D)[c], b1 = cblk<T>(D)[c + 1];
                 if (b0 >= b1) continue;
-                hipLaunchKernelGGL((k_gs_stream<T, 0>), dim3(b1 - b0), dim3(kBlock), 0, s, D.gblk.get() + b0,
+                hipLaunchKernelGGL((k_gs_stream<T, 0>), dim3(b1 - b0), dim3(kBlock), 0, s, gblk<T>(D) + b0,
                                    D.crows.get(), D.gp.get(), D.gci.get(), gv, gd, f, u, w, (T*)nullptr, 0u, g);
             }
         if (from_zero && sweeps == 0) hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, u, D.n, g);
@@ -363,8 +375,8 @@ struct PcAmg : Precond {
         const T* gv;
         if constexpr (std::is_same<T, double>::value) gv = D.gv.get();
         else gv = D.gv32.get();
-        const uint32_t nb = D.h_cblk[D.ncol], zero_from = D.h_cblk[D.ncol - 1];
-        hipLaunchKernelGGL((k_gs_stream<T, 1>), dim3(nb), dim3(kBlock), 0, s, D.gblk.get(), D.crows.get(), D.gp.get(),
+        const uint32_t nb = cblk<T>(D)[D.ncol], zero_from = cblk<T>(D)[D.ncol - 1];
+        hipLaunchKernelGGL((k_gs_stream<T, 1>), dim3(nb), dim3(kBlock), 0, s, gblk<T>(D), D.crows.get(), D.gp.get(),
                            D.gci.get(), gv, (const T*)nullptr, f, u, (T)1, res, zero_from, g);
         return true;
     }

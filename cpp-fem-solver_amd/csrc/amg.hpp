@@ -50,10 +50,11 @@ struct AmgLevelDev {
     DevBuf<double> diag;  // a_ii (natural order)
     DevBuf<float> diag32;
     // colour-grouped copy of A for the GS sweeps; gblk: CSR-stream blocks of
-    // the grouped rows, colour c owning blocks [h_cblk[c], h_cblk[c+1])
+    // the grouped rows, colour c owning blocks [h_cblk[c], h_cblk[c+1]);
+    // h_cblk32/gblk32 the same for the float32 sweeps (larger blocks)
     int lanes = 1;
-    std::vector<uint32_t> h_cblk;
-    DevBuf<uint32_t> gblk;
+    std::vector<uint32_t> h_cblk, h_cblk32;
+    DevBuf<uint32_t> gblk, gblk32;
     DevBuf<uint32_t> gp, gci;
     DevBuf<double> gv, gd;
     DevBuf<float> gv32, gd32;
@@ -81,6 +82,8 @@ void transpose_device(const CsrMat& P, CsrMat& R, hipStream_t s);
 // SGS smoother data of level D (colouring, diagonal, Jacobi weight and, with
 // at most kMaxGsColours colours, the colour-grouped copy of D.A)
 void sgs_setup_device(AmgLevelDev& D, hipStream_t s);
+// the float32 sweep blocks (gblk32, h_cblk32) of a level set up by sgs_setup_device
+void sgs_blocks32(AmgLevelDev& D, hipStream_t s);
 // true if every row of A lists strictly increasing columns
 bool rows_sorted_device(const CsrMat& A, hipStream_t s);
 

@@ -727,6 +727,22 @@ void sgs_setup_device(AmgLevelDev& D, hipStream_t s) {
     FEM_HIP(hipStreamSynchronize(s));  // the host arrays end with this call
 }
 
+void sgs_blocks32(AmgLevelDev& D, hipStream_t s) {
+    if (!D.gp.get() || D.h_cptr.size() < 2) return;
+    const uint32_t n = D.n;
+    std::vector<uint32_t> gp(n + 1);
+    D.gp.download(gp.data(), n + 1, s);
+    FEM_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> blk;
+    D.h_cblk32.assign(1, 0);
+    for (size_t c = 0; c + 1 < D.h_cptr.size(); ++c) {
+        plan_rows(gp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk, stage_nnz<float>(), stage_rows<float>());
+        D.h_cblk32.push_back((uint32_t)blk.size() - 1);
+    }
+    D.gblk32.upload(blk, s);
+    FEM_HIP(hipStreamSynchronize(s));  // blk ends with this call
+}
+
 AmgDevHierarchy upload_hierarchy(AmgHierarchy&& H, hipStream_t s) {
     AmgDevHierarchy D;
     const size_t L = H.A.size();

@@ -18,30 +18,31 @@
 
 namespace fem {
 
-void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint32_t>& b) {
+void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint32_t>& b, uint32_t max_nnz,
+               uint32_t max_rows) {
     if (b.empty()) b.push_back(lo);
     uint32_t start = lo;
     while (start < hi) {
-        if (h_rp[start + 1] - h_rp[start] > (uint32_t)kStageNnz) {
+        if (h_rp[start + 1] - h_rp[start] > max_nnz) {
             ++start;
             b.push_back(start);
             continue;
         }
         uint32_t end = start;
-        while (end < hi && end - start < (uint32_t)kStageRows && h_rp[end + 1] - h_rp[start] <= (uint32_t)kStageNnz)
-            ++end;
+        while (end < hi && end - start < max_rows && h_rp[end + 1] - h_rp[start] <= max_nnz) ++end;
         b.push_back(end);
         start = end;
     }
 }
 
-void SpmvPlan::build(const uint32_t* h_rp, uint32_t n, hipStream_t s) {
+void SpmvPlan::build(const uint32_t* h_rp, uint32_t n, hipStream_t s, uint32_t max_nnz, uint32_t max_rows) {
     std::vector<uint32_t> b;
     b.reserve(n / 128 + 4);
     b.push_back(0);
-    plan_rows(h_rp, 0, n, b);
+    plan_rows(h_rp, 0, n, b, max_nnz, max_rows);
     nblk = (uint32_t)b.size() - 1;
     blk_row.upload(b, s);
+    FEM_HIP(hipStreamSynchronize(s));  // b ends with this call
 }
 
 template <typename T, int EPI>
@@ -58,15 +59,16 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
                                                  const T* __restrict__ v, const T* __restrict__ x, T* __restrict__ y,
                                                  const uint32_t* __restrict__ blk_row, SpmvArgs<T> a, Gate g) {
     if (g.closed()) return;
-    __shared__ T prod[kStageNnz];
+    constexpr int STAGE = stage_nnz<T>();
+    __shared__ T prod[STAGE];
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
     const uint32_t r0 = blk_row[bid], r1 = blk_row[bid + 1];
     const uint32_t k0 = rp[r0], k1 = rp[r1];
     const uint32_t cnt = k1 - k0;
     const uint32_t t = threadIdx.x;
-    if (cnt <= (uint32_t)kStageNnz) {
+    if (cnt <= (uint32_t)STAGE) {
 #pragma unroll
-        for (int u = 0; u < kStageNnz / kBlock; ++u) {
+        for (int u = 0; u < STAGE / kBlock; ++u) {
             const uint32_t k = (uint32_t)u * kBlock + t;
             if (k < cnt) prod[k] = v[k0 + k] * x[ci[k0 + k]];
         }
@@ -141,27 +143,28 @@ void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, 
         FEM_HIP(hipGetLastError());
         return;
     }
-    dim3 grid(A.plan.nblk), block(kBlock);
+    const SpmvPlan& plan = std::is_same<T, double>::value ? A.plan : A.plan32;
+    dim3 grid(plan.nblk), block(kBlock);
     switch (epi) {
         case EPI_STORE:
             hipLaunchKernelGGL((k_spmv<T, EPI_STORE>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               A.plan.blk_row.get(), a, g);
+                               plan.blk_row.get(), a, g);
             break;
         case EPI_RESID:
             hipLaunchKernelGGL((k_spmv<T, EPI_RESID>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               A.plan.blk_row.get(), a, g);
+                               plan.blk_row.get(), a, g);
             break;
         case EPI_ADD:
             hipLaunchKernelGGL((k_spmv<T, EPI_ADD>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               A.plan.blk_row.get(), a, g);
+                               plan.blk_row.get(), a, g);
             break;
         case EPI_JACOBI:
             hipLaunchKernelGGL((k_spmv<T, EPI_JACOBI>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               A.plan.blk_row.get(), a, g);
+                               plan.blk_row.get(), a, g);
             break;
         default:
             hipLaunchKernelGGL((k_spmv<T, EPI_JACOBI_INV>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               A.plan.blk_row.get(), a, g);
+                               plan.blk_row.get(), a, g);
             break;
     }
     FEM_HIP(hipGetLastError());
@@ -182,6 +185,14 @@ void CsrMat::ensure_fp32(hipStream_t s) {
     v32.alloc(nnz + 4);
     FEM_HIP(hipMemsetAsync(v32.get() + nnz, 0, 4 * sizeof(float), s));
     vec_convert<float, double>(v32.get(), v.get(), 1.0, nnz, s);
+    if (host_valid) {
+        plan32.build(h_rp.data(), n, s, stage_nnz<float>(), stage_rows<float>());
+    } else {
+        std::vector<uint32_t> hrp(n + 1);
+        rp.download(hrp.data(), n + 1, s);
+        FEM_HIP(hipStreamSynchronize(s));
+        plan32.build(hrp.data(), n, s, stage_nnz<float>(), stage_rows<float>());
+    }
 }
 
 void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, std::vector<uint32_t> ci_,

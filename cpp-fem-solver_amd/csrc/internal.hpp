@@ -63,13 +63,20 @@ __device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t nb) {
 // most kStageRows consecutive rows in LDS (a longer row gets a block alone)
 constexpr int kStageNnz = 2048;
 constexpr int kStageRows = 256;
+// float32 blocks stage twice the products in the same 16 KB of LDS
+template <typename T>
+constexpr int stage_nnz() { return sizeof(T) == 4 ? 2 * kStageNnz : kStageNnz; }
+template <typename T>
+constexpr int stage_rows() { return sizeof(T) == 4 ? 2 * kStageRows : kStageRows; }
 // append the block boundaries of rows [lo, hi) to b (b starts with lo)
-void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint32_t>& b);
+void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint32_t>& b,
+               uint32_t max_nnz = kStageNnz, uint32_t max_rows = kStageRows);
 
 struct SpmvPlan {
     DevBuf<uint32_t> blk_row;  // [nblk+1]
     uint32_t nblk = 0;
-    void build(const uint32_t* h_rp, uint32_t n, hipStream_t s);
+    void build(const uint32_t* h_rp, uint32_t n, hipStream_t s, uint32_t max_nnz = kStageNnz,
+               uint32_t max_rows = kStageRows);
 };
 
 struct CsrMat : LinOp {
@@ -78,7 +85,7 @@ struct CsrMat : LinOp {
     DevBuf<uint32_t> rp, ci;
     DevBuf<double> v;
     DevBuf<float> v32;
-    SpmvPlan plan;
+    SpmvPlan plan, plan32;  // blockings of the float64 and float32 SpMV
     // host mirror (filled on demand for host-side setup)
     mutable std::vector<uint32_t> h_rp, h_ci;
     mutable std::vector<double> h_v;
