@@ -98,11 +98,13 @@ __global__ __launch_bounds__(1024) void k_gs_small(int ncol, const uint32_t* __r
 // rows instead, and blocks from `zero_from` on write res = 0 without reading
 // anything (the last colour right after a forward sweep with weight 1, whose
 // residual is zero in exact arithmetic).
-template <typename T, int MODE>
+template <typename T, int MODE, bool PK>
 __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict__ blk,
                                                       const uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ gp,
-                                                      const uint32_t* __restrict__ gci, const T* __restrict__ gv,
+                                                      const uint32_t* __restrict__ gci,
+                                                      const uint16_t* __restrict__ gcd,
+                                                      const uint32_t* __restrict__ gcbase, const T* __restrict__ gv,
                                                       const T* __restrict__ gd, const T* __restrict__ f, T* u, T w,
                                                       T* res, uint32_t zero_from, Gate g) {
     if (g.closed()) return;
@@ -118,11 +120,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
     const uint32_t k0 = gp[r0], k1 = gp[r1];
     const uint32_t cnt = k1 - k0;
     if (cnt <= (uint32_t)STAGE) {
-#pragma unroll
-        for (int q = 0; q < STAGE / kBlock; ++q) {
-            const uint32_t k = (uint32_t)q * kBlock + t;
-            if (k < cnt) prod[k] = gv[k0 + k] * u[gci[k0 + k]];
-        }
+        if (cnt > 0) stage_products<T, STAGE, PK>(gci, gcd, gcbase, gv, u, k0, cnt, prod);
         __syncthreads();
         for (uint32_t row = r0 + t; row < r1; row += kBlock) {
             const uint32_t b = gp[row] - k0, e = gp[row + 1] - k0;
@@ -134,7 +132,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
         }
     } else {
         double s = 0.0;
-        for (uint32_t k = k0 + t; k < k1; k += kBlock) s += (double)(gv[k] * u[gci[k]]);
+        for (uint32_t k = k0 + t; k < k1; k += kBlock) s += (double)(gv[k] * u[col_of<PK>(gci, gcd, gcbase, k)]);
         double tot[1] = {s};
         block_sum<1>(tot);
         if (t == 0) {
@@ -338,8 +336,14 @@ struct PcAmg : Precond {
                 }
                 const uint32_t b0 = cblk<T>(D)[c], b1 = cblk<T>(D)[c + 1];
                 if (b0 >= b1) continue;
-                hipLaunchKernelGGL((k_gs_stream<T, 0>), dim3(b1 - b0), dim3(kBlock), 0, s, gblk<T>(D) + b0,
-                                   D.crows.get(), D.gp.get(), D.gci.get(), gv, gd, f, u, w, (T*)nullptr, 0u, g);
+                if (D.gcd.get())
+                    hipLaunchKernelGGL((k_gs_stream<T, 0, true>), dim3(b1 - b0), dim3(kBlock), 0, s, gblk<T>(D) + b0,
+                                       D.crows.get(), D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, gd, f,
+                                       u, w, (T*)nullptr, 0u, g);
+                else
+                    hipLaunchKernelGGL((k_gs_stream<T, 0, false>), dim3(b1 - b0), dim3(kBlock), 0, s, gblk<T>(D) + b0,
+                                       D.crows.get(), D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, gd, f,
+                                       u, w, (T*)nullptr, 0u, g);
             }
         if (from_zero && sweeps == 0) hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, u, D.n, g);
     }
@@ -376,8 +380,14 @@ struct PcAmg : Precond {
         if constexpr (std::is_same<T, double>::value) gv = D.gv.get();
         else gv = D.gv32.get();
         const uint32_t nb = cblk<T>(D)[D.ncol], zero_from = cblk<T>(D)[D.ncol - 1];
-        hipLaunchKernelGGL((k_gs_stream<T, 1>), dim3(nb), dim3(kBlock), 0, s, gblk<T>(D), D.crows.get(), D.gp.get(),
-                           D.gci.get(), gv, (const T*)nullptr, f, u, (T)1, res, zero_from, g);
+        if (D.gcd.get())
+            hipLaunchKernelGGL((k_gs_stream<T, 1, true>), dim3(nb), dim3(kBlock), 0, s, gblk<T>(D), D.crows.get(),
+                               D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, (const T*)nullptr, f, u, (T)1,
+                               res, zero_from, g);
+        else
+            hipLaunchKernelGGL((k_gs_stream<T, 1, false>), dim3(nb), dim3(kBlock), 0, s, gblk<T>(D), D.crows.get(),
+                               D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, (const T*)nullptr, f, u, (T)1,
+                               res, zero_from, g);
         return true;
     }
 

@@ -56,6 +56,8 @@ struct AmgLevelDev {
     std::vector<uint32_t> h_cblk, h_cblk32;
     DevBuf<uint32_t> gblk, gblk32;
     DevBuf<uint32_t> gp, gci;
+    DevBuf<uint16_t> gcd;     // packed gci (empty: not packable)
+    DevBuf<uint32_t> gcbase;
     DevBuf<double> gv, gd;
     DevBuf<float> gv32, gd32;
 };

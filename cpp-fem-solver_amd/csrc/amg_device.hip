@@ -712,6 +712,7 @@ void sgs_setup_device(AmgLevelDev& D, hipStream_t s) {
     hipLaunchKernelGGL(k_group_fill, dim3(g1(n)), dim3(256), 0, s, A.rp.get(), A.ci.get(), A.v.get(), D.diag.get(),
                        D.crows.get(), n, D.gp.get(), D.gci.get(), D.gv.get(), D.gd.get());
     FEM_HIP(hipGetLastError());
+    pack_columns(D.gci.get(), A.nnz, D.gcd, D.gcbase, s);
     std::vector<uint32_t> gp(n + 1);
     D.gp.download(gp.data(), n + 1, s);
     FEM_HIP(hipStreamSynchronize(s));

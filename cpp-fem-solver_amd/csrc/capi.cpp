@@ -224,6 +224,19 @@ int fem_csr_set_row_lanes(fem_csr A, int lanes) {
     });
 }
 
+int fem_csr_pack_columns(fem_csr A, int on, int* packed) {
+    return guard([&] {
+        auto& M = A->A;
+        if (on > 0 && !M.cd.get()) pack_columns(M.ci.get(), M.nnz, M.cd, M.cbase, M.stream);
+        if (on == 0) {
+            FEM_HIP(hipStreamSynchronize(M.stream));
+            M.cd.release();
+            M.cbase.release();
+        }
+        if (packed) *packed = M.cd.get() ? 1 : 0;
+    });
+}
+
 int fem_spmv(fem_csr A, const double* x, double* y) {
     return guard([&] {
         auto& M = A->A;

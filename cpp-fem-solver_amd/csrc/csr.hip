@@ -54,9 +54,11 @@ __device__ __forceinline__ T spmv_epilogue(uint32_t row, T s, const SpmvArgs<T>&
     else return a.u[row] + (T)a.w * (a.d[row] * (a.f[row] - s));
 }
 
-template <typename T, int EPI>
+template <typename T, int EPI, bool PK>
 __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
-                                                 const T* __restrict__ v, const T* __restrict__ x, T* __restrict__ y,
+                                                 const uint16_t* __restrict__ cd,
+                                                 const uint32_t* __restrict__ cbase, const T* __restrict__ v,
+                                                 const T* __restrict__ x, T* __restrict__ y,
                                                  const uint32_t* __restrict__ blk_row, SpmvArgs<T> a, Gate g) {
     if (g.closed()) return;
     constexpr int STAGE = stage_nnz<T>();
@@ -67,11 +69,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
     const uint32_t cnt = k1 - k0;
     const uint32_t t = threadIdx.x;
     if (cnt <= (uint32_t)STAGE) {
-#pragma unroll
-        for (int u = 0; u < STAGE / kBlock; ++u) {
-            const uint32_t k = (uint32_t)u * kBlock + t;
-            if (k < cnt) prod[k] = v[k0 + k] * x[ci[k0 + k]];
-        }
+        if (cnt > 0) stage_products<T, STAGE, PK>(ci, cd, cbase, v, x, k0, cnt, prod);
         __syncthreads();
         for (uint32_t row = r0 + t; row < r1; row += kBlock) {
             const uint32_t b = rp[row] - k0, e = rp[row + 1] - k0;
@@ -81,11 +79,46 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
         }
     } else {
         double s = 0.0;
-        for (uint32_t k = k0 + t; k < k1; k += kBlock) s += (double)(v[k] * x[ci[k]]);
+        for (uint32_t k = k0 + t; k < k1; k += kBlock) s += (double)(v[k] * x[col_of<PK>(ci, cd, cbase, k)]);
         double tot[1] = {s};
         block_sum<1>(tot);
         if (t == 0) y[r0] = spmv_epilogue<T, EPI>(r0, (T)tot[0], a);
     }
+}
+
+// cbase[c] = least column of chunk c (256 entries, one block each); *bad |= 1
+// when the chunk's columns span 65536 or more
+__global__ __launch_bounds__(256) void k_pack_base(const uint32_t* __restrict__ ci, uint64_t nnz,
+                                                   uint32_t* __restrict__ cbase, int* __restrict__ bad) {
+    __shared__ uint32_t smin[4], smax[4];
+    const uint64_t k0 = (uint64_t)blockIdx.x << kPackShift;
+    const uint64_t k = k0 + threadIdx.x;
+    uint32_t mn = ci[k < nnz ? k : k0], mx = mn;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        mn = min(mn, (uint32_t)__shfl_xor(mn, off));
+        mx = max(mx, (uint32_t)__shfl_xor(mx, off));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        smin[threadIdx.x >> 6] = mn;
+        smax[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; ++w) {
+            mn = min(mn, smin[w]);
+            mx = max(mx, smax[w]);
+        }
+        cbase[blockIdx.x] = mn;
+        if (mx - mn > 65535u) atomicOr(bad, 1);
+    }
+}
+
+__global__ void k_pack_delta(const uint32_t* __restrict__ ci, uint64_t nnz, const uint32_t* __restrict__ cbase,
+                             uint16_t* __restrict__ cd) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) cd[k] = (uint16_t)(ci[k] - cbase[k >> kPackShift]);
+    else if (k < nnz + 4) cd[k] = 0;
 }
 
 // Vector-per-row SpMV: LANES consecutive lanes own one row, stride over its
@@ -120,6 +153,45 @@ void spmv_vec_launch(const CsrMat& A, const T* vals, const T* x, T* y, int epi, 
     }
 }
 
+template <typename T, int EPI>
+void spmv_stream_launch(const CsrMat& A, const T* vals, const T* x, T* y, const SpmvArgs<T>& a, hipStream_t s,
+                        Gate g) {
+    const SpmvPlan& plan = std::is_same<T, double>::value ? A.plan : A.plan32;
+    const dim3 grid(plan.nblk), block(kBlock);
+    if (A.cd.get())
+        hipLaunchKernelGGL((k_spmv<T, EPI, true>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
+                           A.cbase.get(), vals, x, y, plan.blk_row.get(), a, g);
+    else
+        hipLaunchKernelGGL((k_spmv<T, EPI, false>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
+                           A.cbase.get(), vals, x, y, plan.blk_row.get(), a, g);
+}
+
+bool pack_columns(const uint32_t* ci, uint64_t nnz, DevBuf<uint16_t>& cd, DevBuf<uint32_t>& cbase, hipStream_t s) {
+    cd.release();
+    cbase.release();
+    if (nnz < kPackMinNnz || nnz > 0xffffffffull) return false;
+    const uint64_t nch = (nnz + (1u << kPackShift) - 1) >> kPackShift;
+    cbase.alloc(nch);
+    DevBuf<int> bad(1);
+    FEM_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
+    hipLaunchKernelGGL(k_pack_base, dim3((unsigned)nch), dim3(1u << kPackShift), 0, s, ci, nnz, cbase.get(),
+                       bad.get());
+    FEM_HIP(hipGetLastError());
+    int h = 0;
+    bad.download(&h, 1, s);
+    FEM_HIP(hipStreamSynchronize(s));
+    if (h) {
+        cbase.release();
+        return false;
+    }
+    cd.alloc(nnz + 4);
+    hipLaunchKernelGGL(k_pack_delta, dim3((unsigned)((nnz + 4 + 255) / 256)), dim3(256), 0, s, ci, nnz, cbase.get(),
+                       cd.get());
+    FEM_HIP(hipGetLastError());
+    FEM_HIP(hipStreamSynchronize(s));  // bad ends with this call
+    return true;
+}
+
 void CsrMat::use_vector_rows() {
     const double avg = n ? (double)nnz / n : 0.0;
     lanes = avg < 40 ? 0 : avg < 96 ? 32 : 64;
@@ -143,29 +215,12 @@ void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, 
         FEM_HIP(hipGetLastError());
         return;
     }
-    const SpmvPlan& plan = std::is_same<T, double>::value ? A.plan : A.plan32;
-    dim3 grid(plan.nblk), block(kBlock);
     switch (epi) {
-        case EPI_STORE:
-            hipLaunchKernelGGL((k_spmv<T, EPI_STORE>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               plan.blk_row.get(), a, g);
-            break;
-        case EPI_RESID:
-            hipLaunchKernelGGL((k_spmv<T, EPI_RESID>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               plan.blk_row.get(), a, g);
-            break;
-        case EPI_ADD:
-            hipLaunchKernelGGL((k_spmv<T, EPI_ADD>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               plan.blk_row.get(), a, g);
-            break;
-        case EPI_JACOBI:
-            hipLaunchKernelGGL((k_spmv<T, EPI_JACOBI>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               plan.blk_row.get(), a, g);
-            break;
-        default:
-            hipLaunchKernelGGL((k_spmv<T, EPI_JACOBI_INV>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y,
-                               plan.blk_row.get(), a, g);
-            break;
+        case EPI_STORE: spmv_stream_launch<T, EPI_STORE>(A, vals, x, y, a, s, g); break;
+        case EPI_RESID: spmv_stream_launch<T, EPI_RESID>(A, vals, x, y, a, s, g); break;
+        case EPI_ADD: spmv_stream_launch<T, EPI_ADD>(A, vals, x, y, a, s, g); break;
+        case EPI_JACOBI: spmv_stream_launch<T, EPI_JACOBI>(A, vals, x, y, a, s, g); break;
+        default: spmv_stream_launch<T, EPI_JACOBI_INV>(A, vals, x, y, a, s, g); break;
     }
     FEM_HIP(hipGetLastError());
 }
@@ -210,6 +265,7 @@ void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, s
     ci.upload(ci_, s);
     v.upload(v_, s);
     plan.build(rp_.data(), n, s);
+    pack_columns(ci.get(), nnz, cd, cbase, s);
     h_rp = std::move(rp_);
     h_ci = std::move(ci_);
     h_v = std::move(v_);
@@ -231,6 +287,7 @@ void CsrMat::adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&&
     rp.download(hrp.data(), n + 1, s);
     FEM_HIP(hipStreamSynchronize(s));
     plan.build(hrp.data(), n, s);
+    pack_columns(ci.get(), nnz, cd, cbase, s);
     h_rp.clear();
     h_ci.clear();
     h_v.clear();

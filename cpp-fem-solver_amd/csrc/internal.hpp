@@ -79,10 +79,64 @@ struct SpmvPlan {
                uint32_t max_rows = kStageRows);
 };
 
+// Packed column indices: col[k] = cbase[k >> kPackShift] + cd[k], 2 bytes per
+// entry instead of 4, for matrices whose columns span less than 65536 inside
+// every chunk of 256 consecutive entries (banded FEM orderings: the
+// z-neighbours of a 128^3 grid sit 16384 rows away). The CSR-stream kernels
+// read cd and cbase instead of ci; products and summation order are unchanged.
+constexpr int kPackShift = 8;
+constexpr uint64_t kPackMinNnz = 1u << 16;  // smaller matrices keep ci only
+// builds cd (nnz + 4 entries, zero padded) and cbase from the device array ci;
+// false (both released) when a chunk spans 65536 columns or more or nnz is small
+bool pack_columns(const uint32_t* ci, uint64_t nnz, DevBuf<uint16_t>& cd, DevBuf<uint32_t>& cbase, hipStream_t s);
+// column of entry k: from the packed form or from ci
+template <bool PK>
+__device__ __forceinline__ uint32_t col_of(const uint32_t* __restrict__ ci, const uint16_t* __restrict__ cd,
+                                           const uint32_t* __restrict__ cbase, uint32_t k) {
+    if constexpr (PK) return cbase[k >> kPackShift] + (uint32_t)cd[k];
+    else return ci[k];
+}
+
+// CSR-stream staging: prod[k] = v[k0+k] * x[col[k0+k]] for k < cnt (cnt >= 1)
+// by a block of kBlock threads, in batches of 8 entries per thread: a thread
+// issues the batch's value and column loads (indices clamped to cnt - 1, so
+// every load is in range and all of them can be in flight), then its gathers
+// of x, then the LDS stores of the products inside the range.
+template <typename T, int STAGE, bool PK>
+__device__ __forceinline__ void stage_products(const uint32_t* __restrict__ ci, const uint16_t* __restrict__ cd,
+                                               const uint32_t* __restrict__ cbase, const T* __restrict__ v,
+                                               const T* x, uint32_t k0, uint32_t cnt, T* prod) {
+    constexpr int U = STAGE / kBlock;
+    constexpr int B = U < 8 ? U : 8;
+    static_assert(U % B == 0, "stage size");
+    const uint32_t t = threadIdx.x;
+#pragma unroll
+    for (int q0 = 0; q0 < U; q0 += B) {
+        if ((uint32_t)q0 * kBlock >= cnt) break;
+        uint32_t c[B];
+        T a[B], xv[B];
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const uint32_t k = k0 + min((uint32_t)(q0 + q) * kBlock + t, cnt - 1);
+            c[q] = col_of<PK>(ci, cd, cbase, k);
+            a[q] = v[k];
+        }
+#pragma unroll
+        for (int q = 0; q < B; ++q) xv[q] = x[c[q]];
+#pragma unroll
+        for (int q = 0; q < B; ++q) {
+            const uint32_t k = (uint32_t)(q0 + q) * kBlock + t;
+            if (k < cnt) prod[k] = a[q] * xv[q];
+        }
+    }
+}
+
 struct CsrMat : LinOp {
     uint32_t n = 0, m = 0;
     uint64_t nnz = 0;
     DevBuf<uint32_t> rp, ci;
+    DevBuf<uint16_t> cd;     // packed columns (empty: not packable)
+    DevBuf<uint32_t> cbase;  // per chunk of 256 entries
     DevBuf<double> v;
     DevBuf<float> v32;
     SpmvPlan plan, plan32;  // blockings of the float64 and float32 SpMV

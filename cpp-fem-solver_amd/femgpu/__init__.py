@@ -113,6 +113,7 @@ _SIGS = {
     "fem_csr_multiply": (C.c_int, [P_vp, P_vp, C.POINTER(P_vp)]),
     "fem_csr_destroy": (C.c_int, [P_vp]),
     "fem_csr_set_row_lanes": (C.c_int, [P_vp, C.c_int]),
+    "fem_csr_pack_columns": (C.c_int, [P_vp, C.c_int, C.POINTER(C.c_int)]),
     "fem_spmv": (C.c_int, [P_vp, P_f64, P_f64]),
     "fem_spmv_device": (C.c_int, [P_vp, P_vp, P_vp, C.c_int]),
     "fem_dot": (C.c_int, [C.c_uint64, P_f64, P_f64, P_f64]),
@@ -269,6 +270,14 @@ class CsrMatrix:
         """SpMV variant: 0 = CSR-stream (bit-exact with SMatrix::spmv),
         4/8/16/32/64 = that many lanes per row."""
         _check(lib().fem_csr_set_row_lanes(self._h, int(lanes)))
+
+    def pack_columns(self, on=-1) -> bool:
+        """16-bit packed column offsets for the CSR-stream SpMV: on = 1 builds
+        them (where every 256-entry chunk spans < 65536 columns), 0 drops
+        them, -1 only asks. Returns whether the packed form is in use."""
+        p = C.c_int()
+        _check(lib().fem_csr_pack_columns(self._h, int(on), C.byref(p)))
+        return bool(p.value)
 
     def bench_spmv(self, fp32=False, reps=100, cold=False) -> float:
         """Average SpMV time (ms): back-to-back launches, or each launch

@@ -167,6 +167,16 @@ int fem_csr_destroy(fem_csr A);
  * stored order and match SMatrix::spmv bit for bit; 4, 8, 16, 32 or 64 give
  * each row that many lanes and a butterfly sum (last bits differ). Default 0. */
 int fem_csr_set_row_lanes(fem_csr A, int lanes);
+/* Column-index storage of A's CSR-stream SpMV (no counterpart in the
+ * reference): with on = 1 the columns are also kept as 16-bit offsets from a
+ * base per 256 entries, read instead of the 32-bit indices (2 bytes less per
+ * nonzero), when every such chunk spans fewer than 65536 columns and nnz >=
+ * 65536; on = 0 drops that copy; on < 0 changes nothing. *packed (may be
+ * NULL) receives 1 if the packed form is in use. Results are identical either
+ * way. Matrices are packed by default where possible. Change it only before
+ * creating preconditioners or solvers on A (captured solves keep the form
+ * they were captured with). */
+int fem_csr_pack_columns(fem_csr A, int on, int* packed);
 
 /* ---- primitives --------------------------------------------------------- */
 /* SMatrix::spmv, array.cpp:3418-3442 */

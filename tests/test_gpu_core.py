@@ -60,6 +60,59 @@ def test_spmv_bitexact(case):
         assert np.array_equal(y, yo)
 
 
+def _spmv32(A, x):
+    """float32 SpMV through fem_spmv_device on raw float32 device buffers."""
+    import ctypes as C
+    n, m = A.rows, A.cols
+    xd, yd = F.DeviceArray((m + 1) // 2), F.DeviceArray((n + 1) // 2)
+    x32 = np.ascontiguousarray(x, dtype=np.float32)
+    assert F._hiprt().hipMemcpy(xd.ptr, x32.ctypes.data_as(C.c_void_p), m * 4, 1) == 0
+    F._check(F.lib().fem_spmv_device(A._h, xd.ptr, yd.ptr, 1))
+    return yd.download().view(np.float32)[:n].copy()
+
+
+@pytest.mark.parametrize("case", ["poisson3d_48", "long_rows", "wide_columns"])
+def test_spmv_packed_columns(case):
+    """16-bit packed column offsets: used where every 256-entry chunk spans
+    < 65536 columns; float64 results stay bit-exact with the oracle and the
+    float32 ones identical to the unpacked kernel's."""
+    rng = np.random.default_rng(17)
+    if case == "poisson3d_48":
+        G = gen.poisson3d(48)        # nnz 774,144; z-neighbours 2304 rows away
+        rp, ci, v, n, m = G.row_ptr, G.col_idx, G.values, G.n, G.m
+        packable, exact = True, True
+    elif case == "long_rows":
+        M = sp.random(2000, 40000, density=0.002, random_state=rng, format="lil")
+        M[5, :30000] = rng.uniform(-1, 1, 30000)    # rows longer than an LDS stage
+        M[1500, 9000:40000] = rng.uniform(-1, 1, 31000)
+        M = M.tocsr()
+        M.sort_indices()
+        rp, ci, v, n, m = M.indptr, M.indices, M.data, 2000, 40000
+        packable, exact = True, False
+    else:
+        M = sp.random(3000, 400000, density=0.0002, random_state=rng, format="csr")
+        M.sort_indices()
+        rp, ci, v, n, m = M.indptr, M.indices, M.data, 3000, 400000
+        packable, exact = False, True
+    assert len(ci) >= 65536
+    A = F.CsrMatrix(n, m, rp, ci, v)
+    assert A.pack_columns() == packable
+    x = rng.uniform(-1, 1, m)
+    yo = O.Csr(n, m, rp, ci, v).spmv(x)
+    y = A.spmv(x)
+    y32 = _spmv32(A, x)
+    if exact:
+        assert np.array_equal(y, yo)
+    else:
+        np.testing.assert_allclose(y, yo, rtol=1e-13, atol=1e-13 * np.abs(yo).max())
+    np.testing.assert_allclose(y32, yo, rtol=1e-4, atol=1e-5 * np.abs(yo).max())
+    assert A.pack_columns(0) is False
+    assert np.array_equal(A.spmv(x), y)
+    assert np.array_equal(_spmv32(A, x), y32)
+    assert A.pack_columns(1) == packable
+    assert np.array_equal(A.spmv(x), y)
+
+
 @pytest.mark.parametrize("lanes", [4, 8, 16, 32, 64])
 def test_spmv_row_lanes(lanes):
     """The lane-per-row SpMV variants sum in another order: within 1e-13 of
