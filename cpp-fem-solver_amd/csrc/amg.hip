@@ -218,6 +218,11 @@ struct PcAmg : Precond {
                 D.res.alloc(D.n);
                 if (smoother == FEM_SMOOTH_SGS) {
                     sgs_setup_device(D, s);
+                    if (residual_shortcut(D)) {
+                        const uint32_t lo = D.h_cptr[D.ncol - 1], hi = D.h_cptr[D.ncol];
+                        drop_columns_device(D.R, D.crows.get() + lo, hi - lo, D.Rz, s);
+                        D.Rz.use_vector_rows();
+                    }
                     tr.lap("smoother colouring and grouped copy");
                 }
             } else {
@@ -241,6 +246,7 @@ struct PcAmg : Precond {
             if (k + 1 < lv.size()) {
                 D.P.ensure_fp32(s);
                 D.R.ensure_fp32(s);
+                if (D.Rz.n) D.Rz.ensure_fp32(s);
                 D.d32.alloc(D.n);
                 vec_convert<float, double>(D.d32.get(), D.d.get(), 1.0, D.n, s);
                 if (D.diag.get()) {
@@ -372,14 +378,20 @@ struct PcAmg : Precond {
 
     // residual f - A u after a forward Gauss-Seidel sweep with weight 1 on a
     // large level: the last colour's rows are zero, the others are computed
-    // on the colour-grouped matrix
+    // on the colour-grouped matrix. With Rz (R without those columns) the
+    // zeros are not written: only Rz reads this residual.
+    bool residual_shortcut(const AmgLevelDev& D) const {
+        return D.use_gs && D.n > 8192 && D.omega == 1.0 && npre >= 1 && D.ncol >= 2;
+    }
     template <typename T>
     bool gs_residual(AmgLevelDev& D, const T* f, T* u, T* res, hipStream_t s, Gate g) {
-        if (!D.use_gs || D.n <= 8192 || D.omega != 1.0 || npre < 1 || D.ncol < 2) return false;
+        if (!residual_shortcut(D)) return false;
         const T* gv;
         if constexpr (std::is_same<T, double>::value) gv = D.gv.get();
         else gv = D.gv32.get();
-        const uint32_t nb = cblk<T>(D)[D.ncol], zero_from = cblk<T>(D)[D.ncol - 1];
+        const uint32_t zero_from = cblk<T>(D)[D.ncol - 1];
+        const uint32_t nb = D.Rz.n ? zero_from : cblk<T>(D)[D.ncol];
+        if (nb == 0) return true;
         if (D.gcd.get())
             hipLaunchKernelGGL((k_gs_stream<T, 1, true>), dim3(nb), dim3(kBlock), 0, s, gblk<T>(D), D.crows.get(),
                                D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, (const T*)nullptr, f, u, (T)1,
@@ -435,11 +447,11 @@ struct PcAmg : Precond {
             ucur[k] = u;
             SpmvArgs<T> ar;
             ar.f = f;
-            if (!(smoother == FEM_SMOOTH_SGS && gs_residual<T>(D, f, u, B.res, s, g)))
-                csr_spmv<T>(D.A, u, B.res, EPI_RESID, ar, s, g);
+            const bool shortcut = smoother == FEM_SMOOTH_SGS && gs_residual<T>(D, f, u, B.res, s, g);
+            if (!shortcut) csr_spmv<T>(D.A, u, B.res, EPI_RESID, ar, s, g);
             AmgLevelDev& C = *lv[k + 1];
             Bufs<T> BC = bufs<T>(C);
-            csr_spmv<T>(D.R, B.res, BC.f, EPI_STORE, {}, s, g);
+            csr_spmv<T>(shortcut && D.Rz.n ? D.Rz : D.R, B.res, BC.f, EPI_STORE, {}, s, g);
             fk[k + 1] = BC.f;
         }
         // coarsest level: u = C f (dense, float64 accumulation)

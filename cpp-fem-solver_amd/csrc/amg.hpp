@@ -57,6 +57,9 @@ struct AmgLevelDev {
     DevBuf<uint32_t> gblk, gblk32;
     DevBuf<uint32_t> gp, gci;
     DevBuf<uint16_t> gcd;     // packed gci (empty: not packable)
+    // R without the columns of the last colour, whose residual is zero after
+    // a forward sweep of weight 1 (rows 0 when that shortcut does not apply)
+    CsrMat Rz;
     DevBuf<uint32_t> gcbase;
     DevBuf<double> gv, gd;
     DevBuf<float> gv32, gd32;
@@ -84,6 +87,8 @@ void transpose_device(const CsrMat& P, CsrMat& R, hipStream_t s);
 // SGS smoother data of level D (colouring, diagonal, Jacobi weight and, with
 // at most kMaxGsColours colours, the colour-grouped copy of D.A)
 void sgs_setup_device(AmgLevelDev& D, hipStream_t s);
+// out = R without the columns rows[0..count) (entries in stored order)
+void drop_columns_device(const CsrMat& R, const uint32_t* rows, uint32_t count, CsrMat& out, hipStream_t s);
 // the float32 sweep blocks (gblk32, h_cblk32) of a level set up by sgs_setup_device
 void sgs_blocks32(AmgLevelDev& D, hipStream_t s);
 // true if every row of A lists strictly increasing columns

@@ -728,6 +728,56 @@ void sgs_setup_device(AmgLevelDev& D, hipStream_t s) {
     FEM_HIP(hipStreamSynchronize(s));  // the host arrays end with this call
 }
 
+namespace {
+__global__ void k_flag_rows(const uint32_t* rows, uint32_t count, uint8_t* flag) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) flag[rows[t]] = 1;
+}
+
+__global__ void k_keep_len(const uint32_t* rp, const uint32_t* ci, const uint8_t* drop, uint32_t n, uint32_t* len) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        uint32_t c = 0;
+        for (uint32_t p = rp[t]; p < rp[t + 1]; ++p) c += drop[ci[p]] ? 0u : 1u;
+        len[t] = c;
+    }
+    if (t == n) len[t] = 0;
+}
+
+__global__ void k_keep_fill(const uint32_t* rp, const uint32_t* ci, const double* v, const uint8_t* drop, uint32_t n,
+                            const uint32_t* op, uint32_t* oci, double* ov) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    uint32_t q = op[t];
+    for (uint32_t p = rp[t]; p < rp[t + 1]; ++p)
+        if (!drop[ci[p]]) {
+            oci[q] = ci[p];
+            ov[q] = v[p];
+            ++q;
+        }
+}
+}  // namespace
+
+void drop_columns_device(const CsrMat& R, const uint32_t* rows, uint32_t count, CsrMat& out, hipStream_t s) {
+    DevBuf<uint8_t> drop(std::max<uint32_t>(R.m, 1));
+    FEM_HIP(hipMemsetAsync(drop.get(), 0, std::max<uint32_t>(R.m, 1), s));
+    if (count) hipLaunchKernelGGL(k_flag_rows, dim3(g1(count)), dim3(256), 0, s, rows, count, drop.get());
+    DCsr O;
+    O.n = R.n;
+    O.m = R.m;
+    DevBuf<uint32_t> len(R.n + 1);
+    hipLaunchKernelGGL(k_keep_len, dim3(g1(R.n + 1)), dim3(256), 0, s, R.rp.get(), R.ci.get(), drop.get(), R.n,
+                       len.get());
+    O.rp.alloc(R.n + 1);
+    scan_u32(len.get(), O.rp.get(), (size_t)R.n + 1, s);
+    O.nnz = read_u32(O.rp.get() + R.n, s);
+    O.alloc_entries(s);
+    hipLaunchKernelGGL(k_keep_fill, dim3(g1(R.n)), dim3(256), 0, s, R.rp.get(), R.ci.get(), R.v.get(), drop.get(),
+                       R.n, O.rp.get(), O.ci.get(), O.v.get());
+    FEM_HIP(hipGetLastError());
+    O.to_mat(out, s);  // synchronises s
+}
+
 void sgs_blocks32(AmgLevelDev& D, hipStream_t s) {
     if (!D.gp.get() || D.h_cptr.size() < 2) return;
     const uint32_t n = D.n;

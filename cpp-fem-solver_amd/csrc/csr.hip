@@ -86,13 +86,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
     }
 }
 
-// cbase[c] = least column of chunk c (256 entries, one block each); *bad |= 1
-// when the chunk's columns span 65536 or more
-__global__ __launch_bounds__(256) void k_pack_base(const uint32_t* __restrict__ ci, uint64_t nnz,
+// cbase[c] = least column of chunk c (64 entries, one wavefront each);
+// *bad |= 1 when the chunk's columns span 65536 or more
+__global__ __launch_bounds__(256) void k_pack_base(const uint32_t* __restrict__ ci, uint64_t nnz, uint64_t nch,
                                                    uint32_t* __restrict__ cbase, int* __restrict__ bad) {
-    __shared__ uint32_t smin[4], smax[4];
-    const uint64_t k0 = (uint64_t)blockIdx.x << kPackShift;
-    const uint64_t k = k0 + threadIdx.x;
+    const uint64_t c = (uint64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (c >= nch) return;
+    const uint64_t k0 = c << kPackShift;
+    const uint64_t k = k0 + (threadIdx.x & 63);
     uint32_t mn = ci[k < nnz ? k : k0], mx = mn;
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -100,16 +101,7 @@ __global__ __launch_bounds__(256) void k_pack_base(const uint32_t* __restrict__ 
         mx = max(mx, (uint32_t)__shfl_xor(mx, off));
     }
     if ((threadIdx.x & 63) == 0) {
-        smin[threadIdx.x >> 6] = mn;
-        smax[threadIdx.x >> 6] = mx;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int w = 1; w < 4; ++w) {
-            mn = min(mn, smin[w]);
-            mx = max(mx, smax[w]);
-        }
-        cbase[blockIdx.x] = mn;
+        cbase[c] = mn;
         if (mx - mn > 65535u) atomicOr(bad, 1);
     }
 }
@@ -174,7 +166,7 @@ bool pack_columns(const uint32_t* ci, uint64_t nnz, DevBuf<uint16_t>& cd, DevBuf
     cbase.alloc(nch);
     DevBuf<int> bad(1);
     FEM_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
-    hipLaunchKernelGGL(k_pack_base, dim3((unsigned)nch), dim3(1u << kPackShift), 0, s, ci, nnz, cbase.get(),
+    hipLaunchKernelGGL(k_pack_base, dim3((unsigned)((nch + 3) / 4)), dim3(256), 0, s, ci, nnz, nch, cbase.get(),
                        bad.get());
     FEM_HIP(hipGetLastError());
     int h = 0;

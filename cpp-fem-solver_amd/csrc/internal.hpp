@@ -81,10 +81,10 @@ struct SpmvPlan {
 
 // Packed column indices: col[k] = cbase[k >> kPackShift] + cd[k], 2 bytes per
 // entry instead of 4, for matrices whose columns span less than 65536 inside
-// every chunk of 256 consecutive entries (banded FEM orderings: the
+// every chunk of 64 consecutive entries (banded FEM orderings: the
 // z-neighbours of a 128^3 grid sit 16384 rows away). The CSR-stream kernels
 // read cd and cbase instead of ci; products and summation order are unchanged.
-constexpr int kPackShift = 8;
+constexpr int kPackShift = 6;
 constexpr uint64_t kPackMinNnz = 1u << 16;  // smaller matrices keep ci only
 // builds cd (nnz + 4 entries, zero padded) and cbase from the device array ci;
 // false (both released) when a chunk spans 65536 columns or more or nnz is small
@@ -136,7 +136,7 @@ struct CsrMat : LinOp {
     uint64_t nnz = 0;
     DevBuf<uint32_t> rp, ci;
     DevBuf<uint16_t> cd;     // packed columns (empty: not packable)
-    DevBuf<uint32_t> cbase;  // per chunk of 256 entries
+    DevBuf<uint32_t> cbase;  // per chunk of 64 entries
     DevBuf<double> v;
     DevBuf<float> v32;
     SpmvPlan plan, plan32;  // blockings of the float64 and float32 SpMV

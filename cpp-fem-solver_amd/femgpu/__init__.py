@@ -273,7 +273,7 @@ class CsrMatrix:
 
     def pack_columns(self, on=-1) -> bool:
         """16-bit packed column offsets for the CSR-stream SpMV: on = 1 builds
-        them (where every 256-entry chunk spans < 65536 columns), 0 drops
+        them (where every 64-entry chunk spans < 65536 columns), 0 drops
         them, -1 only asks. Returns whether the packed form is in use."""
         p = C.c_int()
         _check(lib().fem_csr_pack_columns(self._h, int(on), C.byref(p)))

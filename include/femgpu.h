@@ -169,7 +169,7 @@ int fem_csr_destroy(fem_csr A);
 int fem_csr_set_row_lanes(fem_csr A, int lanes);
 /* Column-index storage of A's CSR-stream SpMV (no counterpart in the
  * reference): with on = 1 the columns are also kept as 16-bit offsets from a
- * base per 256 entries, read instead of the 32-bit indices (2 bytes less per
+ * base per 64 entries, read instead of the 32-bit indices (2 bytes less per
  * nonzero), when every such chunk spans fewer than 65536 columns and nnz >=
  * 65536; on = 0 drops that copy; on < 0 changes nothing. *packed (may be
  * NULL) receives 1 if the packed form is in use. Results are identical either

@@ -73,7 +73,7 @@ def _spmv32(A, x):
 
 @pytest.mark.parametrize("case", ["poisson3d_48", "long_rows", "wide_columns"])
 def test_spmv_packed_columns(case):
-    """16-bit packed column offsets: used where every 256-entry chunk spans
+    """16-bit packed column offsets: used where every 64-entry chunk spans
     < 65536 columns; float64 results stay bit-exact with the oracle and the
     float32 ones identical to the unpacked kernel's."""
     rng = np.random.default_rng(17)
