@@ -8,7 +8,8 @@ CSR, stopping test ||r|| / ||b|| < 1e-10.
 A step is one complete solve of A x = b with the preconditioner already
 built, b and x resident in device memory. The headline method is PCG with
 the smoothed-aggregation AMG V-cycle (2 pre / 2 post symmetric multicolour
-Gauss-Seidel sweeps, weight 1.0) in float32 inside float64 iterative
+Gauss-Seidel sweeps, weight 1.0, coarsening down to at most 300 rows, whose
+dense exact solve ends the V-cycle) in float32 inside float64 iterative
 refinement; plain float64 IC(0)-PCG, float64 SA-AMG-PCG and float64
 Jacobi-PCG are timed beside it (one step each). Every result is checked:
 true float64 relative residual and relative error against x_true.
@@ -288,7 +289,7 @@ def main():
     log(f"[bench] spmv fp64 {spmv_ms * 1e3:.1f} us {spmv_gbs:.0f} GB/s (cold {spmv_cold_ms * 1e3:.1f} us "
         f"{spmv_cold_gbs:.0f} GB/s); fp32 {spmv32_ms * 1e3:.1f} us {spmv32_gbs:.0f} GB/s")
 
-    sa = dict(smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0, coarse_exact=1)
+    sa = dict(smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0, coarse_exact=1, coarse_size=300)
     methods = {
         "saamg_mp": ("saamg", sa, F.FP32_IR),
         "saamg_fp64": ("saamg", sa, F.FP64),
@@ -381,7 +382,7 @@ def main():
         "dtype": "f32-inner/f64-refinement",
         "data": "synthetic (x_true ~ U[-1,1], PCG64 seed 0; b = A x_true)",
         "config": {"workload": f"3-D 7-point Poisson {N}^3 (n={G.n}, nnz={G.nnz}), PCG + SA-AMG "
-                               "(SGS 2/2, w=1.0) FP32 inner / FP64 refinement, tol 1e-10",
+                               "(SGS 2/2, w=1.0, coarsest <= 300 rows) FP32 inner / FP64 refinement, tol 1e-10",
                    "methods": results,
                    "spmv_fp32_us": round(spmv32_ms * 1e3, 2), "spmv_fp32_gbs": round(spmv32_gbs, 1),
                    "device": dev_name},

@@ -120,15 +120,38 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
     const uint32_t k0 = gp[r0], k1 = gp[r1];
     const uint32_t cnt = k1 - k0;
     if (cnt <= (uint32_t)STAGE) {
+        // a block holds at most stage_rows<T>() rows: RPT per thread, whose
+        // bounds and operands are loaded before the staging
+        constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
+        uint32_t rb[RPT], re[RPT], ri[RPT];
+        T fi[RPT], di[RPT], ui[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + t + (uint32_t)j * kBlock;
+            rb[j] = re[j] = ri[j] = 0;
+            fi[j] = di[j] = ui[j] = (T)0;
+            if (row < r1) {
+                rb[j] = gp[row] - k0;
+                re[j] = gp[row + 1] - k0;
+                ri[j] = rows[row];
+                fi[j] = f[ri[j]];
+                if (MODE == 0) {
+                    di[j] = gd[row];
+                    ui[j] = u[ri[j]];
+                }
+            }
+        }
         if (cnt > 0) stage_products<T, STAGE, PK>(gci, gcd, gcbase, gv, u, k0, cnt, prod);
         __syncthreads();
-        for (uint32_t row = r0 + t; row < r1; row += kBlock) {
-            const uint32_t b = gp[row] - k0, e = gp[row + 1] - k0;
-            T s = (T)0;
-            for (uint32_t k = b; k < e; ++k) s += prod[k];
-            const uint32_t i = rows[row];
-            if (MODE == 0) u[i] = u[i] + (w * (f[i] - s)) / gd[row];
-            else res[i] = f[i] - s;
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + t + (uint32_t)j * kBlock;
+            if (row < r1) {
+                T s = (T)0;
+                for (uint32_t k = rb[j]; k < re[j]; ++k) s += prod[k];
+                if (MODE == 0) u[ri[j]] = ui[j] + (w * (fi[j] - s)) / di[j];
+                else res[ri[j]] = fi[j] - s;
+            }
         }
     } else {
         double s = 0.0;
@@ -156,7 +179,9 @@ __global__ void k_gs_first(const uint32_t* __restrict__ rows, uint32_t n, uint32
     u[i] = t < c0_end ? (w * f[i]) / gd[t] : (T)0;
 }
 
-// u = C f with C dense (row-major, float64); f, u of type T
+// u = C f with C dense (row-major, float64); f, u of type T. One wavefront
+// per row: coalesced reads of the row, lane-strided partial sums in float64,
+// butterfly total.
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse(const double* __restrict__ C, uint32_t nc, const T* __restrict__ f,
                                                 T* __restrict__ u, Gate g) {
@@ -164,12 +189,14 @@ __global__ __launch_bounds__(256) void k_coarse(const double* __restrict__ C, ui
     extern __shared__ double fs[];
     for (uint32_t j = threadIdx.x; j < nc; j += blockDim.x) fs[j] = (double)f[j];
     __syncthreads();
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        const double* row = C + (size_t)i * nc;
-        for (uint32_t j = 0; j < nc; ++j) s += row[j] * fs[j];
-        u[i] = (T)s;
-    }
+    const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (i >= nc) return;
+    const double* row = C + (size_t)i * nc;
+    double s = 0.0;
+    for (uint32_t j = lane; j < nc; j += 64) s += row[j] * fs[j];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if (lane == 0) u[i] = (T)s;
 }
 
 template <typename T>
@@ -460,8 +487,8 @@ struct PcAmg : Precond {
             Bufs<T> BC = bufs<T>(C);
             T* u = (L == 1) ? z : BC.u;
             const size_t sh = (size_t)nco * sizeof(double);
-            hipLaunchKernelGGL((k_coarse<T>), dim3((nco + 255) / 256), dim3(256), sh, s, coarse.get(), nco, fk[L - 1],
-                               u, g);
+            hipLaunchKernelGGL((k_coarse<T>), dim3((nco + 3) / 4), dim3(256), sh, s, coarse.get(), nco, fk[L - 1], u,
+                               g);
             ucur[L - 1] = u;
         }
         for (int64_t k = (int64_t)L - 2; k >= 0; --k) {

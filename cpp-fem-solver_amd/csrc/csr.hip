@@ -54,6 +54,25 @@ __device__ __forceinline__ T spmv_epilogue(uint32_t row, T s, const SpmvArgs<T>&
     else return a.u[row] + (T)a.w * (a.d[row] * (a.f[row] - s));
 }
 
+// Epilogue operands of one row, loaded before the block stages its products
+// (their latency overlaps the staging instead of following the barrier).
+template <typename T, int EPI>
+struct EpiIn {
+    T f = (T)0, u = (T)0, d = (T)0;
+    __device__ __forceinline__ void load(uint32_t row, const SpmvArgs<T>& a) {
+        if constexpr (EPI == EPI_RESID || EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV) f = a.f[row];
+        if constexpr (EPI == EPI_ADD || EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV) u = a.u[row];
+        if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV) d = a.d[row];
+    }
+    __device__ __forceinline__ T apply(T s, const SpmvArgs<T>& a) const {
+        if constexpr (EPI == EPI_STORE) return s;
+        else if constexpr (EPI == EPI_RESID) return f - s;
+        else if constexpr (EPI == EPI_ADD) return u + s;
+        else if constexpr (EPI == EPI_JACOBI) return u + ((T)a.w * (f - s)) / d;
+        else return u + (T)a.w * (d * (f - s));
+    }
+};
+
 template <typename T, int EPI, bool PK>
 __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
                                                  const uint16_t* __restrict__ cd,
@@ -69,13 +88,30 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
     const uint32_t cnt = k1 - k0;
     const uint32_t t = threadIdx.x;
     if (cnt <= (uint32_t)STAGE) {
+        // a block holds at most stage_rows<T>() rows: RPT per thread
+        constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
+        uint32_t rb[RPT], re[RPT];
+        EpiIn<T, EPI> ep[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + t + (uint32_t)j * kBlock;
+            rb[j] = re[j] = 0;
+            if (row < r1) {
+                rb[j] = rp[row] - k0;
+                re[j] = rp[row + 1] - k0;
+                ep[j].load(row, a);
+            }
+        }
         if (cnt > 0) stage_products<T, STAGE, PK>(ci, cd, cbase, v, x, k0, cnt, prod);
         __syncthreads();
-        for (uint32_t row = r0 + t; row < r1; row += kBlock) {
-            const uint32_t b = rp[row] - k0, e = rp[row + 1] - k0;
-            T s = (T)0;
-            for (uint32_t k = b; k < e; ++k) s += prod[k];
-            y[row] = spmv_epilogue<T, EPI>(row, s, a);
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + t + (uint32_t)j * kBlock;
+            if (row < r1) {
+                T s = (T)0;
+                for (uint32_t k = rb[j]; k < re[j]; ++k) s += prod[k];
+                y[row] = ep[j].apply(s, a);
+            }
         }
     } else {
         double s = 0.0;
