@@ -41,6 +41,12 @@ def spmv_bytes(nnz, n, m, s=8):
     return nnz * (s + 4) + 4 * (n + 1) + m * s + n * s
 
 
+def spmv_bytes_packed(nnz, n, m, s=8):
+    """Bytes the CSR-stream kernel moves with 16-bit packed column offsets
+    (2 bytes per entry plus a 4-byte base per 64 entries)."""
+    return nnz * (s + 2) + 4 * ((nnz + 63) // 64) + 4 * (n + 1) + m * s + n * s
+
+
 def measured_traffic():
     """HBM bytes per float64 SpMV launch as read by rocprofv3 --pmc
     (FETCH_SIZE + WRITE_SIZE, calibrated on a plain copy), from the committed
@@ -282,6 +288,8 @@ def main():
     # matrix is 183 MB)
     spmv_cold_ms = A.bench_spmv(fp32=False, reps=20, cold=True)
     B64 = spmv_bytes(G.nnz, G.n, G.m, 8)
+    packed = A.pack_columns()
+    B64_moved = spmv_bytes_packed(G.nnz, G.n, G.m, 8) if packed else B64
     B32 = spmv_bytes(G.nnz, G.n, G.m, 4)
     spmv_gbs = B64 / (spmv_ms * 1e-3) / 1e9
     spmv32_gbs = B32 / (spmv32_ms * 1e-3) / 1e9
@@ -386,10 +394,14 @@ def main():
                    "methods": results,
                    "spmv_fp32_us": round(spmv32_ms * 1e3, 2), "spmv_fp32_gbs": round(spmv32_gbs, 1),
                    "device": dev_name},
-        "roofline": {"bound": "hbm", "kernel": "k_spmv<double,0> (CSR-stream SpMV, float64)",
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_spmv<double,0,%s> (CSR-stream SpMV, float64%s)"
+                               % ("true" if packed else "false", ", 16-bit packed column offsets" if packed else ""),
                      "achieved": round(spmv_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(spmv_gbs / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                      "bytes_per_launch": B64, "avg_launch_us": round(spmv_ms * 1e3, 2),
+                     "bytes_moved_per_launch": B64_moved,
+                     "achieved_moved": round(B64_moved / (spmv_ms * 1e-3) / 1e9, 1),
                      "timing": "200 back-to-back launches, HIP events on the library's stream",
                      "achieved_cold": round(spmv_cold_gbs, 1), "frac_cold": round(spmv_cold_gbs / HBM_PEAK_GBS, 4),
                      "avg_launch_us_cold": round(spmv_cold_ms * 1e3, 2)},
