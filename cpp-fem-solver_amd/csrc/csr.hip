@@ -73,12 +73,15 @@ struct EpiIn {
     }
 };
 
-template <typename T, int EPI, bool PK>
+// DOT: the block also writes part[bid] = sum over its rows of x[i] * y[i]
+// (float64; square matrices)
+template <typename T, int EPI, bool PK, bool DOT = false>
 __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
                                                  const uint16_t* __restrict__ cd,
                                                  const uint32_t* __restrict__ cbase, const T* __restrict__ v,
                                                  const T* __restrict__ x, T* __restrict__ y,
-                                                 const uint32_t* __restrict__ blk_row, SpmvArgs<T> a, Gate g) {
+                                                 const uint32_t* __restrict__ blk_row, SpmvArgs<T> a, Gate g,
+                                                 double* __restrict__ part = nullptr) {
     if (g.closed()) return;
     constexpr int STAGE = stage_nnz<T>();
     __shared__ T prod[STAGE];
@@ -92,33 +95,47 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
         constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
         uint32_t rb[RPT], re[RPT];
         EpiIn<T, EPI> ep[RPT];
+        T xr[RPT];
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const uint32_t row = r0 + t + (uint32_t)j * kBlock;
             rb[j] = re[j] = 0;
+            xr[j] = (T)0;
             if (row < r1) {
                 rb[j] = rp[row] - k0;
                 re[j] = rp[row + 1] - k0;
                 ep[j].load(row, a);
+                if (DOT) xr[j] = x[row];
             }
         }
         if (cnt > 0) stage_products<T, STAGE, PK>(ci, cd, cbase, v, x, k0, cnt, prod);
         __syncthreads();
+        double dsum[1] = {0.0};
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const uint32_t row = r0 + t + (uint32_t)j * kBlock;
             if (row < r1) {
                 T s = (T)0;
                 for (uint32_t k = rb[j]; k < re[j]; ++k) s += prod[k];
-                y[row] = ep[j].apply(s, a);
+                const T yv = ep[j].apply(s, a);
+                y[row] = yv;
+                if (DOT) dsum[0] += (double)xr[j] * (double)yv;
             }
+        }
+        if constexpr (DOT) {
+            block_sum<1>(dsum);
+            if (t == 0) part[bid] = dsum[0];
         }
     } else {
         double s = 0.0;
         for (uint32_t k = k0 + t; k < k1; k += kBlock) s += (double)(v[k] * x[col_of<PK>(ci, cd, cbase, k)]);
         double tot[1] = {s};
         block_sum<1>(tot);
-        if (t == 0) y[r0] = spmv_epilogue<T, EPI>(r0, (T)tot[0], a);
+        if (t == 0) {
+            const T yv = spmv_epilogue<T, EPI>(r0, (T)tot[0], a);
+            y[r0] = yv;
+            if (DOT) part[bid] = (double)x[r0] * (double)yv;
+        }
     }
 }
 
@@ -193,6 +210,35 @@ void spmv_stream_launch(const CsrMat& A, const T* vals, const T* x, T* y, const 
         hipLaunchKernelGGL((k_spmv<T, EPI, false>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
                            A.cbase.get(), vals, x, y, plan.blk_row.get(), a, g);
 }
+
+template <typename T>
+unsigned csr_spmv_dot(const CsrMat& A, const T* x, T* y, double* part, hipStream_t s, Gate g) {
+    if (A.lanes > 0 || A.n != A.m || A.n == 0) return 0;
+    const T* vals;
+    if constexpr (std::is_same<T, double>::value) vals = A.v.get();
+    else vals = A.v32.get();
+    if (!vals) fail(FEM_ERR_RUNTIME, "float32 copy of the matrix was not built");
+    const SpmvPlan& plan = std::is_same<T, double>::value ? A.plan : A.plan32;
+    const dim3 grid(plan.nblk), block(kBlock);
+    if (A.cd.get())
+        hipLaunchKernelGGL((k_spmv<T, EPI_STORE, true, true>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
+                           A.cbase.get(), vals, x, y, plan.blk_row.get(), SpmvArgs<T>{}, g, part);
+    else
+        hipLaunchKernelGGL((k_spmv<T, EPI_STORE, false, true>), grid, block, 0, s, A.rp.get(), A.ci.get(),
+                           A.cd.get(), A.cbase.get(), vals, x, y, plan.blk_row.get(), SpmvArgs<T>{}, g, part);
+    FEM_HIP(hipGetLastError());
+    return plan.nblk;
+}
+template unsigned csr_spmv_dot<double>(const CsrMat&, const double*, double*, double*, hipStream_t, Gate);
+template unsigned csr_spmv_dot<float>(const CsrMat&, const float*, float*, double*, hipStream_t, Gate);
+
+unsigned CsrMat::apply_dot(const double* x, double* y, double* part, hipStream_t s, Gate g) const {
+    return csr_spmv_dot<double>(*this, x, y, part, s, g);
+}
+unsigned CsrMat::apply32_dot(const float* x, float* y, double* part, hipStream_t s, Gate g) const {
+    return csr_spmv_dot<float>(*this, x, y, part, s, g);
+}
+unsigned CsrMat::dot_parts() const { return std::max(plan.nblk, plan32.nblk); }
 
 bool pack_columns(const uint32_t* ci, uint64_t nnz, DevBuf<uint16_t>& cd, DevBuf<uint32_t>& cbase, hipStream_t s) {
     cd.release();

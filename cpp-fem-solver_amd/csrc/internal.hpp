@@ -45,6 +45,13 @@ struct LinOp {
     virtual void apply32(const float* x, float* y, hipStream_t s, Gate g = {}) const = 0;
     virtual void ensure_fp32(hipStream_t s) = 0;
     virtual uint64_t bytes_per_apply(bool fp32) const = 0;
+    // y = A x and, per workgroup of the product, the float64 partial sum of
+    // x[i] * y[i] into part[0..count); returns count, or 0 (nothing launched)
+    // when this operator has no fused form
+    virtual unsigned apply_dot(const double*, double*, double*, hipStream_t, Gate = {}) const { return 0; }
+    virtual unsigned apply32_dot(const float*, float*, double*, hipStream_t, Gate = {}) const { return 0; }
+    // most partial sums an apply_dot / apply32_dot call writes
+    virtual unsigned dot_parts() const { return 0; }
 };
 
 // Workgroups are dispatched to the 8 XCDs round robin (workgroup b runs on
@@ -156,6 +163,9 @@ struct CsrMat : LinOp {
     void apply(const double* x, double* y, hipStream_t s, Gate g = {}) const override;
     void apply32(const float* x, float* y, hipStream_t s, Gate g = {}) const override;
     void ensure_fp32(hipStream_t s) override;
+    unsigned apply_dot(const double* x, double* y, double* part, hipStream_t s, Gate g = {}) const override;
+    unsigned apply32_dot(const float* x, float* y, double* part, hipStream_t s, Gate g = {}) const override;
+    unsigned dot_parts() const override;
     uint64_t bytes_per_apply(bool fp32) const override {
         const uint64_t sv = fp32 ? 4 : 8;
         return nnz * (sv + 4) + 4ull * (n + 1) + (uint64_t)m * sv + (uint64_t)n * sv;
@@ -196,6 +206,10 @@ struct SpmvArgs {
 // Launch CSR SpMV with an epilogue (T = double or float values/vectors).
 template <typename T>
 void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, hipStream_t s, Gate g);
+// y = A x with the per-workgroup partial sums of x[i] * y[i] (square A, the
+// CSR-stream kernel); returns their count, 0 when A takes another kernel
+template <typename T>
+unsigned csr_spmv_dot(const CsrMat& A, const T* x, T* y, double* part, hipStream_t s, Gate g);
 
 // ---------------------------------------------------------------------------
 // Preconditioners
@@ -218,6 +232,11 @@ struct Precond {
     // (built on first use).
     virtual int budget() const { return 0; }
     virtual Precond* fallback(hipStream_t) { return nullptr; }
+    // z = M^{-1} r with the per-workgroup partial sums of r[i] * z[i], as
+    // LinOp::apply_dot (0: no fused form)
+    virtual unsigned apply_dot(const double*, double*, double*, hipStream_t, Gate = {}) { return 0; }
+    virtual unsigned apply32_dot(const float*, float*, double*, hipStream_t, Gate = {}) { return 0; }
+    virtual unsigned dot_parts() const { return 0; }
     double setup_ms = 0.0;
     int levels = 0;
     std::vector<uint32_t> lvl_rows;

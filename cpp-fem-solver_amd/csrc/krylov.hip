@@ -645,6 +645,7 @@ KrylovSolver::KrylovSolver(LinOp* A, Precond* M, const fem_solve_opts& o, hipStr
         if (M) M_->ensure_fp32(s);
         pc_mode32_ = mode(true);
     }
+    dotp_.alloc(std::max<unsigned>({1u, A_->dot_parts(), M_ ? M_->dot_parts() : 0u}));
 }
 
 KrylovSolver::~KrylovSolver() {}
@@ -658,6 +659,16 @@ template <typename T>
 void KrylovSolver::matvec(const T* x, T* y, Gate g) {
     if constexpr (std::is_same<T, double>::value) A_->apply(x, y, s_, g);
     else A_->apply32(x, y, s_, g);
+}
+template <typename T>
+unsigned KrylovSolver::matvec_dot(const T* x, T* y, Gate g) {
+    if constexpr (std::is_same<T, double>::value) return A_->apply_dot(x, y, dotp_.get(), s_, g);
+    else return A_->apply32_dot(x, y, dotp_.get(), s_, g);
+}
+template <typename T>
+unsigned KrylovSolver::precond_dot(const T* r, T* z, Gate g) {
+    if constexpr (std::is_same<T, double>::value) return M_->apply_dot(r, z, dotp_.get(), s_, g);
+    else return M_->apply32_dot(r, z, dotp_.get(), s_, g);
 }
 
 #define FIN(nv, ...)                                          \
@@ -710,15 +721,26 @@ void KrylovSolver::pcg_iter(KWork<T>& W) {
         else d = M_->diag32();
     }
     Gate g{&st->done};
-    matvec<T>(W.p.get(), W.q.get(), g);
-    LAUNCH((k_dot2<T>), G, W.p.get(), W.q.get(), n, st, part);
-    FIN(1, FinPcgPq{st});
+    // q = A p and p.q: one fused kernel where the operator has one
+    if (const unsigned np = matvec_dot<T>(W.p.get(), W.q.get(), g)) {
+        launch_finalize<1>(dotp_.get(), np, FinPcgPq{st}, s_);
+        FEM_HIP(hipGetLastError());
+    } else {
+        matvec<T>(W.p.get(), W.q.get(), g);
+        LAUNCH((k_dot2<T>), G, W.p.get(), W.q.get(), n, st, part);
+        FIN(1, FinPcgPq{st});
+    }
     if (pm == PCM_GEN) {
         LAUNCH((k_pcg_xr<T, PCM_GEN>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part);
         FIN(2, FinPcgXr<PCM_GEN>{st});
-        precond<T>(W.r.get(), W.z.get(), g);
-        LAUNCH((k_dot2<T>), G, W.r.get(), W.z.get(), n, st, part);
-        FIN(1, FinPcgRz{st});
+        if (const unsigned np = precond_dot<T>(W.r.get(), W.z.get(), g)) {
+            launch_finalize<1>(dotp_.get(), np, FinPcgRz{st}, s_);
+            FEM_HIP(hipGetLastError());
+        } else {
+            precond<T>(W.r.get(), W.z.get(), g);
+            LAUNCH((k_dot2<T>), G, W.r.get(), W.z.get(), n, st, part);
+            FIN(1, FinPcgRz{st});
+        }
         LAUNCH((k_pcg_p<T, PCM_GEN>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
     } else if (pm == PCM_DIAG) {
         LAUNCH((k_pcg_xr<T, PCM_DIAG>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part);

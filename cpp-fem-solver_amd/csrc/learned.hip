@@ -597,6 +597,13 @@ struct PcLearned : Precond {
     void apply32(const float* r, float* z, hipStream_t s, Gate g) override {
         csr_spmv<float>(S.M, r, z, EPI_STORE, {}, s, g);
     }
+    unsigned apply_dot(const double* r, double* z, double* part, hipStream_t s, Gate g) override {
+        return csr_spmv_dot<double>(S.M, r, z, part, s, g);
+    }
+    unsigned apply32_dot(const float* r, float* z, double* part, hipStream_t s, Gate g) override {
+        return csr_spmv_dot<float>(S.M, r, z, part, s, g);
+    }
+    unsigned dot_parts() const override { return S.M.dot_parts(); }
     bool has_fp32() const override { return fp32; }
     void ensure_fp32(hipStream_t s) override {
         if (fp32) return;

@@ -63,6 +63,11 @@ class KrylovSolver {
     void precond(const T* r, T* z, Gate g);
     template <typename T>
     void matvec(const T* x, T* y, Gate g);
+    // fused forms with the partial sums of x[i] * y[i] in dotp_ (0: none)
+    template <typename T>
+    unsigned precond_dot(const T* r, T* z, Gate g);
+    template <typename T>
+    unsigned matvec_dot(const T* x, T* y, Gate g);
     void replay(GraphCache& gc, int tag, const std::function<void()>& body);
     void read_state();
     double true_residual(const double* b, const double* x, double* work);
@@ -79,6 +84,7 @@ class KrylovSolver {
     GraphSet g64_, g32_;
     DevBuf<double> st_work_, st_zero_, st_out_;   // end-of-solve statistics
     DevBuf<double> tr_out_, nrm_out_;             // single scalars read back by the host
+    DevBuf<double> dotp_;                         // partial sums of the fused product + dot kernels
     std::unique_ptr<KrylovSolver> fb_;            // solver over the fallback preconditioner
     int pc_mode64_ = 0, pc_mode32_ = 0;  // 0 general, 1 none, 2 diag
 };
