@@ -42,7 +42,12 @@ struct BSys {
     uint64_t vec;     // offset into vectors (scalars * NR)
     int cap;          // > 0: at most this many inner iterations (autotuner trials)
     int nr;           // vector layout: 4 interleaved right-hand-side slots, or 1
+    int maxrow;       // most blocks in a block row
 };
+
+// scalar systems whose rows hold at most this many entries take the product
+// path that issues all of a step's loads before any use
+constexpr int kShortRow = 5;
 
 struct BOut {
     int iterations;   // inner iterations (all right-hand sides advance together)
@@ -149,7 +154,7 @@ struct SysShared {
 template <int BLOCK, int BS, int NRT>
 __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShared& sh, BOut* o) {
     constexpr int BV = BS * BS;
-    constexpr int U = BS == 1 ? 4 : 1;  // rows per thread per step
+    constexpr int U = BS == 1 ? (NRT == 1 ? 4 : 2) : 1;  // rows per thread per step
     const uint32_t nb = S.nb;
     const uint32_t* rp = A.rp + S.rp;
     const uint32_t* ci = A.ci + S.ci;
@@ -281,6 +286,35 @@ __device__ void solve_system(const BArgs& A, const BSys& S, double* red, SysShar
                 for (int u = 0; u < U; ++u)
                     for (int a = 0; a < BS; ++a)
                         for (int c = 0; c < NRT; ++c) acc[u][a][c] = 0.f;
+                bool short_rows = false;
+                if constexpr (BS == 1 && NRT == 1) short_rows = S.maxrow <= kShortRow;
+                if (short_rows) {
+                    // three dependent rounds per step (row pointers, then every
+                    // entry and column, then every gather) instead of two per
+                    // entry; an index past a row's end re-reads its first entry
+                    // (rows past nb read entry 0) and its term is not added, so
+                    // the sums are the generic loop's
+                    uint32_t J[U][kShortRow];
+                    float vv[U][kShortRow], pj[U][kShortRow];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int t = 0; t < kShortRow; ++t) {
+                            const uint32_t k = k0[u] + ((uint32_t)t < k1[u] - k0[u] ? (uint32_t)t : 0u);
+                            J[u][t] = ci[k];
+                            vv[u][t] = v32[k];
+                        }
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int t = 0; t < kShortRow; ++t) pj[u][t] = p[J[u][t]];
+#pragma unroll
+                    for (int u = 0; u < U; ++u)
+#pragma unroll
+                        for (int t = 0; t < kShortRow; ++t)
+                            if ((uint32_t)t < k1[u] - k0[u]) acc[u][0][0] += vv[u][t] * pj[u][t];
+                    len = 0;
+                }
                 for (uint32_t t = 0; t < len; ++t) {
 #pragma unroll
                     for (int u = 0; u < U; ++u) {
@@ -676,6 +710,8 @@ int fem_batch_create(int nsys, const fem_batch_system* systems, int nrhs, fem_ba
             v64.insert(v64.end(), in.values, in.values + nnzb * BV);
             dv.insert(dv.end(), inv.begin(), inv.end());
             vec += (uint64_t)nb * bs * NR;
+            for (uint32_t I = 0; I < nb; ++I)
+                S.maxrow = std::max(S.maxrow, (int)(in.row_ptr[I + 1] - in.row_ptr[I]));
             h->h_sys.push_back(S);
             h->nnzb.push_back(nnzb);
             // work estimate: matrix entries x sqrt(rows) (CG iterations grow
