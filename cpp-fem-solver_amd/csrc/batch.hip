@@ -21,6 +21,8 @@
 //   A: q = A p, partial p.q;  B: d += a p, r -= a q, partial r.r, r.z;
 //   C: p = z + beta p (z = D^{-1} r recomputed).
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cmath>
 #include <numeric>
 
@@ -55,6 +57,7 @@ struct BOut {
     int converged;    // number of converged right-hand sides
     int pad;
     double relres[NR];
+    long long t0, t1;  // device wall clock when the item was taken and finished
 };
 
 struct BArgs {
@@ -439,12 +442,17 @@ __global__ __launch_bounds__(BLOCK) void k_batch_pcg(BArgs a) {
         if (k >= a.nsys) break;
         const int s = a.order[k];
         const BSys S = a.sys[s];
+        const long long t0 = wall_clock64();
         if (S.bs == 3) {
             if (S.nr == 1) solve_system<BLOCK, 3, 1>(a, S, red, sh, a.out + s);
             else solve_system<BLOCK, 3, NR>(a, S, red, sh, a.out + s);
         } else {
             if (S.nr == 1) solve_system<BLOCK, 1, 1>(a, S, red, sh, a.out + s);
             else solve_system<BLOCK, 1, NR>(a, S, red, sh, a.out + s);
+        }
+        if (threadIdx.x == 0) {
+            a.out[s].t0 = t0;
+            a.out[s].t1 = wall_clock64();
         }
     }
 }
@@ -485,6 +493,10 @@ struct fem_batch_s {
     int nsys = 0;
     std::vector<BSys> h_sys;            // the systems (all right-hand sides interleaved)
     std::vector<double> work;           // work estimate per system (static, then from the profile)
+    // measured device time of a system's item (unsplit) and of its
+    // right-hand-side items (split) under the tuned schedule; 0 = not
+    // measured. They order the queue when present.
+    std::vector<double> time_whole, time_rhs;
     std::vector<uint64_t> nnzb;         // stored blocks per system
     std::vector<char> split;            // system solved as one work item per right-hand side
     // work items: a whole system, or one right-hand side of a split system
@@ -557,13 +569,14 @@ void apply_split(fem_batch_s* h, int count) {
     h->item_work.clear();
     for (int k = 0; k < h->nsys; ++k) {
         const BSys& S = h->h_sys[k];
+        const bool measured = h->time_whole.size() == (size_t)h->nsys;
         if (!h->split[k]) {
             BSys it = S;
             it.nr = NR;
             h->items.push_back(it);
             h->item_sys.push_back(k);
             h->item_rhs.push_back(-1);
-            h->item_work.push_back(h->work[k]);
+            h->item_work.push_back(measured && h->time_whole[k] > 0 ? h->time_whole[k] : h->work[k]);
             continue;
         }
         const uint64_t n = (uint64_t)S.nb * S.bs;
@@ -578,7 +591,7 @@ void apply_split(fem_batch_s* h, int count) {
             h->items.push_back(it);
             h->item_sys.push_back(k);
             h->item_rhs.push_back(c);
-            h->item_work.push_back(h->work[k] * frac);
+            h->item_work.push_back(measured && h->time_rhs[k] > 0 ? h->time_rhs[k] : h->work[k] * frac);
         }
     }
     h->sys.upload(h->items, h->stream);
@@ -640,6 +653,34 @@ double launch(fem_batch_s* h, const BSys* sys, const int* order, int nitems, BOu
 double run_items(fem_batch_s* h, const fem_batch_opts* o, int max_refine) {
     return launch(h, h->sys.get(), h->order.get(), (int)h->items.size(), h->out.get(), o->tol, o->inner_tol,
                   max_refine, o->max_inner);
+}
+
+// FEM_TRACE_BATCH: the ten items that finished last and the ten that ran
+// longest (device wall clock), to stderr
+void trace_items(fem_batch_s* h, const std::vector<BOut>& o, double ms) {
+    int dev = 0, khz = 0;
+    FEM_HIP(hipGetDevice(&dev));
+    FEM_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (khz <= 0 || o.empty()) return;
+    long long start = o[0].t0;
+    for (const BOut& b : o) start = std::min(start, b.t0);
+    auto msof = [&](long long t) { return (double)t / khz; };
+    std::vector<size_t> id(o.size());
+    for (size_t t = 0; t < id.size(); ++t) id[t] = t;
+    auto show = [&](const char* what) {
+        std::fprintf(stderr, "[femgpu] batch %.1f ms, %s:\n", ms, what);
+        for (size_t j = 0; j < std::min<size_t>(10, id.size()); ++j) {
+            const size_t t = id[j];
+            const BSys& S = h->items[t];
+            std::fprintf(stderr, "[femgpu]   item %zu system %d rhs %d bs %d rows %u it %d: %.2f .. %.2f ms\n", t,
+                         h->item_sys[t], h->item_rhs[t], S.bs, S.nb, o[t].iterations, msof(o[t].t0 - start),
+                         msof(o[t].t1 - start));
+        }
+    };
+    std::sort(id.begin(), id.end(), [&](size_t a, size_t b) { return o[a].t1 > o[b].t1; });
+    show("last to finish");
+    std::sort(id.begin(), id.end(), [&](size_t a, size_t b) { return o[a].t1 - o[a].t0 > o[b].t1 - o[b].t0; });
+    show("longest");
 }
 
 // per-system results from the items' (a split system: the largest counts of
@@ -784,6 +825,7 @@ int fem_batch_solve(fem_batch h, const fem_batch_opts* opts, fem_batch_stats* pe
             fail(FEM_ERR_INVALID_ARGUMENT, "tol > 0, max_refine >= 1 and max_inner >= 1 are required");
         const double t = run_items(h, opts, opts->max_refine);
         if (ms) *ms = t;
+        if (std::getenv("FEM_TRACE_BATCH")) trace_items(h, download_out(h), t);
         if (per_system) {
             const std::vector<fem_batch_stats> st = merge_out(h, download_out(h));
             std::copy(st.begin(), st.end(), per_system);
@@ -811,7 +853,9 @@ int fem_batch_tune(fem_batch h, const fem_batch_opts* opts, int divisor, fem_bat
         run_items(h, opts, opts->max_refine);
         const std::vector<BOut> o = download_out(h);
         // profiled work: inner iterations x bytes of one iteration (matrix in
-        // float32 with its indices, about nine vector passes)
+        // float32 with its indices, about nine vector passes); it ranks the
+        // systems for splitting and orders the queue until measured item
+        // times (below) replace it
         for (int k = 0; k < h->nsys; ++k) {
             const BSys& S = h->h_sys[k];
             const double bv = (double)S.bs * S.bs;
@@ -880,6 +924,39 @@ int fem_batch_tune(fem_batch h, const fem_batch_opts* opts, int divisor, fem_bat
                 if (ms < best_ms) {
                     best_ms = ms;
                     bs = c;
+                }
+            }
+        }
+        // queue order from measured item times: run the chosen schedule,
+        // record every item's device time (a split system: its slowest
+        // right-hand side), order the queue longest first by those times and
+        // keep them if that run is faster
+        if (bs.order == 0) {
+            h->block = bs.block;
+            h->wg_per_cu = bs.wg_per_cu;
+            h->order_kind = 0;
+            apply_split(h, bs.split);
+            run_items(h, opts, opts->max_refine);
+            const std::vector<BOut> ob = download_out(h);
+            std::vector<double> tw(h->nsys, 0.0), tr(h->nsys, 0.0);
+            bool ok = true;
+            for (size_t it = 0; it < ob.size(); ++it) {
+                const double d = (double)(ob[it].t1 - ob[it].t0);
+                ok = ok && d > 0;
+                const int k = h->item_sys[it];
+                if (h->item_rhs[it] < 0) tw[k] = d;
+                else tr[k] = std::max(tr[k], d);
+            }
+            if (ok) {
+                std::swap(h->time_whole, tw);
+                std::swap(h->time_rhs, tr);
+                apply_split(h, bs.split);
+                const double ms = run_items(h, opts, opts->max_refine);
+                if (trial_ms && t < cap) trial_ms[t] = ms;
+                ++t;
+                if (!(ms < best_ms)) {  // keep the previous order
+                    std::swap(h->time_whole, tw);
+                    std::swap(h->time_rhs, tr);
                 }
             }
         }

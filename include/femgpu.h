@@ -338,8 +338,11 @@ int fem_batch_get_solution(fem_batch b, double* X);
  * 1/divisor of its recorded iterations, then times the fastest of each
  * workgroup size on full solves, unsplit and with the systems whose work
  * exceeds an even share per resident workgroup split, and returns the
- * fastest (trial_ms receives up to cap timings: the 18 capped runs, then the
- * full runs). The batch is left unsplit with its previous schedule; apply
+ * fastest; for a largest-first order it then runs that schedule once more to
+ * record every work item's device time, and orders the queue by those times
+ * when that is faster (trial_ms receives up to cap timings: the 18 capped
+ * runs, the full runs, then that run). The batch is left unsplit with its
+ * previous schedule; apply
  * the result with fem_batch_set_schedule. Solutions are left as the last
  * trial computed them. */
 int fem_batch_tune(fem_batch b, const fem_batch_opts* opts, int divisor, fem_batch_schedule* best, double* trial_ms,
