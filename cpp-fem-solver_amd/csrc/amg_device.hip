@@ -719,7 +719,8 @@ void sgs_setup_device(AmgLevelDev& D, hipStream_t s) {
     std::vector<uint32_t> blk;
     D.h_cblk.assign(1, 0);
     for (int c = 0; c < ncol; ++c) {
-        plan_rows(gp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk);
+        plan_rows(gp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk,
+                  block_nnz(gp[D.h_cptr[c + 1]] - gp[D.h_cptr[c]], stage_nnz<double>(), 2048), stage_rows<double>());
         D.h_cblk.push_back((uint32_t)blk.size() - 1);
     }
     D.gblk.upload(blk, s);
@@ -787,7 +788,8 @@ void sgs_blocks32(AmgLevelDev& D, hipStream_t s) {
     std::vector<uint32_t> blk;
     D.h_cblk32.assign(1, 0);
     for (size_t c = 0; c + 1 < D.h_cptr.size(); ++c) {
-        plan_rows(gp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk, stage_nnz<float>(), stage_rows<float>());
+        plan_rows(gp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk,
+                  block_nnz(gp[D.h_cptr[c + 1]] - gp[D.h_cptr[c]], stage_nnz<float>(), 2048), stage_rows<float>());
         D.h_cblk32.push_back((uint32_t)blk.size() - 1);
     }
     D.gblk32.upload(blk, s);

@@ -35,6 +35,18 @@ void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint3
     }
 }
 
+// Nonzeros per CSR-stream block: the LDS stage size, halved (down to 512)
+// while the matrix would give fewer than min_blocks blocks. A launch of fewer
+// blocks than the GPU holds at once runs one round of blocks and is bound by
+// a block's chain of dependent loads; smaller blocks shorten that chain.
+// Measured at 128^3 (SA-AMG V-cycle) and 96^3 (learned SPAI PCG): SpMV plans
+// are best with min_blocks 1024, the Gauss-Seidel colour sweeps with 2048.
+uint32_t block_nnz(uint64_t nnz, uint32_t stage, uint64_t min_blocks) {
+    uint32_t m = stage;
+    while (m > 512 && nnz / m < min_blocks) m /= 2;
+    return m;
+}
+
 void SpmvPlan::build(const uint32_t* h_rp, uint32_t n, hipStream_t s, uint32_t max_nnz, uint32_t max_rows) {
     std::vector<uint32_t> b;
     b.reserve(n / 128 + 4);
@@ -315,12 +327,12 @@ void CsrMat::ensure_fp32(hipStream_t s) {
     FEM_HIP(hipMemsetAsync(v32.get() + nnz, 0, 4 * sizeof(float), s));
     vec_convert<float, double>(v32.get(), v.get(), 1.0, nnz, s);
     if (host_valid) {
-        plan32.build(h_rp.data(), n, s, stage_nnz<float>(), stage_rows<float>());
+        plan32.build(h_rp.data(), n, s, block_nnz(nnz, stage_nnz<float>(), 1024), stage_rows<float>());
     } else {
         std::vector<uint32_t> hrp(n + 1);
         rp.download(hrp.data(), n + 1, s);
         FEM_HIP(hipStreamSynchronize(s));
-        plan32.build(hrp.data(), n, s, stage_nnz<float>(), stage_rows<float>());
+        plan32.build(hrp.data(), n, s, block_nnz(nnz, stage_nnz<float>(), 1024), stage_rows<float>());
     }
 }
 
@@ -338,7 +350,7 @@ void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, s
     FEM_HIP(hipMemsetAsync(v.get() + nnz, 0, 4 * sizeof(double), s));
     ci.upload(ci_, s);
     v.upload(v_, s);
-    plan.build(rp_.data(), n, s);
+    plan.build(rp_.data(), n, s, block_nnz(nnz, stage_nnz<double>(), 1024), stage_rows<double>());
     pack_columns(ci.get(), nnz, cd, cbase, s);
     h_rp = std::move(rp_);
     h_ci = std::move(ci_);
@@ -360,7 +372,7 @@ void CsrMat::adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&&
     std::vector<uint32_t> hrp(n + 1);
     rp.download(hrp.data(), n + 1, s);
     FEM_HIP(hipStreamSynchronize(s));
-    plan.build(hrp.data(), n, s);
+    plan.build(hrp.data(), n, s, block_nnz(nnz, stage_nnz<double>(), 1024), stage_rows<double>());
     pack_columns(ci.get(), nnz, cd, cbase, s);
     h_rp.clear();
     h_ci.clear();

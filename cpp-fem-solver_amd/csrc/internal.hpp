@@ -75,6 +75,9 @@ template <typename T>
 constexpr int stage_nnz() { return sizeof(T) == 4 ? 2 * kStageNnz : kStageNnz; }
 template <typename T>
 constexpr int stage_rows() { return sizeof(T) == 4 ? 2 * kStageRows : kStageRows; }
+// nonzeros per block for an nnz-entry matrix, at most stage, halved while
+// fewer than min_blocks blocks would result (csr.hip)
+uint32_t block_nnz(uint64_t nnz, uint32_t stage, uint64_t min_blocks);
 // append the block boundaries of rows [lo, hi) to b (b starts with lo)
 void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint32_t>& b,
                uint32_t max_nnz = kStageNnz, uint32_t max_rows = kStageRows);
