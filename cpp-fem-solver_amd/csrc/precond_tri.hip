@@ -204,6 +204,15 @@ __global__ void k_gather_rhs(const uint32_t* __restrict__ slot_row, uint32_t s0,
     if (s < s1) out[s] = rhs[slot_row[s]];
 }
 
+// y[slot_row[s]] = ys[s] for the slots of a chain (its results in slot order)
+template <typename T>
+__global__ void k_scatter_slots(const uint32_t* __restrict__ slot_row, uint32_t s0, uint32_t s1,
+                                const T* __restrict__ ys, T* __restrict__ y, Gate g) {
+    if (g.closed()) return;
+    const uint32_t s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < s1) y[slot_row[s]] = ys[s];
+}
+
 // copy the sweep's values (and float32 copy when present) into the chain layout
 static void densify(Sweep& S, hipStream_t s) {
     if (!S.cent) return;
@@ -244,7 +253,8 @@ __global__ __launch_bounds__(256) void k_sweep_level(uint32_t s0, uint32_t s1, c
 }
 
 // A run of consecutive small levels (<= kChainRows rows each) in one
-// workgroup, one row per thread per level. Results go to y and to an LDS ring
+// workgroup, one row per thread per level. Results go to the slot-ordered
+// right-hand-side buffer (in place) and to an LDS ring
 // indexed by sweep slot (position slot % kRing); the kernel first loads the
 // kRing slots before the chain into it, and the host builds chains whose
 // rows depend only on slots at most kRing behind their level's end, so every
@@ -262,8 +272,7 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint
                                                                const uint16_t* __restrict__ len,
                                                                const uint32_t* __restrict__ cds,
                                                                const T* __restrict__ cval, const T* __restrict__ diag,
-                                                               const T* __restrict__ rslot, T* y, T* sink,
-                                                               Gate g) {
+                                                               T* rslot, const T* y, T* sink, Gate g) {
     if (g.closed()) return;
     __shared__ T ring[kRing + 1];  // + a sink for threads without a row
     __shared__ uint32_t lp[kChainLevels + 1];
@@ -277,7 +286,7 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint
     }
     __syncthreads();
     struct Row {
-        uint32_t on, i, L;
+        uint32_t on, L;
         T r, d;
         uint32_t ds[KC];
         T v[KC];
@@ -287,7 +296,6 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint
         const uint32_t sr = lp[lc - l0] + threadIdx.x;
         R.on = (l < l1 && sr < lp[lc - l0 + 1]) ? 1u : 0u;
         const uint32_t s = R.on ? sr : s0;
-        R.i = slot_row[s];
         R.L = len[s];
         R.r = rslot[s];
         R.d = DIV ? diag[s] : (T)1;
@@ -325,10 +333,14 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint
                 }
                 if constexpr (DIV) acc = acc / cur.d;
                 // stores without a branch: a thread without a row writes to
-                // the sinks (ring[kRing], and sink[0] in global memory)
+                // the sinks (ring[kRing], and sink[0] in global memory). The
+                // result goes to the row's slot of rslot, whose right-hand
+                // side was read D levels ago: a level's slots are
+                // consecutive, so the store is coalesced; k_scatter_slots
+                // moves the chain's results to y afterwards.
                 const uint32_t s = lp[l - l0] + threadIdx.x;
                 ring[cur.on ? s % kRing : kRing] = acc;
-                *(cur.on ? y + cur.i : sink) = acc;
+                *(cur.on ? rslot + s : sink) = acc;
             }
             fetch(l + D, cur);
             // Between levels only the LDS ring must be ordered: wait for LDS
@@ -373,6 +385,8 @@ static void run_sweep(const Sweep& S, const T* val, const T* diag, const T* rhs,
             if (sg.kc == 2) chain_launch<T, DIV, 2>(S, sg, cval, diag, rslot, y, s, g);
             else if (sg.kc == 4) chain_launch<T, DIV, 4>(S, sg, cval, diag, rslot, y, s, g);
             else chain_launch<T, DIV, 8>(S, sg, cval, diag, rslot, y, s, g);
+            hipLaunchKernelGGL((k_scatter_slots<T>), dim3((cnt + 255) / 256), dim3(256), 0, s, S.slot_row.get(), sg.s0,
+                               sg.s1, rslot, y, g);
         } else {
             // one launch per level (also for chains with rows longer than 8)
             for (uint32_t l = sg.lvl0; l < sg.lvl1; ++l) {
