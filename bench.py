@@ -183,6 +183,85 @@ def c3_leg(F, gen, args, log):
     return out
 
 
+def rows_leg(F, gen, args, log):
+    """The other rows of the solve path, one GPU, FP64 unless stated: one
+    timed solve per method after a warm-up solve (preconditioner built, b and
+    x in HBM; block-matrix solves: the library's own timer around the solve,
+    which for a new solver object includes capturing its graphs), with
+    iterations, true relative residual and setup time; the block matvec's
+    average time over 50 device calls (each synchronised)."""
+    out = {}
+
+    def csr_solve(name, A, b, kind, solver="pcg", prec=F.FP64, restart=30, max_iter=20000, **prm):
+        bd = F.DeviceArray.from_host(b)
+        xd = F.DeviceArray(A.rows)
+        t = time.time()
+        M = F.Preconditioner(A, kind, fp32=int(prec == F.FP32_IR), **prm) if kind != "none" else None
+        setup_ms = (time.time() - t) * 1e3
+        S = F.Solver(A, M, solver=solver, precision=prec, max_iter=max_iter, device_ptrs=True, restart=restart)
+        S.solve_device(bd.address, xd.address)
+        F.device_synchronize()
+        t = time.perf_counter()
+        st = S.solve_device(bd.address, xd.address)
+        F.device_synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        out[name] = {"solve_ms": round(ms, 2), "setup_ms": round(setup_ms, 1), "iterations": st["iterations"],
+                     "relres": st["true_relres"]}
+        log(f"[bench] rows {name}: setup {setup_ms:.1f} ms solve {ms:.2f} ms it {st['iterations']} "
+            f"relres {st['true_relres']:.2e}")
+        S.close()
+        if M is not None:
+            M.close()
+
+    G = gen.poisson3d(128)
+    _, b = gen.rhs(G, 0)
+    A = F.CsrMatrix.from_gen(G)
+    csr_solve("poisson128_ssor_pcg", A, b, "ssor", omega=1.0)
+    csr_solve("poisson128_ic0_pcg_mp", A, b, "ic0", prec=F.FP32_IR)
+    A.close()
+    # C/F AMG (host-built hierarchy)
+    G = gen.poisson3d(64)
+    _, b = gen.rhs(G, 0)
+    A = F.CsrMatrix.from_gen(G)
+    csr_solve("poisson64_cf_amg_pcg", A, b, "amg")
+    A.close()
+    G = gen.convdiff2d(256)
+    _, b = gen.rhs(G, 0)
+    A = F.CsrMatrix.from_gen(G)
+    csr_solve("convdiff256_bicgstab_ilu0", A, b, "ilu0", solver="bicgstab")
+    csr_solve("convdiff256_gmres30_ilu0", A, b, "ilu0", solver="gmres", restart=30)
+    csr_solve("convdiff256_bicgstab_ilu0_mp", A, b, "ilu0", solver="bicgstab", prec=F.FP32_IR)
+    A.close()
+    # block::BlockMatrix: 3x3 blocks from the reference test's generator, 128^2 block rows
+    S = gen.block_test_scalar(128)
+    Ac = F.CsrMatrix.from_gen(S)
+    B = F.BsrMatrix.condense(Ac, 3, 128 * 128, 128 * 128)
+    x = gen.drand(np.arange(S.n))
+    xd, yd = F.DeviceArray.from_host(x), F.DeviceArray(S.n)
+    B.matvec_device(xd.address, yd.address)
+    t = time.perf_counter()
+    for _ in range(50):
+        B.matvec_device(xd.address, yd.address)
+    mv_us = (time.perf_counter() - t) / 50 * 1e6
+    byt = B.nnzb * (72 + 4) + 4 * (B.nbr + 1) + 2 * 8 * S.n
+    out["bsr3_matvec_49152"] = {"us_per_call_incl_sync": round(mv_us, 1), "nnzb": int(B.nnzb), "bytes": int(byt)}
+    log(f"[bench] rows bsr3 matvec ({B.nbr} block rows, {B.nnzb} blocks): {mv_us:.1f} us per synchronised call")
+    bb = 1.0 + 0.5 * gen.drand(np.arange(S.n) + 7)
+    for name, kind, solver in (("bsr3_bjacobi_minicg", F.PC_BJACOBI, "minicg"),
+                               ("bsr3_bilu0_gmres30", F.PC_BILU0, "gmres")):
+        M = F.Preconditioner(B, kind)
+        F.solve(B, M, bb, solver=solver, tol=1e-8, max_iter=3000, restart=30)
+        _, st = F.solve(B, M, bb, solver=solver, tol=1e-8, max_iter=3000, restart=30)
+        out[name] = {"solve_ms_library_timer": round(st["solve_ms"], 2), "iterations": st["iterations"],
+                     "relres": st["true_relres"]}
+        log(f"[bench] rows {name}: {st['solve_ms']:.2f} ms (library timer) it {st['iterations']} "
+            f"relres {st['true_relres']:.2e}")
+        M.close()
+    B.close()
+    Ac.close()
+    return out
+
+
 def c5_leg(F, gen, args, log):
     """Batch of independent SPD systems (half 2-D Poisson, side U{32..256};
     half 3x3-block 7-point systems, side U{16..48}; 4 right-hand sides each;
@@ -245,6 +324,9 @@ def main():
     ap.add_argument("--c3-applies", type=int, default=0,
                     help="only apply the ILU(0) preconditioner this many times (no solves)")
     ap.add_argument("--c5-count", type=int, default=512, help="systems in the batch")
+    ap.add_argument("--rows", action="store_true",
+                    help="add a timing of the other solve-path rows (SSOR, IC0 mixed precision, C/F AMG, "
+                         "BiCGSTAB/GMRES + ILU(0), block matvec, block-Jacobi, block ILU(0))")
     ap.add_argument("--spmv-probe", action="store_true",
                     help="only run a 512 MiB device copy (counter calibration) and the float64 SpMV; "
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes")
@@ -352,6 +434,7 @@ def main():
     # ---- batched leg: 512 independent systems in one persistent launch ----
     c5 = None if args.no_c5 else c5_leg(F, gen, args, log)
     c3 = c3_leg(F, gen, args, log) if args.c3 else None
+    rows = rows_leg(F, gen, args, log) if args.rows else None
 
     # ---- CPU baseline: the oracle's float64 IC(0)-PCG on this host -------
     cpu = None
@@ -413,6 +496,8 @@ def main():
         out["config"]["c5"] = c5
     if c3 is not None:
         out["config"]["c3"] = c3
+    if rows is not None:
+        out["config"]["rows"] = rows
     if results.get("ic0_fp64") and results.get("saamg_mp"):
         out["config"]["speedup_mp_vs_fp64_ic0"] = round(results["ic0_fp64"]["solve_ms"] / results["saamg_mp"]["solve_ms"], 2)
     print(json.dumps(out))

@@ -299,6 +299,51 @@ class CsrMatrix:
             pass
 
 
+class BsrMatrix:
+    """Device block-CSR matrix handle (block::BlockMatrix<double, BS>)."""
+
+    def __init__(self, bs, nbr, nbc, row_ptr, col_idx, values, _handle=None):
+        self._h = C.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            rp, ci, v = _u32(row_ptr), _u32(col_idx), _f64(values)
+            _check(lib().fem_bsr_create(int(bs), int(nbr), int(nbc), _p(rp, C.c_uint32), _p(ci, C.c_uint32),
+                                        _p(v, C.c_double), C.byref(self._h)))
+        bs_, nbr_, nbc_, nz = C.c_int(), C.c_uint32(), C.c_uint32(), C.c_uint64()
+        _check(lib().fem_bsr_info(self._h, C.byref(bs_), C.byref(nbr_), C.byref(nbc_), C.byref(nz)))
+        self.bs, self.nbr, self.nbc, self.nnzb = bs_.value, nbr_.value, nbc_.value, nz.value
+        self.nrows = self.nbr * self.bs
+
+    @classmethod
+    def condense(cls, A, bs, nbr, nbc):
+        """BlockMatrix::condense of a CSR matrix (on the device)."""
+        h = C.c_void_p()
+        _check(lib().fem_bsr_condense(A._h, int(bs), int(nbr), int(nbc), C.byref(h)))
+        return cls(0, 0, 0, None, None, None, _handle=h)
+
+    def matvec(self, x):
+        x = _f64(x)
+        y = np.empty(self.nrows, np.float64)
+        _check(lib().fem_bsr_matvec(self._h, _p(x, C.c_double), _p(y, C.c_double)))
+        return y
+
+    def matvec_device(self, x_ptr: int, y_ptr: int, fp32=False):
+        """y = A x on device buffers (synchronises the matrix's stream)."""
+        _check(lib().fem_bsr_matvec_device(self._h, C.c_void_p(x_ptr), C.c_void_p(y_ptr), int(fp32)))
+
+    def close(self):
+        if self._h:
+            lib().fem_bsr_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def pc_params(kind: int, **kw) -> PcParams:
     p = PcParams()
     lib().fem_pc_params_default(kind, C.byref(p))
