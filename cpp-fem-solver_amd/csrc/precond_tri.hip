@@ -247,7 +247,27 @@ __global__ __launch_bounds__(256) void k_sweep_level(uint32_t s0, uint32_t s1, c
     const uint32_t base = gptr[s >> 6] + (s & 63);
     const uint32_t L = len[s];
     T acc = rhs[i];
-    for (uint32_t e = 0; e < L; ++e) acc -= val[base + e * 64] * y[col[base + e * 64]];
+    if (L <= 8) {
+        // every entry's column and value, then every gather, are issued
+        // before any use (three dependent rounds instead of two per entry);
+        // entries past L repeat entry 0 and are not subtracted, so the
+        // arithmetic is the loop's below
+        uint32_t cl[8];
+        T vl[8], yl[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) {
+            const uint32_t k = base + 64u * ((uint32_t)e < L ? (uint32_t)e : 0u);
+            cl[e] = L ? col[k] : 0u;
+            vl[e] = L ? val[k] : (T)0;
+        }
+#pragma unroll
+        for (int e = 0; e < 8; ++e) yl[e] = L ? y[cl[e]] : (T)0;
+#pragma unroll
+        for (int e = 0; e < 8; ++e)
+            if ((uint32_t)e < L) acc -= vl[e] * yl[e];
+    } else {
+        for (uint32_t e = 0; e < L; ++e) acc -= val[base + e * 64] * y[col[base + e * 64]];
+    }
     if constexpr (DIV) acc = acc / diag[s];
     y[i] = acc;
 }
