@@ -567,16 +567,21 @@ void apply_split(fem_batch_s* h, int count) {
     h->item_sys.clear();
     h->item_rhs.clear();
     h->item_work.clear();
+    // measured item times order the queue only when every item of this
+    // layout has one (they are device clock ticks, not comparable with the
+    // work model)
+    bool measured = h->time_whole.size() == (size_t)h->nsys && h->time_rhs.size() == (size_t)h->nsys;
+    for (int k = 0; measured && k < h->nsys; ++k)
+        measured = h->split[k] ? h->time_rhs[k] > 0 : h->time_whole[k] > 0;
     for (int k = 0; k < h->nsys; ++k) {
         const BSys& S = h->h_sys[k];
-        const bool measured = h->time_whole.size() == (size_t)h->nsys;
         if (!h->split[k]) {
             BSys it = S;
             it.nr = NR;
             h->items.push_back(it);
             h->item_sys.push_back(k);
             h->item_rhs.push_back(-1);
-            h->item_work.push_back(measured && h->time_whole[k] > 0 ? h->time_whole[k] : h->work[k]);
+            h->item_work.push_back(measured ? h->time_whole[k] : h->work[k]);
             continue;
         }
         const uint64_t n = (uint64_t)S.nb * S.bs;
@@ -591,7 +596,7 @@ void apply_split(fem_batch_s* h, int count) {
             h->items.push_back(it);
             h->item_sys.push_back(k);
             h->item_rhs.push_back(c);
-            h->item_work.push_back(measured && h->time_rhs[k] > 0 ? h->time_rhs[k] : h->work[k] * frac);
+            h->item_work.push_back(measured ? h->time_rhs[k] : h->work[k] * frac);
         }
     }
     h->sys.upload(h->items, h->stream);
