@@ -143,6 +143,7 @@ struct KState {
     int need_true;    // GMRES: cycle starts from the true residual
     int iters;        // iterations run (operator applications in the loop)
     int minicg;       // 1: mini_cg semantics (no breakdown tests, raw loop index)
+    double rz_prev;   // PCG: rz before the current iteration's update (fused p update)
 };
 
 enum SolverErr : int {

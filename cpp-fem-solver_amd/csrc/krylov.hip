@@ -151,8 +151,37 @@ struct FinPcgXr {
         st->normr = sqrt(t[0]);
         if (st->normr / st->normb < st->tol) { st->done = 1; return; }
         if constexpr (PCM != PCM_GEN) pcg_finish(st, t[1]);
+        else st->rz_prev = st->rz;
     }
 };
+
+// k_pcg_xr with the p.q reduction tail folded in: every block adds the
+// product's npq partial sums exactly as the one-block tail k_finalize<1>
+// does (same order, same block size), block 0 runs that tail's state update
+// (FinPcgPq), and every block uses alpha = rz / p.q, the value FinPcgPq
+// stores. st->rz is not written in this kernel.
+template <typename T, int PCM>
+__global__ __launch_bounds__(kBlock) void k_pcg_xr_f(T* x, T* r, const T* p, const T* q, const T* d, size_t n,
+                                                     KState* st, double* part, const double* __restrict__ pq_part,
+                                                     unsigned npq) {
+    DONE_RET(st);
+    double t[1] = {0.0};
+    for (unsigned b = threadIdx.x; b < npq; b += blockDim.x) t[0] += pq_part[b];
+    block_sum<1>(t);
+    if (blockIdx.x == 0 && threadIdx.x == 0) FinPcgPq{st}(t);
+    const T a = (T)(st->rz / t[0]);
+    double rr = 0.0, rz = 0.0;
+    GRID_LOOP(i, n) {
+        x[i] += a * p[i];
+        const T ri = r[i] - a * q[i];
+        r[i] = ri;
+        rr += (double)ri * (double)ri;
+        if constexpr (PCM == PCM_DIAG) rz += (double)ri * (double)(ri / d[i]);
+    }
+    if constexpr (PCM == PCM_NONE) rz = rr;
+    double v[2] = {rr, rz};
+    block_partials<2>(v, part);
+}
 
 struct FinPcgRz {
     KState* st;
@@ -166,6 +195,21 @@ template <typename T, int PCM>
 __global__ void k_pcg_p(const T* r, const T* z, const T* d, T* p, size_t n, KState* st) {
     DONE_RET(st);
     const T b = (T)st->beta;
+    GRID_LOOP(i, n) p[i] = zval<T, PCM>(r, z, d, i) + b * p[i];
+}
+
+// k_pcg_p with the r.z reduction tail folded in (as k_pcg_xr_f): block 0
+// runs FinPcgRz, every block uses beta = r.z / rz_prev (FinPcgXr saved the
+// previous rz there; st->rz itself changes in this kernel)
+template <typename T, int PCM>
+__global__ __launch_bounds__(kBlock) void k_pcg_p_f(const T* r, const T* z, const T* d, T* p, size_t n, KState* st,
+                                                    const double* __restrict__ rz_part, unsigned nrz) {
+    DONE_RET(st);
+    double t[1] = {0.0};
+    for (unsigned b = threadIdx.x; b < nrz; b += blockDim.x) t[0] += rz_part[b];
+    block_sum<1>(t);
+    if (blockIdx.x == 0 && threadIdx.x == 0) FinPcgRz{st}(t);
+    const T b = (T)(t[0] / st->rz_prev);
     GRID_LOOP(i, n) p[i] = zval<T, PCM>(r, z, d, i) + b * p[i];
 }
 
@@ -683,6 +727,13 @@ unsigned KrylovSolver::precond_dot(const T* r, T* z, Gate g) {
         FEM_HIP(hipGetLastError());                                              \
     } while (0)
 
+// grid of a kernel whose every block re-adds np partial sums: at most G
+// blocks and about 4M partial-sum reads in all
+static unsigned fused_grid(unsigned G, unsigned np) {
+    const unsigned cap = (unsigned)std::max<uint64_t>(1, (4ull << 20) / std::max(1u, np));
+    return std::max(1u, std::min(G, cap));
+}
+
 template <typename T>
 void KrylovSolver::pcg_init(KWork<T>& W) {
     const size_t n = W.n;
@@ -721,10 +772,43 @@ void KrylovSolver::pcg_iter(KWork<T>& W) {
         else d = M_->diag32();
     }
     Gate g{&st->done};
-    // q = A p and p.q: one fused kernel where the operator has one
+    // q = A p and p.q: one fused kernel where the operator has one; its
+    // reduction tail then runs inside the x/r update (every block re-adds the
+    // partial sums; the update's grid shrinks so that this stays within about
+    // 4M partial-sum reads)
     if (const unsigned np = matvec_dot<T>(W.p.get(), W.q.get(), g)) {
-        launch_finalize<1>(dotp_.get(), np, FinPcgPq{st}, s_);
-        FEM_HIP(hipGetLastError());
+        if (pm == PCM_GEN) {
+            const unsigned Gf = fused_grid(G, np);
+            LAUNCH((k_pcg_xr_f<T, PCM_GEN>), Gf, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part,
+                   dotp_.get(), np);
+            launch_finalize<2>(part, Gf, FinPcgXr<PCM_GEN>{st}, s_);
+            FEM_HIP(hipGetLastError());
+            if (const unsigned nz = precond_dot<T>(W.r.get(), W.z.get(), g)) {
+                LAUNCH((k_pcg_p_f<T, PCM_GEN>), fused_grid(G, nz), W.r.get(), W.z.get(), d, W.p.get(), n, st,
+                       dotp_.get(), nz);
+            } else {
+                precond<T>(W.r.get(), W.z.get(), g);
+                LAUNCH((k_dot2<T>), G, W.r.get(), W.z.get(), n, st, part);
+                FIN(1, FinPcgRz{st});
+                LAUNCH((k_pcg_p<T, PCM_GEN>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
+            }
+            return;
+        }
+        const unsigned Gf = fused_grid(G, np);
+        if (pm == PCM_DIAG) {
+            LAUNCH((k_pcg_xr_f<T, PCM_DIAG>), Gf, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part,
+                   dotp_.get(), np);
+            launch_finalize<2>(part, Gf, FinPcgXr<PCM_DIAG>{st}, s_);
+            FEM_HIP(hipGetLastError());
+            LAUNCH((k_pcg_p<T, PCM_DIAG>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
+        } else {
+            LAUNCH((k_pcg_xr_f<T, PCM_NONE>), Gf, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part,
+                   dotp_.get(), np);
+            launch_finalize<2>(part, Gf, FinPcgXr<PCM_NONE>{st}, s_);
+            FEM_HIP(hipGetLastError());
+            LAUNCH((k_pcg_p<T, PCM_NONE>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
+        }
+        return;
     } else {
         matvec<T>(W.p.get(), W.q.get(), g);
         LAUNCH((k_dot2<T>), G, W.p.get(), W.q.get(), n, st, part);
