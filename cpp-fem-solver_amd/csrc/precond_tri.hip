@@ -31,6 +31,13 @@ constexpr uint32_t kChainRows = 1024;  // levels this small run inside one workg
 constexpr int kChainThreads = 1024;
 constexpr uint32_t kChainLevels = 4096;  // levels per chain launch (LDS level table)
 constexpr uint32_t kRing = 4096;         // LDS ring of the most recent sweep results
+// wide chains: levels of up to wide_rows(kc) rows, wide_r(kc) rows per thread
+// (two levels of rows are held in registers: 4 rows for rows of at most 2
+// entries, 2 for at most 4, none above), an LDS ring of kRingW results
+// (128 KB of float64)
+constexpr uint32_t kRingW = 16384;
+constexpr int wide_r(uint32_t kc) { return kc <= 2 ? 4 : kc <= 4 ? 2 : 0; }
+constexpr uint32_t wide_rows(uint32_t kc) { return (uint32_t)wide_r(kc) * 1024u; }
 
 // ---------------------------------------------------------------------------
 // Level schedule + SELL-64 sweep layout
@@ -40,6 +47,7 @@ struct Segment {
     uint32_t lvl0, lvl1;   // level range (chain when lvl1 - lvl0 > 1)
     uint32_t kc = 0;       // chain: entries per slot in the dense layout
     uint64_t cbase = 0;    // chain: offset of slot s0 in the dense arrays
+    bool wide = false;     // chain of levels wider than kChainRows (wide-chain kernel)
 };
 
 struct Sweep {
@@ -102,32 +110,45 @@ struct Sweep {
         }
         // segments: chains of consecutive small levels, big levels alone
         segs.clear();
+        // a chain holds only levels whose rows depend on slots at most
+        // `ring` behind the level's end (the LDS ring window; the chain
+        // kernel preloads the ring with the slots before the chain), so every
+        // dependency of a chain is read from LDS
+        auto in_window = [&](uint32_t lev, uint32_t ring) {
+            const uint32_t end = h_lvl_ptr[lev + 1];
+            for (uint32_t q = h_lvl_ptr[lev]; q < end; ++q) {
+                const uint32_t i = order[q];
+                for (uint32_t t = rp[i]; t < rp[i + 1]; ++t)
+                    if (end - slot_of[cols[t]] > ring) return false;
+            }
+            return true;
+        };
+        auto width = [&](uint32_t lev) { return h_lvl_ptr[lev + 1] - h_lvl_ptr[lev]; };
+        // the wide chains' row limit follows the longest row of the sweep
+        uint32_t kcmax = 1;
+        for (uint32_t q = 0; q < n; ++q) kcmax = std::max<uint32_t>(kcmax, ln[q]);
+        const uint32_t kWideRows = std::max(kChainRows, wide_rows(kcmax <= 2 ? 2u : kcmax <= 4 ? 4u : 8u));
         uint32_t l = 0;
         while (l < nlev) {
-            const uint32_t cnt = h_lvl_ptr[l + 1] - h_lvl_ptr[l];
-            if (cnt > kChainRows) {
+            const uint32_t cnt = width(l);
+            if (cnt > kWideRows) {
                 segs.push_back({h_lvl_ptr[l], h_lvl_ptr[l + 1], l, l + 1});
                 ++l;
                 continue;
             }
-            // a chain holds only levels whose rows depend on slots at most
-            // kRing behind the level's end (the LDS ring window; the chain
-            // kernel preloads the kRing slots before the chain into it), so
-            // every dependency of a chain is read from LDS
-            auto in_window = [&](uint32_t lev) {
-                const uint32_t end = h_lvl_ptr[lev + 1];
-                for (uint32_t q = h_lvl_ptr[lev]; q < end; ++q) {
-                    const uint32_t i = order[q];
-                    for (uint32_t t = rp[i]; t < rp[i + 1]; ++t)
-                        if (end - slot_of[cols[t]] > kRing) return false;
-                }
-                return true;
-            };
+            const bool wide = cnt > kChainRows;
             uint32_t e = l;
-            while (e < nlev && e - l < kChainLevels && h_lvl_ptr[e + 1] - h_lvl_ptr[e] <= kChainRows && in_window(e))
-                ++e;
+            if (wide) {
+                while (e < nlev && e - l < kChainLevels && width(e) > kChainRows && width(e) <= kWideRows &&
+                       in_window(e, kRingW))
+                    ++e;
+            } else {
+                while (e < nlev && e - l < kChainLevels && width(e) <= kChainRows && in_window(e, kRing)) ++e;
+            }
             if (e == l) e = l + 1;  // a level alone (one launch)
-            segs.push_back({h_lvl_ptr[l], h_lvl_ptr[e], l, e});
+            Segment sg{h_lvl_ptr[l], h_lvl_ptr[e], l, e};
+            sg.wide = wide && e - l > 1;
+            segs.push_back(sg);
             l = e;
         }
         // dense layout of the chains
@@ -374,15 +395,107 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain(uint32_t l0, uint
     }
 }
 
+// A run of consecutive levels of kChainRows..kWideR * 1024 rows in one
+// workgroup: kWideR rows per thread per level, an LDS ring of the last
+// kRingW results (preloaded like the narrow chain's), each level's rows
+// fetched one level ahead (the registers of kWideR rows leave room for one
+// level in flight). Stores, sinks and the per-row arithmetic order are the
+// narrow chain's.
+template <typename T, bool DIV, int KC, int kWideR>
+__global__ __launch_bounds__(kChainThreads) void k_sweep_chain_wide(uint32_t l0, uint32_t l1,
+                                                                    const uint32_t* __restrict__ lvl_ptr,
+                                                                    const uint32_t* __restrict__ slot_row,
+                                                                    const uint16_t* __restrict__ len,
+                                                                    const uint32_t* __restrict__ cds,
+                                                                    const T* __restrict__ cval,
+                                                                    const T* __restrict__ diag, T* rslot, const T* y,
+                                                                    T* sink, Gate g) {
+    if (g.closed()) return;
+    __shared__ T ring[kRingW + 1];  // + a sink for threads without a row
+    __shared__ uint32_t lp[kChainLevels + 1];
+    for (uint32_t t = threadIdx.x; t <= l1 - l0; t += blockDim.x) lp[t] = lvl_ptr[l0 + t];
+    __syncthreads();
+    const uint32_t s0 = lp[0];
+    for (uint32_t t = threadIdx.x; t < kRingW; t += blockDim.x) {
+        const uint32_t back = kRingW - t;  // slot s0 - back
+        ring[(s0 - back) % kRingW] = back <= s0 ? y[slot_row[s0 - back]] : (T)0;
+    }
+    __syncthreads();
+    struct Row {
+        uint32_t on, slot, L;
+        T r, d;
+        uint32_t ds[KC];
+        T v[KC];
+    };
+    auto fetch = [&](uint32_t l, Row (&R)[kWideR]) {
+        const uint32_t lc = l < l1 ? l : l1 - 1;
+        const uint32_t a = lp[lc - l0], b = lp[lc - l0 + 1];
+#pragma unroll
+        for (int q = 0; q < kWideR; ++q) {
+            const uint32_t sr = a + threadIdx.x + (uint32_t)q * kChainThreads;
+            R[q].on = (l < l1 && sr < b) ? 1u : 0u;
+            R[q].slot = sr;
+            const uint32_t sl = R[q].on ? sr : s0;
+            R[q].L = len[sl];
+            R[q].r = rslot[sl];
+            R[q].d = DIV ? diag[sl] : (T)1;
+            const uint64_t o = (uint64_t)(sl - s0) * KC;
+#pragma unroll
+            for (int e = 0; e < KC; ++e) {
+                R[q].ds[e] = cds[o + e];
+                R[q].v[e] = cval[o + e];
+            }
+        }
+    };
+    auto level = [&](Row (&R)[kWideR]) {
+#pragma unroll
+        for (int q = 0; q < kWideR; ++q) {
+            T acc = R[q].r;
+#pragma unroll
+            for (int e = 0; e < KC; ++e) {
+                const T t = acc - R[q].v[e] * ring[R[q].ds[e] % kRingW];
+                acc = (uint32_t)e < R[q].L ? t : acc;
+            }
+            if constexpr (DIV) acc = acc / R[q].d;
+            ring[R[q].on ? R[q].slot % kRingW : kRingW] = acc;
+            *(R[q].on ? rslot + R[q].slot : sink) = acc;
+        }
+        // LDS-only barrier between levels (the next level's rows stay in flight)
+        __asm__ volatile("" ::: "memory");
+        __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): LDS done
+        __builtin_amdgcn_s_barrier();
+        __asm__ volatile("" ::: "memory");
+    };
+    Row A[kWideR], B[kWideR];
+    fetch(l0, A);
+    for (uint32_t l = l0; l < l1; l += 2) {
+        fetch(l + 1, B);
+        level(A);
+        if (l + 1 >= l1) break;  // uniform over the workgroup
+        fetch(l + 2, A);
+        level(B);
+    }
+}
+
 template <typename T, bool DIV, int KC>
 static void chain_launch(const Sweep& S, const Segment& sg, const T* cval, const T* diag, T* rslot, T* y, hipStream_t s,
                          Gate g) {
     T* sink;
     if constexpr (std::is_same<T, double>::value) sink = S.sink.get();
     else sink = S.sink32.get();
-    hipLaunchKernelGGL((k_sweep_chain<T, DIV, KC>), dim3(1), dim3(kChainThreads), 0, s, sg.lvl0, sg.lvl1,
-                       S.lvl_ptr.get(), S.slot_row.get(), S.len.get(), S.cds.get() + sg.cbase, cval + sg.cbase, diag,
-                       rslot, y, sink, g);
+    constexpr int R = wide_r(KC);
+    if constexpr (R > 0) {
+        if (sg.wide) {
+            hipLaunchKernelGGL((k_sweep_chain_wide<T, DIV, KC, R>), dim3(1), dim3(kChainThreads), 0, s, sg.lvl0,
+                               sg.lvl1, S.lvl_ptr.get(), S.slot_row.get(), S.len.get(), S.cds.get() + sg.cbase,
+                               cval + sg.cbase, diag, rslot, y, sink, g);
+            return;
+        }
+    }
+    if (sg.wide) fail(FEM_ERR_RUNTIME, "triangular sweep: wide chain with rows longer than 4 entries");
+        hipLaunchKernelGGL((k_sweep_chain<T, DIV, KC>), dim3(1), dim3(kChainThreads), 0, s, sg.lvl0, sg.lvl1,
+                           S.lvl_ptr.get(), S.slot_row.get(), S.len.get(), S.cds.get() + sg.cbase, cval + sg.cbase,
+                           diag, rslot, y, sink, g);
 }
 
 template <typename T, bool DIV>

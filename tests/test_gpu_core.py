@@ -216,13 +216,23 @@ def test_graph_and_plain_launches_agree():
     assert s1["iterations"] == s2["iterations"] == s3["iterations"]
 
 
-@pytest.mark.parametrize("case", ["vardiff3d", "poisson3d", "unsorted_rows"])
+@pytest.mark.parametrize("case", ["vardiff3d", "poisson3d", "unsorted_rows", "poisson3d_64", "poisson2d_1500"])
 def test_ic0_apply_bitexact(case):
+    """IC(0) factor + apply against the oracle, bit for bit. poisson3d_64 has
+    levels of up to 3,072 rows (narrow chains, wide chains of two rows per
+    thread, single-level launches); poisson2d_1500 has anti-diagonal levels of
+    up to 1,500 rows (wide chains of four rows per thread)."""
     if case == "vardiff3d":
         G = gen.vardiff3d(14, seed=7)[0]
         rp, ci, v, n = G.row_ptr, G.col_idx, G.values, G.n
     elif case == "poisson3d":
         G = gen.poisson3d(24)
+        rp, ci, v, n = G.row_ptr, G.col_idx, G.values, G.n
+    elif case == "poisson3d_64":
+        G = gen.poisson3d(64)
+        rp, ci, v, n = G.row_ptr, G.col_idx, G.values, G.n
+    elif case == "poisson2d_1500":
+        G = gen.poisson2d(1500)
         rp, ci, v, n = G.row_ptr, G.col_idx, G.values, G.n
     else:
         G = gen.poisson2d(12)
