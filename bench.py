@@ -421,6 +421,22 @@ def main():
             "iterations": st["iterations"], "refinements": st["refinements"], "relres": relres, "rel_err": err,
             "levels": info["levels"],
         }
+        pc_bytes = M.apply_bytes(fp32=prec == F.FP32_IR)
+        if pc_bytes > 0:
+            # one inner PCG iteration: q = A p (with p.q), x += a p and r -= a q
+            # (6 vectors), z = M r, r.z (2), p = z + b p (3). With the diagonal
+            # the update kernels form r / d themselves: d read in both, r in
+            # the p update, no z and no separate r.z. The solve time also holds
+            # the float64 refinement steps.
+            s_b = 4 if prec == F.FP32_IR else 8
+            vec = 11 * G.n * s_b + (0 if kind == "diag" else pc_bytes)
+            it_bytes = spmv_bytes(G.nnz, G.n, G.m, s_b) + vec
+            gbs = st["iterations"] * it_bytes / (np.median(times) * 1e-3) / 1e9
+            results[name]["solve_level"] = {
+                "bytes_per_iteration": int(it_bytes), "precond_bytes": int(pc_bytes) if kind != "diag" else 0,
+                "achieved_GBs": round(gbs, 1), "peak_GBs": 8000.0, "frac": round(gbs / 8000.0, 4)}
+            log(f"[bench] {name}: {it_bytes / 1e6:.1f} MB per iteration (preconditioner {pc_bytes / 1e6:.1f} MB), "
+                f"solve-level {gbs:.0f} GB/s ({gbs / 80:.1f}% of 8 TB/s)")
         log(f"[bench] {name}: setup {setup_ms:.1f} ms solve {np.median(times):.2f} ms it {st['iterations']} "
             f"relres {relres:.2e} err {err:.2e}")
         if is_head:

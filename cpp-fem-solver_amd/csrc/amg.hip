@@ -507,6 +507,79 @@ struct PcAmg : Precond {
 
     void apply(const double* r, double* z, hipStream_t s, Gate g) override { vcycle<double>(r, z, s, g); }
     void apply32(const float* r, float* z, hipStream_t s, Gate g) override { vcycle<float>(r, z, s, g); }
+
+    // Algorithmic bytes of one V-cycle, kernel by kernel in the order vcycle
+    // launches them (s-byte values, uint32 indices).
+    double apply_bytes(int s) override {
+        const size_t L = lv.size();
+        double b = 0.0;
+        for (size_t k = 0; k + 1 < L; ++k) {
+            AmgLevelDev& D = *lv[k];
+            if (smoother == FEM_SMOOTH_SGS && D.use_gs) b += gs_bytes(D, s, npre, true);
+            else b += npre > 0 ? jacobi_bytes(D, s, npre, true) : (double)D.n * s;
+            const bool shortcut = smoother == FEM_SMOOTH_SGS && residual_shortcut(D);
+            b += shortcut ? gs_residual_bytes(D, s) : csr_bytes(D.A, s, 1);
+            b += csr_bytes(shortcut && D.Rz.n ? D.Rz : D.R, s, 0);
+        }
+        b += (double)nco * nco * 8 + 2.0 * nco * s;  // dense coarsest operator, f, u
+        for (int64_t k = (int64_t)L - 2; k >= 0; --k) {
+            AmgLevelDev& D = *lv[k];
+            b += csr_bytes(D.P, s, 1);  // u += P e reads u
+            b += (smoother == FEM_SMOOTH_SGS && D.use_gs) ? gs_bytes(D, s, npost, false)
+                                                          : jacobi_bytes(D, s, npost, false);
+        }
+        return b;
+    }
+
+    // sweep 0 from zero reads f, d and writes u; a sweep from u is an SpMV
+    // whose epilogue reads f and d
+    static double jacobi_bytes(const AmgLevelDev& D, int s, int sweeps, bool from_zero) {
+        double b = 0.0;
+        for (int k = 0; k < sweeps; ++k) b += (from_zero && k == 0) ? 3.0 * D.n * s : csr_bytes(D.A, s, 2);
+        return b;
+    }
+
+    // entries of the colour-grouped matrix in grouped rows [lo, hi)
+    static uint64_t grouped_nnz(const AmgLevelDev& D, uint32_t lo, uint32_t hi) {
+        uint32_t a = 0, e = 0;
+        FEM_HIP(hipMemcpy(&a, D.gp.get() + lo, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        FEM_HIP(hipMemcpy(&e, D.gp.get() + hi, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        return (uint64_t)e - a;
+    }
+
+    // one colour pass over r rows holding z entries: the rows' matrix part
+    // and row list, u gathered once, f, the diagonal and the new u per row
+    static double colour_bytes(const AmgLevelDev& D, int s, uint64_t z, uint32_t r) {
+        return (double)z * (s + 4) + 4.0 * (r + 1) + 4.0 * r + (double)D.n * s + 3.0 * r * s;
+    }
+
+    double gs_bytes(const AmgLevelDev& D, int s, int sweeps, bool from_zero) const {
+        if (sweeps <= 0 && !from_zero) return 0.0;
+        if (D.n <= 8192)  // one workgroup runs every sweep; each re-reads the level
+            return (from_zero ? (double)D.n * s : 0.0) + sweeps * colour_bytes(D, s, D.A.nnz, D.n);
+        if (sweeps == 0) return (double)D.n * s;  // k_zero
+        double b = 0.0;
+        for (int sw = 0; sw < sweeps; ++sw)
+            for (int c = 0; c < D.ncol; ++c) {
+                const uint32_t lo = D.h_cptr[c], hi = D.h_cptr[c + 1];
+                if (from_zero && sw == 0 && c == 0) {
+                    b += 4.0 * D.n + 2.0 * (hi - lo) * s + (double)D.n * s;  // k_gs_first: row list, f, d; u
+                    continue;
+                }
+                if (hi > lo) b += colour_bytes(D, s, grouped_nnz(D, lo, hi), hi - lo);
+            }
+        return b;
+    }
+
+    // the residual of every colour but the last (a zero written for each of
+    // the last colour's rows unless Rz skips them): matrix part, row list,
+    // u gathered once, f read and the residual written per row
+    double gs_residual_bytes(const AmgLevelDev& D, int s) const {
+        const uint32_t r = D.h_cptr[D.ncol - 1];
+        double b = (double)grouped_nnz(D, 0, r) * (s + 4) + 4.0 * (r + 1) + 4.0 * r + (double)D.n * s + 2.0 * r * s;
+        if (!D.Rz.n) b += (4.0 + s) * (D.n - r);
+        return b;
+    }
 };
 
 std::unique_ptr<Precond> make_amg_cf(CsrMat& A, const fem_pc_params& p, hipStream_t s) {

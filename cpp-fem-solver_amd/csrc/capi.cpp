@@ -314,6 +314,13 @@ int fem_precond_info(fem_pc M, int* levels, uint32_t* rows, uint64_t* nnz, int c
     });
 }
 
+int fem_precond_bytes(fem_pc M, int fp32, double* bytes) {
+    return guard([&] {
+        if (!M || !bytes) fail(FEM_ERR_INVALID_ARGUMENT, "fem_precond_bytes: null argument");
+        *bytes = M->pc->apply_bytes(fp32 ? 4 : 8);
+    });
+}
+
 int fem_solve(fem_csr A, fem_pc M, const double* b, double* x, const fem_solve_opts* opts, fem_solve_stats* stats) {
     return guard([&] {
         fem_solver_s h;

@@ -240,11 +240,22 @@ struct Precond {
     virtual unsigned apply_dot(const double*, double*, double*, hipStream_t, Gate = {}) { return 0; }
     virtual unsigned apply32_dot(const float*, float*, double*, hipStream_t, Gate = {}) { return 0; }
     virtual unsigned dot_parts() const { return 0; }
+    // Algorithmic bytes of one application with s-byte values: the sum over
+    // the kernels it launches, each reading its matrix (uint32 indices) and
+    // each vector once and writing its output once. 0: not tallied.
+    virtual double apply_bytes(int /*s*/) { return 0.0; }
     double setup_ms = 0.0;
     int levels = 0;
     std::vector<uint32_t> lvl_rows;
     std::vector<uint64_t> lvl_nnz;
 };
+
+// Algorithmic bytes of one CSR SpMV y = M x with s-byte values: the matrix
+// with uint32 indices, x and y once, and `extra` more vectors of M's rows
+// read by the epilogue.
+inline double csr_bytes(const CsrMat& M, int s, int extra) {
+    return (double)M.nnz * (s + 4) + 4.0 * (M.n + 1) + (double)M.m * s + (double)M.n * s * (1 + extra);
+}
 
 std::unique_ptr<Precond> make_precond(CsrMat& A, int kind, const fem_pc_params& p, hipStream_t s);
 

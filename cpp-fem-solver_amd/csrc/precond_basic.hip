@@ -88,6 +88,7 @@ struct PcDiag : Precond {
         hipLaunchKernelGGL((k_div<float>), dim3(ew_grid(n)), dim3(256), 0, s, r, d32.get(), z, (size_t)n, g);
         FEM_HIP(hipGetLastError());
     }
+    double apply_bytes(int s) override { return 3.0 * n * s; }  // r, d, z
     bool has_fp32() const override { return d32.get() != nullptr; }
     void ensure_fp32(hipStream_t s) override {
         if (d32.get()) return;

@@ -58,7 +58,8 @@ struct Sweep {
     DevBuf<uint16_t> len;
     DevBuf<double> val, diag;
     DevBuf<float> val32, diag32;
-    uint64_t nent = 0;
+    uint64_t nent = 0;  // SELL slots (rows padded to the widest of their group of 64)
+    uint64_t ndep = 0;  // dependency entries
     // chains: dense per-slot copies (entry e of slot s at cbase + (s - s0) kc + e)
     // of the columns, dependency slots and values, the SELL index each value
     // comes from, and right-hand-side staging in slot order
@@ -96,6 +97,7 @@ struct Sweep {
             g[gi + 1] = g[gi] + w * 64;
         }
         nent = g[ngrp];
+        ndep = n ? (uint64_t)rp[n] - rp[0] : 0;
         std::vector<uint32_t> slot_of(n);
         for (uint32_t s = 0; s < n; ++s) slot_of[order[s]] = s;
         std::vector<uint32_t> hc(nent, 0), hm(nent, UINT32_MAX), hd(nent, 0);
@@ -600,6 +602,13 @@ __global__ void k_ic0_level(uint32_t a, uint32_t b, const uint32_t* __restrict__
     diag[i] = r;
 }
 
+// Algorithmic bytes of one sweep: its off-diagonal entries (uint32 indices),
+// the row pointers, the right-hand side read and the result written, the
+// level list, and the diagonal when the sweep divides.
+static double sweep_bytes(const Sweep& w, int s, bool div) {
+    return (double)w.ndep * (s + 4) + 4.0 * (w.n + 1) + 2.0 * w.n * s + 4.0 * w.n + (div ? (double)w.n * s : 0.0);
+}
+
 struct PcIC0 : Precond {
     uint32_t n;
     Sweep fw, bw;       // forward L, backward L^T
@@ -699,6 +708,7 @@ struct PcIC0 : Precond {
         run_sweep<float, true>(fw, fw.val32.get(), fw.diag32.get(), r, y32.get(), s, g);
         run_sweep<float, true>(bw, bw.val32.get(), bw.diag32.get(), y32.get(), z, s, g);
     }
+    double apply_bytes(int s) override { return sweep_bytes(fw, s, true) + sweep_bytes(bw, s, true); }
     bool has_fp32() const override { return y32.get() != nullptr; }
     void ensure_fp32(hipStream_t s) override {
         if (y32.get()) return;
@@ -819,6 +829,7 @@ struct PcILU0 : Precond {
         run_sweep<float, false>(fw, fw.val32.get(), nullptr, r, y32.get(), s, g);
         run_sweep<float, true>(bw, bw.val32.get(), bw.diag32.get(), y32.get(), z, s, g);
     }
+    double apply_bytes(int s) override { return sweep_bytes(fw, s, false) + sweep_bytes(bw, s, true); }
     bool has_fp32() const override { return y32.get() != nullptr; }
     void ensure_fp32(hipStream_t s) override {
         if (y32.get()) return;

@@ -121,6 +121,7 @@ _SIGS = {
     "fem_precond_apply": (C.c_int, [P_vp, P_f64, P_f64]),
     "fem_precond_destroy": (C.c_int, [P_vp]),
     "fem_precond_info": (C.c_int, [P_vp, C.POINTER(C.c_int), P_u32, C.POINTER(C.c_uint64), C.c_int, P_f64]),
+    "fem_precond_bytes": (C.c_int, [P_vp, C.c_int, P_f64]),
     "fem_solve": (C.c_int, [P_vp, P_vp, P_f64, P_f64, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "fem_pcg": (C.c_int, [P_vp, P_vp, P_f64, C.c_double, C.c_int, P_f64, C.POINTER(C.c_int)]),
     "fem_bicgstab": (C.c_int, [P_vp, P_vp, P_f64, C.c_double, C.c_int, P_f64, C.POINTER(C.c_int)]),
@@ -379,6 +380,13 @@ class Preconditioner:
         _check(lib().fem_precond_info(self._h, C.byref(lv), _p(rows, C.c_uint32),
                                       nnz.ctypes.data_as(C.POINTER(C.c_uint64)), cap, C.byref(ms)))
         return {"levels": lv.value, "rows": rows, "nnz": nnz, "setup_ms": ms.value}
+
+    def apply_bytes(self, fp32=False):
+        """Algorithmic bytes of one application (fem_precond_bytes); 0 where
+        the kind is not tallied."""
+        out = C.c_double()
+        _check(lib().fem_precond_bytes(self._h, 1 if fp32 else 0, C.byref(out)))
+        return out.value
 
     def close(self):
         if self._h:

@@ -196,6 +196,12 @@ int fem_precond_destroy(fem_pc M);
  * (IC0/ILU0/block ILU0) or AMG levels; fills up to cap entries of rows/nnz
  * per AMG level. */
 int fem_precond_info(fem_pc M, int* levels, uint32_t* rows, uint64_t* nnz, int cap, double* setup_ms);
+/* Algorithmic bytes of one application of M in float64 (fp32 = 0) or float32:
+ * the sum over the kernels it launches, each reading its matrices (uint32
+ * indices) and vectors once and writing its output once. 0 for the kinds
+ * not tallied (tallied: diag, IC0, ILU0, AMG, SA-AMG). A measurement aid with no
+ * counterpart in the reference. */
+int fem_precond_bytes(fem_pc M, int fp32, double* bytes);
 
 /* ---- Krylov solvers ----------------------------------------------------- */
 /* General entry: solver, precision, graph capture and pointer space from opts.

@@ -217,3 +217,40 @@ def test_saamg_device_setup_equals_host_setup(case, smoother):
     assert np.array_equal(a["rows"], b["rows"]) and np.array_equal(a["nnz"], b["nnz"])
     r = np.random.default_rng(4).standard_normal(G.n)
     assert np.array_equal(Md.apply(r), Mh.apply(r))
+
+
+def _csr_bytes(nnz, n, m, s, extra):
+    return nnz * (s + 4) + 4 * (n + 1) + m * s + n * s * (1 + extra)
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_precond_bytes_tally(fp32):
+    """fem_precond_bytes: z = r / d moves 3 vectors; an IC(0) or ILU(0)
+    application two sweeps over the strictly triangular halves; for SA-AMG with the
+    Jacobi smoother, going from 1+1 to 2+2 smoothing sweeps adds exactly two
+    SpMV sweeps (A_k, x, y, f and d) on every level but the coarsest, so the
+    difference of the two tallies is fixed by the level sizes the hierarchy
+    reports (themselves checked against the oracle above)."""
+    G = gen.poisson3d(32)
+    A = F.CsrMatrix.from_gen(G)
+    s = 4 if fp32 else 8
+    D = F.Preconditioner(A, "diag", fp32=fp32)
+    assert D.apply_bytes(fp32=bool(fp32)) == 3 * G.n * s
+    # IC(0) and ILU(0) of a symmetric pattern: each sweep has (nnz - n) / 2
+    # dependency entries; both IC(0) sweeps and the ILU(0) upper sweep divide
+    half = (G.nnz - G.n) // 2
+    sweep = half * (s + 4) + 4 * (G.n + 1) + 2 * G.n * s + 4 * G.n
+    for kind, ndiv in (("ic0", 2), ("ilu0", 1)):
+        T = F.Preconditioner(A, kind, fp32=fp32)
+        assert T.apply_bytes(fp32=bool(fp32)) == 2 * sweep + ndiv * G.n * s, kind
+    kw = dict(smoother=F.SMOOTH_JACOBI, coarse_exact=1, omega=0.0, p_omega=0.0, fp32=fp32)
+    M1 = F.Preconditioner(A, "saamg", npre=1, npost=1, **kw)
+    M2 = F.Preconditioner(A, "saamg", npre=2, npost=2, **kw)
+    a, b = M1.info(), M2.info()
+    L = b["levels"]
+    assert L == a["levels"] > 2
+    assert np.array_equal(a["rows"], b["rows"]) and np.array_equal(a["nnz"], b["nnz"])
+    want = sum(2 * _csr_bytes(int(b["nnz"][k]), int(b["rows"][k]), int(b["rows"][k]), s, 2) for k in range(L - 1))
+    b1, b2 = M1.apply_bytes(fp32=bool(fp32)), M2.apply_bytes(fp32=bool(fp32))
+    assert b1 > _csr_bytes(G.nnz, G.n, G.n, s, 0)
+    assert b2 - b1 == want
