@@ -35,12 +35,13 @@ namespace fem {
 template <typename T, bool INV>
 __global__ void k_jacobi0(const T* __restrict__ f, const T* __restrict__ d, T* __restrict__ u, uint32_t n, T w,
                           Gate g) {
-    if (g.closed()) return;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const T r = f[i] - (T)0;
-    if constexpr (INV) u[i] = (T)0 + w * (d[i] * r);
-    else u[i] = (T)0 + (w * r) / d[i];
+    const T fi = f[i], di = d[i];  // read while the gate's flag is in flight
+    if (g.closed()) return;
+    const T r = fi - (T)0;
+    if constexpr (INV) u[i] = (T)0 + w * (di * r);
+    else u[i] = (T)0 + (w * r) / di;
 }
 
 // Multicolour Gauss-Seidel on a colour-grouped copy of the level matrix:
@@ -107,17 +108,19 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ gcbase, const T* __restrict__ gv,
                                                       const T* __restrict__ gd, const T* __restrict__ f, T* u, T w,
                                                       T* res, uint32_t zero_from, Gate g) {
-    if (g.closed()) return;
     constexpr int STAGE = stage_nnz<T>();
     __shared__ T prod[STAGE];
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    // the block's bounds are read while the gate's flag is in flight (reads
+    // only: the gate guards the stores)
     const uint32_t r0 = blk[bid], r1 = blk[bid + 1];
+    const uint32_t k0 = gp[r0], k1 = gp[r1];
+    if (g.closed()) return;
     const uint32_t t = threadIdx.x;
     if (MODE == 1 && bid >= zero_from) {
         for (uint32_t row = r0 + t; row < r1; row += kBlock) res[rows[row]] = (T)0;
         return;
     }
-    const uint32_t k0 = gp[r0], k1 = gp[r1];
     const uint32_t cnt = k1 - k0;
     if (cnt <= (uint32_t)STAGE) {
         // a block holds at most stage_rows<T>() rows: RPT per thread, whose
@@ -185,10 +188,10 @@ __global__ void k_gs_first(const uint32_t* __restrict__ rows, uint32_t n, uint32
 template <typename T>
 __global__ __launch_bounds__(256) void k_coarse(const double* __restrict__ C, uint32_t nc, const T* __restrict__ f,
                                                 T* __restrict__ u, Gate g) {
-    if (g.closed()) return;
     extern __shared__ double fs[];
     for (uint32_t j = threadIdx.x; j < nc; j += blockDim.x) fs[j] = (double)f[j];
     __syncthreads();
+    if (g.closed()) return;  // uniform: tested after the staging, before any store to u
     const uint32_t i = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (i >= nc) return;
     const double* row = C + (size_t)i * nc;

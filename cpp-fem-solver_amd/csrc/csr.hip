@@ -94,12 +94,14 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
                                                  const T* __restrict__ x, T* __restrict__ y,
                                                  const uint32_t* __restrict__ blk_row, SpmvArgs<T> a, Gate g,
                                                  double* __restrict__ part = nullptr) {
-    if (g.closed()) return;
     constexpr int STAGE = stage_nnz<T>();
     __shared__ T prod[STAGE];
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    // the block's bounds are read while the gate's flag is in flight (reads
+    // only: the gate guards the stores)
     const uint32_t r0 = blk_row[bid], r1 = blk_row[bid + 1];
     const uint32_t k0 = rp[r0], k1 = rp[r1];
+    if (g.closed()) return;
     const uint32_t cnt = k1 - k0;
     const uint32_t t = threadIdx.x;
     if (cnt <= (uint32_t)STAGE) {
@@ -186,12 +188,16 @@ template <typename T, int EPI, int LANES>
 __global__ __launch_bounds__(kBlock) void k_spmv_vec(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
                                                      const T* __restrict__ v, const T* __restrict__ x,
                                                      T* __restrict__ y, uint32_t n, SpmvArgs<T> a, Gate g) {
-    if (g.closed()) return;
     const uint32_t row = (blockIdx.x * (uint32_t)kBlock + threadIdx.x) / LANES;
     const int lane = threadIdx.x % LANES;
+    uint32_t k0 = 0, k1 = 0;
+    if (row < n) {  // read while the gate's flag is in flight
+        k0 = rp[row];
+        k1 = rp[row + 1];
+    }
+    if (g.closed()) return;
     T s = (T)0;
-    if (row < n)
-        for (uint32_t k = rp[row] + lane; k < rp[row + 1]; k += LANES) s += v[k] * x[ci[k]];
+    for (uint32_t k = k0 + lane; k < k1; k += LANES) s += v[k] * x[ci[k]];
 #pragma unroll
     for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
     if (row < n && lane == 0) y[row] = spmv_epilogue<T, EPI>(row, s, a);

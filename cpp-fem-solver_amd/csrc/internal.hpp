@@ -19,11 +19,17 @@ namespace fem {
 __device__ __forceinline__ int flag_load(const int* p) {
     return *(const __attribute__((address_space(4))) int*)p;
 }
+// flags read in place of an absent gate pointer
+static __device__ const int kFlagClear = 0;
+static __device__ const int kFlagSet = 1;
 struct Gate {
     const int* done = nullptr;
     const int* need = nullptr;
     __device__ __forceinline__ bool closed() const {
-        return (done && flag_load(done)) || (need && !flag_load(need));
+        // both flags are loaded before either is tested (one round trip)
+        const int d = flag_load(done ? done : &kFlagClear);
+        const int r = flag_load(need ? need : &kFlagSet);
+        return (d != 0) | (r == 0);
     }
 };
 
