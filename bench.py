@@ -421,6 +421,11 @@ def main():
             "iterations": st["iterations"], "refinements": st["refinements"], "relres": relres, "rel_err": err,
             "levels": info["levels"],
         }
+        if kind == "saamg":
+            L = info["levels"]
+            results[name]["level_rows"] = [int(v) for v in info["rows"][:L]]
+            results[name]["level_nnz"] = [int(v) for v in info["nnz"][:L]]
+            log(f"[bench] {name}: levels rows {results[name]['level_rows']} nnz {results[name]['level_nnz']}")
         pc_bytes = M.apply_bytes(fp32=prec == F.FP32_IR)
         if pc_bytes > 0:
             # one inner PCG iteration: q = A p (with p.q), x += a p and r -= a q
