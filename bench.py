@@ -431,10 +431,12 @@ def main():
             # one inner PCG iteration: q = A p (with p.q), x += a p and r -= a q
             # (6 vectors), z = M r, r.z (2), p = z + b p (3). With the diagonal
             # the update kernels form r / d themselves: d read in both, r in
-            # the p update, no z and no separate r.z. The solve time also holds
-            # the float64 refinement steps.
+            # the p update, no z and no separate r.z. The SA-AMG V-cycle with
+            # Gauss-Seidel post-smoothing yields r.z from its last passes. The
+            # solve time also holds the float64 refinement steps.
             s_b = 4 if prec == F.FP32_IR else 8
-            vec = 11 * G.n * s_b + (0 if kind == "diag" else pc_bytes)
+            rz_fused = kind == "saamg" and prm.get("smoother") == F.SMOOTH_SGS and prm.get("npost", 1) >= 1
+            vec = (9 if rz_fused else 11) * G.n * s_b + (0 if kind == "diag" else pc_bytes)
             it_bytes = spmv_bytes(G.nnz, G.n, G.m, s_b) + vec
             gbs = st["iterations"] * it_bytes / (np.median(times) * 1e-3) / 1e9
             results[name]["solve_level"] = {

@@ -99,7 +99,10 @@ __global__ __launch_bounds__(1024) void k_gs_small(int ncol, const uint32_t* __r
 // rows instead, and blocks from `zero_from` on write res = 0 without reading
 // anything (the last colour right after a forward sweep with weight 1, whose
 // residual is zero in exact arithmetic).
-template <typename T, int MODE, bool PK>
+// DOT (MODE 0): the block also writes part[bid] = sum over its rows of
+// f_i * u_i (float64, the new u_i): with f = r on the finest level and the
+// last pass of each colour, the partial sums of r.z.
+template <typename T, int MODE, bool PK, bool DOT = false>
 __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict__ blk,
                                                       const uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ gp,
@@ -107,7 +110,8 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
                                                       const uint16_t* __restrict__ gcd,
                                                       const uint32_t* __restrict__ gcbase, const T* __restrict__ gv,
                                                       const T* __restrict__ gd, const T* __restrict__ f, T* u, T w,
-                                                      T* res, uint32_t zero_from, Gate g) {
+                                                      T* res, uint32_t zero_from, Gate g,
+                                                      double* __restrict__ part) {
     constexpr int STAGE = stage_nnz<T>();
     __shared__ T prod[STAGE];
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
@@ -146,25 +150,40 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
         }
         if (cnt > 0) stage_products<T, STAGE, PK>(gci, gcd, gcbase, gv, u, k0, cnt, prod);
         __syncthreads();
+        double dsum[1] = {0.0};
 #pragma unroll
         for (int j = 0; j < RPT; ++j) {
             const uint32_t row = r0 + t + (uint32_t)j * kBlock;
             if (row < r1) {
                 T s = (T)0;
                 for (uint32_t k = rb[j]; k < re[j]; ++k) s += prod[k];
-                if (MODE == 0) u[ri[j]] = ui[j] + (w * (fi[j] - s)) / di[j];
-                else res[ri[j]] = fi[j] - s;
+                if (MODE == 0) {
+                    const T un = ui[j] + (w * (fi[j] - s)) / di[j];
+                    u[ri[j]] = un;
+                    if (DOT) dsum[0] += (double)fi[j] * (double)un;
+                } else {
+                    res[ri[j]] = fi[j] - s;
+                }
             }
+        }
+        if constexpr (DOT) {
+            block_sum<1>(dsum);
+            if (t == 0) part[bid] = dsum[0];
         }
     } else {
         double s = 0.0;
         for (uint32_t k = k0 + t; k < k1; k += kBlock) s += (double)(gv[k] * u[col_of<PK>(gci, gcd, gcbase, k)]);
         double tot[1] = {s};
         block_sum<1>(tot);
-        if (t == 0) {
+        if (t == 0) {  // a block of one long row
             const uint32_t i = rows[r0];
-            if (MODE == 0) u[i] = u[i] + (w * (f[i] - (T)tot[0])) / gd[r0];
-            else res[i] = f[i] - (T)tot[0];
+            if (MODE == 0) {
+                const T un = u[i] + (w * (f[i] - (T)tot[0])) / gd[r0];
+                u[i] = un;
+                if (DOT) part[bid] = (double)f[i] * (double)un;
+            } else {
+                res[i] = f[i] - (T)tot[0];
+            }
         }
     }
 }
@@ -350,9 +369,11 @@ struct PcAmg : Precond {
     }
 
     // `from_zero`: u has not been initialised; the sweep starts from u = 0
+    // `part` (non-null): the last sweep's passes also write the partial sums
+    // of f.u, colour c's blocks at part + cblk[c] (stream kernels only)
     template <typename T, int LANES>
     void gs_launch(AmgLevelDev& D, const T* gv, const T* gd, const T* f, T* u, int sweeps, bool reverse,
-                   bool from_zero, hipStream_t s, Gate g) {
+                   bool from_zero, hipStream_t s, Gate g, double* part) {
         const T w = (T)D.omega;
         if (D.n <= 8192) {
             if (from_zero) hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, u, D.n, g);
@@ -372,21 +393,33 @@ struct PcAmg : Precond {
                 }
                 const uint32_t b0 = cblk<T>(D)[c], b1 = cblk<T>(D)[c + 1];
                 if (b0 >= b1) continue;
-                if (D.gcd.get())
-                    hipLaunchKernelGGL((k_gs_stream<T, 0, true>), dim3(b1 - b0), dim3(kBlock), 0, s, gblk<T>(D) + b0,
-                                       D.crows.get(), D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, gd, f,
-                                       u, w, (T*)nullptr, 0u, g);
-                else
-                    hipLaunchKernelGGL((k_gs_stream<T, 0, false>), dim3(b1 - b0), dim3(kBlock), 0, s, gblk<T>(D) + b0,
-                                       D.crows.get(), D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, gd, f,
-                                       u, w, (T*)nullptr, 0u, g);
+                const dim3 grid(b1 - b0), block(kBlock);
+                const uint32_t* blk = gblk<T>(D) + b0;
+                if (part && sw == sweeps - 1) {
+                    if (D.gcd.get())
+                        hipLaunchKernelGGL((k_gs_stream<T, 0, true, true>), grid, block, 0, s, blk, D.crows.get(),
+                                           D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, gd, f, u, w,
+                                           (T*)nullptr, 0u, g, part + b0);
+                    else
+                        hipLaunchKernelGGL((k_gs_stream<T, 0, false, true>), grid, block, 0, s, blk, D.crows.get(),
+                                           D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, gd, f, u, w,
+                                           (T*)nullptr, 0u, g, part + b0);
+                } else if (D.gcd.get()) {
+                    hipLaunchKernelGGL((k_gs_stream<T, 0, true>), grid, block, 0, s, blk, D.crows.get(), D.gp.get(),
+                                       D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, gd, f, u, w, (T*)nullptr, 0u, g,
+                                       (double*)nullptr);
+                } else {
+                    hipLaunchKernelGGL((k_gs_stream<T, 0, false>), grid, block, 0, s, blk, D.crows.get(), D.gp.get(),
+                                       D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, gd, f, u, w, (T*)nullptr, 0u, g,
+                                       (double*)nullptr);
+                }
             }
         if (from_zero && sweeps == 0) hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, u, D.n, g);
     }
 
     template <typename T>
     void gs(AmgLevelDev& D, const T* f, T* u, int sweeps, bool reverse, hipStream_t s, Gate g,
-            bool from_zero = false) {
+            bool from_zero = false, double* part = nullptr) {
         if (sweeps <= 0 && !from_zero) return;
         const T *gv, *gd;
         if constexpr (std::is_same<T, double>::value) {
@@ -397,12 +430,12 @@ struct PcAmg : Precond {
             gd = D.gd32.get();
         }
         switch (D.lanes) {
-            case 1: gs_launch<T, 1>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
-            case 4: gs_launch<T, 4>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
-            case 8: gs_launch<T, 8>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
-            case 16: gs_launch<T, 16>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
-            case 32: gs_launch<T, 32>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
-            default: gs_launch<T, 64>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g); break;
+            case 1: gs_launch<T, 1>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g, part); break;
+            case 4: gs_launch<T, 4>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g, part); break;
+            case 8: gs_launch<T, 8>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g, part); break;
+            case 16: gs_launch<T, 16>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g, part); break;
+            case 32: gs_launch<T, 32>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g, part); break;
+            default: gs_launch<T, 64>(D, gv, gd, f, u, sweeps, reverse, from_zero, s, g, part); break;
         }
     }
 
@@ -425,11 +458,11 @@ struct PcAmg : Precond {
         if (D.gcd.get())
             hipLaunchKernelGGL((k_gs_stream<T, 1, true>), dim3(nb), dim3(kBlock), 0, s, gblk<T>(D), D.crows.get(),
                                D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, (const T*)nullptr, f, u, (T)1,
-                               res, zero_from, g);
+                               res, zero_from, g, (double*)nullptr);
         else
             hipLaunchKernelGGL((k_gs_stream<T, 1, false>), dim3(nb), dim3(kBlock), 0, s, gblk<T>(D), D.crows.get(),
                                D.gp.get(), D.gci.get(), D.gcd.get(), D.gcbase.get(), gv, (const T*)nullptr, f, u, (T)1,
-                               res, zero_from, g);
+                               res, zero_from, g, (double*)nullptr);
         return true;
     }
 
@@ -441,8 +474,10 @@ struct PcAmg : Precond {
     bool jinv() const { return smoother == FEM_SMOOTH_SGS ? false : inv_diag; }
     double jwt(const AmgLevelDev& D) const { return smoother == FEM_SMOOTH_SGS ? D.jw : D.omega; }
 
+    // part (non-null, dot_fused()): the finest level's last smoothing sweep
+    // also writes the partial sums of r.z
     template <typename T>
-    void vcycle(const T* r, T* z, hipStream_t s, Gate g) {
+    void vcycle(const T* r, T* z, hipStream_t s, Gate g, double* part = nullptr) {
         const size_t L = lv.size();
         std::vector<T*> ucur(L, nullptr), xa(L, nullptr), xb(L, nullptr);
         std::vector<const T*> fk(L, nullptr);
@@ -501,7 +536,7 @@ struct PcAmg : Precond {
             SpmvArgs<T> ar;
             ar.u = u;
             csr_spmv<T>(D.P, ucur[k + 1], u, EPI_ADD, ar, s, g);
-            if (smoother == FEM_SMOOTH_SGS && D.use_gs) gs<T>(D, fk[k], u, npost, true, s, g);
+            if (smoother == FEM_SMOOTH_SGS && D.use_gs) gs<T>(D, fk[k], u, npost, true, s, g, false, k == 0 ? part : nullptr);
             else u = jacobi<T>(D, fk[k], jd(D, B), jinv(), jwt(D), u, xa[k], xb[k], npost, s, g);
             ucur[k] = u;
         }
@@ -510,6 +545,29 @@ struct PcAmg : Precond {
 
     void apply(const double* r, double* z, hipStream_t s, Gate g) override { vcycle<double>(r, z, s, g); }
     void apply32(const float* r, float* z, hipStream_t s, Gate g) override { vcycle<float>(r, z, s, g); }
+
+    // r.z from the V-cycle itself: the finest level's post-smoothing ends
+    // with one stream pass per colour, which reads f = r and writes each
+    // row's final z (rows of one colour are written by that colour's pass
+    // only), so those passes' blocks give the partial sums in block order.
+    bool dot_fused() const {
+        if (smoother != FEM_SMOOTH_SGS || npost < 1 || lv.size() < 2) return false;
+        const AmgLevelDev& D = *lv[0];
+        return D.use_gs && D.n > 8192 && !D.h_cblk.empty();
+    }
+    unsigned dot_parts() const override { return dot_fused() ? lv[0]->h_cblk.back() : 0; }
+    unsigned apply_dot(const double* r, double* z, double* part, hipStream_t s, Gate g) override {
+        if (!dot_fused()) return 0;
+        vcycle<double>(r, z, s, g, part);
+        return lv[0]->h_cblk.back();
+    }
+    unsigned apply32_dot(const float* r, float* z, double* part, hipStream_t s, Gate g) override {
+        // the float32 blocking never has more blocks than the float64 one
+        // that sized the partial buffer; checked here all the same
+        if (!dot_fused() || lv[0]->h_cblk32.empty() || lv[0]->h_cblk32.back() > lv[0]->h_cblk.back()) return 0;
+        vcycle<float>(r, z, s, g, part);
+        return lv[0]->h_cblk32.back();
+    }
 
     // Algorithmic bytes of one V-cycle, kernel by kernel in the order vcycle
     // launches them (s-byte values, uint32 indices).

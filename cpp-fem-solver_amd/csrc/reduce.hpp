@@ -63,7 +63,22 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
     double s[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = 0.0;
-    for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x) {
+    // a thread adds partials t, t + blockDim, t + 2 blockDim, ... in that
+    // order; eight of them are loaded before they are added
+    constexpr unsigned U = 8;
+    unsigned b = threadIdx.x;
+    for (; b + (U - 1) * blockDim.x < nparts; b += U * blockDim.x) {
+        double v[U][NV];
+#pragma unroll
+        for (unsigned u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) v[u][i] = partials[(b + u * blockDim.x) * NV + i];
+#pragma unroll
+        for (unsigned u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) s[i] += v[u][i];
+    }
+    for (; b < nparts; b += blockDim.x) {
 #pragma unroll
         for (int i = 0; i < NV; ++i) s[i] += partials[b * NV + i];
     }

@@ -254,3 +254,27 @@ def test_precond_bytes_tally(fp32):
     b1, b2 = M1.apply_bytes(fp32=bool(fp32)), M2.apply_bytes(fp32=bool(fp32))
     assert b1 > _csr_bytes(G.nnz, G.n, G.n, s, 0)
     assert b2 - b1 == want
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_saamg_sgs_pcg_fused_rz_parity(prec):
+    """SA-AMG with the Gauss-Seidel smoother on a level above 8192 rows: the
+    V-cycle's last colour passes yield PCG's r.z (partial sums per block, in
+    block order) instead of a separate reduction. Float64: the iteration count
+    and solution against the oracle's PCG with the same V-cycle; float32
+    inner / float64 refinement: convergence to 1e-10 and the solution within
+    1e-8 of the oracle's."""
+    G = gen.poisson3d(32)
+    _, b = gen.rhs(G, 3)
+    A = F.CsrMatrix.from_gen(G)
+    Ao = O.Csr.from_gen(G)
+    fp32 = prec == "fp32"
+    M = F.Preconditioner(A, "saamg", smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0,
+                         coarse_exact=1, fp32=int(fp32))
+    Mo = O.Pc(Ao, O.K_SAAMG, [0.25, 1.0, 2, 2, 20, 4, 50, 0.0, 1, 1])
+    x, st = F.solve(A, M, b, solver="pcg", precision=F.FP32_IR if fp32 else F.FP64)
+    xo, ito = O.solve(Ao, 0, Mo, b)
+    assert st["converged"] == 1 and st["true_relres"] < 1.5e-10
+    assert rel(x, xo) < 1e-8
+    if not fp32:
+        assert abs(st["iterations"] - ito) <= 1
