@@ -166,7 +166,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xr_f(T* x, T* r, const T* p, con
                                                      unsigned npq) {
     DONE_RET(st);
     double t[1] = {0.0};
-    for (unsigned b = threadIdx.x; b < npq; b += blockDim.x) t[0] += pq_part[b];
+    add_partials<1>(pq_part, npq, t);
     block_sum<1>(t);
     if (blockIdx.x == 0 && threadIdx.x == 0) FinPcgPq{st}(t);
     const T a = (T)(st->rz / t[0]);
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_f(const T* r, const T* z, cons
                                                     const double* __restrict__ rz_part, unsigned nrz) {
     DONE_RET(st);
     double t[1] = {0.0};
-    for (unsigned b = threadIdx.x; b < nrz; b += blockDim.x) t[0] += rz_part[b];
+    add_partials<1>(rz_part, nrz, t);
     block_sum<1>(t);
     if (blockIdx.x == 0 && threadIdx.x == 0) FinPcgRz{st}(t);
     const T b = (T)(t[0] / st->rz_prev);

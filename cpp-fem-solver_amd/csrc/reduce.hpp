@@ -19,6 +19,27 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
+// s[i] += partials[b * NV + i] for b = t, t + blockDim, t + 2 blockDim, ...
+// (t = threadIdx.x) in that order. Eight are loaded before they are added;
+// past the end a load gives 0.0, whose addition changes no sum.
+template <int NV>
+__device__ __forceinline__ void add_partials(const double* __restrict__ partials, unsigned nparts, double (&s)[NV]) {
+    constexpr unsigned U = 8;
+    for (unsigned b = threadIdx.x; b < nparts; b += U * blockDim.x) {
+        double v[U][NV];
+#pragma unroll
+        for (unsigned u = 0; u < U; ++u) {
+            const unsigned k = b + u * blockDim.x;
+#pragma unroll
+            for (int i = 0; i < NV; ++i) v[u][i] = k < nparts ? partials[k * NV + i] : 0.0;
+        }
+#pragma unroll
+        for (unsigned u = 0; u < U; ++u)
+#pragma unroll
+            for (int i = 0; i < NV; ++i) s[i] += v[u][i];
+    }
+}
+
 // Block-wide sum of NV values; the result is valid in every thread.
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
@@ -63,25 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(const double* __restrict__ 
     double s[NV];
 #pragma unroll
     for (int i = 0; i < NV; ++i) s[i] = 0.0;
-    // a thread adds partials t, t + blockDim, t + 2 blockDim, ... in that
-    // order; eight of them are loaded before they are added
-    constexpr unsigned U = 8;
-    unsigned b = threadIdx.x;
-    for (; b + (U - 1) * blockDim.x < nparts; b += U * blockDim.x) {
-        double v[U][NV];
-#pragma unroll
-        for (unsigned u = 0; u < U; ++u)
-#pragma unroll
-            for (int i = 0; i < NV; ++i) v[u][i] = partials[(b + u * blockDim.x) * NV + i];
-#pragma unroll
-        for (unsigned u = 0; u < U; ++u)
-#pragma unroll
-            for (int i = 0; i < NV; ++i) s[i] += v[u][i];
-    }
-    for (; b < nparts; b += blockDim.x) {
-#pragma unroll
-        for (int i = 0; i < NV; ++i) s[i] += partials[b * NV + i];
-    }
+    add_partials<NV>(partials, nparts, s);
     block_sum<NV>(s);
     if (threadIdx.x == 0) fin(s);
 }
