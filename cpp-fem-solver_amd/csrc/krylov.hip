@@ -931,9 +931,9 @@ void KrylovSolver::gm_cycle(KWork<T>& W) {
     LAUNCH(k_gm_cycle_end, 1, st);
 }
 
-void KrylovSolver::replay(GraphCache& gc, int tag, const std::function<void()>& body) {
+void KrylovSolver::replay(GraphCache& gc, int tag, const std::function<void()>& body, bool launch) {
     if (!opts.use_graph || (M_ && !M_->graph_safe())) {
-        body();
+        if (launch) body();
         return;
     }
     if (gc.tag != tag) {
@@ -951,7 +951,7 @@ void KrylovSolver::replay(GraphCache& gc, int tag, const std::function<void()>& 
         FEM_HIP(hipGraphInstantiate(&gc.exec, gc.graph, nullptr, nullptr, 0));
         gc.tag = tag;
     }
-    FEM_HIP(hipGraphLaunch(gc.exec, s_));
+    if (launch) FEM_HIP(hipGraphLaunch(gc.exec, s_));
 }
 
 void KrylovSolver::read_state() {
@@ -991,7 +991,9 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
     const int cmax = opts.check_every > 0 ? opts.check_every : (W.n < 200000 ? 32 : 16);
     GraphSet& gs = std::is_same<T, double>::value ? g64_ : g32_;
     const double rel0 = hst_.p->normb > 0 ? hst_.p->normr / hst_.p->normb : 1.0;
-    int chunk = opts.check_every > 0 ? cmax : std::min(cmax, 8);
+    const int phase = phase_++;
+    const int hint = solver == FEM_SOLVER_PCG && phase < (int)hint_.size() ? hint_[phase] : 0;
+    int chunk = opts.check_every > 0 ? cmax : std::min(cmax, hint > 0 ? hint : 8);
     auto body = [&] {
         if (solver == FEM_SOLVER_PCG) for (int k = 0; k < chunk; ++k) pcg_iter<T>(W);
         else if (solver == FEM_SOLVER_BICGSTAB) for (int k = 0; k < chunk; ++k) bi_iter<T>(W);
@@ -1000,10 +1002,15 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
     // every replay runs >= 1 iteration (GMRES: one cycle of >= 1 step)
     const long long guard_max = (long long)max_iter + 4;
     long long rounds = 0;
+    static const bool trace = std::getenv("FEM_TRACE_SOLVE") != nullptr;
     while (!hst_.p->done) {
         const int tag = solver * 100000 + (solver == FEM_SOLVER_GMRES ? W.m : chunk);
         replay(gs.at(tag), tag, body);
         read_state();
+        if (trace)
+            std::fprintf(stderr, "[solve] replay of %d: iterations %d, |r|/|b| %.3e, done %d\n",
+                         solver == FEM_SOLVER_GMRES ? W.m : chunk, hst_.p->it,
+                         hst_.p->normb > 0 ? hst_.p->normr / hst_.p->normb : 0.0, hst_.p->done);
         if (++rounds > guard_max) fail(FEM_ERR_RUNTIME, "solver made no progress");
         if (solver == FEM_SOLVER_GMRES || opts.check_every > 0 || hst_.p->done) continue;
         const double rel = hst_.p->normb > 0 ? hst_.p->normr / hst_.p->normb : 0.0;
@@ -1017,9 +1024,25 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
                 while (next < cmax && next < 1.1 * left + 1) next *= 2;
             }
         }
+        // the previous solve's run needed `hint` iterations: end a chunk there
+        const int left_hint = hint - hst_.p->iters;
+        if (left_hint > 0) next = std::min(cmax, left_hint);
         chunk = next;
     }
     read_state();
+    if (solver == FEM_SOLVER_PCG && hst_.p->done == 1 && phase < 64) {
+        if ((int)hint_.size() <= phase) hint_.resize(phase + 1, 0);
+        hint_[phase] = hst_.p->iters;
+        // capture now the chunks a repeated solve replays (cmax-long ones and
+        // the remainder), so that solve captures nothing
+        if (opts.check_every <= 0)
+            for (const int c : {std::min(cmax, hint_[phase]), hint_[phase] % cmax}) {
+                if (c <= 0) continue;
+                chunk = c;
+                const int tag = solver * 100000 + c;
+                replay(gs.at(tag), tag, body, false);
+            }
+    }
     if (hst_.p->done == 2) {
         char ctx[128];
         std::snprintf(ctx, sizeof ctx, " (iteration %d, ||r||/||b|| = %.3e)", hst_.p->it,
@@ -1081,6 +1104,7 @@ void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
 }
 
 void KrylovSolver::solve_once(const double* b, double* x, fem_solve_stats* stats) {
+    phase_ = 0;
     const uint32_t n = A_->rows();
     const int m = opts.solver == FEM_SOLVER_GMRES ? opts.restart : 0;
     fem_solve_stats S{};

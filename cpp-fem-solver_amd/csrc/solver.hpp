@@ -68,7 +68,9 @@ class KrylovSolver {
     unsigned precond_dot(const T* r, T* z, Gate g);
     template <typename T>
     unsigned matvec_dot(const T* x, T* y, Gate g);
-    void replay(GraphCache& gc, int tag, const std::function<void()>& body);
+    // captures body into gc (unless gc holds tag already), then launches it
+    // unless `launch` is false
+    void replay(GraphCache& gc, int tag, const std::function<void()>& body, bool launch = true);
     void read_state();
     double true_residual(const double* b, const double* x, double* work);
 
@@ -82,6 +84,12 @@ class KrylovSolver {
     KWork<float> w32_;
     DevBuf<double> res64_, x64_, ax64_;
     GraphSet g64_, g32_;
+    // PCG: the iterations each Krylov run of the previous solve needed, in
+    // call order (one run per refinement step); a run with an entry replays
+    // chunks that end there, so a repeated solve of the same system
+    // launches no iteration past convergence and no extra replay
+    std::vector<int> hint_;
+    int phase_ = 0;
     DevBuf<double> st_work_, st_zero_, st_out_;   // end-of-solve statistics
     DevBuf<double> tr_out_, nrm_out_;             // single scalars read back by the host
     DevBuf<double> dotp_;                         // partial sums of the fused product + dot kernels
