@@ -977,7 +977,10 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
         LAUNCH((k_gm_init<T>), G, W.b.get(), W.x.get(), (size_t)W.n, st_.get(), part);
         FIN(1, FinGmInit{st_.get()});
     }
-    read_state();
+    // PCG starts from x = 0, so ||r0|| = ||b||: its first chunk is queued
+    // behind the init kernels without a host read (a solve that init marks
+    // done replays one chunk of gated-off kernels)
+    if (solver != FEM_SOLVER_PCG || max_iter <= 0) read_state();
     if (max_iter <= 0 && solver != FEM_SOLVER_GMRES) {
         // the reference's loop body never runs: x stays 0
         return 0;
@@ -990,7 +993,7 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
     // restart cycle at a time.
     const int cmax = opts.check_every > 0 ? opts.check_every : (W.n < 200000 ? 32 : 16);
     GraphSet& gs = std::is_same<T, double>::value ? g64_ : g32_;
-    const double rel0 = hst_.p->normb > 0 ? hst_.p->normr / hst_.p->normb : 1.0;
+    const double rel0 = solver == FEM_SOLVER_PCG ? 1.0 : hst_.p->normb > 0 ? hst_.p->normr / hst_.p->normb : 1.0;
     const int phase = phase_++;
     const int hint = solver == FEM_SOLVER_PCG && phase < (int)hint_.size() ? hint_[phase] : 0;
     int chunk = opts.check_every > 0 ? cmax : std::min(cmax, hint > 0 ? hint : 8);
