@@ -278,3 +278,33 @@ def test_saamg_sgs_pcg_fused_rz_parity(prec):
     assert rel(x, xo) < 1e-8
     if not fp32:
         assert abs(st["iterations"] - ito) <= 1
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_repeated_solve_replays_recorded_chunks(prec):
+    """A solver object keeps the iterations each Krylov run of its last solve
+    needed and ends its graph chunks there: a repeated solve of the same
+    system gives the same iterations and the same solution bit for bit, and
+    a solve with another right-hand side (whose runs need other counts) still
+    converges to the oracle's solution."""
+    G = gen.poisson3d(32)
+    A = F.CsrMatrix.from_gen(G)
+    Ao = O.Csr.from_gen(G)
+    fp32 = prec == "fp32"
+    M = F.Preconditioner(A, "saamg", smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0,
+                         coarse_exact=1, fp32=int(fp32))
+    S = F.Solver(A, M, solver="pcg", precision=F.FP32_IR if fp32 else F.FP64)
+    _, b = gen.rhs(G, 5)
+    x1, s1 = S.solve(b)
+    x2, s2 = S.solve(b)
+    assert s1["converged"] == 1 and s2["converged"] == 1
+    assert s1["iterations"] == s2["iterations"] and s1["refinements"] == s2["refinements"]
+    assert np.array_equal(x1, x2)
+    _, b2 = gen.rhs(G, 6)
+    b2 = b2 * np.linspace(0.5, 2.0, G.n)
+    x3, s3 = S.solve(b2)
+    Mo = O.Pc(Ao, O.K_SAAMG, [0.25, 1.0, 2, 2, 20, 4, 50, 0.0, 1, 1])
+    xo, _ = O.solve(Ao, 0, Mo, b2)
+    assert s3["converged"] == 1 and s3["true_relres"] < 1.5e-10
+    assert rel(x3, xo) < 1e-8
+    S.close()
