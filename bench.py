@@ -332,7 +332,7 @@ def main():
                          "for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE passes")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only",
+        print(json.dumps({"error": "runs on one GPU only; sharding across GPUs is designed, not implemented",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 

@@ -1,4 +1,4 @@
-// smatrix_solvers.cpp — driver in the style of the reference's
+// smatrix_solvers.cpp — a program in the style of the reference's
 // test_SMatrix_pcg/bicgstab/gmres programs, written against the drop-in
 // sparse::SMatrix facade: assembles matrices through the COO constructor,
 // runs every solver x preconditioner combination the reference offers,
@@ -74,7 +74,7 @@ int main() {
     // Without a strong preconditioner BiCGSTAB on this convection-dominated
     // matrix passes through near-breakdowns whose outcome depends on the
     // summation order even for the reference (its OpenMP thread count), so
-    // the driver checks the robust combinations.
+    // this program checks the robust combinations.
     for (const char* pc : {"ilu0", "amg"}) {
         std::string name = std::string("bicgstab/") + pc;
         run(name.c_str(), Cd, b, 1e-10, [&] { return SMatrix::bicgstab(Cd, b, pc, 1e-10, 5000); });

@@ -1,7 +1,8 @@
 // fem_oracle.cpp — CPU restatement of the reference's sparse solve path.
 //
-// TEST INFRASTRUCTURE ONLY. This file is the checker: only tests/, the smoke
-// check of __graft_entry__ and the cpu_baseline leg of bench.py may load it.
+// TEST INFRASTRUCTURE ONLY. The GPU library's results are checked against
+// it: only tests/, the smoke check of __graft_entry__ and the cpu_baseline
+// leg of bench.py may load it.
 // The product (libfemgpu) never links or calls it.
 //
 // Each function cites the reference file:line it restates

@@ -1,7 +1,7 @@
 """ctypes wrapper of the CPU oracle (oracle/liboracle.so).
 
-Test infrastructure only: the oracle is the checker for the GPU library and
-is never part of the product path.
+Test infrastructure only: the GPU library's results are checked against the
+oracle, which is never part of the product path.
 """
 from __future__ import annotations
 
