@@ -25,6 +25,10 @@ namespace fem {
 
 enum PcMode : int { PCM_GEN = 0, PCM_NONE = 1, PCM_DIAG = 2 };
 
+// FP32_IR: scaled inner residual below which a float32 breakdown counts as
+// round-off near convergence rather than a failure of the method
+constexpr double kSoftExit = 1e-3;
+
 __device__ __forceinline__ void kfail(KState* st, int code) {
     if (st->soft) { st->done = 1; st->err = -code; }
     else { st->done = 2; st->err = code; }
@@ -159,7 +163,10 @@ struct FinPcgXr {
 // product's npq partial sums exactly as the one-block tail k_finalize<1>
 // does (same order, same block size), block 0 runs that tail's state update
 // (FinPcgPq), and every block uses alpha = rz / p.q, the value FinPcgPq
-// stores. st->rz is not written in this kernel.
+// stores. st->rz is not written in this kernel. Block 0 may set st->done
+// here (breakdown); the other blocks do not rely on seeing it: each runs the
+// same breakdown test on its own copy of p.q and leaves x and r untouched,
+// as the unfused path (whose update kernel is gated off) does.
 template <typename T, int PCM>
 __global__ __launch_bounds__(kBlock) void k_pcg_xr_f(T* x, T* r, const T* p, const T* q, const T* d, size_t n,
                                                      KState* st, double* part, const double* __restrict__ pq_part,
@@ -169,6 +176,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_xr_f(T* x, T* r, const T* p, con
     add_partials<1>(pq_part, npq, t);
     block_sum<1>(t);
     if (blockIdx.x == 0 && threadIdx.x == 0) FinPcgPq{st}(t);
+    if (!st->minicg && (fabs(t[0]) < 1e-30 || t[0] <= 0.0)) return;
     const T a = (T)(st->rz / t[0]);
     double rr = 0.0, rz = 0.0;
     GRID_LOOP(i, n) {
@@ -1153,6 +1161,22 @@ void KrylovSolver::solve_once(const double* b, double* x, fem_solve_stats* stats
             vec_convert<float, double>(w32_.b.get(), res64_.get(), 1.0 / normr, n, s_);
             const double want = std::max(opts.inner_tol, 0.5 * opts.tol * normb / normr);
             total += run<float>(w32_, want, opts.max_iter - total, true, &S.spmv_count);
+            // A breakdown of the float32 inner solve (p.Ap <= 0, r.z <= 0, ...)
+            // ends it softly only once it has reduced its scaled residual
+            // below kSoftExit: there it is float32 round-off near convergence,
+            // and the refinement step continues from the iterate reached.
+            // Earlier it means A or M is not what the method needs (not SPD
+            // for PCG), and the solve fails with the error the float64 solve
+            // raises.
+            if (hst_.p->err < 0) {
+                const double inner_rel = hst_.p->normb > 0 ? hst_.p->normr / hst_.p->normb : 1.0;
+                if (!(inner_rel < kSoftExit)) {
+                    char ctx[160];
+                    std::snprintf(ctx, sizeof ctx, " (float32 inner solve, refinement step %d, iteration %d)",
+                                  refine + 1, hst_.p->it);
+                    fail(FEM_ERR_RUNTIME, std::string(solver_err_msg(-hst_.p->err)) + ctx);
+                }
+            }
             LAUNCH(k_axpy_f32, red_blocks(n), x64_.get(), w32_.x.get(), normr, (size_t)n);
             A_->apply(x64_.get(), ax64_.get(), s_);
             S.spmv_count += 1;
@@ -1169,6 +1193,12 @@ void KrylovSolver::solve_once(const double* b, double* x, fem_solve_stats* stats
             if (std::getenv("FEMGPU_DEBUG"))
                 std::fprintf(stderr, "[femgpu] refine %d: inner tol %.2e iters %d -> ||r||/||b|| %.3e (inner done %d err %d)\n",
                              refine, want, total, nr / normb, hst_.p->done, hst_.p->err);
+            if (!(nr < normr)) {
+                // the step made the residual larger (or NaN): take it back
+                // and keep the last good iterate and its residual norm
+                LAUNCH(k_axpy_f32, red_blocks(n), x64_.get(), w32_.x.get(), -normr, (size_t)n);
+                break;
+            }
             if (!(nr < normr * 0.999)) { normr = nr; break; }  // stagnation in float32
             normr = nr;
         }

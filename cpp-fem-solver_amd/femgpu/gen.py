@@ -106,6 +106,14 @@ def poisson3d(N: int) -> CSR:
     return _stencil((N, N, N), 6.0, offs)
 
 
+def aniso2d(N: int, eps: float = 0.01) -> CSR:
+    """Anisotropic 5-point Laplacian -u_xx - eps u_yy on an N x N interior
+    grid, row index i*N + j (x along j): diagonal 2 + 2 eps, x neighbours -1,
+    y neighbours -eps. Interior rows sum to zero; for eps below an AMG
+    strength threshold the y couplings are weak."""
+    return _stencil((N, N), 2.0 + 2.0 * eps, [(0, -1, -eps), (0, 1, -eps), (1, -1, -1.0), (1, 1, -1.0)])
+
+
 def convdiff2d(N: int, peclet: float = 10.0) -> CSR:
     """Upwind convection-diffusion on N x N cells, D = 1, h = 1.
 

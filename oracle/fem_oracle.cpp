@@ -552,6 +552,7 @@ struct AmgLevel {
 
 struct PcAmgCF : Pc {
     std::vector<AmgLevel> lv;
+    std::vector<std::vector<char>> split;  // per non-coarsest level: 1 C, 2 F
     vec lu;
     std::vector<uint32_t> piv;
     double omega = 2.0 / 3.0;
@@ -612,6 +613,7 @@ struct PcAmgCF : Pc {
             uint32_t nc = 0;
             for (uint32_t i = 0; i < n; ++i) { mapC[i] = nc; if (st[i] == 1) nc++; }
             if (nc == 0 || nc == n) break;
+            split.push_back(st);
             Csr P;
             P.n = n;
             P.m = nc;
@@ -1828,6 +1830,35 @@ int or_amg_levels(OrPc* h, uint32_t* rows, uint64_t* nnz, int cap) {
     auto* p = static_cast<PcBlockAMG<1>*>(h->pc.get());
     for (int i = 0; i < (int)p->lv.size() && i < cap; ++i) { rows[i] = p->lv[i].n; nnz[i] = p->lv[i].A.nnzb(); }
     return (int)p->lv.size();
+}
+// C/F split of level l of a C/F hierarchy (1 C, 2 F); -1 if l has none.
+int or_amg_cf_split(OrPc* h, int l, int8_t* st) {
+    if (h->kind != K_AMG) return -1;
+    auto* p = static_cast<PcAmgCF*>(h->pc.get());
+    if (l < 0 || l >= (int)p->split.size()) return -1;
+    for (size_t i = 0; i < p->split[l].size(); ++i) st[i] = (int8_t)p->split[l][i];
+    return 0;
+}
+// Block ILU(0) factors: L (strict lower blocks) and U (upper blocks with the
+// diagonal), each as block CSR with bs*bs row-major values. With null arrays
+// only the block counts are returned.
+int or_bilu0_export(OrPc* h, uint64_t* lnnzb, uint64_t* unnzb, uint32_t* lp, uint32_t* lc, double* lv, uint32_t* up,
+                    uint32_t* uc, double* uv) {
+    auto exp = [&](auto* f) {
+        *lnnzb = f->Lc.size();
+        *unnzb = f->Uc.size();
+        if (!lp) return;
+        std::copy(f->Lp.begin(), f->Lp.end(), lp);
+        std::copy(f->Lc.begin(), f->Lc.end(), lc);
+        std::copy(f->Lv.begin(), f->Lv.end(), lv);
+        std::copy(f->Up.begin(), f->Up.end(), up);
+        std::copy(f->Uc.begin(), f->Uc.end(), uc);
+        std::copy(f->Uv.begin(), f->Uv.end(), uv);
+    };
+    if (auto* f3 = dynamic_cast<PcBlockILU0<3>*>(h->pc.get())) exp(f3);
+    else if (auto* f1 = dynamic_cast<PcBlockILU0<1>*>(h->pc.get())) exp(f1);
+    else return -1;
+    return 0;
 }
 // Robust-mode aggregation of a scalar CSR matrix (mis2_agg): agg[n], the
 // final MIS-2 state per node (2 root, 0 other) and the number of aggregates.

@@ -48,6 +48,8 @@ def lib():
         L.or_amg_levels.argtypes = [P, P, P, C.c_int]
         L.or_amg_level_export.argtypes = [P, C.c_int, C.c_int, P, P, P, P]
         L.or_mis2_aggregate.argtypes = [P, C.c_double, P, P, C.POINTER(C.c_int)]
+        L.or_amg_cf_split.argtypes = [P, C.c_int, P]
+        L.or_bilu0_export.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P, P, P, P]
         L.or_spai_features.argtypes = [C.c_uint32, P, P, P, P, P, P]
         L.or_mlp_forward.argtypes = [P, C.c_uint32, P, C.c_int, P]
         L.or_spai_assemble.argtypes = [C.c_uint32, P, P, P, P, P, C.c_int, P]
@@ -215,6 +217,28 @@ class Pc:
         v = np.empty(nz)
         lib().or_amg_level_export(self.h, level, which, _ptr(rp), _ptr(ci), _ptr(v), _ptr(dims))
         return n, m, rp, ci, v
+
+    def cf_split(self, level):
+        """C/F split of a C/F AMG level: int8[n], 1 = C, 2 = F."""
+        n = int(self.amg_levels()[0][level])
+        st = np.empty(n, np.int8)
+        if lib().or_amg_cf_split(self.h, level, _ptr(st)) != 0:
+            raise OracleError(-1, "no C/F split at this level")
+        return st
+
+    def block_ilu0_factors(self, bs):
+        """Block ILU(0) factors as ((Lp, Lc, Lv), (Up, Uc, Uv)), values as
+        (nnzb, bs, bs) arrays: L strict lower, U upper with the diagonal."""
+        ln, un = C.c_uint64(), C.c_uint64()
+        if lib().or_bilu0_export(self.h, C.byref(ln), C.byref(un), None, None, None, None, None, None) != 0:
+            raise OracleError(-1, "not a block ILU(0) preconditioner")
+        nb = self.n // bs
+        lp, up = np.empty(nb + 1, np.uint32), np.empty(nb + 1, np.uint32)
+        lc, uc = np.empty(ln.value, np.uint32), np.empty(un.value, np.uint32)
+        lv, uv = np.empty(ln.value * bs * bs), np.empty(un.value * bs * bs)
+        lib().or_bilu0_export(self.h, C.byref(ln), C.byref(un), _ptr(lp), _ptr(lc), _ptr(lv), _ptr(up), _ptr(uc),
+                              _ptr(uv))
+        return (lp, lc, lv.reshape(-1, bs, bs)), (up, uc, uv.reshape(-1, bs, bs))
 
     def __del__(self):
         try:

@@ -314,6 +314,12 @@ def test_error_paths():
     A = F.CsrMatrix(2, 2, M.indptr, M.indices, M.data)
     with pytest.raises(F.SolverFailure):
         F.solve(A, None, np.array([1.0, 1.0]), solver="pcg")
+    # the same system in float32 inner / float64 refinement: a breakdown of
+    # the inner solve before it has reduced its residual fails the solve as
+    # in float64 (p.Ap = 0 in the first iteration; p.Ap < 0 in the second)
+    for bb in ([1.0, 1.0], [1.0, 0.5]):
+        with pytest.raises(F.SolverFailure):
+            F.solve(A, None, np.array(bb), solver="pcg", precision=F.FP32_IR)
     # zero / negative diagonal -> diag preconditioner failure (array.cpp:2485-2493)
     with pytest.raises(F.SolverFailure):
         F.Preconditioner(A, "diag")

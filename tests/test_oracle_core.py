@@ -326,6 +326,57 @@ def test_cf_amg_hierarchy_invariants():
         assert len(unit_rows) >= m2
 
 
+def cf_interpolation_checks(A, P, st, theta=0.25):
+    """Invariants of a C/F level with direct interpolation, independent of
+    how the split was chosen: C rows of P are unit rows numbering the C points
+    in row order; every F point has a strong C neighbour and its P row lists
+    exactly its strong C neighbours; an F row of zero row sum whose strong
+    neighbours are all C interpolates constants (row sum of P = 1). Returns
+    the number of F rows the last check covered."""
+    A = A.tocsr()
+    P = P.tocsr()
+    n = A.shape[0]
+    assert set(np.unique(st)) <= {1, 2}
+    cpts = np.flatnonzero(st == 1)
+    assert P.shape == (n, cpts.size)
+    mapc = -np.ones(n, np.int64)
+    mapc[cpts] = np.arange(cpts.size)
+    covered = 0
+    for i in range(n):
+        lo, hi = A.indptr[i], A.indptr[i + 1]
+        cols, vals = A.indices[lo:hi], A.data[lo:hi]
+        off = cols != i
+        mx = np.abs(vals[off]).max() if off.any() else 0.0
+        strong = set(cols[off & (np.abs(vals) >= theta * mx)].tolist())
+        pc = P.indices[P.indptr[i]:P.indptr[i + 1]]
+        pv = P.data[P.indptr[i]:P.indptr[i + 1]]
+        if st[i] == 1:
+            assert pc.tolist() == [mapc[i]] and pv.tolist() == [1.0]
+            continue
+        sc = sorted(mapc[j] for j in strong if st[j] == 1)
+        assert sc, f"F point {i} has no strong C neighbour"
+        assert sorted(pc.tolist()) == sc
+        if abs(vals.sum()) < 1e-14 * np.abs(vals).max() and all(st[j] == 1 for j in strong):
+            assert abs(pv.sum() - 1.0) < 1e-13
+            covered += 1
+    return covered
+
+
+def test_cf_amg_interpolation_with_weak_couplings():
+    """Anisotropic Laplacian (eps = 0.01 < theta): the y couplings are weak
+    and are lumped into the interpolation's diagonal, which the constant
+    row-sum check of interior F rows pins (array.cpp:3276-3319)."""
+    G = gen.aniso2d(24, 0.01)
+    A = O.Csr.from_gen(G)
+    pc = O.Pc(A, O.K_AMG, [0.25, 300, 25])
+    rows, _ = pc.amg_levels()
+    assert len(rows) >= 2
+    n, m, rp, ci, v = pc.amg_matrix(0, 1)
+    P = sp.csr_matrix((v, ci, rp), shape=(n, m))
+    covered = cf_interpolation_checks(G.to_scipy(), P, pc.cf_split(0))
+    assert covered > G.n // 4
+
+
 def test_amg_vcycle_symmetric_and_solves():
     G = gen.poisson2d(30)
     A = O.Csr.from_gen(G)
@@ -438,6 +489,39 @@ def test_block_ilu0_invariant_and_T7():
     # (LU)^{-1} A = I + (LU)^{-1}(A - LU); A - LU vanishes on the pattern, so the
     # diagonal blocks of (LU)^{-1}A - I stay small compared with I
     assert np.max(np.abs(np.diag(Minv_A) - 1.0)) < 0.5
+
+
+def block_lu_on_pattern_error(Gb, L, U):
+    """max over the blocks (I, K) of A's pattern of |(L U)_IK - A_IK| / max|A_IK|,
+    L unit block lower (strict part stored), U block upper with diagonal."""
+    (lp, lc, lv), (up, uc, uv) = L, U
+    Ub = {}
+    for J in range(Gb.nbr):
+        for q in range(up[J], up[J + 1]):
+            Ub[(J, int(uc[q]))] = uv[q]
+    blocks = Gb.values.reshape(-1, Gb.bs, Gb.bs)
+    worst = 0.0
+    for I in range(Gb.nbr):
+        for p in range(Gb.row_ptr[I], Gb.row_ptr[I + 1]):
+            K = int(Gb.col_idx[p])
+            acc = Ub[(I, K)].copy() if K >= I else np.zeros((Gb.bs, Gb.bs))
+            for q in range(lp[I], lp[I + 1]):
+                UJK = Ub.get((int(lc[q]), K))
+                if UJK is not None:
+                    acc += lv[q] @ UJK
+            worst = max(worst, np.abs(acc - blocks[p]).max() / np.abs(blocks[p]).max())
+    return worst
+
+
+@pytest.mark.parametrize("t,seed", [(4, 3), (5, 8)])
+def test_block_ilu0_factors_reproduce_A_on_block_pattern(t, seed):
+    """Full block IKJ ILU(0) (blockMatrix.hpp:455-623) gives (L U)_IK = A_IK on
+    every block of A's pattern. Random non-commuting 3x3 blocks, so a block
+    product taken in the wrong order (U^-1 A instead of A U^-1) fails."""
+    Gb = gen.bsr7(t, seed)
+    B = O.Bsr(3, Gb.nbr, Gb.nbc, Gb.row_ptr, Gb.col_idx, Gb.values)
+    L, U = O.Pc.block(B, 1).block_ilu0_factors(3)
+    assert block_lu_on_pattern_error(Gb, L, U) < 1e-12
 
 
 def test_block_ilu0_bs1_equals_scalar_ilu0_on_stencil():
