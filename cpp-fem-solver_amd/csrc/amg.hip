@@ -235,7 +235,8 @@ static inline dim3 g1(uint32_t n) { return dim3((n + 255) / 256); }
 // ---------------------------------------------------------------------------
 struct PcAmg : Precond {
     int kind_;
-    bool device_setup = false;  // hierarchy built by build_sa_hierarchy_device
+    bool device_setup = false;  // hierarchy built on the device
+    std::vector<std::vector<uint8_t>> splits;  // C/F: split per coarsened level
     bool inv_diag;  // SA smoother multiplies by the inverse diagonal
     int smoother, npre, npost;
     double omega;
@@ -286,6 +287,14 @@ struct PcAmg : Precond {
     }
     int kind() const override { return kind_; }
     uint32_t size() const override { return lv[0]->n; }
+    const CsrMat* amg_matrix(int l, int which) const override {
+        if (l < 0 || l >= (int)lv.size()) return nullptr;
+        const CsrMat& M = which == 0 ? lv[l]->A : which == 1 ? lv[l]->P : lv[l]->R;
+        return M.n ? &M : nullptr;
+    }
+    const std::vector<uint8_t>* amg_split(int l) const override {
+        return l >= 0 && l < (int)splits.size() ? &splits[l] : nullptr;
+    }
     bool has_fp32() const override { return fp32; }
     void ensure_fp32(hipStream_t s) override {
         if (fp32) return;
@@ -645,9 +654,24 @@ struct PcAmg : Precond {
 
 std::unique_ptr<Precond> make_amg_cf(CsrMat& A, const fem_pc_params& p, hipStream_t s) {
     if (A.n != A.m) fail(FEM_ERR_INVALID_ARGUMENT, "AMG needs a square matrix");
-    A.sync_host();
-    AmgDevHierarchy H = upload_hierarchy(build_cf_hierarchy(A, p), s);
-    return std::unique_ptr<Precond>(new PcAmg(FEM_PC_AMG, std::move(H), p, s));
+    std::vector<std::vector<uint8_t>> splits;
+    AmgDevHierarchy H;
+    if (rows_sorted_device(A, s)) {
+        H = build_cf_hierarchy_device(A, p, s, &splits);
+    } else {
+        // rows with unsorted or repeated columns: the same operator with its
+        // columns sorted and repeats summed
+        std::vector<uint32_t> rp, ci;
+        std::vector<double> v;
+        condense_device(A, 1, A.n, A.m, rp, ci, v, s);
+        CsrMat B;
+        B.set_host(A.n, A.m, std::move(rp), std::move(ci), std::move(v), s);
+        H = build_cf_hierarchy_device(B, p, s, &splits);
+    }
+    auto pc = std::unique_ptr<PcAmg>(new PcAmg(FEM_PC_AMG, std::move(H), p, s));
+    pc->splits = std::move(splits);
+    pc->device_setup = true;
+    return pc;
 }
 
 std::unique_ptr<Precond> make_saamg(CsrMat& A, const fem_pc_params& p, hipStream_t s) {
@@ -658,13 +682,8 @@ std::unique_ptr<Precond> make_saamg(CsrMat& A, const fem_pc_params& p, hipStream
     bool dev = p.p_omega <= 0.0 && p.setup_device != 0 && rows_sorted_device(A, s);
     AmgDevHierarchy H;
     if (dev) {
-        try {
-            H = build_sa_hierarchy_device(A, p, s);
-        } catch (const DeviceSetupTooLarge&) {
-            dev = false;
-        }
-    }
-    if (!dev) {
+        H = build_sa_hierarchy_device(A, p, s);
+    } else {
         A.sync_host();
         H = upload_hierarchy(build_sa_hierarchy(A, p), s);
     }

@@ -1,5 +1,6 @@
-// amg.hpp — host-side AMG hierarchy description (amg_setup.cpp) consumed by
-// the device V-cycle (amg.hip).
+// amg.hpp — AMG hierarchies: the host-built SA description (amg_setup.cpp),
+// the device-built C/F and SA hierarchies (amg_device.hip), and the level
+// data the device V-cycle (amg.hip) runs on.
 #pragma once
 #include <memory>
 
@@ -71,19 +72,23 @@ struct AmgDevHierarchy {
     std::vector<double> coarse;  // dense coarsest solution operator (row-major)
 };
 
-AmgHierarchy build_cf_hierarchy(const CsrMat& A, const fem_pc_params& p);
+// the C/F hierarchy built on the device (amg_device.hip); `splits` (may be
+// null) receives each coarsened level's split (1 C, 2 F)
+AmgDevHierarchy build_cf_hierarchy_device(const CsrMat& A, const fem_pc_params& p, hipStream_t s,
+                                          std::vector<std::vector<uint8_t>>* splits);
 AmgHierarchy build_sa_hierarchy(const CsrMat& A, const fem_pc_params& p);
 int mis2_aggregate(const HCsr& A, double theta, std::vector<int>& agg);
 // the robust SA hierarchy (p_omega <= 0) built on the device (amg_device.hip);
-// identical to build_sa_hierarchy's. Throws DeviceSetupTooLarge when a level
-// has more columns than the product's LDS bitmap holds (524288).
-struct DeviceSetupTooLarge {};
+// identical to build_sa_hierarchy's
 AmgDevHierarchy build_sa_hierarchy_device(const CsrMat& A, const fem_pc_params& p, hipStream_t s);
 // a host-built hierarchy moved to the device (R_k = P_k^T formed there)
 AmgDevHierarchy upload_hierarchy(AmgHierarchy&& H, hipStream_t s);
 std::vector<double> sa_coarse_operator_host(const HCsr& A, bool exact);
 // R = P^T on the device; each row of R lists its columns in ascending order
 void transpose_device(const CsrMat& P, CsrMat& R, hipStream_t s);
+// C = A B on the device, the exact product with each row's columns ascending;
+// B's rows must list strictly increasing columns
+void spgemm_device(const CsrMat& A, const CsrMat& B, CsrMat& C, hipStream_t s);
 // SGS smoother data of level D (colouring, diagonal, Jacobi weight and, with
 // at most kMaxGsColours colours, the colour-grouped copy of D.A)
 void sgs_setup_device(AmgLevelDev& D, hipStream_t s);

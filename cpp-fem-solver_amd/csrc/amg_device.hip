@@ -268,9 +268,14 @@ __global__ void k_tgather(const uint32_t* idx, const uint32_t* row, const double
 }
 
 // ---- sparse product C = A B --------------------------------------------------------
-// One wavefront per row of C (block = 64 threads); `words` 32-bit bitmap
-// words over B's columns in dynamic LDS, cleared once, then after each row
-// only the words that row touched. PASS 0 counts, PASS 1 writes the columns.
+// One wavefront per row of C (block = 64 threads) with a bitmap of `words`
+// 32-bit words in dynamic LDS. A row's columns span the words [lo, hi]
+// (found from the first and last column of each B row it reads, B's rows
+// being sorted); the span is covered in windows of `words` words: each
+// window's products set their bits, then the window is read out in
+// ascending column order (PASS 1 writes the columns, PASS 0 counts them) and
+// cleared. Rows whose span fits one window (banded and stencil operators)
+// take one pass; wider rows take more, so the product has no column limit.
 template <int PASS>
 __global__ __launch_bounds__(64) void k_spgemm_sym(const uint32_t* arp, const uint32_t* aci, const uint32_t* brp,
                                                    const uint32_t* bci, uint32_t n, uint32_t words, uint32_t* cnt,
@@ -281,52 +286,39 @@ __global__ __launch_bounds__(64) void k_spgemm_sym(const uint32_t* arp, const ui
     __syncthreads();
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const uint32_t a0 = arp[i], a1 = arp[i + 1];
+        uint32_t lo = 0xffffffffu, hi = 0;
         for (uint32_t p = a0 + l; p < a1; p += 64) {
             const uint32_t j = aci[p];
-            for (uint32_t q = brp[j]; q < brp[j + 1]; ++q) {
-                const uint32_t k = bci[q];
-                atomicOr(&bm[k >> 5], 1u << (k & 31));
+            const uint32_t b0 = brp[j], b1 = brp[j + 1];
+            if (b1 > b0) {
+                lo = min(lo, bci[b0] >> 5);
+                hi = max(hi, bci[b1 - 1] >> 5);
             }
         }
-        __syncthreads();
-        if (PASS == 0) {
-            // popcount over the touched words: walk the products again and
-            // count each word once (the first product that sees it clears a
-            // copy in a second bitmap would need more LDS, so count set bits
-            // of words as they are cleared)
-            uint32_t c = 0;
+        for (int off = 32; off > 0; off >>= 1) {
+            lo = min(lo, (uint32_t)__shfl_xor((int)lo, off));
+            hi = max(hi, (uint32_t)__shfl_xor((int)hi, off));
+        }
+        uint32_t c = 0;
+        uint32_t out = PASS == 1 ? crp[i] : 0u;
+        for (uint32_t w0 = lo; lo <= hi && w0 <= hi; w0 += words) {
+            const uint32_t span = min(hi - w0, words - 1) + 1;  // words of this window
             for (uint32_t p = a0 + l; p < a1; p += 64) {
                 const uint32_t j = aci[p];
                 for (uint32_t q = brp[j]; q < brp[j + 1]; ++q) {
-                    const uint32_t w = bci[q] >> 5;
-                    const uint32_t bits = atomicExch(&bm[w], 0u);
-                    c += __popc(bits);
+                    const uint32_t k = bci[q];
+                    const uint32_t w = (k >> 5) - w0;
+                    if ((k >> 5) >= w0 && w < span) atomicOr(&bm[w], 1u << (k & 31));
                 }
             }
-            for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
-            if (l == 0) cnt[i] = c;
-        } else {
-            // read the set bits out in ascending column order: scan the words
-            // between the row's smallest and largest column
-            uint32_t lo = 0xffffffffu, hi = 0;
-            for (uint32_t p = a0 + l; p < a1; p += 64) {
-                const uint32_t j = aci[p];
-                const uint32_t b0 = brp[j], b1 = brp[j + 1];
-                if (b1 > b0) {
-                    lo = min(lo, bci[b0] >> 5);
-                    hi = max(hi, bci[b1 - 1] >> 5);
-                }
-            }
-            for (int off = 32; off > 0; off >>= 1) {
-                lo = min(lo, (uint32_t)__shfl_xor((int)lo, off));
-                hi = max(hi, (uint32_t)__shfl_xor((int)hi, off));
-            }
-            uint32_t out = crp[i];
-            if (lo <= hi) {
-                for (uint32_t base = lo; base <= hi; base += 64) {
-                    const uint32_t w = base + l;
-                    const uint32_t bits = w <= hi ? bm[w] : 0u;
-                    const uint32_t pc = __popc(bits);
+            __syncthreads();
+            for (uint32_t base = 0; base < span; base += 64) {
+                const uint32_t w = base + l;
+                const uint32_t bits = w < span ? bm[w] : 0u;
+                const uint32_t pc = __popc(bits);
+                if (PASS == 0) {
+                    c += pc;
+                } else {
                     // exclusive prefix of popcounts over the wave
                     uint32_t x = pc;
                     for (int off = 1; off < 64; off <<= 1) {
@@ -337,15 +329,19 @@ __global__ __launch_bounds__(64) void k_spgemm_sym(const uint32_t* arp, const ui
                     uint32_t b = bits;
                     while (b) {
                         const int t = __ffs(b) - 1;
-                        cci[pos++] = (w << 5) + (uint32_t)t;
+                        cci[pos++] = ((w0 + w) << 5) + (uint32_t)t;
                         b &= b - 1;
                     }
-                    if (w <= hi) bm[w] = 0u;
                     out += (uint32_t)__shfl((int)x, 63);
                 }
+                if (w < span) bm[w] = 0u;
             }
+            __syncthreads();
         }
-        __syncthreads();
+        if (PASS == 0) {
+            for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+            if (l == 0) cnt[i] = c;
+        }
     }
 }
 
@@ -390,31 +386,40 @@ struct DCsr {
     }
 };
 
-constexpr uint32_t kMaxBitmapWords = 16384;  // 64 KB of LDS: B with up to 524288 columns
+constexpr uint32_t kMaxBitmapWords = 16384;  // 64 KB of LDS: windows of 524288 columns
+
+// C = A B for A (an x am) and B (am x bm) given by their arrays
+DCsr spgemm_arrays(uint32_t an, uint32_t am, const uint32_t* arp, const uint32_t* aci, const double* av, uint32_t bn,
+                   uint32_t bm, const uint32_t* brp, const uint32_t* bci, const double* bv, hipStream_t s) {
+    if (am != bn) fail(FEM_ERR_RUNTIME, "sparse product: shape mismatch");
+    const uint32_t words = std::max(1u, std::min((bm + 31) / 32, kMaxBitmapWords));
+    DCsr C;
+    C.n = an;
+    C.m = bm;
+    DevBuf<uint32_t> cnt(an + 1);
+    C.rp.alloc(an + 1);
+    const unsigned grid = std::max(1u, std::min(an, 8192u));
+    const size_t lds = (size_t)words * 4;
+    if (an)
+        hipLaunchKernelGGL((k_spgemm_sym<0>), dim3(grid), dim3(64), lds, s, arp, aci, brp, bci, an, words, cnt.get(),
+                           nullptr, nullptr);
+    FEM_HIP(hipMemsetAsync(cnt.get() + an, 0, 4, s));
+    scan_u32(cnt.get(), C.rp.get(), (size_t)an + 1, s);
+    C.nnz = read_u32(C.rp.get() + an, s);
+    C.alloc_entries(s);
+    if (an && C.nnz) {
+        hipLaunchKernelGGL((k_spgemm_sym<1>), dim3(grid), dim3(64), lds, s, arp, aci, brp, bci, an, words, nullptr,
+                           C.rp.get(), C.ci.get());
+        hipLaunchKernelGGL(k_spgemm_num, dim3(grid), dim3(64), 0, s, arp, aci, av, brp, bci, bv, C.rp.get(),
+                           C.ci.get(), C.v.get(), an);
+    }
+    FEM_HIP(hipGetLastError());
+    FEM_HIP(hipStreamSynchronize(s));  // the temporaries end with this call
+    return C;
+}
 
 DCsr spgemm_dev(const DCsr& A, const DCsr& B, hipStream_t s) {
-    if (A.m != B.n) fail(FEM_ERR_RUNTIME, "AMG setup: product shape mismatch");
-    const uint32_t words = (B.m + 31) / 32;
-    if (words > kMaxBitmapWords) throw DeviceSetupTooLarge{};
-    DCsr C;
-    C.n = A.n;
-    C.m = B.m;
-    DevBuf<uint32_t> cnt(A.n + 1);
-    C.rp.alloc(A.n + 1);
-    const unsigned grid = std::max(1u, std::min(A.n, 8192u));
-    const size_t lds = (size_t)words * 4;
-    hipLaunchKernelGGL((k_spgemm_sym<0>), dim3(grid), dim3(64), lds, s, A.rp.get(), A.ci.get(), B.rp.get(),
-                       B.ci.get(), A.n, words, cnt.get(), nullptr, nullptr);
-    FEM_HIP(hipMemsetAsync(cnt.get() + A.n, 0, 4, s));
-    scan_u32(cnt.get(), C.rp.get(), (size_t)A.n + 1, s);
-    C.nnz = read_u32(C.rp.get() + A.n, s);
-    C.alloc_entries(s);
-    hipLaunchKernelGGL((k_spgemm_sym<1>), dim3(grid), dim3(64), lds, s, A.rp.get(), A.ci.get(), B.rp.get(),
-                       B.ci.get(), A.n, words, nullptr, C.rp.get(), C.ci.get());
-    hipLaunchKernelGGL(k_spgemm_num, dim3(grid), dim3(64), 0, s, A.rp.get(), A.ci.get(), A.v.get(), B.rp.get(),
-                       B.ci.get(), B.v.get(), C.rp.get(), C.ci.get(), C.v.get(), A.n);
-    FEM_HIP(hipGetLastError());
-    return C;
+    return spgemm_arrays(A.n, A.m, A.rp.get(), A.ci.get(), A.v.get(), B.n, B.m, B.rp.get(), B.ci.get(), B.v.get(), s);
 }
 
 // R = P^T for P given by its arrays (n x m, nnz entries)
@@ -949,6 +954,303 @@ AmgDevHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params&
     H.coarse = sa_coarse_operator_host(to_host(last.A), prm.coarse_exact != 0);
     tr.lap("coarsest operator");
     return H;
+}
+
+// ---------------------------------------------------------------------------
+// Classical (C/F) AMG setup on the device. The operator it builds is the
+// reference's C/F AMG (ode-solver/src/math/array.cpp:3182-3343): strength
+// |a_ij| >= theta max_{k != i} |a_ik|, a C/F split in which every F point
+// strongly depends on a C point, direct interpolation with the weak
+// couplings lumped into the diagonal, Galerkin coarse operators, a dense
+// coarsest solve. The split itself is this library's own, parallel, choice
+// (the reference pops points from a serial priority queue): C points are
+// picked in synchronous rounds, each undecided point whose key beats every
+// undecided neighbour in the symmetrised strength graph becoming C, and
+// every undecided point that strongly depends on a C point becoming F. This
+// is the PMIS selection (De Sterck, Yang & Heys, "Reducing complexity in
+// parallel algebraic multigrid preconditioners", SIAM J. Matrix Anal. Appl.
+// 27(4), 2006) with colour-augmented keys as in Alber & Olson ("Parallel
+// coarse-grid selection", Numer. Linear Algebra Appl. 14, 2007): a point's
+// key is (number of points strongly depending on it, its colour in the
+// greedy row-order colouring of the symmetrised strength graph, a hash of
+// its index, its index). On grid operators the colouring is the red-black
+// (or line-wise) one, and the split comes out as the structured coarsening
+// the serial priority queue finds; keys with a random tie-break instead
+// leave defects that cost about half as many iterations again. Points
+// without strong couplings become C, as in the reference. Nothing depends
+// on thread scheduling.
+namespace {
+
+// strong couplings of row i: per-entry flag and count
+__global__ void k_cf_strength(const uint32_t* rp, const uint32_t* ci, const double* v, double theta, uint32_t n,
+                              uint8_t* sm, uint32_t* cnt) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { cnt[n] = 0u; return; }
+    double mx = 0.0;
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
+        if (ci[p] != i) mx = fmax(mx, fabs(v[p]));
+    const double thr = theta * mx;
+    uint32_t c = 0;
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
+        const double a = fabs(v[p]);
+        const bool st = ci[p] != i && a != 0.0 && a >= thr;
+        sm[p] = st ? 1 : 0;
+        c += st ? 1u : 0u;
+    }
+    cnt[i] = c;
+}
+
+// strength graph S as CSR (columns in A's stored order)
+__global__ void k_cf_sfill(const uint32_t* rp, const uint32_t* ci, const uint8_t* sm, uint32_t n, const uint32_t* srp,
+                           uint32_t* sci) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t o = srp[i];
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
+        if (sm[p]) sci[o++] = ci[p];
+}
+
+// greedy colouring of S + S^T in row order, in rounds: a point whose lower
+// neighbours are all coloured takes the least colour none of them holds
+// (colours from 63 on stored as 63: the colour only orders keys). A colour,
+// once written, is final, so a neighbour coloured earlier in the same round
+// reads as well as one coloured in an earlier round.
+constexpr uint8_t kNoColour = 0xFF;
+__global__ void k_cf_colour(const uint32_t* srp, const uint32_t* sci, const uint32_t* trp, const uint32_t* tci,
+                            uint32_t n, uint8_t* col, uint32_t* left) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || col[i] != kNoColour) return;
+    uint64_t used = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        const uint32_t* rp = pass ? trp : srp;
+        const uint32_t* cl = pass ? tci : sci;
+        for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
+            const uint32_t j = cl[p];
+            if (j >= i) continue;
+            const uint8_t c = __hip_atomic_load(&col[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (c == kNoColour) { atomicAdd(left, 1u); return; }
+            used |= 1ull << (c < 63 ? c : 63);
+        }
+    }
+    const int c = __builtin_ctzll(~used);
+    __hip_atomic_store(&col[i], (uint8_t)(c < 63 ? c : 63), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// key order (lambda, colour, hash(index), index), lambda = |S^T row|
+__device__ __forceinline__ bool cf_beats(uint32_t la, uint8_t ca, uint32_t a, uint32_t lb, uint8_t cb, uint32_t b) {
+    if (la != lb) return la > lb;
+    if (ca != cb) return ca > cb;
+    const uint32_t ha = node_hash(a), hb = node_hash(b);
+    if (ha != hb) return ha > hb;
+    return a > b;
+}
+
+enum : uint8_t { CF_U = 0, CF_C = 1, CF_F = 2 };
+
+// one selection round: an undecided point beating every undecided
+// neighbour in S + S^T becomes C (reads st_in only, writes st_out)
+__global__ void k_cf_select(const uint32_t* srp, const uint32_t* sci, const uint32_t* trp, const uint32_t* tci,
+                            const uint8_t* col, const uint8_t* st_in, uint32_t n, uint8_t* st_out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint8_t si = st_in[i];
+    if (si != CF_U) { st_out[i] = si; return; }
+    const uint32_t li = trp[i + 1] - trp[i];
+    const uint8_t ci = col[i];
+    bool best = true;
+    for (uint32_t p = srp[i]; p < srp[i + 1] && best; ++p) {
+        const uint32_t j = sci[p];
+        if (st_in[j] == CF_U && cf_beats(trp[j + 1] - trp[j], col[j], j, li, ci, i)) best = false;
+    }
+    for (uint32_t p = trp[i]; p < trp[i + 1] && best; ++p) {
+        const uint32_t j = tci[p];
+        if (st_in[j] == CF_U && cf_beats(trp[j + 1] - trp[j], col[j], j, li, ci, i)) best = false;
+    }
+    st_out[i] = best ? CF_C : CF_U;
+}
+
+// undecided points that strongly depend on a C point become F; counts the
+// points left undecided. C states are final and only undecided -> F is
+// written, so the reads of "is C" see the same values in any order.
+__global__ void k_cf_fmark(const uint32_t* srp, const uint32_t* sci, uint32_t n, uint8_t* st, uint32_t* left) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || st[i] != CF_U) return;
+    for (uint32_t p = srp[i]; p < srp[i + 1]; ++p)
+        if (st[sci[p]] == CF_C) { st[i] = CF_F; return; }
+    atomicAdd(left, 1u);
+}
+
+// rows of P: 1 for a C point, its strong C neighbours for an F point;
+// an F point without one counts into *orphan (not produced by the rounds)
+__global__ void k_cf_pcount(const uint32_t* srp, const uint32_t* sci, const uint8_t* st, uint32_t n, uint32_t* cnt,
+                            uint32_t* orphan) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { cnt[n] = 0u; return; }
+    if (st[i] == CF_C) { cnt[i] = 1u; return; }
+    uint32_t c = 0;
+    for (uint32_t p = srp[i]; p < srp[i + 1]; ++p) c += st[sci[p]] == CF_C ? 1u : 0u;
+    if (c == 0) atomicAdd(orphan, 1u);
+    cnt[i] = c;
+}
+
+// direct interpolation: C row i -> (mapc[i], 1); F row i -> for each strong C
+// neighbour j in stored order (mapc[j], -a_ij / den), den = a_ii plus the
+// weak off-diagonal couplings of row i in stored order (a_ii if |den| <
+// 1e-300). mapc is increasing in the row index, so the columns are sorted.
+__global__ void k_cf_pfill(const uint32_t* rp, const uint32_t* ci, const double* v, const uint8_t* sm,
+                           const uint8_t* st, const uint32_t* mapc, const double* d, uint32_t n, const uint32_t* prp,
+                           uint32_t* pci, double* pv) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t o = prp[i];
+    if (st[i] == CF_C) {
+        pci[o] = mapc[i];
+        pv[o] = 1.0;
+        return;
+    }
+    double den = d[i];
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
+        if (ci[p] != i && !sm[p]) den += v[p];
+    if (fabs(den) < 1e-300) den = d[i];
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
+        const uint32_t j = ci[p];
+        if (sm[p] && st[j] == CF_C) {
+            pci[o] = mapc[j];
+            pv[o] = -v[p] / den;
+            ++o;
+        }
+    }
+}
+
+__global__ void k_cf_cflag(const uint8_t* st, uint32_t n, uint32_t* f) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) f[i] = st[i] == CF_C ? 1u : 0u;
+    if (i == n) f[n] = 0u;
+}
+
+}  // namespace
+
+AmgDevHierarchy build_cf_hierarchy_device(const CsrMat& A0, const fem_pc_params& prm, hipStream_t s,
+                                          std::vector<std::vector<uint8_t>>* splits) {
+    AmgDevHierarchy H;
+    SetupTrace tr("device C/F", s);
+    const double theta = prm.theta;
+    const int maxl = std::max(1, prm.max_levels);
+    const uint32_t cth = (uint32_t)std::max(1, prm.coarse_size);
+    DCsr cur;
+    cur.n = A0.n;
+    cur.m = A0.m;
+    cur.nnz = A0.nnz;
+    cur.rp.alloc(A0.n + 1);
+    cur.alloc_entries(s);
+    FEM_HIP(hipMemcpyAsync(cur.rp.get(), A0.rp.get(), (size_t)(A0.n + 1) * 4, hipMemcpyDeviceToDevice, s));
+    FEM_HIP(hipMemcpyAsync(cur.ci.get(), A0.ci.get(), A0.nnz * 4, hipMemcpyDeviceToDevice, s));
+    FEM_HIP(hipMemcpyAsync(cur.v.get(), A0.v.get(), A0.nnz * 8, hipMemcpyDeviceToDevice, s));
+    DevBuf<uint32_t> cnt1(1);
+    for (int k = 0;; ++k) {
+        const uint32_t n = cur.n;
+        auto lv = std::make_unique<AmgLevelDev>();
+        lv->n = n;
+        lv->omega = prm.omega;
+        DevBuf<double> d(n);
+        hipLaunchKernelGGL(k_diag_val, dim3(g1(n)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), cur.v.get(), n,
+                           d.get());
+        auto keep_as_coarsest = [&] {
+            cur.to_mat(lv->A, s);
+            H.lv.push_back(std::move(lv));
+        };
+        if (n <= cth || k + 1 >= maxl) { keep_as_coarsest(); break; }
+        // strength graph S and its transpose
+        DevBuf<uint8_t> sm(std::max<uint64_t>(cur.nnz, 1));
+        DevBuf<uint32_t> scnt(n + 1), srp(n + 1);
+        hipLaunchKernelGGL(k_cf_strength, dim3(g1(n + 1)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), cur.v.get(),
+                           theta, n, sm.get(), scnt.get());
+        scan_u32(scnt.get(), srp.get(), (size_t)n + 1, s);
+        const uint32_t snz = read_u32(srp.get() + n, s);
+        DevBuf<uint32_t> sci(std::max<uint32_t>(snz, 1));
+        hipLaunchKernelGGL(k_cf_sfill, dim3(g1(n)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), sm.get(), n,
+                           srp.get(), sci.get());
+        DevBuf<double> zeros(std::max<uint32_t>(snz, 1));
+        FEM_HIP(hipMemsetAsync(zeros.get(), 0, (size_t)std::max<uint32_t>(snz, 1) * 8, s));
+        DCsr St = transpose_arrays(srp.get(), sci.get(), zeros.get(), n, n, snz, s);
+        tr.lap("strength");
+        // colouring of S + S^T (several rounds per host check)
+        DevBuf<uint8_t> colour(n);
+        FEM_HIP(hipMemsetAsync(colour.get(), kNoColour, n, s));
+        for (uint32_t round = 0;; round += 16) {
+            if (round > n + 16) fail(FEM_ERR_RUNTIME, "AMG setup: colouring did not finish");
+            for (int r = 0; r < 16; ++r) {
+                FEM_HIP(hipMemsetAsync(cnt1.get(), 0, 4, s));
+                hipLaunchKernelGGL(k_cf_colour, dim3(g1(n)), dim3(256), 0, s, srp.get(), sci.get(), St.rp.get(),
+                                   St.ci.get(), n, colour.get(), cnt1.get());
+            }
+            if (read_u32(cnt1.get(), s) == 0) break;
+        }
+        tr.lap("colouring");
+        // C/F split in rounds
+        DevBuf<uint8_t> sa(n), sb(n);
+        FEM_HIP(hipMemsetAsync(sa.get(), CF_U, n, s));
+        uint8_t* st = sa.get();
+        uint8_t* st2 = sb.get();
+        for (uint32_t round = 0;; ++round) {
+            if (round > n) fail(FEM_ERR_RUNTIME, "AMG setup: C/F splitting did not finish");
+            FEM_HIP(hipMemsetAsync(cnt1.get(), 0, 4, s));
+            hipLaunchKernelGGL(k_cf_select, dim3(g1(n)), dim3(256), 0, s, srp.get(), sci.get(), St.rp.get(),
+                               St.ci.get(), colour.get(), st, n, st2);
+            hipLaunchKernelGGL(k_cf_fmark, dim3(g1(n)), dim3(256), 0, s, srp.get(), sci.get(), n, st2, cnt1.get());
+            std::swap(st, st2);
+            if (read_u32(cnt1.get(), s) == 0) break;
+        }
+        DevBuf<uint32_t> f(n + 1), mapc(n + 1);
+        hipLaunchKernelGGL(k_cf_cflag, dim3(g1(n + 1)), dim3(256), 0, s, st, n, f.get());
+        scan_u32(f.get(), mapc.get(), (size_t)n + 1, s);
+        const uint32_t nc = read_u32(mapc.get() + n, s);
+        tr.lap("C/F split");
+        if (nc == 0 || nc == n) { keep_as_coarsest(); break; }
+        // direct interpolation
+        DCsr P;
+        P.n = n;
+        P.m = nc;
+        P.rp.alloc(n + 1);
+        FEM_HIP(hipMemsetAsync(cnt1.get(), 0, 4, s));
+        hipLaunchKernelGGL(k_cf_pcount, dim3(g1(n + 1)), dim3(256), 0, s, srp.get(), sci.get(), st, n, f.get(),
+                           cnt1.get());
+        if (read_u32(cnt1.get(), s) != 0) fail(FEM_ERR_RUNTIME, "AMG setup: an F point has no strong C neighbour");
+        scan_u32(f.get(), P.rp.get(), (size_t)n + 1, s);
+        P.nnz = read_u32(P.rp.get() + n, s);
+        P.alloc_entries(s);
+        hipLaunchKernelGGL(k_cf_pfill, dim3(g1(n)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), cur.v.get(), sm.get(),
+                           st, mapc.get(), d.get(), n, P.rp.get(), P.ci.get(), P.v.get());
+        FEM_HIP(hipGetLastError());
+        if (splits) {
+            splits->emplace_back(n);
+            FEM_HIP(hipMemcpyAsync(splits->back().data(), st, n, hipMemcpyDeviceToHost, s));
+        }
+        tr.lap("interpolation");
+        DCsr R = transpose_dev(P, s);
+        DCsr AP = spgemm_dev(cur, P, s);
+        DCsr Ac = spgemm_dev(R, AP, s);
+        tr.lap("Galerkin products");
+        lv->nc = nc;
+        lv->d = std::move(d);
+        cur.to_mat(lv->A, s);
+        P.to_mat(lv->P, s);
+        R.to_mat(lv->R, s);
+        H.lv.push_back(std::move(lv));
+        cur = std::move(Ac);
+    }
+    H.coarse = sa_coarse_operator_host(to_host(H.lv.back()->A), true);
+    tr.lap("coarsest operator");
+    return H;
+}
+
+// C = A B on the device (columns of each row ascending; every structurally
+// reached column kept). B's rows must list strictly increasing columns.
+void spgemm_device(const CsrMat& A, const CsrMat& B, CsrMat& C, hipStream_t s) {
+    DCsr c = spgemm_arrays(A.n, A.m, A.rp.get(), A.ci.get(), A.v.get(), B.n, B.m, B.rp.get(), B.ci.get(), B.v.get(), s);
+    c.to_mat(C, s);
 }
 
 }  // namespace fem

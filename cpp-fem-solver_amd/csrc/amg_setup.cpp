@@ -1,6 +1,8 @@
-// amg_setup.cpp — AMG hierarchy construction.
-//   classical C/F (make_amg_data):      ode-solver/src/math/array.cpp:3182-3343
-//   smoothed aggregation (block AMG):   ode-solver/include/math/blockMatrix.hpp:1051-1179
+// amg_setup.cpp — host construction of the smoothed-aggregation hierarchy
+// (block AMG, ode-solver/include/math/blockMatrix.hpp:1051-1179): the
+// reference's greedy aggregation (explicit weights) and the host build of the
+// robust variant. The C/F hierarchy and the robust SA hierarchy are built on
+// the device (amg_device.hip).
 // The coarse products are exact sparse products accumulated in the
 // reference's traversal order (row p of A, then row q of B). The coarsest
 // solve is stored as an explicit dense operator C (C f = coarse solve of f,
@@ -75,147 +77,6 @@ static HCsr from_mat(const CsrMat& A) {
 static void check_coarse(uint32_t nc) {
     if (nc > 4096) fail(FEM_ERR_RUNTIME, "AMG: coarsest level has " + std::to_string(nc) +
                                              " rows; raise max_levels or coarse_size");
-}
-
-// dense partial-pivot LU with pivot floor 1e-14 (array.cpp:2252-2333);
-// returns the explicit solution operator
-static std::vector<double> cf_coarse_operator(const HCsr& A) {
-    const uint32_t n = A.n;
-    check_coarse(n);
-    std::vector<double> M((size_t)n * n, 0.0);
-    std::vector<uint32_t> piv(n);
-    for (uint32_t i = 0; i < n; ++i)
-        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) M[(size_t)i * n + A.ci[p]] = A.v[p];
-    for (uint32_t i = 0; i < n; ++i) piv[i] = i;
-    for (uint32_t k = 0; k < n; ++k) {
-        uint32_t pm = k;
-        double vm = std::fabs(M[(size_t)k * n + k]);
-        for (uint32_t i = k + 1; i < n; ++i)
-            if (std::fabs(M[(size_t)i * n + k]) > vm) { vm = std::fabs(M[(size_t)i * n + k]); pm = i; }
-        if (pm != k) {
-            std::swap(piv[k], piv[pm]);
-            for (uint32_t j = 0; j < n; ++j) std::swap(M[(size_t)k * n + j], M[(size_t)pm * n + j]);
-        }
-        if (std::fabs(M[(size_t)k * n + k]) < 1e-14) M[(size_t)k * n + k] = 1e-14;
-        for (uint32_t i = k + 1; i < n; ++i) {
-            M[(size_t)i * n + k] /= M[(size_t)k * n + k];
-            for (uint32_t j = k + 1; j < n; ++j) M[(size_t)i * n + j] -= M[(size_t)i * n + k] * M[(size_t)k * n + j];
-        }
-    }
-    std::vector<double> C((size_t)n * n), x(n);
-    for (uint32_t col = 0; col < n; ++col) {
-        for (uint32_t i = 0; i < n; ++i) x[i] = (piv[i] == col) ? 1.0 : 0.0;
-        for (uint32_t i = 0; i < n; ++i)
-            for (uint32_t j = 0; j < i; ++j) x[i] -= M[(size_t)i * n + j] * x[j];
-        for (int64_t i = (int64_t)n - 1; i >= 0; --i) {
-            for (uint32_t j = i + 1; j < n; ++j) x[i] -= M[(size_t)i * n + j] * x[j];
-            x[i] /= M[(size_t)i * n + i];
-        }
-        for (uint32_t i = 0; i < n; ++i) C[(size_t)i * n + col] = x[i];
-    }
-    return C;
-}
-
-AmgHierarchy build_cf_hierarchy(const CsrMat& A0, const fem_pc_params& prm) {
-    AmgHierarchy H;
-    H.A.push_back(from_mat(A0));
-    const double theta = prm.theta;
-    const uint32_t cth = (uint32_t)std::max(1, prm.coarse_size);
-    const int maxl = std::max(1, prm.max_levels);
-    for (int k = 0;; ++k) {
-        const HCsr& A = H.A[k];
-        const uint32_t n = A.n;
-        std::vector<double> d(n, 0.0);
-        for (uint32_t i = 0; i < n; ++i)
-            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
-                if (A.ci[p] == i) { d[i] = A.v[p]; break; }
-        H.omega.push_back(prm.omega);
-        if (n <= cth || k + 1 >= maxl) break;
-        // strength (array.cpp:3202-3235)
-        std::vector<uint32_t> Sp(n + 1, 0), Sc;
-        Sc.reserve(A.ci.size());
-        for (uint32_t i = 0; i < n; ++i) {
-            double mx = 0.0;
-            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
-                if (A.ci[p] != i) mx = std::max(mx, std::fabs(A.v[p]));
-            const double thr = theta * mx;
-            for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
-                if (A.ci[p] != i && std::fabs(A.v[p]) >= thr) Sc.push_back(A.ci[p]);
-            Sp[i + 1] = (uint32_t)Sc.size();
-        }
-        // C/F splitting with a max-heap on (weight, index) (array.cpp:3237-3270)
-        std::vector<char> st(n, 0);
-        std::vector<uint32_t> w(n);
-        for (uint32_t i = 0; i < n; ++i) w[i] = Sp[i + 1] - Sp[i];
-        std::priority_queue<std::pair<uint32_t, uint32_t>> pq;
-        for (uint32_t i = 0; i < n; ++i) pq.push({w[i], i});
-        while (!pq.empty()) {
-            const uint32_t i = pq.top().second;
-            pq.pop();
-            if (st[i] != 0) continue;
-            st[i] = 1;
-            for (uint32_t p = Sp[i]; p < Sp[i + 1]; ++p) {
-                const uint32_t j = Sc[p];
-                if (st[j] != 0) continue;
-                st[j] = 2;
-                for (uint32_t q = Sp[j]; q < Sp[j + 1]; ++q) {
-                    const uint32_t kk = Sc[q];
-                    if (st[kk] == 0) { w[kk]++; pq.push({w[kk], kk}); }
-                }
-            }
-        }
-        for (uint32_t i = 0; i < n; ++i)
-            if (st[i] == 0) st[i] = 1;
-        for (uint32_t i = 0; i < n; ++i) {
-            if (st[i] != 2) continue;
-            bool hasC = false;
-            for (uint32_t p = Sp[i]; p < Sp[i + 1]; ++p)
-                if (st[Sc[p]] == 1) { hasC = true; break; }
-            if (!hasC) st[i] = 1;
-        }
-        std::vector<uint32_t> mapC(n, 0);
-        uint32_t nc = 0;
-        for (uint32_t i = 0; i < n; ++i) { mapC[i] = nc; if (st[i] == 1) nc++; }
-        if (nc == 0 || nc == n) break;
-        // direct interpolation (array.cpp:3276-3319), without padding slots
-        HCsr P;
-        P.n = n;
-        P.m = nc;
-        P.rp.assign(n + 1, 0);
-        for (uint32_t i = 0; i < n; ++i) {
-            if (st[i] == 1) { P.ci.push_back(mapC[i]); P.v.push_back(1.0); }
-            else {
-                double den = d[i];
-                for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
-                    const uint32_t j = A.ci[p];
-                    if (j == i) continue;
-                    bool strong = false;
-                    for (uint32_t s = Sp[i]; s < Sp[i + 1]; ++s)
-                        if (Sc[s] == j) { strong = true; break; }
-                    if (!strong) den += A.v[p];
-                }
-                if (std::fabs(den) < 1e-300) den = d[i];
-                for (uint32_t s = Sp[i]; s < Sp[i + 1]; ++s) {
-                    const uint32_t j = Sc[s];
-                    if (st[j] != 1) continue;
-                    double aij = 0.0;
-                    for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
-                        if (A.ci[p] == j) { aij = A.v[p]; break; }
-                    P.ci.push_back(mapC[j]);
-                    P.v.push_back(-aij / den);
-                }
-            }
-            P.rp[i + 1] = (uint32_t)P.ci.size();
-        }
-        HCsr AP = spgemm(A, P);
-        HCsr R = transpose(P);
-        HCsr Ac = spgemm(R, AP);
-        H.d.push_back(std::move(d));
-        H.P.push_back(std::move(P));
-        H.A.push_back(std::move(Ac));
-    }
-    H.coarse = cf_coarse_operator(H.A.back());
-    return H;
 }
 
 // 1x1 condensation (blockMatrix.hpp:269-315 with BS = 1): columns sorted and

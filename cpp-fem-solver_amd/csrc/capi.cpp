@@ -9,6 +9,7 @@
 #include <mutex>
 #include <numeric>
 
+#include "amg.hpp"
 #include "handles.hpp"
 
 using namespace fem;
@@ -187,12 +188,9 @@ int fem_csr_export(fem_csr A, uint32_t* row_ptr, uint32_t* col_idx, double* valu
 
 int fem_csr_transpose(fem_csr A, fem_csr* out) {
     return guard([&] {
-        std::vector<uint32_t> rp, ci;
-        std::vector<double> v;
-        csr_transpose_host(A->A, rp, ci, v);
         auto h = std::make_unique<fem_csr_s>();
         h->stream = new_stream();
-        h->A.set_host(A->A.m, A->A.n, std::move(rp), std::move(ci), std::move(v), h->stream);
+        transpose_device(A->A, h->A, h->stream);
         h->red.init(h->stream);
         *out = h.release();
     });
@@ -201,12 +199,20 @@ int fem_csr_transpose(fem_csr A, fem_csr* out) {
 int fem_csr_multiply(fem_csr A, fem_csr B, fem_csr* out) {
     return guard([&] {
         if (A->A.m != B->A.n) fail(FEM_ERR_INVALID_ARGUMENT, "matrix product: inner dimensions differ");
-        std::vector<uint32_t> rp, ci;
-        std::vector<double> v;
-        csr_spgemm_host(A->A, B->A, rp, ci, v);
         auto h = std::make_unique<fem_csr_s>();
         h->stream = new_stream();
-        h->A.set_host(A->A.n, B->A.m, std::move(rp), std::move(ci), std::move(v), h->stream);
+        if (rows_sorted_device(B->A, h->stream)) {
+            spgemm_device(A->A, B->A, h->A, h->stream);
+        } else {
+            // B with unsorted or repeated columns in a row: the same operator
+            // with its columns sorted and repeats summed
+            std::vector<uint32_t> rp, ci;
+            std::vector<double> v;
+            condense_device(B->A, 1, B->A.n, B->A.m, rp, ci, v, h->stream);
+            CsrMat Bs;
+            Bs.set_host(B->A.n, B->A.m, std::move(rp), std::move(ci), std::move(v), h->stream);
+            spgemm_device(A->A, Bs, h->A, h->stream);
+        }
         h->red.init(h->stream);
         *out = h.release();
     });
@@ -311,6 +317,32 @@ int fem_precond_info(fem_pc M, int* levels, uint32_t* rows, uint64_t* nnz, int c
             if (nnz) nnz[i] = M->pc->lvl_nnz[i];
         }
         if (setup_ms) *setup_ms = M->pc->setup_ms;
+    });
+}
+
+int fem_amg_level_export(fem_pc M, int level, int which, uint32_t* dims, uint32_t* row_ptr, uint32_t* col_idx,
+                         double* values) {
+    return guard([&] {
+        if (!M || !dims) fail(FEM_ERR_INVALID_ARGUMENT, "fem_amg_level_export: null argument");
+        const CsrMat* A = M->pc->amg_matrix(level, which);
+        if (!A) fail(FEM_ERR_OUT_OF_RANGE, "fem_amg_level_export: no such AMG level matrix");
+        dims[0] = A->n;
+        dims[1] = A->m;
+        dims[2] = (uint32_t)A->nnz;
+        if (!row_ptr) return;
+        A->sync_host();
+        std::copy(A->h_rp.begin(), A->h_rp.end(), row_ptr);
+        std::copy(A->h_ci.begin(), A->h_ci.begin() + A->nnz, col_idx);
+        std::copy(A->h_v.begin(), A->h_v.begin() + A->nnz, values);
+    });
+}
+
+int fem_amg_split(fem_pc M, int level, uint8_t* cf) {
+    return guard([&] {
+        if (!M || !cf) fail(FEM_ERR_INVALID_ARGUMENT, "fem_amg_split: null argument");
+        const std::vector<uint8_t>* s = M->pc->amg_split(level);
+        if (!s) fail(FEM_ERR_OUT_OF_RANGE, "fem_amg_split: no C/F split at this level");
+        std::copy(s->begin(), s->end(), cf);
     });
 }
 

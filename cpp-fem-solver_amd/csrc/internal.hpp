@@ -250,6 +250,11 @@ struct Precond {
     // the kernels it launches, each reading its matrix (uint32 indices) and
     // each vector once and writing its output once. 0: not tallied.
     virtual double apply_bytes(int /*s*/) { return 0.0; }
+    // AMG: level l's operator (which = 0), prolongator (1) or restriction (2);
+    // null when the level has no such matrix
+    virtual const CsrMat* amg_matrix(int /*level*/, int /*which*/) const { return nullptr; }
+    // C/F AMG: the split of level l (1 C, 2 F); null past the coarsened levels
+    virtual const std::vector<uint8_t>* amg_split(int /*level*/) const { return nullptr; }
     double setup_ms = 0.0;
     int levels = 0;
     std::vector<uint32_t> lvl_rows;
@@ -265,10 +270,6 @@ inline double csr_bytes(const CsrMat& M, int s, int extra) {
 
 std::unique_ptr<Precond> make_precond(CsrMat& A, int kind, const fem_pc_params& p, hipStream_t s);
 
-// Host-side sparse kernels used during setup (csr_ops.cpp).
-void csr_transpose_host(const CsrMat& A, std::vector<uint32_t>& rp, std::vector<uint32_t>& ci, std::vector<double>& v);
-void csr_spgemm_host(const CsrMat& A, const CsrMat& B, std::vector<uint32_t>& rp, std::vector<uint32_t>& ci,
-                     std::vector<double>& v);
 
 // ---------------------------------------------------------------------------
 // Vector kernels (vec.hip)

@@ -122,6 +122,8 @@ _SIGS = {
     "fem_precond_destroy": (C.c_int, [P_vp]),
     "fem_precond_info": (C.c_int, [P_vp, C.POINTER(C.c_int), P_u32, C.POINTER(C.c_uint64), C.c_int, P_f64]),
     "fem_precond_bytes": (C.c_int, [P_vp, C.c_int, P_f64]),
+    "fem_amg_level_export": (C.c_int, [P_vp, C.c_int, C.c_int, P_u32, P_u32, P_u32, P_f64]),
+    "fem_amg_split": (C.c_int, [P_vp, C.c_int, C.POINTER(C.c_uint8)]),
     "fem_solve": (C.c_int, [P_vp, P_vp, P_f64, P_f64, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     "fem_pcg": (C.c_int, [P_vp, P_vp, P_f64, C.c_double, C.c_int, P_f64, C.POINTER(C.c_int)]),
     "fem_bicgstab": (C.c_int, [P_vp, P_vp, P_f64, C.c_double, C.c_int, P_f64, C.POINTER(C.c_int)]),
@@ -359,6 +361,8 @@ class Preconditioner:
     def __init__(self, A, kind, **params):
         self.kind = PC_KINDS[kind] if isinstance(kind, str) else kind
         self._h = C.c_void_p()
+        # the preconditioner runs on its matrix's stream: keep the matrix alive
+        self._A = A
         self.n = A.rows if isinstance(A, CsrMatrix) else A.nrows
         p = pc_params(self.kind, **params)
         if isinstance(A, CsrMatrix):
@@ -380,6 +384,27 @@ class Preconditioner:
         _check(lib().fem_precond_info(self._h, C.byref(lv), _p(rows, C.c_uint32),
                                       nnz.ctypes.data_as(C.POINTER(C.c_uint64)), cap, C.byref(ms)))
         return {"levels": lv.value, "rows": rows, "nnz": nnz, "setup_ms": ms.value}
+
+    def amg_matrix(self, level, which):
+        """AMG level matrix (which: 0 operator, 1 prolongator, 2 restriction)
+        as a scipy CSR matrix (fem_amg_level_export)."""
+        import scipy.sparse as sp
+        dims = np.zeros(3, np.uint32)
+        _check(lib().fem_amg_level_export(self._h, level, which, _p(dims, C.c_uint32), None, None, None))
+        n, m, nz = (int(x) for x in dims)
+        rp = np.empty(n + 1, np.uint32)
+        ci = np.empty(max(nz, 1), np.uint32)
+        v = np.empty(max(nz, 1), np.float64)
+        _check(lib().fem_amg_level_export(self._h, level, which, _p(dims, C.c_uint32), _p(rp, C.c_uint32),
+                                          _p(ci, C.c_uint32), _p(v, C.c_double)))
+        return sp.csr_matrix((v[:nz], ci[:nz], rp), shape=(n, m))
+
+    def amg_split(self, level):
+        """C/F split of a coarsened C/F AMG level: int8[n], 1 = C, 2 = F."""
+        n = int(self.info()["rows"][level])
+        st = np.empty(n, np.uint8)
+        _check(lib().fem_amg_split(self._h, level, st.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return st.astype(np.int8)
 
     def apply_bytes(self, fp32=False):
         """Algorithmic bytes of one application (fem_precond_bytes); 0 where

@@ -23,8 +23,10 @@
  *    gfx950 device or a HIP failure; there is no CPU fallback.
  *  - Non-convergence is not an error (array.cpp:3597, 3841, 4400): solvers
  *    return FEM_OK with the reference's iteration semantics.
- *  - A handle owns one HIP stream; calls on one handle are not reentrant
- *    (the reference's preconditioner closures are not either).
+ *  - A matrix handle owns one HIP stream; calls on one handle are not
+ *    reentrant (the reference's preconditioner closures are not either).
+ *    Preconditioners and solvers run on their matrix's stream and keep
+ *    pointers into it: destroy them before the matrix.
  */
 #ifndef FEMGPU_H
 #define FEMGPU_H
@@ -157,9 +159,10 @@ int fem_csr_create_coo(uint32_t rows, uint32_t cols, uint64_t nnz, const double*
 int fem_csr_info(fem_csr A, uint32_t* rows, uint32_t* cols, uint64_t* nnz);
 /* row_ptr()/col_idx()/values(), array.hpp:300-302 (copies to host) */
 int fem_csr_export(fem_csr A, uint32_t* row_ptr, uint32_t* col_idx, double* values);
-/* SMatrix::transpose, array.cpp:2202-2235 */
+/* SMatrix::transpose, array.cpp:2202-2235 (on the device) */
 int fem_csr_transpose(fem_csr A, fem_csr* out);
-/* SMatrix::operator*, array.cpp:2336-2447 (mathematically exact product) */
+/* SMatrix::operator*, array.cpp:2336-2447 (mathematically exact product, on
+ * the device) */
 int fem_csr_multiply(fem_csr A, fem_csr B, fem_csr* out);
 int fem_csr_destroy(fem_csr A);
 /* SpMV variant of A (no counterpart in the reference, which has one loop):
@@ -196,6 +199,15 @@ int fem_precond_destroy(fem_pc M);
  * (IC0/ILU0/block ILU0) or AMG levels; fills up to cap entries of rows/nnz
  * per AMG level. */
 int fem_precond_info(fem_pc M, int* levels, uint32_t* rows, uint64_t* nnz, int cap, double* setup_ms);
+/* AMG hierarchy inspection (no counterpart in the reference, whose AMG data
+ * stays inside the std::function closure, array.cpp:3390-3398): level l's
+ * operator (which = 0), prolongator (1) or restriction (2) as CSR. dims
+ * receives rows, columns and nnz; with row_ptr NULL only dims is written.
+ * FEM_ERR_OUT_OF_RANGE if the level has no such matrix. */
+int fem_amg_level_export(fem_pc M, int level, int which, uint32_t* dims, uint32_t* row_ptr, uint32_t* col_idx,
+                         double* values);
+/* C/F AMG: the C/F split of coarsened level l (n bytes: 1 C point, 2 F point) */
+int fem_amg_split(fem_pc M, int level, uint8_t* cf);
 /* Algorithmic bytes of one application of M in float64 (fp32 = 0) or float32:
  * the sum over the kernels it launches, each reading its matrices (uint32
  * indices) and vectors once and writing its output once. 0 for the kinds

@@ -8,6 +8,7 @@ preconditioned solves reach 1e-10 with solutions within 1e-8 of the oracle's.
 """
 import numpy as np
 import pytest
+import scipy.sparse as sp
 
 import femgpu as F
 import oracle_api as O
@@ -101,7 +102,94 @@ def _amg_case(kind, exact=0, smoother=F.SMOOTH_JACOBI, npre=1, npost=1, omega=No
     return G, A, M, Ao, Mo
 
 
-@pytest.mark.parametrize("kind,exact", [("amg", 0), ("saamg", 0), ("saamg", 1)])
+def _cf_levels(M):
+    info = M.info()
+    L = info["levels"]
+    As = [M.amg_matrix(l, 0) for l in range(L)]
+    Ps = [M.amg_matrix(l, 1) for l in range(L - 1)]
+    Rs = [M.amg_matrix(l, 2) for l in range(L - 1)]
+    return info, As, Ps, Rs
+
+
+@pytest.mark.parametrize("case", ["poisson2d", "aniso2d", "vardiff3d", "convdiff2d"])
+def test_cf_amg_device_setup_invariants(case):
+    """The device C/F setup (its own parallel split) against what any C/F
+    split with the reference's direct interpolation must satisfy: every F
+    point has a strong C neighbour, C rows of P are unit rows, interior F
+    rows with all strong neighbours C interpolate constants, R = P^T,
+    A_c = P^T A P; and on symmetric strength graphs no two C points are
+    strongly coupled. Then one V-cycle equals a numpy restatement from the
+    exported level operators."""
+    from amg_checks import cf_interpolation_checks, cf_vcycle, galerkin_error
+    G = {"poisson2d": lambda: gen.poisson2d(40), "aniso2d": lambda: gen.aniso2d(40, 0.01),
+         "vardiff3d": lambda: gen.vardiff3d(12, seed=1)[0], "convdiff2d": lambda: gen.convdiff2d(40)}[case]()
+    A = F.CsrMatrix.from_gen(G)
+    M = F.Preconditioner(A, "amg")
+    info, As, Ps, Rs = _cf_levels(M)
+    L = info["levels"]
+    assert L >= 2 and As[-1].shape[0] <= 300 and As[0].shape[0] == G.n
+    np.testing.assert_array_equal(As[0].toarray(), G.to_scipy().toarray())
+    covered = 0
+    for l in range(L - 1):
+        print(f"level {l}: rows {As[l].shape[0]} nnz {As[l].nnz} -> {Ps[l].shape[1]} C points", flush=True)
+        st = M.amg_split(l)
+        covered += cf_interpolation_checks(As[l], Ps[l], st)
+        assert (Rs[l] != Ps[l].T.tocsr()).nnz == 0
+        assert galerkin_error(As[l], Ps[l], As[l + 1]) < 1e-12
+        # two points strongly coupled both ways are never both C
+        Al = As[l].tocoo()
+        off = Al.row != Al.col
+        mx = np.zeros(Al.shape[0])
+        np.maximum.at(mx, Al.row[off], np.abs(Al.data[off]))
+        strong = off & (np.abs(Al.data) >= 0.25 * mx[Al.row]) & (Al.data != 0)
+        S = sp.csr_matrix((strong.astype(np.int8), (Al.row, Al.col)), shape=Al.shape)
+        mutual = S.multiply(S.T).tocoo()
+        assert not np.any((st[mutual.row] == 1) & (st[mutual.col] == 1)), "mutually strong C points"
+    if case in ("poisson2d", "aniso2d"):
+        assert covered > 0
+    r = np.random.default_rng(0).standard_normal(G.n)
+    z = M.apply(r)
+    zr = cf_vcycle(As, Ps, Rs, r)
+    np.testing.assert_allclose(z, zr, rtol=1e-11, atol=1e-11 * np.abs(zr).max())
+
+
+def test_cf_amg_unsorted_rows_and_products():
+    """A CSR given with unsorted, repeated columns is set up as the same
+    operator (columns sorted, repeats summed); the device products behind
+    SMatrix::operator* and transpose match scipy."""
+    import scipy.sparse as sp
+    G = gen.poisson2d(24)
+    S = G.to_scipy().tocoo()
+    rng = np.random.default_rng(3)
+    order = rng.permutation(S.nnz)
+    r, c, v = S.row[order], S.col[order], S.data[order]
+    # split every diagonal entry into two halves stored in the same row
+    d = r == c
+    r2 = np.concatenate([r, r[d]])
+    c2 = np.concatenate([c, c[d]])
+    v2 = np.concatenate([np.where(d, v / 2, v), v[d] / 2])
+    srt = np.argsort(r2, kind="stable")
+    r2, c2, v2 = r2[srt], c2[srt], v2[srt]
+    rp = np.zeros(G.n + 1, np.uint32)
+    np.cumsum(np.bincount(r2, minlength=G.n), out=rp[1:])
+    A = F.CsrMatrix(G.n, G.n, rp, c2, v2)
+    Ms = F.Preconditioner(A, "amg")
+    Mr = F.Preconditioner(F.CsrMatrix.from_gen(G), "amg")
+    x = rng.standard_normal(G.n)
+    np.testing.assert_allclose(Ms.apply(x), Mr.apply(x), rtol=1e-12, atol=1e-12)
+    B = sp.random(G.n, 50, density=0.05, random_state=1, format="csr")
+    B.sort_indices()
+    Af = F.CsrMatrix.from_gen(G)
+    Bf = F.CsrMatrix(G.n, 50, B.indptr, B.indices, B.data)
+    C = (Af @ Bf).export()
+    ref = (G.to_scipy() @ B).toarray()
+    got = sp.csr_matrix((C[2], C[1], C[0]), shape=(G.n, 50)).toarray()
+    np.testing.assert_allclose(got, ref, rtol=1e-14, atol=1e-14)
+    T = Bf.transpose().export()
+    np.testing.assert_array_equal(sp.csr_matrix((T[2], T[1], T[0]), shape=(50, G.n)).toarray(), B.T.toarray())
+
+
+@pytest.mark.parametrize("kind,exact", [("saamg", 0), ("saamg", 1)])
 def test_amg_hierarchy_and_vcycle(kind, exact):
     G, A, M, Ao, Mo = _amg_case(kind, exact)
     info = M.info()
@@ -127,9 +215,15 @@ def test_amg_pcg_parity(kind):
     _, b = gen.rhs(G, 0)
     x, st = F.solve(A, M, b, solver="pcg")
     xo, ito = O.solve(Ao, 0, Mo, b)
+    print(f"{kind}: gpu iterations {st['iterations']}, oracle {ito}", flush=True)
     assert st["true_relres"] < 1.5e-10
     assert rel(x, xo) < 1e-8
-    assert abs(st["iterations"] - ito) <= 2
+    if kind == "saamg":
+        assert abs(st["iterations"] - ito) <= 2
+    else:
+        # the device C/F split is not the reference's serial one: the
+        # hierarchy, hence the iteration count, differs
+        assert st["iterations"] <= 1.3 * ito + 2
 
 
 def test_robust_saamg_lognormal_parity():

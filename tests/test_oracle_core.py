@@ -10,6 +10,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spla
 
 import oracle_api as O
+from amg_checks import cf_interpolation_checks
 from femgpu import gen
 
 
@@ -326,42 +327,6 @@ def test_cf_amg_hierarchy_invariants():
         assert len(unit_rows) >= m2
 
 
-def cf_interpolation_checks(A, P, st, theta=0.25):
-    """Invariants of a C/F level with direct interpolation, independent of
-    how the split was chosen: C rows of P are unit rows numbering the C points
-    in row order; every F point has a strong C neighbour and its P row lists
-    exactly its strong C neighbours; an F row of zero row sum whose strong
-    neighbours are all C interpolates constants (row sum of P = 1). Returns
-    the number of F rows the last check covered."""
-    A = A.tocsr()
-    P = P.tocsr()
-    n = A.shape[0]
-    assert set(np.unique(st)) <= {1, 2}
-    cpts = np.flatnonzero(st == 1)
-    assert P.shape == (n, cpts.size)
-    mapc = -np.ones(n, np.int64)
-    mapc[cpts] = np.arange(cpts.size)
-    covered = 0
-    for i in range(n):
-        lo, hi = A.indptr[i], A.indptr[i + 1]
-        cols, vals = A.indices[lo:hi], A.data[lo:hi]
-        off = cols != i
-        mx = np.abs(vals[off]).max() if off.any() else 0.0
-        strong = set(cols[off & (np.abs(vals) >= theta * mx)].tolist())
-        pc = P.indices[P.indptr[i]:P.indptr[i + 1]]
-        pv = P.data[P.indptr[i]:P.indptr[i + 1]]
-        if st[i] == 1:
-            assert pc.tolist() == [mapc[i]] and pv.tolist() == [1.0]
-            continue
-        sc = sorted(mapc[j] for j in strong if st[j] == 1)
-        assert sc, f"F point {i} has no strong C neighbour"
-        assert sorted(pc.tolist()) == sc
-        if abs(vals.sum()) < 1e-14 * np.abs(vals).max() and all(st[j] == 1 for j in strong):
-            assert abs(pv.sum() - 1.0) < 1e-13
-            covered += 1
-    return covered
-
-
 def test_cf_amg_interpolation_with_weak_couplings():
     """Anisotropic Laplacian (eps = 0.01 < theta): the y couplings are weak
     and are lumped into the interpolation's diagonal, which the constant
@@ -375,6 +340,28 @@ def test_cf_amg_interpolation_with_weak_couplings():
     P = sp.csr_matrix((v, ci, rp), shape=(n, m))
     covered = cf_interpolation_checks(G.to_scipy(), P, pc.cf_split(0))
     assert covered > G.n // 4
+
+
+def test_cf_vcycle_restatement_matches_oracle():
+    """The numpy V-cycle the GPU tests use (amg_checks.cf_vcycle), run on the
+    oracle's own exported hierarchy, equals the oracle's V-cycle."""
+    from amg_checks import cf_vcycle
+    G = gen.poisson2d(30)
+    pc = O.Pc(O.Csr.from_gen(G), O.K_AMG, [0.25, 300, 25])
+    rows, _ = pc.amg_levels()
+    mats = []
+    for l in range(len(rows)):
+        trip = []
+        for which in ((0, 1, 2) if l + 1 < len(rows) else (0,)):
+            n, m, rp, ci, v = pc.amg_matrix(l, which)
+            trip.append(sp.csr_matrix((v, ci, rp), shape=(n, m)))
+        mats.append(trip)
+    As = [t[0] for t in mats]
+    Ps = [t[1] for t in mats[:-1]]
+    Rs = [t[2] for t in mats[:-1]]
+    r = np.random.default_rng(4).standard_normal(G.n)
+    z = pc.apply(r)
+    np.testing.assert_allclose(cf_vcycle(As, Ps, Rs, r), z, rtol=1e-11, atol=1e-11 * np.abs(z).max())
 
 
 def test_amg_vcycle_symmetric_and_solves():
