@@ -281,6 +281,10 @@ int fem_dot(uint64_t n, const double* a, const double* b, double* out) {
     });
 }
 
+int fem_dot_device_noalloc(uint64_t n, const double* a, const double* b, double* out, void* stream) {
+    return guard([&] { dot_noalloc(n, a, b, out, static_cast<hipStream_t>(stream)); });
+}
+
 int fem_precond_create(fem_csr A, int kind, const fem_pc_params* params, fem_pc* out) {
     return guard([&] {
         if (!A || !out) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");

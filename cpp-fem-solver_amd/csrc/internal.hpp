@@ -287,5 +287,9 @@ void vec_convert(TD* y, const TS* x, double scale, size_t n, hipStream_t s, Gate
 // read n doubles (cache eviction without dirty lines); sink: `blocks` doubles
 void cache_flush_read(const double* buf, size_t n, double* sink, unsigned blocks, hipStream_t s);
 double dot_sync(const double* a, const double* b, size_t n, const Reducer& R, hipStream_t s);
+// out = a . b on device pointers, enqueued on s, with no allocation: the
+// partial sums go to a static device array (vec.hip); calls that overlap on
+// one device race on it
+void dot_noalloc(size_t n, const double* a, const double* b, double* out, hipStream_t s);
 
 }  // namespace fem

@@ -136,4 +136,44 @@ double dot_sync(const double* a, const double* b, size_t n, const Reducer& R, hi
     return h;
 }
 
+// ---- dot product without allocation ------------------------------------------
+// The per-block partial sums of dot_noalloc live in this file-scope device
+// array (one copy per device, kept until the process ends), not in memory
+// the library allocates. Host code never passes the array's name to a
+// kernel: the array's address on a device comes from hipGetSymbolAddress,
+// looked up once per device with that device current.
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kMaxRedBlocks];
+
+static double* scratch_on_current_device() {
+    static std::mutex mu;
+    static double* addr[kMaxDevices] = {};
+    int dev = 0;
+    FEM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) fail(FEM_ERR_DEVICE, "dot_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lk(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(FEM_ERR_DEVICE, std::string("dot_noalloc: hipGetSymbolAddress failed: ") + hipGetErrorString(e));
+        addr[dev] = static_cast<double*>(p);
+    }
+    return addr[dev];
+}
+
+// out = a . b on device pointers, enqueued on s: dot_sync's kernels and
+// grid (the same sums in the same order), with the partial sums in the
+// device's scratch array. Nothing is allocated.
+void dot_noalloc(size_t n, const double* a, const double* b, double* out, hipStream_t s) {
+    if (n >= (1ull << 32)) fail(FEM_ERR_INVALID_ARGUMENT, "dot_noalloc: vectors of 2^32 or more entries");
+    if (!a || !b || !out) fail(FEM_ERR_INVALID_ARGUMENT, "dot_noalloc: null pointer");
+    const unsigned G = red_blocks(n);
+    if (G > (unsigned)kMaxRedBlocks) fail(FEM_ERR_INVALID_ARGUMENT, "dot_noalloc: more partial sums than the scratch holds");
+    double* part = scratch_on_current_device();
+    hipLaunchKernelGGL(k_dot, dim3(G), dim3(kBlock), 0, s, a, b, n, part);
+    launch_finalize<1>(part, G, FinOut{out}, s);
+    FEM_HIP(hipGetLastError());
+}
+
 }  // namespace fem

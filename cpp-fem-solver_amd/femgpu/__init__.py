@@ -117,6 +117,7 @@ _SIGS = {
     "fem_spmv": (C.c_int, [P_vp, P_f64, P_f64]),
     "fem_spmv_device": (C.c_int, [P_vp, P_vp, P_vp, C.c_int]),
     "fem_dot": (C.c_int, [C.c_uint64, P_f64, P_f64, P_f64]),
+    "fem_dot_device_noalloc": (C.c_int, [C.c_uint64, P_vp, P_vp, P_vp, P_vp]),
     "fem_precond_create": (C.c_int, [P_vp, C.c_int, C.POINTER(PcParams), C.POINTER(P_vp)]),
     "fem_precond_apply": (C.c_int, [P_vp, P_f64, P_f64]),
     "fem_precond_destroy": (C.c_int, [P_vp]),

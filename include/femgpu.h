@@ -190,6 +190,18 @@ int fem_spmv_device(fem_csr A, const void* x, void* y, int fp32);
 /* SMatrix::dot, array.cpp:3401-3416 */
 int fem_dot(uint64_t n, const double* a, const double* b, double* out);
 
+/* No counterpart in the reference: fem_dot for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops). a, b and
+ * out are device pointers on the calling thread's current device; a[0..n) .
+ * b[0..n) is written to *out, enqueued on `stream` (a hipStream_t; NULL = the
+ * null stream), and the call returns without waiting. It allocates nothing:
+ * its partial sums live in a static device array of 2048 doubles (16 KB) per
+ * device, held until the process ends. Kernels and summation order are
+ * fem_dot's, so the result is bit-identical. n must be below 2^32
+ * (FEM_ERR_INVALID_ARGUMENT otherwise). Calls that overlap on one device
+ * share that array and race: order them (one stream, or events). */
+int fem_dot_device_noalloc(uint64_t n, const double* a, const double* b, double* out, void* stream);
+
 /* ---- preconditioners (SMatrix::make_*_preconditioner, array.hpp:309-313) -- */
 int fem_precond_create(fem_csr A, int kind, const fem_pc_params* params, fem_pc* out);
 /* z = M^{-1} r: the std::function the reference factories return */
