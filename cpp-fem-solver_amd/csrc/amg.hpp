@@ -96,6 +96,12 @@ void sgs_setup_device(AmgLevelDev& D, hipStream_t s);
 void drop_columns_device(const CsrMat& R, const uint32_t* rows, uint32_t count, CsrMat& out, hipStream_t s);
 // the float32 sweep blocks (gblk32, h_cblk32) of a level set up by sgs_setup_device
 void sgs_blocks32(AmgLevelDev& D, hipStream_t s);
+// aggregation by a distance-2 maximal independent set of the strength graph
+// on the device: v holds the entries (norms = false) or one norm per entry
+// (norms = true); agg (n entries) receives the aggregates, their count is
+// returned
+uint32_t mis2_aggregate_device(const uint32_t* rp, const uint32_t* ci, const double* v, bool norms, uint32_t n,
+                               uint64_t nnz, double theta, int* agg, hipStream_t s);
 // true if every row of A lists strictly increasing columns
 bool rows_sorted_device(const CsrMat& A, hipStream_t s);
 

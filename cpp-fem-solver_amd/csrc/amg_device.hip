@@ -94,22 +94,32 @@ __global__ __launch_bounds__(kBlock) void k_gersh(const uint32_t* rp, const doub
     if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
 }
 
+// |a_ij| as sqrt(a_ij^2) (a scalar matrix) or the entry as given (NORM: v
+// already holds a norm per entry, e.g. the Frobenius norms of 3x3 blocks)
+template <bool NORM>
+__device__ __forceinline__ double entry_norm(const double* v, uint32_t p) {
+    if constexpr (NORM) return v[p];
+    else return sqrt(v[p] * v[p]);
+}
+
+template <bool NORM>
 __global__ void k_rowmax(const uint32_t* rp, const uint32_t* ci, const double* v, uint32_t n, double* m) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     double mx = 0.0;
     for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
-        if (ci[p] != i) mx = fmax(mx, sqrt(v[p] * v[p]));
+        if (ci[p] != i) mx = fmax(mx, entry_norm<NORM>(v, p));
     m[i] = mx;
 }
 
+template <bool NORM>
 __global__ void k_strong(const uint32_t* rp, const uint32_t* ci, const double* v, const double* m, double theta,
                          uint32_t n, uint8_t* sm) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
         const uint32_t j = ci[p];
-        const double a = sqrt(v[p] * v[p]);
+        const double a = entry_norm<NORM>(v, p);
         sm[p] = (j != i && a != 0.0 && a >= theta * sqrt(m[i] * m[j])) ? 1 : 0;
     }
 }
@@ -824,6 +834,47 @@ AmgDevHierarchy upload_hierarchy(AmgHierarchy&& H, hipStream_t s) {
     return D;
 }
 
+// Aggregation by a distance-2 maximal independent set of the strength
+// graph (the rounds, attachment phases and singleton leftovers described at
+// the top of this file). v holds the matrix entries (norms = false: |a_ij|
+// is sqrt(a_ij^2)) or a norm per entry (norms = true). agg receives n
+// aggregate indices; returns the number of aggregates.
+uint32_t mis2_aggregate_device(const uint32_t* rp, const uint32_t* ci, const double* v, bool norms, uint32_t n,
+                               uint64_t nnz, double theta, int* agg, hipStream_t s) {
+    DevBuf<double> mrow(n);
+    DevBuf<uint8_t> sm(std::max<uint64_t>(nnz, 1)), st(n);
+    if (norms) {
+        hipLaunchKernelGGL(k_rowmax<true>, dim3(g1(n)), dim3(256), 0, s, rp, ci, v, n, mrow.get());
+        hipLaunchKernelGGL(k_strong<true>, dim3(g1(n)), dim3(256), 0, s, rp, ci, v, mrow.get(), theta, n, sm.get());
+    } else {
+        hipLaunchKernelGGL(k_rowmax<false>, dim3(g1(n)), dim3(256), 0, s, rp, ci, v, n, mrow.get());
+        hipLaunchKernelGGL(k_strong<false>, dim3(g1(n)), dim3(256), 0, s, rp, ci, v, mrow.get(), theta, n, sm.get());
+    }
+    FEM_HIP(hipMemsetAsync(st.get(), 1, n, s));
+    DevBuf<uint64_t> t1(n);
+    DevBuf<uint32_t> und(1);
+    for (int round = 0;; ++round) {
+        if (round > 10000) fail(FEM_ERR_RUNTIME, "AMG setup: aggregation did not finish");
+        FEM_HIP(hipMemsetAsync(und.get(), 0, 4, s));
+        hipLaunchKernelGGL(k_mis_max1, dim3(g1(n)), dim3(256), 0, s, rp, ci, sm.get(), st.get(), n, t1.get());
+        hipLaunchKernelGGL(k_mis_max2, dim3(g1(n)), dim3(256), 0, s, rp, ci, sm.get(), t1.get(), n, st.get(), und.get());
+        if (read_u32(und.get(), s) == 0) break;
+    }
+    DevBuf<uint32_t> f(n + 1), pos(n + 1);
+    hipLaunchKernelGGL(k_flag, dim3(g1(n + 1)), dim3(256), 0, s, st.get(), n, (uint8_t)2, f.get());
+    scan_u32(f.get(), pos.get(), (size_t)n + 1, s);
+    const uint32_t nroots = read_u32(pos.get() + n, s);
+    DevBuf<int> a1(n);
+    hipLaunchKernelGGL(k_phase1, dim3(g1(n)), dim3(256), 0, s, rp, ci, sm.get(), st.get(), pos.get(), n, a1.get());
+    hipLaunchKernelGGL(k_phase2, dim3(g1(n + 1)), dim3(256), 0, s, rp, ci, sm.get(), a1.get(), n, agg, f.get());
+    scan_u32(f.get(), pos.get(), (size_t)n + 1, s);
+    const uint32_t nleft = read_u32(pos.get() + n, s);
+    hipLaunchKernelGGL(k_phase3, dim3(g1(n)), dim3(256), 0, s, pos.get(), nroots, n, agg);
+    FEM_HIP(hipGetLastError());
+    FEM_HIP(hipStreamSynchronize(s));  // the temporaries end with this call
+    return nroots + nleft;
+}
+
 // The robust SA hierarchy built on the device; every level operator,
 // prolongator and restriction stays there (only the coarsest operator is
 // copied to the host, for its dense solve).
@@ -877,36 +928,9 @@ AmgDevHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params&
             break;
         }
         // strength and MIS-2 aggregation
-        DevBuf<double> mrow(n);
-        DevBuf<uint8_t> sm(std::max<uint64_t>(cur.nnz, 1)), st(n);
-        hipLaunchKernelGGL(k_rowmax, dim3(g1(n)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), cur.v.get(), n, mrow.get());
-        hipLaunchKernelGGL(k_strong, dim3(g1(n)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), cur.v.get(), mrow.get(),
-                           theta, n, sm.get());
-        FEM_HIP(hipMemsetAsync(st.get(), 1, n, s));
-        DevBuf<uint64_t> t1(n);
-        DevBuf<uint32_t> und(1);
-        for (int round = 0;; ++round) {
-            if (round > 10000) fail(FEM_ERR_RUNTIME, "AMG setup: aggregation did not finish");
-            FEM_HIP(hipMemsetAsync(und.get(), 0, 4, s));
-            hipLaunchKernelGGL(k_mis_max1, dim3(g1(n)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), sm.get(), st.get(),
-                               n, t1.get());
-            hipLaunchKernelGGL(k_mis_max2, dim3(g1(n)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), sm.get(), t1.get(),
-                               n, st.get(), und.get());
-            if (read_u32(und.get(), s) == 0) break;
-        }
-        DevBuf<uint32_t> f(n + 1), pos(n + 1);
-        hipLaunchKernelGGL(k_flag, dim3(g1(n + 1)), dim3(256), 0, s, st.get(), n, (uint8_t)2, f.get());
-        scan_u32(f.get(), pos.get(), (size_t)n + 1, s);
-        const uint32_t nroots = read_u32(pos.get() + n, s);
-        DevBuf<int> a1(n), agg(n);
-        hipLaunchKernelGGL(k_phase1, dim3(g1(n)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), sm.get(), st.get(),
-                           pos.get(), n, a1.get());
-        hipLaunchKernelGGL(k_phase2, dim3(g1(n + 1)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), sm.get(), a1.get(), n,
-                           agg.get(), f.get());
-        scan_u32(f.get(), pos.get(), (size_t)n + 1, s);
-        const uint32_t nleft = read_u32(pos.get() + n, s);
-        hipLaunchKernelGGL(k_phase3, dim3(g1(n)), dim3(256), 0, s, pos.get(), nroots, n, agg.get());
-        const uint32_t na = nroots + nleft;
+        DevBuf<int> agg(n);
+        const uint32_t na = mis2_aggregate_device(cur.rp.get(), cur.ci.get(), cur.v.get(), false, n, cur.nnz, theta,
+                                                  agg.get(), s);
         tr.lap("strength, MIS-2 aggregation");
         if ((double)na > 0.85 * (double)n) {  // coarsening stalled
             cur.to_mat(lv->A, s);
@@ -918,6 +942,7 @@ AmgDevHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params&
         P.n = n;
         P.m = na;
         P.rp.alloc(n + 1);
+        DevBuf<uint32_t> f(n + 1);
         FEM_HIP(hipMemsetAsync(flag.get(), 0, 4, s));
         hipLaunchKernelGGL(k_pcount, dim3(g1(n + 1)), dim3(256), 0, s, cur.rp.get(), cur.ci.get(), agg.get(), n, f.get(),
                            flag.get());

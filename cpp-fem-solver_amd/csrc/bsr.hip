@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <cmath>
 
+#include <hipcub/hipcub.hpp>
+
 #include "amg.hpp"
 #include "handles.hpp"
 #include "precond.hpp"
@@ -486,13 +488,491 @@ struct PcIdentity : Precond {
     bool has_fp32() const override { return true; }
 };
 
-// condense (blockMatrix.hpp:269-315): block column j/BS, duplicates summed in
-// scalar CSR order, block columns sorted and unique, explicit zeros kept
+// ---------------------------------------------------------------------------
+// Block smoothed-aggregation AMG (make_block_amg_preconditioner,
+// blockMatrix.hpp:868-1179) for BS x BS blocks, set up and applied on the
+// device. Per level, with the block matrix A (BSR, block rows sorted):
+//   D^{-1}      inverse diagonal blocks (Gauss-Jordan with partial pivoting,
+//               zero where the inversion fails, as the reference ignores it)
+//   strength    Frobenius norms of the blocks: the reference's
+//               |A_ij|_F >= theta max_{k != i} |A_ik|_F with greedy aggregates
+//               of at most max_agg block rows (explicit weights, p_omega > 0:
+//               a serial pass over the block rows, run on the host from the
+//               device-computed norms), or the robust variant (p_omega <= 0):
+//               |A_ij|_F >= theta sqrt(m_i m_j) and distance-2 MIS aggregation
+//               on the device
+//   P           blocks P_ig = [g = agg(i)] I - w sum_{k: agg(k) = g} D_i^{-1} A_ik
+//               (per block row the aggregates ascending, each block's terms in
+//               row order: the reference's values bit for bit)
+//   Galerkin    A_c = P^T (A P), formed on the scalar expansions (every block
+//               stored as its BS x BS entries, zeros included) with the
+//               device sparse products, then condensed back to blocks
+//   V-cycle     block-Jacobi sweeps u += w D^{-1} (f - A u) (npre before,
+//               npost after the coarse correction), residual, restriction and
+//               prolongation as scalar SpMVs on the expansions, the coarsest
+//               level solved by its explicit dense inverse.
+// Only the summation order of the products and of the dense coarse solve
+// differs from the reference, which rounds differently in the last bits.
+// ---------------------------------------------------------------------------
+template <int BS>
+__global__ void k_bfro(uint64_t nnzb, const double* v, double* nv) {
+    const uint64_t p = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (p >= nnzb) return;
+    const double* b = v + p * BS * BS;
+    double s = 0.0;
+    for (int t = 0; t < BS * BS; ++t) s += b[t] * b[t];
+    nv[p] = sqrt(s);
+}
+
+template <int BS>
+__global__ void k_bdinv(uint32_t nb, const uint32_t* rp, const uint32_t* ci, const double* v, double* dinv,
+                        int* missing) {
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    uint32_t dp = UINT32_MAX;
+    for (uint32_t p = rp[b]; p < rp[b + 1]; ++p)
+        if (ci[p] == b) { dp = p; break; }
+    double* out = dinv + (size_t)b * BS * BS;
+    if (dp == UINT32_MAX) { atomicMin(missing, (int)b); return; }
+    double t[BS * BS];
+    if (d_blk_invert<BS>(v + (size_t)dp * BS * BS, t))
+        for (int k = 0; k < BS * BS; ++k) out[k] = t[k];
+    else
+        for (int k = 0; k < BS * BS; ++k) out[k] = 0.0;
+}
+
+// per workgroup: max over its block rows and components r of
+// sum_p sum_c |(D_i^{-1} A_ip)_rc| (the Gershgorin bound of D^{-1} A)
+template <int BS>
+__global__ __launch_bounds__(256) void k_bgersh(uint32_t nb, const uint32_t* rp, const double* v, const double* dinv,
+                                                double* partials) {
+    __shared__ double sh[256];
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    double mx = 0.0;
+    if (b < nb) {
+        const double* di = dinv + (size_t)b * BS * BS;
+        double rs[BS] = {};
+        for (uint32_t p = rp[b]; p < rp[b + 1]; ++p) {
+            const double* ab = v + (size_t)p * BS * BS;
+            for (int r = 0; r < BS; ++r)
+                for (int c = 0; c < BS; ++c) {
+                    double sum = 0.0;
+                    for (int m = 0; m < BS; ++m) sum += di[r * BS + m] * ab[m * BS + c];
+                    rs[r] += fabs(sum);
+                }
+        }
+        for (int r = 0; r < BS; ++r) mx = fmax(mx, rs[r]);
+    }
+    sh[threadIdx.x] = mx;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + o]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+constexpr int kMaxBPRow = 64;  // distinct aggregates per block row of P
+
+// distinct aggregates of block row i ({agg[i]} and those of its columns), ascending
+__device__ __forceinline__ int bp_row_aggs(const uint32_t* rp, const uint32_t* ci, const int* agg, uint32_t i,
+                                           int (&c)[kMaxBPRow], int* overflow) {
+    int k = 0;
+    auto add = [&](int g) {
+        for (int t = 0; t < k; ++t)
+            if (c[t] == g) return;
+        if (k == kMaxBPRow) { atomicOr(overflow, 1); return; }
+        int t = k++;
+        while (t > 0 && c[t - 1] > g) { c[t] = c[t - 1]; --t; }
+        c[t] = g;
+    };
+    add(agg[i]);
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) add(agg[ci[p]]);
+    return k;
+}
+
+__global__ void k_bpcount(uint32_t nb, const uint32_t* rp, const uint32_t* ci, const int* agg, uint32_t* cnt,
+                          int* overflow) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > nb) return;
+    if (i == nb) { cnt[nb] = 0u; return; }
+    int c[kMaxBPRow];
+    cnt[i] = (uint32_t)bp_row_aggs(rp, ci, agg, i, c, overflow);
+}
+
+template <int BS>
+__global__ void k_bpfill(uint32_t nb, const uint32_t* rp, const uint32_t* ci, const double* v, const int* agg,
+                         const double* dinv, double pw, const uint32_t* prp, uint32_t* pci, double* pv,
+                         int* overflow) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nb) return;
+    int c[kMaxBPRow];
+    const int k = bp_row_aggs(rp, ci, agg, i, c, overflow);
+    const double* di = dinv + (size_t)i * BS * BS;
+    const uint32_t o = prp[i];
+    for (int t = 0; t < k; ++t) {
+        double blk[BS * BS];
+        for (int r = 0; r < BS; ++r)
+            for (int cc = 0; cc < BS; ++cc) blk[r * BS + cc] = (c[t] == agg[i] && r == cc) ? 1.0 : 0.0;
+        for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
+            if (agg[ci[p]] != c[t]) continue;
+            const double* ab = v + (size_t)p * BS * BS;
+            for (int r = 0; r < BS; ++r)
+                for (int cc = 0; cc < BS; ++cc) {
+                    double sum = 0.0;
+                    for (int m = 0; m < BS; ++m) sum += di[r * BS + m] * ab[m * BS + cc];
+                    blk[r * BS + cc] -= pw * sum;
+                }
+        }
+        pci[o + t] = (uint32_t)c[t];
+        for (int q = 0; q < BS * BS; ++q) pv[(size_t)(o + t) * BS * BS + q] = blk[q];
+    }
+}
+
+// scalar expansion of a block matrix: scalar row b*BS + r lists, block by
+// block, the BS entries of that row of each block (explicit zeros kept)
+template <int BS>
+__global__ void k_bexpand(uint32_t nb, const uint32_t* rp, const uint32_t* ci, const double* v, uint32_t* srp,
+                          uint32_t* sci, double* sv) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row > nb * BS) return;
+    if (row == nb * BS) { srp[row] = rp[nb] * BS * BS; return; }
+    const uint32_t b = row / BS, r = row % BS;
+    const uint32_t len = (rp[b + 1] - rp[b]) * BS;
+    uint32_t o = rp[b] * BS * BS + r * len;
+    srp[row] = o;
+    for (uint32_t p = rp[b]; p < rp[b + 1]; ++p)
+        for (int c = 0; c < BS; ++c) {
+            sci[o] = ci[p] * BS + c;
+            sv[o] = v[(size_t)p * BS * BS + r * BS + c];
+            ++o;
+        }
+}
+
+// u = w D^{-1} f (a first sweep from u = 0), or u += w D^{-1} res
+template <typename T, int BS, bool FROM_ZERO>
+__global__ void k_bjac_sweep(uint32_t nb, const T* __restrict__ dinv, const T* __restrict__ res, T* __restrict__ u,
+                             T w, Gate g) {
+    if (g.closed()) return;
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nb * BS) return;
+    const uint32_t b = row / BS, c = row % BS;
+    const T* d = dinv + (size_t)b * BS * BS + c * BS;
+    const T* x = res + (size_t)b * BS;
+    T corr = (T)0;
+#pragma unroll
+    for (int r = 0; r < BS; ++r) corr += d[r] * x[r];
+    u[row] = FROM_ZERO ? (T)0 + w * corr : u[row] + w * corr;
+}
+
+template <typename T>
+__global__ void k_bzero(uint32_t n, T* u, Gate g) {
+    if (g.closed()) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) u[i] = (T)0;
+}
+
+// u = C f with the dense coarsest inverse C (float64 sums)
+template <typename T>
+__global__ void k_bcoarse(uint32_t n, const double* __restrict__ C, const T* __restrict__ f, T* __restrict__ u,
+                          Gate g) {
+    if (g.closed()) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc = 0.0;
+    for (uint32_t j = 0; j < n; ++j) acc += C[(size_t)i * n + j] * (double)f[j];
+    u[i] = (T)acc;
+}
+
+static inline dim3 grid256(uint64_t n) { return dim3((unsigned)std::max<uint64_t>(1, (n + 255) / 256)); }
+
+template <int BS>
+struct PcBlockAmg : Precond {
+    static constexpr int BV = BS * BS;
+    struct Level {
+        uint32_t nb = 0;
+        double om = 0.0;
+        CsrMat A, P, R;  // scalar expansions
+        DevBuf<double> dinv, f, u, res;
+        DevBuf<float> dinv32, f32, u32, res32;
+    };
+    std::vector<std::unique_ptr<Level>> lv;
+    DevBuf<double> coarse;
+    uint32_t nco = 0;
+    int npre, npost;
+    bool fp32 = false;
+
+    PcBlockAmg(const BsrMat& A0, const fem_pc_params& prm, hipStream_t s) : npre(prm.npre), npost(prm.npost) {
+        if (A0.nbr != A0.nbc) fail(FEM_ERR_INVALID_ARGUMENT, "Block AMG needs a square block matrix");
+        build(A0, prm, s);
+        levels = (int)lv.size();
+        for (auto& L : lv) {
+            lvl_rows.push_back(L->nb);
+            lvl_nnz.push_back(L->A.nnz / BV);
+        }
+    }
+
+    static void expand(uint32_t nb, uint32_t nbc, uint64_t nnzb, const uint32_t* rp, const uint32_t* ci,
+                       const double* v, CsrMat& out, hipStream_t s) {
+        DevBuf<uint32_t> srp((size_t)nb * BS + 1), sci(nnzb * BV + 4);
+        DevBuf<double> sv(nnzb * BV + 4);
+        FEM_HIP(hipMemsetAsync(sci.get() + nnzb * BV, 0, 4 * sizeof(uint32_t), s));
+        FEM_HIP(hipMemsetAsync(sv.get() + nnzb * BV, 0, 4 * sizeof(double), s));
+        hipLaunchKernelGGL((k_bexpand<BS>), grid256((uint64_t)nb * BS + 1), dim3(256), 0, s, nb, rp, ci, v, srp.get(),
+                           sci.get(), sv.get());
+        FEM_HIP(hipGetLastError());
+        out.adopt(nb * BS, nbc * BS, nnzb * BV, std::move(srp), std::move(sci), std::move(sv), s);
+    }
+
+    void build(const BsrMat& A0, const fem_pc_params& prm, hipStream_t s) {
+        const bool robust = prm.p_omega <= 0.0;
+        const bool auto_s = prm.omega <= 0.0;
+        const int maxl = std::max(1, prm.max_levels);
+        const uint32_t cmax = (uint32_t)std::max(1, prm.coarse_size);
+        const int max_agg = std::max(1, prm.max_agg);
+        // current level's block matrix on the device (level 0: the input)
+        uint32_t nb = A0.nbr;
+        uint64_t nnzb = A0.nnzb;
+        DevBuf<uint32_t> brp, bci;
+        DevBuf<double> bv;
+        const uint32_t* rp = A0.rp.get();
+        const uint32_t* ci = A0.ci.get();
+        const double* v = A0.v.get();
+        DevBuf<int> flag(1);
+        for (int l = 0; l < maxl; ++l) {
+            auto L = std::make_unique<Level>();
+            L->nb = nb;
+            L->dinv.alloc((size_t)nb * BV);
+            const int big = 0x7fffffff;
+            FEM_HIP(hipMemcpyAsync(flag.get(), &big, 4, hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL((k_bdinv<BS>), grid256(nb), dim3(256), 0, s, nb, rp, ci, v, L->dinv.get(), flag.get());
+            int miss = big;
+            FEM_HIP(hipMemcpyAsync(&miss, flag.get(), 4, hipMemcpyDeviceToHost, s));
+            double rho = 0.0;
+            if (auto_s || robust) {
+                const unsigned G = grid256(nb).x;
+                DevBuf<double> part(G);
+                hipLaunchKernelGGL((k_bgersh<BS>), dim3(G), dim3(256), 0, s, nb, rp, v, L->dinv.get(), part.get());
+                std::vector<double> hp(G);
+                part.download(hp.data(), G, s);
+                FEM_HIP(hipStreamSynchronize(s));
+                for (double x : hp) rho = std::max(rho, x);
+            }
+            FEM_HIP(hipStreamSynchronize(s));
+            if (miss != big)
+                fail(FEM_ERR_RUNTIME, "Block AMG: block row " + std::to_string(miss) + " has no diagonal block");
+            const double wauto = rho > 0.0 ? (4.0 / 3.0) / rho : 2.0 / 3.0;
+            L->om = auto_s ? wauto : prm.omega;
+            const double pw = robust ? wauto : prm.p_omega;
+            expand(nb, nb, nnzb, rp, ci, v, L->A, s);
+            if (nb <= cmax) { lv.push_back(std::move(L)); break; }
+            // block norms, aggregation
+            DevBuf<double> nv(std::max<uint64_t>(nnzb, 1));
+            hipLaunchKernelGGL((k_bfro<BS>), grid256(nnzb), dim3(256), 0, s, nnzb, v, nv.get());
+            DevBuf<int> agg(nb);
+            uint32_t na = 0;
+            if (robust) {
+                na = mis2_aggregate_device(rp, ci, nv.get(), true, nb, nnzb, prm.theta, agg.get(), s);
+            } else {
+                std::vector<uint32_t> hrp(nb + 1), hci(nnzb);
+                std::vector<double> hnv(nnzb);
+                FEM_HIP(hipMemcpyAsync(hrp.data(), rp, (size_t)(nb + 1) * 4, hipMemcpyDeviceToHost, s));
+                FEM_HIP(hipMemcpyAsync(hci.data(), ci, nnzb * 4, hipMemcpyDeviceToHost, s));
+                nv.download(hnv.data(), nnzb, s);
+                FEM_HIP(hipStreamSynchronize(s));
+                std::vector<int> ha(nb, -1);
+                std::vector<uint32_t> strong;
+                for (uint32_t i = 0; i < nb; ++i) {
+                    if (ha[i] != -1) continue;
+                    double mx = 0.0;
+                    for (uint32_t p = hrp[i]; p < hrp[i + 1]; ++p)
+                        if (hci[p] != i) mx = std::max(mx, hnv[p]);
+                    ha[i] = (int)na;
+                    int size = 1;
+                    for (uint32_t p = hrp[i]; p < hrp[i + 1] && size < max_agg; ++p) {
+                        const uint32_t j = hci[p];
+                        if (j != i && hnv[p] >= prm.theta * mx && ha[j] == -1) { ha[j] = (int)na; ++size; }
+                    }
+                    ++na;
+                }
+                agg.upload(ha, s);
+            }
+            // smoothed block prolongator
+            DevBuf<uint32_t> prp(nb + 1), cnt(nb + 1);
+            FEM_HIP(hipMemsetAsync(flag.get(), 0, 4, s));
+            hipLaunchKernelGGL(k_bpcount, grid256((uint64_t)nb + 1), dim3(256), 0, s, nb, rp, ci, agg.get(), cnt.get(),
+                               flag.get());
+            size_t tmp = 0;
+            FEM_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt.get(), prp.get(), (int)nb + 1, s));
+            DevBuf<char> tbuf(std::max<size_t>(tmp, 1));
+            FEM_HIP(hipcub::DeviceScan::ExclusiveSum(tbuf.get(), tmp, cnt.get(), prp.get(), (int)nb + 1, s));
+            uint32_t pnnzb = 0;
+            FEM_HIP(hipMemcpyAsync(&pnnzb, prp.get() + nb, 4, hipMemcpyDeviceToHost, s));
+            FEM_HIP(hipStreamSynchronize(s));
+            DevBuf<uint32_t> pci(std::max<uint32_t>(pnnzb, 1));
+            DevBuf<double> pv(std::max<uint64_t>((uint64_t)pnnzb * BV, 1));
+            hipLaunchKernelGGL((k_bpfill<BS>), grid256(nb), dim3(256), 0, s, nb, rp, ci, v, agg.get(), L->dinv.get(), pw,
+                               prp.get(), pci.get(), pv.get(), flag.get());
+            int ovf = 0;
+            FEM_HIP(hipMemcpyAsync(&ovf, flag.get(), 4, hipMemcpyDeviceToHost, s));
+            FEM_HIP(hipStreamSynchronize(s));
+            if (ovf) fail(FEM_ERR_RUNTIME, "Block AMG: a prolongator block row has more than 64 aggregates");
+            if (robust && (double)na > 0.85 * (double)nb) { lv.push_back(std::move(L)); break; }
+            expand(nb, na, pnnzb, prp.get(), pci.get(), pv.get(), L->P, s);
+            transpose_device(L->P, L->R, s);
+            CsrMat AP, Ac;
+            spgemm_device(L->A, L->P, AP, s);
+            spgemm_device(L->R, AP, Ac, s);
+            // the coarse operator back in blocks
+            std::vector<uint32_t> hrp, hci;
+            std::vector<double> hv;
+            condense_device(Ac, BS, na, na, hrp, hci, hv, s);
+            nb = na;
+            nnzb = hci.size();
+            brp.upload(hrp, s);
+            bci.upload(hci.empty() ? std::vector<uint32_t>{0} : hci, s);
+            bv.upload(hv.empty() ? std::vector<double>{0.0} : hv, s);
+            FEM_HIP(hipStreamSynchronize(s));
+            rp = brp.get();
+            ci = bci.get();
+            v = bv.get();
+            lv.push_back(std::move(L));
+        }
+        // coarsest: explicit dense inverse of its scalar expansion
+        Level& C = *lv.back();
+        C.P = CsrMat();
+        C.R = CsrMat();
+        C.A.sync_host();
+        HCsr H;
+        H.n = C.A.n;
+        H.m = C.A.m;
+        H.rp = C.A.h_rp;
+        H.ci = C.A.h_ci;
+        H.v = C.A.h_v;
+        nco = H.n;
+        coarse.upload(sa_coarse_operator_host(H, prm.coarse_exact != 0), s);
+        for (size_t k = 0; k < lv.size(); ++k) {
+            Level& D = *lv[k];
+            const size_t n = (size_t)D.nb * BS;
+            D.u.alloc(n);
+            D.res.alloc(n);
+            if (k > 0) D.f.alloc(n);
+            if (k > 0) D.A.use_vector_rows();
+            if (k + 1 < lv.size()) {
+                D.P.use_vector_rows();
+                D.R.use_vector_rows();
+            }
+        }
+        FEM_HIP(hipStreamSynchronize(s));
+    }
+
+    int kind() const override { return FEM_PC_BSAAMG; }
+    uint32_t size() const override { return lv[0]->nb * BS; }
+    bool has_fp32() const override { return fp32; }
+    void ensure_fp32(hipStream_t s) override {
+        if (fp32) return;
+        for (size_t k = 0; k < lv.size(); ++k) {
+            Level& D = *lv[k];
+            const size_t n = (size_t)D.nb * BS;
+            D.A.ensure_fp32(s);
+            if (D.P.n) {
+                D.P.ensure_fp32(s);
+                D.R.ensure_fp32(s);
+            }
+            D.dinv32.alloc((size_t)D.nb * BV);
+            vec_convert<float, double>(D.dinv32.get(), D.dinv.get(), 1.0, (size_t)D.nb * BV, s);
+            D.u32.alloc(n);
+            D.res32.alloc(n);
+            if (k > 0) D.f32.alloc(n);
+        }
+        fp32 = true;
+        FEM_HIP(hipStreamSynchronize(s));
+    }
+    const CsrMat* amg_matrix(int l, int which) const override {
+        if (l < 0 || l >= (int)lv.size()) return nullptr;
+        const CsrMat& M = which == 0 ? lv[l]->A : which == 1 ? lv[l]->P : lv[l]->R;
+        return M.n ? &M : nullptr;
+    }
+
+    template <typename T>
+    void smooth(Level& D, const T* f, T* u, int sweeps, bool from_zero, hipStream_t s, Gate g) {
+        const uint32_t n = D.nb * BS;
+        const T* dinv;
+        T* res;
+        if constexpr (std::is_same<T, double>::value) { dinv = D.dinv.get(); res = D.res.get(); }
+        else { dinv = D.dinv32.get(); res = D.res32.get(); }
+        const T w = (T)D.om;
+        if (from_zero && sweeps == 0) hipLaunchKernelGGL((k_bzero<T>), grid256(n), dim3(256), 0, s, n, u, g);
+        for (int k = 0; k < sweeps; ++k) {
+            if (from_zero && k == 0) {
+                hipLaunchKernelGGL((k_bjac_sweep<T, BS, true>), grid256(n), dim3(256), 0, s, D.nb, dinv, f, u, w, g);
+                continue;
+            }
+            SpmvArgs<T> ar;
+            ar.f = f;
+            csr_spmv<T>(D.A, u, res, EPI_RESID, ar, s, g);
+            hipLaunchKernelGGL((k_bjac_sweep<T, BS, false>), grid256(n), dim3(256), 0, s, D.nb, dinv, res, u, w, g);
+        }
+    }
+
+    template <typename T>
+    void vcycle(const T* r, T* z, hipStream_t s, Gate g) {
+        const size_t L = lv.size();
+        std::vector<const T*> f(L, nullptr);
+        std::vector<T*> u(L, nullptr);
+        f[0] = r;
+        for (size_t k = 0; k < L; ++k) {
+            Level& D = *lv[k];
+            if constexpr (std::is_same<T, double>::value) {
+                u[k] = k == 0 ? z : D.u.get();
+                if (k > 0) f[k] = D.f.get();
+            } else {
+                u[k] = k == 0 ? z : D.u32.get();
+                if (k > 0) f[k] = D.f32.get();
+            }
+        }
+        for (size_t k = 0; k + 1 < L; ++k) {
+            Level& D = *lv[k];
+            T* res;
+            if constexpr (std::is_same<T, double>::value) res = D.res.get();
+            else res = D.res32.get();
+            smooth<T>(D, f[k], u[k], npre, true, s, g);
+            SpmvArgs<T> ar;
+            ar.f = f[k];
+            csr_spmv<T>(D.A, u[k], res, EPI_RESID, ar, s, g);
+            csr_spmv<T>(D.R, res, const_cast<T*>(f[k + 1]), EPI_STORE, {}, s, g);
+        }
+        hipLaunchKernelGGL((k_bcoarse<T>), grid256(nco), dim3(256), 0, s, nco, coarse.get(), f[L - 1], u[L - 1], g);
+        for (int64_t k = (int64_t)L - 2; k >= 0; --k) {
+            Level& D = *lv[k];
+            SpmvArgs<T> ar;
+            ar.u = u[k];
+            csr_spmv<T>(D.P, u[k + 1], u[k], EPI_ADD, ar, s, g);
+            smooth<T>(D, f[k], u[k], npost, false, s, g);
+        }
+        FEM_HIP(hipGetLastError());
+    }
+    void apply(const double* r, double* z, hipStream_t s, Gate g) override { vcycle<double>(r, z, s, g); }
+    void apply32(const float* r, float* z, hipStream_t s, Gate g) override {
+        if (!fp32) fail(FEM_ERR_RUNTIME, "float32 copy of the block AMG hierarchy was not built");
+        vcycle<float>(r, z, s, g);
+    }
+};
+
+// Block SA-AMG: BS = 1 takes the scalar path (condensation, blockMatrix.hpp:
+// 269-315, of a scalar CSR gives the same matrix), BS = 2..4 the block one.
 std::unique_ptr<Precond> make_block_saamg(const BsrMat& A, const fem_pc_params& p, hipStream_t s) {
-    if (A.bs != 1) fail(FEM_ERR_INVALID_ARGUMENT, "block SA-AMG on the device supports BS = 1 (scalar condensation)");
-    CsrMat C;
-    C.set_host(A.nbr, A.nbc, A.h_rp, A.h_ci, A.h_v, s);
-    return make_saamg(C, p, s);
+    switch (A.bs) {
+        case 1: {
+            CsrMat C;
+            C.set_host(A.nbr, A.nbc, A.h_rp, A.h_ci, A.h_v, s);
+            return make_saamg(C, p, s);
+        }
+        case 2: return std::unique_ptr<Precond>(new PcBlockAmg<2>(A, p, s));
+        case 3: return std::unique_ptr<Precond>(new PcBlockAmg<3>(A, p, s));
+        case 4: return std::unique_ptr<Precond>(new PcBlockAmg<4>(A, p, s));
+        default: fail(FEM_ERR_INVALID_ARGUMENT, "block size must be 1..4");
+    }
 }
 
 }  // namespace fem

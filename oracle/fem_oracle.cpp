@@ -1827,6 +1827,10 @@ int or_amg_levels(OrPc* h, uint32_t* rows, uint64_t* nnz, int cap) {
         for (int i = 0; i < (int)p->lv.size() && i < cap; ++i) { rows[i] = p->lv[i].A.n; nnz[i] = p->lv[i].A.nnz(); }
         return (int)p->lv.size();
     }
+    if (auto* p3 = dynamic_cast<PcBlockAMG<3>*>(h->pc.get())) {
+        for (int i = 0; i < (int)p3->lv.size() && i < cap; ++i) { rows[i] = p3->lv[i].n; nnz[i] = p3->lv[i].A.nnzb(); }
+        return (int)p3->lv.size();
+    }
     auto* p = static_cast<PcBlockAMG<1>*>(h->pc.get());
     for (int i = 0; i < (int)p->lv.size() && i < cap; ++i) { rows[i] = p->lv[i].n; nnz[i] = p->lv[i].A.nnzb(); }
     return (int)p->lv.size();
@@ -2030,6 +2034,22 @@ int or_bsr_pc_create(const OrBsr* h, int kind, OrPc** out) {
             else if (kind == 1) p->pc.reset(new PcBlockILU0<1>(*h->b1));
             else p->pc.reset(new PcBlockAMG<1>(*h->b1, 0.25, 2.0 / 3.0, 1, 1, 20, 4, 50, 2.0 / 3.0));
         }
+        *out = p.release();
+    });
+}
+// block SA-AMG with explicit parameters (as or_pc_create's SAAMG list:
+// theta, smoother omega, npre, npost, max_levels, max_agg, coarse_max,
+// prolongator omega, exact coarse solve)
+int or_bsr_pc_create_amg(const OrBsr* h, const double* q, OrPc** out) {
+    return guard([&] {
+        auto p = std::make_unique<OrPc>();
+        p->kind = 102;
+        if (h->bs == 3)
+            p->pc.reset(new PcBlockAMG<3>(*h->b3, q[0], q[1], (int)q[2], (int)q[3], (int)q[4], (int)q[5], (uint32_t)q[6],
+                                          q[7], q[8] != 0.0));
+        else
+            p->pc.reset(new PcBlockAMG<1>(*h->b1, q[0], q[1], (int)q[2], (int)q[3], (int)q[4], (int)q[5], (uint32_t)q[6],
+                                          q[7], q[8] != 0.0));
         *out = p.release();
     });
 }

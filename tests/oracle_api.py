@@ -49,6 +49,7 @@ def lib():
         L.or_amg_level_export.argtypes = [P, C.c_int, C.c_int, P, P, P, P]
         L.or_mis2_aggregate.argtypes = [P, C.c_double, P, P, C.POINTER(C.c_int)]
         L.or_amg_cf_split.argtypes = [P, C.c_int, P]
+        L.or_bsr_pc_create_amg.argtypes = [P, P, P]
         L.or_bilu0_export.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P, P, P, P]
         L.or_spai_features.argtypes = [C.c_uint32, P, P, P, P, P, P]
         L.or_mlp_forward.argtypes = [P, C.c_uint32, P, C.c_int, P]
@@ -167,6 +168,18 @@ class Pc:
         self.h = P()
         self.n = B.nrows
         _chk(lib().or_bsr_pc_create(B.h, kind, C.byref(self.h)))
+        return self
+
+    @classmethod
+    def block_amg(cls, B, params):
+        """Block SA-AMG with explicit parameters: theta, smoother omega, npre,
+        npost, max_levels, max_agg, coarse_max, prolongator omega, exact
+        coarse solve (weights <= 0 select the robust variant)."""
+        self = cls.__new__(cls)
+        self.h = P()
+        self.n = B.nrows
+        self._pp = np.ascontiguousarray(params, dtype=np.float64)
+        _chk(lib().or_bsr_pc_create_amg(B.h, _ptr(self._pp), C.byref(self.h)))
         return self
 
     def apply(self, r):
