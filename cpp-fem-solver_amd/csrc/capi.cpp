@@ -10,6 +10,7 @@
 #include <numeric>
 
 #include "amg.hpp"
+#include "precond.hpp"
 #include "handles.hpp"
 
 using namespace fem;
@@ -294,6 +295,18 @@ int fem_precond_create(fem_csr A, int kind, const fem_pc_params* params, fem_pc*
         auto h = std::make_unique<fem_pc_s>();
         h->stream = A->stream;
         h->pc = make_precond(A->A, kind, p, A->stream);
+        *out = h.release();
+    });
+}
+
+int fem_precond_create_schur(fem_csr B, fem_csr Bt, fem_pc A_pc, fem_pc S_pc, double s_scale, fem_pc* out) {
+    return guard([&] {
+        if (!B || !Bt || !A_pc || !out) fail(FEM_ERR_INVALID_ARGUMENT, "fem_precond_create_schur: null argument");
+        auto h = std::make_unique<fem_pc_s>();
+        h->stream = B->stream;
+        Timer tm;
+        h->pc = make_schur(B->A, Bt->A, A_pc->pc.get(), S_pc ? S_pc->pc.get() : nullptr, s_scale);
+        h->pc->setup_ms = tm.ms();
         *out = h.release();
     });
 }

@@ -108,6 +108,22 @@ int main() {
         for (size_t i = 0; i < bb.size(); ++i) { rr += (bb[i] - Ax[i]) * (bb[i] - Ax[i]); b2 += bb[i] * bb[i]; }
         report("T7", N, std::sqrt(rr / b2), 1e-6, iti <= itn);
         std::printf("     CG+blockILU=%d steps, CG=%d steps\n", iti + 1, itn + 1);
+        // 3x3-block SA-AMG (make_block_amg_preconditioner<double,3>) as the
+        // preconditioner of CG and of the function form of GMRES
+        auto amg3 = make_block_amg_preconditioner(B);
+        auto [xa, ita] = mini_cg(B, bb, amg3, 1e-8, 3000);
+        std::vector<double> Aa;
+        B.matvec(xa, Aa);
+        double ra = 0;
+        for (size_t i = 0; i < bb.size(); ++i) ra += (bb[i] - Aa[i]) * (bb[i] - Aa[i]);
+        std::printf("     CG+block AMG (3x3 blocks)=%d steps\n", ita + 1);
+        report("AMG3", N, std::sqrt(ra / b2), 1e-6, ita <= itn);
+        auto [xag, itag] = SMatrix::gmres(S, bb, amg3, 30);
+        B.matvec(xag, Aa);
+        ra = 0;
+        for (size_t i = 0; i < bb.size(); ++i) ra += (bb[i] - Aa[i]) * (bb[i] - Aa[i]);
+        std::printf("     gmres(function)+block AMG (3x3 blocks): %d steps\n", itag + 1);
+        report("AMG3-GMRES", N, std::sqrt(ra / b2), 1e-9);
         for (const char* pc : {"blockjacobi", "blockilu0", "none"}) {
             auto [xg, itg] = Blk::gmres(B, bb, pc, 1e-10, 2000, 30);
             std::vector<double> Ag;
@@ -224,6 +240,22 @@ int main() {
         std::printf("     3-D Stokes + Schur(ILU0, block AMG) GMRES(100): %d steps, ||x - x_exact|| = %.3e\n", its + 1,
                     std::sqrt(err));
         report("Stokes", N, std::sqrt(rr / bb), 1e-9);
+        // the same preconditioner composed on the device: GMRES runs with no
+        // host call per step; same steps and solution as the host composition
+        auto schur_dev = make_schur_preconditioner(B, Bt, Apc, Lamg, 6.1);
+        auto [xd, itd] = SMatrix::gmres(K, b, schur_dev, 100);
+        double dd = 0, xx = 0;
+        for (uint32_t i = 0; i < nt; ++i) { dd += (xd[i] - xs[i]) * (xd[i] - xs[i]); xx += xs[i] * xs[i]; }
+        std::printf("     3-D Stokes + device Schur(ILU0, block AMG) GMRES(100): %d steps, |x_dev - x_host|/|x_host| = %.3e\n",
+                    itd + 1, std::sqrt(dd / xx));
+        report("Stokes-device", N, std::sqrt(dd / xx), 1e-8, std::abs(itd - its) <= 2);
+        bool threw = false;
+        try {
+            make_schur_preconditioner(B, Bt, Spc, Spc, 1.0);
+        } catch (const std::invalid_argument&) {
+            threw = true;
+        }
+        report("Stokes-device-args", N, threw ? 0.0 : 1.0, 0.5);
     }
     std::printf("%s (%d failures)\n", failures ? "FAILED" : "PASSED", failures);
     return failures ? 1 : 0;

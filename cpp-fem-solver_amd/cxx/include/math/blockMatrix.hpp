@@ -142,13 +142,10 @@ std::function<void(const std::vector<T>&, std::vector<T>&)> block_pc(const Block
                                                                      const fem_pc_params* p = nullptr) {
     fem_pc_s* h = nullptr;
     femgpu::check(fem_bsr_precond_create(A.device(), kind, p, &h));
-    std::shared_ptr<fem_pc_s> pc(h, PcDel());
-    const uint32_t n = A.rows();
-    return [pc, n](const std::vector<T>& r, std::vector<T>& z) {
-        if (r.size() != n) throw std::invalid_argument("preconditioner: vector length differs from the matrix");
-        z.resize(n);
-        femgpu::check(fem_precond_apply(pc.get(), r.data(), z.data()));
-    };
+    femgpu::DevicePrecond d;
+    d.pc = std::shared_ptr<fem_pc_s>(h, PcDel());
+    d.n = A.rows();
+    return d;
 }
 }  // namespace detail
 
@@ -235,6 +232,32 @@ inline std::function<void(const std::vector<double>&, std::vector<double>&)> mak
         std::copy(w.z_u.begin(), w.z_u.end(), z.begin());
         std::copy(w.z_p.begin(), w.z_p.end(), z.begin() + nv);
     };
+}
+
+// The same preconditioner composed on the device (fem_precond_create_schur;
+// no counterpart in the reference): from the matrices B (n_pres x n_vel) and
+// B^T and preconditioners made by this library's factories (S_precond may
+// be empty: the identity), with S_precond's output multiplied by s_scale (the
+// reference test's S^{-1} ~ 6.1 L_p^{-1}). SMatrix::gmres(K, b, M, restart)
+// then runs the whole solve on the device.
+inline std::function<void(const std::vector<double>&, std::vector<double>&)> make_schur_preconditioner(
+    const sparse::SMatrix& B, const sparse::SMatrix& Bt,
+    const std::function<void(const std::vector<double>&, std::vector<double>&)>& A_precond,
+    const std::function<void(const std::vector<double>&, std::vector<double>&)>& S_precond, double s_scale = 1.0) {
+    const auto* a = A_precond.target<femgpu::DevicePrecond>();
+    const auto* sp = S_precond ? S_precond.target<femgpu::DevicePrecond>() : nullptr;
+    if (!a || (S_precond && !sp))
+        throw std::invalid_argument("device Schur preconditioner: A_precond and S_precond must come from this library");
+    fem_pc_s* h = nullptr;
+    femgpu::check(fem_precond_create_schur(B.device(), Bt.device(), a->pc.get(), sp ? sp->pc.get() : nullptr, s_scale,
+                                           &h));
+    femgpu::DevicePrecond d;
+    d.pc = std::shared_ptr<fem_pc_s>(h, detail::PcDel());
+    d.n = B.rows() + B.cols();
+    // keep what the device preconditioner points into alive with it
+    d.keep = {std::make_shared<sparse::SMatrix>(B), std::make_shared<sparse::SMatrix>(Bt), a->pc};
+    if (sp) d.keep.push_back(sp->pc);
+    return d;
 }
 
 }  // namespace block

@@ -56,11 +56,10 @@ PcPtr make_pc(const SMatrix& A, int kind, double omega = 1.2) {
 }
 
 std::function<void(const std::vector<double>&, std::vector<double>&)> wrap(PcPtr pc, uint32_t n) {
-    return [pc, n](const std::vector<double>& r, std::vector<double>& z) {
-        if (r.size() != n) throw std::invalid_argument("preconditioner: vector length differs from the matrix");
-        z.resize(n);
-        femgpu::check(fem_precond_apply(pc.get(), r.data(), z.data()));
-    };
+    femgpu::DevicePrecond d;
+    d.pc = std::move(pc);
+    d.n = n;
+    return d;
 }
 
 std::tuple<std::vector<double>, int> run(const SMatrix& A, const std::vector<double>& b, int solver,
@@ -218,8 +217,15 @@ std::tuple<std::vector<double>, int> SMatrix::gmres(
     std::vector<double> x(n, 0.0);
     const double normb = std::sqrt(SMatrix::dot(b, b));
     if (normb < 1e-300) return {x, 0};
-    CbCtx ctx{&solveM, {}, {}};
     int it = 0;
+    // a preconditioner made by this library runs on the device: the solve
+    // needs no host call per step (same algorithm, array.cpp:4402-4533)
+    if (const auto* dp = solveM.target<femgpu::DevicePrecond>()) {
+        if (dp->n != n) throw std::invalid_argument("GMRES: preconditioner size differs from the vector length");
+        femgpu::check(fem_gmres(A.device(), dp->pc.get(), b.data(), restart, tol, max_iter, x.data(), &it));
+        return {x, it};
+    }
+    CbCtx ctx{&solveM, {}, {}};
     femgpu::check(fem_gmres_callback(A.device(), trampoline, &ctx, b.data(), restart, tol, max_iter, x.data(), &it));
     return {x, it};
 }

@@ -15,7 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfemgpu.so")
 
 # enums (femgpu.h)
-PC_NONE, PC_DIAG, PC_IC0, PC_ILU0, PC_SSOR, PC_AMG, PC_SAAMG, PC_LEARNED, PC_BJACOBI, PC_BILU0, PC_BSAAMG = range(11)
+PC_NONE, PC_DIAG, PC_IC0, PC_ILU0, PC_SSOR, PC_AMG, PC_SAAMG, PC_LEARNED, PC_BJACOBI, PC_BILU0, PC_BSAAMG, PC_SCHUR = \
+    range(12)
 PCG, BICGSTAB, GMRES, MINICG = 0, 1, 2, 3
 FP64, FP32_IR = 0, 1
 SMOOTH_JACOBI, SMOOTH_SGS = 0, 1
@@ -120,6 +121,7 @@ _SIGS = {
     "fem_dot_device_noalloc": (C.c_int, [C.c_uint64, P_vp, P_vp, P_vp, P_vp]),
     "fem_precond_create": (C.c_int, [P_vp, C.c_int, C.POINTER(PcParams), C.POINTER(P_vp)]),
     "fem_precond_apply": (C.c_int, [P_vp, P_f64, P_f64]),
+    "fem_precond_create_schur": (C.c_int, [P_vp, P_vp, P_vp, P_vp, C.c_double, C.POINTER(P_vp)]),
     "fem_precond_destroy": (C.c_int, [P_vp]),
     "fem_precond_info": (C.c_int, [P_vp, C.POINTER(C.c_int), P_u32, C.POINTER(C.c_uint64), C.c_int, P_f64]),
     "fem_precond_bytes": (C.c_int, [P_vp, C.c_int, P_f64]),
@@ -424,6 +426,21 @@ class Preconditioner:
             self.close()
         except Exception:
             pass
+
+
+class SchurPreconditioner(Preconditioner):
+    """make_schur_preconditioner (fem_precond_create_schur) for the saddle-point
+    matrix [[A, B^T], [B, 0]]: B (n_pres x n_vel) and Bt as CsrMatrix, A_pc and
+    S_pc (None: identity) as Preconditioner; S_pc's output is scaled by
+    `scale`. Composed on the device."""
+
+    def __init__(self, B, Bt, A_pc, S_pc=None, scale=1.0):
+        self.kind = PC_SCHUR
+        self._h = C.c_void_p()
+        self.n = B.rows + B.cols
+        self._A = (B, Bt, A_pc, S_pc)
+        _check(lib().fem_precond_create_schur(B._h, Bt._h, A_pc._h, S_pc._h if S_pc is not None else None, scale,
+                                              C.byref(self._h)))
 
 
 class LearnedPreconditioner(Preconditioner):

@@ -217,6 +217,55 @@ def block_test_scalar(N: int, BS: int = 3) -> CSR:
     return _assemble(n, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals))
 
 
+def stokes3d(N: int, alpha: float = 0.1) -> dict:
+    """Periodic 3-D MAC Stokes test system of test_BlockMatrix_schur3d.cpp:39-129
+    on an N^3 grid (point (i, j, k) -> i + j N + k N^2): velocity block A =
+    three periodic 7-point Laplacians plus alpha I (diagonal 6 + alpha),
+    divergence B (pressure p, velocity component c: +1 at p, -1 at the point
+    before it along c), B^T, the pressure Laplacian Lp (diagonal 6 + 1e-8)
+    and K = [[A, B^T], [B, 0]]. Repeated (row, column) pairs are summed and
+    zeros dropped, as the reference's COO constructor does. Returns CSR
+    matrices A, B, Bt, Lp, K and the sizes nv, np."""
+    import scipy.sparse as sp
+    N3 = N ** 3
+    nv, npres = 3 * N3, N3
+    i, j, k = np.meshgrid(np.arange(N), np.arange(N), np.arange(N), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    p = i + j * N + k * N * N
+    nbrs = [((i + 1) % N, j, k), ((i - 1) % N, j, k), (i, (j + 1) % N, k), (i, (j - 1) % N, k),
+            (i, j, (k + 1) % N), (i, j, (k - 1) % N)]
+    nidx = [a + b * N + c * N * N for a, b, c in nbrs]
+
+    def lap(n_off, diag):
+        rows = [p + n_off] + [p + n_off for _ in nidx]
+        cols = [p + n_off] + [q + n_off for q in nidx]
+        vals = [np.full(N3, diag)] + [np.full(N3, -1.0) for _ in nidx]
+        return np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+
+    rA, cA, vA = zip(*[lap(c * N3, 6.0 + alpha) for c in range(3)])
+    rA, cA, vA = np.concatenate(rA), np.concatenate(cA), np.concatenate(vA)
+    rL, cL, vL = lap(0, 6.0 + 1e-8)
+    prev = [((i - 1) % N) + j * N + k * N * N, i + ((j - 1) % N) * N + k * N * N, i + j * N + ((k - 1) % N) * N * N]
+    rB = np.concatenate([np.concatenate([p, p]) for _ in range(3)])
+    cB = np.concatenate([np.concatenate([c * N3 + p, c * N3 + prev[c]]) for c in range(3)])
+    vB = np.concatenate([np.concatenate([np.ones(N3), -np.ones(N3)]) for _ in range(3)])
+
+    def csr(r, c, v, shape):
+        M = sp.coo_matrix((v, (r, c)), shape=shape).tocsr()
+        M.sum_duplicates()
+        M.eliminate_zeros()
+        M.sort_indices()
+        return CSR(shape[0], shape[1], M.indptr.astype(np.uint32), M.indices.astype(np.uint32), M.data.copy())
+
+    A = csr(rA, cA, vA, (nv, nv))
+    B = csr(rB, cB, vB, (npres, nv))
+    Bt = csr(cB, rB, vB, (nv, npres))
+    Lp = csr(rL, cL, vL, (npres, npres))
+    K = csr(np.concatenate([rA, rB + nv, cB]), np.concatenate([cA, cB, rB + nv]), np.concatenate([vA, vB, vB]),
+            (nv + npres, nv + npres))
+    return dict(A=A, B=B, Bt=Bt, Lp=Lp, K=K, nv=nv, np=npres)
+
+
 @dataclass
 class BSR:
     bs: int

@@ -67,7 +67,10 @@ enum fem_pc_kind {
     FEM_PC_BJACOBI = 8, /* block-Jacobi, blockMatrix.hpp:714-735 and
                            test_BlockMatrix_mpi.cpp:132-155                   */
     FEM_PC_BILU0 = 9,   /* block ILU(0), blockMatrix.hpp:454-623             */
-    FEM_PC_BSAAMG = 10  /* block SA-AMG on a BSR matrix, blockMatrix.hpp:1051 */
+    FEM_PC_BSAAMG = 10, /* block SA-AMG on a BSR matrix, blockMatrix.hpp:1051 */
+    FEM_PC_SCHUR = 11   /* Schur-complement preconditioner of a saddle-point
+                           matrix, blockMatrix.hpp:625-685 (see
+                           fem_precond_create_schur)                         */
 };
 
 enum fem_solver_kind {
@@ -204,6 +207,16 @@ int fem_dot_device_noalloc(uint64_t n, const double* a, const double* b, double*
 
 /* ---- preconditioners (SMatrix::make_*_preconditioner, array.hpp:309-313) -- */
 int fem_precond_create(fem_csr A, int kind, const fem_pc_params* params, fem_pc* out);
+/* make_schur_preconditioner(sys, A_precond, S_precond), blockMatrix.hpp:
+ * 625-685, for the saddle-point matrix [[A, B^T], [B, 0]]: B is n_pres x
+ * n_vel, Bt its transpose (n_vel x n_pres), A_pc a preconditioner of size
+ * n_vel, S_pc one of size n_pres or NULL (identity); S_pc's output is
+ * multiplied by s_scale. z = M^{-1} r is
+ *   w_u = A_pc(r_u), z_p = s_scale S_pc(B w_u - r_p), z_u = A_pc(r_u - B^T z_p),
+ * composed on the device (no host round trip; usable inside captured
+ * solves). The result keeps pointers to B, Bt, A_pc and S_pc: destroy it
+ * first. */
+int fem_precond_create_schur(fem_csr B, fem_csr Bt, fem_pc A_pc, fem_pc S_pc, double s_scale, fem_pc* out);
 /* z = M^{-1} r: the std::function the reference factories return */
 int fem_precond_apply(fem_pc M, const double* r, double* z);
 int fem_precond_destroy(fem_pc M);

@@ -1411,6 +1411,49 @@ struct PcBlockAMG : Pc {
     }
 };
 
+// make_schur_preconditioner (block.hpp:625-685): block upper-triangular
+// preconditioner of the saddle-point matrix [[A, B^T], [B, 0]] from a
+// velocity preconditioner A_pc and a Schur-complement preconditioner S_pc:
+//   w_u = A_pc(r_u); z_p = S_pc(B w_u - r_p); z_u = A_pc(r_u - B^T z_p).
+// S_pc may be absent (the identity, the reference test's baseline); its
+// output is multiplied by s_scale (the reference test's scaled AMG,
+// test_BlockMatrix_schur3d.cpp:149-160).
+struct PcSchur : Pc {
+    const Csr* B;
+    const Csr* Bt;
+    Pc* A;
+    Pc* S;
+    double scale;
+    PcSchur(const Csr* B_, const Csr* Bt_, Pc* A_, Pc* S_, double sc) : B(B_), Bt(Bt_), A(A_), S(S_), scale(sc) {
+        if (Bt->n != B->m || Bt->m != B->n) throw std::invalid_argument("Schur: B^T must be the transpose shape of B");
+    }
+    void apply(const vec& r, vec& z) override {
+        const size_t nv = B->m, np = B->n;
+        vec ru(r.begin(), r.begin() + nv), rp(r.begin() + nv, r.begin() + nv + np);
+        vec wu, wp, zp, wbt, zu;
+        A->apply(ru, wu);
+        spmv(*B, wu, wp);
+        vec rmod(np);
+        for (size_t i = 0; i < np; ++i) rmod[i] = wp[i] - rp[i];
+        if (S) {
+            vec w;
+            S->apply(rmod, w);
+            zp.resize(np);
+            for (size_t i = 0; i < np; ++i) zp[i] = scale * w[i];
+        } else {
+            zp.resize(np);
+            for (size_t i = 0; i < np; ++i) zp[i] = scale * rmod[i];
+        }
+        spmv(*Bt, zp, wbt);
+        vec umod(nv);
+        for (size_t i = 0; i < nv; ++i) umod[i] = ru[i] - wbt[i];
+        A->apply(umod, zu);
+        z.resize(nv + np);
+        std::copy(zu.begin(), zu.end(), z.begin());
+        std::copy(zp.begin(), zp.end(), z.begin() + nv);
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Krylov solvers
 // ---------------------------------------------------------------------------
@@ -2034,6 +2077,16 @@ int or_bsr_pc_create(const OrBsr* h, int kind, OrPc** out) {
             else if (kind == 1) p->pc.reset(new PcBlockILU0<1>(*h->b1));
             else p->pc.reset(new PcBlockAMG<1>(*h->b1, 0.25, 2.0 / 3.0, 1, 1, 20, 4, 50, 2.0 / 3.0));
         }
+        *out = p.release();
+    });
+}
+// Schur-complement preconditioner over oracle matrices and preconditioners
+// (S may be null); it keeps pointers to them
+int or_pc_create_schur(const Csr* B, const Csr* Bt, OrPc* A, OrPc* S, double scale, OrPc** out) {
+    return guard([&] {
+        auto p = std::make_unique<OrPc>();
+        p->kind = 200;
+        p->pc.reset(new PcSchur(B, Bt, A->pc.get(), S ? S->pc.get() : nullptr, scale));
         *out = p.release();
     });
 }

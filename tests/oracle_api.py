@@ -50,6 +50,7 @@ def lib():
         L.or_mis2_aggregate.argtypes = [P, C.c_double, P, P, C.POINTER(C.c_int)]
         L.or_amg_cf_split.argtypes = [P, C.c_int, P]
         L.or_bsr_pc_create_amg.argtypes = [P, P, P]
+        L.or_pc_create_schur.argtypes = [P, P, P, P, C.c_double, P]
         L.or_bilu0_export.argtypes = [P, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), P, P, P, P, P, P]
         L.or_spai_features.argtypes = [C.c_uint32, P, P, P, P, P, P]
         L.or_mlp_forward.argtypes = [P, C.c_uint32, P, C.c_int, P]
@@ -168,6 +169,18 @@ class Pc:
         self.h = P()
         self.n = B.nrows
         _chk(lib().or_bsr_pc_create(B.h, kind, C.byref(self.h)))
+        return self
+
+    @classmethod
+    def schur(cls, B, Bt, A_pc, S_pc=None, scale=1.0):
+        """make_schur_preconditioner (blockMatrix.hpp:625-685) over oracle
+        CSR matrices B, B^T and preconditioners (S_pc None: identity)."""
+        self = cls.__new__(cls)
+        self.h = P()
+        self.n = B.m + B.n
+        self._keep = (B, Bt, A_pc, S_pc)
+        _chk(lib().or_pc_create_schur(B.h, Bt.h, A_pc.h, S_pc.h if S_pc is not None else None, scale,
+                                      C.byref(self.h)))
         return self
 
     @classmethod

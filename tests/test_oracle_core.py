@@ -720,3 +720,59 @@ def test_saamg_sgs_vcycle_matches_dense_restatement(case):
     # the smoother option takes effect (the Jacobi V-cycle differs)
     pj = O.Pc(A, O.K_SAAMG, [0.25, 1.0, 2, 2, 20, 4, 50, 0.0, 1, 0])
     assert np.linalg.norm(pj.apply(r) - z) > 1e-6 * np.linalg.norm(z)
+
+
+# ---------------------------------------------------------------- Schur complement
+def test_schur_preconditioner_block_triangular_identity():
+    """make_schur_preconditioner (blockMatrix.hpp:625-685) with an exact
+    velocity solve (ILU(0) of a tridiagonal A has no fill) and S_pc = D^-1
+    (Jacobi of a diagonal matrix), scaled by c: z = M^-1 r for the dense
+    M = [[A, B^T], [B, B A^-1 B^T - D / c]], which follows from
+    z_p = c S_pc(B A_pc(r_u) - r_p), z_u = A_pc(r_u - B^T z_p). A flipped
+    sign, a missing scale or B in place of B^T breaks it."""
+    nv, npres, c = 12, 5, 1.7
+    A = sp.diags([-np.ones(nv - 1), 2.5 * np.ones(nv), -np.ones(nv - 1)], [-1, 0, 1]).tocsr()
+    rng = np.random.default_rng(3)
+    Bd = rng.uniform(-1, 1, (npres, nv)) * (rng.random((npres, nv)) < 0.4)
+    B, Bt = sp.csr_matrix(Bd), sp.csr_matrix(Bd.T)
+    d = rng.uniform(1.0, 3.0, npres)
+    D = sp.diags(d).tocsr()
+    oc = lambda M: O.Csr(M.shape[0], M.shape[1], M.indptr, M.indices, M.data)
+    Ao, Bo, Bto, Do = oc(A), oc(B), oc(Bt), oc(D)
+    pa, pd = O.Pc(Ao, O.K_ILU0), O.Pc(Do, O.K_DIAG)
+    S = O.Pc.schur(Bo, Bto, pa, pd, c)
+    Ad = A.toarray()
+    M = np.block([[Ad, Bd.T], [Bd, Bd @ np.linalg.inv(Ad) @ Bd.T - np.diag(d) / c]])
+    for seed in range(3):
+        r = np.random.default_rng(10 + seed).standard_normal(nv + npres)
+        z = S.apply(r)
+        assert np.linalg.norm(M @ z - r) < 1e-12 * np.linalg.norm(r) * np.linalg.cond(M)
+    # S_pc absent: the identity, z_p = c (B A^-1 r_u - r_p)
+    S0 = O.Pc.schur(Bo, Bto, pa, None, c)
+    M0 = np.block([[Ad, Bd.T], [Bd, Bd @ np.linalg.inv(Ad) @ Bd.T - np.eye(npres) / c]])
+    r = rng.standard_normal(nv + npres)
+    assert np.linalg.norm(M0 @ S0.apply(r) - r) < 1e-12 * np.linalg.norm(r) * np.linalg.cond(M0)
+
+
+def test_schur_gmres_on_stokes3d():
+    """The reference test (test_BlockMatrix_schur3d.cpp:134-193): GMRES(100)
+    with the Schur preconditioner (ILU(0) velocity block, scaled SA-AMG of the
+    pressure Laplacian) solves the 3-D Stokes system. On the periodic grid
+    K has the null vector (0, constant pressure), so the solution matches the
+    drand one up to a pressure constant: velocity and mean-free pressure
+    errors stay below the test's 1e-6."""
+    S3 = gen.stokes3d(6)
+    K, A, B, Bt, Lp = (O.Csr.from_gen(S3[k]) for k in ("K", "A", "B", "Bt", "Lp"))
+    pa = O.Pc(A, O.K_ILU0)
+    pl = O.Pc(Lp, O.K_SAAMG, [0.25, 0.0, 1, 1, 20, 4, 50, 0.0, 1])
+    S = O.Pc.schur(B, Bt, pa, pl, 6.1)
+    xe = gen.drand(np.arange(K.n))
+    b = S3["K"].matvec(xe)
+    nv = S3["nv"]
+    for Sp in (S, O.Pc.schur(B, Bt, pa, None, 1.0)):
+        x, it = O.solve(K, 2, Sp, b, tol=1e-10, max_iter=2000, restart=100)
+        d = x - xe
+        assert np.linalg.norm(b - S3["K"].matvec(x)) < 1e-10 * np.linalg.norm(b)
+        assert np.linalg.norm(d[:nv]) < 1e-6
+        assert np.linalg.norm(d[nv:] - d[nv:].mean()) < 1e-6
+
