@@ -57,6 +57,39 @@ def test_batch_matches_oracle_pcg():
         assert rel(x[0], xo) < tol_x(G)
 
 
+def test_batch_c5_size_extremes_match_oracle():
+    """Systems at the ends of the batched workload's size ranges (2-D Poisson
+    sides 32 and 256, 3x3-block sides 16 and 48: up to 331,776 unknowns),
+    solved together in one launch, each right-hand side against the float64
+    residual bar, the first one against the oracle's PCG solve."""
+    sysl = []
+    for k, side in enumerate((32, 256)):
+        A = gen.poisson2d(side)
+        X, Bm = gen.rhs(A, 1000 + k, 4)
+        sysl.append(dict(kind="poisson2d", A=A, X=X, B=Bm))
+    for k, t in enumerate((16, 48)):
+        Ab = gen.bsr7(t, 2000 + k)
+        X = np.random.default_rng(3000 + k).uniform(-1.0, 1.0, (4, Ab.nbr * 3))
+        sysl.append(dict(kind="bsr3", A=Ab, X=X, B=np.ascontiguousarray((Ab.to_scipy() @ X.T).T)))
+    B = F.Batch([s["A"] for s in sysl], nrhs=4)
+    B.set_rhs([s["B"] for s in sysl])
+    stats, _ = B.solve(tol=1e-10)
+    X = B.solution()
+    for s, st, x in zip(sysl, stats, X):
+        print(f"{s['kind']} n={x[0].size}: inner iterations {st['iterations']}, refinements {st['refinements']}, "
+              f"relres {st['max_relres']:.2e}", flush=True)
+        assert st["converged"] == 4 and st["max_relres"] < 1e-10
+        G = _scalar(s["A"])
+        Ao = O.Csr.from_gen(G)
+        if s["kind"] == "poisson2d":
+            xo, _ = O.solve(Ao, 0, O.Pc(Ao, O.K_DIAG), s["B"][0], tol=1e-10, max_iter=20000)
+        else:
+            Ab = s["A"]
+            Bo = O.Bsr(3, Ab.nbr, Ab.nbc, Ab.row_ptr, Ab.col_idx, Ab.values)
+            xo, _ = Bo.solve(0, O.Pc.block(Bo, 0), s["B"][0], tol=1e-10, max_iter=20000)
+        assert rel(x[0], xo) < tol_x(G)
+
+
 def test_batch_deterministic_and_schedules():
     sysl = small_batch(seed=7)
     B = F.Batch([s["A"] for s in sysl], nrhs=4)
