@@ -70,14 +70,16 @@ def measured_traffic():
 
 
 def c4_leg(F, gen, args, log):
-    """Learned SPAI on the variable-coefficient 96^3 diffusion system
-    (kappa = exp(g), g a Gaussian random field, sigma 1, correlation length 8
-    cells). The network is trained on the GPU on `c4_train` seeded fields of
-    32^3 cells (same field statistics; the network only sees 7-point
-    stencils) and tested on `c4_test` held-out 96^3 seeds: learned SPAI PCG
-    in float32-inner/float64-refinement form against plain float64 IC(0)-PCG
-    (every 4th seed), float64 Jacobi-PCG and SA-AMG PCG. Solve times are
-    single timed solves after one warm-up, b and x in HBM."""
+    """Learned factorized SPAI (M = G^T G) on the variable-coefficient 96^3
+    diffusion system (kappa = exp(g), g a Gaussian random field, sigma 1,
+    correlation length 8 cells). The network is trained on the GPU on
+    `c4_train` seeded fields of 32^3 cells (same field statistics; the network
+    only sees 7-point stencils) and tested on `c4_test` held-out 96^3 seeds:
+    learned SPAI PCG in float32-inner/float64-refinement form against plain
+    float64 Jacobi-PCG (the judge of the learned solve's iteration count) and,
+    every 4th seed, float64 IC(0)-PCG, SA-AMG PCG and the non-learned
+    approximate inverses of the same patterns (Neumann, exact FSAI). Solve
+    times are single timed solves after one warm-up, b and x in HBM."""
     from femgpu import learned as Lr
     t0 = time.time()
     fields = Lr.make_fields(32, range(args.c4_train))
@@ -85,7 +87,7 @@ def c4_leg(F, gen, args, log):
     net = F.Mlp(seed=0)
     t0 = time.time()
     for f in fields:
-        f.errors = Lr.error_vectors(f) if args.c4_errors == "ic0" else np.empty((0, f.G.n))
+        f.errors = Lr.error_vectors(f) if args.c4_errors == "solvers" else np.empty((0, f.G.n))
     err_s = time.time() - t0
     steps = 3 * len(fields)
     hist, train_s = Lr.train(net, fields, steps=steps)
@@ -94,7 +96,8 @@ def c4_leg(F, gen, args, log):
         f"(field generation {gen_s:.1f} s, error vectors {err_s:.1f} s), loss {hist[:10].mean():.3f} -> {hist[-50:].mean():.3f}")
     N = 96
     sa = dict(smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0, coarse_exact=1, fp32=1)
-    rows = {k: [] for k in ("learned_mp", "learned_fp64", "jacobi_fp64", "ic0_fp64", "saamg_mp")}
+    rows = {k: [] for k in ("learned_mp", "learned_fp64", "jacobi_fp64", "ic0_fp64", "saamg_mp", "neumann_mp",
+                            "fsai_mp")}
     want = list(rows) if args.c4_methods == "all" else args.c4_methods.split(",")
     worst = {"relres": 0.0, "rel_err": 0.0}
     fallbacks = 0
@@ -128,21 +131,40 @@ def c4_leg(F, gen, args, log):
             S.close()
             return M, st
 
+        # the plain preconditioned solve (Jacobi) judges the learned one: a
+        # learned solve needing more iterations than it is a regression and
+        # falls back to IC(0)
+        judge = None
+        if "jacobi_fp64" in want:
+            _, stj = run("jacobi_fp64", lambda: F.Preconditioner(A, "diag"), F.FP64)
+            judge = stj["inner_iterations"]
+
+        def learned():
+            M = F.LearnedPreconditioner(A, (N, N, N), net, fallback="ic0", budget=4000)
+            if judge:
+                M.reference(judge)
+            return M
+
         Ml = None
         if "learned_mp" in want:
-            Ml, st = run("learned_mp", lambda: F.LearnedPreconditioner(A, (N, N, N), net, fallback="ic0",
-                                                                        budget=2000), F.FP32_IR)
+            Ml, st = run("learned_mp", learned, F.FP32_IR)
             fallbacks += st["fallback"]
         if "learned_fp64" in want:
-            run("learned_fp64", (lambda: Ml) if Ml is not None else
-                (lambda: F.LearnedPreconditioner(A, (N, N, N), net, fallback="ic0", budget=2000)), F.FP64)
-        if "jacobi_fp64" in want:
-            run("jacobi_fp64", lambda: F.Preconditioner(A, "diag"), F.FP64)
+            run("learned_fp64", (lambda: Ml) if Ml is not None else learned, F.FP64)
         if t % 4 == 0:
             if "ic0_fp64" in want:
                 run("ic0_fp64", lambda: F.Preconditioner(A, "ic0"), F.FP64)
             if "saamg_mp" in want:
                 run("saamg_mp", lambda: F.Preconditioner(A, "saamg", **sa), F.FP32_IR)
+            # non-learned approximate inverses: the Neumann form of A's own
+            # pattern and cost, and the exact FSAI of the learned G's pattern
+            # and cost (host-built matrices, device applies)
+            if "neumann_mp" in want:
+                Mn = F.CsrMatrix.from_gen(Lr.neumann_spai(G))
+                run("neumann_mp", lambda: F.SpaiPreconditioner(A, Mn), F.FP32_IR)
+            if "fsai_mp" in want:
+                Gf = F.CsrMatrix.from_gen(Lr.fsai_exact(G))
+                run("fsai_mp", lambda: F.SpaiPreconditioner(A, Gf, factorized=True), F.FP32_IR)
         if t == 0 and not args.no_cpu:
             # the oracle's float64 IC(0)-PCG solve of the first held-out seed
             import oracle_api as O
@@ -163,6 +185,8 @@ def c4_leg(F, gen, args, log):
     ic0_seeds = [r[0] for r in rows["ic0_fp64"]]
     lm = {r[0]: r[1] for r in rows["learned_mp"]}
     speed = float(np.median([r[1] / lm[r[0]] for r in rows["ic0_fp64"]])) if ic0_seeds and lm else None
+    jm = {r[0]: r[1] for r in rows["jacobi_fp64"]}
+    vs_jac = float(np.median([lm[k] / jm[k] for k in lm if k in jm])) if lm and jm else None
     for k, (ms, it, su) in med.items():
         log(f"[bench] c4 {k}: median {ms:.2f} ms, {it:.0f} iterations, setup {su:.1f} ms")
     if speed:
@@ -176,6 +200,7 @@ def c4_leg(F, gen, args, log):
             "median_setup_ms": {k: round(v[2], 2) for k, v in med.items()},
             "rel_err_vs_oracle": oracle_err,
             "speedup_learned_mp_vs_ic0_fp64": round(speed, 2) if speed else None,
+            "learned_mp_time_over_jacobi_fp64": round(vs_jac, 3) if vs_jac else None,
             "fallbacks": int(fallbacks), "worst_true_relres": worst["relres"], "worst_rel_err": worst["rel_err"]}
 
 
@@ -431,10 +456,11 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the learned-preconditioner leg")
     ap.add_argument("--c4-train", type=int, default=256, help="training fields (32^3 cells each)")
     ap.add_argument("--c4-test", type=int, default=32, help="held-out 96^3 fields")
-    ap.add_argument("--c4-errors", choices=("ic0", "none"), default="ic0",
-                    help="training probes besides random signs: IC(0)-PCG error vectors, or none")
+    ap.add_argument("--c4-errors", choices=("solvers", "none"), default="solvers",
+                    help="training probes besides random signs: error vectors of IC(0)/ILU(0)/SA-AMG PCG, or none")
     ap.add_argument("--c4-methods", default="all",
-                    help="comma list of 96^3 comparisons: learned_mp,learned_fp64,jacobi_fp64,ic0_fp64,saamg_mp")
+                    help="comma list of 96^3 comparisons: learned_mp,learned_fp64,jacobi_fp64,ic0_fp64,saamg_mp,"
+                         "neumann_mp,fsai_mp")
     ap.add_argument("--no-c5", action="store_true", help="skip the batched-solve leg")
     ap.add_argument("--no-c3", action="store_true", help="skip the convection-diffusion ILU(0) leg (about 20 s)")
     ap.add_argument("--c3-n", type=int, default=1024, help="grid side of the convection-diffusion leg")

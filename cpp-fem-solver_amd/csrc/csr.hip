@@ -85,9 +85,9 @@ struct EpiIn {
     }
 };
 
-// DOT: the block also writes part[bid] = sum over its rows of x[i] * y[i]
-// (float64; square matrices)
-template <typename T, int EPI, bool PK, bool DOT = false>
+// DOT = 1: the block also writes part[bid] = sum over its rows of
+// x[i] * y[i] (float64; square matrices); DOT = 2: the sum of y[i]^2
+template <typename T, int EPI, bool PK, int DOT = 0>
 __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
                                                  const uint16_t* __restrict__ cd,
                                                  const uint32_t* __restrict__ cbase, const T* __restrict__ v,
@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
                 rb[j] = rp[row] - k0;
                 re[j] = rp[row + 1] - k0;
                 ep[j].load(row, a);
-                if (DOT) xr[j] = x[row];
+                if (DOT == 1) xr[j] = x[row];
             }
         }
         if (cnt > 0) stage_products<T, STAGE, PK>(ci, cd, cbase, v, x, k0, cnt, prod);
@@ -133,10 +133,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
                 for (uint32_t k = rb[j]; k < re[j]; ++k) s += prod[k];
                 const T yv = ep[j].apply(s, a);
                 y[row] = yv;
-                if (DOT) dsum[0] += (double)xr[j] * (double)yv;
+                if (DOT == 1) dsum[0] += (double)xr[j] * (double)yv;
+                if (DOT == 2) dsum[0] += (double)yv * (double)yv;
             }
         }
-        if constexpr (DOT) {
+        if constexpr (DOT != 0) {
             block_sum<1>(dsum);
             if (t == 0) part[bid] = dsum[0];
         }
@@ -148,7 +149,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
         if (t == 0) {
             const T yv = spmv_epilogue<T, EPI>(r0, (T)tot[0], a);
             y[r0] = yv;
-            if (DOT) part[bid] = (double)x[r0] * (double)yv;
+            if (DOT == 1) part[bid] = (double)x[r0] * (double)yv;
+            if (DOT == 2) part[bid] = (double)yv * (double)yv;
         }
     }
 }
@@ -249,6 +251,27 @@ unsigned csr_spmv_dot(const CsrMat& A, const T* x, T* y, double* part, hipStream
 }
 template unsigned csr_spmv_dot<double>(const CsrMat&, const double*, double*, double*, hipStream_t, Gate);
 template unsigned csr_spmv_dot<float>(const CsrMat&, const float*, float*, double*, hipStream_t, Gate);
+
+template <typename T>
+unsigned csr_spmv_selfdot(const CsrMat& A, const T* x, T* y, double* part, hipStream_t s, Gate g) {
+    if (A.lanes > 0 || A.n == 0) return 0;
+    const T* vals;
+    if constexpr (std::is_same<T, double>::value) vals = A.v.get();
+    else vals = A.v32.get();
+    if (!vals) fail(FEM_ERR_RUNTIME, "float32 copy of the matrix was not built");
+    const SpmvPlan& plan = std::is_same<T, double>::value ? A.plan : A.plan32;
+    const dim3 grid(plan.nblk), block(kBlock);
+    if (A.cd.get())
+        hipLaunchKernelGGL((k_spmv<T, EPI_STORE, true, 2>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
+                           A.cbase.get(), vals, x, y, plan.blk_row.get(), SpmvArgs<T>{}, g, part);
+    else
+        hipLaunchKernelGGL((k_spmv<T, EPI_STORE, false, 2>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
+                           A.cbase.get(), vals, x, y, plan.blk_row.get(), SpmvArgs<T>{}, g, part);
+    FEM_HIP(hipGetLastError());
+    return plan.nblk;
+}
+template unsigned csr_spmv_selfdot<double>(const CsrMat&, const double*, double*, double*, hipStream_t, Gate);
+template unsigned csr_spmv_selfdot<float>(const CsrMat&, const float*, float*, double*, hipStream_t, Gate);
 
 unsigned CsrMat::apply_dot(const double* x, double* y, double* part, hipStream_t s, Gate g) const {
     return csr_spmv_dot<double>(*this, x, y, part, s, g);

@@ -219,6 +219,10 @@ void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, 
 // CSR-stream kernel); returns their count, 0 when A takes another kernel
 template <typename T>
 unsigned csr_spmv_dot(const CsrMat& A, const T* x, T* y, double* part, hipStream_t s, Gate g);
+// y = A x with the per-workgroup partial sums of y[i]^2 (the CSR-stream
+// kernel); returns their count, 0 when A takes another kernel
+template <typename T>
+unsigned csr_spmv_selfdot(const CsrMat& A, const T* x, T* y, double* part, hipStream_t s, Gate g);
 
 // ---------------------------------------------------------------------------
 // Preconditioners
@@ -241,6 +245,9 @@ struct Precond {
     // (built on first use).
     virtual int budget() const { return 0; }
     virtual Precond* fallback(hipStream_t) { return nullptr; }
+    // the fallback needed this many Krylov iterations (recorded as the count
+    // a later learned solve must not exceed)
+    virtual void fallback_ran(int /*iterations*/) {}
     // z = M^{-1} r with the per-workgroup partial sums of r[i] * z[i], as
     // LinOp::apply_dot (0: no fused form)
     virtual unsigned apply_dot(const double*, double*, double*, hipStream_t, Gate = {}) { return 0; }

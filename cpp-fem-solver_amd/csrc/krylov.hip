@@ -1083,7 +1083,10 @@ double KrylovSolver::true_residual(const double* b, const double* x, double* wor
 // A preconditioner with an iteration budget (the learned SPAI) gets one
 // attempt of at most that many iterations; if it does not converge (or breaks
 // down) the system is solved again from x0 = 0 with its fallback
-// preconditioner, and stats->fallback is set.
+// preconditioner, and stats->fallback is set. The fallback's iteration count
+// is handed back to the preconditioner, which can make it the budget of later
+// solves (a learned solve that needs more iterations than the fallback did is
+// a regression).
 void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
     const int budget = M_ ? M_->budget() : 0;
     if (budget <= 0) {
@@ -1107,6 +1110,7 @@ void KrylovSolver::solve(const double* b, double* x, fem_solve_stats* stats) {
         const long long first = S.spmv_count;
         if (!fb_) fb_.reset(new KrylovSolver(A_, M_->fallback(s_), opts, s_));
         fb_->solve(b, x, &S);
+        if (S.converged) M_->fallback_ran(S.inner_iterations);
         S.fallback = 1;
         S.spmv_count += first;
     }

@@ -1,7 +1,9 @@
 // learned.hip — learned sparse approximate inverse (no counterpart in the
 // reference): a small MLP reads each row's local stencil couplings and emits
-// the row's entries of an SPD approximate inverse M on A's own pattern, so a
-// preconditioner apply is one SpMV.
+// the row's entries of a lower-triangular G on A's own lower pattern; the
+// preconditioner is M = G^T G (factorized sparse approximate inverse), SPD
+// for any network output, applied as two sparse products instead of a
+// serial triangular solve.
 //
 //   features  f_i[d] = -6 a_ij / sqrt(a_ii a_jj), d = 0..5 the directions
 //             -nx*ny, -nx, -1, +1, +nx, +nx*ny (0 where row i has no entry)
@@ -9,12 +11,11 @@
 //             operands on the matrix cores (v_mfma_f32_16x16x16_bf16), float32
 //             accumulation; flat parameter layout W1[6][64], b1[64], W2[64][64],
 //             b2[64], W3[64][7], b3[7] (W[in][out]), 5063 floats
-//   M         s_ij = (o_i[1+d] + o_j[1+(5-d)]) / 2, m_ij = s_ij dis_i dis_j,
-//             m_ii = (softplus(o_i[0]) + sum_d |s_i,d|) dis_i^2, dis = a_ii^-1/2:
-//             symmetric, strictly diagonally dominant after the symmetric
-//             diagonal scaling, hence SPD (usable in PCG)
-//   training  L = 1/(n k) sum_c ||M A v_c - v_c||^2 over k probe vectors
-//             (random signs and solver error vectors), Adam
+//   G         G_ii = softplus(o_i[0]) dis_i, G_ij = o_i[1+d] dis_j for the
+//             lower neighbours (d = 0, 1, 2), dis = a^-1/2; outputs 4..6 unused
+//   training  L = 1/(n k) sum_c ||v_c - G^T G A v_c||^2 over k probe vectors
+//             (random signs and solver error vectors), Adam; with R = v - M A v
+//             and W = A v, dL/dG_ij = -2/(n k) sum_c ((G W)_i R_j + (G R)_i W_j)
 //
 // MFMA tiling: one wavefront owns a tile of 16 rows. For
 // v_mfma_f32_16x16x16_bf16 lane l holds A[l%16][4(l/16)+v], B[4(l/16)+v][l%16]
@@ -30,6 +31,7 @@
 #include <cmath>
 #include <random>
 
+#include "amg.hpp"
 #include "handles.hpp"
 #include "learned.hpp"
 #include "reduce.hpp"
@@ -368,29 +370,34 @@ __global__ void k_spai_features(uint32_t n, const uint32_t* __restrict__ ci, con
 
 __device__ __forceinline__ double softplus(double x) { return x > 30.0 ? x : log1p(exp(x)); }
 
-template <typename T>
-__global__ void k_spai_assemble(uint32_t n, const uint32_t* __restrict__ ci, const int* __restrict__ epos,
-                                const int* __restrict__ dpos, const double* __restrict__ dis,
-                                const float* __restrict__ o, T* __restrict__ m) {
+// G (lower, A's pattern) and G^T (upper) values from the network outputs;
+// gdir[i*4 + 0] is the diagonal's position in G, gdir[i*4 + 1 + d] that of
+// the lower neighbour in direction d (-1: none); tpos maps a G position to
+// the position of the same entry in G^T
+__global__ void k_fsai_assemble(uint32_t n, const uint32_t* __restrict__ gci, const int* __restrict__ gdir,
+                                const int* __restrict__ tpos, const double* __restrict__ dis,
+                                const float* __restrict__ o, double* __restrict__ gv, double* __restrict__ tv) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double sabs = 0.0;
-    for (int d = 0; d < 6; ++d) {
-        const int p = epos[(size_t)i * 6 + d];
+    const int pd = gdir[(size_t)i * 4];
+    const double gd = softplus((double)o[(size_t)i * 8]) * dis[i];
+    gv[pd] = gd;
+    tv[tpos[pd]] = gd;
+    for (int d = 0; d < 3; ++d) {
+        const int p = gdir[(size_t)i * 4 + 1 + d];
         if (p < 0) continue;
-        const uint32_t j = ci[p];
-        const double s = 0.5 * ((double)o[(size_t)i * 8 + 1 + d] + (double)o[(size_t)j * 8 + 1 + (5 - d)]);
-        m[p] = (T)(s * dis[i] * dis[j]);
-        sabs += fabs(s);
+        const double g = (double)o[(size_t)i * 8 + 1 + d] * dis[gci[p]];
+        gv[p] = g;
+        tv[tpos[p]] = g;
     }
-    m[dpos[i]] = (T)((softplus((double)o[(size_t)i * 8]) + sabs) * dis[i] * dis[i]);
 }
 
-// g_p = scale * sum_c R_c[i] W_c[j] for the entries p of row i (R = v - M A v,
-// W = A v, probe c at offset c*n), plus per-block partial sums of sum_c R^2
-__global__ __launch_bounds__(kBlock) void k_spai_gm(uint32_t n, int k, const uint32_t* __restrict__ rp,
+// g_p = scale * sum_c (GW_c[i] R_c[j] + GR_c[i] W_c[j]) for the entries p =
+// (i, j) of G (probe c at offset c*n), plus per-block partial sums of sum_c R^2
+__global__ __launch_bounds__(kBlock) void k_fsai_gm(uint32_t n, int k, const uint32_t* __restrict__ rp,
                                                     const uint32_t* __restrict__ ci, const double* __restrict__ R,
-                                                    const double* __restrict__ W, double scale,
+                                                    const double* __restrict__ W, const double* __restrict__ GR,
+                                                    const double* __restrict__ GW, double scale,
                                                     double* __restrict__ g, double* partials) {
     double ss = 0.0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -398,7 +405,8 @@ __global__ __launch_bounds__(kBlock) void k_spai_gm(uint32_t n, int k, const uin
         for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
             const uint32_t j = ci[p];
             double s = 0.0;
-            for (int c = 0; c < k; ++c) s += R[(size_t)c * n + i] * W[(size_t)c * n + j];
+            for (int c = 0; c < k; ++c)
+                s += GW[(size_t)c * n + i] * R[(size_t)c * n + j] + GR[(size_t)c * n + i] * W[(size_t)c * n + j];
             g[p] = scale * s;
         }
     }
@@ -412,29 +420,19 @@ struct FinLoss {
     __device__ void operator()(double (&t)[1]) const { *out = t[0] * scale; }
 };
 
-// dL/do from dL/dm (chain through the symmetric assembly)
-__global__ void k_spai_dout(uint32_t n, const uint32_t* __restrict__ ci, const int* __restrict__ epos,
-                            const int* __restrict__ dpos, const double* __restrict__ dis, const float* __restrict__ o,
-                            const double* __restrict__ g, float* __restrict__ dout) {
+// dL/do from dL/dG (o[0] through softplus, o[1..3] linear; o[4..7] unused)
+__global__ void k_fsai_dout(uint32_t n, const uint32_t* __restrict__ gci, const int* __restrict__ gdir,
+                            const double* __restrict__ dis, const float* __restrict__ o, const double* __restrict__ g,
+                            float* __restrict__ dout) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double gii = g[dpos[i]];
     const double o0 = (double)o[(size_t)i * 8];
-    dout[(size_t)i * 8] = (float)(gii * dis[i] * dis[i] / (1.0 + exp(-o0)));
-    for (int d = 0; d < 6; ++d) {
-        const int p = epos[(size_t)i * 6 + d];
-        float r = 0.f;
-        if (p >= 0) {
-            const uint32_t j = ci[p];
-            const int q = epos[(size_t)j * 6 + (5 - d)];
-            const double s = 0.5 * ((double)o[(size_t)i * 8 + 1 + d] + (double)o[(size_t)j * 8 + 1 + (5 - d)]);
-            const double sg = s > 0.0 ? 1.0 : (s < 0.0 ? -1.0 : 0.0);
-            const double ge = dis[i] * dis[j] * (g[p] + g[q]) + sg * (gii * dis[i] * dis[i] + g[dpos[j]] * dis[j] * dis[j]);
-            r = (float)(0.5 * ge);
-        }
-        dout[(size_t)i * 8 + 1 + d] = r;
+    dout[(size_t)i * 8] = (float)(g[gdir[(size_t)i * 4]] * dis[i] / (1.0 + exp(-o0)));
+    for (int d = 0; d < 3; ++d) {
+        const int p = gdir[(size_t)i * 4 + 1 + d];
+        dout[(size_t)i * 8 + 1 + d] = p < 0 ? 0.f : (float)(g[p] * dis[gci[p]]);
     }
-    dout[(size_t)i * 8 + 7] = 0.f;
+    for (int d = 4; d < 8; ++d) dout[(size_t)i * 8 + d] = 0.f;
 }
 
 inline unsigned g1(size_t n) { return (unsigned)((n + 255) / 256); }
@@ -536,18 +534,53 @@ void SpaiPattern::build(const CsrMat& A, const uint32_t* grid, hipStream_t s, bo
     if (e & 1) fail(FEM_ERR_INVALID_ARGUMENT, "learned SPAI: the matrix has entries outside the 7-point stencil of the grid");
     if (e & 2) fail(FEM_ERR_INVALID_ARGUMENT, "learned SPAI: every row needs a positive diagonal entry");
     if (e & 4) fail(FEM_ERR_INVALID_ARGUMENT, "learned SPAI: the sparsity pattern must be symmetric");
-    // M shares A's pattern
+    // G on A's lower pattern (with the diagonal), G^T on the upper one
     A.sync_host();
-    M.set_host(A.n, A.m, A.h_rp, A.h_ci, std::vector<double>(A.nnz, 0.0), s);
+    std::vector<int> hd(n), he((size_t)n * 6);
+    dpos.download(hd.data(), n, s);
+    epos.download(he.data(), (size_t)n * 6, s);
+    FEM_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> grp(n + 1, 0), gci, trp(n + 1, 0), tci;
+    std::vector<int> a2g(A.nnz, -1), a2t(A.nnz, -1);
+    for (uint32_t i = 0; i < n; ++i) {
+        for (uint32_t p = A.h_rp[i]; p < A.h_rp[i + 1]; ++p) {
+            const uint32_t j = A.h_ci[p];
+            if (j <= i) { a2g[p] = (int)gci.size(); gci.push_back(j); }
+            if (j >= i) { a2t[p] = (int)tci.size(); tci.push_back(j); }
+        }
+        grp[i + 1] = (uint32_t)gci.size();
+        trp[i + 1] = (uint32_t)tci.size();
+    }
+    std::vector<int> gdir((size_t)n * 4, -1), tp(gci.size(), -1);
+    for (uint32_t i = 0; i < n; ++i) {
+        gdir[(size_t)i * 4] = a2g[hd[i]];
+        tp[a2g[hd[i]]] = a2t[hd[i]];
+        for (int d = 0; d < 3; ++d) {
+            const int p = he[(size_t)i * 6 + d];
+            if (p < 0) continue;
+            // the mirrored entry (j, i) is row j's neighbour in direction 5 - d
+            const uint32_t j = A.h_ci[p];
+            const int q = he[(size_t)j * 6 + (5 - d)];
+            gdir[(size_t)i * 4 + 1 + d] = a2g[p];
+            tp[a2g[p]] = a2t[q];
+        }
+    }
+    const size_t gnz = gci.size();
+    this->G.set_host(n, n, std::move(grp), std::move(gci), std::vector<double>(gnz, 0.0), s);
+    this->Gt.set_host(n, n, std::move(trp), std::move(tci), std::vector<double>(gnz, 0.0), s);
+    this->gdir.upload(gdir, s);
+    tpos.upload(tp, s);
     out.alloc((size_t)n * 8);
+    FEM_HIP(hipStreamSynchronize(s));
 }
 
 void SpaiPattern::assemble(MlpNet& net, hipStream_t s) {
     net.forward(feat.get(), n, out.get());
-    hipLaunchKernelGGL((k_spai_assemble<double>), dim3(g1(n)), dim3(256), 0, s, n, M.ci.get(), epos.get(), dpos.get(),
-                       dis.get(), out.get(), M.v.get());
+    hipLaunchKernelGGL(k_fsai_assemble, dim3(g1(n)), dim3(256), 0, s, n, G.ci.get(), gdir.get(), tpos.get(), dis.get(),
+                       out.get(), G.v.get(), Gt.v.get());
     FEM_HIP(hipGetLastError());
-    if (M.v32.get()) vec_convert<float, double>(M.v32.get(), M.v.get(), 1.0, M.nnz, s);
+    if (G.v32.get()) vec_convert<float, double>(G.v32.get(), G.v.get(), 1.0, G.nnz, s);
+    if (Gt.v32.get()) vec_convert<float, double>(Gt.v32.get(), Gt.v.get(), 1.0, Gt.nnz, s);
 }
 
 // loss and gradient (left in net.g) for k probe vectors V (device, k x n)
@@ -556,21 +589,25 @@ double spai_loss_grad(MlpNet& net, const CsrMat& A, SpaiPattern& S, const double
     const uint32_t n = S.n;
     if (k < 1) fail(FEM_ERR_INVALID_ARGUMENT, "at least one probe vector is needed");
     S.assemble(net, s);
-    DevBuf<double> W((size_t)n * k), R((size_t)n * k), g(S.M.nnz), out(1);
+    const size_t nk = (size_t)n * k;
+    DevBuf<double> W(nk), R(nk), GW(nk), GR(nk), g(S.G.nnz), out(1);
     DevBuf<float> dout((size_t)n * 8);
     for (int c = 0; c < k; ++c) {
-        csr_spmv<double>(A, V + (size_t)c * n, W.get() + (size_t)c * n, EPI_STORE, {}, s, Gate{});
+        const size_t o = (size_t)c * n;
+        csr_spmv<double>(A, V + o, W.get() + o, EPI_STORE, {}, s, Gate{});
+        csr_spmv<double>(S.G, W.get() + o, GW.get() + o, EPI_STORE, {}, s, Gate{});
         SpmvArgs<double> ar;
-        ar.f = V + (size_t)c * n;
-        csr_spmv<double>(S.M, W.get() + (size_t)c * n, R.get() + (size_t)c * n, EPI_RESID, ar, s, Gate{});
+        ar.f = V + o;
+        csr_spmv<double>(S.Gt, GW.get() + o, R.get() + o, EPI_RESID, ar, s, Gate{});  // R = v - G^T G A v
+        csr_spmv<double>(S.G, R.get() + o, GR.get() + o, EPI_STORE, {}, s, Gate{});
     }
     const double sc = 1.0 / ((double)n * k);
     const unsigned G = red_blocks(n);
-    hipLaunchKernelGGL(k_spai_gm, dim3(G), dim3(kBlock), 0, s, n, k, S.M.rp.get(), S.M.ci.get(), R.get(), W.get(),
-                       -2.0 * sc, g.get(), red.partials.get());
+    hipLaunchKernelGGL(k_fsai_gm, dim3(G), dim3(kBlock), 0, s, n, k, S.G.rp.get(), S.G.ci.get(), R.get(), W.get(),
+                       GR.get(), GW.get(), -2.0 * sc, g.get(), red.partials.get());
     launch_finalize<1>(red.partials.get(), G, FinLoss{out.get(), sc}, s);
-    hipLaunchKernelGGL(k_spai_dout, dim3(g1(n)), dim3(256), 0, s, n, S.M.ci.get(), S.epos.get(), S.dpos.get(),
-                       S.dis.get(), S.out.get(), g.get(), dout.get());
+    hipLaunchKernelGGL(k_fsai_dout, dim3(g1(n)), dim3(256), 0, s, n, S.G.ci.get(), S.gdir.get(), S.dis.get(),
+                       S.out.get(), g.get(), dout.get());
     FEM_HIP(hipGetLastError());
     net.backward(S.feat.get(), n, dout.get());
     double L = 0.0;
@@ -582,36 +619,86 @@ double spai_loss_grad(MlpNet& net, const CsrMat& A, SpaiPattern& S, const double
 // ---------------------------------------------------------------------------
 // the preconditioner
 // ---------------------------------------------------------------------------
-struct PcLearned : Precond {
+// The apply of a learned G (PcLearned) and of a given sparse approximate
+// inverse (PcExplicitSpai: M, or G with M = G^T G).
+struct PcFactorized : Precond {
+    const CsrMat* Gm = nullptr;   // G (lower), or M when !factorized
+    const CsrMat* Gtm = nullptr;  // G^T
+    bool factorized = true;
+    uint32_t n = 0;
+    DevBuf<double> t64;
+    DevBuf<float> t32;  // G r
+    bool fp32 = false;
+    uint32_t size() const override { return n; }
+    // z = G^T (G r) as two CSR-stream products through t; with DOT the first
+    // also leaves the partial sums of t_i^2 (||G r||^2 = r^T M r = r.z)
+    template <typename T, bool DOT>
+    unsigned two_products(const T* r, T* z, double* part, hipStream_t s, Gate g) {
+        T* t;
+        if constexpr (std::is_same<T, double>::value) t = t64.get();
+        else t = t32.get();
+        unsigned np = 0;
+        if constexpr (DOT) np = csr_spmv_selfdot<T>(*Gm, r, t, part, s, g);
+        else csr_spmv<T>(*Gm, r, t, EPI_STORE, {}, s, g);
+        csr_spmv<T>(*Gtm, t, z, EPI_STORE, {}, s, g);
+        return np;
+    }
+    void apply(const double* r, double* z, hipStream_t s, Gate g) override {
+        if (!factorized) { csr_spmv<double>(*Gm, r, z, EPI_STORE, {}, s, g); return; }
+        two_products<double, false>(r, z, nullptr, s, g);
+    }
+    void apply32(const float* r, float* z, hipStream_t s, Gate g) override {
+        if (!factorized) { csr_spmv<float>(*Gm, r, z, EPI_STORE, {}, s, g); return; }
+        two_products<float, false>(r, z, nullptr, s, g);
+    }
+    // r.z from the apply's own partial sums
+    unsigned apply_dot(const double* r, double* z, double* part, hipStream_t s, Gate g) override {
+        return factorized ? two_products<double, true>(r, z, part, s, g) : csr_spmv_dot<double>(*Gm, r, z, part, s, g);
+    }
+    unsigned apply32_dot(const float* r, float* z, double* part, hipStream_t s, Gate g) override {
+        return factorized ? two_products<float, true>(r, z, part, s, g) : csr_spmv_dot<float>(*Gm, r, z, part, s, g);
+    }
+    unsigned dot_parts() const override { return Gm->dot_parts(); }
+    bool has_fp32() const override { return fp32; }
+    void ensure_fp32(hipStream_t s) override {
+        if (fp32) return;
+        const_cast<CsrMat*>(Gm)->ensure_fp32(s);
+        if (factorized) {
+            const_cast<CsrMat*>(Gtm)->ensure_fp32(s);
+            t32.alloc(n);
+        }
+        FEM_HIP(hipStreamSynchronize(s));
+        fp32 = true;
+    }
+    double apply_bytes(int sz) override {
+        return factorized ? csr_bytes(*Gm, sz, 0) + csr_bytes(*Gtm, sz, 0) : csr_bytes(*Gm, sz, 0);
+    }
+};
+
+struct PcLearned : PcFactorized {
     SpaiPattern S;
     CsrMat* A = nullptr;
     int fb_kind = FEM_PC_IC0;
     int budget_ = 0;
+    int ref_iters = 0;  // the fallback's iteration count on this matrix (0: unknown)
     std::unique_ptr<Precond> fb;
-    bool fp32 = false;
+    void bind() {
+        Gm = &S.G;
+        Gtm = &S.Gt;
+        factorized = true;
+        n = S.n;
+        t64.alloc(n);
+    }
     int kind() const override { return FEM_PC_LEARNED; }
-    uint32_t size() const override { return S.n; }
-    void apply(const double* r, double* z, hipStream_t s, Gate g) override {
-        csr_spmv<double>(S.M, r, z, EPI_STORE, {}, s, g);
+    // a learned solve stops once it needs more iterations than the explicit
+    // budget or than the fallback needed (an iteration-count regression)
+    int budget() const override {
+        if (budget_ > 0 && ref_iters > 0) return std::min(budget_, ref_iters);
+        return budget_ > 0 ? budget_ : ref_iters;
     }
-    void apply32(const float* r, float* z, hipStream_t s, Gate g) override {
-        csr_spmv<float>(S.M, r, z, EPI_STORE, {}, s, g);
+    void fallback_ran(int iterations) override {
+        if (iterations > 0) ref_iters = iterations;
     }
-    unsigned apply_dot(const double* r, double* z, double* part, hipStream_t s, Gate g) override {
-        return csr_spmv_dot<double>(S.M, r, z, part, s, g);
-    }
-    unsigned apply32_dot(const float* r, float* z, double* part, hipStream_t s, Gate g) override {
-        return csr_spmv_dot<float>(S.M, r, z, part, s, g);
-    }
-    unsigned dot_parts() const override { return S.M.dot_parts(); }
-    bool has_fp32() const override { return fp32; }
-    void ensure_fp32(hipStream_t s) override {
-        if (fp32) return;
-        S.M.ensure_fp32(s);
-        FEM_HIP(hipStreamSynchronize(s));
-        fp32 = true;
-    }
-    int budget() const override { return budget_; }
     Precond* fallback(hipStream_t s) override {
         if (!fb) {
             fem_pc_params p;
@@ -620,6 +707,11 @@ struct PcLearned : Precond {
         }
         return fb.get();
     }
+};
+
+struct PcExplicitSpai : PcFactorized {
+    CsrMat Gt_own;  // G^T (factorized)
+    int kind() const override { return FEM_PC_SPAI; }
 };
 
 }  // namespace fem
@@ -772,10 +864,45 @@ int fem_precond_create_learned(fem_csr A, const uint32_t* grid, fem_mlp net, int
         pc->fb_kind = fallback_kind;
         pc->budget_ = budget;
         pc->S.build(A->A, grid, s, false);
+        pc->bind();
         // the network runs on its own stream: order it after the analysis
         FEM_HIP(hipStreamSynchronize(s));
         pc->S.assemble(*net->net, net->stream);
         FEM_HIP(hipStreamSynchronize(net->stream));
+        pc->setup_ms = tm.ms();
+        auto h = std::make_unique<fem_pc_s>();
+        h->stream = s;
+        h->pc = std::move(pc);
+        *out = h.release();
+    });
+}
+
+int fem_precond_learned_reference(fem_pc M, int iterations) {
+    return api_guard([&] {
+        auto* p = M ? dynamic_cast<PcLearned*>(M->pc.get()) : nullptr;
+        if (!p) fail(FEM_ERR_INVALID_ARGUMENT, "not a learned preconditioner");
+        if (iterations < 0) fail(FEM_ERR_INVALID_ARGUMENT, "iterations must be >= 0");
+        p->ref_iters = iterations;
+    });
+}
+
+int fem_precond_create_spai(fem_csr A, fem_csr Mat, int factorized, fem_pc* out) {
+    return api_guard([&] {
+        if (!A || !Mat || !out) fail(FEM_ERR_INVALID_ARGUMENT, "null argument");
+        if (Mat->A.n != A->A.n || Mat->A.m != A->A.n)
+            fail(FEM_ERR_INVALID_ARGUMENT, "the approximate inverse must be square and sized as A");
+        Timer tm;
+        hipStream_t s = A->stream;
+        auto pc = std::make_unique<PcExplicitSpai>();
+        pc->n = A->A.n;
+        pc->Gm = &Mat->A;
+        pc->factorized = factorized != 0;
+        if (pc->factorized) {
+            transpose_device(Mat->A, pc->Gt_own, s);
+            pc->Gtm = &pc->Gt_own;
+            pc->t64.alloc(pc->n);
+        }
+        FEM_HIP(hipStreamSynchronize(s));
         pc->setup_ms = tm.ms();
         auto h = std::make_unique<fem_pc_s>();
         h->stream = s;

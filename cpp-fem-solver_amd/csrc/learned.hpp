@@ -26,14 +26,17 @@ struct MlpNet {
 };
 
 // per-matrix data of the learned SPAI: entry positions per direction, the
-// diagonal scaling, the network's features and outputs, and M (A's pattern)
+// diagonal scaling, the network's features and outputs, G on A's lower
+// pattern and G^T on its upper one (gdir: per row the G positions of the
+// diagonal and of the three lower neighbours; tpos: G position -> G^T
+// position of the same entry)
 struct SpaiPattern {
     uint32_t n = 0;
-    DevBuf<int> epos, dpos;
+    DevBuf<int> epos, dpos, gdir, tpos;
     DevBuf<double> dis;
     DevBuf<short> feat;
     DevBuf<float> feat32, out;
-    CsrMat M;
+    CsrMat G, Gt;
     void build(const CsrMat& A, const uint32_t* grid, hipStream_t s, bool keep_f32);
     void assemble(MlpNet& net, hipStream_t s);
 };

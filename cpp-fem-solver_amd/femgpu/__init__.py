@@ -15,8 +15,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfemgpu.so")
 
 # enums (femgpu.h)
-PC_NONE, PC_DIAG, PC_IC0, PC_ILU0, PC_SSOR, PC_AMG, PC_SAAMG, PC_LEARNED, PC_BJACOBI, PC_BILU0, PC_BSAAMG, PC_SCHUR = \
-    range(12)
+PC_NONE, PC_DIAG, PC_IC0, PC_ILU0, PC_SSOR, PC_AMG, PC_SAAMG, PC_LEARNED, PC_BJACOBI, PC_BILU0, PC_BSAAMG, PC_SCHUR, \
+    PC_SPAI = range(13)
 PCG, BICGSTAB, GMRES, MINICG = 0, 1, 2, 3
 FP64, FP32_IR = 0, 1
 SMOOTH_JACOBI, SMOOTH_SGS = 0, 1
@@ -157,6 +157,8 @@ _SIGS = {
     "fem_spai_loss_grad": (C.c_int, [P_vp, P_vp, P_u32, P_f64, C.c_int, C.c_int, P_f64, P_f32]),
     "fem_spai_train_step": (C.c_int, [P_vp, P_vp, P_u32, P_f64, C.c_int, C.c_int, C.POINTER(AdamOpts), P_f64]),
     "fem_precond_create_learned": (C.c_int, [P_vp, P_u32, P_vp, C.c_int, C.c_int, C.POINTER(P_vp)]),
+    "fem_precond_learned_reference": (C.c_int, [P_vp, C.c_int]),
+    "fem_precond_create_spai": (C.c_int, [P_vp, P_vp, C.c_int, C.POINTER(P_vp)]),
     "fem_batch_create": (C.c_int, [C.c_int, C.POINTER(BatchSystem), C.c_int, C.POINTER(P_vp)]),
     "fem_batch_set_rhs": (C.c_int, [P_vp, P_f64]),
     "fem_batch_solve": (C.c_int, [P_vp, C.POINTER(BatchOpts), C.POINTER(BatchStats), P_f64]),
@@ -444,17 +446,35 @@ class SchurPreconditioner(Preconditioner):
 
 
 class LearnedPreconditioner(Preconditioner):
-    """Learned SPAI M (fem_precond_create_learned): one SpMV per apply; a
-    solve stops after `budget` iterations (0: no limit) and is redone with
-    the fallback preconditioner if it has not converged."""
+    """Learned factorized SPAI M = G^T G (fem_precond_create_learned): two
+    sparse products per apply; a solve stops after `budget` iterations (0: no
+    explicit limit) or after the fallback's recorded iteration count, and is
+    then redone with the fallback preconditioner."""
 
     def __init__(self, A, grid, net, fallback="ic0", budget=0):
         self.kind = PC_LEARNED
         self._h = C.c_void_p()
         self.n = A.rows
+        self._A = (A, net)
         fb = PC_KINDS[fallback] if isinstance(fallback, str) else fallback
         g = _u32(grid)
         _check(lib().fem_precond_create_learned(A._h, _p(g, C.c_uint32), net._h, fb, int(budget), C.byref(self._h)))
+
+    def reference(self, iterations: int):
+        """The fallback's iteration count on this matrix (fem_precond_learned_reference)."""
+        _check(lib().fem_precond_learned_reference(self._h, int(iterations)))
+
+
+class SpaiPreconditioner(Preconditioner):
+    """A given sparse approximate inverse (fem_precond_create_spai): z = M r,
+    or with factorized=True, Mat is G and z = G^T (G r)."""
+
+    def __init__(self, A, Mat, factorized=False):
+        self.kind = PC_SPAI
+        self._h = C.c_void_p()
+        self.n = A.rows
+        self._A = (A, Mat)
+        _check(lib().fem_precond_create_spai(A._h, Mat._h, 1 if factorized else 0, C.byref(self._h)))
 
 
 def solve_opts(solver="pcg", tol=1e-10, max_iter=2000, restart=30, precision=FP64, inner_tol=None,

@@ -68,9 +68,12 @@ enum fem_pc_kind {
                            test_BlockMatrix_mpi.cpp:132-155                   */
     FEM_PC_BILU0 = 9,   /* block ILU(0), blockMatrix.hpp:454-623             */
     FEM_PC_BSAAMG = 10, /* block SA-AMG on a BSR matrix, blockMatrix.hpp:1051 */
-    FEM_PC_SCHUR = 11   /* Schur-complement preconditioner of a saddle-point
+    FEM_PC_SCHUR = 11,  /* Schur-complement preconditioner of a saddle-point
                            matrix, blockMatrix.hpp:625-685 (see
                            fem_precond_create_schur)                         */
+    FEM_PC_SPAI = 12    /* a given sparse approximate inverse M, or G with
+                           M = G^T G (no reference counterpart; see
+                           fem_precond_create_spai)                          */
 };
 
 enum fem_solver_kind {
@@ -291,15 +294,16 @@ int fem_bsr_destroy(fem_bsr A);
  * No counterpart in the reference: an MLP (6 -> 64 -> 64 -> 7, ReLU) reads
  * each row's scaled stencil couplings f_i[d] = -6 a_ij / sqrt(a_ii a_jj) of a
  * 7-point grid matrix (rows (z*ny + y)*nx + x, directions d = 0..5 for the
- * column offsets -nx*ny, -nx, -1, +1, +nx, +nx*ny) and emits the row's entries
- * of an SPD approximate inverse M on A's pattern:
- *   s_ij = (o_i[1+d] + o_j[1+(5-d)]) / 2,  m_ij = s_ij / sqrt(a_ii a_jj),
- *   m_ii = (softplus(o_i[0]) + sum_d |s_i,d|) / a_ii.
- * Parameters are float32, flat, FEM_MLP_NPARAM of them: W1[6][64], b1[64],
- * W2[64][64], b2[64], W3[64][7], b3[7] (W[in][out]). The device kernels run
- * the layers on the matrix cores with bfloat16 operands and float32
- * accumulation. Training minimises L = 1/(n k) sum_c ||M A v_c - v_c||^2 over
- * k probe vectors with Adam. grid[3] = {nx, ny, nz}. */
+ * column offsets -nx*ny, -nx, -1, +1, +nx, +nx*ny) and emits the row's
+ * entries of a lower-triangular G on A's lower pattern:
+ *   G_ii = softplus(o_i[0]) / sqrt(a_ii), G_ij = o_i[1+d] / sqrt(a_jj) for the
+ *   lower neighbours (d = 0, 1, 2; outputs 4..6 unused).
+ * The preconditioner is M = G^T G, SPD for any output, applied as two sparse
+ * products. Parameters are float32, flat, FEM_MLP_NPARAM of them: W1[6][64],
+ * b1[64], W2[64][64], b2[64], W3[64][7], b3[7] (W[in][out]). The device
+ * kernels run the layers on the matrix cores with bfloat16 operands and
+ * float32 accumulation. Training minimises L = 1/(n k) sum_c ||v_c - M A v_c||^2
+ * over k probe vectors with Adam. grid[3] = {nx, ny, nz}. */
 #define FEM_MLP_NPARAM 5063
 typedef struct fem_mlp_s* fem_mlp;
 typedef struct fem_adam_opts {
@@ -322,13 +326,24 @@ int fem_spai_loss_grad(fem_mlp net, fem_csr A, const uint32_t* grid, const doubl
 /* one Adam step on that loss; *loss is the loss before the step */
 int fem_spai_train_step(fem_mlp net, fem_csr A, const uint32_t* grid, const double* V, int k, int device_ptrs,
                         const fem_adam_opts* opt, double* loss);
-/* M from the network (one SpMV per apply). A solve with this preconditioner
- * runs at most `budget` iterations (0: no limit); if it has not converged by
- * then, or breaks down, the system is solved again with the fallback
- * preconditioner (FEM_PC_IC0, FEM_PC_ILU0, FEM_PC_AMG, FEM_PC_SAAMG or
- * FEM_PC_DIAG, built on first use) and fem_solve_stats.fallback is set. */
+/* M = G^T G from the network (two sparse products per apply). A solve with
+ * this preconditioner stops after `budget` iterations (0: no explicit limit)
+ * or, once the fallback's iteration count on this matrix is known, after
+ * that many (a learned solve needing more is a regression); it is then
+ * solved again with the fallback preconditioner (FEM_PC_IC0, FEM_PC_ILU0,
+ * FEM_PC_AMG, FEM_PC_SAAMG or FEM_PC_DIAG, built on first use) and
+ * fem_solve_stats.fallback is set. The fallback's count is recorded whenever
+ * it runs, or given by fem_precond_learned_reference. */
 int fem_precond_create_learned(fem_csr A, const uint32_t* grid, fem_mlp net, int fallback_kind, int budget,
                                fem_pc* out);
+/* the fallback's iteration count on this matrix (0: unknown), e.g. from one
+ * solve with the fallback preconditioner */
+int fem_precond_learned_reference(fem_pc M, int iterations);
+/* A given sparse approximate inverse (no counterpart in the reference; the
+ * non-learned baselines of the learned one use it): factorized = 0 applies
+ * z = Mat r, factorized = 1 takes Mat as G and applies z = G^T (G r). Mat
+ * must be n x n for A n x n; the result keeps a pointer to Mat. */
+int fem_precond_create_spai(fem_csr A, fem_csr Mat, int factorized, fem_pc* out);
 
 /* ---- batched solves ----------------------------------------------------------
  * No counterpart in the reference: many independent SPD systems (block size

@@ -1,23 +1,29 @@
 // spai_oracle.cpp — CPU restatement (test infrastructure only) of the
 // learned sparse approximate inverse: the local features, the MLP forward
-// and backward pass, the symmetric SPD assembly of M on A's pattern and the
-// training loss with its gradient. No counterpart in the reference; the
-// product implements the same definitions in cpp-fem-solver_amd/csrc/learned.hip.
+// and backward pass, the factorized SPD approximate inverse M = G^T G with G
+// on the lower triangle of A's pattern, and the training loss with its
+// gradient. No counterpart in the reference; the product implements the same
+// definitions in cpp-fem-solver_amd/csrc/learned.hip.
 //
 // Definitions (7-point grid matrix, rows (z*ny + y)*nx + x):
 //   directions d = 0..5 <-> column offsets -nx*ny, -nx, -1, +1, +nx, +nx*ny;
-//   the opposite of d is 5 - d. dis_i = 1 / sqrt(a_ii).
+//   the opposite of d is 5 - d; d = 0, 1, 2 are the lower neighbours.
+//   dis_i = 1 / sqrt(a_ii).
 //   features  f_i[d] = -6 a_ij dis_i dis_j for the neighbour j in direction d
 //             (0 when row i has no entry there)
 //   network   o_i = W3^T relu(W2^T relu(W1^T f_i + b1) + b2) + b3, 6 -> 64 ->
 //             64 -> 7, parameters float32, flat layout W1[6][64], b1[64],
 //             W2[64][64], b2[64], W3[64][7], b3[7] (W[in][out])
-//   edge      s_ij = (o_i[1+d] + o_j[1+(5-d)]) / 2 (symmetric)
-//   M         m_ij = s_ij dis_i dis_j (j != i),
-//             m_ii = (softplus(o_i[0]) + sum_d |s_i,d|) dis_i^2;
-//             M = D^{-1/2} Mt D^{-1/2} with Mt symmetric and strictly
-//             diagonally dominant with a positive diagonal, hence SPD.
-//   loss      L = 1/(n k) sum_c || M A v_c - v_c ||^2 over k probe vectors.
+//   G         lower triangular on A's pattern:
+//             G_ii = softplus(o_i[0]) dis_i, G_ij = o_i[1+d] dis_j for the
+//             lower neighbour j in direction d (outputs 4..6 are unused);
+//             M = G^T G (an approximation of A^{-1} = D^{-1/2} Ahat^{-1}
+//             D^{-1/2} through Ghat = G D^{1/2}, Ghat^T Ghat ~ Ahat^{-1}); G
+//             has a positive diagonal, so M is SPD for any network output
+//   loss      L = 1/(n k) sum_c || v_c - G^T G A v_c ||^2 over k probe vectors
+//   gradient  with R_c = v_c - M A v_c and W_c = A v_c,
+//             dL/dG_ij = -2/(n k) sum_c ((G W_c)_i R_c,j + (G R_c)_i W_c,j)
+//             on G's pattern (from dM = dG^T G + G^T dG).
 // With bf16 = 1 the matrix-product operands are rounded to bfloat16 (round to
 // nearest even) exactly where the device kernels round them (features,
 // weights, hidden activations, and in the backward pass dL/do and the masked
@@ -137,20 +143,16 @@ void forward(const float* P, const std::vector<double>& feat, uint32_t n, bool b
 double softplus(double x) { return x > 30.0 ? x : std::log1p(std::exp(x)); }
 double sigmoid(double x) { return 1.0 / (1.0 + std::exp(-x)); }
 
+// G's values on A's pattern (entries above the diagonal stay 0)
 void assemble(const Mat& A, const std::vector<double>& dis, const std::vector<int64_t>& dpos,
-              const std::vector<int64_t>& epos, const std::vector<double>& o, std::vector<double>& m) {
-    m.assign(A.rp[A.n], 0.0);
+              const std::vector<int64_t>& epos, const std::vector<double>& o, std::vector<double>& g) {
+    g.assign(A.rp[A.n], 0.0);
     for (uint32_t i = 0; i < A.n; ++i) {
-        double sabs = 0.0;
-        for (int d = 0; d < 6; ++d) {
+        for (int d = 0; d < 3; ++d) {
             const int64_t p = epos[(size_t)i * 6 + d];
-            if (p < 0) continue;
-            const uint32_t j = A.ci[p];
-            const double s = 0.5 * (o[(size_t)i * NO + 1 + d] + o[(size_t)j * NO + 1 + (5 - d)]);
-            m[p] = s * dis[i] * dis[j];
-            sabs += std::fabs(s);
+            if (p >= 0) g[p] = o[(size_t)i * NO + 1 + d] * dis[A.ci[p]];
         }
-        m[dpos[i]] = (softplus(o[(size_t)i * NO]) + sabs) * dis[i] * dis[i];
+        g[dpos[i]] = softplus(o[(size_t)i * NO]) * dis[i];
     }
 }
 
@@ -160,6 +162,13 @@ void spmv(const Mat& A, const double* vals, const double* x, double* y) {
         for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) s += vals[p] * x[A.ci[p]];
         y[i] = s;
     }
+}
+
+// y = G^T x for G stored on A's pattern
+void spmv_t(const Mat& A, const double* vals, const double* x, double* y) {
+    for (uint32_t j = 0; j < A.n; ++j) y[j] = 0.0;
+    for (uint32_t i = 0; i < A.n; ++i)
+        for (uint32_t p = A.rp[i]; p < A.rp[i + 1]; ++p) y[A.ci[p]] += vals[p] * x[i];
 }
 
 template <class F>
@@ -203,7 +212,7 @@ int or_mlp_forward(const float* params, uint32_t n, const double* feat, int bf16
     });
 }
 
-// values of M on A's pattern (nnz)
+// values of G on A's pattern (nnz; zero above the diagonal)
 int or_spai_assemble(uint32_t n, const uint32_t* rp, const uint32_t* ci, const double* v, const uint32_t* grid,
                      const float* params, int bf16, double* mvals) {
     return guard([&] {
@@ -232,40 +241,40 @@ int or_spai_loss_grad(uint32_t n, const uint32_t* rp, const uint32_t* ci, const 
         forward(params, feat, n, b16, F);
         assemble(A, dis, dp, ep, F.o, m);
         const uint64_t nnz = rp[n];
-        std::vector<double> W((size_t)n * k), R((size_t)n * k), g(nnz, 0.0);
+        std::vector<double> W((size_t)n * k), R((size_t)n * k), GW((size_t)n * k), GR((size_t)n * k), g(nnz, 0.0),
+            t(n);
         double L = 0.0;
         for (int c = 0; c < k; ++c) {
-            spmv(A, v, V + (size_t)c * n, &W[(size_t)c * n]);
-            spmv(A, m.data(), &W[(size_t)c * n], &R[(size_t)c * n]);
+            double* Wc = &W[(size_t)c * n];
+            double* Rc = &R[(size_t)c * n];
+            spmv(A, v, V + (size_t)c * n, Wc);
+            spmv(A, m.data(), Wc, &GW[(size_t)c * n]);
+            spmv_t(A, m.data(), &GW[(size_t)c * n], t.data());
             for (uint32_t i = 0; i < n; ++i) {
-                const double r = R[(size_t)c * n + i] - V[(size_t)c * n + i];
-                R[(size_t)c * n + i] = r;
-                L += r * r;
+                Rc[i] = V[(size_t)c * n + i] - t[i];
+                L += Rc[i] * Rc[i];
             }
+            spmv(A, m.data(), Rc, &GR[(size_t)c * n]);
         }
         const double sc = 1.0 / ((double)n * k);
         *loss = L * sc;
-        // dL/dm_p = 2/(nk) sum_c R_ic W_jc
+        // dL/dG_ij on the lower pattern
         for (uint32_t i = 0; i < n; ++i)
             for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
+                const uint32_t j = ci[p];
+                if (j > i) continue;
                 double s = 0.0;
-                for (int c = 0; c < k; ++c) s += R[(size_t)c * n + i] * W[(size_t)c * n + ci[p]];
-                g[p] = 2.0 * sc * s;
+                for (int c = 0; c < k; ++c)
+                    s += GW[(size_t)c * n + i] * R[(size_t)c * n + j] + GR[(size_t)c * n + i] * W[(size_t)c * n + j];
+                g[p] = -2.0 * sc * s;
             }
         // dL/do
         std::vector<double> dO((size_t)n * NO, 0.0);
         for (uint32_t i = 0; i < n; ++i) {
-            const double gii = g[dp[i]];
-            dO[(size_t)i * NO] = gii * dis[i] * dis[i] * sigmoid(F.o[(size_t)i * NO]);
-            for (int d = 0; d < 6; ++d) {
+            dO[(size_t)i * NO] = g[dp[i]] * dis[i] * sigmoid(F.o[(size_t)i * NO]);
+            for (int d = 0; d < 3; ++d) {
                 const int64_t p = ep[(size_t)i * 6 + d];
-                if (p < 0) continue;
-                const uint32_t j = ci[p];
-                const int64_t q = ep[(size_t)j * 6 + (5 - d)];
-                const double s = 0.5 * (F.o[(size_t)i * NO + 1 + d] + F.o[(size_t)j * NO + 1 + (5 - d)]);
-                const double sg = s > 0.0 ? 1.0 : (s < 0.0 ? -1.0 : 0.0);
-                const double ge = dis[i] * dis[j] * (g[p] + g[q]) + sg * (gii * dis[i] * dis[i] + g[dp[j]] * dis[j] * dis[j]);
-                dO[(size_t)i * NO + 1 + d] = 0.5 * ge;
+                if (p >= 0) dO[(size_t)i * NO + 1 + d] = g[p] * dis[ci[p]];
             }
         }
         // backward through the network

@@ -89,45 +89,48 @@ def _const_output_params(b3):
 
 
 def test_assembly_closed_form_and_spd():
-    """Constant network output o = b3: s_d = (b3[1+d] + b3[1+5-d]) / 2,
-    m_ij = s_d dis_i dis_j, m_ii = (softplus(b3[0]) + sum_d |s_d|) / a_ii."""
+    """Constant network output o = b3: G_ii = softplus(b3[0]) / sqrt(a_ii),
+    G_ij = b3[1+d] / sqrt(a_jj) for the lower neighbour j in direction d
+    (d = 0, 1, 2: offsets -16, -4, -1 on the 4^3 grid), zero above the
+    diagonal; M = G^T G is SPD for any output."""
     G = gen.vardiff3d(4, seed=2, ell=1.5)[0]
     b3 = np.array([0.3, 0.1, -0.2, 0.05, 0.4, -0.1, 0.2], np.float32)
-    m = O.spai_assemble(G, [4, 4, 4], _const_output_params(b3))
+    g = O.spai_assemble(G, [4, 4, 4], _const_output_params(b3))
     A = G.to_scipy().tocsr()
     d = A.diagonal()
-    off = [-16, -4, -1, 1, 4, 16]
-    sd = [0.5 * (float(b3[1 + k]) + float(b3[1 + 5 - k])) for k in range(6)]
-    ref = np.empty_like(m)
+    off = [-16, -4, -1]
+    ref = np.zeros_like(g)
     for i in range(G.n):
-        tot = 0.0
         for p in range(A.indptr[i], A.indptr[i + 1]):
             j = A.indices[p]
             if j == i:
-                continue
-            k = off.index(j - i)
-            ref[p] = sd[k] / math.sqrt(d[i] * d[j])
-            tot += abs(sd[k])
-        pd = [p for p in range(A.indptr[i], A.indptr[i + 1]) if A.indices[p] == i][0]
-        ref[pd] = (math.log1p(math.exp(float(b3[0]))) + tot) / d[i]
-    np.testing.assert_allclose(m, ref, rtol=1e-14)
-    # random network: symmetric, strictly diagonally dominant after scaling, SPD
-    m = O.spai_assemble(G, [4, 4, 4], _rand_params(3))
-    M = sp.csr_matrix((m, A.indices, A.indptr), shape=A.shape).toarray()
-    assert np.allclose(M, M.T, rtol=0, atol=1e-15)
-    assert np.linalg.eigvalsh(M).min() > 0
+                ref[p] = math.log1p(math.exp(float(b3[0]))) / math.sqrt(d[i])
+            elif j - i in off:
+                ref[p] = float(b3[1 + off.index(j - i)]) / math.sqrt(d[j])
+    np.testing.assert_allclose(g, ref, rtol=1e-14, atol=0)
+    # random network: G lower triangular with a positive diagonal, M SPD
+    g = O.spai_assemble(G, [4, 4, 4], _rand_params(3))
+    Gm = sp.csr_matrix((g, A.indices, A.indptr), shape=A.shape).toarray()
+    assert np.all(np.triu(Gm, 1) == 0) and np.all(np.diag(Gm) > 0)
+    M = Gm.T @ Gm
+    assert np.linalg.eigvalsh(0.5 * (M + M.T)).min() > 0
 
 
 def test_loss_known_answers():
-    """A diagonal (no neighbours): M = softplus(b3[0]) A^{-1}, so M A v = c v."""
+    """A diagonal (no neighbours): G = softplus(b3[0]) D^{-1/2}, so
+    G^T G A v = c^2 v with c = softplus(b3[0]) and L = (c^2 - 1)^2 mean(v^2)."""
     n = 27
     G = gen.CSR(n, n, np.arange(n + 1, dtype=np.uint32), np.arange(n, dtype=np.uint32),
                 np.random.default_rng(0).uniform(1, 3, n))
     V = np.random.default_rng(1).standard_normal((3, n))
-    for c in (1.0, 2.0):
+    for c in (1.0, 1.5):
         b0 = math.log(math.expm1(c))
         L, _ = O.spai_loss_grad(G, [3, 3, 3], _const_output_params([b0, 0, 0, 0, 0, 0, 0]), V)
-        assert abs(L - (c - 1) ** 2 * np.mean(V ** 2)) < 1e-6 * max(1.0, L)
+        assert abs(L - (c * c - 1) ** 2 * np.mean(V ** 2)) < 1e-6 * max(1.0, L)
+    # G ~ 0 (softplus(-80) ~ 2e-35): M A v ~ 0 and L = mean(v^2)
+    P0 = _const_output_params([-80.0, 0, 0, 0, 0, 0, 0])
+    L0, _ = O.spai_loss_grad(G, [3, 3, 3], P0, V)
+    assert abs(L0 - np.mean(V ** 2)) < 1e-9 * np.mean(V ** 2)
 
 
 def test_gradient_central_differences():
