@@ -13,11 +13,13 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
 
 
-def tol_x(G, tol=1e-10):
-    if G.n > 5000:
-        return TOL_X
-    c = np.linalg.cond(G.to_scipy().toarray())
-    return max(TOL_X, 2.0 * tol * c)
+def tol_x(G, tol=1e-10, cond=None):
+    """cond: cond(A) where it is known in closed form (any size)."""
+    if cond is None:
+        if G.n > 5000:
+            return TOL_X
+        cond = np.linalg.cond(G.to_scipy().toarray())
+    return max(TOL_X, 2.0 * tol * cond)
 
 
 # ---- condition numbers and bounds for the large workloads -------------------

@@ -43,11 +43,19 @@ def test_schur_apply_and_gmres_match_oracle(spc):
     # 4361); the true residual of x follows it to rounding, as the oracle's
     res_o = np.linalg.norm(b - S3["K"].matvec(xo)) / np.linalg.norm(b)
     assert st["true_relres"] < max(1.5e-10, 1.5 * res_o)
-    assert np.linalg.norm(x - xo) / np.linalg.norm(xo) < 1e-8
+    # K is symmetric with the one null vector (0, constant pressure), which
+    # GMRES leaves where its Krylov space puts it: off that direction two
+    # iterates differ by at most cond'(K) (r1 + r2), cond' = |lambda|_max /
+    # the smallest nonzero |lambda| (130 for N = 10)
+    lam = np.sort(np.abs(np.linalg.eigvalsh(S3["K"].to_scipy().toarray())))
+    assert lam[0] < 1e-12 * lam[-1] < lam[1]
+    nv = S3["nv"]
+    d = x - xo
+    d[nv:] -= d[nv:].mean()
+    assert np.linalg.norm(d) / np.linalg.norm(xo) < max(1e-8, lam[-1] / lam[1] * (st["true_relres"] + res_o))
     assert abs(st["iterations"] - ito) <= 2
     # K's null vector is (0, constant pressure): the drand solution is met up
     # to a pressure constant (the reference test's 1e-6 bar)
-    nv = S3["nv"]
     d = x - xe
     assert np.linalg.norm(d[:nv]) < 1e-6 and np.linalg.norm(d[nv:] - d[nv:].mean()) < 1e-6
 
