@@ -498,6 +498,38 @@ int fem_bench_spmv(fem_csr A, int fp32, int reps, double* avg_ms) {
     });
 }
 
+// Average device time of `reps` back-to-back preconditioner applications
+// on the preconditioner's stream, input and output resident in HBM.
+int fem_bench_precond(fem_pc M, int fp32, int reps, double* avg_ms) {
+    return guard([&] {
+        auto& P = *M->pc;
+        hipStream_t s = M->stream;
+        const uint32_t n = P.size();
+        if (fp32 && !P.has_fp32()) P.ensure_fp32(s);
+        const size_t es = fp32 ? 4 : 8;
+        DevBuf<char> r((size_t)n * es), z((size_t)n * es);
+        if (fp32) vec_fill<float>((float*)r.get(), 1.0f, n, s);
+        else vec_fill<double>((double*)r.get(), 1.0, n, s);
+        hipEvent_t e0, e1;
+        FEM_HIP(hipEventCreate(&e0));
+        FEM_HIP(hipEventCreate(&e1));
+        auto one = [&] {
+            if (fp32) P.apply32((const float*)r.get(), (float*)z.get(), s);
+            else P.apply((const double*)r.get(), (double*)z.get(), s);
+        };
+        one();
+        FEM_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < reps; ++i) one();
+        FEM_HIP(hipEventRecord(e1, s));
+        FEM_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = reps > 0 ? ms / reps : 0.0;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    });
+}
+
 // Same SpMV, each launch timed alone after reading 1 GiB of other data has
 // evicted the L2 and the 256 MB Infinity Cache (MALL) with clean lines, so
 // the matrix streams from HBM and no write-back is charged to the SpMV. All

@@ -11,13 +11,18 @@
 // arithmetic (left-looking two-pointer merges for IC0; the "upper-only" IKJ
 // update for ILU0), so the factors are bit-identical to the reference's.
 //
-// Apply: each sweep stores its rows in level order as SELL-64 (groups of 64
-// consecutive level-ordered rows, entries column-major within a group) so
-// every load of the sweep is coalesced; each row keeps the reference's
-// summation order (ascending column for the forward sweep and for U,
-// ascending row for the L^T sweep of IC0). Levels with many rows get one
-// multi-block launch; runs of consecutive small levels are processed by a
-// single workgroup with a barrier between levels.
+// Apply: a sweep whose rows have at most 16 entries (every stencil matrix
+// here) is one launch of the block sweep (k_sweep_blk): the rows, in
+// processing order, are cut into blocks of kBlkRows; a workgroup takes a
+// block, orders its rows by level, keeps the block's results in LDS and
+// reads rows of earlier blocks through a sentinel-initialized array, so no
+// launch, grid barrier or L2 round trip separates the levels inside a
+// block. Sweeps with longer rows use the level schedule: rows in level order
+// as SELL-64 (groups of 64 level-ordered rows, entries column-major within a
+// group), runs of small levels in one workgroup with an LDS ring (chains),
+// runs of wide levels in one sync-free launch. Every path keeps the
+// reference's per-row summation order (ascending column for the forward
+// sweep and for U, ascending row for the L^T sweep of IC0).
 #include <algorithm>
 #include <cmath>
 #include <numeric>
@@ -37,17 +42,54 @@ constexpr uint32_t kRing = 4096;         // LDS ring of the most recent sweep re
 // (128 KB of float64)
 constexpr uint32_t kRingW = 16384;
 constexpr int wide_r(uint32_t kc) { return kc <= 2 ? 4 : kc <= 4 ? 2 : 0; }
+// block sweeps: rows per block (results of a block stay in LDS) and the
+// widest row (entries) the block kernel holds in registers
+constexpr uint32_t kBlkRows = 8176;
+constexpr uint32_t kBlkMaxEntries = 16;
+constexpr uint32_t kExtDep = 0x80000000u;  // dependency on a row of an earlier block
+constexpr uint16_t kExtUse = 0x8000;       // entry count flag: a later block reads this row
+constexpr uint32_t kXLead = 1;             // block sweep: steps between the first external reading and use
+constexpr int kBlkWaves = 2;               // block sweep: wavefronts per block, taking steps in turn
 constexpr uint32_t wide_rows(uint32_t kc) { return (uint32_t)wide_r(kc) * 1024u; }
 
 // ---------------------------------------------------------------------------
 // Level schedule + SELL-64 sweep layout
 // ---------------------------------------------------------------------------
+// A row whose input has not arrived after this long (seconds, on the
+// device's constant wall clock) gives up and stores NaN: the solve then
+// reports a breakdown instead of the kernel never ending. Rows only wait on
+// rows of lower tickets, so this is a safety net.
+constexpr double kSfWaitSeconds = 2.0;
+
+// (queried at set-up, outside any stream capture)
+static int sf_cus() {
+    static int c = [] {
+        int dev = 0, cu = 0;
+        FEM_HIP(hipGetDevice(&dev));
+        FEM_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+        return cu;
+    }();
+    return c;
+}
+
+static long long sf_wait_ticks() {
+    static long long ticks = [] {
+        int dev = 0, khz = 0;
+        FEM_HIP(hipGetDevice(&dev));
+        FEM_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        return (long long)(kSfWaitSeconds * 1e3 * (khz > 0 ? khz : 100000));
+    }();
+    return ticks;
+}
+
 struct Segment {
     uint32_t s0, s1;       // slot range
     uint32_t lvl0, lvl1;   // level range (chain when lvl1 - lvl0 > 1)
     uint32_t kc = 0;       // chain: entries per slot in the dense layout
     uint64_t cbase = 0;    // chain: offset of slot s0 in the dense arrays
     bool wide = false;     // chain of levels wider than kChainRows (wide-chain kernel)
+    bool sf = false;       // run of levels swept by one sync-free launch
+    uint32_t tick = 0;     // sync-free: index of its ticket pair in Sweep::tick
 };
 
 struct Sweep {
@@ -69,12 +111,32 @@ struct Sweep {
     DevBuf<double> cval, rslot;
     DevBuf<float> cval32, rslot32;
     uint64_t cent = 0;
+    // sync-free segments: results in slot order (a sentinel until written)
+    // and per segment a (ticket, finished) counter pair
+    DevBuf<double> ys;
+    DevBuf<float> ys32;
+    DevBuf<uint32_t> tick;
+    uint32_t nsf = 0;
+    // block sweep (used when every row has at most kBlkMaxEntries entries):
+    // the rows, in processing order, cut into blocks of kBlkRows; block b
+    // holds global slots [b kBlkRows, ...), ordered by level within the
+    // block. Per global slot: its row, entry count, the diagonal's SELL slot
+    // and bkc entries (dependency: local slot inside the block, or
+    // kExtDep | global slot of a row of an earlier block; values and their
+    // SELL index). bstp: per block (from bsptr[b]) the first local slot of
+    // each step (a level, or 64 rows of a wider level).
+    bool blk = false;
+    uint32_t bkc = 0, nblk = 0;
+    DevBuf<uint32_t> brow, bdep, bmap, bdmap, bsptr;
+    DevBuf<uint16_t> blen, bstp;
+    DevBuf<double> bval, bdiag;
+    DevBuf<float> bval32, bdiag32;
 
     // rows: dependency lists per row (entries in summation order), as
     // (col, position-in-factor-values). lev: level per row.
     void build(uint32_t n_, const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cols,
                const std::vector<uint32_t>& pos, const std::vector<uint32_t>& lev, const std::vector<uint32_t>* diagpos,
-               hipStream_t s) {
+               bool forward, hipStream_t s) {
         n = n_;
         nlev = n ? 1 + *std::max_element(lev.begin(), lev.end()) : 0;
         h_lvl_ptr.assign(nlev + 1, 0);
@@ -110,8 +172,37 @@ struct Sweep {
                 hm[base + e * 64] = pos[rp[i] + e];
             }
         }
-        // segments: chains of consecutive small levels, big levels alone
         segs.clear();
+        uint32_t kcmax = 1;
+        for (uint32_t q = 0; q < n; ++q) kcmax = std::max<uint32_t>(kcmax, ln[q]);
+        if (kcmax <= kBlkMaxEntries) {
+            // the block sweep covers this sweep: no level-schedule segments
+            build_blocks(rp, cols, lev, g, slot_of, forward, diagpos != nullptr, s);
+        } else {
+            build_segments(rp, cols, order, slot_of, g, ln, hd, kcmax, s);
+        }
+        lvl_ptr.upload(h_lvl_ptr, s);
+        slot_row.upload(order, s);
+        gptr.upload(g, s);
+        col.upload(hc, s);
+        dslot.upload(hd, s);
+        vmap.upload(hm, s);
+        len.upload(ln, s);
+        val.alloc(std::max<uint64_t>(nent, 1));
+        if (diagpos) {
+            std::vector<uint32_t> dm(n);
+            for (uint32_t s2 = 0; s2 < n; ++s2) dm[s2] = (*diagpos)[order[s2]];
+            dmap.upload(dm, s);
+            diag.alloc(n);
+        }
+    }
+
+    // Level-schedule segments (sweeps with rows longer than the block sweep
+    // holds): chains of consecutive small levels, big levels alone.
+    void build_segments(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cols,
+                        const std::vector<uint32_t>& order, const std::vector<uint32_t>& slot_of,
+                        const std::vector<uint32_t>& g, const std::vector<uint16_t>& ln,
+                        const std::vector<uint32_t>& hd, uint32_t kcmax, hipStream_t s) {
         // a chain holds only levels whose rows depend on slots at most
         // `ring` behind the level's end (the LDS ring window; the chain
         // kernel preloads the ring with the slots before the chain), so every
@@ -127,8 +218,6 @@ struct Sweep {
         };
         auto width = [&](uint32_t lev) { return h_lvl_ptr[lev + 1] - h_lvl_ptr[lev]; };
         // the wide chains' row limit follows the longest row of the sweep
-        uint32_t kcmax = 1;
-        for (uint32_t q = 0; q < n; ++q) kcmax = std::max<uint32_t>(kcmax, ln[q]);
         const uint32_t kWideRows = std::max(kChainRows, wide_rows(kcmax <= 2 ? 2u : kcmax <= 4 ? 4u : 8u));
         uint32_t l = 0;
         while (l < nlev) {
@@ -174,6 +263,34 @@ struct Sweep {
                 }
             }
         }
+        // every run of segments without a dense chain copy (levels too wide
+        // for one workgroup, and chains of rows longer than 8 entries)
+        // becomes one sync-free segment
+        {
+            std::vector<Segment> merged;
+            nsf = 0;
+            for (const Segment& sg : segs) {
+                if (sg.lvl1 - sg.lvl0 > 1 && sg.kc != 0) {
+                    merged.push_back(sg);
+                } else if (!merged.empty() && merged.back().sf) {
+                    merged.back().s1 = sg.s1;
+                    merged.back().lvl1 = sg.lvl1;
+                } else {
+                    Segment m{sg.s0, sg.s1, sg.lvl0, sg.lvl1};
+                    m.sf = true;
+                    m.tick = nsf++;
+                    merged.push_back(m);
+                }
+            }
+            segs.swap(merged);
+            if (nsf) {
+                tick.alloc(2 * nsf);
+                FEM_HIP(hipMemsetAsync(tick.get(), 0, 2 * nsf * sizeof(uint32_t), s));
+                ys.alloc(n);
+                sf_wait_ticks();
+                sf_cus();
+            }
+        }
         cent = dd.size();
         if (cent) {
             sink.alloc(1);
@@ -183,20 +300,83 @@ struct Sweep {
             cval.alloc(cent);
             rslot.alloc(n);
         }
-        lvl_ptr.upload(h_lvl_ptr, s);
-        slot_row.upload(order, s);
-        gptr.upload(g, s);
-        col.upload(hc, s);
-        dslot.upload(hd, s);
-        vmap.upload(hm, s);
-        len.upload(ln, s);
-        val.alloc(std::max<uint64_t>(nent, 1));
-        if (diagpos) {
-            std::vector<uint32_t> dm(n);
-            for (uint32_t s2 = 0; s2 < n; ++s2) dm[s2] = (*diagpos)[order[s2]];
-            dmap.upload(dm, s);
-            diag.alloc(n);
+    }
+
+    // Block layout (see the fields). Within a block, rows are ordered by
+    // their level in the whole sweep, so a block runs about one step behind
+    // the blocks it reads from wherever the block boundaries fall; a row's
+    // entries keep their summation order.
+    void build_blocks(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cols,
+                      const std::vector<uint32_t>& lev, const std::vector<uint32_t>& sell_g,
+                      const std::vector<uint32_t>& slot_of, bool forward, bool has_diag, hipStream_t s) {
+        blk = false;
+        uint32_t kmax = 0;
+        for (uint32_t i = 0; i < n; ++i) kmax = std::max(kmax, rp[i + 1] - rp[i]);
+        if (!n || kmax > kBlkMaxEntries) return;
+        bkc = kmax <= 4 ? 4u : kmax <= 8 ? 8u : 16u;
+        nblk = (n + kBlkRows - 1) / kBlkRows;
+        auto row_at = [&](uint32_t q) { return forward ? q : n - 1 - q; };
+        auto pos_of = [&](uint32_t i) { return forward ? i : n - 1 - i; };
+        std::vector<uint32_t> hrow(n), hdep((size_t)n * bkc, 0), hmap((size_t)n * bkc, UINT32_MAX), hdm(n),
+            gslot(n, UINT32_MAX), sp(nblk + 1, 0), cnt;
+        std::vector<uint16_t> hlen(n, 0), stp;
+        for (uint32_t b = 0; b < nblk; ++b) {
+            const uint32_t q0 = b * kBlkRows, q1 = std::min(n, q0 + kBlkRows);
+            for (uint32_t q = q0; q < q1; ++q) {
+                const uint32_t i = row_at(q);
+                for (uint32_t t = rp[i]; t < rp[i + 1]; ++t)
+                    if (pos_of(cols[t]) >= q) fail(FEM_ERR_RUNTIME, "triangular sweep: a row depends on a later row");
+            }
+            // rows by level (counting sort, stable in processing order); a
+            // step is a level's rows in this block, 64 at a time
+            uint32_t lo = UINT32_MAX, hi = 0;
+            for (uint32_t q = q0; q < q1; ++q) {
+                lo = std::min(lo, lev[row_at(q)]);
+                hi = std::max(hi, lev[row_at(q)]);
+            }
+            cnt.assign(hi - lo + 2, 0);
+            for (uint32_t q = q0; q < q1; ++q) cnt[lev[row_at(q)] - lo + 1]++;
+            for (uint32_t l = 0; l + 1 < cnt.size(); ++l) cnt[l + 1] += cnt[l];
+            for (uint32_t l = 0; l + 1 < cnt.size(); ++l)
+                for (uint32_t c = cnt[l]; c < cnt[l + 1]; c += 64) stp.push_back((uint16_t)c);
+            for (uint32_t q = q0; q < q1; ++q) gslot[row_at(q)] = q0 + cnt[lev[row_at(q)] - lo]++;
+            sp[b + 1] = (uint32_t)stp.size();
+            for (uint32_t q = q0; q < q1; ++q) {
+                const uint32_t i = row_at(q), gs = gslot[i], sl = slot_of[i];
+                const uint32_t L = rp[i + 1] - rp[i];
+                hrow[gs] = i;
+                hlen[gs] = (uint16_t)(L | (hlen[gs] & kExtUse));
+                hdm[gs] = sl;
+                const uint32_t sbase = sell_g[sl >> 6] + (sl & 63);
+                for (uint32_t e = 0; e < L; ++e) {
+                    const uint32_t gj = gslot[cols[rp[i] + e]];
+                    hdep[(size_t)gs * bkc + e] = gj >= q0 ? gj - q0 : (kExtDep | gj);
+                    if (gj < q0) hlen[gj] |= kExtUse;  // a row of an earlier block (already laid out)
+                    hmap[(size_t)gs * bkc + e] = sbase + 64u * e;
+                }
+            }
         }
+        brow.upload(hrow, s);
+        blen.upload(hlen, s);
+        bdep.upload(hdep, s);
+        bmap.upload(hmap, s);
+        bsptr.upload(sp, s);
+        bstp.upload(stp, s);
+        bval.alloc((size_t)n * bkc);
+        if (has_diag) {
+            bdmap.upload(hdm, s);
+            bdiag.alloc(n);
+        }
+        if (!tick.get()) tick.alloc(2);  // the block sweep's counters (first pair)
+        if (nsf == 0) FEM_HIP(hipMemsetAsync(tick.get(), 0, 2 * sizeof(uint32_t), s));
+        ys.alloc(n + 2);  // slot n: target of the stores of lanes without a row; n + 1: read dummy
+        if (!sink.get()) {
+            sink.alloc(1);
+            sink32.alloc(1);
+        }
+        sf_wait_ticks();
+        sf_cus();
+        blk = true;
     }
 };
 
@@ -238,6 +418,27 @@ __global__ void k_scatter_slots(const uint32_t* __restrict__ slot_row, uint32_t 
 
 // copy the sweep's values (and float32 copy when present) into the chain layout
 static void densify(Sweep& S, hipStream_t s) {
+    if ((S.nsf || S.blk) && S.val32.get() && !S.ys32.get()) S.ys32.alloc(S.n + 2);
+    if (S.blk) {
+        const uint64_t ne = (uint64_t)S.n * S.bkc;
+        const unsigned G = (unsigned)((ne + 255) / 256), Gd = (S.n + 255) / 256;
+        hipLaunchKernelGGL((k_gather_dense<double>), dim3(G), dim3(256), 0, s, S.val.get(), S.bmap.get(), S.bval.get(),
+                           ne);
+        if (S.bdiag.get())
+            hipLaunchKernelGGL((k_gather_dense<double>), dim3(Gd), dim3(256), 0, s, S.diag.get(), S.bdmap.get(),
+                               S.bdiag.get(), (uint64_t)S.n);
+        if (S.val32.get()) {
+            S.bval32.alloc(ne);
+            hipLaunchKernelGGL((k_gather_dense<float>), dim3(G), dim3(256), 0, s, S.val32.get(), S.bmap.get(),
+                               S.bval32.get(), ne);
+            if (S.bdiag.get()) {
+                S.bdiag32.alloc(S.n);
+                hipLaunchKernelGGL((k_gather_dense<float>), dim3(Gd), dim3(256), 0, s, S.diag32.get(), S.bdmap.get(),
+                                   S.bdiag32.get(), (uint64_t)S.n);
+            }
+        }
+        FEM_HIP(hipGetLastError());
+    }
     if (!S.cent) return;
     const unsigned G = (unsigned)((S.cent + 255) / 256);
     hipLaunchKernelGGL((k_gather_dense<double>), dim3(G), dim3(256), 0, s, S.val.get(), S.cmap.get(), S.cval.get(), S.cent);
@@ -479,6 +680,353 @@ __global__ __launch_bounds__(kChainThreads) void k_sweep_chain_wide(uint32_t l0,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Sync-free sweep of a run of wide levels in one launch (level schedule)
+// ---------------------------------------------------------------------------
+// Every slot's result is also stored, in slot order, into ys, which holds a
+// signalling-NaN sentinel until then: a row waits for exactly the rows it
+// reads (their ys entries leave the sentinel) instead of for whole levels,
+// so rows of many levels are in flight at once and no launch or grid
+// barrier separates levels. Arithmetic never produces a signalling NaN, and
+// a result that equals the sentinel bit for bit (a right-hand side copied
+// through) is stored as the quiet NaN instead, so a finished row is never
+// mistaken for a pending one.
+//
+// Workgroups take slot blocks in ticket order (an atomic counter), so every
+// block a workgroup waits on is running or done: the lowest unfinished slot
+// always has its inputs and the sweep makes progress. Within a wavefront a
+// row may read a row of a lower lane; each lane therefore polls, computes
+// and stores inside one loop, and a lane that is ready finishes while the
+// others keep polling. ys is written and polled with relaxed device-scope
+// atomics (the value is its own flag, so no fence is needed); rows of
+// earlier launches are read from y. Per-row arithmetic is the level
+// kernel's: rhs first, then the entries in ascending order.
+template <typename T>
+struct SfBits;
+template <>
+struct SfBits<double> {
+    using U = unsigned long long;
+    static constexpr U kSent = 0x7FF4DEADBEEF0001ull;  // signalling NaN
+    static constexpr U kQuiet = 0x7FF8000000000000ull;
+};
+template <>
+struct SfBits<float> {
+    using U = unsigned int;
+    static constexpr U kSent = 0x7F80DEADu;  // signalling NaN
+    static constexpr U kQuiet = 0x7FC00000u;
+};
+
+constexpr int kSfThreads = 256;
+
+template <typename T>
+__global__ void k_sf_fill(T* ys, uint32_t s0, uint32_t s1, Gate g) {
+    if (g.closed()) return;
+    using U = typename SfBits<T>::U;
+    const uint32_t s = s0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < s1) reinterpret_cast<U*>(ys)[s] = SfBits<T>::kSent;
+}
+
+template <typename T, bool DIV>
+__global__ __launch_bounds__(kSfThreads) void k_sweep_sf(uint32_t s0, uint32_t s1, const uint32_t* __restrict__ slot_row,
+                                                         const uint16_t* __restrict__ len,
+                                                         const uint32_t* __restrict__ gptr,
+                                                         const uint32_t* __restrict__ col,
+                                                         const uint32_t* __restrict__ dslot,
+                                                         const T* __restrict__ val, const T* __restrict__ diag,
+                                                         const T* __restrict__ rhs, T* y, T* ys, uint32_t* tick,
+                                                         long long wait_ticks, Gate g) {
+    using B = SfBits<T>;
+    using U = typename B::U;
+    if (g.closed()) return;
+    __shared__ uint32_t blk_sh;
+    const uint32_t nblk = (s1 - s0 + kSfThreads - 1) / kSfThreads;
+    U* yb = reinterpret_cast<U*>(ys);
+    bool timed_out = false;
+    for (;;) {
+        __syncthreads();  // blk_sh is rewritten below
+        if (threadIdx.x == 0) blk_sh = atomicAdd(tick, 1u);
+        __syncthreads();
+        const uint32_t blk = blk_sh;
+        if (blk >= nblk) break;  // uniform over the workgroup
+        const long long t0 = wall_clock64();  // the waits of this slot block
+        const uint32_t first = s0 + blk * kSfThreads;
+        const uint32_t s = first + threadIdx.x;
+        const bool on = s < s1;
+        uint32_t i = 0, base = 0, L = 0;
+        T acc = (T)0, d = (T)1;
+        if (on) {
+            i = slot_row[s];
+            base = gptr[s >> 6] + (s & 63);
+            L = len[s];
+            acc = rhs[i];
+            if constexpr (DIV) d = diag[s];
+        }
+        // rows of at most 8 entries keep the whole row in registers
+        const bool reg = L <= 8;
+        uint32_t ds[8];
+        T vl[8], yl[8];
+        uint32_t pend = 0;
+        if (on && reg) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+                const uint32_t k = base + 64u * ((uint32_t)e < L ? (uint32_t)e : 0u);
+                ds[e] = L ? dslot[k] : 0u;
+                vl[e] = L ? val[k] : (T)0;
+                yl[e] = (T)0;
+                if ((uint32_t)e < L) {
+                    if (ds[e] < s0) yl[e] = y[col[k]];
+                    else pend |= 1u << e;
+                }
+            }
+        }
+        bool done = !on;
+        uint32_t e = 0;
+        while (!done) {
+            bool ready;
+            if (reg) {
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    if ((pend >> q) & 1u) {
+                        const U b = __hip_atomic_load(yb + ds[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                        if (b != B::kSent) {
+                            yl[q] = __builtin_bit_cast(T, b);
+                            pend &= ~(1u << q);
+                        }
+                    }
+                ready = pend == 0;
+                if (ready) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if ((uint32_t)q < L) acc -= vl[q] * yl[q];
+                }
+            } else {
+                // longer rows: entries in order, each as soon as its input is in
+                while (e < L) {
+                    const uint32_t k = base + 64u * e;
+                    const uint32_t dk = dslot[k];
+                    const U b = dk < s0 ? __builtin_bit_cast(U, y[col[k]])
+                                        : __hip_atomic_load(yb + dk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    if (b == B::kSent) break;
+                    acc -= val[k] * __builtin_bit_cast(T, b);
+                    ++e;
+                }
+                ready = e == L;
+            }
+            if (!ready && (timed_out || wall_clock64() - t0 >= wait_ticks)) {
+                timed_out = true;
+                acc = __builtin_bit_cast(T, B::kQuiet);
+                ready = true;
+            }
+            if (ready) {
+                if constexpr (DIV) acc = acc / d;
+                U ab = __builtin_bit_cast(U, acc);
+                if (ab == B::kSent) ab = B::kQuiet;
+                __hip_atomic_store(yb + s, ab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                y[i] = __builtin_bit_cast(T, ab);
+                done = true;
+            } else {
+                __builtin_amdgcn_s_sleep(1);
+            }
+        }
+    }
+    // the workgroup that finishes last resets the counters for the next sweep
+    if (threadIdx.x == 0 && atomicAdd(tick + 1, 1u) == gridDim.x - 1) {
+        __hip_atomic_store(tick, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(tick + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block sweep: one workgroup per block of kBlkRows consecutive rows
+// ---------------------------------------------------------------------------
+// A block's rows, ordered by level, are processed in steps (the block's rows
+// of one level, 64 at a time). W wavefronts take the steps in turn (step q
+// by wavefront q mod W) and hand over through a step counter in LDS, so a
+// step costs an LDS round trip while each wavefront has W steps of time for
+// the loads of its next one. Results of the block stay in LDS, where later
+// steps read them. A row of an earlier block is read from ys (slot order, a
+// signalling-NaN sentinel until the row is done): rows a later block reads
+// are published there with relaxed device-scope stores (the value is its own
+// flag), first read with a plain load kXLead steps before use (it returns
+// the sentinel or the final value, the only two values the slot holds) and,
+// if still pending, polled with device-scope loads. Blocks are taken in
+// ticket order by persistent workgroups, so every block a workgroup waits
+// on is running or done and the sweep always progresses; a wait that runs
+// out of time (wait_ticks) stores NaN, so the solve reports a breakdown
+// instead of hanging. Per-row arithmetic is the level kernel's: rhs first,
+// then the entries in ascending order, then the division.
+template <typename T, bool DIV, int KC, int D, int W>
+__global__ __launch_bounds__(64 * W) void k_sweep_blk(uint32_t n, uint32_t nblk, const uint32_t* __restrict__ sptr,
+                                                       const uint16_t* __restrict__ stp,
+                                                       const uint32_t* __restrict__ brow,
+                                                       const uint16_t* __restrict__ blen,
+                                                       const uint32_t* __restrict__ bdep, const T* __restrict__ bval,
+                                                       const T* __restrict__ bdiag, const T* __restrict__ rhs,
+                                                       T* __restrict__ y, T* ys, uint32_t* tick, long long wait_ticks,
+                                                       Gate g) {
+    static_assert(D >= 2, "stage B reads the row fetched one own step earlier");
+    using B = SfBits<T>;
+    using U = typename B::U;
+    if (g.closed()) return;
+    // results and step starts of one block, the steps done and the block
+    // index: two workgroups fit one CU's 160 KB
+    __shared__ T res[kBlkRows];
+    __shared__ uint16_t st[kBlkRows];
+    __shared__ uint32_t done_sh, blk_sh;
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = threadIdx.x >> 6;
+    U* yb = reinterpret_cast<U*>(ys);
+    const U* yplain = yb;
+    bool timed_out = false;
+    struct Row {
+        uint32_t on, L, row, gs;
+        T d, r;
+        uint32_t dep[KC];
+        T v[KC];
+        U x[KC];
+    };
+    for (;;) {
+        if (threadIdx.x == 0) {
+            blk_sh = atomicAdd(tick, 1u);
+            done_sh = 0;
+        }
+        __syncthreads();
+        const uint32_t b = blk_sh;
+        if (b >= nblk) break;  // uniform over the workgroup
+        const uint32_t g0 = b * kBlkRows;
+        const uint32_t cnt = min(kBlkRows, n - g0);
+        const uint32_t s0 = sptr[b], ns = sptr[b + 1] - s0;
+        for (uint32_t t = threadIdx.x; t < ns; t += 64 * W) st[t] = stp[s0 + t];
+        __syncthreads();
+        // Wavefront w takes steps w, w + W, w + 2W, ... ("own steps"). Stage
+        // A (D own steps ahead) loads what the slot addresses; stage B (one
+        // own step ahead) the right-hand side, which stage A's row index
+        // addresses; stage X the first readings of the external inputs,
+        // kXLead steps before use. Loads are issued by every lane (lanes
+        // without a row use slot g0) and in the order they are consumed, so
+        // the compiler's waits (which count loads in issue order) never drain
+        // the loads of later steps.
+        auto stage_a = [&](uint32_t q, Row& R) {
+            const uint32_t qc = q < ns ? q : ns - 1;
+            const uint32_t ls = (uint32_t)st[qc] + lane;
+            const uint32_t end = qc + 1 < ns ? (uint32_t)st[qc + 1] : cnt;
+            R.on = (q < ns && ls < end) ? 1u : 0u;
+            const uint32_t gs = g0 + (R.on ? ls : 0u);
+            R.gs = gs;
+            R.row = brow[gs];
+            R.L = blen[gs];
+            R.d = DIV ? bdiag[gs] : (T)1;
+            const uint64_t o = (uint64_t)gs * KC;
+#pragma unroll
+            for (int e = 0; e < KC; ++e) {
+                R.dep[e] = bdep[o + e];
+                R.v[e] = bval[o + e];
+            }
+        };
+        auto stage_b = [&](Row& R) { R.r = rhs[R.row]; };
+        auto stage_x = [&](Row& R) {
+#pragma unroll
+            for (int e = 0; e < KC; ++e) {
+                const uint32_t dp = R.dep[e];
+                // a plain load: it returns the sentinel or the final value
+                // (the only two values the slot holds during the sweep);
+                // internal inputs read the never-written slot n + 1
+                R.x[e] = yplain[(dp & kExtDep) ? (dp & ~kExtDep) : n + 1];
+            }
+        };
+        const uint32_t nown = ns > wave ? (ns - wave + W - 1) / W : 0u;
+        Row buf[D];
+#pragma unroll
+        for (int k = 0; k < D; ++k) stage_a(wave + W * (uint32_t)k, buf[k]);
+        stage_b(buf[0]);
+        for (uint32_t jb = 0; jb < nown; jb += D) {
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                const uint32_t j = jb + (uint32_t)k;
+                if (j >= nown) break;  // uniform over the wavefront
+                const uint32_t q = wave + W * j;
+                Row& cur = buf[k];
+                const uint32_t L = cur.L & ~(uint32_t)kExtUse;
+                // first readings of the external inputs kXLead steps before
+                // this one (late enough to find most of them published, early
+                // enough for the load to return), then wait until the steps
+                // before q are done (their results are in res)
+                while (__hip_atomic_load(&done_sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) + kXLead < q)
+                    __builtin_amdgcn_s_sleep(1);
+                stage_x(cur);
+                while (__hip_atomic_load(&done_sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != q)
+                    __builtin_amdgcn_s_sleep(1);
+                // external inputs still pending at stage B: poll them
+                bool late = false;
+#pragma unroll
+                for (int e = 0; e < KC; ++e)
+                    late |= cur.on && (uint32_t)e < L && (cur.dep[e] & kExtDep) && cur.x[e] == B::kSent;
+                if (__any(late)) {
+                    const long long w0 = wall_clock64();
+#pragma unroll
+                    for (int e = 0; e < KC; ++e) {
+                        if (cur.on && (uint32_t)e < L && (cur.dep[e] & kExtDep)) {
+                            while (cur.x[e] == B::kSent) {
+                                if (timed_out || wall_clock64() - w0 >= wait_ticks) {
+                                    timed_out = true;
+                                    cur.x[e] = B::kQuiet;
+                                    break;
+                                }
+                                cur.x[e] = __hip_atomic_load(yb + (cur.dep[e] & ~kExtDep), __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT);
+                            }
+                        }
+                    }
+                }
+                T acc = cur.r;
+#pragma unroll
+                for (int e = 0; e < KC; ++e) {
+                    const uint32_t dp = cur.dep[e];
+                    const T in = res[(dp & kExtDep) ? 0u : dp];
+                    const T yv = (dp & kExtDep) ? __builtin_bit_cast(T, cur.x[e]) : in;
+                    const T t = acc - cur.v[e] * yv;
+                    acc = (uint32_t)e < L ? t : acc;
+                }
+                if constexpr (DIV) acc = acc / cur.d;
+                U ab = __builtin_bit_cast(U, acc);
+                if (ab == B::kSent) ab = B::kQuiet;
+                const T out = __builtin_bit_cast(T, ab);
+                if (cur.on) res[cur.gs - g0] = out;
+                // the step's LDS results, then the hand-off to the next step
+                __asm__ volatile("" ::: "memory");
+                __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
+                __builtin_amdgcn_wave_barrier();
+                __asm__ volatile("" ::: "memory");
+                if (lane == 0) __hip_atomic_store(&done_sh, q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                if (cur.on) {
+                    y[cur.row] = out;
+                    // only rows a later block reads are published (write-through)
+                    if (cur.L & kExtUse)
+                        __hip_atomic_store(yb + cur.gs, ab, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                // the next own step's stage B after this step's polls (a poll
+                // waits for every load issued before it), then a stage A
+                stage_b(buf[(k + 1) % D]);
+                stage_a(wave + W * (j + D), cur);
+            }
+        }
+        __syncthreads();  // st and res are rewritten by the next block
+    }
+    // the workgroup that finishes last resets the counters for the next sweep
+    if (threadIdx.x == 0 && atomicAdd(tick + 1, 1u) == gridDim.x - 1) {
+        __hip_atomic_store(tick, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(tick + 1, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+template <typename T, bool DIV, int KC, int D>
+static void blk_launch(const Sweep& S, const T* val, const T* diag, const T* rhs, T* y, T* ys, hipStream_t s, Gate g) {
+    const uint32_t grid = std::min<uint32_t>(S.nblk, (uint32_t)(2 * sf_cus()));
+    hipLaunchKernelGGL((k_sweep_blk<T, DIV, KC, D, kBlkWaves>), dim3(grid), dim3(64 * kBlkWaves), 0, s, S.n, S.nblk,
+                       S.bsptr.get(), S.bstp.get(), S.brow.get(), S.blen.get(), S.bdep.get(), val, diag, rhs, y, ys,
+                       S.tick.get(), sf_wait_ticks(), g);
+}
+
 template <typename T, bool DIV, int KC>
 static void chain_launch(const Sweep& S, const Segment& sg, const T* cval, const T* diag, T* rslot, T* y, hipStream_t s,
                          Gate g) {
@@ -511,7 +1059,36 @@ static void run_sweep(const Sweep& S, const T* val, const T* diag, const T* rhs,
         cval = S.cval32.get();
         rslot = S.rslot32.get();
     }
+    T* ys;
+    if constexpr (std::is_same<T, double>::value) ys = S.ys.get();
+    else ys = S.ys32.get();
+    if (S.blk) {
+        const T *bv, *bd;
+        if constexpr (std::is_same<T, double>::value) {
+            bv = S.bval.get();
+            bd = S.bdiag.get();
+        } else {
+            bv = S.bval32.get();
+            bd = S.bdiag32.get();
+        }
+        hipLaunchKernelGGL((k_sf_fill<T>), dim3((S.n + 255) / 256), dim3(256), 0, s, ys, 0u, S.n, g);
+        if (S.bkc == 4) blk_launch<T, DIV, 4, 3>(S, bv, bd, rhs, y, ys, s, g);
+        else if (S.bkc == 8) blk_launch<T, DIV, 8, 3>(S, bv, bd, rhs, y, ys, s, g);
+        else blk_launch<T, DIV, 16, 2>(S, bv, bd, rhs, y, ys, s, g);
+        FEM_HIP(hipGetLastError());
+        return;
+    }
     for (const Segment& sg : S.segs) {
+        if (sg.sf) {
+            const uint32_t cnt = sg.s1 - sg.s0;
+            hipLaunchKernelGGL((k_sf_fill<T>), dim3((cnt + 255) / 256), dim3(256), 0, s, ys, sg.s0, sg.s1, g);
+            // persistent: one workgroup per CU takes slot blocks in ticket order
+            const uint32_t grid = std::min<uint32_t>((cnt + kSfThreads - 1) / kSfThreads, (uint32_t)sf_cus());
+            hipLaunchKernelGGL((k_sweep_sf<T, DIV>), dim3(grid), dim3(kSfThreads), 0, s, sg.s0, sg.s1,
+                               S.slot_row.get(), S.len.get(), S.gptr.get(), S.col.get(), S.dslot.get(), val, diag,
+                               rhs, y, ys, S.tick.get() + 2 * sg.tick, sf_wait_ticks(), g);
+            continue;
+        }
         // a chain with a dense copy: its stride sg.kc equals the kernel width
         if (sg.lvl1 - sg.lvl0 > 1 && (sg.kc == 2 || sg.kc == 4 || sg.kc == 8)) {
             const uint32_t cnt = sg.s1 - sg.s0;
@@ -665,8 +1242,8 @@ struct PcIC0 : Precond {
         }
         const auto lf = levels_of(n, Fp, Fc, true);
         const auto lb = levels_of(n, Bp, Bc, false);
-        fw.build(n, Fp, Fc, Fpos, lf, &dpos, s);
-        bw.build(n, Bp, Bc, Bpos, lb, &dpos, s);
+        fw.build(n, Fp, Fc, Fpos, lf, &dpos, true, s);
+        bw.build(n, Bp, Bc, Bpos, lb, &dpos, false, s);
         levels = (int)fw.nlev;
         // numeric factorization on the device, level by level
         DevBuf<uint32_t> dLp, dLc, dmap, ddpos;
@@ -789,8 +1366,8 @@ struct PcILU0 : Precond {
         std::iota(Lpos.begin(), Lpos.end(), 0u);
         const auto lf = levels_of(n, Lp, Lc, true);
         const auto lb = levels_of(n, Bp, Bc, false);
-        fw.build(n, Lp, Lc, Lpos, lf, nullptr, s);
-        bw.build(n, Bp, Bc, Bpos, lb, &dU, s);
+        fw.build(n, Lp, Lc, Lpos, lf, nullptr, true, s);
+        bw.build(n, Bp, Bc, Bpos, lb, &dU, false, s);
         levels = (int)fw.nlev;
         DevBuf<uint32_t> dLp, dLc, dUp, dUc, ddU, dLm, dUm;
         DevBuf<double> Lv(std::max<size_t>(Lc.size(), 1)), Uv(std::max<size_t>(Uc.size(), 1));

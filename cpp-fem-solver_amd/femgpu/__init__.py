@@ -125,6 +125,7 @@ _SIGS = {
     "fem_precond_destroy": (C.c_int, [P_vp]),
     "fem_precond_info": (C.c_int, [P_vp, C.POINTER(C.c_int), P_u32, C.POINTER(C.c_uint64), C.c_int, P_f64]),
     "fem_precond_bytes": (C.c_int, [P_vp, C.c_int, P_f64]),
+    "fem_bench_precond": (C.c_int, [P_vp, C.c_int, C.c_int, P_f64]),
     "fem_amg_level_export": (C.c_int, [P_vp, C.c_int, C.c_int, P_u32, P_u32, P_u32, P_f64]),
     "fem_amg_split": (C.c_int, [P_vp, C.c_int, C.POINTER(C.c_uint8)]),
     "fem_solve": (C.c_int, [P_vp, P_vp, P_f64, P_f64, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -410,6 +411,13 @@ class Preconditioner:
         st = np.empty(n, np.uint8)
         _check(lib().fem_amg_split(self._h, level, st.ctypes.data_as(C.POINTER(C.c_uint8))))
         return st.astype(np.int8)
+
+    def bench_apply(self, fp32=False, reps=20) -> float:
+        """Average device time (ms) of back-to-back applications
+        (fem_bench_precond)."""
+        ms = C.c_double()
+        _check(lib().fem_bench_precond(self._h, int(fp32), reps, C.byref(ms)))
+        return ms.value
 
     def apply_bytes(self, fp32=False):
         """Algorithmic bytes of one application (fem_precond_bytes); 0 where

@@ -415,6 +415,10 @@ int fem_bench_spmv(fem_csr A, int fp32, int reps, double* avg_ms);
 /* The same, each launch timed alone after a 1 GiB write has evicted the L2
  * and the 256 MB Infinity Cache, so the matrix streams from HBM. */
 int fem_bench_spmv_cold(fem_csr A, int fp32, int reps, double* avg_ms);
+/* Average device time of `reps` back-to-back applications of M (float32
+ * sweeps when fp32 != 0), timed with HIP events on M's stream, input and
+ * output resident in HBM. */
+int fem_bench_precond(fem_pc M, int fp32, int reps, double* avg_ms);
 
 #ifdef __cplusplus
 }

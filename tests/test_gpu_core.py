@@ -12,7 +12,7 @@ import scipy.sparse as sp
 import femgpu as F
 import oracle_api as O
 from femgpu import gen
-from parity import tol_x
+from parity import cond_poisson, tol_x
 
 pytestmark = pytest.mark.gpu
 
@@ -247,6 +247,90 @@ def test_ic0_apply_bitexact(case):
     z = F.Preconditioner(A, "ic0").apply(r)
     zo = O.Pc(Ao, O.K_IC0).apply(r)
     assert np.array_equal(z, zo)
+
+
+def _stencil27(N):
+    """27-point stencil on N^3 points: diagonal 26, every neighbour -1 (SPD:
+    27 I minus the Kronecker cube of tridiag(1, 1, 1))."""
+    T = sp.diags([1.0, 1.0, 1.0], [-1, 0, 1], shape=(N, N))
+    A = (27.0 * sp.identity(N ** 3) - sp.kron(sp.kron(T, T), T)).tocsr()
+    A.sort_indices()
+    return A
+
+
+def _banded(n, half):
+    """SPD band matrix: diagonal 3 half, every other entry within the band -1
+    (rows of `half` lower entries, more than the block sweep holds)."""
+    offs = list(range(-half, half + 1))
+    A = sp.diags([3.0 * half if o == 0 else -1.0 for o in offs], offs, shape=(n, n)).tocsr()
+    A.sort_indices()
+    return A
+
+
+@pytest.mark.parametrize("case", ["poisson3d_96", "stencil27_40", "banded_20", "banded_20_wide"])
+def test_ic0_apply_sync_free_bitexact(case):
+    """Block sweeps (rows of at most 16 entries) and the level-schedule
+    fallback, bit for bit against the oracle. poisson3d_96: 109 blocks of
+    8,176 rows whose boundaries cut grid planes (7-point rows); stencil27_40:
+    13 entries per row (the widest register layout); banded_20: rows of 20
+    lower entries and one row per level (level schedule: sync-free segment);
+    banded_20_wide: the same band over 20 independent copies (levels of 20
+    rows). Two applications in a row (the second reuses the sentinel buffer
+    and the ticket counters)."""
+    if case == "poisson3d_96":
+        G = gen.poisson3d(96)
+        rp, ci, v, n = G.row_ptr, G.col_idx, G.values, G.n
+    else:
+        if case == "stencil27_40":
+            M = _stencil27(40)
+        elif case == "banded_20":
+            M = _banded(3000, 20)
+        else:
+            M = sp.block_diag([_banded(400, 20)] * 20, format="csr")
+            M.sort_indices()
+        rp, ci, v, n = M.indptr.astype(np.uint32), M.indices.astype(np.uint32), M.data, M.shape[0]
+    A = F.CsrMatrix(n, n, rp, ci, v)
+    Ao = O.Csr(n, n, rp, ci, v)
+    Mg, Mo = F.Preconditioner(A, "ic0"), O.Pc(Ao, O.K_IC0)
+    for seed in (4, 5):
+        r = np.random.default_rng(seed).standard_normal(n)
+        assert np.array_equal(Mg.apply(r), Mo.apply(r)), seed
+
+
+def test_ilu0_apply_sync_free_bitexact():
+    """ILU(0) of a nonsymmetric 3-D 7-point matrix (lower couplings 1.3, upper
+    0.7) on 64^3 points: the unit-lower sweep (no division) and the upper
+    sweep both have levels of up to 3,072 rows, swept sync-free."""
+    G = gen.poisson3d(64)
+    rows = np.repeat(np.arange(G.n), np.diff(G.row_ptr.astype(np.int64)))
+    cols = G.col_idx.astype(np.int64)
+    v = np.where(cols < rows, G.values * 1.3, np.where(cols > rows, G.values * 0.7, G.values))
+    A = F.CsrMatrix(G.n, G.n, G.row_ptr, G.col_idx, v)
+    Ao = O.Csr(G.n, G.n, G.row_ptr, G.col_idx, v)
+    Mg, Mo = F.Preconditioner(A, "ilu0"), O.Pc(Ao, O.K_ILU0)
+    for seed in (6, 7):
+        r = np.random.default_rng(seed).standard_normal(G.n)
+        assert np.array_equal(Mg.apply(r), Mo.apply(r)), seed
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_ic0_sync_free_in_graphs(fp32):
+    """IC(0)-PCG on the 7-point Laplacian on 64^3 points through captured
+    graph chunks and through plain launches: identical iterates (float64),
+    or convergence with float32 sweeps inside float64 refinement; the
+    float64 iteration count and the solution against the oracle's."""
+    G = gen.poisson3d(64)
+    _, b = gen.rhs(G, 3)
+    A = F.CsrMatrix.from_gen(G)
+    M = F.Preconditioner(A, "ic0", fp32=fp32)
+    x1, s1 = F.solve(A, M, b, use_graph=True)
+    x2, s2 = F.solve(A, M, b, use_graph=False)
+    xo, ito = O.solve_named(O.Csr.from_gen(G), 0, "ic0", b)
+    assert s1["true_relres"] < 1.5e-10 and s2["true_relres"] < 1.5e-10
+    if not fp32:
+        assert np.array_equal(x1, x2) and s1["iterations"] == s2["iterations"]
+        assert abs(s1["iterations"] - ito) <= 1
+    assert rel(x1, xo) < tol_x(G, cond=cond_poisson(64, 3))
 
 
 def test_ilu0_apply_bitexact():
