@@ -522,6 +522,11 @@ def main():
     spmv_gbs = B64 / (spmv_ms * 1e-3) / 1e9
     spmv32_gbs = B32 / (spmv32_ms * 1e-3) / 1e9
     spmv_cold_gbs = B64 / (spmv_cold_ms * 1e-3) / 1e9
+    # the device's streaming-read ceiling for the same bytes, warm and cold
+    stream_warm_gbs = B64_moved / (F.bench_stream(B64_moved, False, 20) * 1e-3) / 1e9
+    stream_cold_gbs = B64_moved / (F.bench_stream(B64_moved, True, 10) * 1e-3) / 1e9
+    log(f"[bench] streaming read of {B64_moved / 1e6:.0f} MB: warm {stream_warm_gbs:.0f} GB/s, "
+        f"cold {stream_cold_gbs:.0f} GB/s")
     log(f"[bench] spmv fp64 {spmv_ms * 1e3:.1f} us {spmv_gbs:.0f} GB/s (cold {spmv_cold_ms * 1e3:.1f} us "
         f"{spmv_cold_gbs:.0f} GB/s); fp32 {spmv32_ms * 1e3:.1f} us {spmv32_gbs:.0f} GB/s")
 
@@ -714,7 +719,13 @@ def main():
                      "achieved_moved": round(B64_moved / (spmv_ms * 1e-3) / 1e9, 1),
                      "timing": "200 back-to-back launches, HIP events on the library's stream",
                      "achieved_cold": round(spmv_cold_gbs, 1), "frac_cold": round(spmv_cold_gbs / HBM_PEAK_GBS, 4),
-                     "avg_launch_us_cold": round(spmv_cold_ms * 1e3, 2)},
+                     "avg_launch_us_cold": round(spmv_cold_ms * 1e3, 2),
+                     "stream_read": {"warm_GBs": round(stream_warm_gbs, 1), "cold_GBs": round(stream_cold_gbs, 1),
+                                     "note": "a plain 16-byte-load read of the moved bytes (fem_bench_stream): the "
+                                             "device's practical ceiling; SpMV moved-bytes rate over it is "
+                                             "frac_of_stream"},
+                     "frac_of_stream": round(B64_moved / (spmv_ms * 1e-3) / 1e9 / stream_warm_gbs, 4),
+                     "frac_of_stream_cold": round(B64_moved / (spmv_cold_ms * 1e-3) / 1e9 / stream_cold_gbs, 4)},
         "cpu_baseline": cpu,
     }
     if c4 is not None:

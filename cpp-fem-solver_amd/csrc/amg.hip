@@ -58,8 +58,7 @@ __device__ __forceinline__ void gs_rows(uint32_t t, uint32_t hi, int lane, const
     T s = (T)0;
     if (t < hi)
         for (uint32_t p = gp[t] + lane; p < gp[t + 1]; p += LANES) s += gv[p] * u[gci[p]];
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
+    s = group_sum<LANES>(s);
     if (t < hi && lane == 0) {
         const uint32_t i = rows[t];
         u[i] = u[i] + (w * (f[i] - s)) / gd[t];
@@ -103,7 +102,7 @@ __global__ __launch_bounds__(1024) void k_gs_small(int ncol, const uint32_t* __r
 // f_i * u_i (float64, the new u_i): with f = r on the finest level and the
 // last pass of each colour, the partial sums of r.z.
 template <typename T, int MODE, bool PK, bool DOT = false>
-__global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict__ blk,
+__global__ __launch_bounds__(kBlock) void k_gs_stream(const uint2* __restrict__ blk,
                                                       const uint32_t* __restrict__ rows,
                                                       const uint32_t* __restrict__ gp,
                                                       const uint32_t* __restrict__ gci,
@@ -117,8 +116,8 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint32_t* __restrict
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
     // the block's bounds are read while the gate's flag is in flight (reads
     // only: the gate guards the stores)
-    const uint32_t r0 = blk[bid], r1 = blk[bid + 1];
-    const uint32_t k0 = gp[r0], k1 = gp[r1];
+    const uint2 lo = blk[bid], hi = blk[bid + 1];
+    const uint32_t r0 = lo.x, r1 = hi.x, k0 = lo.y, k1 = hi.y;
     if (g.closed()) return;
     const uint32_t t = threadIdx.x;
     if (MODE == 1 && bid >= zero_from) {
@@ -216,8 +215,7 @@ __global__ __launch_bounds__(256) void k_coarse(const double* __restrict__ C, ui
     const double* row = C + (size_t)i * nc;
     double s = 0.0;
     for (uint32_t j = lane; j < nc; j += 64) s += row[j] * fs[j];
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    s = wave_sum(s);
     if (lane == 0) u[i] = (T)s;
 }
 
@@ -373,7 +371,7 @@ struct PcAmg : Precond {
         return std::is_same<T, double>::value ? D.h_cblk : D.h_cblk32;
     }
     template <typename T>
-    static const uint32_t* gblk(const AmgLevelDev& D) {
+    static const uint2* gblk(const AmgLevelDev& D) {
         return std::is_same<T, double>::value ? D.gblk.get() : D.gblk32.get();
     }
 
@@ -403,7 +401,7 @@ struct PcAmg : Precond {
                 const uint32_t b0 = cblk<T>(D)[c], b1 = cblk<T>(D)[c + 1];
                 if (b0 >= b1) continue;
                 const dim3 grid(b1 - b0), block(kBlock);
-                const uint32_t* blk = gblk<T>(D) + b0;
+                const uint2* blk = gblk<T>(D) + b0;
                 if (part && sw == sweeps - 1) {
                     if (D.gcd.get())
                         hipLaunchKernelGGL((k_gs_stream<T, 0, true, true>), grid, block, 0, s, blk, D.crows.get(),

@@ -55,7 +55,7 @@ struct AmgLevelDev {
     // h_cblk32/gblk32 the same for the float32 sweeps (larger blocks)
     int lanes = 1;
     std::vector<uint32_t> h_cblk, h_cblk32;
-    DevBuf<uint32_t> gblk, gblk32;
+    DevBuf<uint2> gblk, gblk32;
     DevBuf<uint32_t> gp, gci;
     DevBuf<uint16_t> gcd;     // packed gci (empty: not packable)
     // R without the columns of the last colour, whose residual is zero after

@@ -349,7 +349,7 @@ __global__ __launch_bounds__(64) void k_spgemm_sym(const uint32_t* arp, const ui
             __syncthreads();
         }
         if (PASS == 0) {
-            for (int off = 32; off > 0; off >>= 1) c += __shfl_xor(c, off);
+            c = group_sum<64>(c);
             if (l == 0) cnt[i] = c;
         }
     }
@@ -738,7 +738,7 @@ void sgs_setup_device(AmgLevelDev& D, hipStream_t s) {
                   block_nnz(gp[D.h_cptr[c + 1]] - gp[D.h_cptr[c]], stage_nnz<double>(), 2048), stage_rows<double>());
         D.h_cblk.push_back((uint32_t)blk.size() - 1);
     }
-    D.gblk.upload(blk, s);
+    D.gblk.upload(block_bounds(blk, gp.data()), s);
     const double avg = n ? (double)A.nnz / n : 0.0;
     D.lanes = avg <= 1.5 ? 1 : avg <= 3 ? 4 : avg <= 12 ? 8 : avg <= 24 ? 16 : avg <= 48 ? 32 : 64;
     FEM_HIP(hipStreamSynchronize(s));  // the host arrays end with this call
@@ -807,7 +807,7 @@ void sgs_blocks32(AmgLevelDev& D, hipStream_t s) {
                   block_nnz(gp[D.h_cptr[c + 1]] - gp[D.h_cptr[c]], stage_nnz<float>(), 2048), stage_rows<float>());
         D.h_cblk32.push_back((uint32_t)blk.size() - 1);
     }
-    D.gblk32.upload(blk, s);
+    D.gblk32.upload(block_bounds(blk, gp.data()), s);
     FEM_HIP(hipStreamSynchronize(s));  // blk ends with this call
 }
 

@@ -27,6 +27,7 @@
 #include <numeric>
 
 #include "handles.hpp"
+#include "reduce.hpp"
 
 namespace fem {
 namespace {
@@ -88,7 +89,7 @@ __device__ __forceinline__ void wg_sum(double (&v)[K], double* red) {
     constexpr int NW = BLOCK / 64;
 #pragma unroll
     for (int k = 0; k < K; ++k)
-        for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off);
+        v[k] = wave_sum(v[k]);
     const int w = threadIdx.x / 64;
     if ((threadIdx.x & 63) == 0)
 #pragma unroll

@@ -530,6 +530,61 @@ int fem_bench_precond(fem_pc M, int fp32, int reps, double* avg_ms) {
     });
 }
 
+// Streaming-read ceiling: `bytes` of 0.25f read with 16-byte loads (2048
+// workgroups), back to back (cold = 0) or each read timed alone after a
+// 1 GiB read has evicted the L2 and the Infinity Cache (cold != 0).
+int fem_bench_stream(uint64_t bytes, int cold, int reps, double* avg_ms) {
+    return guard([&] {
+        if (bytes < 16 || reps < 1) fail(FEM_ERR_INVALID_ARGUMENT, "fem_bench_stream: bytes >= 16 and reps >= 1");
+        bytes &= ~(uint64_t)15;
+        hipStream_t s;
+        FEM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        struct Done {
+            hipStream_t s;
+            ~Done() { (void)hipStreamDestroy(s); }
+        } done{s};
+        DevBuf<float> a(bytes / 4), sink(2048);
+        vec_fill<float>(a.get(), 0.25f, bytes / 4, s);
+        const size_t flush_n = cold ? (size_t)1 << 27 : 0;  // doubles: 1 GiB
+        DevBuf<double> flush(flush_n), fsink(2048);
+        if (cold) vec_fill<double>(flush.get(), 1.0, flush_n, s);
+        stream_read(a.get(), bytes, sink.get(), 2048, s);  // warms instruction caches
+        if (!cold) {
+            // back to back between one pair of events, like fem_bench_spmv
+            hipEvent_t e0, e1;
+            FEM_HIP(hipEventCreate(&e0));
+            FEM_HIP(hipEventCreate(&e1));
+            FEM_HIP(hipEventRecord(e0, s));
+            for (int i = 0; i < reps; ++i) stream_read(a.get(), bytes, sink.get(), 2048, s);
+            FEM_HIP(hipEventRecord(e1, s));
+            FEM_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            FEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            *avg_ms = ms / reps;
+            return;
+        }
+        std::vector<hipEvent_t> ev(2 * (size_t)reps);
+        for (auto& e : ev) FEM_HIP(hipEventCreate(&e));
+        for (int i = 0; i < reps; ++i) {
+            cache_flush_read(flush.get(), flush_n, fsink.get(), 2048, s);
+            FEM_HIP(hipEventRecord(ev[2 * i], s));
+            stream_read(a.get(), bytes, sink.get(), 2048, s);
+            FEM_HIP(hipEventRecord(ev[2 * i + 1], s));
+        }
+        FEM_HIP(hipEventSynchronize(ev.back()));
+        double tot = 0.0;
+        for (int i = 0; i < reps; ++i) {
+            float ms = 0.f;
+            FEM_HIP(hipEventElapsedTime(&ms, ev[2 * i], ev[2 * i + 1]));
+            tot += ms;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        *avg_ms = tot / reps;
+    });
+}
+
 // Same SpMV, each launch timed alone after reading 1 GiB of other data has
 // evicted the L2 and the 256 MB Infinity Cache (MALL) with clean lines, so
 // the matrix streams from HBM and no write-back is charged to the SpMV. All

@@ -41,6 +41,12 @@ void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint3
 // a block's chain of dependent loads; smaller blocks shorten that chain.
 // Measured at 128^3 (SA-AMG V-cycle) and 96^3 (learned SPAI PCG): SpMV plans
 // are best with min_blocks 1024, the Gauss-Seidel colour sweeps with 2048.
+std::vector<uint2> block_bounds(const std::vector<uint32_t>& b, const uint32_t* h_rp) {
+    std::vector<uint2> o(b.size());
+    for (size_t i = 0; i < b.size(); ++i) o[i] = make_uint2(b[i], h_rp[b[i]]);
+    return o;
+}
+
 uint32_t block_nnz(uint64_t nnz, uint32_t stage, uint64_t min_blocks) {
     uint32_t m = stage;
     while (m > 512 && nnz / m < min_blocks) m /= 2;
@@ -53,7 +59,7 @@ void SpmvPlan::build(const uint32_t* h_rp, uint32_t n, hipStream_t s, uint32_t m
     b.push_back(0);
     plan_rows(h_rp, 0, n, b, max_nnz, max_rows);
     nblk = (uint32_t)b.size() - 1;
-    blk_row.upload(b, s);
+    bnd.upload(block_bounds(b, h_rp), s);
     FEM_HIP(hipStreamSynchronize(s));  // b ends with this call
 }
 
@@ -92,15 +98,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
                                                  const uint16_t* __restrict__ cd,
                                                  const uint32_t* __restrict__ cbase, const T* __restrict__ v,
                                                  const T* __restrict__ x, T* __restrict__ y,
-                                                 const uint32_t* __restrict__ blk_row, SpmvArgs<T> a, Gate g,
+                                                 const uint2* __restrict__ bnd, SpmvArgs<T> a, Gate g,
                                                  double* __restrict__ part = nullptr) {
     constexpr int STAGE = stage_nnz<T>();
     __shared__ T prod[STAGE];
     const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
     // the block's bounds are read while the gate's flag is in flight (reads
     // only: the gate guards the stores)
-    const uint32_t r0 = blk_row[bid], r1 = blk_row[bid + 1];
-    const uint32_t k0 = rp[r0], k1 = rp[r1];
+    const uint2 lo = bnd[bid], hi = bnd[bid + 1];
+    const uint32_t r0 = lo.x, r1 = hi.x, k0 = lo.y, k1 = hi.y;
     if (g.closed()) return;
     const uint32_t cnt = k1 - k0;
     const uint32_t t = threadIdx.x;
@@ -200,8 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_vec(const uint32_t* __restrict_
     if (g.closed()) return;
     T s = (T)0;
     for (uint32_t k = k0 + lane; k < k1; k += LANES) s += v[k] * x[ci[k]];
-#pragma unroll
-    for (int off = LANES / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, LANES);
+    s = group_sum<LANES>(s);
     if (row < n && lane == 0) y[row] = spmv_epilogue<T, EPI>(row, s, a);
 }
 
@@ -225,10 +230,10 @@ void spmv_stream_launch(const CsrMat& A, const T* vals, const T* x, T* y, const 
     const dim3 grid(plan.nblk), block(kBlock);
     if (A.cd.get())
         hipLaunchKernelGGL((k_spmv<T, EPI, true>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
-                           A.cbase.get(), vals, x, y, plan.blk_row.get(), a, g);
+                           A.cbase.get(), vals, x, y, plan.bnd.get(), a, g);
     else
         hipLaunchKernelGGL((k_spmv<T, EPI, false>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
-                           A.cbase.get(), vals, x, y, plan.blk_row.get(), a, g);
+                           A.cbase.get(), vals, x, y, plan.bnd.get(), a, g);
 }
 
 template <typename T>
@@ -242,10 +247,10 @@ unsigned csr_spmv_dot(const CsrMat& A, const T* x, T* y, double* part, hipStream
     const dim3 grid(plan.nblk), block(kBlock);
     if (A.cd.get())
         hipLaunchKernelGGL((k_spmv<T, EPI_STORE, true, true>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
-                           A.cbase.get(), vals, x, y, plan.blk_row.get(), SpmvArgs<T>{}, g, part);
+                           A.cbase.get(), vals, x, y, plan.bnd.get(), SpmvArgs<T>{}, g, part);
     else
         hipLaunchKernelGGL((k_spmv<T, EPI_STORE, false, true>), grid, block, 0, s, A.rp.get(), A.ci.get(),
-                           A.cd.get(), A.cbase.get(), vals, x, y, plan.blk_row.get(), SpmvArgs<T>{}, g, part);
+                           A.cd.get(), A.cbase.get(), vals, x, y, plan.bnd.get(), SpmvArgs<T>{}, g, part);
     FEM_HIP(hipGetLastError());
     return plan.nblk;
 }
@@ -263,10 +268,10 @@ unsigned csr_spmv_selfdot(const CsrMat& A, const T* x, T* y, double* part, hipSt
     const dim3 grid(plan.nblk), block(kBlock);
     if (A.cd.get())
         hipLaunchKernelGGL((k_spmv<T, EPI_STORE, true, 2>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
-                           A.cbase.get(), vals, x, y, plan.blk_row.get(), SpmvArgs<T>{}, g, part);
+                           A.cbase.get(), vals, x, y, plan.bnd.get(), SpmvArgs<T>{}, g, part);
     else
         hipLaunchKernelGGL((k_spmv<T, EPI_STORE, false, 2>), grid, block, 0, s, A.rp.get(), A.ci.get(), A.cd.get(),
-                           A.cbase.get(), vals, x, y, plan.blk_row.get(), SpmvArgs<T>{}, g, part);
+                           A.cbase.get(), vals, x, y, plan.bnd.get(), SpmvArgs<T>{}, g, part);
     FEM_HIP(hipGetLastError());
     return plan.nblk;
 }

@@ -88,8 +88,13 @@ uint32_t block_nnz(uint64_t nnz, uint32_t stage, uint64_t min_blocks);
 void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint32_t>& b,
                uint32_t max_nnz = kStageNnz, uint32_t max_rows = kStageRows);
 
+// block boundaries (first row, first entry) as one 8-byte load: a block
+// reads its bounds and starts streaming entries without first reading the
+// row pointers
+std::vector<uint2> block_bounds(const std::vector<uint32_t>& b, const uint32_t* h_rp);
+
 struct SpmvPlan {
-    DevBuf<uint32_t> blk_row;  // [nblk+1]
+    DevBuf<uint2> bnd;  // [nblk+1]
     uint32_t nblk = 0;
     void build(const uint32_t* h_rp, uint32_t n, hipStream_t s, uint32_t max_nnz = kStageNnz,
                uint32_t max_rows = kStageRows);
@@ -293,6 +298,8 @@ void vec_convert(TD* y, const TS* x, double scale, size_t n, hipStream_t s, Gate
 // out = a . b, synchronous helper (host result)
 // read n doubles (cache eviction without dirty lines); sink: `blocks` doubles
 void cache_flush_read(const double* buf, size_t n, double* sink, unsigned blocks, hipStream_t s);
+// reads `bytes` (a multiple of 16) with 16-byte loads, `blocks` workgroups
+void stream_read(const void* buf, size_t bytes, float* sink, unsigned blocks, hipStream_t s);
 double dot_sync(const double* a, const double* b, size_t n, const Reducer& R, hipStream_t s);
 // out = a . b on device pointers, enqueued on s, with no allocation: the
 // partial sums go to a static device array (vec.hip); calls that overlap on

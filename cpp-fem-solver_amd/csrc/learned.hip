@@ -178,8 +178,7 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
                 }
             {
                 float s = gb[0] + gb[1] + gb[2] + gb[3];
-                s += __shfl_xor(s, 16);
-                s += __shfl_xor(s, 32);
+                s = swap_sum<true>(swap_sum<false>(s));
                 ab3 += s;
             }
             // dW3 += h2^T g
@@ -197,8 +196,7 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
                     s += g;
                     G2[w][4 * lg + v][16 * nt + lr] = f2bf(g);
                 }
-                s += __shfl_xor(s, 16);
-                s += __shfl_xor(s, 32);
+                s = swap_sum<true>(swap_sum<false>(s));
                 ab2[nt] += s;
             }
             __syncthreads();
@@ -226,8 +224,7 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
                         s += g;
                         G1[w][4 * lg + v][16 * nt + lr] = f2bf(g);
                     }
-                    s += __shfl_xor(s, 16);
-                    s += __shfl_xor(s, 32);
+                    s = swap_sum<true>(swap_sum<false>(s));
                     ab1[nt] += s;
                 }
             }

@@ -126,6 +126,7 @@ _SIGS = {
     "fem_precond_info": (C.c_int, [P_vp, C.POINTER(C.c_int), P_u32, C.POINTER(C.c_uint64), C.c_int, P_f64]),
     "fem_precond_bytes": (C.c_int, [P_vp, C.c_int, P_f64]),
     "fem_bench_precond": (C.c_int, [P_vp, C.c_int, C.c_int, P_f64]),
+    "fem_bench_stream": (C.c_int, [C.c_uint64, C.c_int, C.c_int, P_f64]),
     "fem_amg_level_export": (C.c_int, [P_vp, C.c_int, C.c_int, P_u32, P_u32, P_u32, P_f64]),
     "fem_amg_split": (C.c_int, [P_vp, C.c_int, C.POINTER(C.c_uint8)]),
     "fem_solve": (C.c_int, [P_vp, P_vp, P_f64, P_f64, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
@@ -212,6 +213,14 @@ def _f64(a):
 
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def bench_stream(nbytes: int, cold: bool = False, reps: int = 10) -> float:
+    """Average time (ms) of a streaming read of nbytes (fem_bench_stream):
+    the device's read-bandwidth ceiling, warm or after a cache flush."""
+    ms = C.c_double()
+    _check(lib().fem_bench_stream(int(nbytes), int(bool(cold)), int(reps), C.byref(ms)))
+    return ms.value
 
 
 def device_info() -> tuple[str, int]:

@@ -419,6 +419,11 @@ int fem_bench_spmv_cold(fem_csr A, int fp32, int reps, double* avg_ms);
  * sweeps when fp32 != 0), timed with HIP events on M's stream, input and
  * output resident in HBM. */
 int fem_bench_precond(fem_pc M, int fp32, int reps, double* avg_ms);
+/* The device's streaming-read ceiling: average time of a read of `bytes`
+ * (16-byte loads), back to back (cold = 0) or each after a 1 GiB read has
+ * evicted the L2 and the Infinity Cache (cold != 0). No counterpart in the
+ * reference: it calibrates the roofline fractions. */
+int fem_bench_stream(uint64_t bytes, int cold, int reps, double* avg_ms);
 
 #ifdef __cplusplus
 }
