@@ -1924,6 +1924,23 @@ int or_amg_level_export(OrPc* h, int l, int which, uint32_t* rp, uint32_t* ci, d
     if (h->kind == K_AMG) {
         auto* p = static_cast<PcAmgCF*>(h->pc.get());
         M = which == 0 ? &p->lv[l].A : which == 1 ? &p->lv[l].P : &p->lv[l].R;
+    } else if (auto* p3 = dynamic_cast<PcBlockAMG<3>*>(h->pc.get())) {
+        // 3x3 blocks expanded to scalar CSR: scalar row 3I+a holds, block by
+        // block in block-column order, the columns 3J+0..2 of row a of block (I, J)
+        const Bsr<3>& B = which == 0 ? p3->lv[l].A : which == 1 ? p3->lv[l].P : p3->lv[l].R;
+        tmp.n = B.nbr * 3;
+        tmp.m = B.nbc * 3;
+        tmp.rp.assign(1, 0);
+        for (uint32_t I = 0; I < B.nbr; ++I)
+            for (int a = 0; a < 3; ++a) {
+                for (uint32_t q = B.rp[I]; q < B.rp[I + 1]; ++q)
+                    for (int b = 0; b < 3; ++b) {
+                        tmp.ci.push_back(B.ci[q] * 3 + b);
+                        tmp.v.push_back(B.blk(q)[a * 3 + b]);
+                    }
+                tmp.rp.push_back((uint32_t)tmp.ci.size());
+            }
+        M = &tmp;
     } else {
         auto* p = static_cast<PcBlockAMG<1>*>(h->pc.get());
         const Bsr<1>& B = which == 0 ? p->lv[l].A : which == 1 ? p->lv[l].P : p->lv[l].R;
