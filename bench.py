@@ -7,7 +7,9 @@ stopping test ||r|| / ||b|| < 1e-10.
 
 A step is one complete solve of A x = b_k for a right-hand side not solved
 before, with the preconditioner already built and b and x resident in device
-memory; `value` is the median over the timed steps. The headline method is
+memory; `value` is the time of the timed steps between two device
+synchronisations divided by their number (`solve_ms` per method is the
+median of the per-step times). The headline method is
 PCG with the smoothed-aggregation AMG V-cycle (2 pre / 2 post symmetric
 multicolour Gauss-Seidel sweeps, weight 1.0, coarsening down to at most 300
 rows, whose dense exact solve ends the V-cycle) in float32 inside float64
