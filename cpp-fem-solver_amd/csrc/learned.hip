@@ -8,7 +8,8 @@
 //   features  f_i[d] = -6 a_ij / sqrt(a_ii a_jj), d = 0..5 the directions
 //             -nx*ny, -nx, -1, +1, +nx, +nx*ny (0 where row i has no entry)
 //   network   6 -> 64 -> 64 -> 7, ReLU, float32 master weights, bfloat16
-//             operands on the matrix cores (v_mfma_f32_16x16x16_bf16), float32
+//             operands on the matrix cores (gfx950's v_mfma_f32_16x16x32_bf16
+//             for the K = 64 products, v_mfma_f32_16x16x16_bf16 for K = 16), float32
 //             accumulation; flat parameter layout W1[6][64], b1[64], W2[64][64],
 //             b2[64], W3[64][7], b3[7] (W[in][out]), 5063 floats
 //   G         G_ii = softplus(o_i[0]) dis_i, G_ij = o_i[1+d] dis_j for the
@@ -19,10 +20,15 @@
 //
 // MFMA tiling: one wavefront owns a tile of 16 rows. For
 // v_mfma_f32_16x16x16_bf16 lane l holds A[l%16][4(l/16)+v], B[4(l/16)+v][l%16]
-// and D[4(l/16)+v][l%16], v = 0..3. Hidden activations go through LDS
+// and D[4(l/16)+v][l%16], v = 0..3; for v_mfma_f32_16x16x32_bf16
+// A[l%16][8(l/16)+v] and B[8(l/16)+v][l%16], v = 0..7, D the same. Layers 2
+// and 3 and the back-propagation through W2 (K = 64) take two K = 32
+// products; layer 1 (K = 6, padded to 16), the back-propagation through W3
+// (K = 7) and the weight gradients (K = the tile's 16 rows) take K = 16. Hidden activations go through LDS
 // (16 x 64 bfloat16 per wave) to turn the D layout into the next layer's A
-// layout. Weights are packed once per update into B-operand fragments
-// (64 lanes x 4 bfloat16 each): W1 (4), W2 (16), W3 (4), W3^T (4), W2^T (16).
+// layout. Weights are packed once per update into B-operand fragments:
+// 64 lanes x 4 bfloat16 for W1 (4) and W3^T (4), 64 lanes x 8 for W2 (8),
+// W3 (2) and W2^T (8).
 // The backward pass recomputes the forward pass per tile, accumulates weight
 // gradients per wave in registers (fixed tile order), sums the 4 waves of a
 // block in LDS in wave order and writes one partial per block; a second
@@ -43,10 +49,16 @@ namespace {
 constexpr int NI = 6, NH = 64, NO = 7;
 constexpr int OFF_W1 = 0, OFF_B1 = OFF_W1 + NI * NH, OFF_W2 = OFF_B1 + NH, OFF_B2 = OFF_W2 + NH * NH,
               OFF_W3 = OFF_B2 + NH, OFF_B3 = OFF_W3 + NH * NO;
-constexpr int FR_W1 = 0, FR_W2 = 4, FR_W3 = 20, FR_W3T = 24, FR_W2T = 28, NFRAG = 44;
+// B-operand fragments of the K = 16 products (layer 1, the W3^T back-propagation)
+constexpr int FR_W1 = 0, FR_W3T = 4, NFRAG = 8;
+// B-operand fragments of the K = 64 products, as two K = 32 chunks each
+// (gfx950's v_mfma_f32_16x16x32_bf16): W2 (2 x 4), W3 (2), W2^T (2 x 4)
+constexpr int F8_W2 = 0, F8_W3 = 8, F8_W2T = 10, NFRAG8 = 18;
 constexpr int kLds = 68;  // LDS row stride (bfloat16) of a 16 x 64 activation tile
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef short bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ short f2bf(float f) {
@@ -57,6 +69,18 @@ __device__ __forceinline__ short f2bf(float f) {
 
 __device__ __forceinline__ f32x4 mma(bf16x4 a, bf16x4 b, f32x4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+// 16 x 16 x 32: lane l holds A[l%16][8(l/16)+v] and B[8(l/16)+v][l%16],
+// v = 0..7; D as for 16 x 16 x 16
+__device__ __forceinline__ f32x4 mma32(bf16x8 a, bf16x8 b, f32x4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b), c,
+                                                   0, 0, 0);
+}
+// eight consecutive bfloat16 of an LDS tile row (two 8-byte reads: rows are
+// 8-byte aligned)
+__device__ __forceinline__ bf16x8 lds8(const short* p) {
+    const bf16x4 lo = *reinterpret_cast<const bf16x4*>(p), hi = *reinterpret_cast<const bf16x4*>(p + 4);
+    return __builtin_shufflevector(lo, hi, 0, 1, 2, 3, 4, 5, 6, 7);
 }
 
 // element (k, n) of the B operand of fragment family `fam`
@@ -70,25 +94,33 @@ __device__ __forceinline__ float frag_elem(const float* P, int fam, int k, int n
     }
 }
 
-// fragments: NFRAG x 64 lanes x 4 bfloat16
-__global__ void k_pack(const float* __restrict__ P, bf16x4* __restrict__ frag) {
+// fragments: NFRAG x 64 lanes x 4 bfloat16, then NFRAG8 x 64 lanes x 8
+__global__ void k_pack(const float* __restrict__ P, bf16x4* __restrict__ frag, bf16x8* __restrict__ frag8) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= NFRAG * 64) return;
-    const int f = t / 64, l = t % 64;
+    if (t < NFRAG * 64) {
+        const int f = t / 64, l = t % 64;
+        const int fam = f < FR_W3T ? 0 : 3, nt = f < FR_W3T ? f - FR_W1 : f - FR_W3T;
+        bf16x4 r;
+        for (int v = 0; v < 4; ++v) r[v] = f2bf(frag_elem(P, fam, 4 * (l / 16) + v, 16 * nt + l % 16));
+        frag[t] = r;
+        return;
+    }
+    const int u = t - NFRAG * 64;
+    if (u >= NFRAG8 * 64) return;
+    const int f = u / 64, l = u % 64;
     int fam, kt, nt;
-    if (f < FR_W2) { fam = 0; kt = 0; nt = f - FR_W1; }
-    else if (f < FR_W3) { fam = 1; kt = (f - FR_W2) / 4; nt = (f - FR_W2) % 4; }
-    else if (f < FR_W3T) { fam = 2; kt = f - FR_W3; nt = 0; }
-    else if (f < FR_W2T) { fam = 3; kt = 0; nt = f - FR_W3T; }
-    else { fam = 4; kt = (f - FR_W2T) / 4; nt = (f - FR_W2T) % 4; }
-    bf16x4 r;
-    for (int v = 0; v < 4; ++v) r[v] = f2bf(frag_elem(P, fam, 16 * kt + 4 * (l / 16) + v, 16 * nt + l % 16));
-    frag[t] = r;
+    if (f < F8_W3) { fam = 1; kt = (f - F8_W2) / 4; nt = (f - F8_W2) % 4; }
+    else if (f < F8_W2T) { fam = 2; kt = f - F8_W3; nt = 0; }
+    else { fam = 4; kt = (f - F8_W2T) / 4; nt = (f - F8_W2T) % 4; }
+    bf16x8 r;
+    for (int v = 0; v < 8; ++v) r[v] = f2bf(frag_elem(P, fam, 32 * kt + 8 * (l / 16) + v, 16 * nt + l % 16));
+    frag8[u] = r;
 }
 
 struct MlpArgs {
     const short* feat;    // n x 8 bfloat16 bits (features 0..5, 6..7 zero)
     const bf16x4* frag;   // NFRAG x 64
+    const bf16x8* frag8;  // NFRAG8 x 64
     const float* P;       // master parameters (biases)
     uint32_t n;
     float* out;           // forward: n x 8 (outputs 0..6)
@@ -135,11 +167,11 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         }
         __syncthreads();
         // ---- layer 2: z2 = h1 W2 + b2 -----------------------------------
-        bf16x4 ha[4];
-        for (int kt = 0; kt < 4; ++kt) ha[kt] = *reinterpret_cast<const bf16x4*>(&H1[w][lr][16 * kt + 4 * lg]);
+        bf16x8 ha[2];
+        for (int kt = 0; kt < 2; ++kt) ha[kt] = lds8(&H1[w][lr][32 * kt + 8 * lg]);
         for (int nt = 0; nt < 4; ++nt) {
             f32x4 acc = zero;
-            for (int kt = 0; kt < 4; ++kt) acc = mma(ha[kt], a.frag[(FR_W2 + kt * 4 + nt) * 64 + l], acc);
+            for (int kt = 0; kt < 2; ++kt) acc = mma32(ha[kt], a.frag8[(F8_W2 + kt * 4 + nt) * 64 + l], acc);
             const float b = a.P[OFF_B2 + 16 * nt + lr];
             for (int v = 0; v < 4; ++v) {
                 acc[v] += b;
@@ -151,9 +183,8 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
         // ---- layer 3: o = h2 W3 + b3 ------------------------------------
         if constexpr (!BWD) {
             f32x4 o = zero;
-            for (int kt = 0; kt < 4; ++kt)
-                o = mma(*reinterpret_cast<const bf16x4*>(&H2[w][lr][16 * kt + 4 * lg]),
-                        a.frag[(FR_W3 + kt) * 64 + l], o);
+            for (int kt = 0; kt < 2; ++kt)
+                o = mma32(lds8(&H2[w][lr][32 * kt + 8 * lg]), a.frag8[(F8_W3 + kt) * 64 + l], o);
             if (lr < 8) {
                 const float b = lr < NO ? a.P[OFF_B3 + lr] : 0.f;
                 for (int v = 0; v < 4; ++v) {
@@ -213,11 +244,11 @@ __global__ __launch_bounds__(256) void k_mlp(MlpArgs a) {
             }
             // dz1 = (dz2 W2^T) * [z1 > 0]
             {
-                bf16x4 ga[4];
-                for (int kt = 0; kt < 4; ++kt) ga[kt] = *reinterpret_cast<const bf16x4*>(&G2[w][lr][16 * kt + 4 * lg]);
+                bf16x8 ga[2];
+                for (int kt = 0; kt < 2; ++kt) ga[kt] = lds8(&G2[w][lr][32 * kt + 8 * lg]);
                 for (int nt = 0; nt < 4; ++nt) {
                     f32x4 d = zero;
-                    for (int kt = 0; kt < 4; ++kt) d = mma(ga[kt], a.frag[(FR_W2T + kt * 4 + nt) * 64 + l], d);
+                    for (int kt = 0; kt < 2; ++kt) d = mma32(ga[kt], a.frag8[(F8_W2T + kt * 4 + nt) * 64 + l], d);
                     float s = 0.f;
                     for (int v = 0; v < 4; ++v) {
                         const float g = z1[nt][v] > 0.f ? d[v] : 0.f;
@@ -450,7 +481,7 @@ MlpNet::MlpNet(hipStream_t s) : stream(s) {
     m.alloc(kMlpParams);
     v.alloc(kMlpParams);
     g.alloc(kMlpParams);
-    frag.alloc((size_t)NFRAG * 64 * 4);
+    frag.alloc((size_t)NFRAG * 64 * 4 + (size_t)NFRAG8 * 64 * 8);
     vec_fill<float>(m.get(), 0.f, kMlpParams, s);
     vec_fill<float>(v.get(), 0.f, kMlpParams, s);
 }
@@ -468,14 +499,16 @@ void MlpNet::get(float* host) const {
 
 void MlpNet::pack() {
     if (packed) return;
-    hipLaunchKernelGGL(k_pack, dim3(g1(NFRAG * 64)), dim3(256), 0, stream, P.get(), (bf16x4*)frag.get());
+    hipLaunchKernelGGL(k_pack, dim3(g1((NFRAG + NFRAG8) * 64)), dim3(256), 0, stream, P.get(), (bf16x4*)frag.get(),
+                       (bf16x8*)(frag.get() + (size_t)NFRAG * 64 * 4));
     FEM_HIP(hipGetLastError());
     packed = true;
 }
 
 void MlpNet::forward(const short* feat, uint32_t n, float* out) {
     pack();
-    MlpArgs a{feat, (const bf16x4*)frag.get(), P.get(), n, out, nullptr, nullptr};
+    MlpArgs a{feat, (const bf16x4*)frag.get(), (const bf16x8*)(frag.get() + (size_t)NFRAG * 64 * 4), P.get(), n, out,
+              nullptr, nullptr};
     const unsigned ntiles = (n + 15) / 16;
     const unsigned blocks = std::max(1u, std::min(4096u, (ntiles + 3) / 4));
     hipLaunchKernelGGL((k_mlp<false>), dim3(blocks), dim3(256), 0, stream, a);
@@ -488,7 +521,8 @@ void MlpNet::backward(const short* feat, uint32_t n, const float* dout) {
     // at least 4 tiles per wave, at most 256 block partials
     const unsigned blocks = std::max(1u, std::min(256u, (ntiles + 15) / 16));
     if (part.n < (size_t)blocks * kMlpParams) part.alloc((size_t)blocks * kMlpParams);
-    MlpArgs a{feat, (const bf16x4*)frag.get(), P.get(), n, nullptr, dout, part.get()};
+    MlpArgs a{feat, (const bf16x4*)frag.get(), (const bf16x8*)(frag.get() + (size_t)NFRAG * 64 * 4), P.get(), n, nullptr,
+              dout, part.get()};
     hipLaunchKernelGGL((k_mlp<true>), dim3(blocks), dim3(256), 0, stream, a);
     hipLaunchKernelGGL(k_grad_reduce, dim3(g1(kMlpParams)), dim3(256), 0, stream, part.get(), blocks, 1.0, g.get());
     FEM_HIP(hipGetLastError());
