@@ -992,11 +992,12 @@ __global__ __launch_bounds__(64 * W) void k_sweep_blk(uint32_t n, uint32_t nblk,
                 if (ab == B::kSent) ab = B::kQuiet;
                 const T out = __builtin_bit_cast(T, ab);
                 if (cur.on) res[cur.gs - g0] = out;
-                // the step's LDS results, then the hand-off to the next step
+                // the step's LDS results, then the hand-off to the next step:
+                // a wavefront's LDS operations complete in issue order, so a
+                // reader that sees the new count also sees the results (no
+                // wait for the stores in between)
                 __asm__ volatile("" ::: "memory");
-                __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0)
                 __builtin_amdgcn_wave_barrier();
-                __asm__ volatile("" ::: "memory");
                 if (lane == 0) __hip_atomic_store(&done_sh, q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
                 if (cur.on) {
                     y[cur.row] = out;
