@@ -954,8 +954,8 @@ __global__ __launch_bounds__(64 * W) void k_sweep_blk(uint32_t n, uint32_t nblk,
                 while (__hip_atomic_load(&done_sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) + kXLead < q)
                     __builtin_amdgcn_s_sleep(1);
                 stage_x(cur);
-                while (__hip_atomic_load(&done_sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != q)
-                    __builtin_amdgcn_s_sleep(1);
+                while (__hip_atomic_load(&done_sh, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) != q) {
+                }
                 // external inputs still pending at stage B: poll them
                 bool late = false;
 #pragma unroll
