@@ -302,20 +302,28 @@ def rows_leg(F, gen, args, log):
         F.device_synchronize()
         ms = (time.perf_counter() - t) * 1e3
         out[name] = {"solve_ms": round(ms, 2), "setup_ms": round(setup_ms, 1), "iterations": st["iterations"],
-                     "relres": st["true_relres"]}
+                     "refinements": st["refinements"], "converged": st["converged"], "relres": st["true_relres"]}
         log(f"[bench] rows {name}: setup {setup_ms:.1f} ms solve {ms:.2f} ms it {st['iterations']} "
-            f"relres {st['true_relres']:.2e}")
+            f"(refinements {st['refinements']}) converged {st['converged']} relres {st['true_relres']:.2e}")
         S.close()
         if M is not None:
             M.close()
 
+    # C1: 2-D Poisson 64^2, Jacobi PCG (cache-resident: time only)
+    G = gen.poisson2d(64)
+    _, b = gen.rhs(G, 0)
+    A = F.CsrMatrix.from_gen(G)
+    csr_solve("c1_poisson64_2d_jacobi_pcg", A, b, "diag", max_iter=5000)
+    A.close()
     G = gen.poisson3d(128)
     _, b = gen.rhs(G, 0)
     A = F.CsrMatrix.from_gen(G)
     csr_solve("poisson128_ssor_pcg", A, b, "ssor", omega=1.0)
     csr_solve("poisson128_ic0_pcg_mp", A, b, "ic0", prec=F.FP32_IR)
+    # C2 with the C/F AMG (device setup), FP64 and FP32 inner / FP64 refinement
+    csr_solve("poisson128_cf_amg_pcg", A, b, "amg")
+    csr_solve("poisson128_cf_amg_pcg_mp", A, b, "amg", prec=F.FP32_IR)
     A.close()
-    # C/F AMG (host-built hierarchy)
     G = gen.poisson3d(64)
     _, b = gen.rhs(G, 0)
     A = F.CsrMatrix.from_gen(G)
@@ -327,6 +335,15 @@ def rows_leg(F, gen, args, log):
     csr_solve("convdiff256_bicgstab_ilu0", A, b, "ilu0", solver="bicgstab")
     csr_solve("convdiff256_gmres30_ilu0", A, b, "ilu0", solver="gmres", restart=30)
     csr_solve("convdiff256_bicgstab_ilu0_mp", A, b, "ilu0", solver="bicgstab", prec=F.FP32_IR)
+    A.close()
+    # C3 at full size in FP32 inner / FP64 refinement
+    G = gen.convdiff2d(args.c3_n)
+    _, b = gen.rhs(G, 0)
+    A = F.CsrMatrix.from_gen(G)
+    csr_solve(f"convdiff{args.c3_n}_bicgstab_ilu0_mp", A, b, "ilu0", solver="bicgstab", prec=F.FP32_IR,
+              max_iter=args.c3_max_iter)
+    csr_solve(f"convdiff{args.c3_n}_gmres30_ilu0_mp", A, b, "ilu0", solver="gmres", restart=30, prec=F.FP32_IR,
+              max_iter=args.c3_max_iter)
     A.close()
     # block::BlockMatrix: 3x3 blocks from the reference test's generator, 128^2 block rows
     S = gen.block_test_scalar(128)
