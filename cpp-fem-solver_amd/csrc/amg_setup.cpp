@@ -141,28 +141,52 @@ std::vector<double> sa_coarse_operator_host(const HCsr& A, bool exact) {
             }
         }
     }
-    std::vector<double> C((size_t)sz * sz), x(sz);
-    for (int col = 0; col < sz; ++col) {
-        if (exact) {
-            for (int i = 0; i < sz; ++i) x[i] = (pv[i] == col) ? 1.0 : 0.0;
-            for (int i = 0; i < sz; ++i)
-                for (int j = 0; j < i; ++j) x[i] -= d[(size_t)i * sz + j] * x[j];
-            for (int i = sz - 1; i >= 0; --i) {
-                for (int j = i + 1; j < sz; ++j) x[i] -= d[(size_t)i * sz + j] * x[j];
-                x[i] /= d[(size_t)i * sz + i];
+    // C = A^-1: every column of the identity solved at once, row by row
+    // (row i of C holds element i of every column's solution), so the inner
+    // loops run over contiguous columns; each element sees the same
+    // operations in the same order as a column-by-column solve.
+    std::vector<double> C((size_t)sz * sz, 0.0);
+    auto row = [&](int i) { return C.data() + (size_t)i * sz; };
+    if (exact) {
+        for (int i = 0; i < sz; ++i) row(i)[pv[i]] = 1.0;
+        for (int i = 0; i < sz; ++i)
+            for (int j = 0; j < i; ++j) {
+                const double f = d[(size_t)i * sz + j];
+                double* ci = row(i);
+                const double* cj = row(j);
+                for (int c = 0; c < sz; ++c) ci[c] -= f * cj[c];
             }
-        } else {
-            for (int i = 0; i < sz; ++i) x[i] = (i == col) ? 1.0 : 0.0;
-            for (int k = 0; k < sz; ++k) {
-                const double dk = fl(d[(size_t)k * sz + k]);
-                for (int i = k + 1; i < sz; ++i) x[i] -= (d[(size_t)i * sz + k] / dk) * x[k];
+        for (int i = sz - 1; i >= 0; --i) {
+            double* ci = row(i);
+            for (int j = i + 1; j < sz; ++j) {
+                const double f = d[(size_t)i * sz + j];
+                const double* cj = row(j);
+                for (int c = 0; c < sz; ++c) ci[c] -= f * cj[c];
             }
-            for (int i = sz - 1; i >= 0; --i) {
-                for (int j = i + 1; j < sz; ++j) x[i] -= d[(size_t)i * sz + j] * x[j];
-                x[i] /= fl(d[(size_t)i * sz + i]);
+            const double di = d[(size_t)i * sz + i];
+            for (int c = 0; c < sz; ++c) ci[c] /= di;
+        }
+    } else {
+        for (int i = 0; i < sz; ++i) row(i)[i] = 1.0;
+        for (int k = 0; k < sz; ++k) {
+            const double dk = fl(d[(size_t)k * sz + k]);
+            const double* ck = row(k);
+            for (int i = k + 1; i < sz; ++i) {
+                const double f = d[(size_t)i * sz + k] / dk;
+                double* ci = row(i);
+                for (int c = 0; c < sz; ++c) ci[c] -= f * ck[c];
             }
         }
-        for (int i = 0; i < sz; ++i) C[(size_t)i * sz + col] = x[i];
+        for (int i = sz - 1; i >= 0; --i) {
+            double* ci = row(i);
+            for (int j = i + 1; j < sz; ++j) {
+                const double f = d[(size_t)i * sz + j];
+                const double* cj = row(j);
+                for (int c = 0; c < sz; ++c) ci[c] -= f * cj[c];
+            }
+            const double di = fl(d[(size_t)i * sz + i]);
+            for (int c = 0; c < sz; ++c) ci[c] /= di;
+        }
     }
     return C;
 }
