@@ -144,6 +144,7 @@ struct KState {
     int iters;        // iterations run (operator applications in the loop)
     int minicg;       // 1: mini_cg semantics (no breakdown tests, raw loop index)
     double rz_prev;   // PCG: rz before the current iteration's update (fused p update)
+    int fac_go;       // PCG with M = G^T G: this iteration's t = G r ran (gate of the p update)
 };
 
 enum SolverErr : int {

@@ -118,15 +118,17 @@ __device__ __forceinline__ uint32_t col_of(const uint32_t* __restrict__ ci, cons
     else return ci[k];
 }
 
-// CSR-stream staging: prod[k] = v[k0+k] * x[col[k0+k]] for k < cnt (cnt >= 1)
-// by a block of kBlock threads, in batches of 8 entries per thread: a thread
-// issues the batch's value and column loads (indices clamped to cnt - 1, so
-// every load is in range and all of them can be in flight), then its gathers
-// of x, then the LDS stores of the products inside the range.
-template <typename T, int STAGE, bool PK>
-__device__ __forceinline__ void stage_products(const uint32_t* __restrict__ ci, const uint16_t* __restrict__ cd,
-                                               const uint32_t* __restrict__ cbase, const T* __restrict__ v,
-                                               const T* x, uint32_t k0, uint32_t cnt, T* prod) {
+// CSR-stream staging: prod[k] = v[k0+k] * gx(col[k0+k]) for k < cnt
+// (cnt >= 1) by a block of kBlock threads, in batches of 8 entries per
+// thread: a thread issues the batch's value and column loads (indices clamped
+// to cnt - 1, so every load is in range and all of them can be in flight),
+// then its gathers gx (x[c], or a value formed from several vectors), then
+// the LDS stores of the products inside the range. (One batch of 16 in
+// float32 was slower: 28.4 -> 31.6 us for the 128^3 SpMV.)
+template <typename T, int STAGE, bool PK, class GX>
+__device__ __forceinline__ void stage_products_g(const uint32_t* __restrict__ ci, const uint16_t* __restrict__ cd,
+                                                 const uint32_t* __restrict__ cbase, const T* __restrict__ v, GX gx,
+                                                 uint32_t k0, uint32_t cnt, T* prod) {
     constexpr int U = STAGE / kBlock;
     constexpr int B = U < 8 ? U : 8;
     static_assert(U % B == 0, "stage size");
@@ -143,13 +145,19 @@ __device__ __forceinline__ void stage_products(const uint32_t* __restrict__ ci, 
             a[q] = v[k];
         }
 #pragma unroll
-        for (int q = 0; q < B; ++q) xv[q] = x[c[q]];
+        for (int q = 0; q < B; ++q) xv[q] = gx(c[q]);
 #pragma unroll
         for (int q = 0; q < B; ++q) {
             const uint32_t k = (uint32_t)(q0 + q) * kBlock + t;
             if (k < cnt) prod[k] = a[q] * xv[q];
         }
     }
+}
+template <typename T, int STAGE, bool PK>
+__device__ __forceinline__ void stage_products(const uint32_t* __restrict__ ci, const uint16_t* __restrict__ cd,
+                                               const uint32_t* __restrict__ cbase, const T* __restrict__ v,
+                                               const T* x, uint32_t k0, uint32_t cnt, T* prod) {
+    stage_products_g<T, STAGE, PK>(ci, cd, cbase, v, [x](uint32_t c) { return x[c]; }, k0, cnt, prod);
 }
 
 struct CsrMat : LinOp {
@@ -258,6 +266,10 @@ struct Precond {
     virtual unsigned apply_dot(const double*, double*, double*, hipStream_t, Gate = {}) { return 0; }
     virtual unsigned apply32_dot(const float*, float*, double*, hipStream_t, Gate = {}) { return 0; }
     virtual unsigned dot_parts() const { return 0; }
+    // M = G^T G with G and G^T held as CSR matrices (factorized sparse
+    // approximate inverses): PCG then forms the products inside its own
+    // update kernels. False for every other preconditioner.
+    virtual bool factors(const CsrMat*& /*G*/, const CsrMat*& /*Gt*/) const { return false; }
     // Algorithmic bytes of one application with s-byte values: the sum over
     // the kernels it launches, each reading its matrix (uint32 indices) and
     // each vector once and writing its output once. 0: not tallied.

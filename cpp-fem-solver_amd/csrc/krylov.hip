@@ -29,6 +29,10 @@ enum PcMode : int { PCM_GEN = 0, PCM_NONE = 1, PCM_DIAG = 2 };
 // round-off near convergence rather than a failure of the method
 constexpr double kSoftExit = 1e-3;
 
+// PCG with M = G^T G: most partial sums re-read in all by the blocks of one
+// of its fused kernels (blocks x partial sums per block)
+constexpr uint64_t kFacFold = 8ull << 20;
+
 __device__ __forceinline__ void kfail(KState* st, int code) {
     if (st->soft) { st->done = 1; st->err = -code; }
     else { st->done = 2; st->err = code; }
@@ -219,6 +223,194 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_f(const T* r, const T* z, cons
     if (blockIdx.x == 0 && threadIdx.x == 0) FinPcgRz{st}(t);
     const T b = (T)(t[0] / st->rz_prev);
     GRID_LOOP(i, n) p[i] = zval<T, PCM>(r, z, d, i) + b * p[i];
+}
+
+// ---------------------------------------------------------------------------
+// PCG with a factorized sparse approximate inverse M = G^T G in three
+// launches per iteration instead of six (q = A p; x/r update; ||r|| tail;
+// t = G r; z = G^T t; p update):
+//   1. q = A p with the partial sums of p.q (the operator's fused product);
+//   2. k_pcg_fac_t, over G's CSR-stream blocking: the p.q tail (every block
+//      re-adds the partial sums in the one-block tail's order, block 0 stores
+//      the state), then t = G (r - alpha q) with r - alpha q formed in the
+//      gather (r and q are only read here), and per block the partial sums of
+//      ||r - alpha q||^2 and ||t||^2 = r^T M r over its rows;
+//   3. k_pcg_fac_p, over G^T's blocking: per row x += alpha p and
+//      r -= alpha q; then both tails, and unless ||r|| met the tolerance,
+//      p = G^T t + beta p.
+// x, r, t, p and r.z equal those of the six-launch sequence bit for bit
+// (same expressions, same summation order); ||r||^2 is summed over G's row
+// blocks instead of the vector kernels' grid, so ||r|| may differ in its
+// last bits.
+// ---------------------------------------------------------------------------
+template <typename T, bool PK>
+__global__ __launch_bounds__(kBlock) void k_pcg_fac_t(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                                      const uint16_t* __restrict__ cd,
+                                                      const uint32_t* __restrict__ cbase, const T* __restrict__ v,
+                                                      const uint2* __restrict__ bnd, const T* __restrict__ r,
+                                                      const T* __restrict__ q, T* __restrict__ t, KState* st,
+                                                      const double* __restrict__ pq_part, unsigned npq,
+                                                      double* __restrict__ part) {
+    constexpr int STAGE = stage_nnz<T>();
+    __shared__ T prod[STAGE];
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    const uint2 lo = bnd[bid], hi = bnd[bid + 1];
+    if (flag_load(&st->done)) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->fac_go = 0;
+        return;
+    }
+    double pq[1] = {0.0};
+    add_partials<1>(pq_part, npq, pq);
+    block_sum<1>(pq);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        FinPcgPq{st}(pq);
+        st->rz_prev = st->rz;  // beta's denominator: block 0 of k_pcg_fac_p rewrites rz
+        st->fac_go = st->done ? 0 : 1;
+    }
+    if (!st->minicg && (fabs(pq[0]) < 1e-30 || pq[0] <= 0.0)) return;
+    const T a = (T)(st->rz / pq[0]);
+    const auto gx = [r, q, a](uint32_t c) { return r[c] - a * q[c]; };
+    const uint32_t r0 = lo.x, r1 = hi.x, k0 = lo.y, k1 = hi.y;
+    const uint32_t cnt = k1 - k0;
+    const uint32_t tid = threadIdx.x;
+    double s2[2] = {0.0, 0.0};  // ||r - alpha q||^2, ||t||^2 over the block's rows
+    if (cnt <= (uint32_t)STAGE) {
+        constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
+        uint32_t rb[RPT], re[RPT];
+        T rn[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + tid + (uint32_t)j * kBlock;
+            rb[j] = re[j] = 0;
+            rn[j] = (T)0;
+            if (row < r1) {
+                rb[j] = rp[row] - k0;
+                re[j] = rp[row + 1] - k0;
+                rn[j] = gx(row);
+            }
+        }
+        if (cnt > 0) stage_products_g<T, STAGE, PK>(ci, cd, cbase, v, gx, k0, cnt, prod);
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + tid + (uint32_t)j * kBlock;
+            if (row < r1) {
+                T s = (T)0;
+                for (uint32_t k = rb[j]; k < re[j]; ++k) s += prod[k];
+                t[row] = s;
+                s2[0] += (double)rn[j] * (double)rn[j];
+                s2[1] += (double)s * (double)s;
+            }
+        }
+    } else {
+        // one row longer than the stage: strided products, block tree
+        double s = 0.0;
+        for (uint32_t k = k0 + tid; k < k1; k += kBlock) s += (double)(v[k] * gx(col_of<PK>(ci, cd, cbase, k)));
+        double tot[1] = {s};
+        block_sum<1>(tot);
+        if (tid == 0) {
+            const T tv = (T)tot[0], rv = gx(r0);
+            t[r0] = tv;
+            s2[0] = (double)rv * (double)rv;
+            s2[1] = (double)tv * (double)tv;
+        }
+    }
+    block_sum<2>(s2);
+    if (tid == 0) {
+        part[2 * bid] = s2[0];
+        part[2 * bid + 1] = s2[1];
+    }
+}
+
+// block 0 of k_pcg_fac_p: the ||r|| stopping test (FinPcgXr) and, if the
+// solve goes on, the r.z recurrence (FinPcgRz); rz_prev is left as
+// k_pcg_fac_t set it
+struct FinPcgFac {
+    KState* st;
+    __device__ void operator()(double (&t)[2]) const {
+        if (st->done) return;
+        st->normr = sqrt(t[0]);
+        if (st->normr / st->normb < st->tol) { st->done = 1; return; }
+        pcg_finish(st, t[1]);
+    }
+};
+
+template <typename T, bool PK>
+__global__ __launch_bounds__(kBlock) void k_pcg_fac_p(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                                      const uint16_t* __restrict__ cd,
+                                                      const uint32_t* __restrict__ cbase, const T* __restrict__ v,
+                                                      const uint2* __restrict__ bnd, const T* __restrict__ t, T* x,
+                                                      T* r, T* p, const T* q, KState* st,
+                                                      const double* __restrict__ fpart, unsigned nf) {
+    constexpr int STAGE = stage_nnz<T>();
+    __shared__ T prod[STAGE];
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    const uint2 lo = bnd[bid], hi = bnd[bid + 1];
+    // gated on fac_go, which only k_pcg_fac_t writes: block 0 below may set
+    // done, and a block that starts after that must still update x and r
+    if (!flag_load(&st->fac_go)) return;
+    const T a = (T)st->alpha;
+    const uint32_t r0 = lo.x, r1 = hi.x, k0 = lo.y, k1 = hi.y;
+    const uint32_t cnt = k1 - k0;
+    const uint32_t tid = threadIdx.x;
+    // the products and the rows' operands do not depend on the tails: their
+    // loads are issued first, the tails' partial sums are read meanwhile
+    if (cnt <= (uint32_t)STAGE) {
+        constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
+        uint32_t rb[RPT], re[RPT];
+        T xv[RPT], rv[RPT], pv[RPT], qv[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + tid + (uint32_t)j * kBlock;
+            rb[j] = re[j] = 0;
+            xv[j] = rv[j] = pv[j] = qv[j] = (T)0;
+            if (row < r1) {
+                rb[j] = rp[row] - k0;
+                re[j] = rp[row + 1] - k0;
+                xv[j] = x[row];
+                rv[j] = r[row];
+                pv[j] = p[row];
+                qv[j] = q[row];
+            }
+        }
+        if (cnt > 0) stage_products<T, STAGE, PK>(ci, cd, cbase, v, t, k0, cnt, prod);
+        double s2[2] = {0.0, 0.0};
+        add_partials<2>(fpart, nf, s2);
+        block_sum<2>(s2);  // its barriers also order the LDS stores of prod
+        if (blockIdx.x == 0 && tid == 0) FinPcgFac{st}(s2);
+        const bool conv = sqrt(s2[0]) / st->normb < st->tol;
+        const T b = (T)(s2[1] / st->rz_prev);
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + tid + (uint32_t)j * kBlock;
+            if (row < r1) {
+                x[row] = xv[j] + a * pv[j];
+                r[row] = rv[j] - a * qv[j];
+                if (!conv) {
+                    T s = (T)0;
+                    for (uint32_t k = rb[j]; k < re[j]; ++k) s += prod[k];
+                    p[row] = s + b * pv[j];
+                }
+            }
+        }
+    } else {
+        double s = 0.0;
+        for (uint32_t k = k0 + tid; k < k1; k += kBlock) s += (double)(v[k] * t[col_of<PK>(ci, cd, cbase, k)]);
+        double tot[1] = {s};
+        block_sum<1>(tot);
+        double s2[2] = {0.0, 0.0};
+        add_partials<2>(fpart, nf, s2);
+        block_sum<2>(s2);
+        if (blockIdx.x == 0 && tid == 0) FinPcgFac{st}(s2);
+        const bool conv = sqrt(s2[0]) / st->normb < st->tol;
+        const T b = (T)(s2[1] / st->rz_prev);
+        if (tid == 0) {
+            const T pv = p[r0];
+            x[r0] += a * pv;
+            r[r0] = r[r0] - a * q[r0];
+            if (!conv) p[r0] = (T)tot[0] + b * pv;
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -698,6 +890,24 @@ KrylovSolver::KrylovSolver(LinOp* A, Precond* M, const fem_solve_opts& o, hipStr
         pc_mode32_ = mode(true);
     }
     dotp_.alloc(std::max<unsigned>({1u, A_->dot_parts(), M_ ? M_->dot_parts() : 0u}));
+    // PCG with M = G^T G in three launches per iteration, where both factors
+    // take the CSR-stream kernels and the folded tails stay small (each block
+    // of the two kernels re-adds every partial sum of the kernel before).
+    // FEM_PCG_FUSED=0 in the environment keeps the six-launch form (same
+    // iterates; for comparison).
+    const char* fused_env = std::getenv("FEM_PCG_FUSED");
+    const bool fused_ok = !(fused_env && fused_env[0] == '0');
+    if (fused_ok && M && o.solver == FEM_SOLVER_PCG && M->factors(fG_, fGt_) && fG_->lanes == 0 && fGt_->lanes == 0 &&
+        fG_->n == A->rows() && fG_->m == A->rows() && fGt_->n == A->rows() && fGt_->m == A->rows()) {
+        auto fits = [&](const SpmvPlan& pg, const SpmvPlan& pgt, unsigned npq) {
+            return pg.nblk > 0 && pgt.nblk > 0 && (uint64_t)npq * pg.nblk <= kFacFold &&
+                   2ull * pg.nblk * pgt.nblk <= kFacFold;
+        };
+        fac64_ = fits(fG_->plan, fGt_->plan, A_->dot_parts());
+        if (o.precision == FEM_PREC_FP32_IR && fG_->v32.get() && fGt_->v32.get())
+            fac32_ = fits(fG_->plan32, fGt_->plan32, A_->dot_parts());
+        if (fac64_ || fac32_) facp_.alloc(2ull * std::max(fG_->plan.nblk, fG_->plan32.nblk));
+    }
 }
 
 KrylovSolver::~KrylovSolver() {}
@@ -783,6 +993,10 @@ void KrylovSolver::pcg_iter(KWork<T>& W) {
     // reduction tail then runs inside the x/r update when fold_tail allows
     // (every block re-adds the partial sums), else as its own launch
     const unsigned np = matvec_dot<T>(W.p.get(), W.q.get(), g);
+    if (np && (std::is_same<T, double>::value ? fac64_ : fac32_)) {
+        pcg_iter_fac<T>(W, np);
+        return;
+    }
     if (fold_tail(G, np)) {
         if (pm == PCM_GEN) {
             LAUNCH((k_pcg_xr_f<T, PCM_GEN>), G, W.x.get(), W.r.get(), W.p.get(), W.q.get(), d, n, st, part,
@@ -847,6 +1061,40 @@ void KrylovSolver::pcg_iter(KWork<T>& W) {
         FIN(2, FinPcgXr<PCM_NONE>{st});
         LAUNCH((k_pcg_p<T, PCM_NONE>), G, W.r.get(), W.z.get(), d, W.p.get(), n, st);
     }
+}
+
+// the rest of a PCG iteration with M = G^T G after q = A p (whose npq
+// partial sums of p.q are in dotp_): t = G (r - alpha q) into z, then the
+// x/r/p update with p = G^T t + beta p
+template <typename T>
+void KrylovSolver::pcg_iter_fac(KWork<T>& W, unsigned npq) {
+    constexpr bool D = std::is_same<T, double>::value;
+    const CsrMat& Gm = *fG_;
+    const CsrMat& Gt = *fGt_;
+    const SpmvPlan& pg = D ? Gm.plan : Gm.plan32;
+    const SpmvPlan& pgt = D ? Gt.plan : Gt.plan32;
+    const T* gv;
+    const T* gtv;
+    if constexpr (D) {
+        gv = Gm.v.get();
+        gtv = Gt.v.get();
+    } else {
+        gv = Gm.v32.get();
+        gtv = Gt.v32.get();
+    }
+    KState* st = st_.get();
+    if (Gm.cd.get())
+        LAUNCH((k_pcg_fac_t<T, true>), pg.nblk, Gm.rp.get(), Gm.ci.get(), Gm.cd.get(), Gm.cbase.get(), gv,
+               pg.bnd.get(), W.r.get(), W.q.get(), W.z.get(), st, dotp_.get(), npq, facp_.get());
+    else
+        LAUNCH((k_pcg_fac_t<T, false>), pg.nblk, Gm.rp.get(), Gm.ci.get(), Gm.cd.get(), Gm.cbase.get(), gv,
+               pg.bnd.get(), W.r.get(), W.q.get(), W.z.get(), st, dotp_.get(), npq, facp_.get());
+    if (Gt.cd.get())
+        LAUNCH((k_pcg_fac_p<T, true>), pgt.nblk, Gt.rp.get(), Gt.ci.get(), Gt.cd.get(), Gt.cbase.get(), gtv,
+               pgt.bnd.get(), W.z.get(), W.x.get(), W.r.get(), W.p.get(), W.q.get(), st, facp_.get(), pg.nblk);
+    else
+        LAUNCH((k_pcg_fac_p<T, false>), pgt.nblk, Gt.rp.get(), Gt.ci.get(), Gt.cd.get(), Gt.cbase.get(), gtv,
+               pgt.bnd.get(), W.z.get(), W.x.get(), W.r.get(), W.p.get(), W.q.get(), st, facp_.get(), pg.nblk);
 }
 
 template <typename T>

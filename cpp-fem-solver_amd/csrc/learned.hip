@@ -690,6 +690,12 @@ struct PcFactorized : Precond {
         return factorized ? two_products<float, true>(r, z, part, s, g) : csr_spmv_dot<float>(*Gm, r, z, part, s, g);
     }
     unsigned dot_parts() const override { return Gm->dot_parts(); }
+    bool factors(const CsrMat*& G, const CsrMat*& Gt) const override {
+        if (!factorized) return false;
+        G = Gm;
+        Gt = Gtm;
+        return true;
+    }
     bool has_fp32() const override { return fp32; }
     void ensure_fp32(hipStream_t s) override {
         if (fp32) return;

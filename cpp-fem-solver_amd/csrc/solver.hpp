@@ -95,6 +95,15 @@ class KrylovSolver {
     DevBuf<double> dotp_;                         // partial sums of the fused product + dot kernels
     std::unique_ptr<KrylovSolver> fb_;            // solver over the fallback preconditioner
     int pc_mode64_ = 0, pc_mode32_ = 0;  // 0 general, 1 none, 2 diag
+    // PCG with M = G^T G in three launches per iteration (pcg_iter_fac):
+    // G, G^T, whether each precision uses it, and the two-value partial sums
+    // of the t = G r kernel (one pair per block of G's SpMV blocking)
+    const CsrMat* fG_ = nullptr;
+    const CsrMat* fGt_ = nullptr;
+    bool fac64_ = false, fac32_ = false;
+    DevBuf<double> facp_;
+    template <typename T>
+    void pcg_iter_fac(KWork<T>& W, unsigned npq);
 };
 
 }  // namespace fem

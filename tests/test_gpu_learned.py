@@ -164,6 +164,50 @@ def test_explicit_spai_apply_matches_scipy():
     np.testing.assert_allclose(np.diag(Gd @ Ad @ Gd.T), 1.0, rtol=1e-12)
 
 
+def _long_row_factor(A):
+    """Lower-triangular G = D^-1/2 plus a full last row and a full first
+    column (small entries): rows of G and of G^T longer than a block's LDS
+    stage (4096 float32 / 2048 float64 products)."""
+    import scipy.sparse as sp
+    As = A.to_scipy()
+    n = As.shape[0]
+    rng = np.random.default_rng(11)
+    rows = [np.arange(n), np.full(n - 1, n - 1), np.arange(1, n - 1)]
+    cols = [np.arange(n), np.arange(n - 1), np.zeros(n - 2, dtype=np.int64)]
+    vals = [1.0 / np.sqrt(As.diagonal()), 1e-3 * rng.standard_normal(n - 1), 1e-3 * rng.standard_normal(n - 2)]
+    Gs = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    Gs.sort_indices()
+    return F.CsrMatrix(n, n, Gs.indptr, Gs.indices, Gs.data)
+
+
+@pytest.mark.parametrize("prec", [F.FP64, F.FP32_IR], ids=["fp64", "fp32_ir"])
+@pytest.mark.parametrize("case", ["fsai_vardiff24", "long_rows_poisson80"])
+def test_fused_factorized_pcg_matches_six_launch_form(monkeypatch, case, prec):
+    """PCG with M = G^T G runs three launches per iteration (t = G (r - alpha q)
+    with the p.q tail; x/r update and p = G^T t + beta p with the ||r|| and
+    r.z tails). Its iterates equal those of the six-launch form
+    (FEM_PCG_FUSED=0) bit for bit, and the solve reaches 1e-10."""
+    if case == "fsai_vardiff24":
+        Gg = gen.vardiff3d(24, 905)[0]
+        Gf = F.CsrMatrix.from_gen(Lr.fsai_exact(Gg))
+    else:
+        Gg = gen.poisson2d(80)
+        Gf = _long_row_factor(Gg)
+    A = F.CsrMatrix.from_gen(Gg)
+    _, b = gen.rhs(Gg, 4)
+    out = {}
+    for mode in ("1", "0"):
+        monkeypatch.setenv("FEM_PCG_FUSED", mode)
+        M = F.SpaiPreconditioner(A, Gf, factorized=True)
+        x, st = F.solve(A, M, b, solver="pcg", precision=prec, tol=1e-10, max_iter=20000)
+        assert st["converged"] == 1 and st["true_relres"] < 1.5e-10, (mode, st)
+        out[mode] = (x, st["iterations"])
+        M.close()
+    print(f"{case}: {out['1'][1]} iterations", flush=True)
+    assert out["1"][1] == out["0"][1]
+    assert np.array_equal(out["1"][0], out["0"][0])
+
+
 def test_learned_apply_is_spd_operator(trained):
     net, _ = trained
     G = gen.vardiff3d(10, seed=77, ell=3.0)[0]
