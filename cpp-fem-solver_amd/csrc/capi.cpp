@@ -278,7 +278,7 @@ int fem_dot(uint64_t n, const double* a, const double* b, double* out) {
         DevBuf<double> da(std::max<uint64_t>(n, 1)), db(std::max<uint64_t>(n, 1));
         da.upload(a, n, s);
         db.upload(b, n, s);
-        *out = n ? dot_sync(da.get(), db.get(), n, Reducer{red.partials.get(), red.counter.get()}, s) : 0.0;
+        *out = n ? dot_sync(da.get(), db.get(), n, red.partials.get(), s) : 0.0;
     });
 }
 

@@ -104,19 +104,18 @@ inline unsigned red_blocks(size_t n, int per_thread = 4) {
     return (unsigned)b;
 }
 
-// Scratch for two-stage deterministic reductions: per-block partials plus an
-// arrival counter that the last block resets to zero after use (so the same
-// scratch can be replayed inside a hipGraph without a memset node).
+// Scratch for two-stage deterministic reductions: the per-block partial sums
+// of stage 1, read by the stage-2 kernel (or by the blocks of a kernel that
+// folds stage 2 in)
 struct RedScratch {
     DevBuf<double> partials;  // kMaxRedBlocks * kMaxRedVals
-    DevBuf<unsigned> counter; // one per concurrent use slot
     static constexpr int kMaxRedVals = 4;
-    static constexpr int kSlots = 16;
     void init(hipStream_t s);
 };
 
-// Device-resident Krylov state shared by the kernels of one solve. Only the
-// last block of a reduction kernel (or a single-thread kernel) writes it.
+// Device-resident Krylov state shared by the kernels of one solve. It is
+// written by one thread: the one-block tail kernel of a reduction, or thread
+// 0 of block 0 of a kernel that folds the tail in.
 struct KState {
     double normb;     // ||b||
     double tol;       // relative tolerance

@@ -298,21 +298,16 @@ std::unique_ptr<Precond> make_precond(CsrMat& A, int kind, const fem_pc_params& 
 // ---------------------------------------------------------------------------
 // Vector kernels (vec.hip)
 // ---------------------------------------------------------------------------
-struct Reducer {
-    double* partials = nullptr;   // kMaxRedBlocks * kMaxRedVals doubles
-    unsigned* counters = nullptr; // RedScratch::kSlots counters
-};
-
 template <typename T>
 void vec_fill(T* y, T val, size_t n, hipStream_t s, Gate g = {});
 template <typename TD, typename TS>
 void vec_convert(TD* y, const TS* x, double scale, size_t n, hipStream_t s, Gate g = {});
-// out = a . b, synchronous helper (host result)
 // read n doubles (cache eviction without dirty lines); sink: `blocks` doubles
 void cache_flush_read(const double* buf, size_t n, double* sink, unsigned blocks, hipStream_t s);
 // reads `bytes` (a multiple of 16) with 16-byte loads, `blocks` workgroups
 void stream_read(const void* buf, size_t bytes, float* sink, unsigned blocks, hipStream_t s);
-double dot_sync(const double* a, const double* b, size_t n, const Reducer& R, hipStream_t s);
+// out = a . b, synchronous (host result); partials: kMaxRedBlocks doubles
+double dot_sync(const double* a, const double* b, size_t n, double* partials, hipStream_t s);
 // out = a . b on device pointers, enqueued on s, with no allocation: the
 // partial sums go to a static device array (vec.hip); calls that overlap on
 // one device race on it

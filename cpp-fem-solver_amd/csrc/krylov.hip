@@ -5,9 +5,11 @@
 //   bicgstab  ode-solver/src/math/array.cpp:3602-3842 (true-residual replacement)
 //   gmres     ode-solver/src/math/array.cpp:4190-4401 (right preconditioning,
 //             modified Gram-Schmidt, Givens rotations)
-// The scalars live in a device KState; every reduction ends in the last
-// block of its kernel, which also evaluates the scalar recurrences, the
-// breakdown tests and the stopping test. All kernels are gated on the state,
+// The scalars live in a device KState. A reduction writes per-block partial
+// sums; its tail (a one-block kernel, or every block of the next kernel
+// re-adding them in the same order) forms the total, and one thread evaluates
+// the scalar recurrences, the breakdown tests and the stopping test. All
+// kernels are gated on the state,
 // so a block of iterations can be captured once as a hipGraph and replayed
 // until the device reports completion; the host reads back 4 bytes per
 // replay. Reductions are deterministic (fixed grid and order).

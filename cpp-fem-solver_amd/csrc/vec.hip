@@ -48,12 +48,6 @@ const char* solver_err_msg(int code) {
 void RedScratch::init(hipStream_t s) {
     if (partials.get()) return;
     partials.alloc((size_t)kMaxRedBlocks * kMaxRedVals);
-    counter.alloc(kSlots);
-    // one-time clear outside any graph; afterwards the last block of every
-    // reduction resets its counter
-    std::vector<unsigned> z(kSlots, 0u);
-    counter.upload(z, s);
-    FEM_HIP(hipStreamSynchronize(s));
 }
 
 template <typename T>
@@ -139,11 +133,11 @@ struct FinOut {
     __device__ void operator()(double (&t)[1]) const { *out = t[0]; }
 };
 
-double dot_sync(const double* a, const double* b, size_t n, const Reducer& R, hipStream_t s) {
+double dot_sync(const double* a, const double* b, size_t n, double* partials, hipStream_t s) {
     DevBuf<double> out(1);
     const unsigned G = red_blocks(n);
-    hipLaunchKernelGGL(k_dot, dim3(G), dim3(kBlock), 0, s, a, b, n, R.partials);
-    launch_finalize<1>(R.partials, G, FinOut{out.get()}, s);
+    hipLaunchKernelGGL(k_dot, dim3(G), dim3(kBlock), 0, s, a, b, n, partials);
+    launch_finalize<1>(partials, G, FinOut{out.get()}, s);
     FEM_HIP(hipGetLastError());
     double h = 0.0;
     out.download(&h, 1, s);
