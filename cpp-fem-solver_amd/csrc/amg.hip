@@ -21,6 +21,7 @@
 // Every level starts from u = 0, as the reference's V-cycle does.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "amg.hpp"
 #include "precond.hpp"
@@ -187,6 +188,110 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint2* __restrict__ 
     }
 }
 
+// One pass over the colour-ordered form of a level (rows in colour order,
+// the off-diagonal entries O with columns given as positions, the diagonal
+// d): a block stages the products of its rows' entries in LDS like the
+// CSR-stream SpMV, then one thread per row finishes the row.
+//   MODE 0: u_t = (f_t - sum_k o_tk u_k) / d_t   Gauss-Seidel with weight 1
+//           (u_t itself is not read; rows of one colour read only other
+//           colours' entries, so blocks never read what this launch writes);
+//           DOT: part[bid] = sum of f_t u_t over the block's rows (new u_t)
+//   MODE 1: y_t = f_t - (sum_k o_tk u_k + d_t u_t)   residual
+//   MODE 2: y_t = sum_k o_tk u_k + d_t u_t           product y = A u;
+//           DOT: part[bid] = sum of u_t y_t
+template <typename T, int MODE, bool PK, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_gsp(const uint2* __restrict__ blk, const uint32_t* __restrict__ rp,
+                                                const uint32_t* __restrict__ ci, const uint16_t* __restrict__ cd,
+                                                const uint32_t* __restrict__ cbase, const T* __restrict__ v,
+                                                const T* __restrict__ d, const T* __restrict__ f, T* u,
+                                                T* __restrict__ y, Gate g, double* __restrict__ part) {
+    constexpr int STAGE = stage_nnz<T>();
+    __shared__ T prod[STAGE];
+    const uint32_t bid = xcd_block(blockIdx.x, gridDim.x);
+    const uint2 lo = blk[bid], hi = blk[bid + 1];
+    const uint32_t r0 = lo.x, r1 = hi.x, k0 = lo.y, k1 = hi.y;
+    if (g.closed()) return;
+    const uint32_t t = threadIdx.x;
+    const uint32_t cnt = k1 - k0;
+    if (cnt <= (uint32_t)STAGE) {
+        constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
+        uint32_t rb[RPT], re[RPT];
+        T fi[RPT], di[RPT], ui[RPT];
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + t + (uint32_t)j * kBlock;
+            rb[j] = re[j] = 0;
+            fi[j] = di[j] = ui[j] = (T)0;
+            if (row < r1) {
+                rb[j] = rp[row] - k0;
+                re[j] = rp[row + 1] - k0;
+                di[j] = d[row];
+                if (MODE != 2) fi[j] = f[row];
+                if (MODE != 0) ui[j] = u[row];
+            }
+        }
+        if (cnt > 0) stage_products<T, STAGE, PK>(ci, cd, cbase, v, u, k0, cnt, prod);
+        __syncthreads();
+        double dsum[1] = {0.0};
+#pragma unroll
+        for (int j = 0; j < RPT; ++j) {
+            const uint32_t row = r0 + t + (uint32_t)j * kBlock;
+            if (row < r1) {
+                T sg = (T)0;
+                for (uint32_t k = rb[j]; k < re[j]; ++k) sg += prod[k];
+                if constexpr (MODE == 0) {
+                    const T un = (fi[j] - sg) / di[j];
+                    u[row] = un;
+                    if (DOT) dsum[0] += (double)fi[j] * (double)un;
+                } else if constexpr (MODE == 1) {
+                    y[row] = fi[j] - (sg + di[j] * ui[j]);
+                } else {
+                    const T yv = sg + di[j] * ui[j];
+                    y[row] = yv;
+                    if (DOT) dsum[0] += (double)ui[j] * (double)yv;
+                }
+            }
+        }
+        if constexpr (DOT) {
+            block_sum<1>(dsum);
+            if (t == 0) part[bid] = dsum[0];
+        }
+    } else {
+        double sd = 0.0;
+        for (uint32_t k = k0 + t; k < k1; k += kBlock) sd += (double)(v[k] * u[col_of<PK>(ci, cd, cbase, k)]);
+        double tot[1] = {sd};
+        block_sum<1>(tot);
+        if (t == 0) {  // a block of one long row
+            const T sg = (T)tot[0];
+            if constexpr (MODE == 0) {
+                const T un = (f[r0] - sg) / d[r0];
+                u[r0] = un;
+                if (DOT) part[bid] = (double)f[r0] * (double)un;
+            } else if constexpr (MODE == 1) {
+                y[r0] = f[r0] - (sg + d[r0] * u[r0]);
+            } else {
+                const T yv = sg + d[r0] * u[r0];
+                y[r0] = yv;
+                if (DOT) part[bid] = (double)u[r0] * (double)yv;
+            }
+        }
+    }
+}
+
+// The start of a forward sweep from u = 0 in the colour-ordered form: the
+// first colour's rows (positions below c0_end) get u = f / d, the rows at
+// positions from z_lo on are set to zero (the colours a pass reads before
+// their own pass writes them); n is the end of the launch.
+template <typename T>
+__global__ void k_gsp_first(uint32_t c0_end, uint32_t z_lo, uint32_t n, const T* __restrict__ d,
+                            const T* __restrict__ f, T* __restrict__ u, Gate g) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    if (g.closed()) return;
+    if (t < c0_end) u[t] = f[t] / d[t];
+    else if (t >= z_lo) u[t] = (T)0;
+}
+
 // The first colour of a sweep from u = 0 and the zero start for the other
 // colours in one pass: grouped rows t < c0_end get u = w f / a_ii (the
 // Gauss-Seidel update with every neighbour still zero), the rest u = 0.
@@ -231,9 +336,50 @@ static inline dim3 g1(uint32_t n) { return dim3((n + 255) / 256); }
 // ---------------------------------------------------------------------------
 // preconditioner
 // ---------------------------------------------------------------------------
+struct PcAmg;
+// Q A Q^T on the colour-ordered form of an AMG hierarchy's finest level
+struct PermOp : LinOp {
+    AmgLevelDev* D = nullptr;
+    uint32_t rows() const override { return D->n; }
+    uint32_t cols() const override { return D->n; }
+    template <typename T>
+    unsigned run(const T* x, T* y, double* part, hipStream_t s, Gate g) const;
+    void apply(const double* x, double* y, hipStream_t s, Gate g = {}) const override { run<double>(x, y, nullptr, s, g); }
+    void apply32(const float* x, float* y, hipStream_t s, Gate g = {}) const override { run<float>(x, y, nullptr, s, g); }
+    unsigned apply_dot(const double* x, double* y, double* part, hipStream_t s, Gate g = {}) const override {
+        return run<double>(x, y, part, s, g);
+    }
+    unsigned apply32_dot(const float* x, float* y, double* part, hipStream_t s, Gate g = {}) const override {
+        return run<float>(x, y, part, s, g);
+    }
+    unsigned dot_parts() const override { return std::max(D->O.plan.nblk, D->O.plan32.nblk); }
+    void ensure_fp32(hipStream_t) override {}
+    uint64_t bytes_per_apply(bool fp32) const override {
+        const uint64_t sv = fp32 ? 4 : 8;
+        return D->O.nnz * (sv + 4) + 4ull * (D->n + 1) + 4ull * D->n * sv;  // entries, d, x, y (x gathered once)
+    }
+};
+// Q M^{-1} Q^T: the V-cycle with the finest level in colour order (float32)
+struct PermPc : Precond {
+    PcAmg* M = nullptr;
+    int kind() const override;
+    uint32_t size() const override;
+    void apply(const double*, double*, hipStream_t, Gate = {}) override {
+        fail(FEM_ERR_RUNTIME, "AMG: the colour-ordered form runs in float32 only");
+    }
+    void apply32(const float* r, float* z, hipStream_t s, Gate g = {}) override;
+    bool has_fp32() const override { return true; }
+    unsigned apply32_dot(const float* r, float* z, double* part, hipStream_t s, Gate g = {}) override;
+    unsigned dot_parts() const override;
+    double apply_bytes(int s) override;
+};
+
 struct PcAmg : Precond {
     int kind_;
     bool device_setup = false;  // hierarchy built on the device
+    uint64_t src_uid = 0;       // the matrix the hierarchy was built from (CsrMat::uid)
+    PermOp pop_;
+    PermPc ppc_;
     std::vector<std::vector<uint8_t>> splits;  // C/F: split per coarsened level
     bool inv_diag;  // SA smoother multiplies by the inverse diagonal
     int smoother, npre, npost;
@@ -324,7 +470,39 @@ struct PcAmg : Precond {
             if (k > 0) D.f32.alloc(D.n);
         }
         fp32 = true;
+        if (perm_ok()) {
+            perm_setup_device(*lv[0], s);
+            pop_.D = lv[0].get();
+            ppc_.M = this;
+        }
         FEM_HIP(hipStreamSynchronize(s));
+    }
+
+    // The float32 inner solves run on the colour-ordered form of the finest
+    // level when its Gauss-Seidel passes have weight 1 and take the residual
+    // shortcut (FEM_AMG_PERM=0 in the environment keeps A's own order, for
+    // comparison). In that order every colour is a contiguous range of the
+    // vectors (no row list, no strided access), the operator's diagonal is
+    // kept apart (the off-diagonal entries of a row all sit in other colours,
+    // so 16-bit column packing applies), the Gauss-Seidel update with weight
+    // 1 does not read the row's own old value, and the prolongation fills
+    // only the rows the post-smoothing reads before rewriting them (the last
+    // colour's rows are rewritten first).
+    bool perm_ok() const {
+        if (smoother != FEM_SMOOTH_SGS || lv.size() < 2 || npost < 1) return false;
+        const AmgLevelDev& D = *lv[0];
+        if (!D.use_gs || D.ncol < 2 || !residual_shortcut(D) || !D.Rz.n) return false;
+        const char* e = std::getenv("FEM_AMG_PERM");
+        return !(e && e[0] == '0');
+    }
+    bool permuted32(const LinOp* A, LinOp*& op, Precond*& pc, const uint32_t*& rows) override {
+        if (!lv[0]->perm || src_uid == 0) return false;
+        const CsrMat* c = dynamic_cast<const CsrMat*>(A);
+        if (!c || c->uid != src_uid) return false;
+        op = &pop_;
+        pc = &ppc_;
+        rows = lv[0]->crows.get();
+        return true;
     }
 
     template <typename T>
@@ -483,9 +661,70 @@ struct PcAmg : Precond {
 
     // part (non-null, dot_fused()): the finest level's last smoothing sweep
     // also writes the partial sums of r.z
+    // sweeps over the colour-ordered form of level D (weight 1), colours
+    // ascending or descending; `from_zero`: u has not been initialised;
+    // `part`: the last sweep's passes write the partial sums of f.u
+    void gsp(AmgLevelDev& D, const float* f, float* u, int sweeps, bool reverse, bool from_zero, hipStream_t s,
+             Gate g, double* part) {
+        const CsrMat& O = D.O;
+        const float* v = O.v32.get();
+        const float* dg = D.gd32.get();
+        for (int sw = 0; sw < sweeps; ++sw)
+            for (int cc = 0; cc < D.ncol; ++cc) {
+                const int c = reverse ? D.ncol - 1 - cc : cc;
+                if (from_zero && sw == 0 && cc == 0) {
+                    const uint32_t c0 = D.h_cptr[1];
+                    const uint32_t zlo = D.ncol > 2 ? D.h_cptr[2] : D.n;
+                    const uint32_t end = D.ncol > 2 ? D.n : c0;
+                    hipLaunchKernelGGL((k_gsp_first<float>), g1(end), dim3(256), 0, s, c0, zlo, end, dg, f, u, g);
+                    continue;
+                }
+                const uint32_t b0 = D.h_oblk32[c], b1 = D.h_oblk32[c + 1];
+                if (b0 >= b1) continue;
+                const dim3 grid(b1 - b0), block(kBlock);
+                const uint2* blk = D.oblk32.get() + b0;
+                if (part && sw == sweeps - 1) {
+                    if (O.cd.get())
+                        hipLaunchKernelGGL((k_gsp<float, 0, true, true>), grid, block, 0, s, blk, O.rp.get(),
+                                           O.ci.get(), O.cd.get(), O.cbase.get(), v, dg, f, u, (float*)nullptr, g,
+                                           part + b0);
+                    else
+                        hipLaunchKernelGGL((k_gsp<float, 0, false, true>), grid, block, 0, s, blk, O.rp.get(),
+                                           O.ci.get(), O.cd.get(), O.cbase.get(), v, dg, f, u, (float*)nullptr, g,
+                                           part + b0);
+                } else if (O.cd.get()) {
+                    hipLaunchKernelGGL((k_gsp<float, 0, true, false>), grid, block, 0, s, blk, O.rp.get(), O.ci.get(),
+                                       O.cd.get(), O.cbase.get(), v, dg, f, u, (float*)nullptr, g, (double*)nullptr);
+                } else {
+                    hipLaunchKernelGGL((k_gsp<float, 0, false, false>), grid, block, 0, s, blk, O.rp.get(),
+                                       O.ci.get(), O.cd.get(), O.cbase.get(), v, dg, f, u, (float*)nullptr, g,
+                                       (double*)nullptr);
+                }
+            }
+    }
+    // residual of every colour but the last after a forward sweep (the last
+    // colour's residual is zero and Rzp does not read it)
+    void gsp_residual(AmgLevelDev& D, const float* f, float* u, float* res, hipStream_t s, Gate g) {
+        const CsrMat& O = D.O;
+        const uint32_t nb = D.h_oblk32[D.ncol - 1];
+        if (nb == 0) return;
+        if (O.cd.get())
+            hipLaunchKernelGGL((k_gsp<float, 1, true, false>), dim3(nb), dim3(kBlock), 0, s, D.oblk32.get(),
+                               O.rp.get(), O.ci.get(), O.cd.get(), O.cbase.get(), O.v32.get(), D.gd32.get(), f, u,
+                               res, g, (double*)nullptr);
+        else
+            hipLaunchKernelGGL((k_gsp<float, 1, false, false>), dim3(nb), dim3(kBlock), 0, s, D.oblk32.get(),
+                               O.rp.get(), O.ci.get(), O.cd.get(), O.cbase.get(), O.v32.get(), D.gd32.get(), f, u,
+                               res, g, (double*)nullptr);
+    }
+
+    // `perm`: r and z are in the colour order of the finest level's
+    // colour-ordered form (float32 only)
     template <typename T>
-    void vcycle(const T* r, T* z, hipStream_t s, Gate g, double* part = nullptr) {
+    void vcycle(const T* r, T* z, hipStream_t s, Gate g, double* part = nullptr, bool perm = false) {
         const size_t L = lv.size();
+        if (perm && (!std::is_same<T, float>::value || !lv[0]->perm))
+            fail(FEM_ERR_RUNTIME, "AMG: no colour-ordered float32 form of this hierarchy");
         std::vector<T*> ucur(L, nullptr), xa(L, nullptr), xb(L, nullptr);
         std::vector<const T*> fk(L, nullptr);
         fk[0] = r;
@@ -494,6 +733,18 @@ struct PcAmg : Precond {
             Bufs<T> B = bufs<T>(D);
             const T* f = fk[k];
             T* u;
+            if constexpr (std::is_same<T, float>::value) {
+                if (perm && k == 0) {
+                    gsp(D, f, z, npre, false, true, s, g, nullptr);
+                    gsp_residual(D, f, z, B.res, s, g);
+                    AmgLevelDev& C = *lv[1];
+                    Bufs<T> BC = bufs<T>(C);
+                    csr_spmv<T>(D.Rzp, B.res, BC.f, EPI_STORE, {}, s, g);
+                    ucur[0] = z;
+                    fk[1] = BC.f;
+                    continue;
+                }
+            }
             if (smoother == FEM_SMOOTH_SGS && D.use_gs) {
                 u = (k == 0) ? z : B.u;
                 gs<T>(D, f, u, npre, false, s, g, true);
@@ -542,6 +793,13 @@ struct PcAmg : Precond {
             T* u = ucur[k];
             SpmvArgs<T> ar;
             ar.u = u;
+            if constexpr (std::is_same<T, float>::value) {
+                if (perm && k == 0) {
+                    csr_spmv<T>(D.Pp, ucur[1], u, EPI_ADD, ar, s, g);
+                    gsp(D, fk[0], u, npost, true, false, s, g, part);
+                    continue;
+                }
+            }
             csr_spmv<T>(D.P, ucur[k + 1], u, EPI_ADD, ar, s, g);
             if (smoother == FEM_SMOOTH_SGS && D.use_gs) gs<T>(D, fk[k], u, npost, true, s, g, false, k == 0 ? part : nullptr);
             else u = jacobi<T>(D, fk[k], jd(D, B), jinv(), jwt(D), u, xa[k], xb[k], npost, s, g);
@@ -578,11 +836,18 @@ struct PcAmg : Precond {
 
     // Algorithmic bytes of one V-cycle, kernel by kernel in the order vcycle
     // launches them (s-byte values, uint32 indices).
-    double apply_bytes(int s) override {
+    // (s = 4 with the colour-ordered form built: that form, the one float32
+    // inner solves run)
+    double apply_bytes(int s) override { return bytes_of(s, s == 4 && lv[0]->perm); }
+    double bytes_of(int s, bool perm) {
         const size_t L = lv.size();
         double b = 0.0;
         for (size_t k = 0; k + 1 < L; ++k) {
             AmgLevelDev& D = *lv[k];
+            if (perm && k == 0) {
+                b += gsp_bytes(D, s, npre, true) + gsp_residual_bytes(D, s) + csr_bytes(D.Rzp, s, 0);
+                continue;
+            }
             if (smoother == FEM_SMOOTH_SGS && D.use_gs) b += gs_bytes(D, s, npre, true);
             else b += npre > 0 ? jacobi_bytes(D, s, npre, true) : (double)D.n * s;
             const bool shortcut = smoother == FEM_SMOOTH_SGS && residual_shortcut(D);
@@ -592,6 +857,10 @@ struct PcAmg : Precond {
         b += (double)nco * nco * 8 + 2.0 * nco * s;  // dense coarsest operator, f, u
         for (int64_t k = (int64_t)L - 2; k >= 0; --k) {
             AmgLevelDev& D = *lv[k];
+            if (perm && k == 0) {
+                b += csr_bytes(D.Pp, s, 1) + gsp_bytes(D, s, npost, false);
+                continue;
+            }
             b += csr_bytes(D.P, s, 1);  // u += P e reads u
             b += (smoother == FEM_SMOOTH_SGS && D.use_gs) ? gs_bytes(D, s, npost, false)
                                                           : jacobi_bytes(D, s, npost, false);
@@ -639,6 +908,32 @@ struct PcAmg : Precond {
         return b;
     }
 
+    // colour-ordered passes: the colour's entries and row bounds, d, f and
+    // the new u per row, the other colours' u gathered once; the first pass
+    // from zero reads f and d and writes u for the first colour, zeros from
+    // the third colour on
+    double gsp_bytes(const AmgLevelDev& D, int s, int sweeps, bool from_zero) const {
+        double b = 0.0;
+        for (int sw = 0; sw < sweeps; ++sw)
+            for (int c = 0; c < D.ncol; ++c) {
+                const double r = (double)D.h_cptr[c + 1] - D.h_cptr[c];
+                if (from_zero && sw == 0 && c == 0) {
+                    b += 3.0 * r * s + (D.ncol > 2 ? (double)(D.n - D.h_cptr[2]) * s : 0.0);
+                    continue;
+                }
+                if (r > 0) b += (double)D.h_onnz[c] * (s + 4) + 4.0 * (r + 1) + 3.0 * r * s + (D.n - r) * s;
+            }
+        return b;
+    }
+    // residual of the colours before the last: their entries and bounds, d,
+    // f, u and the residual per row, the last colour's u gathered once
+    double gsp_residual_bytes(const AmgLevelDev& D, int s) const {
+        const double r = D.h_cptr[D.ncol - 1];
+        double z = 0.0;
+        for (int c = 0; c + 1 < D.ncol; ++c) z += (double)D.h_onnz[c];
+        return z * (s + 4) + 4.0 * (r + 1) + 4.0 * r * s + (D.n - r) * s;
+    }
+
     // the residual of every colour but the last (a zero written for each of
     // the last colour's rows unless Rz skips them): matrix part, row list,
     // u gathered once, f read and the residual written per row
@@ -649,6 +944,53 @@ struct PcAmg : Precond {
         return b;
     }
 };
+
+template <typename T>
+unsigned PermOp::run(const T* x, T* y, double* part, hipStream_t s, Gate g) const {
+    const CsrMat& O = D->O;
+    constexpr bool F64 = std::is_same<T, double>::value;
+    const SpmvPlan& plan = F64 ? O.plan : O.plan32;
+    const T* v;
+    const T* dg;
+    if constexpr (F64) {
+        v = O.v.get();
+        dg = D->gd.get();
+    } else {
+        v = O.v32.get();
+        dg = D->gd32.get();
+    }
+    if (!v || !dg || plan.nblk == 0) fail(FEM_ERR_RUNTIME, "AMG: colour-ordered operator not built");
+    const dim3 grid(plan.nblk), block(kBlock);
+    T* xu = const_cast<T*>(x);  // MODE 2 only reads it
+    if (part) {
+        if (O.cd.get())
+            hipLaunchKernelGGL((k_gsp<T, 2, true, true>), grid, block, 0, s, plan.bnd.get(), O.rp.get(), O.ci.get(),
+                               O.cd.get(), O.cbase.get(), v, dg, (const T*)nullptr, xu, y, g, part);
+        else
+            hipLaunchKernelGGL((k_gsp<T, 2, false, true>), grid, block, 0, s, plan.bnd.get(), O.rp.get(), O.ci.get(),
+                               O.cd.get(), O.cbase.get(), v, dg, (const T*)nullptr, xu, y, g, part);
+    } else {
+        if (O.cd.get())
+            hipLaunchKernelGGL((k_gsp<T, 2, true, false>), grid, block, 0, s, plan.bnd.get(), O.rp.get(), O.ci.get(),
+                               O.cd.get(), O.cbase.get(), v, dg, (const T*)nullptr, xu, y, g, (double*)nullptr);
+        else
+            hipLaunchKernelGGL((k_gsp<T, 2, false, false>), grid, block, 0, s, plan.bnd.get(), O.rp.get(),
+                               O.ci.get(), O.cd.get(), O.cbase.get(), v, dg, (const T*)nullptr, xu, y, g,
+                               (double*)nullptr);
+    }
+    FEM_HIP(hipGetLastError());
+    return part ? plan.nblk : 0;
+}
+
+int PermPc::kind() const { return M->kind(); }
+uint32_t PermPc::size() const { return M->size(); }
+void PermPc::apply32(const float* r, float* z, hipStream_t s, Gate g) { M->vcycle<float>(r, z, s, g, nullptr, true); }
+unsigned PermPc::apply32_dot(const float* r, float* z, double* part, hipStream_t s, Gate g) {
+    M->vcycle<float>(r, z, s, g, part, true);
+    return M->lv[0]->h_oblk32.back();
+}
+unsigned PermPc::dot_parts() const { return M->lv[0]->h_oblk32.back(); }
+double PermPc::apply_bytes(int s) { return M->bytes_of(s, true); }
 
 std::unique_ptr<Precond> make_amg_cf(CsrMat& A, const fem_pc_params& p, hipStream_t s) {
     if (A.n != A.m) fail(FEM_ERR_INVALID_ARGUMENT, "AMG needs a square matrix");
@@ -667,6 +1009,7 @@ std::unique_ptr<Precond> make_amg_cf(CsrMat& A, const fem_pc_params& p, hipStrea
         H = build_cf_hierarchy_device(B, p, s, &splits);
     }
     auto pc = std::unique_ptr<PcAmg>(new PcAmg(FEM_PC_AMG, std::move(H), p, s));
+    pc->src_uid = A.uid;
     pc->splits = std::move(splits);
     pc->device_setup = true;
     return pc;
@@ -687,6 +1030,7 @@ std::unique_ptr<Precond> make_saamg(CsrMat& A, const fem_pc_params& p, hipStream
     }
     tr.lap(dev ? "hierarchy (device)" : "hierarchy (host)");
     auto pc = std::unique_ptr<PcAmg>(new PcAmg(FEM_PC_SAAMG, std::move(H), p, s));
+    pc->src_uid = A.uid;
     tr.lap("levels ready");
     pc->device_setup = dev;
     return pc;

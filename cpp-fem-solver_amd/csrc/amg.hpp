@@ -64,6 +64,18 @@ struct AmgLevelDev {
     DevBuf<uint32_t> gcbase;
     DevBuf<double> gv, gd;
     DevBuf<float> gv32, gd32;
+    // Colour-ordered form of the level (float32 inner solves on the finest
+    // level): position t holds row crows[t]; O holds the off-diagonal entries
+    // of A in that row order with columns given as positions (the diagonal
+    // is gd); Pp the rows of P at positions [0, cptr[ncol-1]) (the rows the
+    // post-smoothing reads before it rewrites them); Rzp is Rz with its
+    // columns given as positions. O's float32 CSR-stream blocks per colour:
+    // colour c owns [h_oblk32[c], h_oblk32[c+1]) of oblk32.
+    bool perm = false;
+    CsrMat O, Pp, Rzp;
+    std::vector<uint32_t> h_oblk32;
+    std::vector<uint64_t> h_onnz;  // entries of O per colour
+    DevBuf<uint2> oblk32;
 };
 
 // A hierarchy on the device, ready for the V-cycle (smoother data aside).
@@ -96,6 +108,9 @@ void sgs_setup_device(AmgLevelDev& D, hipStream_t s);
 void drop_columns_device(const CsrMat& R, const uint32_t* rows, uint32_t count, CsrMat& out, hipStream_t s);
 // the float32 sweep blocks (gblk32, h_cblk32) of a level set up by sgs_setup_device
 void sgs_blocks32(AmgLevelDev& D, hipStream_t s);
+// the colour-ordered form (O, Pp, Rzp, oblk32) of a level with the grouped
+// copy, R without the last colour's columns and float32 shadows
+void perm_setup_device(AmgLevelDev& D, hipStream_t s);
 // aggregation by a distance-2 maximal independent set of the strength graph
 // on the device: v holds the entries (norms = false) or one norm per entry
 // (norms = true); agg (n entries) receives the aggregates, their count is

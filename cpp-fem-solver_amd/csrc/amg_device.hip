@@ -811,6 +811,146 @@ void sgs_blocks32(AmgLevelDev& D, hipStream_t s) {
     FEM_HIP(hipStreamSynchronize(s));  // blk ends with this call
 }
 
+namespace {
+__global__ void k_positions(const uint32_t* rows, uint32_t n, uint32_t* pos) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) pos[rows[t]] = t;
+}
+
+// entries of row rows[t] other than its first diagonal entry (the one the
+// smoother's diagonal holds)
+__global__ void k_offdiag_len(const uint32_t* rp, const uint32_t* ci, const uint32_t* rows, uint32_t n,
+                              uint32_t* len) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) {
+        const uint32_t i = rows[t];
+        uint32_t c = rp[i + 1] - rp[i];
+        for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
+            if (ci[p] == i) { --c; break; }
+        len[t] = c;
+    }
+    if (t == n) len[t] = 0;
+}
+
+__global__ void k_offdiag_fill(const uint32_t* rp, const uint32_t* ci, const double* v, const uint32_t* rows,
+                               const uint32_t* pos, uint32_t n, const uint32_t* op, uint32_t* oci, double* ov) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const uint32_t i = rows[t];
+    uint32_t q = op[t];
+    bool diag = false;
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
+        if (!diag && ci[p] == i) { diag = true; continue; }
+        oci[q] = pos[ci[p]];
+        ov[q] = v[p];
+        ++q;
+    }
+}
+
+// rows [0, nr) of the row-permuted copy: row t is row rows[t]
+__global__ void k_prow_len(const uint32_t* rp, const uint32_t* rows, uint32_t nr, uint32_t* len) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nr) len[t] = rp[rows[t] + 1] - rp[rows[t]];
+    if (t == nr) len[t] = 0;
+}
+
+__global__ void k_prow_fill(const uint32_t* rp, const uint32_t* ci, const double* v, const uint32_t* rows,
+                            uint32_t nr, const uint32_t* op, uint32_t* oci, double* ov) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nr) return;
+    const uint32_t i = rows[t];
+    uint32_t q = op[t];
+    for (uint32_t p = rp[i]; p < rp[i + 1]; ++p, ++q) {
+        oci[q] = ci[p];
+        ov[q] = v[p];
+    }
+}
+
+__global__ void k_map_cols(const uint32_t* ci, uint64_t nnz, const uint32_t* pos, uint32_t* out) {
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < nnz) out[k] = pos[ci[k]];
+}
+}  // namespace
+
+void perm_setup_device(AmgLevelDev& D, hipStream_t s) {
+    const uint32_t n = D.n;
+    if (!D.gp.get() || D.ncol < 2 || !D.Rz.n || D.h_cptr.size() != (size_t)D.ncol + 1)
+        fail(FEM_ERR_RUNTIME, "AMG: colour-ordered level without the smoother's grouped data");
+    const CsrMat& A = D.A;
+    DevBuf<uint32_t> pos(n);
+    hipLaunchKernelGGL(k_positions, dim3(g1(n)), dim3(256), 0, s, D.crows.get(), n, pos.get());
+    {
+        DCsr O;
+        O.n = O.m = n;
+        DevBuf<uint32_t> len(n + 1);
+        hipLaunchKernelGGL(k_offdiag_len, dim3(g1(n + 1)), dim3(256), 0, s, A.rp.get(), A.ci.get(), D.crows.get(), n,
+                           len.get());
+        O.rp.alloc(n + 1);
+        scan_u32(len.get(), O.rp.get(), (size_t)n + 1, s);
+        O.nnz = read_u32(O.rp.get() + n, s);
+        O.alloc_entries(s);
+        hipLaunchKernelGGL(k_offdiag_fill, dim3(g1(n)), dim3(256), 0, s, A.rp.get(), A.ci.get(), A.v.get(),
+                           D.crows.get(), pos.get(), n, O.rp.get(), O.ci.get(), O.v.get());
+        FEM_HIP(hipGetLastError());
+        O.to_mat(D.O, s);
+    }
+    {
+        const uint32_t nr = D.h_cptr[D.ncol - 1];
+        const CsrMat& P = D.P;
+        DCsr Q;
+        Q.n = nr;
+        Q.m = P.m;
+        DevBuf<uint32_t> len(nr + 1);
+        hipLaunchKernelGGL(k_prow_len, dim3(g1(nr + 1)), dim3(256), 0, s, P.rp.get(), D.crows.get(), nr, len.get());
+        Q.rp.alloc(nr + 1);
+        scan_u32(len.get(), Q.rp.get(), (size_t)nr + 1, s);
+        Q.nnz = read_u32(Q.rp.get() + nr, s);
+        Q.alloc_entries(s);
+        if (nr)
+            hipLaunchKernelGGL(k_prow_fill, dim3(g1(nr)), dim3(256), 0, s, P.rp.get(), P.ci.get(), P.v.get(),
+                               D.crows.get(), nr, Q.rp.get(), Q.ci.get(), Q.v.get());
+        FEM_HIP(hipGetLastError());
+        Q.to_mat(D.Pp, s);
+    }
+    {
+        const CsrMat& R = D.Rz;
+        DCsr Q;
+        Q.n = R.n;
+        Q.m = R.m;
+        Q.rp.alloc(R.n + 1);
+        FEM_HIP(hipMemcpyAsync(Q.rp.get(), R.rp.get(), (size_t)(R.n + 1) * 4, hipMemcpyDeviceToDevice, s));
+        Q.nnz = R.nnz;
+        Q.alloc_entries(s);
+        if (R.nnz) {
+            hipLaunchKernelGGL(k_map_cols, dim3(g1(R.nnz)), dim3(256), 0, s, R.ci.get(), R.nnz, pos.get(),
+                               Q.ci.get());
+            FEM_HIP(hipMemcpyAsync(Q.v.get(), R.v.get(), (size_t)R.nnz * 8, hipMemcpyDeviceToDevice, s));
+        }
+        FEM_HIP(hipGetLastError());
+        Q.to_mat(D.Rzp, s);
+    }
+    D.Pp.use_vector_rows();
+    D.Rzp.use_vector_rows();
+    D.O.ensure_fp32(s);
+    D.Pp.ensure_fp32(s);
+    D.Rzp.ensure_fp32(s);
+    std::vector<uint32_t> orp(n + 1);
+    D.O.rp.download(orp.data(), n + 1, s);
+    FEM_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> blk;
+    D.h_oblk32.assign(1, 0);
+    D.h_onnz.assign(D.ncol, 0);
+    for (int c = 0; c < D.ncol; ++c) {
+        D.h_onnz[c] = orp[D.h_cptr[c + 1]] - orp[D.h_cptr[c]];
+        plan_rows(orp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk,
+                  block_nnz(orp[D.h_cptr[c + 1]] - orp[D.h_cptr[c]], stage_nnz<float>(), 2048), stage_rows<float>());
+        D.h_oblk32.push_back((uint32_t)blk.size() - 1);
+    }
+    D.oblk32.upload(block_bounds(blk, orp.data()), s);
+    FEM_HIP(hipStreamSynchronize(s));  // blk, pos end with this call
+    D.perm = true;
+}
+
 AmgDevHierarchy upload_hierarchy(AmgHierarchy&& H, hipStream_t s) {
     AmgDevHierarchy D;
     const size_t L = H.A.size();

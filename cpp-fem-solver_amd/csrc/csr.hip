@@ -12,6 +12,7 @@
 // to the reference's sequential row sum. Rows longer than the staging area
 // get a block each and are reduced with a strided sum + block tree.
 #include <algorithm>
+#include <atomic>
 
 #include "internal.hpp"
 #include "reduce.hpp"
@@ -370,8 +371,14 @@ void CsrMat::ensure_fp32(hipStream_t s) {
     }
 }
 
+static uint64_t next_uid() {
+    static std::atomic<uint64_t> c{0};
+    return ++c;
+}
+
 void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, std::vector<uint32_t> ci_,
                       std::vector<double> v_, hipStream_t s) {
+    uid = next_uid();
     n = rows;
     m = cols;
     nnz = v_.size();
@@ -396,6 +403,7 @@ void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, s
 
 void CsrMat::adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&& rp_, DevBuf<uint32_t>&& ci_,
                    DevBuf<double>&& v_, hipStream_t s) {
+    uid = next_uid();
     n = rows;
     m = cols;
     nnz = nz;

@@ -174,6 +174,9 @@ struct CsrMat : LinOp {
     mutable std::vector<double> h_v;
     mutable bool host_valid = false;
     hipStream_t stream = nullptr;  // owning stream
+    // a fresh number each time set_host or adopt fills the matrix (user
+    // matrices are never changed after that)
+    uint64_t uid = 0;
     // 0: CSR-stream with sequential row sums, bit-exact with the reference's
     // spmv; > 0 (operators whose summation order is free: the AMG transfer
     // and coarse operators): that many lanes per row and a butterfly sum
@@ -279,6 +282,14 @@ struct Precond {
     virtual const CsrMat* amg_matrix(int /*level*/, int /*which*/) const { return nullptr; }
     // C/F AMG: the split of level l (1 C, 2 F); null past the coarsened levels
     virtual const std::vector<uint8_t>* amg_split(int /*level*/) const { return nullptr; }
+    // A float32 form of the system Q A Q^T, Q a row permutation, for inner
+    // solves: op applies Q A Q^T, pc applies Q M^{-1} Q^T, and rows[t] is the
+    // row of A at position t. True only when this preconditioner was built
+    // from A itself (the same matrix contents) and holds such a form.
+    virtual bool permuted32(const LinOp* /*A*/, LinOp*& /*op*/, Precond*& /*pc*/,
+                            const uint32_t*& /*rows*/) {
+        return false;
+    }
     double setup_ms = 0.0;
     int levels = 0;
     std::vector<uint32_t> lvl_rows;

@@ -844,6 +844,14 @@ struct FinStore {
 __global__ void k_axpy_f32(double* x, const float* d, double a, size_t n) {
     GRID_LOOP(i, n) x[i] += a * (double)d[i];
 }
+// the same with d in permuted order: x[perm[t]] += a d[t]
+__global__ void k_axpy_f32_perm(double* x, const float* d, const uint32_t* perm, double a, size_t n) {
+    GRID_LOOP(t, n) x[perm[t]] += a * (double)d[t];
+}
+// y[t] = scale * x[perm[t]] in float32
+__global__ void k_gather_f32(float* y, const double* x, const uint32_t* perm, double scale, size_t n) {
+    GRID_LOOP(t, n) y[t] = (float)(scale * x[perm[t]]);
+}
 
 // ---------------------------------------------------------------------------
 // host side
@@ -886,12 +894,23 @@ KrylovSolver::KrylovSolver(LinOp* A, Precond* M, const fem_solve_opts& o, hipStr
         return (int)PCM_GEN;
     };
     pc_mode64_ = mode(false);
+    A32_ = A_;
+    M32_ = M_;
     if (o.precision == FEM_PREC_FP32_IR) {
         A_->ensure_fp32(s);
         if (M) M_->ensure_fp32(s);
         pc_mode32_ = mode(true);
+        LinOp* op = nullptr;
+        Precond* pc = nullptr;
+        const uint32_t* rows = nullptr;
+        if (M && M->permuted32(A, op, pc, rows)) {
+            A32_ = op;
+            M32_ = pc;
+            perm_ = rows;
+        }
     }
-    dotp_.alloc(std::max<unsigned>({1u, A_->dot_parts(), M_ ? M_->dot_parts() : 0u}));
+    dotp_.alloc(std::max<unsigned>({1u, A_->dot_parts(), M_ ? M_->dot_parts() : 0u, A32_->dot_parts(),
+                                    M32_ ? M32_->dot_parts() : 0u}));
     // PCG with M = G^T G in three launches per iteration, where both factors
     // take the CSR-stream kernels and the folded tails stay small (each block
     // of the two kernels re-adds every partial sum of the kernel before).
@@ -917,22 +936,22 @@ KrylovSolver::~KrylovSolver() {}
 template <typename T>
 void KrylovSolver::precond(const T* r, T* z, Gate g) {
     if constexpr (std::is_same<T, double>::value) M_->apply(r, z, s_, g);
-    else M_->apply32(r, z, s_, g);
+    else M32_->apply32(r, z, s_, g);
 }
 template <typename T>
 void KrylovSolver::matvec(const T* x, T* y, Gate g) {
     if constexpr (std::is_same<T, double>::value) A_->apply(x, y, s_, g);
-    else A_->apply32(x, y, s_, g);
+    else A32_->apply32(x, y, s_, g);
 }
 template <typename T>
 unsigned KrylovSolver::matvec_dot(const T* x, T* y, Gate g) {
     if constexpr (std::is_same<T, double>::value) return A_->apply_dot(x, y, dotp_.get(), s_, g);
-    else return A_->apply32_dot(x, y, dotp_.get(), s_, g);
+    else return A32_->apply32_dot(x, y, dotp_.get(), s_, g);
 }
 template <typename T>
 unsigned KrylovSolver::precond_dot(const T* r, T* z, Gate g) {
     if constexpr (std::is_same<T, double>::value) return M_->apply_dot(r, z, dotp_.get(), s_, g);
-    else return M_->apply32_dot(r, z, dotp_.get(), s_, g);
+    else return M32_->apply32_dot(r, z, dotp_.get(), s_, g);
 }
 
 #define FIN(nv, ...)                                          \
@@ -1412,7 +1431,8 @@ void KrylovSolver::solve_once(const double* b, double* x, fem_solve_stats* stats
         if (normb < 1e-300) conv = true;
         while (!conv && refine < max_ref && total < opts.max_iter) {
             if (normr / normb < opts.tol) { conv = true; break; }
-            vec_convert<float, double>(w32_.b.get(), res64_.get(), 1.0 / normr, n, s_);
+            if (perm_) LAUNCH(k_gather_f32, red_blocks(n), w32_.b.get(), res64_.get(), perm_, 1.0 / normr, (size_t)n);
+            else vec_convert<float, double>(w32_.b.get(), res64_.get(), 1.0 / normr, n, s_);
             const double want = std::max(opts.inner_tol, 0.5 * opts.tol * normb / normr);
             total += run<float>(w32_, want, opts.max_iter - total, true, &S.spmv_count);
             // A breakdown of the float32 inner solve (p.Ap <= 0, r.z <= 0, ...)
@@ -1431,7 +1451,8 @@ void KrylovSolver::solve_once(const double* b, double* x, fem_solve_stats* stats
                     fail(FEM_ERR_RUNTIME, std::string(solver_err_msg(-hst_.p->err)) + ctx);
                 }
             }
-            LAUNCH(k_axpy_f32, red_blocks(n), x64_.get(), w32_.x.get(), normr, (size_t)n);
+            if (perm_) LAUNCH(k_axpy_f32_perm, red_blocks(n), x64_.get(), w32_.x.get(), perm_, normr, (size_t)n);
+            else LAUNCH(k_axpy_f32, red_blocks(n), x64_.get(), w32_.x.get(), normr, (size_t)n);
             A_->apply(x64_.get(), ax64_.get(), s_);
             S.spmv_count += 1;
             {
@@ -1450,7 +1471,9 @@ void KrylovSolver::solve_once(const double* b, double* x, fem_solve_stats* stats
             if (!(nr < normr)) {
                 // the step made the residual larger (or NaN): take it back
                 // and keep the last good iterate and its residual norm
-                LAUNCH(k_axpy_f32, red_blocks(n), x64_.get(), w32_.x.get(), -normr, (size_t)n);
+                if (perm_)
+                    LAUNCH(k_axpy_f32_perm, red_blocks(n), x64_.get(), w32_.x.get(), perm_, -normr, (size_t)n);
+                else LAUNCH(k_axpy_f32, red_blocks(n), x64_.get(), w32_.x.get(), -normr, (size_t)n);
                 break;
             }
             if (!(nr < normr * 0.999)) { normr = nr; break; }  // stagnation in float32

@@ -76,6 +76,13 @@ class KrylovSolver {
 
     LinOp* A_;
     Precond* M_;
+    // the float32 inner solves' operator and preconditioner: A_ and M_, or
+    // the preconditioner's permuted form Q A Q^T, Q M^{-1} Q^T with perm_[t]
+    // the row of A at position t (vectors of the inner solves are then in
+    // that order)
+    LinOp* A32_ = nullptr;
+    Precond* M32_ = nullptr;
+    const uint32_t* perm_ = nullptr;
     hipStream_t s_;
     RedScratch red_;
     DevBuf<KState> st_;

@@ -402,3 +402,53 @@ def test_repeated_solve_replays_recorded_chunks(prec):
     assert s3["converged"] == 1 and s3["true_relres"] < 1.5e-10
     assert rel(x3, xo) < 1e-8
     S.close()
+
+
+def ninept2d(N):
+    """9-point Laplacian on an N x N grid (8 on the diagonal, -1 to each of the
+    eight neighbours): its greedy row-order colouring takes four colours."""
+    T = sp.diags([np.ones(N - 1), np.ones(N), np.ones(N - 1)], [-1, 0, 1])
+    A = (9.0 * sp.identity(N * N) - sp.kron(T, T)).tocsr()
+    A.sort_indices()
+    return gen.CSR(A.shape[0], A.shape[1], A.indptr.astype(np.uint32), A.indices.astype(np.uint32),
+                   A.data.astype(np.float64))
+
+
+@pytest.mark.parametrize("case", ["poisson3d", "ninept2d"])
+def test_amg_colour_ordered_inner_solve(case, monkeypatch):
+    """Float32 inner solves on the colour-ordered form of the finest level
+    (Q A Q^T with the Gauss-Seidel colours as contiguous ranges, the diagonal
+    apart, the prolongation only on the rows the post-smoothing reads) against
+    the same solve in A's own order (FEM_AMG_PERM=0) and the oracle's float64
+    solve: both converge to 1e-10, their iteration counts differ by at most 2,
+    and the solutions agree with the oracle's within the parity bar. Two
+    colours (7-point Poisson) and four (9-point 2-D Laplacian: colours from
+    the third on start from zero)."""
+    from parity import tol_x, cond_poisson
+    if case == "poisson3d":
+        G = gen.poisson3d(40)
+        bar = tol_x(G, cond=cond_poisson(40, 3))
+    else:
+        G = ninept2d(160)
+        # lambda(9 I - T (x) T) = 9 - t_i t_j, t = 1 + 2 cos(k pi / (N + 1))
+        t = 1 + 2 * np.cos(np.arange(1, 161) * np.pi / 161)
+        lam = 9 - np.outer(t, t)
+        bar = tol_x(G, cond=float(lam.max() / lam.min()))
+    _, b = gen.rhs(G, 11)
+    A = F.CsrMatrix.from_gen(G)
+    Ao = O.Csr.from_gen(G)
+    kw = dict(smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0, coarse_exact=1, fp32=1)
+    M = F.Preconditioner(A, "saamg", **kw)
+    monkeypatch.setenv("FEM_AMG_PERM", "0")
+    Mp = F.Preconditioner(A, "saamg", **kw)
+    monkeypatch.delenv("FEM_AMG_PERM")
+    x, st = F.solve(A, M, b, solver="pcg", precision=F.FP32_IR)
+    xp, stp = F.solve(A, Mp, b, solver="pcg", precision=F.FP32_IR)
+    Mo = O.Pc(Ao, O.K_SAAMG, [0.25, 1.0, 2, 2, 20, 4, 50, 0.0, 1, 1])
+    xo, _ = O.solve(Ao, 0, Mo, b)
+    for xx, ss in ((x, st), (xp, stp)):
+        assert ss["converged"] == 1 and ss["true_relres"] < 1.5e-10
+        assert rel(xx, xo) < bar
+    assert abs(st["iterations"] - stp["iterations"]) <= 2
+    # the bytes of one float32 application follow the form the inner solves run
+    assert 0 < M.apply_bytes(fp32=True) < Mp.apply_bytes(fp32=True)
