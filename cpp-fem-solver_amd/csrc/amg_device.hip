@@ -940,6 +940,9 @@ void perm_setup_device(AmgLevelDev& D, hipStream_t s) {
     std::vector<uint32_t> blk;
     D.h_oblk32.assign(1, 0);
     D.h_onnz.assign(D.ncol, 0);
+    // the blocking of the grouped copy's passes (block sizes 512, 1024,
+    // 3072 and 4096 entries and the product's plan at 2048 and 3072 entries
+    // per block measured within 1% of this at 128^3)
     for (int c = 0; c < D.ncol; ++c) {
         D.h_onnz[c] = orp[D.h_cptr[c + 1]] - orp[D.h_cptr[c]];
         plan_rows(orp.data(), D.h_cptr[c], D.h_cptr[c + 1], blk,

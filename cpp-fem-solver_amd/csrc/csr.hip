@@ -64,13 +64,25 @@ void SpmvPlan::build(const uint32_t* h_rp, uint32_t n, hipStream_t s, uint32_t m
     FEM_HIP(hipStreamSynchronize(s));  // b ends with this call
 }
 
+// the vector the product reads at column c: x[c], or for the fused first
+// two Jacobi sweeps the first sweep's value jac0(f[c], d[c])
+template <typename T, int EPI>
+__device__ __forceinline__ T spmv_x(const T* x, const SpmvArgs<T>& a, uint32_t c) {
+    if constexpr (EPI == EPI_JACOBI0) return jac0<T, false>(a.f[c], a.d[c], (T)a.w);
+    else if constexpr (EPI == EPI_JACOBI0_INV) return jac0<T, true>(a.f[c], a.d[c], (T)a.w);
+    else return x[c];
+}
+
 template <typename T, int EPI>
 __device__ __forceinline__ T spmv_epilogue(uint32_t row, T s, const SpmvArgs<T>& a) {
     if constexpr (EPI == EPI_STORE) return s;
     else if constexpr (EPI == EPI_RESID) return a.f[row] - s;
     else if constexpr (EPI == EPI_ADD) return a.u[row] + s;
     else if constexpr (EPI == EPI_JACOBI) return a.u[row] + ((T)a.w * (a.f[row] - s)) / a.d[row];
-    else return a.u[row] + (T)a.w * (a.d[row] * (a.f[row] - s));
+    else if constexpr (EPI == EPI_JACOBI_INV) return a.u[row] + (T)a.w * (a.d[row] * (a.f[row] - s));
+    else if constexpr (EPI == EPI_JACOBI0)
+        return jac0<T, false>(a.f[row], a.d[row], (T)a.w) + ((T)a.w * (a.f[row] - s)) / a.d[row];
+    else return jac0<T, true>(a.f[row], a.d[row], (T)a.w) + (T)a.w * (a.d[row] * (a.f[row] - s));
 }
 
 // Epilogue operands of one row, loaded before the block stages its products
@@ -79,16 +91,19 @@ template <typename T, int EPI>
 struct EpiIn {
     T f = (T)0, u = (T)0, d = (T)0;
     __device__ __forceinline__ void load(uint32_t row, const SpmvArgs<T>& a) {
-        if constexpr (EPI == EPI_RESID || EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV) f = a.f[row];
+        constexpr bool J0 = EPI == EPI_JACOBI0 || EPI == EPI_JACOBI0_INV;
+        if constexpr (EPI == EPI_RESID || EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV || J0) f = a.f[row];
         if constexpr (EPI == EPI_ADD || EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV) u = a.u[row];
-        if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV) d = a.d[row];
+        if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV || J0) d = a.d[row];
     }
     __device__ __forceinline__ T apply(T s, const SpmvArgs<T>& a) const {
         if constexpr (EPI == EPI_STORE) return s;
         else if constexpr (EPI == EPI_RESID) return f - s;
         else if constexpr (EPI == EPI_ADD) return u + s;
         else if constexpr (EPI == EPI_JACOBI) return u + ((T)a.w * (f - s)) / d;
-        else return u + (T)a.w * (d * (f - s));
+        else if constexpr (EPI == EPI_JACOBI_INV) return u + (T)a.w * (d * (f - s));
+        else if constexpr (EPI == EPI_JACOBI0) return jac0<T, false>(f, d, (T)a.w) + ((T)a.w * (f - s)) / d;
+        else return jac0<T, true>(f, d, (T)a.w) + (T)a.w * (d * (f - s));
     }
 };
 
@@ -129,7 +144,9 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
                 if (DOT == 1) xr[j] = x[row];
             }
         }
-        if (cnt > 0) stage_products<T, STAGE, PK>(ci, cd, cbase, v, x, k0, cnt, prod);
+        if (cnt > 0)
+            stage_products_g<T, STAGE, PK>(
+                ci, cd, cbase, v, [&](uint32_t c) { return spmv_x<T, EPI>(x, a, c); }, k0, cnt, prod);
         __syncthreads();
         double dsum[1] = {0.0};
 #pragma unroll
@@ -150,7 +167,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
         }
     } else {
         double s = 0.0;
-        for (uint32_t k = k0 + t; k < k1; k += kBlock) s += (double)(v[k] * x[col_of<PK>(ci, cd, cbase, k)]);
+        for (uint32_t k = k0 + t; k < k1; k += kBlock)
+            s += (double)(v[k] * spmv_x<T, EPI>(x, a, col_of<PK>(ci, cd, cbase, k)));
         double tot[1] = {s};
         block_sum<1>(tot);
         if (t == 0) {
@@ -206,7 +224,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv_vec(const uint32_t* __restrict_
     }
     if (g.closed()) return;
     T s = (T)0;
-    for (uint32_t k = k0 + lane; k < k1; k += LANES) s += v[k] * x[ci[k]];
+    for (uint32_t k = k0 + lane; k < k1; k += LANES) s += v[k] * spmv_x<T, EPI>(x, a, ci[k]);
     s = group_sum<LANES>(s);
     if (row < n && lane == 0) y[row] = spmv_epilogue<T, EPI>(row, s, a);
 }
@@ -220,7 +238,9 @@ void spmv_vec_launch(const CsrMat& A, const T* vals, const T* x, T* y, int epi, 
         case EPI_RESID: hipLaunchKernelGGL((k_spmv_vec<T, EPI_RESID, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
         case EPI_ADD: hipLaunchKernelGGL((k_spmv_vec<T, EPI_ADD, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
         case EPI_JACOBI: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
-        default: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI_INV, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+        case EPI_JACOBI_INV: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI_INV, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+        case EPI_JACOBI0: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI0, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+        default: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI0_INV, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
     }
 }
 
@@ -341,7 +361,9 @@ void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, 
         case EPI_RESID: spmv_stream_launch<T, EPI_RESID>(A, vals, x, y, a, s, g); break;
         case EPI_ADD: spmv_stream_launch<T, EPI_ADD>(A, vals, x, y, a, s, g); break;
         case EPI_JACOBI: spmv_stream_launch<T, EPI_JACOBI>(A, vals, x, y, a, s, g); break;
-        default: spmv_stream_launch<T, EPI_JACOBI_INV>(A, vals, x, y, a, s, g); break;
+        case EPI_JACOBI_INV: spmv_stream_launch<T, EPI_JACOBI_INV>(A, vals, x, y, a, s, g); break;
+        case EPI_JACOBI0: spmv_stream_launch<T, EPI_JACOBI0>(A, vals, x, y, a, s, g); break;
+        default: spmv_stream_launch<T, EPI_JACOBI0_INV>(A, vals, x, y, a, s, g); break;
     }
     FEM_HIP(hipGetLastError());
 }

@@ -217,8 +217,22 @@ enum SpmvEpi : int {
     EPI_RESID = 1,     // y = f - A x
     EPI_ADD = 2,       // y = u + A x
     EPI_JACOBI = 3,    // y = u + w (f - A x) / d
-    EPI_JACOBI_INV = 4 // y = u + w (dinv (f - A x))
+    EPI_JACOBI_INV = 4, // y = u + w (dinv (f - A x))
+    // the first two damped Jacobi sweeps from u = 0 in one launch: EPI_JACOBI
+    // (EPI_JACOBI_INV) with x = u = jac0(f, d) formed where it is read (the
+    // first sweep's values, bit for bit; the x argument is not read)
+    EPI_JACOBI0 = 5,
+    EPI_JACOBI0_INV = 6
 };
+
+// One damped Jacobi sweep from u = 0 at one row: f - A 0 = f, then the
+// update of u = 0 with weight w by the diagonal d (INV: d holds 1 / a_ii).
+template <typename T, bool INV>
+__device__ __forceinline__ T jac0(T f, T d, T w) {
+    const T r = f - (T)0;
+    if constexpr (INV) return (T)0 + w * (d * r);
+    else return (T)0 + (w * r) / d;
+}
 
 template <typename T>
 struct SpmvArgs {
