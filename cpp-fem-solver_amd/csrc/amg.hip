@@ -124,7 +124,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_stream(const uint2* __restrict__ 
         return;
     }
     const uint32_t cnt = k1 - k0;
-    if (cnt <= (uint32_t)STAGE) {
+    if (stage_fits<STAGE>(k0, cnt)) {
         // a block holds at most stage_rows<T>() rows: RPT per thread, whose
         // bounds and operands are loaded before the staging
         constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(kBlock) void k_gsp(const uint2* __restrict__ blk, c
     if (g.closed()) return;
     const uint32_t t = threadIdx.x;
     const uint32_t cnt = k1 - k0;
-    if (cnt <= (uint32_t)STAGE) {
+    if (stage_fits<STAGE>(k0, cnt)) {
         constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
         uint32_t rb[RPT], re[RPT];
         T fi[RPT], di[RPT], ui[RPT];

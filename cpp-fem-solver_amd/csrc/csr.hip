@@ -22,15 +22,18 @@ namespace fem {
 void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint32_t>& b, uint32_t max_nnz,
                uint32_t max_rows) {
     if (b.empty()) b.push_back(lo);
+    // a block's entries, counted from its first entry rounded down to a
+    // multiple of 4 (the staging's groups), fit max_nnz; a row that does not
+    // fit alone gets a block of its own
     uint32_t start = lo;
     while (start < hi) {
-        if (h_rp[start + 1] - h_rp[start] > max_nnz) {
+        if (stage_span(h_rp[start], h_rp[start + 1]) > max_nnz) {
             ++start;
             b.push_back(start);
             continue;
         }
         uint32_t end = start;
-        while (end < hi && end - start < max_rows && h_rp[end + 1] - h_rp[start] <= max_nnz) ++end;
+        while (end < hi && end - start < max_rows && stage_span(h_rp[start], h_rp[end + 1]) <= max_nnz) ++end;
         b.push_back(end);
         start = end;
     }
@@ -126,7 +129,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
     if (g.closed()) return;
     const uint32_t cnt = k1 - k0;
     const uint32_t t = threadIdx.x;
-    if (cnt <= (uint32_t)STAGE) {
+    if (stage_fits<STAGE>(k0, cnt)) {
         // a block holds at most stage_rows<T>() rows: RPT per thread
         constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
         uint32_t rb[RPT], re[RPT];

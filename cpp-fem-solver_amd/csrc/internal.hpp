@@ -118,39 +118,81 @@ __device__ __forceinline__ uint32_t col_of(const uint32_t* __restrict__ ci, cons
     else return ci[k];
 }
 
+// CSR-stream staging works on groups of 4 consecutive entries starting at a
+// multiple of 4: a block's entries [k0, k0 + cnt) are covered by the groups
+// from ka = k0 rounded down to a multiple of 4. A block is staged when those
+// groups fit the stage (the plans cut blocks by the same rule).
+template <int STAGE>
+__device__ __forceinline__ bool stage_fits(uint32_t k0, uint32_t cnt) {
+    return cnt + (k0 & 3u) <= (uint32_t)STAGE;
+}
+__host__ __device__ inline uint32_t stage_span(uint32_t k0, uint32_t k1) { return k1 - (k0 & ~3u); }
+
+// four consecutive values from a 16-byte-aligned address (float32: one
+// 16-byte load; float64: two)
+__device__ __forceinline__ void load4(const float* p, float (&a)[4]) {
+    const float4 w = *reinterpret_cast<const float4*>(p);
+    a[0] = w.x; a[1] = w.y; a[2] = w.z; a[3] = w.w;
+}
+__device__ __forceinline__ void load4(const double* p, double (&a)[4]) {
+    const double2 w0 = *reinterpret_cast<const double2*>(p);
+    const double2 w1 = *reinterpret_cast<const double2*>(p + 2);
+    a[0] = w0.x; a[1] = w0.y; a[2] = w1.x; a[3] = w1.y;
+}
+// the columns of entries k..k+3 (k a multiple of 4, so one 64-entry chunk):
+// one 8-byte load of packed offsets plus the chunk base, or one 16-byte load
+template <bool PK>
+__device__ __forceinline__ void cols4(const uint32_t* __restrict__ ci, const uint16_t* __restrict__ cd,
+                                      const uint32_t* __restrict__ cbase, uint32_t k, uint32_t (&c)[4]) {
+    if constexpr (PK) {
+        const uint32_t b = cbase[k >> kPackShift];
+        const ushort4 d = *reinterpret_cast<const ushort4*>(cd + k);
+        c[0] = b + d.x; c[1] = b + d.y; c[2] = b + d.z; c[3] = b + d.w;
+    } else {
+        const uint4 w = *reinterpret_cast<const uint4*>(ci + k);
+        c[0] = w.x; c[1] = w.y; c[2] = w.z; c[3] = w.w;
+    }
+}
+
 // CSR-stream staging: prod[k] = v[k0+k] * gx(col[k0+k]) for k < cnt
-// (cnt >= 1) by a block of kBlock threads, in batches of 8 entries per
-// thread: a thread issues the batch's value and column loads (indices clamped
-// to cnt - 1, so every load is in range and all of them can be in flight),
-// then its gathers gx (x[c], or a value formed from several vectors), then
-// the LDS stores of the products inside the range. (One batch of 16 in
-// float32 was slower: 28.4 -> 31.6 us for the 128^3 SpMV.)
+// (cnt >= 1, stage_fits) by a block of kBlock threads. A thread takes two
+// groups of 4 entries per batch: it issues their value and column loads (16-
+// and 8-byte loads from ka + 4 g; group indices clamped to the last group, so
+// every load is in range: the entry arrays hold 4 padding entries past nnz
+// and every entry, padding included, names a valid column), then its 8
+// gathers gx (x[c], or a value formed from several vectors), then the LDS
+// stores of the products inside [k0, k0 + cnt). Four times fewer load
+// instructions than one entry per lane and load; the products and their
+// order are unchanged.
 template <typename T, int STAGE, bool PK, class GX>
 __device__ __forceinline__ void stage_products_g(const uint32_t* __restrict__ ci, const uint16_t* __restrict__ cd,
                                                  const uint32_t* __restrict__ cbase, const T* __restrict__ v, GX gx,
                                                  uint32_t k0, uint32_t cnt, T* prod) {
-    constexpr int U = STAGE / kBlock;
-    constexpr int B = U < 8 ? U : 8;
-    static_assert(U % B == 0, "stage size");
+    constexpr int B = 2;  // groups per thread and batch
     const uint32_t t = threadIdx.x;
+    const uint32_t ka = k0 & ~3u, lead = k0 - ka;
+    const uint32_t ng = (cnt + lead + 3) >> 2;
+    for (uint32_t g0 = 0; g0 < ng; g0 += B * kBlock) {
+        T a[B][4], xv[B][4];
+        uint32_t c[B][4];
 #pragma unroll
-    for (int q0 = 0; q0 < U; q0 += B) {
-        if ((uint32_t)q0 * kBlock >= cnt) break;
-        uint32_t c[B];
-        T a[B], xv[B];
-#pragma unroll
-        for (int q = 0; q < B; ++q) {
-            const uint32_t k = k0 + min((uint32_t)(q0 + q) * kBlock + t, cnt - 1);
-            c[q] = col_of<PK>(ci, cd, cbase, k);
-            a[q] = v[k];
+        for (int b = 0; b < B; ++b) {
+            const uint32_t k = ka + 4 * min(g0 + (uint32_t)b * kBlock + t, ng - 1);
+            load4(v + k, a[b]);
+            cols4<PK>(ci, cd, cbase, k, c[b]);
         }
 #pragma unroll
-        for (int q = 0; q < B; ++q) xv[q] = gx(c[q]);
+        for (int b = 0; b < B; ++b)
 #pragma unroll
-        for (int q = 0; q < B; ++q) {
-            const uint32_t k = (uint32_t)(q0 + q) * kBlock + t;
-            if (k < cnt) prod[k] = a[q] * xv[q];
-        }
+            for (int j = 0; j < 4; ++j) xv[b][j] = gx(c[b][j]);
+#pragma unroll
+        for (int b = 0; b < B; ++b)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                // position in the block's range (wraps below 0 for the lead entries)
+                const uint32_t k = 4 * (g0 + (uint32_t)b * kBlock + t) + j - lead;
+                if (k < cnt) prod[k] = a[b][j] * xv[b][j];
+            }
     }
 }
 template <typename T, int STAGE, bool PK>

@@ -276,7 +276,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_fac_t(const uint32_t* __restrict
     const uint32_t cnt = k1 - k0;
     const uint32_t tid = threadIdx.x;
     double s2[2] = {0.0, 0.0};  // ||r - alpha q||^2, ||t||^2 over the block's rows
-    if (cnt <= (uint32_t)STAGE) {
+    if (stage_fits<STAGE>(k0, cnt)) {
         constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
         uint32_t rb[RPT], re[RPT];
         T rn[RPT];
@@ -357,7 +357,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_fac_p(const uint32_t* __restrict
     const uint32_t tid = threadIdx.x;
     // the products and the rows' operands do not depend on the tails: their
     // loads are issued first, the tails' partial sums are read meanwhile
-    if (cnt <= (uint32_t)STAGE) {
+    if (stage_fits<STAGE>(k0, cnt)) {
         constexpr int RPT = (stage_rows<T>() + kBlock - 1) / kBlock;
         uint32_t rb[RPT], re[RPT];
         T xv[RPT], rv[RPT], pv[RPT], qv[RPT];
