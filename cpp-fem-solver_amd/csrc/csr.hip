@@ -51,6 +51,11 @@ std::vector<uint2> block_bounds(const std::vector<uint32_t>& b, const uint32_t* 
     return o;
 }
 
+// minimum block count of a matrix's SpMV plans (float64 and float32): 512,
+// 2048 and 4096 measured no better at 128^3 (SA-AMG PCG) and 96^3 (learned
+// SPAI PCG, whose folded reduction tails grow with the block count)
+constexpr uint64_t kPlanMinBlocks = 1024;
+
 uint32_t block_nnz(uint64_t nnz, uint32_t stage, uint64_t min_blocks) {
     uint32_t m = stage;
     while (m > 512 && nnz / m < min_blocks) m /= 2;
@@ -387,12 +392,12 @@ void CsrMat::ensure_fp32(hipStream_t s) {
     FEM_HIP(hipMemsetAsync(v32.get() + nnz, 0, 4 * sizeof(float), s));
     vec_convert<float, double>(v32.get(), v.get(), 1.0, nnz, s);
     if (host_valid) {
-        plan32.build(h_rp.data(), n, s, block_nnz(nnz, stage_nnz<float>(), 1024), stage_rows<float>());
+        plan32.build(h_rp.data(), n, s, block_nnz(nnz, stage_nnz<float>(), kPlanMinBlocks), stage_rows<float>());
     } else {
         std::vector<uint32_t> hrp(n + 1);
         rp.download(hrp.data(), n + 1, s);
         FEM_HIP(hipStreamSynchronize(s));
-        plan32.build(hrp.data(), n, s, block_nnz(nnz, stage_nnz<float>(), 1024), stage_rows<float>());
+        plan32.build(hrp.data(), n, s, block_nnz(nnz, stage_nnz<float>(), kPlanMinBlocks), stage_rows<float>());
     }
 }
 
@@ -416,7 +421,7 @@ void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, s
     FEM_HIP(hipMemsetAsync(v.get() + nnz, 0, 4 * sizeof(double), s));
     ci.upload(ci_, s);
     v.upload(v_, s);
-    plan.build(rp_.data(), n, s, block_nnz(nnz, stage_nnz<double>(), 1024), stage_rows<double>());
+    plan.build(rp_.data(), n, s, block_nnz(nnz, stage_nnz<double>(), kPlanMinBlocks), stage_rows<double>());
     pack_columns(ci.get(), nnz, cd, cbase, s);
     h_rp = std::move(rp_);
     h_ci = std::move(ci_);
@@ -439,7 +444,7 @@ void CsrMat::adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&&
     std::vector<uint32_t> hrp(n + 1);
     rp.download(hrp.data(), n + 1, s);
     FEM_HIP(hipStreamSynchronize(s));
-    plan.build(hrp.data(), n, s, block_nnz(nnz, stage_nnz<double>(), 1024), stage_rows<double>());
+    plan.build(hrp.data(), n, s, block_nnz(nnz, stage_nnz<double>(), kPlanMinBlocks), stage_rows<double>());
     pack_columns(ci.get(), nnz, cd, cbase, s);
     h_rp.clear();
     h_ci.clear();
