@@ -440,6 +440,7 @@ struct PcAmg : Precond {
     bool has_fp32() const override { return fp32; }
     void ensure_fp32(hipStream_t s) override {
         if (fp32) return;
+        SetupTrace tr("AMG float32", s);
         for (size_t k = 0; k < lv.size(); ++k) {
             auto& D = *lv[k];
             D.A.ensure_fp32(s);
@@ -466,12 +467,14 @@ struct PcAmg : Precond {
             }
             D.u32.alloc(D.n);
             if (k > 0) D.f32.alloc(D.n);
+            tr.lap(k == 0 ? "shadows, level 0" : "shadows, level >= 1");
         }
         fp32 = true;
         if (perm_ok()) {
             perm_setup_device(*lv[0], s);
             pop_.D = lv[0].get();
             ppc_.M = this;
+            tr.lap("colour-ordered form");
         }
         FEM_HIP(hipStreamSynchronize(s));
     }
@@ -902,10 +905,7 @@ struct PcAmg : Precond {
 
     // entries of the colour-grouped matrix in grouped rows [lo, hi)
     static uint64_t grouped_nnz(const AmgLevelDev& D, uint32_t lo, uint32_t hi) {
-        uint32_t a = 0, e = 0;
-        FEM_HIP(hipMemcpy(&a, D.gp.get() + lo, sizeof(uint32_t), hipMemcpyDeviceToHost));
-        FEM_HIP(hipMemcpy(&e, D.gp.get() + hi, sizeof(uint32_t), hipMemcpyDeviceToHost));
-        return (uint64_t)e - a;
+        return (uint64_t)D.h_gp.at(hi) - D.h_gp.at(lo);  // the host copy sgs_setup_device keeps
     }
 
     // one colour pass over r rows holding z entries: the rows' matrix part

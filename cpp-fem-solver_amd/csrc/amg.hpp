@@ -57,6 +57,7 @@ struct AmgLevelDev {
     std::vector<uint32_t> h_cblk, h_cblk32;
     DevBuf<uint2> gblk, gblk32;
     DevBuf<uint32_t> gp, gci;
+    std::vector<uint32_t> h_gp;  // gp on the host (for the float32 blocking)
     DevBuf<uint16_t> gcd;     // packed gci (empty: not packable)
     // R without the columns of the last colour, whose residual is zero after
     // a forward sweep of weight 1 (rows 0 when that shortcut does not apply)

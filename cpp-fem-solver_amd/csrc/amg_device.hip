@@ -728,9 +728,10 @@ void sgs_setup_device(AmgLevelDev& D, hipStream_t s) {
                        D.crows.get(), n, D.gp.get(), D.gci.get(), D.gv.get(), D.gd.get());
     FEM_HIP(hipGetLastError());
     pack_columns(D.gci.get(), A.nnz, D.gcd, D.gcbase, s);
-    std::vector<uint32_t> gp(n + 1);
-    D.gp.download(gp.data(), n + 1, s);
+    D.h_gp.resize(n + 1);
+    D.gp.download(D.h_gp.data(), n + 1, s);
     FEM_HIP(hipStreamSynchronize(s));
+    const std::vector<uint32_t>& gp = D.h_gp;
     std::vector<uint32_t> blk;
     D.h_cblk.assign(1, 0);
     for (int c = 0; c < ncol; ++c) {
@@ -797,9 +798,8 @@ void drop_columns_device(const CsrMat& R, const uint32_t* rows, uint32_t count, 
 void sgs_blocks32(AmgLevelDev& D, hipStream_t s) {
     if (!D.gp.get() || D.h_cptr.size() < 2) return;
     const uint32_t n = D.n;
-    std::vector<uint32_t> gp(n + 1);
-    D.gp.download(gp.data(), n + 1, s);
-    FEM_HIP(hipStreamSynchronize(s));
+    const std::vector<uint32_t>& gp = D.h_gp;  // kept by sgs_setup_device
+    if (gp.size() != (size_t)n + 1) fail(FEM_ERR_RUNTIME, "AMG: grouped row pointers not on the host");
     std::vector<uint32_t> blk;
     D.h_cblk32.assign(1, 0);
     for (size_t c = 0; c + 1 < D.h_cptr.size(); ++c) {
@@ -934,9 +934,8 @@ void perm_setup_device(AmgLevelDev& D, hipStream_t s) {
     D.O.ensure_fp32(s);
     D.Pp.ensure_fp32(s);
     D.Rzp.ensure_fp32(s);
-    std::vector<uint32_t> orp(n + 1);
-    D.O.rp.download(orp.data(), n + 1, s);
-    FEM_HIP(hipStreamSynchronize(s));
+    const std::vector<uint32_t>& orp = D.O.h_rp;  // kept by adopt
+    if (!D.O.rp_host || orp.size() != (size_t)n + 1) fail(FEM_ERR_RUNTIME, "AMG: row pointers of O not on the host");
     std::vector<uint32_t> blk;
     D.h_oblk32.assign(1, 0);
     D.h_onnz.assign(D.ncol, 0);

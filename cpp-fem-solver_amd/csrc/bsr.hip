@@ -1093,6 +1093,7 @@ int fem_bsr_precond_create(fem_bsr A, int kind, const fem_pc_params* params, fem
         fem_pc_params p;
         if (params) p = *params;
         else fem_pc_params_default(kind, &p);
+        AllocScope sc(A->stream);
         auto h = std::make_unique<fem_pc_s>();
         h->stream = A->stream;
         Timer tm;
@@ -1112,6 +1113,7 @@ int fem_bsr_precond_create(fem_bsr A, int kind, const fem_pc_params* params, fem
 
 int fem_bsr_solve(fem_bsr A, fem_pc M, const double* b, double* x, const fem_solve_opts* opts, fem_solve_stats* stats) {
     return api_guard([&] {
+        AllocScope sc(A->stream);  // declared first: the solver's buffers return to the pool in stream order
         fem_solver_s h;
         fem_solver_make(&A->A, A->stream, M, opts, &h);
         fem_solver_run(&h, b, x, stats);

@@ -147,6 +147,7 @@ int fem_csr_create(uint32_t rows, uint32_t cols, const uint32_t* row_ptr, const 
         check_csr(rows, cols, row_ptr, col_idx);
         auto h = std::make_unique<fem_csr_s>();
         h->stream = new_stream();
+        AllocScope sc(h->stream);
         const uint32_t nnz = row_ptr[rows];
         h->A.set_host(rows, cols, std::vector<uint32_t>(row_ptr, row_ptr + rows + 1),
                       std::vector<uint32_t>(col_idx, col_idx + nnz), std::vector<double>(values, values + nnz),
@@ -163,6 +164,7 @@ int fem_csr_create_coo(uint32_t rows, uint32_t cols, uint64_t nnz, const double*
         if (nnz && (!values || !rowlist || !collist)) fail(FEM_ERR_INVALID_ARGUMENT, "null COO arrays");
         auto h = std::make_unique<fem_csr_s>();
         h->stream = new_stream();
+        AllocScope sc(h->stream);
         coo_to_csr_device(rows, cols, nnz, values, rowlist, collist, h->A, h->stream);
         h->red.init(h->stream);
         *out = h.release();
@@ -292,6 +294,7 @@ int fem_precond_create(fem_csr A, int kind, const fem_pc_params* params, fem_pc*
         fem_pc_params p;
         if (params) p = *params;
         else fem_pc_params_default(kind, &p);
+        AllocScope sc(A->stream);
         auto h = std::make_unique<fem_pc_s>();
         h->stream = A->stream;
         h->pc = make_precond(A->A, kind, p, A->stream);
@@ -372,6 +375,7 @@ int fem_precond_bytes(fem_pc M, int fp32, double* bytes) {
 
 int fem_solve(fem_csr A, fem_pc M, const double* b, double* x, const fem_solve_opts* opts, fem_solve_stats* stats) {
     return guard([&] {
+        AllocScope sc(A->stream);  // declared first: the solver's buffers return to the pool in stream order
         fem_solver_s h;
         fem_solver_make(&A->A, A->stream, M, opts, &h);
         fem_solver_run(&h, b, x, stats);
@@ -455,6 +459,7 @@ int fem_gmres_callback(fem_csr A, fem_host_precond_fn fn, void* user, const doub
 
 int fem_solver_create(fem_csr A, fem_pc M, const fem_solve_opts* opts, fem_solver* out) {
     return guard([&] {
+        AllocScope sc(A->stream);
         auto h = std::make_unique<fem_solver_s>();
         h->A = A;
         fem_solver_make(&A->A, A->stream, M, opts, h.get());
@@ -463,7 +468,10 @@ int fem_solver_create(fem_csr A, fem_pc M, const fem_solve_opts* opts, fem_solve
 }
 
 int fem_solver_solve(fem_solver S, const double* b, double* x, fem_solve_stats* stats) {
-    return guard([&] { fem_solver_run(S, b, x, stats); });
+    return guard([&] {
+        AllocScope sc(S->stream);
+        fem_solver_run(S, b, x, stats);
+    });
 }
 
 int fem_solver_destroy(fem_solver S) {

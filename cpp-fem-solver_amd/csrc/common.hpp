@@ -36,30 +36,74 @@ struct Error : std::runtime_error {
                                             " at " + __FILE__ + ":" + std::to_string(__LINE__)); \
     } while (0)
 
+// Stream-ordered allocation scope. Inside an AllocScope(s) every DevBuf is
+// allocated from the device's stream-ordered pool (hipMallocAsync on s), and a
+// buffer released while the same scope stream is current goes back to the pool
+// in stream order (hipFreeAsync on s): setup code that allocates and frees
+// hundreds of temporaries then neither maps and unmaps memory nor waits for the
+// device at every free. Such buffers are used on s only (the library runs all
+// of a handle's work on its stream). A buffer released elsewhere (another
+// scope, none, its stream gone) is freed with hipFree, which waits for the
+// device. Allocations while s is being captured into a graph use hipMalloc.
+inline hipStream_t& alloc_stream() {
+    thread_local hipStream_t s = nullptr;
+    return s;
+}
+// keep freed pool memory for reuse instead of returning it at every
+// synchronisation (once per device)
+void pool_keep_memory();
+struct AllocScope {
+    hipStream_t prev;
+    explicit AllocScope(hipStream_t s) : prev(alloc_stream()) {
+        pool_keep_memory();
+        alloc_stream() = s;
+    }
+    ~AllocScope() { alloc_stream() = prev; }
+    AllocScope(const AllocScope&) = delete;
+    AllocScope& operator=(const AllocScope&) = delete;
+};
+
 // Owning device buffer.
 template <typename T>
 struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
+    hipStream_t st = nullptr;  // pool memory allocated in a scope on this stream
     DevBuf() = default;
     explicit DevBuf(size_t count) { alloc(count); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DevBuf(DevBuf&& o) noexcept : p(o.p), n(o.n), st(o.st) { o.p = nullptr; o.n = 0; o.st = nullptr; }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        if (this != &o) {
+            release();
+            p = o.p; n = o.n; st = o.st;
+            o.p = nullptr; o.n = 0; o.st = nullptr;
+        }
         return *this;
     }
     ~DevBuf() { release(); }
     void alloc(size_t count) {
         release();
         n = count;
-        if (count) FEM_HIP(hipMalloc(&p, count * sizeof(T)));
+        if (!count) return;
+        hipStream_t s = alloc_stream();
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusActive;
+        if (s && hipStreamIsCapturing(s, &cs) == hipSuccess && cs == hipStreamCaptureStatusNone) {
+            FEM_HIP(hipMallocAsync(reinterpret_cast<void**>(&p), count * sizeof(T), s));
+            st = s;
+        } else {
+            FEM_HIP(hipMalloc(&p, count * sizeof(T)));
+        }
     }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            if (st && st == alloc_stream()) (void)hipFreeAsync(p, st);
+            else (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
+        st = nullptr;
     }
     void upload(const T* h, size_t count, hipStream_t s) {
         if (count > n) alloc(count);

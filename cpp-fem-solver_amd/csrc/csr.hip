@@ -25,6 +25,8 @@ void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint3
     // a block's entries, counted from its first entry rounded down to a
     // multiple of 4 (the staging's groups), fit max_nnz; a row that does not
     // fit alone gets a block of its own
+    // (the span grows with the block's last row, so the block's end is found
+    // by binary search on the row pointers)
     uint32_t start = lo;
     while (start < hi) {
         if (stage_span(h_rp[start], h_rp[start + 1]) > max_nnz) {
@@ -32,8 +34,11 @@ void plan_rows(const uint32_t* h_rp, uint32_t lo, uint32_t hi, std::vector<uint3
             b.push_back(start);
             continue;
         }
-        uint32_t end = start;
-        while (end < hi && end - start < max_rows && stage_span(h_rp[start], h_rp[end + 1]) <= max_nnz) ++end;
+        const uint64_t lim64 = (uint64_t)(h_rp[start] & ~3u) + max_nnz;
+        const uint32_t lim = lim64 > 0xffffffffull ? 0xffffffffu : (uint32_t)lim64;
+        const uint32_t emax = (uint32_t)std::min<uint64_t>(hi, (uint64_t)start + max_rows);
+        // the last end in [start + 1, emax] with rp[end] <= lim
+        const uint32_t end = (uint32_t)(std::upper_bound(h_rp + start + 1, h_rp + emax + 1, lim) - h_rp) - 1;
         b.push_back(end);
         start = end;
     }
@@ -391,7 +396,7 @@ void CsrMat::ensure_fp32(hipStream_t s) {
     v32.alloc(nnz + 4);
     FEM_HIP(hipMemsetAsync(v32.get() + nnz, 0, 4 * sizeof(float), s));
     vec_convert<float, double>(v32.get(), v.get(), 1.0, nnz, s);
-    if (host_valid) {
+    if (host_valid || rp_host) {
         plan32.build(h_rp.data(), n, s, block_nnz(nnz, stage_nnz<float>(), kPlanMinBlocks), stage_rows<float>());
     } else {
         std::vector<uint32_t> hrp(n + 1);
@@ -427,6 +432,7 @@ void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, s
     h_ci = std::move(ci_);
     h_v = std::move(v_);
     host_valid = true;
+    rp_host = true;
     v32.release();
     FEM_HIP(hipStreamSynchronize(s));
 }
@@ -446,10 +452,11 @@ void CsrMat::adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&&
     FEM_HIP(hipStreamSynchronize(s));
     plan.build(hrp.data(), n, s, block_nnz(nnz, stage_nnz<double>(), kPlanMinBlocks), stage_rows<double>());
     pack_columns(ci.get(), nnz, cd, cbase, s);
-    h_rp.clear();
+    h_rp = std::move(hrp);  // kept for later plans (rp_host)
     h_ci.clear();
     h_v.clear();
     host_valid = false;
+    rp_host = true;
     v32.release();
     lanes = 0;
     FEM_HIP(hipStreamSynchronize(s));

@@ -215,6 +215,7 @@ struct CsrMat : LinOp {
     mutable std::vector<uint32_t> h_rp, h_ci;
     mutable std::vector<double> h_v;
     mutable bool host_valid = false;
+    bool rp_host = false;  // h_rp holds the row pointers (set by set_host and adopt)
     hipStream_t stream = nullptr;  // owning stream
     // a fresh number each time set_host or adopt fills the matrix (user
     // matrices are never changed after that)

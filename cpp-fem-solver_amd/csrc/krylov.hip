@@ -897,16 +897,18 @@ KrylovSolver::KrylovSolver(LinOp* A, Precond* M, const fem_solve_opts& o, hipStr
     A32_ = A_;
     M32_ = M_;
     if (o.precision == FEM_PREC_FP32_IR) {
-        A_->ensure_fp32(s);
         if (M) M_->ensure_fp32(s);
         pc_mode32_ = mode(true);
         LinOp* op = nullptr;
         Precond* pc = nullptr;
         const uint32_t* rows = nullptr;
         if (M && M->permuted32(A, op, pc, rows)) {
+            // the inner solves never apply A itself in float32
             A32_ = op;
             M32_ = pc;
             perm_ = rows;
+        } else {
+            A_->ensure_fp32(s);
         }
     }
     dotp_.alloc(std::max<unsigned>({1u, A_->dot_parts(), M_ ? M_->dot_parts() : 0u, A32_->dot_parts(),

@@ -21,6 +21,18 @@ void ensure_device() {
     FEM_HIP(hipSetDevice(g_dev));
 }
 
+void pool_keep_memory() {
+    // the library runs on one device (ensure_device), so one flag suffices
+    static std::once_flag once;
+    std::call_once(once, [] {
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetDefaultMemPool(&pool, dev) != hipSuccess) return;
+        uint64_t keep = ~0ull;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    });
+}
+
 bool trace_setup() {
     static const bool on = [] {
         const char* e = std::getenv("FEM_TRACE_SETUP");
