@@ -52,9 +52,11 @@ inline hipStream_t& alloc_stream() {
 // keep freed pool memory for reuse instead of returning it at every
 // synchronisation (once per device)
 void pool_keep_memory();
+bool pool_enabled();  // false with FEM_POOL=0 in the environment (plain hipMalloc / hipFree)
 struct AllocScope {
     hipStream_t prev;
     explicit AllocScope(hipStream_t s) : prev(alloc_stream()) {
+        if (!pool_enabled()) return;
         pool_keep_memory();
         alloc_stream() = s;
     }

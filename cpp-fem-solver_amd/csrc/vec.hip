@@ -21,6 +21,14 @@ void ensure_device() {
     FEM_HIP(hipSetDevice(g_dev));
 }
 
+bool pool_enabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("FEM_POOL");
+        return !(e && e[0] == '0');
+    }();
+    return on;
+}
+
 void pool_keep_memory() {
     // the library runs on one device (ensure_device), so one flag suffices
     static std::once_flag once;
