@@ -8,6 +8,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -15,6 +17,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "femgpu.h"
@@ -111,7 +114,10 @@ struct DevBuf {
         if (count > n) alloc(count);
         if (count) FEM_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
     }
-    void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+    template <typename A>
+    void upload(const std::vector<T, A>& h, hipStream_t s) {
+        upload(h.data(), h.size(), s);
+    }
     void download(T* h, size_t count, hipStream_t s) const {
         if (count) FEM_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
     }
@@ -207,6 +213,54 @@ enum SolverErr : int {
 };
 
 const char* solver_err_msg(int code);
+
+// std::vector storage that resize and the size constructor leave
+// uninitialized: host setup arrays that a (parallel) pass writes in full, so
+// their pages are first touched by the threads that fill them
+template <typename T>
+struct NoInitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <typename U>
+    NoInitAlloc(const NoInitAlloc<U>&) {}
+    template <typename U>
+    void construct(U* p) noexcept {
+        ::new ((void*)p) U;
+    }
+    template <typename U, typename... Args>
+    void construct(U* p, Args&&... a) {
+        ::new ((void*)p) U(std::forward<Args>(a)...);
+    }
+};
+template <typename T>
+using HostVec = std::vector<T, NoInitAlloc<T>>;
+
+// Host-side setup loops: f(0), ..., f(count - 1) on up to 16 threads, items
+// taken in order from a shared counter. f must not throw (workers would end
+// the process); it reports failures through state the caller checks after.
+template <typename F>
+void host_parallel(uint32_t count, F&& f) {
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const unsigned nt = std::min<unsigned>(std::min(hw, 16u), count);
+    if (nt <= 1) {
+        for (uint32_t i = 0; i < count; ++i) f(i);
+        return;
+    }
+    std::atomic<uint32_t> next{0};
+    auto work = [&]() {
+        for (uint32_t i = next.fetch_add(1, std::memory_order_relaxed); i < count;
+             i = next.fetch_add(1, std::memory_order_relaxed))
+            f(i);
+    };
+    std::vector<std::thread> th;
+    th.reserve(nt - 1);
+    for (unsigned t = 1; t < nt; ++t) th.emplace_back(work);
+    work();
+    for (auto& t : th) t.join();
+}
 
 // Setup-phase timing: with FEM_TRACE_SETUP=1 in the environment, lap() waits
 // for the stream and prints "[femgpu] <scope>: <phase> <ms>" to stderr, the

@@ -52,6 +52,8 @@ constexpr uint32_t kXLead = 1;             // block sweep: steps between the fir
 constexpr int kBlkWaves = 2;               // block sweep: wavefronts per block, taking steps in turn
 constexpr uint32_t wide_rows(uint32_t kc) { return (uint32_t)wide_r(kc) * 1024u; }
 
+using Idx = HostVec<uint32_t>;  // host index arrays of the setup (filled in full before use)
+
 // ---------------------------------------------------------------------------
 // Level schedule + SELL-64 sweep layout
 // ---------------------------------------------------------------------------
@@ -120,11 +122,11 @@ struct Sweep {
     // block sweep (used when every row has at most kBlkMaxEntries entries):
     // the rows, in processing order, cut into blocks of kBlkRows; block b
     // holds global slots [b kBlkRows, ...), ordered by level within the
-    // block. Per global slot: its row, entry count, the diagonal's SELL slot
-    // and bkc entries (dependency: local slot inside the block, or
-    // kExtDep | global slot of a row of an earlier block; values and their
-    // SELL index). bstp: per block (from bsptr[b]) the first local slot of
-    // each step (a level, or 64 rows of a wider level).
+    // block. Per global slot: its row, entry count, the diagonal's position
+    // in the factor and bkc entries (dependency: local slot inside the block,
+    // or kExtDep | global slot of a row of an earlier block; values and their
+    // positions in the factor). bstp: per block (from bsptr[b]) the first
+    // local slot of each step (a level, or 64 rows of a wider level).
     bool blk = false;
     uint32_t bkc = 0, nblk = 0;
     DevBuf<uint32_t> brow, bdep, bmap, bdmap, bsptr;
@@ -134,35 +136,45 @@ struct Sweep {
 
     // rows: dependency lists per row (entries in summation order), as
     // (col, position-in-factor-values). lev: level per row.
-    void build(uint32_t n_, const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cols,
-               const std::vector<uint32_t>& pos, const std::vector<uint32_t>& lev, const std::vector<uint32_t>* diagpos,
+    void build(uint32_t n_, const Idx& rp, const Idx& cols, const Idx& pos, const Idx& lev, const Idx* diagpos,
                bool forward, hipStream_t s) {
+        SetupTrace tr(forward ? "sweep layout (forward)" : "sweep layout (backward)", s);
         n = n_;
         nlev = n ? 1 + *std::max_element(lev.begin(), lev.end()) : 0;
         h_lvl_ptr.assign(nlev + 1, 0);
         for (uint32_t i = 0; i < n; ++i) h_lvl_ptr[lev[i] + 1]++;
         for (uint32_t l = 0; l < nlev; ++l) h_lvl_ptr[l + 1] += h_lvl_ptr[l];
-        std::vector<uint32_t> order(n), off(h_lvl_ptr.begin(), h_lvl_ptr.end() - 1);
+        Idx order(n);
+        std::vector<uint32_t> off(h_lvl_ptr.begin(), h_lvl_ptr.end() - 1);
         for (uint32_t i = 0; i < n; ++i) order[off[lev[i]]++] = i;  // stable: ascending row within a level
+        tr.lap("level order");
+        ndep = n ? (uint64_t)rp[n] - rp[0] : 0;
+        uint32_t kmax = 0;
+        for (uint32_t i = 0; i < n; ++i) kmax = std::max(kmax, rp[i + 1] - rp[i]);
+        if (kmax > 65535) fail(FEM_ERR_RUNTIME, "triangular factor row too long for the sweep layout");
+        segs.clear();
+        blk = false;
+        if (n && kmax <= kBlkMaxEntries) {
+            // the block sweep covers this sweep: its own layout, values
+            // gathered straight from the factor (no level schedule)
+            build_blocks(rp, cols, pos, lev, diagpos, forward, kmax, tr, s);
+            slot_row.upload(order, s);  // level order (the numeric factorization's launches)
+            tr.lap("blocks, uploads");
+            return;
+        }
         const uint32_t ngrp = (n + 63) / 64;
         std::vector<uint32_t> g(ngrp + 1, 0);
         std::vector<uint16_t> ln(n);
-        for (uint32_t s = 0; s < n; ++s) {
-            const uint32_t i = order[s];
-            const uint32_t L = rp[i + 1] - rp[i];
-            if (L > 65535) fail(FEM_ERR_RUNTIME, "triangular factor row too long for the sweep layout");
-            ln[s] = (uint16_t)L;
-        }
+        for (uint32_t s = 0; s < n; ++s) ln[s] = (uint16_t)(rp[order[s] + 1] - rp[order[s]]);
         for (uint32_t gi = 0; gi < ngrp; ++gi) {
             uint32_t w = 0;
             for (uint32_t s = gi * 64; s < std::min(n, gi * 64 + 64); ++s) w = std::max<uint32_t>(w, ln[s]);
             g[gi + 1] = g[gi] + w * 64;
         }
         nent = g[ngrp];
-        ndep = n ? (uint64_t)rp[n] - rp[0] : 0;
-        std::vector<uint32_t> slot_of(n);
+        Idx slot_of(n);
         for (uint32_t s = 0; s < n; ++s) slot_of[order[s]] = s;
-        std::vector<uint32_t> hc(nent, 0), hm(nent, UINT32_MAX), hd(nent, 0);
+        Idx hc(nent, 0), hm(nent, UINT32_MAX), hd(nent, 0);
         for (uint32_t s = 0; s < n; ++s) {
             const uint32_t i = order[s];
             const uint32_t base = g[s >> 6] + (s & 63);
@@ -172,15 +184,9 @@ struct Sweep {
                 hm[base + e * 64] = pos[rp[i] + e];
             }
         }
-        segs.clear();
-        uint32_t kcmax = 1;
-        for (uint32_t q = 0; q < n; ++q) kcmax = std::max<uint32_t>(kcmax, ln[q]);
-        if (kcmax <= kBlkMaxEntries) {
-            // the block sweep covers this sweep: no level-schedule segments
-            build_blocks(rp, cols, lev, g, slot_of, forward, diagpos != nullptr, s);
-        } else {
-            build_segments(rp, cols, order, slot_of, g, ln, hd, kcmax, s);
-        }
+        tr.lap("level order, SELL layout");
+        if (n) build_segments(rp, cols, order, slot_of, g, ln, hd, std::max(kmax, 1u), s);
+        tr.lap("segments");
         lvl_ptr.upload(h_lvl_ptr, s);
         slot_row.upload(order, s);
         gptr.upload(g, s);
@@ -195,14 +201,14 @@ struct Sweep {
             dmap.upload(dm, s);
             diag.alloc(n);
         }
+        tr.lap("uploads");
     }
 
     // Level-schedule segments (sweeps with rows longer than the block sweep
     // holds): chains of consecutive small levels, big levels alone.
-    void build_segments(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cols,
-                        const std::vector<uint32_t>& order, const std::vector<uint32_t>& slot_of,
-                        const std::vector<uint32_t>& g, const std::vector<uint16_t>& ln,
-                        const std::vector<uint32_t>& hd, uint32_t kcmax, hipStream_t s) {
+    void build_segments(const Idx& rp, const Idx& cols, const Idx& order, const Idx& slot_of,
+                        const std::vector<uint32_t>& g, const std::vector<uint16_t>& ln, const Idx& hd, uint32_t kcmax,
+                        hipStream_t s) {
         // a chain holds only levels whose rows depend on slots at most
         // `ring` behind the level's end (the LDS ring window; the chain
         // kernel preloads the ring with the slots before the chain), so every
@@ -305,57 +311,76 @@ struct Sweep {
     // Block layout (see the fields). Within a block, rows are ordered by
     // their level in the whole sweep, so a block runs about one step behind
     // the blocks it reads from wherever the block boundaries fall; a row's
-    // entries keep their summation order.
-    void build_blocks(const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cols,
-                      const std::vector<uint32_t>& lev, const std::vector<uint32_t>& sell_g,
-                      const std::vector<uint32_t>& slot_of, bool forward, bool has_diag, hipStream_t s) {
-        blk = false;
-        uint32_t kmax = 0;
-        for (uint32_t i = 0; i < n; ++i) kmax = std::max(kmax, rp[i + 1] - rp[i]);
-        if (!n || kmax > kBlkMaxEntries) return;
+    // entries keep their summation order. Values come straight from the
+    // factor: bmap holds each entry's position in it (pos), bdmap the
+    // diagonal's (diagpos). Blocks are laid out in parallel on the host.
+    void build_blocks(const Idx& rp, const Idx& cols, const Idx& pos, const Idx& lev, const Idx* diagpos,
+                      bool forward, uint32_t kmax, SetupTrace& tr, hipStream_t s) {
         bkc = kmax <= 4 ? 4u : kmax <= 8 ? 8u : 16u;
         nblk = (n + kBlkRows - 1) / kBlkRows;
         auto row_at = [&](uint32_t q) { return forward ? q : n - 1 - q; };
         auto pos_of = [&](uint32_t i) { return forward ? i : n - 1 - i; };
-        std::vector<uint32_t> hrow(n), hdep((size_t)n * bkc, 0), hmap((size_t)n * bkc, UINT32_MAX), hdm(n),
-            gslot(n, UINT32_MAX), sp(nblk + 1, 0), cnt;
-        std::vector<uint16_t> hlen(n, 0), stp;
-        for (uint32_t b = 0; b < nblk; ++b) {
+        // every slot and entry of these is written below (gslot in step 1,
+        // the others in step 2)
+        Idx hrow(n), hdep((size_t)n * bkc), hmap((size_t)n * bkc), hdm(diagpos ? n : 0), gslot(n);
+        std::vector<uint32_t> sp(nblk + 1, 0);
+        HostVec<uint16_t> hlen(n);
+        std::vector<uint16_t> stp;
+        std::vector<uint8_t> ext(n, 0);
+        std::vector<std::vector<uint16_t>> steps(nblk);
+        std::atomic<int> later{0};
+        // 1. per block: rows by level (counting sort, stable in processing
+        //    order) and the first local slot of each step (a level's rows in
+        //    this block, 64 at a time)
+        host_parallel(nblk, [&](uint32_t b) {
             const uint32_t q0 = b * kBlkRows, q1 = std::min(n, q0 + kBlkRows);
+            uint32_t lo = UINT32_MAX, hi = 0;
             for (uint32_t q = q0; q < q1; ++q) {
                 const uint32_t i = row_at(q);
                 for (uint32_t t = rp[i]; t < rp[i + 1]; ++t)
-                    if (pos_of(cols[t]) >= q) fail(FEM_ERR_RUNTIME, "triangular sweep: a row depends on a later row");
+                    if (pos_of(cols[t]) >= q) later.store(1, std::memory_order_relaxed);
+                lo = std::min(lo, lev[i]);
+                hi = std::max(hi, lev[i]);
             }
-            // rows by level (counting sort, stable in processing order); a
-            // step is a level's rows in this block, 64 at a time
-            uint32_t lo = UINT32_MAX, hi = 0;
-            for (uint32_t q = q0; q < q1; ++q) {
-                lo = std::min(lo, lev[row_at(q)]);
-                hi = std::max(hi, lev[row_at(q)]);
-            }
-            cnt.assign(hi - lo + 2, 0);
+            std::vector<uint32_t> cnt(hi - lo + 2, 0);
             for (uint32_t q = q0; q < q1; ++q) cnt[lev[row_at(q)] - lo + 1]++;
             for (uint32_t l = 0; l + 1 < cnt.size(); ++l) cnt[l + 1] += cnt[l];
             for (uint32_t l = 0; l + 1 < cnt.size(); ++l)
-                for (uint32_t c = cnt[l]; c < cnt[l + 1]; c += 64) stp.push_back((uint16_t)c);
+                for (uint32_t c = cnt[l]; c < cnt[l + 1]; c += 64) steps[b].push_back((uint16_t)c);
             for (uint32_t q = q0; q < q1; ++q) gslot[row_at(q)] = q0 + cnt[lev[row_at(q)] - lo]++;
-            sp[b + 1] = (uint32_t)stp.size();
+        });
+        if (later.load()) fail(FEM_ERR_RUNTIME, "triangular sweep: a row depends on a later row");
+        tr.lap("block order");
+        for (uint32_t b = 0; b < nblk; ++b) {
+            sp[b + 1] = sp[b] + (uint32_t)steps[b].size();
+            stp.insert(stp.end(), steps[b].begin(), steps[b].end());
+        }
+        // 2. per block: rows, entry counts, dependencies (local slot, or
+        //    kExtDep | global slot of an earlier block's row), factor positions
+        host_parallel(nblk, [&](uint32_t b) {
+            const uint32_t q0 = b * kBlkRows, q1 = std::min(n, q0 + kBlkRows);
             for (uint32_t q = q0; q < q1; ++q) {
-                const uint32_t i = row_at(q), gs = gslot[i], sl = slot_of[i];
+                const uint32_t i = row_at(q), gs = gslot[i];
                 const uint32_t L = rp[i + 1] - rp[i];
                 hrow[gs] = i;
-                hlen[gs] = (uint16_t)(L | (hlen[gs] & kExtUse));
-                hdm[gs] = sl;
-                const uint32_t sbase = sell_g[sl >> 6] + (sl & 63);
+                hlen[gs] = (uint16_t)L;
+                if (diagpos) hdm[gs] = (*diagpos)[i];
                 for (uint32_t e = 0; e < L; ++e) {
                     const uint32_t gj = gslot[cols[rp[i] + e]];
                     hdep[(size_t)gs * bkc + e] = gj >= q0 ? gj - q0 : (kExtDep | gj);
-                    if (gj < q0) hlen[gj] |= kExtUse;  // a row of an earlier block (already laid out)
-                    hmap[(size_t)gs * bkc + e] = sbase + 64u * e;
+                    // a row of an earlier block: flag it (every writer stores 1)
+                    if (gj < q0) __atomic_store_n(&ext[gj], (uint8_t)1, __ATOMIC_RELAXED);
+                    hmap[(size_t)gs * bkc + e] = pos[rp[i] + e];
+                }
+                for (uint32_t e = L; e < bkc; ++e) {  // padding: no dependency, value 0
+                    hdep[(size_t)gs * bkc + e] = 0;
+                    hmap[(size_t)gs * bkc + e] = UINT32_MAX;
                 }
             }
-        }
+        });
+        for (uint32_t q = 0; q < n; ++q)
+            if (ext[q]) hlen[q] |= kExtUse;  // a later block reads this row
+        tr.lap("block entries");
         brow.upload(hrow, s);
         blen.upload(hlen, s);
         bdep.upload(hdep, s);
@@ -363,17 +388,15 @@ struct Sweep {
         bsptr.upload(sp, s);
         bstp.upload(stp, s);
         bval.alloc((size_t)n * bkc);
-        if (has_diag) {
+        if (diagpos) {
             bdmap.upload(hdm, s);
             bdiag.alloc(n);
         }
-        if (!tick.get()) tick.alloc(2);  // the block sweep's counters (first pair)
-        if (nsf == 0) FEM_HIP(hipMemsetAsync(tick.get(), 0, 2 * sizeof(uint32_t), s));
+        tick.alloc(2);  // the block sweep's counters
+        FEM_HIP(hipMemsetAsync(tick.get(), 0, 2 * sizeof(uint32_t), s));
         ys.alloc(n + 2);  // slot n: target of the stores of lanes without a row; n + 1: read dummy
-        if (!sink.get()) {
-            sink.alloc(1);
-            sink32.alloc(1);
-        }
+        sink.alloc(1);
+        sink32.alloc(1);
         sf_wait_ticks();
         sf_cus();
         blk = true;
@@ -416,29 +439,10 @@ __global__ void k_scatter_slots(const uint32_t* __restrict__ slot_row, uint32_t 
     if (s < s1) y[slot_row[s]] = ys[s];
 }
 
-// copy the sweep's values (and float32 copy when present) into the chain layout
+// copy the sweep's values (and float32 copy when present) into the chain
+// layout (level-schedule sweeps)
 static void densify(Sweep& S, hipStream_t s) {
-    if ((S.nsf || S.blk) && S.val32.get() && !S.ys32.get()) S.ys32.alloc(S.n + 2);
-    if (S.blk) {
-        const uint64_t ne = (uint64_t)S.n * S.bkc;
-        const unsigned G = (unsigned)((ne + 255) / 256), Gd = (S.n + 255) / 256;
-        hipLaunchKernelGGL((k_gather_dense<double>), dim3(G), dim3(256), 0, s, S.val.get(), S.bmap.get(), S.bval.get(),
-                           ne);
-        if (S.bdiag.get())
-            hipLaunchKernelGGL((k_gather_dense<double>), dim3(Gd), dim3(256), 0, s, S.diag.get(), S.bdmap.get(),
-                               S.bdiag.get(), (uint64_t)S.n);
-        if (S.val32.get()) {
-            S.bval32.alloc(ne);
-            hipLaunchKernelGGL((k_gather_dense<float>), dim3(G), dim3(256), 0, s, S.val32.get(), S.bmap.get(),
-                               S.bval32.get(), ne);
-            if (S.bdiag.get()) {
-                S.bdiag32.alloc(S.n);
-                hipLaunchKernelGGL((k_gather_dense<float>), dim3(Gd), dim3(256), 0, s, S.diag32.get(), S.bdmap.get(),
-                                   S.bdiag32.get(), (uint64_t)S.n);
-            }
-        }
-        FEM_HIP(hipGetLastError());
-    }
+    if (S.nsf && S.val32.get() && !S.ys32.get()) S.ys32.alloc(S.n + 2);
     if (!S.cent) return;
     const unsigned G = (unsigned)((S.cent + 255) / 256);
     hipLaunchKernelGGL((k_gather_dense<double>), dim3(G), dim3(256), 0, s, S.val.get(), S.cmap.get(), S.cval.get(), S.cent);
@@ -1050,15 +1054,19 @@ static void chain_launch(const Sweep& S, const Segment& sg, const T* cval, const
 }
 
 template <typename T, bool DIV>
-static void run_sweep(const Sweep& S, const T* val, const T* diag, const T* rhs, T* y, hipStream_t s, Gate g) {
-    const T* cval;
+static void run_sweep(const Sweep& S, const T* rhs, T* y, hipStream_t s, Gate g) {
+    const T *cval, *val, *diag;
     T* rslot;
     if constexpr (std::is_same<T, double>::value) {
         cval = S.cval.get();
         rslot = S.rslot.get();
+        val = S.val.get();
+        diag = DIV ? S.diag.get() : nullptr;
     } else {
         cval = S.cval32.get();
         rslot = S.rslot32.get();
+        val = S.val32.get();
+        diag = DIV ? S.diag32.get() : nullptr;
     }
     T* ys;
     if constexpr (std::is_same<T, double>::value) ys = S.ys.get();
@@ -1067,10 +1075,10 @@ static void run_sweep(const Sweep& S, const T* val, const T* diag, const T* rhs,
         const T *bv, *bd;
         if constexpr (std::is_same<T, double>::value) {
             bv = S.bval.get();
-            bd = S.bdiag.get();
+            bd = DIV ? S.bdiag.get() : nullptr;
         } else {
             bv = S.bval32.get();
-            bd = S.bdiag32.get();
+            bd = DIV ? S.bdiag32.get() : nullptr;
         }
         hipLaunchKernelGGL((k_sf_fill<T>), dim3((S.n + 255) / 256), dim3(256), 0, s, ys, 0u, S.n, g);
         if (S.bkc == 4) blk_launch<T, DIV, 4, 3>(S, bv, bd, rhs, y, ys, s, g);
@@ -1112,11 +1120,85 @@ static void run_sweep(const Sweep& S, const T* val, const T* diag, const T* rhs,
     FEM_HIP(hipGetLastError());
 }
 
+// A row-wise selection from a CSR pattern, built in parallel over chunks of
+// rows with the serial loop's result: keep(i, p) selects entry p of row i;
+// out_p receives the row pointers of the selection, size(total) is called
+// once with the number of kept entries, then emit(i, p, q) for every kept
+// entry with q its position in the selection (rows in order, entries in
+// stored order).
+template <typename R, typename Keep, typename Size, typename Emit>
+static void select_rows(uint32_t n, const R& rp, Idx& out_p, Keep keep,
+                        Size size, Emit emit) {
+    constexpr uint32_t kRowsPerChunk = 8192;
+    const uint32_t nch = std::max(1u, (n + kRowsPerChunk - 1) / kRowsPerChunk);
+    out_p.assign((size_t)n + 1, 0);
+    std::vector<uint64_t> base(nch + 1, 0);
+    host_parallel(nch, [&](uint32_t c) {
+        const uint32_t i0 = c * kRowsPerChunk, i1 = std::min(n, i0 + kRowsPerChunk);
+        uint64_t k = 0;
+        for (uint32_t i = i0; i < i1; ++i)
+            for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) k += keep(i, p) ? 1 : 0;
+        base[c + 1] = k;
+    });
+    for (uint32_t c = 0; c < nch; ++c) base[c + 1] += base[c];
+    if (base[nch] > UINT32_MAX) fail(FEM_ERR_RUNTIME, "triangular factor too large for 32-bit positions");
+    size(base[nch]);
+    host_parallel(nch, [&](uint32_t c) {
+        const uint32_t i0 = c * kRowsPerChunk, i1 = std::min(n, i0 + kRowsPerChunk);
+        uint32_t q = (uint32_t)base[c];
+        for (uint32_t i = i0; i < i1; ++i) {
+            for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
+                if (keep(i, p)) emit(i, p, q++);
+            out_p[i + 1] = q;
+        }
+    });
+}
+
+// The transposed lists of a lower-triangular selection: for every entry p of
+// row i with column j = cols[p] < i, row i is listed under j with position
+// p; each list ascending in i (then p), as a serial pass over the rows in
+// order produces. Counted and filled in parallel, each list then sorted.
+static void transpose_lists(uint32_t n, const Idx& rp, const Idx& cols, Idx& Tp, Idx& Tc, Idx& Tpos) {
+    constexpr uint32_t kRowsPerChunk = 8192;
+    const uint32_t nch = std::max(1u, (n + kRowsPerChunk - 1) / kRowsPerChunk);
+    Tp.assign((size_t)n + 1, 0);
+    host_parallel(nch, [&](uint32_t c) {
+        const uint32_t i0 = c * kRowsPerChunk, i1 = std::min(n, i0 + kRowsPerChunk);
+        for (uint32_t i = i0; i < i1; ++i)
+            for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
+                if (cols[p] < i) __atomic_fetch_add(&Tp[cols[p] + 1], 1u, __ATOMIC_RELAXED);
+    });
+    for (uint32_t i = 0; i < n; ++i) Tp[i + 1] += Tp[i];
+    Tc.resize(Tp[n]);
+    Tpos.resize(Tp[n]);
+    std::vector<uint32_t> off(Tp.begin(), Tp.end() - 1);
+    host_parallel(nch, [&](uint32_t c) {
+        const uint32_t i0 = c * kRowsPerChunk, i1 = std::min(n, i0 + kRowsPerChunk);
+        for (uint32_t i = i0; i < i1; ++i)
+            for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
+                if (cols[p] < i) {
+                    const uint32_t q = __atomic_fetch_add(&off[cols[p]], 1u, __ATOMIC_RELAXED);
+                    Tc[q] = i;
+                    Tpos[q] = p;
+                }
+    });
+    host_parallel(nch, [&](uint32_t c) {
+        const uint32_t j0 = c * kRowsPerChunk, j1 = std::min(n, j0 + kRowsPerChunk);
+        for (uint32_t j = j0; j < j1; ++j)  // insertion sort by (row, position): lists are short
+            for (uint32_t a = Tp[j] + 1; a < Tp[j + 1]; ++a)
+                for (uint32_t b = a; b > Tp[j] && (Tc[b - 1] > Tc[b] || (Tc[b - 1] == Tc[b] && Tpos[b - 1] > Tpos[b]));
+                     --b) {
+                    std::swap(Tc[b - 1], Tc[b]);
+                    std::swap(Tpos[b - 1], Tpos[b]);
+                }
+    });
+}
+
 // levels of a triangular dependency graph given per-row dependency lists;
 // forward = dependencies have smaller index (process ascending)
-static std::vector<uint32_t> levels_of(uint32_t n, const std::vector<uint32_t>& rp, const std::vector<uint32_t>& cols,
+static Idx levels_of(uint32_t n, const Idx& rp, const Idx& cols,
                                        bool forward) {
-    std::vector<uint32_t> lev(n, 0);
+    Idx lev(n);  // every row's level is written before a later row reads it
     if (forward) {
         for (uint32_t i = 0; i < n; ++i) {
             uint32_t l = 0;
@@ -1144,6 +1226,31 @@ static void to_f32(const DevBuf<double>& a, DevBuf<float>& b, uint64_t n, hipStr
     if (!n) return;
     hipLaunchKernelGGL(k_to_f32<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a.get(), b.get(), n);
     FEM_HIP(hipGetLastError());
+}
+
+// a sweep's values (and diagonal, if it divides) from the factor values fac
+static void sweep_values(Sweep& S, const double* fac, hipStream_t s) {
+    if (S.blk) {
+        gather(fac, S.bmap.get(), S.bval.get(), (uint64_t)S.n * S.bkc, s);
+        if (S.bdiag.get()) gather(fac, S.bdmap.get(), S.bdiag.get(), S.n, s);
+        return;
+    }
+    gather(fac, S.vmap.get(), S.val.get(), S.nent, s);
+    if (S.diag.get()) gather(fac, S.dmap.get(), S.diag.get(), S.n, s);
+    densify(S, s);
+}
+
+// the float32 copies of a sweep's values (and its float32 work arrays)
+static void sweep_values_f32(Sweep& S, hipStream_t s) {
+    if (S.blk) {
+        to_f32(S.bval, S.bval32, (uint64_t)S.n * S.bkc, s);
+        if (S.bdiag.get()) to_f32(S.bdiag, S.bdiag32, S.n, s);
+        S.ys32.alloc(S.n + 2);
+        return;
+    }
+    to_f32(S.val, S.val32, S.nent, s);
+    if (S.diag.get()) to_f32(S.diag, S.diag32, S.n, s);
+    densify(S, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -1195,57 +1302,56 @@ struct PcIC0 : Precond {
 
     PcIC0(CsrMat& A, hipStream_t s) : n(A.n) {
         if (A.n != A.m) fail(FEM_ERR_INVALID_ARGUMENT, "IC(0) preconditioner needs a square matrix");
+        SetupTrace tr("IC(0) setup", s);
         A.sync_host();
+        tr.lap("matrix on the host");
         const auto& rp = A.h_rp;
         const auto& ci = A.h_ci;
         // lower pattern with diagonal (array.cpp:2539-2574)
-        std::vector<uint32_t> Lp(n + 1, 0), Lc, amap, dpos(n, UINT32_MAX);
-        Lc.reserve(A.nnz / 2 + n);
-        amap.reserve(A.nnz / 2 + n);
-        for (uint32_t i = 0; i < n; ++i) {
-            for (uint32_t p = rp[i]; p < rp[i + 1]; ++p)
-                if (ci[p] <= i) {
-                    if (ci[p] == i && dpos[i] == UINT32_MAX) dpos[i] = (uint32_t)Lc.size();
-                    Lc.push_back(ci[p]);
-                    amap.push_back(p);
-                }
-            Lp[i + 1] = (uint32_t)Lc.size();
-        }
+        Idx Lp, Lc, amap, dpos(n, UINT32_MAX);
+        select_rows(
+            n, rp, Lp, [&](uint32_t i, uint32_t p) { return ci[p] <= i; },
+            [&](uint64_t k) {
+                Lc.resize(k);
+                amap.resize(k);
+            },
+            [&](uint32_t i, uint32_t p, uint32_t q) {
+                if (ci[p] == i && dpos[i] == UINT32_MAX) dpos[i] = q;
+                Lc[q] = ci[p];
+                amap[q] = p;
+            });
         // The reference stops a row's off-diagonal loop at the first column
         // >= i (array.cpp:2589); with sorted columns that is the diagonal.
         for (uint32_t i = 0; i < n; ++i)
             if (dpos[i] == UINT32_MAX) fail(FEM_ERR_RUNTIME, "IC(0) failed: the matrix lacks a diagonal entry");
         const uint64_t nL = Lc.size();
+        tr.lap("lower pattern");
         // strictly-lower dependency lists (forward) and L^T lists (backward)
-        std::vector<uint32_t> Fp(n + 1, 0), Fc, Fpos;
-        Fc.reserve(nL);
-        Fpos.reserve(nL);
-        for (uint32_t i = 0; i < n; ++i) {
-            for (uint32_t p = Lp[i]; p < dpos[i]; ++p) { Fc.push_back(Lc[p]); Fpos.push_back(p); }
-            Fp[i + 1] = (uint32_t)Fc.size();
-        }
+        Idx Fp, Fc, Fpos;
+        select_rows(
+            n, Lp, Fp, [&](uint32_t i, uint32_t p) { return p < dpos[i]; },
+            [&](uint64_t k) {
+                Fc.resize(k);
+                Fpos.resize(k);
+            },
+            [&](uint32_t, uint32_t p, uint32_t q) {
+                Fc[q] = Lc[p];
+                Fpos[q] = p;
+            });
+        tr.lap("forward lists");
         // L^T holds every strictly-lower entry of L (array.cpp:2659-2695)
-        std::vector<uint32_t> Bp(n + 1, 0);
-        for (uint32_t i = 0; i < n; ++i)
-            for (uint32_t p = Lp[i]; p < Lp[i + 1]; ++p)
-                if (Lc[p] < i) Bp[Lc[p] + 1]++;
-        for (uint32_t i = 0; i < n; ++i) Bp[i + 1] += Bp[i];
-        std::vector<uint32_t> Bc(Bp[n]), Bpos(Bp[n]);
-        {
-            std::vector<uint32_t> off(Bp.begin(), Bp.end() - 1);
-            for (uint32_t i = 0; i < n; ++i)
-                for (uint32_t p = Lp[i]; p < Lp[i + 1]; ++p)
-                    if (Lc[p] < i) {
-                        const uint32_t q = off[Lc[p]]++;
-                        Bc[q] = i;
-                        Bpos[q] = p;
-                    }
-        }
+        Idx Bp, Bc, Bpos;
+        transpose_lists(n, Lp, Lc, Bp, Bc, Bpos);
+        tr.lap("dependency lists");
+        Idx lb;  // the two sweeps' levels side by side
+        std::thread tb([&] { lb = levels_of(n, Bp, Bc, false); });
         const auto lf = levels_of(n, Fp, Fc, true);
-        const auto lb = levels_of(n, Bp, Bc, false);
+        tb.join();
+        tr.lap("levels");
         fw.build(n, Fp, Fc, Fpos, lf, &dpos, true, s);
         bw.build(n, Bp, Bc, Bpos, lb, &dpos, false, s);
         levels = (int)fw.nlev;
+        tr.lap("sweep layouts");
         // numeric factorization on the device, level by level
         DevBuf<uint32_t> dLp, dLc, dmap, ddpos;
         DevBuf<double> Lv(std::max<uint64_t>(nL, 1)), dg(n);
@@ -1267,35 +1373,29 @@ struct PcIC0 : Precond {
         FEM_HIP(hipMemcpyAsync(&hb, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
         FEM_HIP(hipStreamSynchronize(s));
         if (hb != big) fail(FEM_ERR_RUNTIME, "IC(0) failed: the matrix may not be SPD, or needs a shifted IC");
-        gather(Lv.get(), fw.vmap.get(), fw.val.get(), fw.nent, s);
-        gather(Lv.get(), bw.vmap.get(), bw.val.get(), bw.nent, s);
-        gather(Lv.get(), fw.dmap.get(), fw.diag.get(), n, s);
-        gather(Lv.get(), bw.dmap.get(), bw.diag.get(), n, s);
-        densify(fw, s);
-        densify(bw, s);
+        tr.lap("numeric factorization");
+        sweep_values(fw, Lv.get(), s);
+        sweep_values(bw, Lv.get(), s);
         y.alloc(n);
         FEM_HIP(hipStreamSynchronize(s));
+        tr.lap("sweep values");
     }
     int kind() const override { return FEM_PC_IC0; }
     uint32_t size() const override { return n; }
     void apply(const double* r, double* z, hipStream_t s, Gate g) override {
-        run_sweep<double, true>(fw, fw.val.get(), fw.diag.get(), r, y.get(), s, g);
-        run_sweep<double, true>(bw, bw.val.get(), bw.diag.get(), y.get(), z, s, g);
+        run_sweep<double, true>(fw, r, y.get(), s, g);
+        run_sweep<double, true>(bw, y.get(), z, s, g);
     }
     void apply32(const float* r, float* z, hipStream_t s, Gate g) override {
-        run_sweep<float, true>(fw, fw.val32.get(), fw.diag32.get(), r, y32.get(), s, g);
-        run_sweep<float, true>(bw, bw.val32.get(), bw.diag32.get(), y32.get(), z, s, g);
+        run_sweep<float, true>(fw, r, y32.get(), s, g);
+        run_sweep<float, true>(bw, y32.get(), z, s, g);
     }
     double apply_bytes(int s) override { return sweep_bytes(fw, s, true) + sweep_bytes(bw, s, true); }
     bool has_fp32() const override { return y32.get() != nullptr; }
     void ensure_fp32(hipStream_t s) override {
         if (y32.get()) return;
-        to_f32(fw.val, fw.val32, fw.nent, s);
-        to_f32(bw.val, bw.val32, bw.nent, s);
-        to_f32(fw.diag, fw.diag32, n, s);
-        to_f32(bw.diag, bw.diag32, n, s);
-        densify(fw, s);
-        densify(bw, s);
+        sweep_values_f32(fw, s);
+        sweep_values_f32(bw, s);
         y32.alloc(n);
     }
 };
@@ -1337,39 +1437,59 @@ struct PcILU0 : Precond {
 
     PcILU0(CsrMat& A, hipStream_t s) : n(A.n) {
         if (A.n != A.m) fail(FEM_ERR_INVALID_ARGUMENT, "ILU(0) preconditioner needs a square matrix");
+        SetupTrace tr("ILU(0) setup", s);
         A.sync_host();
+        tr.lap("matrix on the host");
         const auto& rp = A.h_rp;
         const auto& ci = A.h_ci;
-        std::vector<uint32_t> Lp(n + 1, 0), Up(n + 1, 0), Lc, Uc, Lmap, Umap, dU(n, UINT32_MAX);
-        for (uint32_t i = 0; i < n; ++i) {
-            for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) {
-                const uint32_t j = ci[p];
-                if (j < i) { Lc.push_back(j); Lmap.push_back(p); }
-                else {
-                    if (j == i) dU[i] = (uint32_t)Uc.size();
-                    Uc.push_back(j);
-                    Umap.push_back(p);
-                }
-            }
-            Lp[i + 1] = (uint32_t)Lc.size();
-            Up[i + 1] = (uint32_t)Uc.size();
-        }
+        Idx Lp, Up, Lc, Uc, Lmap, Umap, dU(n, UINT32_MAX);
+        select_rows(
+            n, rp, Lp, [&](uint32_t i, uint32_t p) { return ci[p] < i; },
+            [&](uint64_t k) {
+                Lc.resize(k);
+                Lmap.resize(k);
+            },
+            [&](uint32_t, uint32_t p, uint32_t q) {
+                Lc[q] = ci[p];
+                Lmap[q] = p;
+            });
+        select_rows(
+            n, rp, Up, [&](uint32_t i, uint32_t p) { return ci[p] >= i; },
+            [&](uint64_t k) {
+                Uc.resize(k);
+                Umap.resize(k);
+            },
+            [&](uint32_t i, uint32_t p, uint32_t q) {
+                if (ci[p] == i) dU[i] = q;
+                Uc[q] = ci[p];
+                Umap[q] = p;
+            });
         for (uint32_t i = 0; i < n; ++i)
             if (dU[i] == UINT32_MAX) fail(FEM_ERR_RUNTIME, "ILU(0) failed: the matrix lacks a diagonal entry");
         // backward dependency lists: U row without the diagonal
-        std::vector<uint32_t> Bp(n + 1, 0), Bc, Bpos;
-        for (uint32_t i = 0; i < n; ++i) {
-            for (uint32_t p = Up[i]; p < Up[i + 1]; ++p)
-                if (Uc[p] != i) { Bc.push_back(Uc[p]); Bpos.push_back(p); }
-            Bp[i + 1] = (uint32_t)Bc.size();
-        }
-        std::vector<uint32_t> Lpos(Lc.size());
+        Idx Bp, Bc, Bpos;
+        select_rows(
+            n, Up, Bp, [&](uint32_t i, uint32_t p) { return Uc[p] != i; },
+            [&](uint64_t k) {
+                Bc.resize(k);
+                Bpos.resize(k);
+            },
+            [&](uint32_t, uint32_t p, uint32_t q) {
+                Bc[q] = Uc[p];
+                Bpos[q] = p;
+            });
+        tr.lap("patterns");
+        Idx Lpos(Lc.size());
         std::iota(Lpos.begin(), Lpos.end(), 0u);
+        Idx lb;  // the two sweeps' levels side by side
+        std::thread tb([&] { lb = levels_of(n, Bp, Bc, false); });
         const auto lf = levels_of(n, Lp, Lc, true);
-        const auto lb = levels_of(n, Bp, Bc, false);
+        tb.join();
+        tr.lap("levels");
         fw.build(n, Lp, Lc, Lpos, lf, nullptr, true, s);
         bw.build(n, Bp, Bc, Bpos, lb, &dU, false, s);
         levels = (int)fw.nlev;
+        tr.lap("sweep layouts");
         DevBuf<uint32_t> dLp, dLc, dUp, dUc, ddU, dLm, dUm;
         DevBuf<double> Lv(std::max<size_t>(Lc.size(), 1)), Uv(std::max<size_t>(Uc.size(), 1));
         DevBuf<int> bad(1);
@@ -1389,33 +1509,29 @@ struct PcILU0 : Precond {
         FEM_HIP(hipMemcpyAsync(&hb, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
         FEM_HIP(hipStreamSynchronize(s));
         if (hb != big) fail(FEM_ERR_RUNTIME, "ILU(0) breakdown: U_kk close to 0, pivoting may be needed");
-        gather(Lv.get(), fw.vmap.get(), fw.val.get(), fw.nent, s);
-        gather(Uv.get(), bw.vmap.get(), bw.val.get(), bw.nent, s);
-        gather(Uv.get(), bw.dmap.get(), bw.diag.get(), n, s);
-        densify(fw, s);
-        densify(bw, s);
+        tr.lap("numeric factorization");
+        sweep_values(fw, Lv.get(), s);
+        sweep_values(bw, Uv.get(), s);
         y.alloc(n);
         FEM_HIP(hipStreamSynchronize(s));
+        tr.lap("sweep values");
     }
     int kind() const override { return FEM_PC_ILU0; }
     uint32_t size() const override { return n; }
     void apply(const double* r, double* z, hipStream_t s, Gate g) override {
-        run_sweep<double, false>(fw, fw.val.get(), nullptr, r, y.get(), s, g);
-        run_sweep<double, true>(bw, bw.val.get(), bw.diag.get(), y.get(), z, s, g);
+        run_sweep<double, false>(fw, r, y.get(), s, g);
+        run_sweep<double, true>(bw, y.get(), z, s, g);
     }
     void apply32(const float* r, float* z, hipStream_t s, Gate g) override {
-        run_sweep<float, false>(fw, fw.val32.get(), nullptr, r, y32.get(), s, g);
-        run_sweep<float, true>(bw, bw.val32.get(), bw.diag32.get(), y32.get(), z, s, g);
+        run_sweep<float, false>(fw, r, y32.get(), s, g);
+        run_sweep<float, true>(bw, y32.get(), z, s, g);
     }
     double apply_bytes(int s) override { return sweep_bytes(fw, s, false) + sweep_bytes(bw, s, true); }
     bool has_fp32() const override { return y32.get() != nullptr; }
     void ensure_fp32(hipStream_t s) override {
         if (y32.get()) return;
-        to_f32(fw.val, fw.val32, fw.nent, s);
-        to_f32(bw.val, bw.val32, bw.nent, s);
-        to_f32(bw.diag, bw.diag32, n, s);
-        densify(fw, s);
-        densify(bw, s);
+        sweep_values_f32(fw, s);
+        sweep_values_f32(bw, s);
         y32.alloc(n);
     }
 };
