@@ -396,7 +396,11 @@ struct DCsr {
     }
 };
 
-constexpr uint32_t kMaxBitmapWords = 16384;  // 64 KB of LDS: windows of 524288 columns
+// symbolic product window: 4 KB of LDS per 64-lane workgroup (windows of
+// 32768 columns), so LDS does not limit the workgroups resident per CU (the
+// 128^3 Poisson level-0 Galerkin products: 9.1 ms; 64 KB windows 17.0 ms,
+// 1 KB 9.4 ms, 256 B 13.1 ms)
+constexpr uint32_t kMaxBitmapWords = 1024;
 
 // C = A B for A (an x am) and B (am x bm) given by their arrays
 DCsr spgemm_arrays(uint32_t an, uint32_t am, const uint32_t* arp, const uint32_t* aci, const double* av, uint32_t bn,
