@@ -386,6 +386,7 @@ struct PcAmg : Precond {
     DevBuf<double> coarse;  // nc x nc
     uint32_t nco = 0;
     bool fp32 = false;
+    bool plain0_ = true;  // the finest level's grouped copy (and shadows) built
 
     PcAmg(int kind, AmgDevHierarchy&& H, const fem_pc_params& p, hipStream_t s)
         : kind_(kind), inv_diag(kind == FEM_PC_SAAMG || kind == FEM_PC_BSAAMG), smoother(p.smoother), npre(p.npre),
@@ -409,7 +410,17 @@ struct PcAmg : Precond {
                 D.u2.alloc(D.n);
                 D.res.alloc(D.n);
                 if (smoother == FEM_SMOOTH_SGS) {
-                    sgs_setup_device(D, s);
+                    // a float32 hierarchy whose finest level may take the
+                    // colour-ordered form builds that level's grouped copy
+                    // (and its float32 shadows) only when a solve or an apply
+                    // needs the level in its own order (ensure_plain)
+                    const bool lazy = k == 0 && p.fp32 && npost >= 1 && D.n > 8192 && D.omega == 1.0 && npre >= 1 &&
+                                      perm_env();
+                    sgs_setup_device(D, s, !lazy);
+                    if (lazy && D.use_gs) {
+                        if (D.ncol >= 2) plain0_ = false;
+                        else sgs_grouped_device(D, s);
+                    }
                     if (residual_shortcut(D)) {
                         const uint32_t lo = D.h_cptr[D.ncol - 1], hi = D.h_cptr[D.ncol];
                         drop_columns_device(D.R, D.crows.get() + lo, hi - lo, D.Rz, s);
@@ -438,29 +449,38 @@ struct PcAmg : Precond {
         return l >= 0 && l < (int)splits.size() ? &splits[l] : nullptr;
     }
     bool has_fp32() const override { return fp32; }
+    // float32 shadows of a level's operators in their own order (A, P, R, Rz
+    // and the grouped copy with its blocking)
+    static void plain_shadows32(AmgLevelDev& D, bool last, hipStream_t s) {
+        D.A.ensure_fp32(s);
+        if (last) return;
+        D.P.ensure_fp32(s);
+        D.R.ensure_fp32(s);
+        if (D.Rz.n) D.Rz.ensure_fp32(s);
+        if (D.gv.get()) {
+            const size_t nz = D.gv.n;
+            D.gv32.alloc(nz);
+            vec_convert<float, double>(D.gv32.get(), D.gv.get(), 1.0, nz, s);
+            sgs_blocks32(D, s);
+        }
+    }
     void ensure_fp32(hipStream_t s) override {
         if (fp32) return;
         SetupTrace tr("AMG float32", s);
         for (size_t k = 0; k < lv.size(); ++k) {
             auto& D = *lv[k];
-            D.A.ensure_fp32(s);
-            if (k + 1 < lv.size()) {
-                D.P.ensure_fp32(s);
-                D.R.ensure_fp32(s);
-                if (D.Rz.n) D.Rz.ensure_fp32(s);
+            const bool last = k + 1 == lv.size();
+            if (k > 0 || plain0_) plain_shadows32(D, last, s);
+            if (!last) {
                 D.d32.alloc(D.n);
                 vec_convert<float, double>(D.d32.get(), D.d.get(), 1.0, D.n, s);
                 if (D.diag.get()) {
                     D.diag32.alloc(D.n);
                     vec_convert<float, double>(D.diag32.get(), D.diag.get(), 1.0, D.n, s);
                 }
-                if (D.gv.get()) {
-                    const size_t nz = D.gv.n;
-                    D.gv32.alloc(nz);
-                    vec_convert<float, double>(D.gv32.get(), D.gv.get(), 1.0, nz, s);
+                if (D.gd.get()) {
                     D.gd32.alloc(D.n);
                     vec_convert<float, double>(D.gd32.get(), D.gd.get(), 1.0, D.n, s);
-                    sgs_blocks32(D, s);
                 }
                 D.u232.alloc(D.n);
                 D.res32.alloc(D.n);
@@ -470,6 +490,7 @@ struct PcAmg : Precond {
             tr.lap(k == 0 ? "shadows, level 0" : "shadows, level >= 1");
         }
         fp32 = true;
+        if (!plain0_ && !perm_ok()) ensure_plain(s);  // not expected: the form's conditions held at setup
         if (perm_ok()) {
             perm_setup_device(*lv[0], s);
             pop_.D = lv[0].get();
@@ -493,8 +514,30 @@ struct PcAmg : Precond {
         if (smoother != FEM_SMOOTH_SGS || lv.size() < 2 || npost < 1) return false;
         const AmgLevelDev& D = *lv[0];
         if (!D.use_gs || D.ncol < 2 || !residual_shortcut(D) || !D.Rz.n) return false;
+        return perm_env();
+    }
+    static bool perm_env() {
         const char* e = std::getenv("FEM_AMG_PERM");
         return !(e && e[0] == '0');
+    }
+    // the finest level in its own order (its grouped copy and, for a float32
+    // hierarchy, that level's float32 shadows), built on first need; never
+    // while the stream is being captured into a graph (a solver prepares it
+    // before it captures anything)
+    void ensure_plain(hipStream_t s) {
+        if (plain0_) return;
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        FEM_HIP(hipStreamIsCapturing(s, &cs));
+        if (cs != hipStreamCaptureStatusNone)
+            fail(FEM_ERR_RUNTIME, "AMG: the finest level's own-order data are built outside graph capture");
+        AmgLevelDev& D = *lv[0];
+        sgs_grouped_device(D, s);
+        if (fp32) plain_shadows32(D, lv.size() == 1, s);
+        FEM_HIP(hipStreamSynchronize(s));
+        plain0_ = true;
+    }
+    void prepare(bool plain64, bool plain32, hipStream_t s) override {
+        if (plain64 || plain32) ensure_plain(s);
     }
     bool permuted32(const LinOp* A, LinOp*& op, Precond*& pc, const uint32_t*& rows) override {
         if (!lv[0]->perm || src_uid == 0) return false;
@@ -823,8 +866,14 @@ struct PcAmg : Precond {
         FEM_HIP(hipGetLastError());
     }
 
-    void apply(const double* r, double* z, hipStream_t s, Gate g) override { vcycle<double>(r, z, s, g); }
-    void apply32(const float* r, float* z, hipStream_t s, Gate g) override { vcycle<float>(r, z, s, g); }
+    void apply(const double* r, double* z, hipStream_t s, Gate g) override {
+        ensure_plain(s);
+        vcycle<double>(r, z, s, g);
+    }
+    void apply32(const float* r, float* z, hipStream_t s, Gate g) override {
+        ensure_plain(s);
+        vcycle<float>(r, z, s, g);
+    }
 
     // r.z from the V-cycle itself: the finest level's post-smoothing ends
     // with one stream pass per colour, which reads f = r and writes each
@@ -837,6 +886,7 @@ struct PcAmg : Precond {
     }
     unsigned dot_parts() const override { return dot_fused() ? lv[0]->h_cblk.back() : 0; }
     unsigned apply_dot(const double* r, double* z, double* part, hipStream_t s, Gate g) override {
+        ensure_plain(s);
         if (!dot_fused()) return 0;
         vcycle<double>(r, z, s, g, part);
         return lv[0]->h_cblk.back();
@@ -844,6 +894,7 @@ struct PcAmg : Precond {
     unsigned apply32_dot(const float* r, float* z, double* part, hipStream_t s, Gate g) override {
         // the float32 blocking never has more blocks than the float64 one
         // that sized the partial buffer; checked here all the same
+        ensure_plain(s);
         if (!dot_fused() || lv[0]->h_cblk32.empty() || lv[0]->h_cblk32.back() > lv[0]->h_cblk.back()) return 0;
         vcycle<float>(r, z, s, g, part);
         return lv[0]->h_cblk32.back();
@@ -905,7 +956,13 @@ struct PcAmg : Precond {
 
     // entries of the colour-grouped matrix in grouped rows [lo, hi)
     static uint64_t grouped_nnz(const AmgLevelDev& D, uint32_t lo, uint32_t hi) {
-        return (uint64_t)D.h_gp.at(hi) - D.h_gp.at(lo);  // the host copy sgs_setup_device keeps
+        if (!D.h_gp.empty()) return (uint64_t)D.h_gp.at(hi) - D.h_gp.at(lo);  // the host copy sgs_setup_device keeps
+        // no grouped copy (yet): for [lo, hi) at colour bounds, the
+        // colour-ordered form's off-diagonal entries plus one diagonal per row
+        uint64_t z = hi - lo;
+        for (int c = 0; c < D.ncol; ++c)
+            if (D.h_cptr[c] >= lo && D.h_cptr[c + 1] <= hi) z += D.h_onnz.at(c);
+        return z;
     }
 
     // one colour pass over r rows holding z entries: the rows' matrix part

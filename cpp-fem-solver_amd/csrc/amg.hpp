@@ -103,14 +103,17 @@ void transpose_device(const CsrMat& P, CsrMat& R, hipStream_t s);
 // B's rows must list strictly increasing columns
 void spgemm_device(const CsrMat& A, const CsrMat& B, CsrMat& C, hipStream_t s);
 // SGS smoother data of level D (colouring, diagonal, Jacobi weight and, with
-// at most kMaxGsColours colours, the colour-grouped copy of D.A)
-void sgs_setup_device(AmgLevelDev& D, hipStream_t s);
+// at most kMaxGsColours colours, the colour-grouped copy of D.A, or with
+// grouped = false only the diagonal in colour order, gd)
+void sgs_setup_device(AmgLevelDev& D, hipStream_t s, bool grouped = true);
+// the colour-grouped copy of a level coloured by sgs_setup_device
+void sgs_grouped_device(AmgLevelDev& D, hipStream_t s);
 // out = R without the columns rows[0..count) (entries in stored order)
 void drop_columns_device(const CsrMat& R, const uint32_t* rows, uint32_t count, CsrMat& out, hipStream_t s);
 // the float32 sweep blocks (gblk32, h_cblk32) of a level set up by sgs_setup_device
 void sgs_blocks32(AmgLevelDev& D, hipStream_t s);
-// the colour-ordered form (O, Pp, Rzp, oblk32) of a level with the grouped
-// copy, R without the last colour's columns and float32 shadows
+// the colour-ordered form (O, Pp, Rzp, oblk32) of a coloured level with R
+// without the last colour's columns and the float32 diagonal gd32
 void perm_setup_device(AmgLevelDev& D, hipStream_t s);
 // aggregation by a distance-2 maximal independent set of the strength graph
 // on the device: v holds the entries (norms = false) or one norm per entry

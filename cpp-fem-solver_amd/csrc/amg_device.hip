@@ -600,6 +600,11 @@ __global__ void k_group_fill(const uint32_t* rp, const uint32_t* ci, const doubl
     gd[t] = diag[i];
 }
 
+__global__ void k_colour_diag(const double* diag, const uint32_t* rows, uint32_t n, double* gd) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) gd[t] = diag[rows[t]];
+}
+
 __global__ void k_rows_sorted(const uint32_t* rp, const uint32_t* ci, uint32_t n, int* bad) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -635,7 +640,7 @@ void transpose_device(const CsrMat& P, CsrMat& R, hipStream_t s) {
 // beyond kMaxGsColours. Then the diagonal, the Jacobi weight and, when the
 // colours fit, the rows grouped by colour (ascending row order within a
 // colour) with a CSR copy of A in that order and its CSR-stream blocking.
-void sgs_setup_device(AmgLevelDev& D, hipStream_t s) {
+void sgs_setup_device(AmgLevelDev& D, hipStream_t s, bool grouped) {
     const CsrMat& A = D.A;
     const uint32_t n = A.n;
     D.diag.alloc(n);
@@ -718,7 +723,23 @@ void sgs_setup_device(AmgLevelDev& D, hipStream_t s) {
                                                    (int64_t)n, 0, 4, s));
         FEM_HIP(hipStreamSynchronize(s));
     }
-    // the grouped copy
+    if (grouped) {
+        sgs_grouped_device(D, s);
+        return;
+    }
+    // the diagonal in colour order only (the grouped copy comes later, if at all)
+    D.gd.alloc(std::max<uint32_t>(n, 1));
+    hipLaunchKernelGGL(k_colour_diag, dim3(g1(n)), dim3(256), 0, s, D.diag.get(), D.crows.get(), n, D.gd.get());
+    FEM_HIP(hipGetLastError());
+    FEM_HIP(hipStreamSynchronize(s));  // the temporaries end with this call
+}
+
+void sgs_grouped_device(AmgLevelDev& D, hipStream_t s) {
+    const CsrMat& A = D.A;
+    const uint32_t n = A.n;
+    const int ncol = D.ncol;
+    if (!D.use_gs || !D.crows.get() || D.h_cptr.size() != (size_t)ncol + 1)
+        fail(FEM_ERR_RUNTIME, "AMG: grouped copy of a level without its colouring");
     DevBuf<uint32_t> len(n + 1);
     hipLaunchKernelGGL(k_group_len, dim3(g1(n + 1)), dim3(256), 0, s, A.rp.get(), D.crows.get(), n, len.get());
     D.gp.alloc(n + 1);
@@ -878,8 +899,8 @@ __global__ void k_map_cols(const uint32_t* ci, uint64_t nnz, const uint32_t* pos
 
 void perm_setup_device(AmgLevelDev& D, hipStream_t s) {
     const uint32_t n = D.n;
-    if (!D.gp.get() || D.ncol < 2 || !D.Rz.n || D.h_cptr.size() != (size_t)D.ncol + 1)
-        fail(FEM_ERR_RUNTIME, "AMG: colour-ordered level without the smoother's grouped data");
+    if (!D.crows.get() || !D.gd32.get() || D.ncol < 2 || !D.Rz.n || D.h_cptr.size() != (size_t)D.ncol + 1)
+        fail(FEM_ERR_RUNTIME, "AMG: colour-ordered level without the smoother's colouring");
     const CsrMat& A = D.A;
     DevBuf<uint32_t> pos(n);
     hipLaunchKernelGGL(k_positions, dim3(g1(n)), dim3(256), 0, s, D.crows.get(), n, pos.get());

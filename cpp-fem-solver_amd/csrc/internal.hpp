@@ -309,6 +309,10 @@ struct Precond {
     virtual void apply32(const float* r, float* z, hipStream_t s, Gate g = {}) = 0;
     virtual bool has_fp32() const { return false; }
     virtual void ensure_fp32(hipStream_t) {}
+    // a solver will apply this preconditioner to vectors in the matrix's own
+    // row order, in float64 (plain64) and/or float32 (plain32): data built on
+    // first need are built now, before any graph capture
+    virtual void prepare(bool /*plain64*/, bool /*plain32*/, hipStream_t) {}
     virtual bool graph_safe() const { return true; }
     // Non-null for FEM_PC_DIAG: the diagonal d (z = r / d) for fused kernels.
     virtual const double* diag64() const { return nullptr; }

@@ -911,6 +911,7 @@ KrylovSolver::KrylovSolver(LinOp* A, Precond* M, const fem_solve_opts& o, hipStr
             A_->ensure_fp32(s);
         }
     }
+    if (M) M_->prepare(o.precision != FEM_PREC_FP32_IR, o.precision == FEM_PREC_FP32_IR && perm_ == nullptr, s);
     dotp_.alloc(std::max<unsigned>({1u, A_->dot_parts(), M_ ? M_->dot_parts() : 0u, A32_->dot_parts(),
                                     M32_ ? M32_->dot_parts() : 0u}));
     // PCG with M = G^T G in three launches per iteration, where both factors

@@ -58,6 +58,10 @@ struct PcSchur : Precond {
     int kind() const override { return FEM_PC_SCHUR; }
     uint32_t size() const override { return nv + np; }
     bool graph_safe() const override { return Apc->graph_safe() && (!Spc || Spc->graph_safe()); }
+    void prepare(bool plain64, bool plain32, hipStream_t s) override {
+        Apc->prepare(plain64, plain32, s);
+        if (Spc) Spc->prepare(plain64, plain32, s);
+    }
     bool has_fp32() const override { return fp32; }
     void ensure_fp32(hipStream_t s) override {
         if (fp32) return;

@@ -460,3 +460,30 @@ def test_amg_colour_ordered_inner_solve(case, monkeypatch):
     assert abs(st["iterations"] - stp["iterations"]) <= 2
     # the bytes of one float32 application follow the form the inner solves run
     assert 0 < M.apply_bytes(fp32=True) < Mp.apply_bytes(fp32=True)
+
+
+def test_amg_own_order_data_on_first_need(monkeypatch):
+    """A float32 SA-AMG whose finest level takes the colour-ordered form
+    builds that level's own-order data (grouped copy, float32 shadows) only
+    when something applies it in A's own order: float64 and float32
+    applications and a float64 PCG solve then equal, bit for bit, those of a
+    hierarchy that built them at setup (FEM_AMG_PERM=0 builds everything
+    eagerly); the colour-ordered float32 solve runs before either."""
+    G = gen.poisson3d(24)  # 13,824 rows: above the one-workgroup level size
+    _, b = gen.rhs(G, 5)
+    A = F.CsrMatrix.from_gen(G)
+    kw = dict(smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0, coarse_exact=1, fp32=1)
+    M = F.Preconditioner(A, "saamg", **kw)
+    monkeypatch.setenv("FEM_AMG_PERM", "0")
+    Me = F.Preconditioner(A, "saamg", **kw)
+    monkeypatch.delenv("FEM_AMG_PERM")
+    b0 = M.apply_bytes(fp32=False)
+    x32, st32 = F.solve(A, M, b, solver="pcg", precision=F.FP32_IR)
+    assert st32["converged"] == 1 and st32["true_relres"] < 1.5e-10
+    assert b0 == Me.apply_bytes(fp32=False)  # tallied before the grouped copy exists
+    r = np.random.default_rng(8).standard_normal(G.n)
+    assert np.array_equal(M.apply(r), Me.apply(r))
+    x, st = F.solve(A, M, b, solver="pcg")
+    xe, ste = F.solve(A, Me, b, solver="pcg")
+    assert np.array_equal(x, xe) and st["iterations"] == ste["iterations"]
+    assert M.bench_apply(fp32=True, reps=2) > 0.0  # float32 in A's own order
