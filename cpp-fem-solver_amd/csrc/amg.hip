@@ -322,6 +322,219 @@ __global__ __launch_bounds__(256) void k_coarse(const double* __restrict__ C, ui
     if (lane == 0) u[i] = (T)s;
 }
 
+// ---------------------------------------------------------------------------
+// Fused tail of the V-cycle on the level above a dense coarsest level, when
+// that level smooths by damped Jacobi with two pre-sweeps: per V-cycle two
+// launches replace its residual, the restriction, the coarse solve, the
+// prolongation and the first post-sweep. With S = 2w Dj^-1 - w^2 Dj^-1 A
+// Dj^-1 the operator of the two pre-sweeps from zero (Dj^-1: division by
+// the smoother's diagonal d, or multiplication by its inverse), the coarse
+// right-hand side is R (f - A S f) = Mpre f, and with C the coarse solution
+// operator the prolongation and the first post-sweep give
+// u = S f + W fc,  out = u + w Dj^-1 (f - A S f - AW fc),
+// W = P C and AW = A P C (set up in float64 from the float64 coarse
+// operator; float vectors read float32 copies of Mpre, W and AW, and every
+// product with fc is summed in float64).
+// ---------------------------------------------------------------------------
+template <typename T, bool INV>
+__device__ __forceinline__ T jdiv(T x, T d) {
+    if constexpr (INV) return d * x;
+    else return x / d;
+}
+// w Dj^-1 r with the rounding of the Jacobi epilogues
+template <typename T, bool INV>
+__device__ __forceinline__ T jstep(T w, T r, T d) {
+    if constexpr (INV) return w * (d * r);
+    else return (w * r) / d;
+}
+
+// blocks [0, nc nq): the coarse right-hand side Mpre f in nq chunks of
+// kTailChunk columns, fcp[q nc + c] the float64 partial of chunk q of coarse
+// row c; the rest: the two pre-sweeps from zero, LANES lanes per row (the
+// float arithmetic of EPI_JACOBI0 / EPI_JACOBI0_INV). Loads are issued in
+// batches before their values are used.
+constexpr int kTailBatch = 4;      // sparse-row loads per lane and batch
+constexpr int kTailRowBatch = 8;   // Mpre row loads per thread
+constexpr uint32_t kTailChunk = kTailRowBatch * kBlock;  // Mpre columns per block
+constexpr int kTailWBatch = 16;    // W / AW loads per lane and batch
+template <typename T, bool INV, int LANES>
+__global__ __launch_bounds__(kBlock) void k_tail_pre(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                                     const T* __restrict__ v, const T* __restrict__ f,
+                                                     const T* __restrict__ d, T w, uint32_t n, T* __restrict__ upre,
+                                                     const T* __restrict__ mpre, uint32_t nc,
+                                                     double* __restrict__ fcp, Gate g) {
+    constexpr int U = kTailBatch;
+    const uint32_t nq = (n + kTailChunk - 1) / kTailChunk;
+    if (blockIdx.x < nc * nq) {
+        // a chunk of kTailChunk columns of a row of Mpre (the longest work
+        // of the launch: its blocks come first), one batch of loads
+        constexpr int UC = kTailRowBatch;
+        const uint32_t c = blockIdx.x / nq, q = blockIdx.x % nq;
+        const T* row = mpre + (size_t)c * n;
+        double t[1] = {0.0};
+        const uint32_t j0 = q * kTailChunk + threadIdx.x;
+        T mv[UC], fv[UC];
+#pragma unroll
+        for (int u = 0; u < UC; ++u) {
+            const uint32_t j = j0 + u * kBlock;
+            mv[u] = j < n ? row[j] : (T)0;
+            fv[u] = j < n ? f[j] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < UC; ++u) t[0] += (double)mv[u] * (double)fv[u];
+        block_sum<1>(t);
+        if (g.closed()) return;  // uniform in the block, after its barriers
+        if (threadIdx.x == 0) fcp[(size_t)q * nc + c] = t[0];
+        return;
+    }
+    const uint32_t row = ((blockIdx.x - nc * nq) * (uint32_t)kBlock + threadIdx.x) / LANES;
+    const int lane = threadIdx.x % LANES;
+    uint32_t k0 = 0, k1 = 0;
+    if (row < n) {
+        k0 = rp[row];
+        k1 = rp[row + 1];
+    }
+    if (g.closed()) return;
+    T s = (T)0;
+    for (uint32_t kb = k0 + lane; kb < k1; kb += U * LANES) {
+        uint32_t cv[U];
+        T av[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = kb + u * LANES;
+            cv[u] = k < k1 ? ci[k] : 0u;
+            av[u] = k < k1 ? v[k] : (T)0;
+        }
+        T fv[U], dv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            fv[u] = f[cv[u]];
+            dv[u] = d[cv[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (kb + u * LANES < k1) s += av[u] * jac0<T, INV>(fv[u], dv[u], w);
+    }
+    s = group_sum<LANES>(s);
+    if (row < n && lane == 0) upre[row] = jac0<T, INV>(f[row], d[row], w) + jstep<T, INV>(w, f[row] - s, d[row]);
+}
+
+// the prolongation u = upre + W fc and the first post-sweep in one pass:
+// out = u + w Dj^-1 (f - A upre - AW fc); LANES lanes per row; W and AW in
+// WT (float64, or float32 copies for float vectors), their products with fc
+// summed in float64
+template <typename T, typename WT, bool INV, int LANES>
+__global__ __launch_bounds__(kBlock) void k_tail_post(const uint32_t* __restrict__ rp, const uint32_t* __restrict__ ci,
+                                                      const T* __restrict__ v, const T* __restrict__ f,
+                                                      const T* __restrict__ d, T w, uint32_t n,
+                                                      const T* __restrict__ upre, const WT* __restrict__ W,
+                                                      const WT* __restrict__ AW, uint32_t nc,
+                                                      const double* __restrict__ fcp, uint32_t nq,
+                                                      T* __restrict__ out, Gate g) {
+    constexpr int U = kTailBatch, UW = kTailWBatch;
+    const uint32_t row = (blockIdx.x * (uint32_t)kBlock + threadIdx.x) / LANES;
+    const int lane = threadIdx.x % LANES;
+    const bool on = row < n;
+    const uint32_t rr = on ? row : 0u;  // rows past the end read row 0 and store nothing
+    const uint32_t k0 = rp[rr], k1 = on ? rp[rr + 1] : k0;
+    const WT* wr = W + (size_t)rr * nc;
+    const WT* ar = AW + (size_t)rr * nc;
+    T s1 = (T)0;
+    double s2 = 0.0, s3 = 0.0;
+    // fc into LDS (its chunk partials added in chunk order), loads issued
+    // before the row's own
+    extern __shared__ double fs[];
+    for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+        double x = fcp[c];
+        for (uint32_t q = 1; q < nq; ++q) x += fcp[(size_t)q * nc + c];
+        fs[c] = x;
+    }
+    WT wv[UW], avv[UW];
+    // a batch of this row's W and AW entries
+    auto load = [&](uint32_t cb) {
+#pragma unroll
+        for (int u = 0; u < UW; ++u) {
+            const uint32_t c = cb + u * LANES;
+            wv[u] = c < nc ? wr[c] : (WT)0;
+            avv[u] = c < nc ? ar[c] : (WT)0;
+        }
+    };
+    auto add = [&](uint32_t cb) {
+#pragma unroll
+        for (int u = 0; u < UW; ++u) {
+            const uint32_t c = cb + u * LANES;
+            const double x = c < nc ? fs[c] : 0.0;
+            s2 += (double)wv[u] * x;
+            s3 += (double)avv[u] * x;
+        }
+    };
+    load((uint32_t)lane);  // issued before the sparse part
+    for (uint32_t kb = k0 + lane; kb < k1; kb += U * LANES) {
+        uint32_t cv[U];
+        T av[U], uv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint32_t k = kb + u * LANES;
+            cv[u] = k < k1 ? ci[k] : 0u;
+            av[u] = k < k1 ? v[k] : (T)0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) uv[u] = upre[cv[u]];
+#pragma unroll
+        for (int u = 0; u < U; ++u) s1 += av[u] * uv[u];
+    }
+    __syncthreads();  // fs complete
+    add((uint32_t)lane);
+    for (uint32_t cb = lane + UW * LANES; cb < nc; cb += UW * LANES) {
+        load(cb);
+        add(cb);
+    }
+    s1 = group_sum<LANES>(s1);
+    s2 = group_sum<LANES>(s2);
+    s3 = group_sum<LANES>(s3);
+    if (g.closed()) return;
+    if (on && lane == 0) {
+        const double u = (double)upre[row] + s2;
+        const double r = (double)f[row] - ((double)s1 + s3);
+        out[row] = (T)(u + jstep<double, INV>((double)w, r, (double)d[row]));
+    }
+}
+
+// set-up products of the fused tail, float64 and deterministic (every
+// output element is one thread's fixed-order sum); dense n x nc matrices are
+// row-major, so the products with the sparse n x n operator are sparse times
+// dense: the transposes of R A and R A Dj^-1 A are At R^T and At (...).
+// X (n x nc) := R^T for the CSR matrix R (nc x n); X zeroed before
+__global__ void k_dense_of_csr_t(const uint32_t* rp, const uint32_t* ci, const double* v, uint32_t nc, double* X) {
+    const uint32_t c = blockIdx.x;
+    if (c >= nc) return;
+    for (uint32_t p = rp[c] + threadIdx.x; p < rp[c + 1]; p += blockDim.x) X[(size_t)ci[p] * nc + c] = v[p];
+}
+// Y = P X (P: n x m CSR, X: m x nc dense), one block per row of P; row i
+// then divided by d[i] (INV: multiplied by it) when d is not null
+template <bool INV>
+__global__ void k_csr_times_dense(const uint32_t* rp, const uint32_t* ci, const double* v, uint32_t n,
+                                  const double* X, uint32_t nc, const double* d, double* Y) {
+    const uint32_t i = blockIdx.x;
+    if (i >= n) return;
+    for (uint32_t c = threadIdx.x; c < nc; c += blockDim.x) {
+        double s = 0.0;
+        for (uint32_t p = rp[i]; p < rp[i + 1]; ++p) s += v[p] * X[(size_t)ci[p] * nc + c];
+        if (d) s = jdiv<double, INV>(s, d[i]);
+        Y[(size_t)i * nc + c] = s;
+    }
+}
+// Mpre = (X1 - 2w Xa + w^2 Dj^-1 X4)^T for n x nc inputs, written nc x n
+template <bool INV>
+__global__ void k_tail_mpre(const double* X1, const double* Xa, const double* X4, const double* d, double w,
+                            uint32_t n, uint32_t nc, double* M) {
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= (uint64_t)n * nc) return;
+    const uint32_t c = (uint32_t)(t / n), j = (uint32_t)(t % n);  // output order: coalesced stores
+    const size_t q = (size_t)j * nc + c;
+    M[t] = X1[q] - 2.0 * w * Xa[q] + w * w * jdiv<double, INV>(X4[q], d[j]);
+}
+
 template <typename T>
 __global__ void k_zero(T* u, uint32_t n, Gate g) {
     if (g.closed()) return;
@@ -387,6 +600,11 @@ struct PcAmg : Precond {
     uint32_t nco = 0;
     bool fp32 = false;
     bool plain0_ = true;  // the finest level's grouped copy (and shadows) built
+    // fused tail on level lv.size() - 2 (k_tail_pre / k_tail_post)
+    bool tail_ = false;
+    DevBuf<double> tW_, tAW_, tM_;  // W = P C, AW = A P C (n x nco), Mpre (nco x n)
+    DevBuf<float> tM32_, tW32_, tAW32_;
+    DevBuf<double> tFcp_;  // the coarse right-hand side in column-chunk partials
 
     PcAmg(int kind, AmgDevHierarchy&& H, const fem_pc_params& p, hipStream_t s)
         : kind_(kind), inv_diag(kind == FEM_PC_SAAMG || kind == FEM_PC_BSAAMG), smoother(p.smoother), npre(p.npre),
@@ -437,6 +655,120 @@ struct PcAmg : Precond {
         coarse.upload(H.coarse, s);
         FEM_HIP(hipStreamSynchronize(s));
         tr.lap("vectors, coarsest operator");
+        build_tail(s);
+        if (tail_) tr.lap("fused tail operators");
+    }
+
+    // The fused tail applies on the level above a dense coarsest level that
+    // smooths by damped Jacobi with two pre-sweeps and at least one
+    // post-sweep, while its dense operators stay small (FEM_AMG_TAIL=0 in
+    // the environment keeps the separate launches, for comparison).
+    static constexpr uint64_t kTailDense = 2ull << 20;  // entries per dense tail operator
+    static double tail_chunks(const AmgLevelDev& D) { return (double)((D.n + kTailChunk - 1) / kTailChunk); }
+    bool tail_ok() const {
+        const size_t L = lv.size();
+        if (L < 3 || npre != 2 || npost < 1) return false;
+        const AmgLevelDev& D = *lv[L - 2];
+        if (smoother == FEM_SMOOTH_SGS && D.use_gs) return false;
+        if (!fuse_first2(D, npre) || (uint64_t)D.n * nco > kTailDense || D.nc != nco) return false;
+        const char* e = std::getenv("FEM_AMG_TAIL");
+        return !(e && e[0] == '0');
+    }
+    void build_tail(hipStream_t s) {
+        if (!tail_ok()) return;
+        AmgLevelDev& D = *lv[lv.size() - 2];
+        const uint32_t n = D.n, nc = nco;
+        const size_t mn = (size_t)n * nc;
+        const double* d = smoother == FEM_SMOOTH_SGS ? D.diag.get() : D.d.get();
+        const bool inv = jinv();
+        const double w = jwt(D);
+        CsrMat At;
+        transpose_device(D.A, At, s);
+        // transposed: X1 = R^T, Xa = Dj^-1 At X1, X4 = At Xa; Mpre = R - 2w
+        // Xa^T + w^2 (Dj^-1 X4)^T
+        DevBuf<double> X1(mn), Xa(mn), X4(mn);
+        FEM_HIP(hipMemsetAsync(X1.get(), 0, mn * sizeof(double), s));
+        hipLaunchKernelGGL(k_dense_of_csr_t, dim3(nc), dim3(256), 0, s, D.R.rp.get(), D.R.ci.get(), D.R.v.get(), nc,
+                           X1.get());
+        if (inv)
+            hipLaunchKernelGGL(k_csr_times_dense<true>, dim3(n), dim3(256), 0, s, At.rp.get(), At.ci.get(),
+                               At.v.get(), n, X1.get(), nc, d, Xa.get());
+        else
+            hipLaunchKernelGGL(k_csr_times_dense<false>, dim3(n), dim3(256), 0, s, At.rp.get(), At.ci.get(),
+                               At.v.get(), n, X1.get(), nc, d, Xa.get());
+        hipLaunchKernelGGL(k_csr_times_dense<false>, dim3(n), dim3(256), 0, s, At.rp.get(), At.ci.get(), At.v.get(),
+                           n, Xa.get(), nc, (const double*)nullptr, X4.get());
+        tM_.alloc(mn);
+        const dim3 G((unsigned)((mn + 255) / 256));
+        if (inv)
+            hipLaunchKernelGGL(k_tail_mpre<true>, G, dim3(256), 0, s, X1.get(), Xa.get(), X4.get(), d, w, n, nc,
+                               tM_.get());
+        else
+            hipLaunchKernelGGL(k_tail_mpre<false>, G, dim3(256), 0, s, X1.get(), Xa.get(), X4.get(), d, w, n, nc,
+                               tM_.get());
+        // W = P C, AW = A W
+        tW_.alloc(mn);
+        tAW_.alloc(mn);
+        tFcp_.alloc((size_t)((n + kTailChunk - 1) / kTailChunk) * nc);
+        hipLaunchKernelGGL(k_csr_times_dense<false>, dim3(n), dim3(256), 0, s, D.P.rp.get(), D.P.ci.get(),
+                           D.P.v.get(), n, coarse.get(), nc, (const double*)nullptr, tW_.get());
+        hipLaunchKernelGGL(k_csr_times_dense<false>, dim3(n), dim3(256), 0, s, D.A.rp.get(), D.A.ci.get(),
+                           D.A.v.get(), n, tW_.get(), nc, (const double*)nullptr, tAW_.get());
+        FEM_HIP(hipGetLastError());
+        FEM_HIP(hipStreamSynchronize(s));  // the temporaries end with this call
+        tail_ = true;
+    }
+    template <typename T, bool INV, int LANES>
+    void tail_launch(bool pre, AmgLevelDev& D, const T* f, const T* d, const T* upre_in, T* out, hipStream_t s,
+                     Gate g) {
+        constexpr bool F64 = std::is_same<T, double>::value;
+        const T* v;
+        const T* mp;
+        if constexpr (F64) {
+            v = D.A.v.get();
+            mp = tM_.get();
+        } else {
+            v = D.A.v32.get();
+            mp = tM32_.get();
+        }
+        if (!v || !mp) fail(FEM_ERR_RUNTIME, "AMG: fused tail without its operators in this precision");
+        const T w = (T)jwt(D);
+        const uint32_t nrb = (uint32_t)(((uint64_t)D.n * LANES + kBlock - 1) / kBlock);
+        const uint32_t nq = (D.n + kTailChunk - 1) / kTailChunk;
+        if (pre) {
+            hipLaunchKernelGGL((k_tail_pre<T, INV, LANES>), dim3(nrb + nco * nq), dim3(kBlock), 0, s, D.A.rp.get(),
+                               D.A.ci.get(), v, f, d, w, D.n, out, mp, nco, tFcp_.get(), g);
+        } else {
+            const T* W;
+            const T* AW;
+            if constexpr (F64) {
+                W = tW_.get();
+                AW = tAW_.get();
+            } else {
+                W = tW32_.get();
+                AW = tAW32_.get();
+            }
+            if (!W || !AW) fail(FEM_ERR_RUNTIME, "AMG: fused tail without its operators in this precision");
+            hipLaunchKernelGGL((k_tail_post<T, T, INV, LANES>), dim3(nrb), dim3(kBlock), (size_t)nco * sizeof(double), s,
+                               D.A.rp.get(),
+                               D.A.ci.get(), v, f, d, w, D.n, upre_in, W, AW, nco, (const double*)tFcp_.get(), nq,
+                               out, g);
+        }
+        FEM_HIP(hipGetLastError());
+    }
+    // pre: u_pre into out and the coarse right-hand side into tFcp_; post:
+    // the prolongation of its coarse solution and the first post-sweep from
+    // upre into out
+    template <typename T>
+    void tail(bool pre, AmgLevelDev& D, const T* f, const T* d, const T* upre, T* out, hipStream_t s, Gate g) {
+        const bool wide = D.A.lanes == 64;
+        if (jinv()) {
+            if (wide) tail_launch<T, true, 64>(pre, D, f, d, upre, out, s, g);
+            else tail_launch<T, true, 32>(pre, D, f, d, upre, out, s, g);
+        } else {
+            if (wide) tail_launch<T, false, 64>(pre, D, f, d, upre, out, s, g);
+            else tail_launch<T, false, 32>(pre, D, f, d, upre, out, s, g);
+        }
     }
     int kind() const override { return kind_; }
     uint32_t size() const override { return lv[0]->n; }
@@ -488,6 +820,17 @@ struct PcAmg : Precond {
             D.u32.alloc(D.n);
             if (k > 0) D.f32.alloc(D.n);
             tr.lap(k == 0 ? "shadows, level 0" : "shadows, level >= 1");
+        }
+        if (tail_) {
+            // float32 copies: the products with fc are still summed in
+            // float64 (the coarse correction was rounded to float32 before,
+            // as u_c and through P)
+            tM32_.alloc(tM_.n);
+            vec_convert<float, double>(tM32_.get(), tM_.get(), 1.0, tM_.n, s);
+            tW32_.alloc(tW_.n);
+            vec_convert<float, double>(tW32_.get(), tW_.get(), 1.0, tW_.n, s);
+            tAW32_.alloc(tAW_.n);
+            vec_convert<float, double>(tAW32_.get(), tAW_.get(), 1.0, tAW_.n, s);
         }
         fp32 = true;
         if (!plain0_ && !perm_ok()) ensure_plain(s);  // not expected: the form's conditions held at setup
@@ -803,6 +1146,15 @@ struct PcAmg : Precond {
                     continue;
                 }
             }
+            if (tail_ && k + 2 == L) {
+                // the two pre-sweeps (into u2, where the fused sweeps put
+                // them) and the coarse right-hand side in one launch
+                xa[k] = B.u;
+                xb[k] = B.u2;
+                tail<T>(true, D, f, jd(D, B), nullptr, B.u2, s, g);
+                ucur[k] = B.u2;
+                continue;
+            }
             if (smoother == FEM_SMOOTH_SGS && D.use_gs) {
                 u = (k == 0) ? z : B.u;
                 gs<T>(D, f, u, npre, false, s, g, true);
@@ -835,8 +1187,9 @@ struct PcAmg : Precond {
             csr_spmv<T>(shortcut && D.Rz.n ? D.Rz : D.R, B.res, BC.f, EPI_STORE, {}, s, g);
             fk[k + 1] = BC.f;
         }
-        // coarsest level: u = C f (dense, float64 accumulation)
-        {
+        // coarsest level: u = C f (dense, float64 accumulation; inside the
+        // fused tail's second launch when it applies)
+        if (!tail_) {
             AmgLevelDev& C = *lv[L - 1];
             Bufs<T> BC = bufs<T>(C);
             T* u = (L == 1) ? z : BC.u;
@@ -857,6 +1210,12 @@ struct PcAmg : Precond {
                     gsp(D, fk[0], u, npost, true, false, s, g, part);
                     continue;
                 }
+            }
+            if (tail_ && k + 2 == (int64_t)L) {
+                tail<T>(false, D, fk[k], jd(D, B), u, xa[k], s, g);
+                ucur[k] = npost > 1 ? jacobi<T>(D, fk[k], jd(D, B), jinv(), jwt(D), xa[k], xa[k], xb[k], npost - 1, s, g)
+                                    : xa[k];
+                continue;
             }
             csr_spmv<T>(D.P, ucur[k + 1], u, EPI_ADD, ar, s, g);
             if (smoother == FEM_SMOOTH_SGS && D.use_gs) gs<T>(D, fk[k], u, npost, true, s, g, false, k == 0 ? part : nullptr);
@@ -914,15 +1273,28 @@ struct PcAmg : Precond {
                 b += gsp_bytes(D, s, npre, true) + gsp_residual_bytes(D, s) + csr_bytes(D.Rzp, s, 0);
                 continue;
             }
+            if (tail_ && k + 2 == L) {
+                // the fused pre-sweeps (f gathered, d, u_pre) and Mpre f
+                // (its float64 chunk partials)
+                b += csr_bytes(D.A, s, 1) + (double)nco * D.n * s + 8.0 * nco * tail_chunks(D);
+                continue;
+            }
             if (smoother == FEM_SMOOTH_SGS && D.use_gs) b += gs_bytes(D, s, npre, true);
             else b += npre > 0 ? jacobi_bytes(D, s, npre, true) : (double)D.n * s;
             const bool shortcut = smoother == FEM_SMOOTH_SGS && residual_shortcut(D);
             b += shortcut ? gs_residual_bytes(D, s) : csr_bytes(D.A, s, 1);
             b += csr_bytes(shortcut && D.Rz.n ? D.Rz : D.R, s, 0);
         }
-        b += (double)nco * nco * 8 + 2.0 * nco * s;  // dense coarsest operator, f, u
+        if (!tail_) b += (double)nco * nco * 8 + 2.0 * nco * s;  // dense coarsest operator, f, u
         for (int64_t k = (int64_t)L - 2; k >= 0; --k) {
             AmgLevelDev& D = *lv[k];
+            if (tail_ && k + 2 == (int64_t)L) {
+                // u_pre gathered through A, W and AW, f, d, fc and the
+                // output; then the remaining post-sweeps
+                b += csr_bytes(D.A, s, 2) + 2.0 * D.n * nco * s + 8.0 * nco * tail_chunks(D);
+                for (int sw = 1; sw < npost; ++sw) b += csr_bytes(D.A, s, 2);
+                continue;
+            }
             if (perm && k == 0) {
                 b += csr_bytes(D.Pp, s, 1) + gsp_bytes(D, s, npost, false);
                 continue;

@@ -318,7 +318,7 @@ def _csr_bytes(nnz, n, m, s, extra):
 
 
 @pytest.mark.parametrize("fp32", [0, 1])
-def test_precond_bytes_tally(fp32):
+def test_precond_bytes_tally(fp32, monkeypatch):
     """fem_precond_bytes: z = r / d moves 3 vectors; an IC(0) or ILU(0)
     application two sweeps over the strictly triangular halves; for SA-AMG with the
     Jacobi smoother, going from 1+1 to 2+2 smoothing sweeps changes the tally
@@ -328,7 +328,9 @@ def test_precond_bytes_tally(fp32):
     in one launch (A_k, f gathered, d and the output) in place of the single
     sweep from zero (f, d and u), so the difference of the two
     tallies is fixed by the level sizes the hierarchy reports (themselves
-    checked against the oracle above)."""
+    checked against the oracle above). The V-cycles here run without the
+    fused tail (FEM_AMG_TAIL=0; its tally is checked in
+    test_amg_fused_tail)."""
     G = gen.poisson3d(32)
     A = F.CsrMatrix.from_gen(G)
     s = 4 if fp32 else 8
@@ -342,8 +344,10 @@ def test_precond_bytes_tally(fp32):
         T = F.Preconditioner(A, kind, fp32=fp32)
         assert T.apply_bytes(fp32=bool(fp32)) == 2 * sweep + ndiv * G.n * s, kind
     kw = dict(smoother=F.SMOOTH_JACOBI, coarse_exact=1, omega=0.0, p_omega=0.0, fp32=fp32)
+    monkeypatch.setenv("FEM_AMG_TAIL", "0")
     M1 = F.Preconditioner(A, "saamg", npre=1, npost=1, **kw)
     M2 = F.Preconditioner(A, "saamg", npre=2, npost=2, **kw)
+    monkeypatch.delenv("FEM_AMG_TAIL")
     a, b = M1.info(), M2.info()
     L = b["levels"]
     assert L == a["levels"] > 2
@@ -487,3 +491,49 @@ def test_amg_own_order_data_on_first_need(monkeypatch):
     xe, ste = F.solve(A, Me, b, solver="pcg")
     assert np.array_equal(x, xe) and st["iterations"] == ste["iterations"]
     assert M.bench_apply(fp32=True, reps=2) > 0.0  # float32 in A's own order
+
+
+@pytest.mark.parametrize("smoother", ["sgs", "jacobi"])
+def test_amg_fused_tail(smoother, monkeypatch):
+    """The level above a dense coarsest level, smoothed by damped Jacobi with
+    two pre-sweeps, runs its residual, restriction, coarse solve,
+    prolongation and first post-sweep as two launches over set-up products
+    (Mpre = R (I - A S), W = P C, A W): against the separate launches
+    (FEM_AMG_TAIL=0) one float64 V-cycle agrees to 1e-12, float64 PCG takes
+    the same iterations, float32-inner PCG converges with at most one more,
+    and both solutions match the oracle's within the parity bar."""
+    from parity import tol_x, cond_poisson
+    G = gen.poisson3d(40)
+    _, b = gen.rhs(G, 13)
+    A = F.CsrMatrix.from_gen(G)
+    sm = F.SMOOTH_SGS if smoother == "sgs" else F.SMOOTH_JACOBI
+    kw = dict(smoother=sm, npre=2, npost=2, omega=1.0 if smoother == "sgs" else 0.0, p_omega=0.0,
+              coarse_exact=1, coarse_size=300)
+    for fp32 in (0, 1):
+        M = F.Preconditioner(A, "saamg", fp32=fp32, **kw)
+        monkeypatch.setenv("FEM_AMG_TAIL", "0")
+        Ms = F.Preconditioner(A, "saamg", fp32=fp32, **kw)
+        monkeypatch.delenv("FEM_AMG_TAIL")
+        info = M.info()
+        L = info["levels"]
+        assert L >= 3
+        # the tally of the two launches in place of the separate ones, from
+        # the sizes of the level above the coarsest (n, its A, P and R) and
+        # of the coarsest (nc)
+        n, nz, nc = int(info["rows"][L - 2]), int(info["nnz"][L - 2]), int(info["rows"][L - 1])
+        nzp, nzr = M.amg_matrix(L - 2, 1).nnz, M.amg_matrix(L - 2, 2).nnz
+        for sb in ((8, 4) if fp32 else (8,)):
+            nq = -(-n // 2048)  # Mpre f in float64 partials over chunks of 2048 columns
+            fused = (_csr_bytes(nz, n, n, sb, 1) + nc * n * sb + 8 * nc * nq) + (_csr_bytes(nz, n, n, sb, 2) +
+                                                                               2 * n * nc * sb + 8 * nc * nq)
+            separate = (2 * _csr_bytes(nz, n, n, sb, 1) + _csr_bytes(nzr, nc, n, sb, 0) + nc * nc * 8 + 2 * nc * sb +
+                        _csr_bytes(nzp, n, nc, sb, 1) + _csr_bytes(nz, n, n, sb, 2))
+            assert M.apply_bytes(fp32=sb == 4) - Ms.apply_bytes(fp32=sb == 4) == fused - separate
+        r = np.random.default_rng(3).standard_normal(G.n)
+        assert rel(M.apply(r), Ms.apply(r)) < 1e-12
+        prec = F.FP32_IR if fp32 else F.FP64
+        x, st = F.solve(A, M, b, solver="pcg", precision=prec)
+        xs, sts = F.solve(A, Ms, b, solver="pcg", precision=prec)
+        assert st["converged"] == 1 and st["true_relres"] < 1.5e-10
+        assert abs(st["iterations"] - sts["iterations"]) <= (1 if fp32 else 0)
+        assert rel(x, xs) < tol_x(G, cond=cond_poisson(40, 3))
