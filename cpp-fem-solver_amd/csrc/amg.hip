@@ -663,6 +663,35 @@ struct PcAmg : Precond {
     // smooths by damped Jacobi with two pre-sweeps and at least one
     // post-sweep, while its dense operators stay small (FEM_AMG_TAIL=0 in
     // the environment keeps the separate launches, for comparison).
+    // Level j (>= 1) takes its first damped Jacobi sweep from zero from the
+    // restriction that writes its right-hand side (EPI_STORE_J0 /
+    // EPI_STORE_J0_INV: o = jac0(f, d)) wherever that sweep would otherwise
+    // be a launch of its own (not on levels whose first two sweeps run as one
+    // launch, nor on the fused tail's level).
+    // FEM_AMG_J0=0 in the environment keeps the sweep's own launch (the
+    // results are the same bit for bit; for comparison).
+    bool j0_from_restriction(size_t j) const {
+        if (j == 0 || j + 1 >= lv.size() || npre < 1 || !j0_env_) return false;
+        const AmgLevelDev& D = *lv[j];
+        if (smoother == FEM_SMOOTH_SGS && D.use_gs) return false;
+        return !fuse_first2(D, npre);
+    }
+    const bool j0_env_ = [] {
+        const char* e = std::getenv("FEM_AMG_J0");
+        return !(e && e[0] == '0');
+    }();
+    // the restriction's epilogue into level j (its right-hand side f, and
+    // the first sweep into u when j0_from_restriction(j))
+    template <typename T>
+    int restrict_epi(size_t j, SpmvArgs<T>& a) {
+        if (!j0_from_restriction(j)) return EPI_STORE;
+        AmgLevelDev& C = *lv[j];
+        Bufs<T> BC = bufs<T>(C);
+        a.d = jd(C, BC);
+        a.w = jwt(C);
+        a.o = BC.u;
+        return jinv() ? EPI_STORE_J0_INV : EPI_STORE_J0;
+    }
     static constexpr uint64_t kTailDense = 2ull << 20;  // entries per dense tail operator
     static double tail_chunks(const AmgLevelDev& D) { return (double)((D.n + kTailChunk - 1) / kTailChunk); }
     bool tail_ok() const {
@@ -1140,7 +1169,9 @@ struct PcAmg : Precond {
                     gsp_residual(D, f, z, B.res, s, g);
                     AmgLevelDev& C = *lv[1];
                     Bufs<T> BC = bufs<T>(C);
-                    csr_spmv<T>(D.Rzp, B.res, BC.f, EPI_STORE, {}, s, g);
+                    SpmvArgs<T> ra;
+                    const int repi = restrict_epi<T>(1, ra);
+                    csr_spmv<T>(D.Rzp, B.res, BC.f, repi, ra, s, g);
                     ucur[0] = z;
                     fk[1] = BC.f;
                     continue;
@@ -1170,7 +1201,10 @@ struct PcAmg : Precond {
                 }
                 xa[k] = X;
                 xb[k] = Y;
-                if (npre > 0) {
+                if (npre > 0 && j0_from_restriction(k)) {
+                    // the first sweep is in u (= X) already
+                    u = npre > 1 ? jacobi<T>(D, f, jd(D, B), jinv(), jwt(D), X, X, Y, npre - 1, s, g) : X;
+                } else if (npre > 0) {
                     u = jacobi<T>(D, f, jd(D, B), jinv(), jwt(D), nullptr, X, Y, npre, s, g);
                 } else {
                     hipLaunchKernelGGL((k_zero<T>), g1(D.n), dim3(256), 0, s, X, D.n, g);
@@ -1184,7 +1218,9 @@ struct PcAmg : Precond {
             if (!shortcut) csr_spmv<T>(D.A, u, B.res, EPI_RESID, ar, s, g);
             AmgLevelDev& C = *lv[k + 1];
             Bufs<T> BC = bufs<T>(C);
-            csr_spmv<T>(shortcut && D.Rz.n ? D.Rz : D.R, B.res, BC.f, EPI_STORE, {}, s, g);
+            SpmvArgs<T> ra;
+            const int repi = restrict_epi<T>(k + 1, ra);
+            csr_spmv<T>(shortcut && D.Rz.n ? D.Rz : D.R, B.res, BC.f, repi, ra, s, g);
             fk[k + 1] = BC.f;
         }
         // coarsest level: u = C f (dense, float64 accumulation; inside the
@@ -1269,8 +1305,10 @@ struct PcAmg : Precond {
         double b = 0.0;
         for (size_t k = 0; k + 1 < L; ++k) {
             AmgLevelDev& D = *lv[k];
+            // the restriction's second output: d read, the first sweep written
+            const double j0x = j0_from_restriction(k + 1) ? 2.0 * lv[k + 1]->n * s : 0.0;
             if (perm && k == 0) {
-                b += gsp_bytes(D, s, npre, true) + gsp_residual_bytes(D, s) + csr_bytes(D.Rzp, s, 0);
+                b += gsp_bytes(D, s, npre, true) + gsp_residual_bytes(D, s) + csr_bytes(D.Rzp, s, 0) + j0x;
                 continue;
             }
             if (tail_ && k + 2 == L) {
@@ -1280,10 +1318,11 @@ struct PcAmg : Precond {
                 continue;
             }
             if (smoother == FEM_SMOOTH_SGS && D.use_gs) b += gs_bytes(D, s, npre, true);
+            else if (j0_from_restriction(k)) b += jacobi_bytes(D, s, npre - 1, false);
             else b += npre > 0 ? jacobi_bytes(D, s, npre, true) : (double)D.n * s;
             const bool shortcut = smoother == FEM_SMOOTH_SGS && residual_shortcut(D);
             b += shortcut ? gs_residual_bytes(D, s) : csr_bytes(D.A, s, 1);
-            b += csr_bytes(shortcut && D.Rz.n ? D.Rz : D.R, s, 0);
+            b += csr_bytes(shortcut && D.Rz.n ? D.Rz : D.R, s, 0) + j0x;
         }
         if (!tail_) b += (double)nco * nco * 8 + 2.0 * nco * s;  // dense coarsest operator, f, u
         for (int64_t k = (int64_t)L - 2; k >= 0; --k) {

@@ -88,7 +88,7 @@ __device__ __forceinline__ T spmv_x(const T* x, const SpmvArgs<T>& a, uint32_t c
 
 template <typename T, int EPI>
 __device__ __forceinline__ T spmv_epilogue(uint32_t row, T s, const SpmvArgs<T>& a) {
-    if constexpr (EPI == EPI_STORE) return s;
+    if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_J0 || EPI == EPI_STORE_J0_INV) return s;
     else if constexpr (EPI == EPI_RESID) return a.f[row] - s;
     else if constexpr (EPI == EPI_ADD) return a.u[row] + s;
     else if constexpr (EPI == EPI_JACOBI) return a.u[row] + ((T)a.w * (a.f[row] - s)) / a.d[row];
@@ -96,6 +96,14 @@ __device__ __forceinline__ T spmv_epilogue(uint32_t row, T s, const SpmvArgs<T>&
     else if constexpr (EPI == EPI_JACOBI0)
         return jac0<T, false>(a.f[row], a.d[row], (T)a.w) + ((T)a.w * (a.f[row] - s)) / a.d[row];
     else return jac0<T, true>(a.f[row], a.d[row], (T)a.w) + (T)a.w * (a.d[row] * (a.f[row] - s));
+}
+
+// the second output of EPI_STORE_J0 / EPI_STORE_J0_INV: the first Jacobi
+// sweep from zero with right-hand side y
+template <typename T, int EPI>
+__device__ __forceinline__ void spmv_second(uint32_t row, T y, T d, const SpmvArgs<T>& a) {
+    if constexpr (EPI == EPI_STORE_J0) a.o[row] = jac0<T, false>(y, d, (T)a.w);
+    else if constexpr (EPI == EPI_STORE_J0_INV) a.o[row] = jac0<T, true>(y, d, (T)a.w);
 }
 
 // Epilogue operands of one row, loaded before the block stages its products
@@ -107,10 +115,12 @@ struct EpiIn {
         constexpr bool J0 = EPI == EPI_JACOBI0 || EPI == EPI_JACOBI0_INV;
         if constexpr (EPI == EPI_RESID || EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV || J0) f = a.f[row];
         if constexpr (EPI == EPI_ADD || EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV) u = a.u[row];
-        if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV || J0) d = a.d[row];
+        if constexpr (EPI == EPI_JACOBI || EPI == EPI_JACOBI_INV || J0 || EPI == EPI_STORE_J0 ||
+                      EPI == EPI_STORE_J0_INV)
+            d = a.d[row];
     }
     __device__ __forceinline__ T apply(T s, const SpmvArgs<T>& a) const {
-        if constexpr (EPI == EPI_STORE) return s;
+        if constexpr (EPI == EPI_STORE || EPI == EPI_STORE_J0 || EPI == EPI_STORE_J0_INV) return s;
         else if constexpr (EPI == EPI_RESID) return f - s;
         else if constexpr (EPI == EPI_ADD) return u + s;
         else if constexpr (EPI == EPI_JACOBI) return u + ((T)a.w * (f - s)) / d;
@@ -170,6 +180,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
                 for (uint32_t k = rb[j]; k < re[j]; ++k) s += prod[k];
                 const T yv = ep[j].apply(s, a);
                 y[row] = yv;
+                spmv_second<T, EPI>(row, yv, ep[j].d, a);
                 if (DOT == 1) dsum[0] += (double)xr[j] * (double)yv;
                 if (DOT == 2) dsum[0] += (double)yv * (double)yv;
             }
@@ -187,6 +198,7 @@ __global__ __launch_bounds__(kBlock) void k_spmv(const uint32_t* __restrict__ rp
         if (t == 0) {
             const T yv = spmv_epilogue<T, EPI>(r0, (T)tot[0], a);
             y[r0] = yv;
+            if constexpr (EPI == EPI_STORE_J0 || EPI == EPI_STORE_J0_INV) spmv_second<T, EPI>(r0, yv, a.d[r0], a);
             if (DOT == 1) part[bid] = (double)x[r0] * (double)yv;
             if (DOT == 2) part[bid] = (double)yv * (double)yv;
         }
@@ -239,7 +251,11 @@ __global__ __launch_bounds__(kBlock) void k_spmv_vec(const uint32_t* __restrict_
     T s = (T)0;
     for (uint32_t k = k0 + lane; k < k1; k += LANES) s += v[k] * spmv_x<T, EPI>(x, a, ci[k]);
     s = group_sum<LANES>(s);
-    if (row < n && lane == 0) y[row] = spmv_epilogue<T, EPI>(row, s, a);
+    if (row < n && lane == 0) {
+        const T yv = spmv_epilogue<T, EPI>(row, s, a);
+        y[row] = yv;
+        if constexpr (EPI == EPI_STORE_J0 || EPI == EPI_STORE_J0_INV) spmv_second<T, EPI>(row, yv, a.d[row], a);
+    }
 }
 
 template <typename T, int LANES>
@@ -253,6 +269,8 @@ void spmv_vec_launch(const CsrMat& A, const T* vals, const T* x, T* y, int epi, 
         case EPI_JACOBI: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
         case EPI_JACOBI_INV: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI_INV, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
         case EPI_JACOBI0: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI0, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+        case EPI_STORE_J0: hipLaunchKernelGGL((k_spmv_vec<T, EPI_STORE_J0, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
+        case EPI_STORE_J0_INV: hipLaunchKernelGGL((k_spmv_vec<T, EPI_STORE_J0_INV, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
         default: hipLaunchKernelGGL((k_spmv_vec<T, EPI_JACOBI0_INV, LANES>), grid, block, 0, s, A.rp.get(), A.ci.get(), vals, x, y, A.n, a, g); break;
     }
 }
@@ -376,6 +394,8 @@ void csr_spmv(const CsrMat& A, const T* x, T* y, int epi, const SpmvArgs<T>& a, 
         case EPI_JACOBI: spmv_stream_launch<T, EPI_JACOBI>(A, vals, x, y, a, s, g); break;
         case EPI_JACOBI_INV: spmv_stream_launch<T, EPI_JACOBI_INV>(A, vals, x, y, a, s, g); break;
         case EPI_JACOBI0: spmv_stream_launch<T, EPI_JACOBI0>(A, vals, x, y, a, s, g); break;
+        case EPI_STORE_J0: spmv_stream_launch<T, EPI_STORE_J0>(A, vals, x, y, a, s, g); break;
+        case EPI_STORE_J0_INV: spmv_stream_launch<T, EPI_STORE_J0_INV>(A, vals, x, y, a, s, g); break;
         default: spmv_stream_launch<T, EPI_JACOBI0_INV>(A, vals, x, y, a, s, g); break;
     }
     FEM_HIP(hipGetLastError());

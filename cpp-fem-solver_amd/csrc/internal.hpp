@@ -265,7 +265,12 @@ enum SpmvEpi : int {
     // (EPI_JACOBI_INV) with x = u = jac0(f, d) formed where it is read (the
     // first sweep's values, bit for bit; the x argument is not read)
     EPI_JACOBI0 = 5,
-    EPI_JACOBI0_INV = 6
+    EPI_JACOBI0_INV = 6,
+    // y = A x, and the first damped Jacobi sweep from zero of the level y is
+    // the right-hand side of: o = jac0(y, d) (the restriction hands the
+    // next level its first sweep)
+    EPI_STORE_J0 = 7,
+    EPI_STORE_J0_INV = 8
 };
 
 // One damped Jacobi sweep from u = 0 at one row: f - A 0 = f, then the
@@ -283,6 +288,7 @@ struct SpmvArgs {
     const T* u = nullptr;
     const T* d = nullptr;
     double w = 0.0;
+    T* o = nullptr;  // second output (EPI_STORE_J0, EPI_STORE_J0_INV)
 };
 
 // Launch CSR SpMV with an epilogue (T = double or float values/vectors).

@@ -329,8 +329,9 @@ def test_precond_bytes_tally(fp32, monkeypatch):
     sweep from zero (f, d and u), so the difference of the two
     tallies is fixed by the level sizes the hierarchy reports (themselves
     checked against the oracle above). The V-cycles here run without the
-    fused tail (FEM_AMG_TAIL=0; its tally is checked in
-    test_amg_fused_tail)."""
+    fused tail and keep the first sweep's own launch (FEM_AMG_TAIL=0,
+    FEM_AMG_J0=0; those tallies are checked in test_amg_fused_tail and
+    test_amg_first_sweep_from_restriction)."""
     G = gen.poisson3d(32)
     A = F.CsrMatrix.from_gen(G)
     s = 4 if fp32 else 8
@@ -345,9 +346,11 @@ def test_precond_bytes_tally(fp32, monkeypatch):
         assert T.apply_bytes(fp32=bool(fp32)) == 2 * sweep + ndiv * G.n * s, kind
     kw = dict(smoother=F.SMOOTH_JACOBI, coarse_exact=1, omega=0.0, p_omega=0.0, fp32=fp32)
     monkeypatch.setenv("FEM_AMG_TAIL", "0")
+    monkeypatch.setenv("FEM_AMG_J0", "0")
     M1 = F.Preconditioner(A, "saamg", npre=1, npost=1, **kw)
     M2 = F.Preconditioner(A, "saamg", npre=2, npost=2, **kw)
     monkeypatch.delenv("FEM_AMG_TAIL")
+    monkeypatch.delenv("FEM_AMG_J0")
     a, b = M1.info(), M2.info()
     L = b["levels"]
     assert L == a["levels"] > 2
@@ -537,3 +540,30 @@ def test_amg_fused_tail(smoother, monkeypatch):
         assert st["converged"] == 1 and st["true_relres"] < 1.5e-10
         assert abs(st["iterations"] - sts["iterations"]) <= (1 if fp32 else 0)
         assert rel(x, xs) < tol_x(G, cond=cond_poisson(40, 3))
+
+
+def test_amg_first_sweep_from_restriction(monkeypatch):
+    """A level of more than 65,536 rows smoothed by damped Jacobi takes its
+    first sweep from zero from the restriction that writes its right-hand
+    side (a second output of that launch): float64 applications and float32
+    inner solves equal those with the sweep's own launch (FEM_AMG_J0=0) bit
+    for bit, and the byte tally drops by that launch's f read less the
+    restriction's d read (one vector per level)."""
+    G = gen.poisson3d(96)  # level 1 holds about 80,000 rows
+    _, b = gen.rhs(G, 17)
+    A = F.CsrMatrix.from_gen(G)
+    kw = dict(smoother=F.SMOOTH_SGS, npre=2, npost=2, omega=1.0, p_omega=0.0, coarse_exact=1, fp32=1)
+    M = F.Preconditioner(A, "saamg", **kw)
+    monkeypatch.setenv("FEM_AMG_J0", "0")
+    Ms = F.Preconditioner(A, "saamg", **kw)
+    monkeypatch.delenv("FEM_AMG_J0")
+    info = M.info()
+    n1 = int(info["rows"][1])
+    assert n1 > 65536 and info["levels"] >= 3
+    r = np.random.default_rng(21).standard_normal(G.n)
+    assert np.array_equal(M.apply(r), Ms.apply(r))
+    x, st = F.solve(A, M, b, solver="pcg", precision=F.FP32_IR)
+    xs, sts = F.solve(A, Ms, b, solver="pcg", precision=F.FP32_IR)
+    assert st["converged"] == 1 and np.array_equal(x, xs) and st["iterations"] == sts["iterations"]
+    for fp32, sb in ((False, 8), (True, 4)):
+        assert Ms.apply_bytes(fp32=fp32) - M.apply_bytes(fp32=fp32) == n1 * sb
