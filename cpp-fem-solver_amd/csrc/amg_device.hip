@@ -356,26 +356,110 @@ __global__ __launch_bounds__(64) void k_spgemm_sym(const uint32_t* arp, const ui
 }
 
 // C_ik = sum over p in row i of A (ascending) of a_ip * B[j_p][k]
+// C_ik = sum over p in row i of A (ascending) of a_ip * B[j_p][k], C's
+// columns given (symbolic pass). Rows of C with at most kNumCap entries:
+// the row's columns and accumulators sit in LDS; the products a_ip b_jq are
+// laid out p-major, 64 per round (all loads of a round issued together), and
+// each lands on its column's accumulator by a binary search in LDS, the
+// products of one round added in ascending p (those of one p go to distinct
+// columns): per output the same additions in the same order as a search per
+// (output, p). Longer rows search B's row for each (output, p).
+constexpr uint32_t kNumCap = 512;
 __global__ __launch_bounds__(64) void k_spgemm_num(const uint32_t* arp, const uint32_t* aci, const double* av,
                                                    const uint32_t* brp, const uint32_t* bci, const double* bv,
                                                    const uint32_t* crp, const uint32_t* cci, double* cv, uint32_t n) {
+    __shared__ uint32_t cols[kNumCap];
+    __shared__ double acc[kNumCap];
+    __shared__ uint32_t s_off[65], s_bs[64];
+    __shared__ double s_av[64];
+    const uint32_t l = threadIdx.x;
     for (uint32_t i = blockIdx.x; i < n; i += gridDim.x) {
         const uint32_t a0 = arp[i], a1 = arp[i + 1];
-        for (uint32_t t = crp[i] + threadIdx.x; t < crp[i + 1]; t += 64) {
-            const uint32_t k = cci[t];
-            double acc = 0.0;
-            for (uint32_t p = a0; p < a1; ++p) {
-                const uint32_t j = aci[p];
-                uint32_t lo = brp[j], hi = brp[j + 1];
-                while (lo < hi) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (bci[mid] < k) lo = mid + 1;
-                    else hi = mid;
+        const uint32_t c0 = crp[i], nc = crp[i + 1] - c0;
+        if (nc > kNumCap) {
+            for (uint32_t t = c0 + l; t < c0 + nc; t += 64) {
+                const uint32_t k = cci[t];
+                double sum = 0.0;
+                for (uint32_t p = a0; p < a1; ++p) {
+                    const uint32_t j = aci[p];
+                    uint32_t lo = brp[j], hi = brp[j + 1];
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (bci[mid] < k) lo = mid + 1;
+                        else hi = mid;
+                    }
+                    if (lo < brp[j + 1] && bci[lo] == k) sum += av[p] * bv[lo];
                 }
-                if (lo < brp[j + 1] && bci[lo] == k) acc += av[p] * bv[lo];
+                cv[t] = sum;
             }
-            cv[t] = acc;
+            continue;  // uniform: nc is the row's
         }
+        for (uint32_t t = l; t < nc; t += 64) {
+            cols[t] = cci[c0 + t];
+            acc[t] = 0.0;
+        }
+        for (uint32_t pb = a0; pb < a1; pb += 64) {
+            // this chunk's entries of A: B row start, length, value; offsets
+            // of their products (inclusive scan of the lengths)
+            const uint32_t p = pb + l;
+            uint32_t bs = 0, len = 0;
+            double a = 0.0;
+            if (p < a1) {
+                const uint32_t j = aci[p];
+                bs = brp[j];
+                len = brp[j + 1] - bs;
+                a = av[p];
+            }
+            uint32_t x = len;
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t y = (uint32_t)__shfl_up((int)x, off);
+                if ((int)l >= off) x += y;
+            }
+            __syncthreads();  // the previous chunk's readers of s_* are done
+            s_off[l + 1] = x;
+            if (l == 0) s_off[0] = 0;
+            s_bs[l] = bs;
+            s_av[l] = a;
+            __syncthreads();
+            const uint32_t total = s_off[64];
+            const uint32_t np = min(64u, a1 - pb);
+            for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+                const uint32_t f = r0 + l;
+                uint32_t o = 0, slot = 0;
+                double prod = 0.0;
+                if (f < total) {
+                    // owner: the last entry whose products start at or before f
+                    uint32_t lo = 0, hi = np;  // s_off[lo] <= f < s_off[hi]
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (s_off[mid] <= f) lo = mid;
+                        else hi = mid;
+                    }
+                    o = lo;
+                    const uint32_t q = s_bs[o] + (f - s_off[o]);
+                    const uint32_t k = bci[q];
+                    prod = s_av[o] * bv[q];
+                    uint32_t a2 = 0, b2 = nc;  // lower bound of k in cols
+                    while (a2 < b2) {
+                        const uint32_t mid = (a2 + b2) >> 1;
+                        if (cols[mid] < k) a2 = mid + 1;
+                        else b2 = mid;
+                    }
+                    slot = a2 < nc ? a2 : 0u;  // k is among the row's columns (symbolic pass)
+                }
+                // owners of this round in ascending order
+                const uint32_t ofirst = (uint32_t)__shfl((int)o, 0);
+                const uint32_t last = min(total - r0, 64u) - 1;
+                const uint32_t olast = (uint32_t)__shfl((int)o, (int)last);
+                for (uint32_t oo = ofirst; oo <= olast; ++oo) {
+                    if (f < total && o == oo) acc[slot] += prod;
+                    __syncthreads();
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t t = l; t < nc; t += 64) cv[c0 + t] = acc[t];
+        __syncthreads();  // cols and acc are rewritten by the next row
     }
 }
 
