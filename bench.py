@@ -651,6 +651,7 @@ def main():
         log(f"[bench] {name}: setup {setup_ms:.1f} ms, first solve {first_ms:.1f} ms, solve {solve_ms:.2f} ms "
             f"(fresh right-hand sides), it {st['iterations']}, relres {relres:.2e}, err vs x_true {err:.2e}")
         if is_head:
+            log(f"[bench] {name}: per-step ms {[round(t, 3) for t in times]}")
             headline = (total_ms / steps, times)
         S.close()
         M.close()

@@ -1277,18 +1277,39 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
     const int phase = phase_++;
     const int hint = solver == FEM_SOLVER_PCG && phase < (int)hint_.size() ? hint_[phase] : 0;
     int chunk = opts.check_every > 0 ? cmax : std::min(cmax, hint > 0 ? hint : 8);
+    // the length of the graph being captured or launched
+    int glen = chunk;
     auto body = [&] {
-        if (solver == FEM_SOLVER_PCG) for (int k = 0; k < chunk; ++k) pcg_iter<T>(W);
-        else if (solver == FEM_SOLVER_BICGSTAB) for (int k = 0; k < chunk; ++k) bi_iter<T>(W);
+        if (solver == FEM_SOLVER_PCG) for (int k = 0; k < glen; ++k) pcg_iter<T>(W);
+        else if (solver == FEM_SOLVER_BICGSTAB) for (int k = 0; k < glen; ++k) bi_iter<T>(W);
         else gm_cycle<T>(W);
+    };
+    // Without a fixed check interval (cmax a power of two) a chunk of any
+    // length runs as back-to-back launches of power-of-two graphs (13 = 8 +
+    // 4 + 1) before one host read, so at most log2(cmax) + 1 graphs exist
+    // and a solve of a new right-hand side, whose chunk lengths differ from
+    // the previous solve's, captures none once they do.
+    const bool pow2 = opts.check_every <= 0 && solver != FEM_SOLVER_GMRES;
+    auto launch_chunk = [&] {
+        if (solver == FEM_SOLVER_GMRES) {
+            const int tag = solver * 100000 + W.m;
+            replay(gs.at(tag), tag, body);
+            return;
+        }
+        for (int p = pow2 ? cmax : chunk; p >= 1; p >>= 1) {
+            if (pow2 && !(chunk & p)) continue;
+            glen = p;
+            const int tag = solver * 100000 + p;
+            replay(gs.at(tag), tag, body);
+            if (!pow2) break;
+        }
     };
     // every replay runs >= 1 iteration (GMRES: one cycle of >= 1 step)
     const long long guard_max = (long long)max_iter + 4;
     long long rounds = 0;
     static const bool trace = std::getenv("FEM_TRACE_SOLVE") != nullptr;
     while (!hst_.p->done) {
-        const int tag = solver * 100000 + (solver == FEM_SOLVER_GMRES ? W.m : chunk);
-        replay(gs.at(tag), tag, body);
+        launch_chunk();
         read_state();
         if (trace)
             std::fprintf(stderr, "[solve] replay of %d: iterations %d, |r|/|b| %.3e, done %d\n",
@@ -1316,15 +1337,17 @@ int KrylovSolver::run(KWork<T>& W, double tol, int max_iter, bool soft, long lon
     if (solver == FEM_SOLVER_PCG && hst_.p->done == 1 && phase < 64) {
         if ((int)hint_.size() <= phase) hint_.resize(phase + 1, 0);
         hint_[phase] = hst_.p->iters;
-        // capture now the chunks a repeated solve replays (cmax-long ones and
-        // the remainder), so that solve captures nothing
-        if (opts.check_every <= 0)
-            for (const int c : {std::min(cmax, hint_[phase]), hint_[phase] % cmax}) {
-                if (c <= 0) continue;
-                chunk = c;
-                const int tag = solver * 100000 + c;
-                replay(gs.at(tag), tag, body, false);
-            }
+        // capture now the graphs a repeated solve launches (the power-of-two
+        // parts of the cmax-long chunks and of the remainder), so that solve
+        // captures nothing
+        if (pow2)
+            for (const int c : {std::min(cmax, hint_[phase]), hint_[phase] % cmax})
+                for (int p = 1; p <= c; p <<= 1) {
+                    if (!(c & p)) continue;
+                    glen = p;
+                    const int tag = solver * 100000 + p;
+                    replay(gs.at(tag), tag, body, false);
+                }
     }
     if (hst_.p->done == 2) {
         char ctx[128];
