@@ -838,8 +838,7 @@ void sgs_grouped_device(AmgLevelDev& D, hipStream_t s) {
     FEM_HIP(hipGetLastError());
     pack_columns(D.gci.get(), A.nnz, D.gcd, D.gcbase, s);
     D.h_gp.resize(n + 1);
-    D.gp.download(D.h_gp.data(), n + 1, s);
-    FEM_HIP(hipStreamSynchronize(s));
+    download_words(D.gp.get(), (size_t)n + 1, D.h_gp.data(), s);
     const std::vector<uint32_t>& gp = D.h_gp;
     std::vector<uint32_t> blk;
     D.h_cblk.assign(1, 0);
@@ -1214,7 +1213,10 @@ AmgDevHierarchy build_sa_hierarchy_device(const CsrMat& A0, const fem_pc_params&
         tr.lap("Galerkin products");
         lv->nc = na;
         lv->d = std::move(dinv);
-        cur.to_mat(lv->A, s);
+        // level 0 is a copy of A0: its host row pointers and packed columns
+        // are A0's
+        if (l == 0) lv->A.adopt(cur.n, cur.m, cur.nnz, std::move(cur.rp), std::move(cur.ci), std::move(cur.v), s, &A0);
+        else cur.to_mat(lv->A, s);
         P.to_mat(lv->P, s);
         R.to_mat(lv->R, s);
         H.lv.push_back(std::move(lv));

@@ -11,6 +11,8 @@
 // library is built with -ffp-contract=off), so FP64 results are bit-identical
 // to the reference's sequential row sum. Rows longer than the staging area
 // get a block each and are reduced with a strided sum + block tree.
+#include <mutex>
+#include <cstring>
 #include <algorithm>
 #include <atomic>
 
@@ -420,8 +422,7 @@ void CsrMat::ensure_fp32(hipStream_t s) {
         plan32.build(h_rp.data(), n, s, block_nnz(nnz, stage_nnz<float>(), kPlanMinBlocks), stage_rows<float>());
     } else {
         std::vector<uint32_t> hrp(n + 1);
-        rp.download(hrp.data(), n + 1, s);
-        FEM_HIP(hipStreamSynchronize(s));
+        download_words(rp.get(), (size_t)n + 1, hrp.data(), s);
         plan32.build(hrp.data(), n, s, block_nnz(nnz, stage_nnz<float>(), kPlanMinBlocks), stage_rows<float>());
     }
 }
@@ -457,8 +458,27 @@ void CsrMat::set_host(uint32_t rows, uint32_t cols, std::vector<uint32_t> rp_, s
     FEM_HIP(hipStreamSynchronize(s));
 }
 
+void download_words(const uint32_t* d, size_t count, uint32_t* h, hipStream_t s) {
+    // below this size a pageable copy costs no more than the staging step
+    constexpr size_t kStageMin = (size_t)1 << 16;
+    if (count < kStageMin) {
+        if (count) FEM_HIP(hipMemcpyAsync(h, d, count * 4, hipMemcpyDeviceToHost, s));
+        FEM_HIP(hipStreamSynchronize(s));
+        return;
+    }
+    // one staging buffer for the process, grown on demand and kept (pinned
+    // host memory is not freed while the runtime may be shutting down)
+    static std::mutex mu;
+    static PinnedBuf<uint32_t>* stage = new PinnedBuf<uint32_t>();
+    std::lock_guard<std::mutex> lock(mu);
+    stage->ensure(count);
+    FEM_HIP(hipMemcpyAsync(stage->p, d, count * 4, hipMemcpyDeviceToHost, s));
+    FEM_HIP(hipStreamSynchronize(s));
+    std::memcpy(h, stage->p, count * 4);
+}
+
 void CsrMat::adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&& rp_, DevBuf<uint32_t>&& ci_,
-                   DevBuf<double>&& v_, hipStream_t s) {
+                   DevBuf<double>&& v_, hipStream_t s, const CsrMat* same) {
     uid = next_uid();
     n = rows;
     m = cols;
@@ -467,11 +487,29 @@ void CsrMat::adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&&
     rp = std::move(rp_);
     ci = std::move(ci_);
     v = std::move(v_);
-    std::vector<uint32_t> hrp(n + 1);
-    rp.download(hrp.data(), n + 1, s);
-    FEM_HIP(hipStreamSynchronize(s));
+    const bool copy = same && same->rp_host && same->n == n && same->nnz == nnz && same->h_rp.size() == (size_t)n + 1;
+    std::vector<uint32_t> hrp;
+    if (copy) {
+        hrp = same->h_rp;
+    } else {
+        hrp.resize((size_t)n + 1);
+        download_words(rp.get(), (size_t)n + 1, hrp.data(), s);
+    }
     plan.build(hrp.data(), n, s, block_nnz(nnz, stage_nnz<double>(), kPlanMinBlocks), stage_rows<double>());
-    pack_columns(ci.get(), nnz, cd, cbase, s);
+    if (copy) {
+        // packing is a function of the columns alone
+        cd.release();
+        cbase.release();
+        if (same->cd.get() && same->cbase.get()) {
+            cd.alloc(same->cd.n);
+            cbase.alloc(same->cbase.n);
+            FEM_HIP(hipMemcpyAsync(cd.get(), same->cd.get(), same->cd.n * sizeof(uint16_t), hipMemcpyDeviceToDevice, s));
+            FEM_HIP(hipMemcpyAsync(cbase.get(), same->cbase.get(), same->cbase.n * sizeof(uint32_t),
+                                   hipMemcpyDeviceToDevice, s));
+        }
+    } else {
+        pack_columns(ci.get(), nnz, cd, cbase, s);
+    }
     h_rp = std::move(hrp);  // kept for later plans (rp_host)
     h_ci.clear();
     h_v.clear();

@@ -243,10 +243,16 @@ struct CsrMat : LinOp {
                   std::vector<double> v_, hipStream_t s);
     void sync_host() const;  // download into the host mirror if needed
     // take ownership of device arrays (ci, v with 4 padding entries); the
-    // host mirror is filled on demand
+    // host mirror is filled on demand. `same` (optional): a matrix with the
+    // same row pointers and columns, whose host row pointers and packed
+    // columns are copied instead of being downloaded and recomputed.
     void adopt(uint32_t rows, uint32_t cols, uint64_t nz, DevBuf<uint32_t>&& rp_, DevBuf<uint32_t>&& ci_,
-               DevBuf<double>&& v_, hipStream_t s);
+               DevBuf<double>&& v_, hipStream_t s, const CsrMat* same = nullptr);
 };
+
+// count row pointers (or other 32-bit words) from the device into h, through
+// a pinned staging buffer for large counts; synchronises s
+void download_words(const uint32_t* d, size_t count, uint32_t* h, hipStream_t s);
 
 // the input step on the device (coo_device.hip)
 void coo_to_csr_device(uint32_t n, uint32_t m, uint64_t nin, const double* v, const uint32_t* r, const uint32_t* c,
